@@ -1,0 +1,46 @@
+/*
+ * semisim_hip_ext.h - entry points of libsemisim_hip.so with no counterpart
+ * in the reference core.  They select the multigrid precision, report device
+ * and kernel statistics, and expose single kernels on the device state for
+ * parity tests.  Plain C types only.
+ */
+#ifndef SEMISIM_HIP_EXT_H
+#define SEMISIM_HIP_EXT_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* multigrid precision: 0 FP64 hierarchy (the plain method), 1 FP32 hierarchy,
+   smoothers and preconditioner SpMV inside FP64 Krylov (mixed precision).
+   Also selected by SEMISIM_MG_FP32=1 at api3_init.  1 = done. */
+int  sdx_set_precision(int mode);
+int  sdx_get_precision(void);
+int  sdx_device_name(char *buf, int n);
+/* out[0] V-cycles, out[1] level-0 SpMVs, out[2] nodes, out[3] MG levels */
+int  sdx_kernel_stats(double *out, int n);
+void sdx_reset_stats(void);
+
+/* ---- kernel-level hooks on the prepared device (tests) ----
+   A7: 7 blocks of N doubles (c, xm, xp, ym, yp, zm, zp). */
+int    sdx_test_prepare(void);
+void   sdx_test_set_state(const double *psi, const double *phin, const double *phip);
+void   sdx_test_get_node(int which, double *out);   /* 0 psi 1 phin 2 phip 3 n 4 p 5 mu_n0 6 mu_p0 */
+double sdx_test_poisson(int build, const double *Va, double *F, double *A7, double *sc);
+void   sdx_test_upd_mob(double *fmu6);
+void   sdx_test_set_fmu(const double *fmu6);
+void   sdx_test_cont(int hole, double *A7, double *rhs, double *sc, double *dsc);
+void   sdx_test_load_op(const double *A7, int qsign, const double *w);
+void   sdx_test_matvec(const double *x, double *y);
+int    sdx_test_mg_setup(void);                     /* number of levels */
+void   sdx_test_level_dims(int L, int *dims);      /* Nx Ny Nz cx cy cz */
+void   sdx_test_level_op(int L, double *A7);        /* coarse operator, unscaled (rs * Ahat) */
+void   sdx_test_precond(const double *r, double *z);
+int    sdx_test_pcg(const double *b, double *x, int maxit, double rtol);
+double sdx_test_bicgstab(const double *b, double *x, int maxit, double rtol, double atol, int *its);
+void   sdx_test_currents(const double *Va, double *I, double *S, double *In);
+double sdx_test_cont_update(int hole, const double *u, const double *sc, double *dq, double lo, double hi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

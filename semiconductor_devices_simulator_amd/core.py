@@ -1,0 +1,125 @@
+"""Loader and ctypes bindings of the api3_* C ABI.
+
+The product is the HIP-backed shared library ``libsemisim_hip.so`` built from
+``csrc/`` by :mod:`semiconductor_devices_simulator_amd.build`.  It exports the
+same C entry points as the reference OpenMP core (semiconductor_3d.c:4567-5353),
+so the reference's ctypes front end can load it unchanged through its
+``SEMISIM_SO`` hook (semibuild.py:516-522, sentinel symbol ``api3_set_caxis``
+semibuild.py:28).  This module mirrors that loader and the argument/return
+types the reference binds (main.py:172-217, 3494-3513).
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_NAME = "libsemisim_hip.so"
+LIB_PATH = os.path.join(HERE, "lib", LIB_NAME)
+SENTINEL = "api3_set_caxis"          # semibuild.py:28
+
+D = ctypes.c_double
+I = ctypes.c_int
+LL = ctypes.c_longlong
+DP = ctypes.POINTER(D)
+IP = ctypes.POINTER(I)
+
+# (name, restype, argtypes): the reference's hard and soft bindings
+# (main.py:172-217, 3485, 3494-3513; validate.py:31-49 for the test hooks).
+SIGNATURES = [
+    ("api3_init", None, []),
+    ("api3_add_material", I, [I, D]),
+    ("api3_set_grid", None, [I, I, I, D, D, D]),
+    ("api3_set_grid_coords", I, [I, I, I, DP, DP, DP]),
+    ("api3_set_solver", None, [I, I, I, D, D, D, D]),
+    ("api3_set_region", None, [I, I, I, I, I, I, I, D, D]),
+    ("api3_add_contact", None, [I, I, I, I, I, I, D, D]),
+    ("api3_add_contact_ex", I, [I, I, I, I, I, I, I, I, D, D, D, D]),
+    ("api3_clear_contacts", None, []),
+    ("api3_add_sheet_charge", None, [I, I, I, I, I, I, D]),
+    ("api3_add_fixed_charge", None, [I, I, I, I, I, I, D]),
+    ("api3_set_models", None, [I]),
+    ("api3_set_caxis", None, [I]),
+    ("api3_get_caxis", I, []),
+    ("api3_solve", I, []),
+    ("api3_resolve", I, []),
+    ("api3_set_contact_V", None, [I, D]),
+    ("api3_get_contact_I", D, [I]),
+    ("api3_get_contact_A", D, [I]),
+    ("api3_get_contact_In", D, [I]),
+    ("api3_get_contact_Ifloor", D, [I]),
+    ("api3_get_contact_Iraw", D, [I]),
+    ("api3_get_contact_Vsolved", D, [I]),
+    ("api3_get_stat", LL, [I]),
+    ("api3_get_time", D, [I]),
+    ("api3_get_Nx", I, []), ("api3_get_Ny", I, []), ("api3_get_Nz", I, []),
+    ("api3_get_conv", I, []), ("api3_get_iter", I, []), ("api3_get_res", D, []),
+    ("api3_get_xs", D, [I]), ("api3_get_ys", D, [I]), ("api3_get_zs", D, [I]),
+    ("api3_mat_ni", D, [I]), ("api3_mat_Eg", D, [I]),
+    ("api3_bulk_copy", None, [I, DP]),
+    ("api3_get_Emax", D, []),
+    ("api3_face_J", I, [I, I, DP]),
+    ("api3_iv_sweep", None, [I, D, D, I, DP, DP, IP]),
+    ("api3_iv_sweep2", None, [I, D, D, I, DP, DP, DP, IP]),
+    ("api3_get_ncon", I, []),
+    ("api3_request_stop", None, []),
+    ("api3_clear_stop", None, []),
+    ("api3_stop_requested", I, []),
+    ("api3_get_progress", None, [DP]),
+    ("api3_set_threads", None, [I]),
+    ("api3_set_verbose", None, [I]),
+    ("api3_get_aa_depth", I, []),
+    ("api3_set_node_data", None, [IP, DP, DP, DP]),
+    ("api3_set_mat_tau", I, [I, D, D]),
+    ("api3_carrier_sums", None, [DP]),
+    ("api3_get_state", I, [DP, DP, DP]),
+    ("api3_set_state", I, [DP, DP, DP]),
+    ("api3_set_predictor", None, [I]),
+    ("api3_get_Cref", D, []),
+]
+
+# Entry points of the HIP library with no counterpart in the reference
+# (precision mode, device info, kernel-level test hooks): bound when present.
+EXT_SIGNATURES = [
+    ("sdx_set_precision", I, [I]),
+    ("sdx_get_precision", I, []),
+    ("sdx_device_name", I, [ctypes.c_char_p, I]),
+    ("sdx_kernel_stats", I, [DP, I]),
+    ("sdx_reset_stats", None, []),
+]
+
+
+class CoreError(RuntimeError):
+    pass
+
+
+def bind(lib, extra=True):
+    """Attach restype/argtypes for every api3_* symbol the library exports."""
+    for name, res, args in SIGNATURES + (EXT_SIGNATURES if extra else []):
+        f = getattr(lib, name, None)
+        if f is None:
+            continue
+        f.restype = res
+        f.argtypes = list(args)
+    return lib
+
+
+def load_core(path=None):
+    """The HIP core as a bound ctypes.CDLL (SEMISIM_SO overrides the path,
+    semibuild.py:516-522).  Raises CoreError when the library is missing:
+    there is no CPU fallback."""
+    path = path or os.environ.get("SEMISIM_SO") or LIB_PATH
+    if not os.path.isfile(path):
+        raise CoreError(f"HIP solver core not built: {path} is missing "
+                        "(run __graft_entry__.build() or python -m "
+                        "semiconductor_devices_simulator_amd.build)")
+    lib = ctypes.CDLL(path)
+    if not hasattr(lib, SENTINEL):
+        raise CoreError(f"{path} does not export {SENTINEL}: not an api3 solver core")
+    return bind(lib)
+
+
+def dptr(a):
+    return a.ctypes.data_as(DP)
+
+
+def iptr(a):
+    return a.ctypes.data_as(IP)

@@ -1,0 +1,1335 @@
+// engine.cpp - solver drivers of the MI355X drift-diffusion core.
+//
+// Control flow follows the reference (semiconductor_3d.c): prepare :1638-1775,
+// pcg :2182-2210, bicgstab :2216-2320, poisson_newton :2791-2823, solve_cont
+// :3580-3594, float_fix :3159-3228, terminal_currents :3598-3623, aa_step
+// :3709-3757, gummel :3781-3930, region_shift :3956-3987, equilibrium
+// :3989-4004, cont_path :4018-4129, run_continuation :4135-4173.  All vector
+// work runs on the device (k_physics.hip, k_linalg.hip); the host keeps the
+// scalar decisions, reading a handful of reduction results per iteration.
+#include "engine.h"
+#include <chrono>
+#include <cstdlib>
+#include <algorithm>
+#include <queue>
+
+Engine G;
+volatile int g_stop = 0;
+volatile double g_prog[8];
+
+static int g_err = 0;
+#define HC(x)                                                                                   \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess) {                                                                 \
+            fprintf(stderr, "semisim-hip: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_),  \
+                    __FILE__, __LINE__);                                                        \
+            g_err = 1;                                                                          \
+        }                                                                                       \
+    } while (0)
+
+int eng_ok() { return !g_err; }
+
+static double wtime() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ------------------------------------------------------------ memory
+template <class T> static T *dalloc(size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) { g_err = 1; return nullptr; }
+    G.dev_allocs.push_back(p);
+    return (T *)p;
+}
+
+void eng_sync() { HC(hipStreamSynchronize(G.st)); }
+void eng_download(double *dst, const double *src, size_t n) {
+    HC(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, G.st));
+    eng_sync();
+}
+void eng_upload(double *dst, const double *src, size_t n) {
+    HC(hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, G.st));
+    eng_sync();
+}
+// read scalar slots [a, b) into the pinned mirror (one copy, one sync)
+static void rd(int a, int b) {
+    HC(hipMemcpyAsync(G.h_scal + a, G.rb.scal + a, (size_t)(b - a) * 8, hipMemcpyDeviceToHost, G.st));
+    eng_sync();
+}
+static double S(int slot) { return G.h_scal[slot]; }
+
+template <class T> static void hier_free(Hier<T> &H) { H.lev.clear(); H.inv = nullptr; H.alloc = false; }
+
+void eng_free() {
+    if (G.st) hipStreamSynchronize(G.st);
+    for (void *p : G.dev_allocs) hipFree(p);
+    G.dev_allocs.clear();
+    hier_free(G.H64);
+    hier_free(G.H32);
+    G.d_b32 = G.d_x32 = nullptr;
+    G.allocated = 0;
+    G.have_solution = 0;
+}
+
+// Multigrid hierarchy shape: 2x2x2 aggregation of every axis longer than 2
+// until N <= 64 (semiconductor_3d.c:1894-1921).
+template <class T> static int hier_alloc(Hier<T> &H, bool level0_own) {
+    H.lev.clear();
+    MGLevel<T> f{};
+    f.Nx = G.Nx; f.Ny = G.Ny; f.Nz = G.Nz; f.N = G.N;
+    H.lev.push_back(f);
+    while ((int)H.lev.size() < 24) {
+        MGLevel<T> &a = H.lev.back();
+        if (a.N <= 64) break;
+        int cx = a.Nx > 2, cy = a.Ny > 2, cz = a.Nz > 2;
+        if (!cx && !cy && !cz) break;
+        a.cx = cx; a.cy = cy; a.cz = cz;
+        MGLevel<T> c{};
+        c.Nx = cx ? (a.Nx + 1) / 2 : a.Nx;
+        c.Ny = cy ? (a.Ny + 1) / 2 : a.Ny;
+        c.Nz = cz ? (a.Nz + 1) / 2 : a.Nz;
+        c.N = c.Nx * c.Ny * c.Nz;
+        H.lev.push_back(c);
+    }
+    for (size_t L = 0; L < H.lev.size(); L++) {
+        MGLevel<T> &v = H.lev[L];
+        size_t n = v.N;
+        if (L == 0) {
+            if (level0_own) {
+                v.A.c = dalloc<T>(n); v.A.xm = dalloc<T>(n); v.A.xp = dalloc<T>(n); v.A.ym = dalloc<T>(n);
+                v.A.yp = dalloc<T>(n); v.A.zm = dalloc<T>(n); v.A.zp = dalloc<T>(n);
+            }
+        } else {
+            v.A.c = nullptr;
+            v.A.xm = dalloc<T>(n); v.A.xp = dalloc<T>(n); v.A.ym = dalloc<T>(n);
+            v.A.yp = dalloc<T>(n); v.A.zm = dalloc<T>(n); v.A.zp = dalloc<T>(n);
+            v.rs = dalloc<double>(n);
+            v.ph = dalloc<double>(n);
+            v.x = dalloc<T>(n);
+            v.b = dalloc<T>(n);
+        }
+        v.pw = dalloc<T>(n);
+    }
+    int nz = H.lev.back().N;
+    H.inv = dalloc<double>((size_t)nz * nz);
+    H.alloc = !g_err;
+    H.pw_one = false;
+    return H.alloc;
+}
+
+int eng_alloc(int Nx, int Ny, int Nz) {
+    eng_free();
+    g_err = 0;
+    if (!G.st) {
+        if (hipStreamCreateWithFlags(&G.st, hipStreamNonBlocking) != hipSuccess) return 0;
+    }
+    size_t N = (size_t)Nx * Ny * Nz;
+    if (N > 2000000000ULL) return 0;
+    G.Nx = Nx; G.Ny = Ny; G.Nz = Nz; G.N = (int)N;
+    G.d_geo = dalloc<double>(2 * (Nx + Ny + Nz));
+    G.d_kind = dalloc<unsigned char>(N); G.d_mat = dalloc<unsigned char>(N); G.d_flag = dalloc<unsigned char>(N);
+    G.d_con_mask = dalloc<int>(N); G.d_gate_mask = dalloc<int>(N); G.d_changed = dalloc<int>(4);
+    double **nd[] = {&G.d_Nd, &G.d_Na, &G.d_Nf, &G.d_eps, &G.d_bgn, &G.d_robA, &G.d_lncK, &G.d_lnvK, &G.d_niK,
+                     &G.d_psi, &G.d_phin, &G.d_phip, &G.d_n, &G.d_p, &G.d_mun, &G.d_mup, &G.d_gii,
+                     &G.A.c, &G.A.xm, &G.A.xp, &G.A.ym, &G.A.yp, &G.A.zm, &G.A.zp,
+                     &G.d_rhs, &G.d_sc, &G.d_dsc, &G.d_sol, &G.d_xsave, &G.d_tr, &G.d_Ft,
+                     &G.d_xk, &G.d_dqn, &G.d_dqp, &G.d_qn0, &G.d_qp0,
+                     &G.d_Jx, &G.d_Jy, &G.d_Jz, &G.d_Rt, &G.d_mudn, &G.d_mudp};
+    for (double **p : nd) *p = dalloc<double>(N);
+    for (int q = 0; q < 7; q++) G.d_wk[q] = dalloc<double>(N);
+    for (int a = 0; a < 3; a++) { G.d_h1[a] = dalloc<double>(N); G.d_h2[a] = dalloc<double>(N); G.d_e2[a] = dalloc<double>(N); }
+    G.d_fmu = dalloc<double>(6 * N);
+    G.d_aaf = dalloc<double>(3 * N);
+    G.d_aag = dalloc<double>(3 * N);
+    for (int q = 0; q < G.aa_m; q++) { G.d_aadF[q] = dalloc<float>(3 * N); G.d_aadG[q] = dalloc<float>(3 * N); }
+    G.d_mats = dalloc<MatDev>(SD_NM_MAX);
+    G.d_cons = dalloc<ConDev>(SD_NC_MAX);
+    G.rb.part = dalloc<double>((size_t)SD_RED_MAXV * SD_RED_MAXB);
+    G.rb.cnt = dalloc<unsigned>(4);
+    G.rb.scal = dalloc<double>(S_NSLOT);
+    if (!G.h_scal) {
+        if (hipHostMalloc((void **)&G.h_scal, S_NSLOT * 8, hipHostMallocDefault) != hipSuccess) g_err = 1;
+    }
+    if (g_err) { eng_free(); return 0; }
+    HC(hipMemsetAsync(G.rb.cnt, 0, 4 * sizeof(unsigned), G.st));
+    HC(hipMemsetAsync(G.rb.scal, 0, S_NSLOT * 8, G.st));
+    HC(hipMemsetAsync(G.d_gii, 0, N * 8, G.st));
+    for (double *p : {G.d_psi, G.d_phin, G.d_phip, G.d_n, G.d_p, G.d_mun, G.d_mup, G.d_Jx, G.d_Jy, G.d_Jz, G.d_Rt,
+                      G.d_mudn, G.d_mudp})
+        HC(hipMemsetAsync(p, 0, N * 8, G.st));
+    HC(hipMemsetAsync(G.d_fmu, 0, 6 * N * 8, G.st));
+    if (!hier_alloc(G.H64, false)) { eng_free(); return 0; }
+    G.H64.lev[0].A = G.A;
+    eng_sync();
+    if (g_err) { eng_free(); return 0; }
+    G.allocated = 1;
+    G.have_solution = 0;
+    G.dirty = 1;
+    return 1;
+}
+
+static int ensure_h32() {
+    if (G.H32.alloc) return 1;
+    if (!hier_alloc(G.H32, true)) return 0;
+    G.d_b32 = dalloc<float>(G.N);
+    G.d_x32 = dalloc<float>(G.N);
+    return !g_err;
+}
+
+DevView eng_view() {
+    DevView d;
+    d.g.Nx = G.Nx; d.g.Ny = G.Ny; d.g.Nz = G.Nz; d.g.N = G.N;
+    d.g.xs = G.d_geo; d.g.ys = G.d_geo + G.Nx; d.g.zs = G.d_geo + G.Nx + G.Ny;
+    int o = G.Nx + G.Ny + G.Nz;
+    d.g.Lx = G.d_geo + o; d.g.Ly = G.d_geo + o + G.Nx; d.g.Lz = G.d_geo + o + G.Nx + G.Ny;
+    d.kind = G.d_kind; d.con_mask = G.d_con_mask; d.gate_mask = G.d_gate_mask; d.mat = G.d_mat;
+    d.Nd = G.d_Nd; d.Na = G.d_Na; d.Nf = G.d_Nf; d.eps = G.d_eps; d.bgn = G.d_bgn; d.robA = G.d_robA;
+    d.lncK = G.d_lncK; d.lnvK = G.d_lnvK; d.niK = G.d_niK;
+    d.psi = G.d_psi; d.phin = G.d_phin; d.phip = G.d_phip; d.n = G.d_n; d.p = G.d_p;
+    d.mu_n = G.d_mun; d.mu_p = G.d_mup; d.fmu = G.d_fmu;
+    d.gii = (G.models & 256) ? G.d_gii : nullptr;
+    d.mats = G.d_mats; d.cons = G.d_cons;
+    d.models = G.models; d.caxis = G.caxis; d.ref_mat = G.ref_mat; d.Cref = G.Cref;
+    return d;
+}
+
+// ------------------------------------------------------------ prepare
+static void mat_prepare() {
+    G.ref_mat = 0;
+    for (int i = 0; i < G.n_mats; i++) if (!G.mats[i].insulator) { G.ref_mat = i; break; }
+    const HostMat &r = G.mats[G.ref_mat];
+    G.Cref = r.insulator ? 4.6 : r.chi + 0.5 * r.Eg + 0.5 * r.VT * log(r.Nc / r.Nv);
+    for (int i = 0; i < G.n_mats; i++) {
+        HostMat &m = G.mats[i];
+        if (m.insulator) { m.lnc = m.lnv = -1e300; continue; }
+        m.lnc = log(m.Nc) + (m.chi - G.Cref) / m.VT;
+        m.lnv = log(m.Nv) + (G.Cref - m.chi - m.Eg) / m.VT;
+    }
+}
+
+// nodes of a face contact (face 0..5) or of a box (face 6), v6 index
+// convention, clipped to the mesh (semiconductor_3d.c:1047-1065)
+template <class F> static void for_con_nodes(const HostCon &cn, F body) {
+    int ia = 0, ib = G.Nx - 1, ja = 0, jb = G.Ny - 1, la = 0, lb = G.Nz - 1;
+    switch (cn.face) {
+    case 0: ia = ib = 0; ja = cn.i0; jb = cn.i1; la = cn.j0; lb = cn.j1; break;
+    case 1: ia = ib = G.Nx - 1; ja = cn.i0; jb = cn.i1; la = cn.j0; lb = cn.j1; break;
+    case 2: ja = jb = 0; ia = cn.i0; ib = cn.i1; la = cn.j0; lb = cn.j1; break;
+    case 3: ja = jb = G.Ny - 1; ia = cn.i0; ib = cn.i1; la = cn.j0; lb = cn.j1; break;
+    case 4: la = lb = 0; ia = cn.i0; ib = cn.i1; ja = cn.j0; jb = cn.j1; break;
+    case 5: la = lb = G.Nz - 1; ia = cn.i0; ib = cn.i1; ja = cn.j0; jb = cn.j1; break;
+    default: ia = cn.i0; ib = cn.i1; ja = cn.j0; jb = cn.j1; la = cn.k0; lb = cn.k1;
+    }
+    if (ia < 0) ia = 0; if (ja < 0) ja = 0; if (la < 0) la = 0;
+    if (ib > G.Nx - 1) ib = G.Nx - 1; if (jb > G.Ny - 1) jb = G.Ny - 1; if (lb > G.Nz - 1) lb = G.Nz - 1;
+    for (int l = la; l <= lb; l++)
+        for (int j = ja; j <= jb; j++)
+            for (int i = ia; i <= ib; i++) {
+                int k = i + G.Nx * (j + G.Ny * l);
+                double area = cn.face <= 1 ? G.Ly[j] * G.Lz[l] : (cn.face <= 3 ? G.Lx[i] * G.Lz[l] : (cn.face <= 5 ? G.Lx[i] * G.Ly[j] : 0.));
+                body(i, j, l, k, area);
+            }
+}
+
+static bool host_nbr(int i, int j, int l, int k, int dir, int &kb, double &h, double &ar) {
+    switch (dir) {
+    case 0: if (i == 0) return false; kb = k - 1; h = G.xs[i] - G.xs[i - 1]; ar = G.Ly[j] * G.Lz[l]; return true;
+    case 1: if (i == G.Nx - 1) return false; kb = k + 1; h = G.xs[i + 1] - G.xs[i]; ar = G.Ly[j] * G.Lz[l]; return true;
+    case 2: if (j == 0) return false; kb = k - G.Nx; h = G.ys[j] - G.ys[j - 1]; ar = G.Lx[i] * G.Lz[l]; return true;
+    case 3: if (j == G.Ny - 1) return false; kb = k + G.Nx; h = G.ys[j + 1] - G.ys[j]; ar = G.Lx[i] * G.Lz[l]; return true;
+    case 4: if (l == 0) return false; kb = k - G.Nx * G.Ny; h = G.zs[l] - G.zs[l - 1]; ar = G.Lx[i] * G.Ly[j]; return true;
+    default: if (l == G.Nz - 1) return false; kb = k + G.Nx * G.Ny; h = G.zs[l + 1] - G.zs[l]; ar = G.Lx[i] * G.Ly[j]; return true;
+    }
+}
+static void host_unidx(int k, int &i, int &j, int &l) {
+    int NxNy = G.Nx * G.Ny;
+    l = k / NxNy; int r = k - l * NxNy; j = r / G.Nx; i = r - j * G.Nx;
+}
+
+int eng_prepare() {
+    if (!G.allocated || G.n_mats < 1) return 0;
+    {   // physics outside this core is rejected where it would take effect, never approximated
+        const char *why = nullptr;
+        if (G.models & (8 | 16 | 32)) why = "quantum confinement / surface mobility / gate tunnelling (models bits 3-5)";
+        for (int c = 0; c < G.n_cons && !why; c++)
+            if ((G.models & (64 | 128)) && G.cons[c].bc == 1 && G.cons[c].face <= 5)
+                why = "metal-contact emission on a Schottky face contact (models bits 6-7)";
+        for (int i = 0; i < G.n_mats && !why; i++)
+            if ((G.models & 512) && G.mats[i].id == 4) why = "incomplete ionisation (models bit 9)";
+        if (why) { fprintf(stderr, "semisim-hip: %s is outside this core: not solved\n", why); return 0; }
+    }
+    if (!G.dirty) return 1;
+    const int N = G.N;
+    mat_prepare();
+    G.kind.assign(N, 0); G.mat8.assign(N, 0); G.eps.assign(N, 0.); G.bgn.assign(N, 0.);
+    G.con_mask.assign(N, 0); G.gate_mask.assign(N, 0); G.robA.assign(N, 0.);
+    for (int k = 0; k < N; k++) {
+        int mi = G.mat_idx[k];
+        if (mi < 0 || mi >= G.n_mats) mi = G.mat_idx[k] = 0;
+        const HostMat &m = G.mats[mi];
+        G.mat8[k] = (unsigned char)mi;
+        G.kind[k] = m.insulator ? K_INS : K_SEMI;
+        G.eps[k] = m.eps_r * SD_EPS0;
+        G.bgn[k] = (!m.insulator && (G.models & 1)) ? host_bgn_half(m, G.Nd[k] + G.Na[k]) : 0.;
+    }
+    // contacts; later contacts override earlier ones on shared nodes
+    int nrob = 0;
+    for (int c = 0; c < G.n_cons; c++) G.cons[c].te = 0;
+    for (int c = 0; c < G.n_cons; c++) {
+        HostCon &cn = G.cons[c];
+        cn.A = 0.;
+        for_con_nodes(cn, [&](int, int, int, int k, double area) {
+            if (cn.face <= 5) cn.A += area;
+            if (cn.bc == 2 && cn.face <= 5 && G.kind[k] == K_SEMI) { G.gate_mask[k] = c + 1; nrob++; }
+            else if (cn.bc == 2 || G.kind[k] != K_SEMI) { G.kind[k] = K_METAL; G.con_mask[k] = c + 1; G.gate_mask[k] = 0; }
+            else { G.con_mask[k] = c + 1; G.gate_mask[k] = 0; }
+        });
+    }
+    G.n_rob = 0;
+    if (nrob > 0) {
+        for (int c = 0; c < G.n_cons; c++) {
+            HostCon &cn = G.cons[c];
+            if (cn.bc != 2 || cn.face > 5) continue;
+            for_con_nodes(cn, [&](int, int, int, int k, double area) {
+                if (G.gate_mask[k] == c + 1 && !G.con_mask[k] && G.n_rob < nrob) { G.robA[k] = area; G.n_rob++; }
+            });
+        }
+    }
+    for (int k = 0; k < N; k++) if (G.gate_mask[k] && (G.con_mask[k] || G.robA[k] == 0.)) G.gate_mask[k] = 0;
+    // box contacts: area = interface with non-contact semiconductor
+    for (int c = 0; c < G.n_cons; c++) {
+        HostCon &cn = G.cons[c];
+        if (cn.face <= 5) continue;
+        double Ab = 0.;
+        for_con_nodes(cn, [&](int i, int j, int l, int k, double) {
+            if (G.con_mask[k] != c + 1) return;
+            for (int dir = 0; dir < 6; dir++) {
+                int kb; double h, ar;
+                if (host_nbr(i, j, l, k, dir, kb, h, ar) && G.kind[kb] == K_SEMI && G.con_mask[kb] != c + 1) Ab += ar;
+            }
+        });
+        cn.A = Ab;
+    }
+    // node band data and low-field mobility (constant for the device)
+    G.lncK.assign(N, -1e300); G.lnvK.assign(N, -1e300); G.niK.assign(N, 0.); G.mu0n.assign(N, 0.); G.mu0p.assign(N, 0.);
+    for (int k = 0; k < N; k++) {
+        const HostMat &m = G.mats[G.mat8[k]];
+        if (m.insulator) { G.lncK[k] = G.lnvK[k] = m.lnc; continue; }
+        G.lncK[k] = m.lnc + G.bgn[k];
+        G.lnvK[k] = m.lnv + G.bgn[k];
+        G.niK[k] = m.ni * exp(G.bgn[k]);
+        if (G.kind[k] == K_SEMI) {
+            double Nt = G.Nd[k] + G.Na[k];
+            G.mu0n[k] = host_mob_low(m, Nt, 0);
+            G.mu0p[k] = host_mob_low(m, Nt, 1);
+        }
+    }
+    // terminal-current edges, grouped by contact in node order
+    std::vector<int> ek, ekb, eoff(G.n_cons + 1, 0);
+    std::vector<double> eh, ear;
+    std::vector<std::vector<int>> per(G.n_cons);
+    for (int k = 0; k < N; k++) {
+        int c = G.con_mask[k];
+        if (!c || G.kind[k] != K_SEMI || G.cons[c - 1].te) continue;
+        per[c - 1].push_back(k);
+    }
+    for (int c = 0; c < G.n_cons; c++) {
+        eoff[c] = (int)ek.size();
+        for (int k : per[c]) {
+            int i, j, l;
+            host_unidx(k, i, j, l);
+            for (int dir = 0; dir < 6; dir++) {
+                int kb; double h, ar;
+                if (!host_nbr(i, j, l, k, dir, kb, h, ar) || G.kind[kb] != K_SEMI || G.con_mask[kb] == c + 1) continue;
+                ek.push_back(k); ekb.push_back(kb); eh.push_back(h); ear.push_back(ar);
+            }
+        }
+    }
+    eoff[G.n_cons] = (int)ek.size();
+    G.n_edges = (int)ek.size();
+    // upload
+    std::vector<double> geo;
+    geo.insert(geo.end(), G.xs.begin(), G.xs.end()); geo.insert(geo.end(), G.ys.begin(), G.ys.end());
+    geo.insert(geo.end(), G.zs.begin(), G.zs.end()); geo.insert(geo.end(), G.Lx.begin(), G.Lx.end());
+    geo.insert(geo.end(), G.Ly.begin(), G.Ly.end()); geo.insert(geo.end(), G.Lz.begin(), G.Lz.end());
+    hipStream_t s = G.st;
+    HC(hipMemcpyAsync(G.d_geo, geo.data(), geo.size() * 8, hipMemcpyHostToDevice, s));
+    HC(hipMemcpyAsync(G.d_kind, G.kind.data(), N, hipMemcpyHostToDevice, s));
+    HC(hipMemcpyAsync(G.d_mat, G.mat8.data(), N, hipMemcpyHostToDevice, s));
+    HC(hipMemcpyAsync(G.d_con_mask, G.con_mask.data(), N * 4, hipMemcpyHostToDevice, s));
+    HC(hipMemcpyAsync(G.d_gate_mask, G.gate_mask.data(), N * 4, hipMemcpyHostToDevice, s));
+    const std::vector<double> *hv[] = {&G.Nd, &G.Na, &G.Nf, &G.eps, &G.bgn, &G.robA, &G.lncK, &G.lnvK, &G.niK, &G.mu0n, &G.mu0p};
+    double *dv[] = {G.d_Nd, G.d_Na, G.d_Nf, G.d_eps, G.d_bgn, G.d_robA, G.d_lncK, G.d_lnvK, G.d_niK, G.d_mun, G.d_mup};
+    for (int q = 0; q < 11; q++) HC(hipMemcpyAsync(dv[q], hv[q]->data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
+    MatDev md[SD_NM_MAX];
+    for (int i = 0; i < G.n_mats; i++) md[i] = host_mat_dev(G.mats[i]);
+    HC(hipMemcpyAsync(G.d_mats, md, sizeof(MatDev) * G.n_mats, hipMemcpyHostToDevice, s));
+    // edge arrays are reallocated per prepare (small)
+    G.d_ek = dalloc<int>(std::max(1, G.n_edges)); G.d_ekb = dalloc<int>(std::max(1, G.n_edges));
+    G.d_eh = dalloc<double>(std::max(1, G.n_edges)); G.d_ear = dalloc<double>(std::max(1, G.n_edges));
+    G.d_eoff = dalloc<int>(G.n_cons + 1);
+    if (G.n_edges) {
+        HC(hipMemcpyAsync(G.d_ek, ek.data(), ek.size() * 4, hipMemcpyHostToDevice, s));
+        HC(hipMemcpyAsync(G.d_ekb, ekb.data(), ekb.size() * 4, hipMemcpyHostToDevice, s));
+        HC(hipMemcpyAsync(G.d_eh, eh.data(), eh.size() * 8, hipMemcpyHostToDevice, s));
+        HC(hipMemcpyAsync(G.d_ear, ear.data(), ear.size() * 8, hipMemcpyHostToDevice, s));
+    }
+    HC(hipMemcpyAsync(G.d_eoff, eoff.data(), eoff.size() * 4, hipMemcpyHostToDevice, s));
+    eng_sync();
+    G.dirty = 0;
+    return eng_ok();
+}
+
+// device contact table at the applied voltages Va
+void eng_upload_cons(const double *Va) {
+    ConDev cd[SD_NC_MAX];
+    for (int c = 0; c < G.n_cons; c++) {
+        const HostCon &cn = G.cons[c];
+        cd[c].V = Va[c];
+        cd[c].phi_m = cn.phi_m;
+        cd[c].Cox = SD_EPS_OX / (cn.tox > 0. ? cn.tox : 10e-9);
+        double pm = cn.phi_m > 0. ? cn.phi_m : G.Cref;
+        cd[c].VGp = Va[c] + G.Cref - pm;
+        cd[c].qox = cn.qox;
+        cd[c].bc = cn.bc;
+        cd[c].te = cn.te;
+    }
+    if (G.n_cons) HC(hipMemcpyAsync(G.d_cons, cd, sizeof(ConDev) * G.n_cons, hipMemcpyHostToDevice, G.st));
+    // the host array is a stack temporary: finish the copy before returning
+    eng_sync();
+}
+
+void eng_set_vrange(const double *Va) {
+    int any = 0;
+    double lo = 0., hi = 0.;
+    for (int c = 0; c < G.n_cons; c++) {
+        if (G.cons[c].bc == 2 && !G.cons[c].ntp) continue;
+        if (!any) { lo = hi = Va[c]; any = 1; }
+        else { if (Va[c] < lo) lo = Va[c]; if (Va[c] > hi) hi = Va[c]; }
+    }
+    G.vlo = lo; G.vhi = hi;
+}
+void eng_clamp_qf() { ph_clamp_qf(G.st, eng_view(), G.vlo, G.vhi); }
+void eng_rebuild_np() { ph_rebuild_np(G.st, eng_view()); }
+void eng_upd_mob() { ph_upd_mob(G.st, eng_view(), G.d_phin, G.d_phip); }
+void eng_apply_bc() { ph_apply_bc(G.st, eng_view(), G.have_solution); }
+static void apply_bc(const double *Va) {
+    eng_upload_cons(Va);
+    eng_apply_bc();
+}
+
+// ------------------------------------------------------------ multigrid
+template <class T> static Hier<T> &hier();
+template <> Hier<double> &hier<double>() { return G.H64; }
+template <> Hier<float> &hier<float>() { return G.H32; }
+
+template <class T> static void mg_setup_T() {
+    Hier<T> &H = hier<T>();
+    hipStream_t s = G.st;
+    DevView d = eng_view();
+    int nl = (int)H.lev.size();
+    if (G.mg_qsign == 0) {
+        if (!H.pw_one) {
+            for (int L = 0; L < nl; L++) mg_set_pw_one(s, H.lev[L]);
+            H.pw_one = true;
+        }
+    } else {
+        double VT = G.mats[G.ref_mat].VT;
+        for (int L = 0; L < nl - 1; L++) mg_weights_level(s, H.lev[L], H.lev[L + 1], L, d, G.mg_qsign, VT);
+        H.pw_one = false;
+    }
+    H.lev[0].rs = const_cast<double *>(G.mg_w);
+    for (int L = 0; L < nl - 1; L++) mg_galerkin(s, H.lev[L], H.lev[L + 1]);
+    mg_coarse_factor(s, H.lev[nl - 1], H.inv);
+}
+static void mg_setup() {
+    double t0 = wtime();
+    if (G.precision == 1) {
+        mg_sten_to_float(G.st, G.A, G.H32.lev[0].A, G.N);
+        mg_setup_T<float>();
+    } else mg_setup_T<double>();
+    G.t_mg += wtime() - t0;
+}
+
+template <class T> static void vcycle_T(int L, const T *b, T *x) {
+    Hier<T> &H = hier<T>();
+    hipStream_t s = G.st;
+    int nl = (int)H.lev.size();
+    MGLevel<T> &v = H.lev[L];
+    if (L == nl - 1) { mg_coarse_solve(s, v, H.inv, b, x); return; }
+    MGLevel<T> &c = H.lev[L + 1];
+    mg_rbgs(s, v, b, x, 0, 1);          // pre-smoothing from x = 0: red, black
+    mg_rbgs(s, v, b, x, 1, 0);
+    mg_resid_restrict(s, v, c, b, x);   // coarse right-hand side
+    vcycle_T<T>(L + 1, c.b, c.x);
+    mg_prolong(s, v, c, x);
+    mg_rbgs(s, v, b, x, 1, 0);          // post-smoothing: black, red (symmetric V-cycle)
+    mg_rbgs(s, v, b, x, 0, 0);
+}
+// smoother only (the reference's ILU-only retry): one symmetric sweep from 0
+template <class T> static void smooth_only_T(const T *b, T *x) {
+    Hier<T> &H = hier<T>();
+    mg_rbgs(G.st, H.lev[0], b, x, 0, 1);
+    mg_rbgs(G.st, H.lev[0], b, x, 1, 0);
+    mg_rbgs(G.st, H.lev[0], b, x, 0, 0);
+}
+// z = M^-1 r
+static void precond(const double *r, double *z) {
+    G.n_vcycles++;
+    if (G.precision == 1) {
+        mg_to_float(G.st, r, G.d_b32, G.N);
+        if (G.mg_off || G.H32.lev.size() < 2) smooth_only_T<float>(G.d_b32, G.d_x32);
+        else vcycle_T<float>(0, G.d_b32, G.d_x32);
+        mg_to_double(G.st, G.d_x32, z, G.N);
+    } else {
+        if (G.mg_off || G.H64.lev.size() < 2) smooth_only_T<double>(r, z);
+        else vcycle_T<double>(0, r, z);
+    }
+}
+static inline void matvec(const double *x, double *y) {
+    G.n_spmv++;
+    sd_spmv(G.st, eng_view().g, G.A, x, nullptr, y);
+}
+
+// ------------------------------------------------------------ Krylov
+// Preconditioned CG on the SPD Poisson Jacobian; stops on the M^-1 norm
+static int pcg(const double *b, double *x, int maxit, double rtol) {
+    const int N = G.N;
+    hipStream_t s = G.st;
+    double *r = G.d_wk[0], *z = G.d_wk[1], *p = G.d_wk[2], *q = G.d_wk[3];
+    mg_setup();
+    Geo g = eng_view().g;
+    sd_spmv(s, g, G.A, x, b, r);          // r = b - A x
+    precond(r, z);
+    sd_dot(s, r, z, N, G.rb, S_RZ);
+    rd(S_RZ, S_RZ + 1);
+    double rz0 = S(S_RZ);
+    if (!(rz0 > 0.)) return 0;
+    sd_copy(s, p, z, N);
+    int it;
+    for (it = 1; it <= maxit; it++) {
+        matvec(p, q);
+        sd_dot(s, p, q, N, G.rb, S_PQ);
+        sd_scalar_op(s, SOP_PCG_AL, G.rb);
+        sd_pcg_update_xr(s, x, r, p, q, N, G.rb);
+        precond(r, z);
+        sd_dot(s, r, z, N, G.rb, S_TMP0);
+        rd(S_RZ, S_TMP0 + 1);
+        double pq = S(S_PQ), rzn = S(S_TMP0);
+        if (!(pq > 0.)) {
+            if (G.verbose > 3) fprintf(stderr, "      pcg breakdown pq=%.3e it=%d\n", pq, it);
+            break;
+        }
+        if (!(rzn > rtol * rtol * rz0)) break;
+        sd_scalar_op(s, SOP_PCG_BE, G.rb);
+        sd_pcg_update_p(s, p, z, N, G.rb);
+    }
+    if (G.verbose > 3) fprintf(stderr, "      pcg its=%d rz0=%.3e\n", it, rz0);
+    G.pcg_its += it;
+    return it;
+}
+
+static double bicgstab_core(const double *b, double *x, int maxit, double rtol, double atol, int *its, double *r0out) {
+    const int N = G.N;
+    hipStream_t s = G.st;
+    double *r = G.d_wk[0], *rh = G.d_wk[1], *p = G.d_wk[2], *v = G.d_wk[3], *ph = G.d_wk[4], *sh = G.d_wk[5],
+           *t = G.d_wk[6];
+    Geo g = eng_view().g;
+    if (!G.mg_off) mg_setup();
+    matvec(x, t);
+    sd_bi_init(s, r, rh, p, v, b, t, N, G.rb);
+    sd_floor(s, g, G.A, x, b, G.rb, S_FL);
+    rd(S_RR, S_FL + 1);
+    double rr = S(S_RR), fl = S(S_FL);
+    double r0 = sqrt(rr), tgt = fmax(rtol * r0, atol * sqrt(fl));
+    if (G.verbose > 3) fprintf(stderr, "      r0=%.3e tgt=%.3e\n", r0, tgt);
+    *its = 0;
+    *r0out = r0;
+    if (!(r0 > tgt)) return r0;
+    sd_scalar_op(s, SOP_BI_RESTART, G.rb);       // rho = al = om = 1
+    double rbest = r0;
+    int stall = 0, it, restarted = 0;
+    for (it = 1; it <= maxit; it++) {
+        sd_dot(s, rh, r, N, G.rb, S_RHO1);
+    again:
+        sd_scalar_op(s, SOP_BI_BE, G.rb);
+        sd_bi_p(s, p, r, v, N, G.rb);
+        precond(p, ph);
+        matvec(ph, v);
+        sd_dot(s, rh, v, N, G.rb, S_RV);
+        sd_scalar_op(s, SOP_BI_AL, G.rb);
+        sd_bi_r1(s, r, v, N, G.rb);
+        rd(S_AL, S_SS + 1);
+        double rho1 = S(S_RHO1), rv = S(S_RV), ss = S(S_SS);
+        if (fabs(rho1) < 1e-300 && !restarted) {
+            // breakdown: restart the shadow residual (al was ~0, r unchanged)
+            sd_bi_restart(s, rh, p, v, r, N);
+            sd_dot(s, r, r, N, G.rb, S_RHO1);
+            sd_scalar_op(s, SOP_BI_RESTART, G.rb);
+            rd(S_RHO1, S_RHO1 + 1);
+            if (!(S(S_RHO1) > 0.)) break;
+            restarted = 1;
+            goto again;
+        }
+        restarted = 0;
+        if (fabs(rv) < 1e-300) break;
+        if (sqrt(ss) <= tgt) {
+            sd_bi_x1(s, x, ph, N, G.rb);
+            rr = ss;
+            break;
+        }
+        precond(r, sh);
+        matvec(sh, t);
+        sd_dot2(s, t, t, r, N, G.rb, S_TT, S_TS);
+        sd_scalar_op(s, SOP_BI_OM, G.rb);
+        sd_bi_xr2(s, x, r, ph, sh, t, N, G.rb);
+        rd(S_OM, S_RR + 1);
+        double om = S(S_OM);
+        rr = S(S_RR);
+        double rn = sqrt(rr);
+        if (G.verbose > 5) fprintf(stderr, "        it %d rn=%.3e om=%.3e\n", it, rn, om);
+        if (rn <= tgt || om == 0.) break;
+        if (rn < 0.99 * rbest) { rbest = rn; stall = 0; }
+        else if (++stall > 40) break;
+        sd_scalar_op(s, SOP_BI_NEXT, G.rb);
+    }
+    if (it > maxit) it = maxit;
+    G.bicg_its += it;
+    *its = it;
+    return sqrt(rr);
+}
+
+static double bicgstab(const double *b, double *x, int maxit, double rtol, double atol, int *its) {
+    const int N = G.N;
+    sd_copy(G.st, G.d_xsave, x, N);
+    double r0, rn = bicgstab_core(b, x, maxit, rtol, atol, its, &r0);
+    int nl = (int)(G.precision == 1 ? G.H32.lev.size() : G.H64.lev.size());
+    if (!(rn <= fmax(r0, 1e-300)) && nl > 1) {
+        if (G.verbose) fprintf(stderr, "semisim-hip: MG-BiCGSTAB diverged (%.2e > %.2e), retry smoother only\n", rn, r0);
+        sd_copy(G.st, x, G.d_xsave, N);
+        int its2;
+        G.mg_off = 1;
+        rn = bicgstab_core(b, x, 4 * maxit, rtol, atol, &its2, &r0);
+        G.mg_off = 0;
+        *its += its2;
+    }
+    if (!(rn <= fmax(r0, 1e-300)) || !std::isfinite(rn)) {
+        sd_copy(G.st, x, G.d_xsave, N);
+        return INFINITY;
+    }
+    return rn;
+}
+
+// ------------------------------------------------------------ Poisson
+static double poisson_eval(const double *phi, int build, double *F) {
+    ph_poisson(G.st, eng_view(), phi, build, F, G.A, G.d_sc, G.rb, S_NRM);
+    rd(S_NRM, S_NRM + 1);
+    return sqrt(S(S_NRM));
+}
+
+static int poisson_newton(double ntol, int maxit) {
+    const int N = G.N;
+    hipStream_t s = G.st;
+    double *dx = G.d_sol, *tr = G.d_tr, *Ft = G.d_Ft;
+    for (int it = 0; it < maxit; it++) {
+        G.newton_its++;
+        double f0 = poisson_eval(G.d_psi, 1, G.d_rhs);
+        if (!(f0 > 0.)) return f0 == 0.;
+        sd_fill(s, dx, 0.0, N);
+        G.mg_w = nullptr;
+        G.mg_qsign = 0;
+        pcg(G.d_rhs, dx, G.max_iter_p > 0 ? G.max_iter_p : 2000, G.pcg_rtol);
+        sd_maxabs(s, dx, N, G.rb, S_MAX0);
+        rd(S_MAX0, S_MAX0 + 1);
+        double dm = S(S_MAX0);
+        if (dm != dm) return 0;
+        if (dm < ntol) {
+            ph_update_psi(s, N, G.d_psi, dx, 1.0, G.d_psi);
+            return 1;
+        }
+        double lam = 1.;
+        for (int ls = 0; ls < 40; ls++) {
+            ph_update_psi(s, N, G.d_psi, dx, lam, tr);
+            double f1 = poisson_eval(tr, 0, Ft);
+            if (f1 <= (1. - 1e-4 * lam) * f0) break;
+            lam *= 0.5;
+        }
+        sd_copy(s, G.d_psi, tr, N);
+        if (G.verbose > 4) fprintf(stderr, "      newton it=%d f0=%.3e dm=%.3e lam=%.3e\n", it, f0, dm, lam);
+        if (lam * dm < ntol) return 1;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------ continuity
+// Floating majority regions (semiconductor_3d.c:3159-3228): the device finds
+// whether any majority block lacks a contact; only then the block balance
+// runs on the host.
+static int float_fix_host(int hole);
+static int float_fix(int hole) {
+    DevView d = eng_view();
+    hipStream_t s = G.st;
+    ph_maj_flags(s, d, hole, G.d_sol, G.d_sc, G.d_flag);
+    for (int round = 0; round < 64; round++) {
+        HC(hipMemsetAsync(G.d_changed, 0, 4, s));
+        for (int rep = 0; rep < 2; rep++)
+            for (int ax = 0; ax < 3; ax++) ph_maj_propagate(s, d, G.d_flag, ax, G.d_changed);
+        int ch = 0;
+        HC(hipMemsetAsync(G.d_changed + 1, 0, 4, s));
+        // second pass over the axes must report no change
+        for (int ax = 0; ax < 3; ax++) ph_maj_propagate(s, d, G.d_flag, ax, G.d_changed + 1);
+        HC(hipMemcpyAsync(&G.h_scal[S_TMP3], G.d_changed + 1, 4, hipMemcpyDeviceToHost, s));
+        eng_sync();
+        memcpy(&ch, &G.h_scal[S_TMP3], 4);
+        if (!ch) break;
+    }
+    ph_maj_count(s, G.N, G.d_flag, G.rb, S_TMP2);
+    rd(S_TMP2, S_TMP2 + 1);
+    if (S(S_TMP2) < 0.5) return 0;
+    return float_fix_host(hole);
+}
+
+// host block balance, only when a floating block exists
+static int float_fix_host(int hole) {
+    const int N = G.N;
+    std::vector<double> u(N), sc(N), oth(N), psi(N), fmu(6 * (size_t)N), car(N);
+    eng_download(u.data(), G.d_sol, N);
+    eng_download(sc.data(), G.d_sc, N);
+    eng_download(oth.data(), hole ? G.d_n : G.d_p, N);
+    eng_download(car.data(), hole ? G.d_p : G.d_n, N);
+    eng_download(psi.data(), G.d_psi, N);
+    eng_download(fmu.data(), G.d_fmu, 6 * (size_t)N);
+    std::vector<double> nn(N), pp(N);
+    eng_download(nn.data(), G.d_n, N);
+    eng_download(pp.data(), G.d_p, N);
+    auto dnet = [&](int k) { return fabs(G.Nd[k] - G.Na[k]); };
+    auto MAJ = [&](int k) {
+        double c = sc[k] * u[k];
+        return G.kind[k] == K_SEMI && !G.con_mask[k] && c >= oth[k] && c >= 0.5 * dnet(k) && sc[k] >= 0.5 * dnet(k) &&
+               c > 10. * G.niK[k];
+    };
+    std::vector<int> lab(N, 0), q(N);
+    int nfix = 0, blk = 0;
+    for (int k0 = 0; k0 < N; k0++) {
+        if (lab[k0] || !MAJ(k0)) continue;
+        blk++;
+        int head = 0, tail = 0;
+        q[tail++] = k0;
+        lab[k0] = blk;
+        int anch = 0;
+        while (head < tail) {
+            int k = q[head++], i, j, l;
+            host_unidx(k, i, j, l);
+            for (int dir = 0; dir < 6; dir++) {
+                int kb; double h, ar;
+                if (!host_nbr(i, j, l, k, dir, kb, h, ar) || G.kind[kb] != K_SEMI) continue;
+                if (G.con_mask[kb]) { anch = 1; continue; }
+                if (!lab[kb] && MAJ(kb)) { lab[kb] = blk; q[tail++] = kb; }
+            }
+        }
+        if (anch) continue;
+        double Aout = 0., Bin = 0.;
+        for (int t = 0; t < tail; t++) {
+            int k = q[t], i, j, l;
+            host_unidx(k, i, j, l);
+            const HostMat &ma = G.mats[G.mat8[k]];
+            double VT = ma.VT;
+            for (int dir = 0; dir < 6; dir++) {
+                int kb; double h, ar;
+                if (!host_nbr(i, j, l, k, dir, kb, h, ar) || G.kind[kb] != K_SEMI || lab[kb] == blk) continue;
+                double dl = hole ? -(G.lnvK[kb] - G.lnvK[k]) : (G.lncK[kb] - G.lncK[k]);
+                double D = (psi[kb] - psi[k]) / VT + dl;
+                int kl = k < kb ? k : kb;
+                double gg = ar * fmu[(size_t)((hole ? 3 : 0) + dir / 2) * N + kl] * VT / h;
+                double self = hole ? gg * host_B(D) : gg * host_B(-D), nb = hole ? gg * host_B(-D) : gg * host_B(D);
+                double cb = G.con_mask[kb] ? car[kb] : sc[kb] * u[kb];
+                Aout += self * sc[k] * u[k];
+                Bin += nb * cb;
+            }
+        }
+        double sl = 0.;
+        for (int it = 0; it < 60; it++) {
+            double lam = exp(sl), Gv = lam * Aout - Bin, dG = lam * Aout;
+            for (int t = 0; t < tail; t++) {
+                int k = q[t], i, j, l;
+                host_unidx(k, i, j, l);
+                const HostMat &m = G.mats[G.mat8[k]];
+                double c = lam * sc[k] * u[k], R, dRn, dRp;
+                double tf = ((G.models & 2) && m.tau_Nref > 0.) ? 1. / (1. + (G.Nd[k] + G.Na[k]) / m.tau_Nref) : 1.;
+                host_recomb(m, G.niK[k], tf, hole ? oth[k] : c, hole ? c : oth[k], &R, &dRn, &dRp);
+                double vol = G.Lx[i] * G.Ly[j] * G.Lz[l];
+                Gv += vol * R;
+                dG += vol * (hole ? dRp : dRn) * c;
+            }
+            if (!(dG > 0.)) break;
+            double ds = -Gv / dG;
+            if (ds > 2.) ds = 2.; else if (ds < -2.) ds = -2.;
+            sl += ds;
+            if (fabs(ds) < 1e-13) break;
+        }
+        if (sl == sl && fabs(sl) > 1e-15 && fabs(sl) < 200.) {
+            double lam = exp(sl);
+            for (int t = 0; t < tail; t++) u[q[t]] *= lam;
+            nfix++;
+        }
+    }
+    (void)nn; (void)pp;
+    if (nfix) eng_upload(G.d_sol, u.data(), N);
+    return nfix;
+}
+
+static double CONT_ATOL = 2e-14;
+static int solve_cont(int hole, double *dq, double lo, double hi, double *dmax) {
+    const int N = G.N;
+    hipStream_t s = G.st;
+    DevView d = eng_view();
+    ph_cont(s, d, hole, G.A, G.d_rhs, G.d_sc, G.d_dsc);
+    G.mg_w = G.d_dsc;
+    G.mg_qsign = hole ? -1 : 1;
+    sd_fill(s, G.d_sol, 1.0, N);
+    double atol = CONT_ATOL * G.cont_atol_f;
+    int its;
+    double rn = bicgstab(G.d_rhs, G.d_sol, G.max_iter_c > 0 ? G.max_iter_c : 1000, G.bicg_rtol, atol, &its);
+    if (G.verbose > 2) fprintf(stderr, "    bicgstab(%c) its=%d |r|=%.3e\n", hole ? 'p' : 'n', its, rn);
+    int ok = std::isfinite(rn);
+    if (ok) float_fix(hole);
+    ph_cont_update(s, d, hole, G.d_sol, G.d_sc, dq, lo, hi, G.rb, S_MAX1);
+    rd(S_MAX1, S_MAX1 + 1);
+    *dmax = S(S_MAX1);
+    return ok;
+}
+
+static void terminal_currents(double *I, double *Sv) {
+    CurEdges e;
+    e.k = G.d_ek; e.kb = G.d_ekb; e.h = G.d_eh; e.ar = G.d_ear; e.off = G.d_eoff; e.ncon = G.n_cons;
+    ph_currents(G.st, eng_view(), e, G.rb);
+    rd(S_CUR, S_CUR + 3 * std::max(1, G.n_cons));
+    for (int c = 0; c < G.n_cons; c++) {
+        I[c] = S(S_CUR + 3 * c);
+        Sv[c] = S(S_CUR + 3 * c + 1);
+        G.cons[c].In = S(S_CUR + 3 * c + 2);
+    }
+}
+
+// ------------------------------------------------------------ Anderson
+static int AA_DEPTH = 12;
+static void aa_reset() { G.aa_n = 0; G.aa_pos = 0; G.aa_have = 0; }
+static void aa_step(double *const x[3], double *const g[3]) {
+    const int N = G.N;
+    const long L = 3L * N;
+    hipStream_t s = G.st;
+    DevView d = eng_view();
+    int M = G.aa_m < AA_DEPTH ? G.aa_m : AA_DEPTH;
+    if (M > 0 && G.aa_have) {
+        int sl = G.aa_pos;
+        ph_aa_diff(s, d, x[0], x[1], x[2], g[0], g[1], g[2], G.d_aaf, G.d_aag, G.d_aadF[sl], G.d_aadG[sl], 1);
+        if (G.aa_n < M) G.aa_n++;
+        ph_aa_dots(s, (int)L, G.d_aadF, G.aa_n, G.d_aadF[sl], nullptr, G.rb, S_AA0);
+        rd(S_AA0, S_AA0 + G.aa_n);
+        for (int i = 0; i < G.aa_n; i++) G.aa_gram[i][sl] = G.aa_gram[sl][i] = S(S_AA0 + i);
+        G.aa_pos = (sl + 1) % M;
+    } else {
+        ph_aa_diff(s, d, x[0], x[1], x[2], g[0], g[1], g[2], G.d_aaf, G.d_aag, nullptr, nullptr, 0);
+    }
+    G.aa_have = 1;
+    int m = G.aa_n;
+    if (m == 0) return;
+    double A[12][12], bb[12], gam[12];
+    ph_aa_dots(s, (int)L, G.d_aadF, m, nullptr, G.d_aaf, G.rb, S_AA0);
+    rd(S_AA0, S_AA0 + m);
+    for (int i = 0; i < m; i++) { bb[i] = S(S_AA0 + i); for (int j = 0; j < m; j++) A[i][j] = G.aa_gram[i][j]; }
+    double tr = 0.;
+    for (int i = 0; i < m; i++) tr += A[i][i];
+    for (int i = 0; i < m; i++) A[i][i] += 1e-9 * tr + 1e-300;
+    for (int c = 0; c < m; c++) {
+        int pv = c;
+        for (int r = c + 1; r < m; r++) if (fabs(A[r][c]) > fabs(A[pv][c])) pv = r;
+        if (pv != c) {
+            for (int q = 0; q < m; q++) std::swap(A[c][q], A[pv][q]);
+            std::swap(bb[c], bb[pv]);
+        }
+        for (int r = c + 1; r < m; r++) {
+            double f = A[r][c] / A[c][c];
+            for (int q = c; q < m; q++) A[r][q] -= f * A[c][q];
+            bb[r] -= f * bb[c];
+        }
+    }
+    for (int r = m - 1; r >= 0; r--) {
+        double t = bb[r];
+        for (int q = r + 1; q < m; q++) t -= A[r][q] * gam[q];
+        gam[r] = t / A[r][r];
+    }
+    for (int i = 0; i < m; i++) if (!(gam[i] == gam[i])) return;
+    ph_aa_mix(s, d, g[0], g[1], g[2], G.d_aadG, gam, m);
+}
+
+// ------------------------------------------------------------ Gummel
+static double newton_tol(const double *Va) {
+    double sc = 1.;
+    for (int c = 0; c < G.n_cons; c++) if (fabs(Va[c]) > sc) sc = fabs(Va[c]);
+    return 2e-14 + 2e-15 * sc;
+}
+
+static int gummel(const double *Va, int maxit, double gtol, int *iters, double *res) {
+    const int N = G.N, nc = G.n_cons;
+    hipStream_t s = G.st;
+    DevView d = eng_view();
+    double lo = G.vlo, hi = G.vhi, nt0 = newton_tol(Va);
+    double I[SD_NC_MAX], Sv[SD_NC_MAX], Iold[SD_NC_MAX];
+    int haveI = 0;
+    double itol = G.tol_c > 0. ? G.tol_c : 1e-6;
+    const int iion = (G.models & 256) != 0;
+    const double tol_ii = fmax(itol, 1e-4);
+    double Gs_old = -1., dG_old = 0., ii_rel = 1., ii_best = 1e300, Erem = 0.;
+    double brtol0 = G.bicg_rtol;
+    if (iion && G.bicg_rtol > 1e-8) G.bicg_rtol = 1e-8;
+    G.ii_err = 0.;
+    apply_bc(Va);
+    aa_reset();
+    poisson_newton(fmax(nt0, 1e-6), 100);
+    int it, conv = 0, it_mark = 0;
+    double dpsi = 1., dq = 1., best = 1e300, prev = 1., best_mark = 1e300;
+    for (it = 1; it <= maxit; it++) {
+        if (g_stop) break;
+        double t0 = wtime();
+        eng_clamp_qf();
+        sd_copy(s, G.d_qn0, G.d_phin, N);
+        sd_copy(s, G.d_qp0, G.d_phip, N);
+        eng_rebuild_np();
+        eng_upd_mob();
+        int iiok = 1;
+        if (iion) {
+            ph_ii_update(s, d);
+            sd_dot(s, G.d_gii, nullptr, N, G.rb, S_TMP1);        // total generation (sum)
+            rd(S_TMP1, S_TMP1 + 1);
+            double Gs = S(S_TMP1);
+            if (Gs > 0.) {
+                if (Gs_old < 0.) { iiok = 0; Erem = Gs; }
+                else {
+                    double dG = fabs(Gs - Gs_old), rho = dG_old > 0. ? dG / dG_old : 0.;
+                    if (rho > 0.99) rho = 0.99;
+                    Erem = dG + dG * rho / (1. - rho);
+                    iiok = dG <= tol_ii * Gs && Erem <= tol_ii * Gs;
+                    dG_old = dG;
+                }
+                ii_rel = Erem / Gs;
+                Gs_old = Gs;
+            } else { Erem = 0.; ii_rel = 0.; }
+        }
+        double t1 = wtime();
+        G.t_mob += t1 - t0;
+        double dn = 0., dp = 0.;
+        if (!solve_cont(0, G.d_dqn, lo, hi, &dn)) break;
+        if (!solve_cont(1, G.d_dqp, lo, hi, &dp)) break;
+        double t2 = wtime();
+        G.t_cont += t2 - t1;
+        dq = fmax(dn, dp);
+        terminal_currents(I, Sv);
+        G.t_cur += wtime() - t2;
+        int icv = haveI;
+        if (haveI)
+            for (int c = 0; c < nc; c++)
+                if (fabs(I[c] - Iold[c]) > fmax(itol * fabs(I[c]), 3e-14 * Sv[c])) icv = 0;
+        if (iion && !iiok) {
+            double Sm = 0.;
+            for (int c = 0; c < nc; c++) if (Sv[c] > Sm) Sm = Sv[c];
+            if (SD_Q * Erem <= 3e-14 * Sm) iiok = 1;
+        }
+        memcpy(Iold, I, sizeof(double) * nc);
+        haveI = 1;
+        sd_copy(s, G.d_xk, G.d_psi, N);
+        ph_neutral_pred(s, d, G.d_dqn, G.d_dqp);
+        double ntol = fmax(nt0, G.ntol_fac * fmin(prev, 1.));
+        double t3 = wtime();
+        poisson_newton(ntol, 100);
+        G.t_pois += wtime() - t3;
+        sd_maxabsdiff(s, G.d_psi, G.d_xk, N, G.rb, S_MAX2);
+        rd(S_MAX2, S_MAX2 + 1);
+        dpsi = S(S_MAX2);
+        if (G.verbose > 1)
+            fprintf(stderr, "  gummel %3d dpsi=%.3e dq=%.3e I0=%.6e aa=%d\n", it, dpsi, dq, nc ? I[0] : 0., G.aa_n);
+        if (dpsi != dpsi || dq != dq) break;
+        double r = fmax(dpsi, dq);
+        g_prog[1] = it; g_prog[2] = r; g_prog[5] = (double)(G.gummel_its + it);
+        if (dq < gtol && dpsi < gtol && icv && iiok) {      // accept psi_k with its carriers
+            sd_copy(s, G.d_psi, G.d_xk, N);
+            conv = 1;
+            break;
+        }
+        prev = r;
+        if (r < 0.5 * best_mark) { best_mark = r; it_mark = it; }
+        if (iion && ii_rel < 0.5 * ii_best) { ii_best = ii_rel; it_mark = it; }
+        if (it - it_mark > 20 && r <= 1e3 * gtol && icv && iiok) {
+            sd_copy(s, G.d_psi, G.d_xk, N);
+            conv = 1;
+            if (G.verbose) fprintf(stderr, "semisim-hip: accepted at noise floor (res %.2e)\n", r);
+            break;
+        }
+        if (it - it_mark > 60) break;
+        if (r < best) best = r;
+        else if (r > 30. * best) aa_reset();
+        double t4 = wtime();
+        double *xx[3] = {G.d_xk, G.d_qn0, G.d_qp0}, *gg[3] = {G.d_psi, G.d_phin, G.d_phip};
+        aa_step(xx, gg);
+        G.t_aa += wtime() - t4;
+    }
+    if (it > maxit) it = maxit;
+    if (iion && brtol0 > 1e-8 && G.bicg_rtol == 1e-8) G.bicg_rtol = brtol0;
+    if (iion) G.ii_err = SD_Q * Erem;
+    G.gummel_its += it;
+    *iters = it;
+    *res = fmax(dpsi, dq);
+    return conv;
+}
+
+// ------------------------------------------------------------ continuation
+static void save_state(double **h) {
+    sd_copy(G.st, h[0], G.d_psi, G.N); sd_copy(G.st, h[1], G.d_phin, G.N); sd_copy(G.st, h[2], G.d_phip, G.N);
+}
+static void load_state(double **h) {
+    sd_copy(G.st, G.d_psi, h[0], G.N); sd_copy(G.st, G.d_phin, h[1], G.N); sd_copy(G.st, G.d_phip, h[2], G.N);
+}
+
+// quasi-neutral regions of ohmic contacts move rigidly with their voltage (host BFS)
+static void region_shift(const double *dV) {
+    const int N = G.N;
+    bool any = false;
+    for (int c = 0; c < G.n_cons; c++) if (G.cons[c].bc == 0 && fabs(dV[c]) >= 1e-15) any = true;
+    if (!any) return;
+    std::vector<double> n(N), p(N), psi(N), qn(N), qp(N);
+    eng_download(n.data(), G.d_n, N); eng_download(p.data(), G.d_p, N);
+    eng_download(psi.data(), G.d_psi, N); eng_download(qn.data(), G.d_phin, N); eng_download(qp.data(), G.d_phip, N);
+    std::vector<unsigned char> mark(N, 0);
+    std::vector<int> q(N);
+    for (int c = 0; c < G.n_cons; c++) {
+        if (G.cons[c].bc != 0 || fabs(dV[c]) < 1e-15) continue;
+        int head = 0, tail = 0, type = 0;
+        for (int k = 0; k < N; k++)
+            if (G.con_mask[k] == c + 1 && G.kind[k] == K_SEMI && !mark[k]) {
+                mark[k] = (unsigned char)(c + 1);
+                q[tail++] = k;
+                if (!type) { double D = G.Nd[k] - G.Na[k]; type = D > 0. ? 1 : (D < 0. ? -1 : 0); }
+            }
+        if (!type) continue;
+        while (head < tail) {
+            int k = q[head++], i, j, l;
+            host_unidx(k, i, j, l);
+            for (int dir = 0; dir < 6; dir++) {
+                int kb; double h, ar;
+                if (!host_nbr(i, j, l, k, dir, kb, h, ar) || mark[kb] || G.kind[kb] != K_SEMI) continue;
+                if (G.con_mask[kb] && G.con_mask[kb] != c + 1) continue;
+                double D = G.Nd[kb] - G.Na[kb];
+                if (D * type <= 0.) continue;
+                double maj = type > 0 ? n[kb] : p[kb];
+                if (maj < 0.5 * fabs(D)) continue;
+                mark[kb] = (unsigned char)(c + 1);
+                q[tail++] = kb;
+            }
+        }
+        for (int t = 0; t < tail; t++) {
+            int k = q[t];
+            if (G.con_mask[k]) continue;
+            psi[k] += dV[c]; qn[k] += dV[c]; qp[k] += dV[c];
+        }
+    }
+    eng_upload(G.d_psi, psi.data(), N); eng_upload(G.d_phin, qn.data(), N); eng_upload(G.d_phip, qp.data(), N);
+}
+
+int eng_equilibrium() {
+    double Va[SD_NC_MAX];
+    G.e_n = 0;
+    for (int c = 0; c < G.n_cons; c++) Va[c] = G.cons[c].bc == 2 ? G.cons[c].V : 0.;
+    DevView d = eng_view();
+    eng_upload_cons(Va);
+    ph_init_eq(G.st, d);
+    eng_set_vrange(Va);
+    eng_apply_bc();
+    g_prog[4] = 1.;
+    int ok = poisson_newton(newton_tol(Va), 300);
+    eng_rebuild_np();
+    eng_upd_mob();
+    for (int c = 0; c < G.n_cons; c++) G.cons[c].Vprev = Va[c];
+    G.have_solution = 1;
+    if (G.verbose) fprintf(stderr, "semisim-hip: equilibrium %s (newton its %lld)\n", ok ? "ok" : "NOT converged", G.newton_its);
+    return ok;
+}
+
+static int PRED_ON = 1;
+static double GROW_SLOW = 2.2;
+static double gummel_tol() {
+    double g = G.tol_p;
+    if (!(g > 0.)) g = 1e-9;
+    if (g > 1e-4) g = 1e-4;
+    if (g < 1e-13) g = 1e-13;
+    return g;
+}
+
+static int cont_path() {
+    const int nc = G.n_cons, N = G.N;
+    double Vf[SD_NC_MAX], Vt[SD_NC_MAX], Va[SD_NC_MAX], dV[SD_NC_MAX];
+    double span = 0.;
+    for (int c = 0; c < nc; c++) {
+        Vf[c] = G.cons[c].Vprev; Vt[c] = G.cons[c].V;
+        if (fabs(Vt[c] - Vf[c]) > span) span = fabs(Vt[c] - Vf[c]);
+    }
+    int maxg = G.max_gummel > 0 ? G.max_gummel : 100;
+    double gtol = gummel_tol();
+    eng_upd_mob();
+    save_state(G.d_h1);
+    double rx = 0.;
+    if (PRED_ON && G.e_n >= 1 && span > 0.) {
+        double num = 0., den = 0., err = 0.;
+        int same = 1;
+        for (int c = 0; c < nc; c++) {
+            double a = Vt[c] - Vf[c], b = Vf[c] - G.Ve2[c];
+            num += a * b; den += b * b;
+            if (fabs(Vf[c] - G.Ve1[c]) > 1e-12) same = 0;
+        }
+        if (same && den > 0.) {
+            double r = num / den;
+            for (int c = 0; c < nc; c++) { double a = Vt[c] - Vf[c], b = Vf[c] - G.Ve2[c]; err = fmax(err, fabs(a - r * b)); }
+            if (r >= -2. && r <= 3. && fabs(r) > 1e-12 && err <= 1e-9 + 1e-6 * span) rx = r;
+        }
+    }
+    int first = 1;
+    double t = 0., tp = -1., dt = span > 0. ? fmin(1., 0.1 / span) : 1.;
+    int ok_all = 0, its = 0, fails = 0, cfail = 0, steps = 0;
+    double res = 0.;
+    int max_cfail = (G.models & 256) ? 3 : (1 << 30);
+    g_prog[0] = 0.; g_prog[3] = 0.;
+    for (int guard = 0; guard < 5000; guard++) {
+        if (g_stop) break;
+        double tn = fmin(1., t + dt);
+        for (int c = 0; c < nc; c++) Va[c] = Vf[c] + tn * (Vt[c] - Vf[c]);
+        eng_set_vrange(Va);
+        if (tp >= 0.) {
+            double r = (tn - t) / (t - tp);
+            ph_extrap(G.st, N, G.d_psi, G.d_phin, G.d_phip, G.d_h1[0], G.d_h1[1], G.d_h1[2], G.d_h2[0], G.d_h2[1],
+                      G.d_h2[2], r);
+        } else if (rx != 0.) {
+            double r = tn * rx;
+            ph_extrap(G.st, N, G.d_psi, G.d_phin, G.d_phip, G.d_h1[0], G.d_h1[1], G.d_h1[2], G.d_e2[0], G.d_e2[1],
+                      G.d_e2[2], r);
+        } else {
+            load_state(G.d_h1);
+            if (span > 0.) {
+                for (int c = 0; c < nc; c++) dV[c] = (tn - t) * (Vt[c] - Vf[c]);
+                region_shift(dV);
+            }
+        }
+        if (first) {
+            for (int a = 0; a < 3; a++) sd_copy(G.st, G.d_e2[a], G.d_h1[a], N);
+            first = 0;
+        }
+        eng_clamp_qf();
+        apply_bc(Va);
+        eng_rebuild_np();
+        int ok = gummel(Va, maxg, gtol, &its, &res);
+        G.conv_iter = its;
+        G.conv_res = res;
+        if (G.verbose)
+            fprintf(stderr, "semisim-hip: step t=%.4f (V0=%.4f) %s its=%d res=%.2e\n", tn, nc ? Va[0] : 0., ok ? "ok" : "FAIL",
+                    its, res);
+        if (ok) {
+            for (int a = 0; a < 3; a++) std::swap(G.d_h1[a], G.d_h2[a]);
+            save_state(G.d_h1);
+            tp = t; t = tn; steps++;
+            g_prog[0] = t; g_prog[3] = steps;
+            for (int c = 0; c < nc; c++) G.cons[c].Vprev = Va[c];
+            if (t >= 1.) { ok_all = 1; break; }
+            dt *= its <= 8 ? 2.0 : (its <= 25 ? 1.5 : GROW_SLOW);
+            cfail = 0;
+        } else {
+            fails++; cfail++; rx = 0.;
+            if (span == 0. || dt * span < 2e-5 || cfail >= max_cfail) break;
+            dt *= 0.3;
+        }
+    }
+    if (ok_all && span > 0.) {
+        for (int c = 0; c < nc; c++) { G.Ve2[c] = Vf[c]; G.Ve1[c] = Vt[c]; }
+        G.e_n = G.e_n < 2 ? G.e_n + 1 : 2;
+    } else if (!ok_all) G.e_n = 0;
+    if (!ok_all && (span > 0. || g_stop)) {
+        load_state(G.d_h1);
+        for (int c = 0; c < nc; c++) Va[c] = G.cons[c].Vprev;
+        eng_set_vrange(Va);
+        apply_bc(Va);
+        eng_rebuild_np();
+        eng_upd_mob();
+        if (G.verbose) fprintf(stderr, "semisim-hip: continuation stopped at t=%.4f\n", t);
+    }
+    (void)fails;
+    return ok_all;
+}
+
+int eng_run_continuation() {
+    const int nc = G.n_cons;
+    int gch = 0, cch = 0, ok_all;
+    for (int c = 0; c < nc; c++) {
+        if (fabs(G.cons[c].V - G.cons[c].Vprev) <= 1e-12) continue;
+        if (G.cons[c].bc == 2) gch = 1; else cch = 1;
+    }
+    if (gch && cch) {
+        double Vt[SD_NC_MAX];
+        for (int c = 0; c < nc; c++) { Vt[c] = G.cons[c].V; if (G.cons[c].bc != 2) G.cons[c].V = G.cons[c].Vprev; }
+        g_prog[4] = 2.;
+        ok_all = cont_path();
+        for (int c = 0; c < nc; c++) G.cons[c].V = Vt[c];
+        g_prog[4] = 3.;
+        if (ok_all) ok_all = cont_path();
+    } else {
+        g_prog[4] = gch ? 2. : 3.;
+        ok_all = cont_path();
+    }
+    double I[SD_NC_MAX], Sv[SD_NC_MAX];
+    terminal_currents(I, Sv);
+    double ks = 0., F = 0., fl[SD_NC_MAX];
+    int ncar = 0;
+    for (int c = 0; c < nc; c++) { ks += I[c]; fl[c] = 3e-14 * Sv[c]; F += fl[c]; if (G.cons[c].bc != 2) ncar++; }
+    double scl = F > 0. ? fmax(1., 2. * fabs(ks) / F) : 1.;
+    for (int c = 0; c < nc; c++) {
+        double Iv = I[c], fv = fl[c], fo = F - fl[c];
+        if (G.cons[c].bc != 2 && ncar > 1 && fo < fv) { Iv = -(ks - I[c]); fv = fo; }
+        G.cons[c].I = Iv;
+        G.cons[c].Iflr = fv * scl + (ok_all ? G.ii_err : 0.);
+        G.cons[c].Iraw = I[c];
+    }
+    return ok_all;
+}
+
+void eng_display() {
+    DevView d = eng_view();
+    ph_upd_J(G.st, d, G.d_Jx, G.d_Jy, G.d_Jz, G.d_Rt);
+    ph_mob_display(G.st, d, G.d_mudn, G.d_mudp);
+    eng_sync();
+}
+
+// environment knobs (read by api3_init, semiconductor_3d.c:4578-4596)
+void eng_read_env() {
+    const char *e;
+    G.pcg_rtol = (e = getenv("SEMISIM_PCGRTOL")) ? atof(e) : 1e-3;
+    G.ntol_fac = (e = getenv("SEMISIM_NTOLF")) ? atof(e) : 0.1;
+    G.bicg_rtol = (e = getenv("SEMISIM_BRTOL")) ? atof(e) : 1e-4;
+    G.verbose = (e = getenv("SEMISIM_VERBOSE")) ? atoi(e) : 0;
+    if ((e = getenv("SEMISIM_AAM"))) { int q = atoi(e); AA_DEPTH = q < 0 ? 0 : (q > 12 ? 12 : q); }
+    if ((e = getenv("SEMISIM_PRED"))) PRED_ON = atoi(e) != 0;
+    if ((e = getenv("SEMISIM_GROW"))) { double g = atof(e); if (g >= 1. && g <= 4.) GROW_SLOW = g; }
+    if ((e = getenv("SEMISIM_CATOL"))) CONT_ATOL = atof(e);
+    if ((e = getenv("SEMISIM_MG_FP32"))) G.precision = atoi(e) ? 1 : 0;
+}
+int eng_aa_depth() { return G.aa_m < AA_DEPTH ? G.aa_m : AA_DEPTH; }
+void eng_set_predictor(int on) { PRED_ON = on != 0; }
+int eng_set_precision(int p) {
+    if (p == 1 && G.allocated && !ensure_h32()) return 0;
+    G.precision = p ? 1 : 0;
+    return 1;
+}
+int eng_prepare_precision() {
+    if (G.precision == 1) return ensure_h32();
+    return 1;
+}
+
+// ------------------------------------------------ kernel-level test hooks
+// (declared in include/semisim_hip_ext.h; operate on the prepared device)
+#include "../../include/semisim_hip_ext.h"
+template <class T> static void level_op_T(Hier<T> &H, int L, double *A7) {
+    MGLevel<T> &v = H.lev[L];
+    size_t n = v.N;
+    std::vector<T> tmp(n);
+    std::vector<double> rs(n, 1.0);
+    if (L > 0) eng_download(rs.data(), v.rs, n);
+    T *a[7] = {v.A.c, v.A.xm, v.A.xp, v.A.ym, v.A.yp, v.A.zm, v.A.zp};
+    for (int q = 0; q < 7; q++) {
+        if (!a[q]) { for (size_t k = 0; k < n; k++) A7[q * n + k] = rs[k]; continue; }
+        HC(hipMemcpyAsync(tmp.data(), a[q], n * sizeof(T), hipMemcpyDeviceToHost, G.st));
+        eng_sync();
+        for (size_t k = 0; k < n; k++) A7[q * n + k] = (double)tmp[k] * rs[k];
+    }
+}
+extern "C" {
+int sdx_test_prepare(void) { return eng_prepare() && eng_prepare_precision(); }
+void sdx_test_set_state(const double *psi, const double *phin, const double *phip) {
+    eng_upload(G.d_psi, psi, G.N); eng_upload(G.d_phin, phin, G.N); eng_upload(G.d_phip, phip, G.N);
+    eng_rebuild_np();
+    eng_sync();
+}
+void sdx_test_get_node(int which, double *out) {
+    const double *src[] = {G.d_psi, G.d_phin, G.d_phip, G.d_n, G.d_p, G.d_mun, G.d_mup};
+    if (which >= 0 && which < 7) eng_download(out, src[which], G.N);
+}
+static void get_A7(double *A7) {
+    double *a[7] = {G.A.c, G.A.xm, G.A.xp, G.A.ym, G.A.yp, G.A.zm, G.A.zp};
+    for (int q = 0; q < 7; q++) eng_download(A7 + (size_t)q * G.N, a[q], G.N);
+}
+double sdx_test_poisson(int build, const double *Va, double *F, double *A7, double *sc) {
+    eng_upload_cons(Va);
+    double r = poisson_eval(G.d_psi, build, G.d_rhs);
+    eng_download(F, G.d_rhs, G.N);
+    if (build) { get_A7(A7); eng_download(sc, G.d_sc, G.N); }
+    return r;
+}
+void sdx_test_upd_mob(double *fmu6) {
+    eng_upd_mob();
+    eng_download(fmu6, G.d_fmu, 6 * (size_t)G.N);
+}
+void sdx_test_set_fmu(const double *fmu6) { eng_upload(G.d_fmu, fmu6, 6 * (size_t)G.N); }
+void sdx_test_cont(int hole, double *A7, double *rhs, double *sc, double *dsc) {
+    ph_cont(G.st, eng_view(), hole, G.A, G.d_rhs, G.d_sc, G.d_dsc);
+    get_A7(A7);
+    eng_download(rhs, G.d_rhs, G.N); eng_download(sc, G.d_sc, G.N); eng_download(dsc, G.d_dsc, G.N);
+}
+void sdx_test_load_op(const double *A7, int qsign, const double *w) {
+    double *a[7] = {G.A.c, G.A.xm, G.A.xp, G.A.ym, G.A.yp, G.A.zm, G.A.zp};
+    for (int q = 0; q < 7; q++) eng_upload(a[q], A7 + (size_t)q * G.N, G.N);
+    if (w) { eng_upload(G.d_dsc, w, G.N); G.mg_w = G.d_dsc; } else G.mg_w = nullptr;
+    G.mg_qsign = qsign;
+}
+void sdx_test_matvec(const double *x, double *y) {
+    eng_upload(G.d_wk[0], x, G.N);
+    matvec(G.d_wk[0], G.d_wk[1]);
+    eng_download(y, G.d_wk[1], G.N);
+}
+int sdx_test_mg_setup(void) {
+    mg_setup();
+    eng_sync();
+    return (int)(G.precision == 1 ? G.H32.lev.size() : G.H64.lev.size());
+}
+void sdx_test_level_dims(int L, int *dims) {
+    if (G.precision == 1) { auto &v = G.H32.lev[L]; int d[6] = {v.Nx, v.Ny, v.Nz, v.cx, v.cy, v.cz}; memcpy(dims, d, sizeof d); }
+    else { auto &v = G.H64.lev[L]; int d[6] = {v.Nx, v.Ny, v.Nz, v.cx, v.cy, v.cz}; memcpy(dims, d, sizeof d); }
+}
+void sdx_test_level_op(int L, double *A7) {
+    if (G.precision == 1) level_op_T(G.H32, L, A7); else level_op_T(G.H64, L, A7);
+}
+void sdx_test_precond(const double *r, double *z) {
+    eng_upload(G.d_wk[2], r, G.N);
+    precond(G.d_wk[2], G.d_wk[3]);
+    eng_download(z, G.d_wk[3], G.N);
+}
+int sdx_test_pcg(const double *b, double *x, int maxit, double rtol) {
+    eng_upload(G.d_rhs, b, G.N); eng_upload(G.d_sol, x, G.N);
+    int it = pcg(G.d_rhs, G.d_sol, maxit, rtol);
+    eng_download(x, G.d_sol, G.N);
+    return it;
+}
+double sdx_test_bicgstab(const double *b, double *x, int maxit, double rtol, double atol, int *its) {
+    eng_upload(G.d_rhs, b, G.N); eng_upload(G.d_sol, x, G.N);
+    double rn = bicgstab(G.d_rhs, G.d_sol, maxit, rtol, atol, its);
+    eng_download(x, G.d_sol, G.N);
+    return rn;
+}
+void sdx_test_currents(const double *Va, double *I, double *Sv, double *In) {
+    eng_upload_cons(Va);
+    terminal_currents(I, Sv);
+    for (int c = 0; c < G.n_cons; c++) In[c] = G.cons[c].In;
+}
+double sdx_test_cont_update(int hole, const double *u, const double *sc, double *dq, double lo, double hi) {
+    eng_upload(G.d_sol, u, G.N); eng_upload(G.d_sc, sc, G.N);
+    ph_cont_update(G.st, eng_view(), hole, G.d_sol, G.d_sc, G.d_wk[0], lo, hi, G.rb, S_MAX1);
+    rd(S_MAX1, S_MAX1 + 1);
+    eng_download(dq, G.d_wk[0], G.N);
+    return S(S_MAX1);
+}
+}

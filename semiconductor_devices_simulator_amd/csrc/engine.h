@@ -1,0 +1,122 @@
+// engine.h - host-side state of the MI355X drift-diffusion core.
+//
+// One device per process, held in a singleton (the reference ABI keeps a
+// global `gdev`, semiconductor_3d.c:366).  Setup data live on the host and are
+// uploaded by prepare(); the solver state (psi, phi_n, phi_p, carriers,
+// operators, multigrid hierarchy, Krylov and Anderson work vectors) lives in
+// HBM for the whole solve and only scalars (norms, dots, terminal currents)
+// come back to the host to steer the iteration.
+#pragma once
+#include "sd_physics.h"
+#include "sd_host.h"
+#include <vector>
+
+struct HostCon {
+    int face, i0, i1, j0, j1, k0, k1, bc;
+    double V, phi_m, tox, qox;
+    double Vprev, I, A, In, Iflr, Iraw;
+    int te, ntp;
+};
+
+template <class T> struct Hier {
+    std::vector<MGLevel<T>> lev;
+    double *inv = nullptr;         // coarsest-level inverse (row-major)
+    bool alloc = false;
+    bool pw_one = false;
+};
+
+struct Engine {
+    // ------------------------------------------------------ setup (host)
+    int Nx = 0, Ny = 0, Nz = 0, N = 0;
+    std::vector<double> xs, ys, zs, Lx, Ly, Lz;
+    std::vector<int> mat_idx;
+    std::vector<double> Nd, Na, Nf;
+    HostMat mats[SD_NM_MAX];
+    int n_mats = 0;
+    HostCon cons[SD_NC_MAX];
+    int n_cons = 0;
+    int models = 7, caxis = 1;
+    int max_iter_p = 2000, max_iter_c = 1000, max_gummel = 500;
+    double tol_p = 1e-9, tol_c = 1e-6, omega_p = 1., omega_c = 1.;
+    double pcg_rtol = 1e-3, ntol_fac = 0.1, bicg_rtol = 1e-4, cont_atol_f = 1.;
+    int converged = 0, conv_iter = 0;
+    double conv_res = 0.;
+    int allocated = 0, have_solution = 0, dirty = 1, verbose = 0, nthreads = 1;
+    double vlo = 0., vhi = 0.;
+    long long pcg_its = 0, bicg_its = 0, newton_its = 0, gummel_its = 0;
+    double t_pois = 0, t_cont = 0, t_mob = 0, t_cur = 0, t_aa = 0, t_mg = 0;
+    double ii_err = 0.;
+    int precision = 0;                 // 0 FP64 multigrid, 1 FP32 multigrid inside FP64 Krylov
+    int unsupported = 0;               // a requested feature outside this core's scope
+
+    // derived by prepare()
+    std::vector<unsigned char> kind, mat8;
+    std::vector<int> con_mask, gate_mask;
+    std::vector<double> robA, eps, bgn, lncK, lnvK, niK, mu0n, mu0p;
+    double Cref = 0.;
+    int ref_mat = 0, n_rob = 0;
+
+    // sweep history (continuation predictor)
+    double Ve1[SD_NC_MAX], Ve2[SD_NC_MAX];
+    int e_n = 0;
+
+    // ------------------------------------------------------ device state
+    hipStream_t st = nullptr;
+    double *d_geo = nullptr;           // xs|ys|zs|Lx|Ly|Lz
+    unsigned char *d_kind = nullptr, *d_mat = nullptr, *d_flag = nullptr;
+    int *d_con_mask = nullptr, *d_gate_mask = nullptr, *d_changed = nullptr;
+    double *d_Nd, *d_Na, *d_Nf, *d_eps, *d_bgn, *d_robA, *d_lncK, *d_lnvK, *d_niK;
+    double *d_psi, *d_phin, *d_phip, *d_n, *d_p, *d_mun, *d_mup, *d_fmu, *d_gii;
+    Sten7<double> A;
+    double *d_rhs, *d_sc, *d_dsc, *d_sol, *d_wk[7], *d_xsave, *d_tr, *d_Ft;
+    double *d_xk, *d_dqn, *d_dqp, *d_qn0, *d_qp0;
+    double *d_h1[3], *d_h2[3], *d_e2[3];
+    float *d_aadF[12], *d_aadG[12];
+    double *d_aaf, *d_aag;
+    double *d_Jx, *d_Jy, *d_Jz, *d_Rt, *d_mudn, *d_mudp;
+    MatDev *d_mats = nullptr;
+    ConDev *d_cons = nullptr;
+    int *d_ek = nullptr, *d_ekb = nullptr, *d_eoff = nullptr;
+    double *d_eh = nullptr, *d_ear = nullptr;
+    int n_edges = 0;
+    RedBuf rb{};
+    double *h_scal = nullptr;          // pinned mirror of the scalar slots
+    std::vector<void *> dev_allocs;
+
+    Hier<double> H64;
+    Hier<float> H32;
+    float *d_b32 = nullptr, *d_x32 = nullptr;   // level-0 FP32 vectors
+    const double *mg_w = nullptr;
+    int mg_qsign = 0;
+    int mg_off = 0;
+
+    // Anderson
+    int aa_m = 12, aa_n = 0, aa_pos = 0, aa_have = 0;
+    double aa_gram[12][12];
+
+    // statistics of the kernels (see sdx_kernel_stats)
+    long long n_vcycles = 0, n_spmv = 0;
+};
+
+extern Engine G;
+extern volatile int g_stop;
+extern volatile double g_prog[8];
+
+// engine API used by api.cpp
+int eng_alloc(int Nx, int Ny, int Nz);
+void eng_free();
+int eng_prepare();
+int eng_equilibrium();
+int eng_run_continuation();
+void eng_display();
+DevView eng_view();
+void eng_sync();
+void eng_download(double *dst, const double *src, size_t n);
+void eng_upload(double *dst, const double *src, size_t n);
+void eng_upload_cons(const double *Va);
+void eng_set_vrange(const double *Va);
+void eng_clamp_qf();
+void eng_rebuild_np();
+void eng_upd_mob();
+void eng_apply_bc();
+int eng_ok();

@@ -1,0 +1,507 @@
+// k_linalg.hip - Krylov and multigrid kernels of the MI355X drift-diffusion
+// core (gfx950).  Every kernel is HBM-bandwidth bound: 7-point stencils as
+// SoA diagonals, one node per lane, coalesced along x; reductions are
+// deterministic (wavefront shuffle, LDS, fixed-order partial sums).
+//
+// Reference counterparts (semiconductor_3d.c): matvec_L :1975-1993,
+// galerkin :2003-2027, restrict_sum/prolong_add :2028-2048, dense LU
+// :2050-2081, mg_weights :2085-2117, vcycle :2151-2165, dot :2174-2179,
+// pcg :2182-2210, bicgstab_core :2234-2320.  The chunked D-ILU smoother of the
+// reference is replaced by red-black Gauss-Seidel (symmetric ordering in the
+// V-cycle), which changes iteration counts but not the solution.
+#include "sd_reduce.h"
+
+#define PH_NONE (-1e300)
+
+// ------------------------------------------------------------ reductions
+
+__global__ void __launch_bounds__(SD_BLOCK) k_dot(const double *a, const double *b, int n, RedBuf rb, int slot) {
+    double acc[1] = {0.0};
+    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < n; k += gridDim.x * SD_BLOCK) acc[0] += b ? a[k] * b[k] : a[k];
+    red_finish<1, false>(acc, rb, slot);
+}
+__global__ void __launch_bounds__(SD_BLOCK) k_dot2(const double *a, const double *b, const double *c, int n,
+                                                   RedBuf rb, int slot) {
+    double acc[2] = {0.0, 0.0};
+    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < n; k += gridDim.x * SD_BLOCK) {
+        double x = a[k];
+        acc[0] += x * b[k];
+        acc[1] += x * c[k];
+    }
+    red_finish<2, false>(acc, rb, slot);
+}
+__global__ void __launch_bounds__(SD_BLOCK) k_maxabs(const double *a, const double *b, int n, RedBuf rb, int slot) {
+    double acc[1] = {0.0};
+    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < n; k += gridDim.x * SD_BLOCK) {
+        double x = b ? fabs(a[k] - b[k]) : fabs(a[k]);
+        acc[0] = sd_nanmax(acc[0], x);
+    }
+    red_finish<1, true>(acc, rb, slot);
+}
+
+void sd_dot(hipStream_t s, const double *a, const double *b, int n, RedBuf rb, int slot) {
+    k_dot<<<red_grid(n), SD_BLOCK, 0, s>>>(a, b, n, rb, slot);
+}
+void sd_dot2(hipStream_t s, const double *a, const double *b, const double *c, int n, RedBuf rb, int sab, int sac) {
+    // slots must be adjacent: sac == sab + 1
+    (void)sac;
+    k_dot2<<<red_grid(n), SD_BLOCK, 0, s>>>(a, b, c, n, rb, sab);
+}
+void sd_nrm2sq(hipStream_t s, const double *a, int n, RedBuf rb, int slot) { sd_dot(s, a, a, n, rb, slot); }
+void sd_maxabs(hipStream_t s, const double *a, int n, RedBuf rb, int slot) {
+    k_maxabs<<<red_grid(n), SD_BLOCK, 0, s>>>(a, nullptr, n, rb, slot);
+}
+void sd_maxabsdiff(hipStream_t s, const double *a, const double *b, int n, RedBuf rb, int slot) {
+    k_maxabs<<<red_grid(n), SD_BLOCK, 0, s>>>(a, b, n, rb, slot);
+}
+
+// -------------------------------------------------------- vector kernels
+__global__ void k_fill(double *a, double v, int n) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k < n) a[k] = v;
+}
+__global__ void k_copy(double *d, const double *s, int n) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k < n) d[k] = s[k];
+}
+__global__ void k_axpy(double *y, const double *x, double a, int n) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k < n) y[k] += a * x[k];
+}
+void sd_fill(hipStream_t s, double *a, double v, int n) { k_fill<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(a, v, n); }
+void sd_copy(hipStream_t s, double *d, const double *src, int n) {
+    hipMemcpyAsync(d, src, (size_t)n * 8, hipMemcpyDeviceToDevice, s);
+}
+void sd_axpy(hipStream_t s, double *y, const double *x, double a, int n) {
+    k_axpy<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(y, x, a, n);
+}
+
+// -------------------------------------------------------------- stencil
+template <class T, class V>
+__device__ __forceinline__ V sten_apply(const Sten7<T> &A, const V *x, int k, int i, int j, int l,
+                                        int Nx, int Ny, int Nz, bool unitdiag) {
+    int NxNy = Nx * Ny;
+    V s = unitdiag ? (V)x[k] : (V)A.c[k] * x[k];
+    if (i > 0) s += (V)A.xm[k] * x[k - 1];
+    if (i < Nx - 1) s += (V)A.xp[k] * x[k + 1];
+    if (j > 0) s += (V)A.ym[k] * x[k - Nx];
+    if (j < Ny - 1) s += (V)A.yp[k] * x[k + Nx];
+    if (l > 0) s += (V)A.zm[k] * x[k - NxNy];
+    if (l < Nz - 1) s += (V)A.zp[k] * x[k + NxNy];
+    return s;
+}
+
+template <class T>
+__global__ void __launch_bounds__(SD_BLOCK) k_spmv(int Nx, int Ny, int Nz, Sten7<T> A, const T *x, const T *b, T *y) {
+    int N = Nx * Ny * Nz;
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= N) return;
+    int NxNy = Nx * Ny, l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
+    T s = sten_apply<T, T>(A, x, k, i, j, l, Nx, Ny, Nz, false);
+    y[k] = b ? b[k] - s : s;
+}
+void sd_spmv(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, const double *b, double *y) {
+    k_spmv<double><<<sd_nblocks(g.N), SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, b, y);
+}
+template <class T> void sd_spmv_T(hipStream_t s, const Geo &g, Sten7<T> A, const T *x, const T *b, T *y) {
+    k_spmv<T><<<sd_nblocks(g.N), SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, b, y);
+}
+template void sd_spmv_T<double>(hipStream_t, const Geo &, Sten7<double>, const double *, const double *, double *);
+template void sd_spmv_T<float>(hipStream_t, const Geo &, Sten7<float>, const float *, const float *, float *);
+
+// rounding floor of a continuity solve, semiconductor_3d.c:2243-2265
+__global__ void __launch_bounds__(SD_BLOCK) k_floor(int Nx, int Ny, int Nz, Sten7<double> A, const double *x,
+                                                    const double *b, RedBuf rb, int slot) {
+    int N = Nx * Ny * Nz, NxNy = Nx * Ny;
+    double acc[1] = {0.0};
+    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < N; k += gridDim.x * SD_BLOCK) {
+        int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
+        double s = fabs(A.c[k] * x[k]) + fabs(b[k]);
+        if (i > 0) s += fabs(A.xm[k] * x[k - 1]);
+        if (i < Nx - 1) s += fabs(A.xp[k] * x[k + 1]);
+        if (j > 0) s += fabs(A.ym[k] * x[k - Nx]);
+        if (j < Ny - 1) s += fabs(A.yp[k] * x[k + Nx]);
+        if (l > 0) s += fabs(A.zm[k] * x[k - NxNy]);
+        if (l < Nz - 1) s += fabs(A.zp[k] * x[k + NxNy]);
+        acc[0] += s * s;
+    }
+    red_finish<1, false>(acc, rb, slot);
+}
+void sd_floor(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, const double *b, RedBuf rb, int slot) {
+    k_floor<<<red_grid(g.N), SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, b, rb, slot);
+}
+
+// ------------------------------------------------------- Krylov updates
+__global__ void k_scalar_op(int op, double *sc) {
+    if (threadIdx.x != 0) return;
+    switch (op) {
+    case SOP_PCG_AL: sc[S_AL] = (sc[S_PQ] > 0.) ? sc[S_RZ] / sc[S_PQ] : 0.0; break;
+    case SOP_PCG_BE: sc[S_BE] = sc[S_TMP0] / sc[S_RZ]; sc[S_RZ] = sc[S_TMP0]; break;
+    case SOP_BI_BE: sc[S_BE] = (sc[S_RHO1] / sc[S_RHO]) * (sc[S_AL] / sc[S_OM]); break;
+    case SOP_BI_AL: sc[S_AL] = (fabs(sc[S_RV]) < 1e-300) ? 0.0 : sc[S_RHO1] / sc[S_RV]; break;
+    case SOP_BI_OM: sc[S_OM] = sc[S_TT] > 0. ? sc[S_TS] / sc[S_TT] : 0.; break;
+    case SOP_BI_NEXT: sc[S_RHO] = sc[S_RHO1]; break;
+    case SOP_BI_RESTART: sc[S_RHO] = sc[S_AL] = sc[S_OM] = 1.0; break;
+    }
+}
+void sd_scalar_op(hipStream_t s, int op, RedBuf rb) { k_scalar_op<<<1, 64, 0, s>>>(op, rb.scal); }
+
+__global__ void k_pcg_xr(double *x, double *r, const double *p, const double *q, int n, const double *sc) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= n || !(sc[S_PQ] > 0.)) return;
+    double al = sc[S_AL];
+    x[k] += al * p[k];
+    r[k] -= al * q[k];
+}
+__global__ void k_pcg_p(double *p, const double *z, int n, const double *sc) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= n) return;
+    p[k] = z[k] + sc[S_BE] * p[k];
+}
+void sd_pcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q, int n, RedBuf rb) {
+    k_pcg_xr<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(x, r, p, q, n, rb.scal);
+}
+void sd_pcg_update_p(hipStream_t s, double *p, const double *z, int n, RedBuf rb) {
+    k_pcg_p<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(p, z, n, rb.scal);
+}
+
+__global__ void k_bi_p(double *p, const double *r, const double *v, int n, const double *sc) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= n) return;
+    double be = sc[S_BE], om = sc[S_OM];
+    p[k] = r[k] + be * (p[k] - om * v[k]);
+}
+__global__ void __launch_bounds__(SD_BLOCK) k_bi_r1(double *r, const double *v, int n, RedBuf rb) {
+    double al = rb.scal[S_AL];
+    double acc[1] = {0.0};
+    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < n; k += gridDim.x * SD_BLOCK) {
+        double x = r[k] - al * v[k];
+        r[k] = x;
+        acc[0] += x * x;
+    }
+    red_finish<1, false>(acc, rb, S_SS);
+}
+__global__ void k_bi_x1(double *x, const double *ph, int n, const double *sc) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k < n) x[k] += sc[S_AL] * ph[k];
+}
+__global__ void __launch_bounds__(SD_BLOCK) k_bi_xr2(double *x, double *r, const double *ph, const double *sh,
+                                                     const double *t, int n, RedBuf rb) {
+    double al = rb.scal[S_AL], om = rb.scal[S_OM];
+    double acc[1] = {0.0};
+    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < n; k += gridDim.x * SD_BLOCK) {
+        x[k] += al * ph[k] + om * sh[k];
+        double y = r[k] - om * t[k];
+        r[k] = y;
+        acc[0] += y * y;
+    }
+    red_finish<1, false>(acc, rb, S_RR);
+}
+__global__ void __launch_bounds__(SD_BLOCK) k_bi_init(double *r, double *rh, double *p, double *v, const double *b,
+                                                      const double *t, int n, RedBuf rb) {
+    double acc[1] = {0.0};
+    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < n; k += gridDim.x * SD_BLOCK) {
+        double y = b[k] - t[k];
+        r[k] = y; rh[k] = y; p[k] = 0.; v[k] = 0.;
+        acc[0] += y * y;
+    }
+    red_finish<1, false>(acc, rb, S_RR);
+}
+__global__ void k_bi_restart(double *rh, double *p, double *v, const double *r, int n) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k < n) { rh[k] = r[k]; p[k] = 0.; v[k] = 0.; }
+}
+void sd_bi_p(hipStream_t s, double *p, const double *r, const double *v, int n, RedBuf rb) {
+    k_bi_p<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(p, r, v, n, rb.scal);
+}
+void sd_bi_r1(hipStream_t s, double *r, const double *v, int n, RedBuf rb) {
+    k_bi_r1<<<red_grid(n), SD_BLOCK, 0, s>>>(r, v, n, rb);
+}
+void sd_bi_x1(hipStream_t s, double *x, const double *ph, int n, RedBuf rb) {
+    k_bi_x1<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(x, ph, n, rb.scal);
+}
+void sd_bi_xr2(hipStream_t s, double *x, double *r, const double *ph, const double *sh, const double *t, int n,
+               RedBuf rb) {
+    k_bi_xr2<<<red_grid(n), SD_BLOCK, 0, s>>>(x, r, ph, sh, t, n, rb);
+}
+void sd_bi_init(hipStream_t s, double *r, double *rh, double *p, double *v, const double *b, const double *t, int n,
+                RedBuf rb) {
+    k_bi_init<<<red_grid(n), SD_BLOCK, 0, s>>>(r, rh, p, v, b, t, n, rb);
+}
+void sd_bi_restart(hipStream_t s, double *rh, double *p, double *v, const double *r, int n) {
+    k_bi_restart<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(rh, p, v, r, n);
+}
+
+// ------------------------------------------------------------ multigrid
+// aggregate of coarse node (I,J,L) on the fine level: 2x2x2 where the fine
+// level is coarsened along that axis (semiconductor_3d.c:1995-1998)
+struct Agg { int i0, i1, j0, j1, l0, l1; };
+__device__ __forceinline__ Agg agg_range(int fNx, int fNy, int fNz, int cx, int cy, int cz, int I, int J, int L) {
+    Agg a;
+    a.i0 = cx ? 2 * I : I; a.i1 = cx ? 2 * I + 1 : I; if (a.i1 > fNx - 1) a.i1 = fNx - 1;
+    a.j0 = cy ? 2 * J : J; a.j1 = cy ? 2 * J + 1 : J; if (a.j1 > fNy - 1) a.j1 = fNy - 1;
+    a.l0 = cz ? 2 * L : L; a.l1 = cz ? 2 * L + 1 : L; if (a.l1 > fNz - 1) a.l1 = fNz - 1;
+    return a;
+}
+
+// Slotboom weights of level L into level L+1 (semiconductor_3d.c:2085-2117)
+template <class T>
+__global__ void __launch_bounds__(SD_BLOCK) k_mg_weights(MGLevel<T> f, MGLevel<T> c, int L, DevView d, int qsign,
+                                                         double VT) {
+    int K = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (K >= c.N) return;
+    int cNxNy = c.Nx * c.Ny, LL = K / cNxNy, rr = K - LL * cNxNy, J = rr / c.Nx, I = rr - J * c.Nx;
+    Agg a = agg_range(f.Nx, f.Ny, f.Nz, f.cx, f.cy, f.cz, I, J, LL);
+    double pv[8];
+    int kk[8], m = 0;
+    double mx = PH_NONE;
+    for (int l = a.l0; l <= a.l1; l++)
+        for (int j = a.j0; j <= a.j1; j++)
+            for (int i = a.i0; i <= a.i1; i++) {
+                int k = i + f.Nx * (j + f.Ny * l);
+                double p;
+                if (L == 0) {
+                    if (d.kind[k] != K_SEMI || d.con_mask[k]) p = PH_NONE;
+                    else p = qsign > 0 ? d.phin[k] / VT : -d.phip[k] / VT;
+                } else p = f.ph[k];
+                if (p > mx) mx = p;
+                pv[m] = p; kk[m] = k; m++;
+            }
+    c.ph[K] = mx;
+    for (int q = 0; q < m; q++) {
+        double e = (pv[q] <= 0.5 * PH_NONE) ? -800. : pv[q] - mx;
+        f.pw[kk[q]] = (T)(e < -700. ? 0. : exp(e));
+    }
+}
+template <class T>
+void mg_weights_level(hipStream_t s, const MGLevel<T> &f, MGLevel<T> &c, int L, const DevView &d, int qsign,
+                      double VT) {
+    k_mg_weights<T><<<sd_nblocks(c.N), SD_BLOCK, 0, s>>>(f, c, L, d, qsign, VT);
+}
+
+template <class T> __global__ void k_fill_T(T *a, T v, int n) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k < n) a[k] = v;
+}
+template <class T> void mg_set_pw_one(hipStream_t s, MGLevel<T> &v) {
+    k_fill_T<T><<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(v.pw, (T)1, v.N);
+}
+
+// Galerkin collapse P^T W A P of the 7-point stencil (semiconductor_3d.c:
+// 2003-2027).  W = row scale of the fine level; the coarse row is stored
+// scaled to unit diagonal with its FP64 scale in c.rs.
+template <class T>
+__global__ void __launch_bounds__(SD_BLOCK) k_galerkin(MGLevel<T> f, MGLevel<T> c) {
+    int K = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (K >= c.N) return;
+    int cNxNy = c.Nx * c.Ny, LL = K / cNxNy, rr = K - LL * cNxNy, J = rr / c.Nx, I = rr - J * c.Nx;
+    Agg a = agg_range(f.Nx, f.Ny, f.Nz, f.cx, f.cy, f.cz, I, J, LL);
+    const int NxNy = f.Nx * f.Ny;
+    double cc = 0, xm = 0, xp = 0, ym = 0, yp = 0, zm = 0, zp = 0;
+    for (int l = a.l0; l <= a.l1; l++)
+        for (int j = a.j0; j <= a.j1; j++)
+            for (int i = a.i0; i <= a.i1; i++) {
+                int k = i + f.Nx * (j + f.Ny * l);
+                double wk = f.rs ? f.rs[k] : 1.0, t;
+                if (wk == 0.0) continue;
+                double ac = f.A.c ? (double)f.A.c[k] : 1.0;
+                cc += wk * ac * (double)f.pw[k];
+                if (i > 0) { t = wk * (double)f.A.xm[k] * (double)f.pw[k - 1]; if (i - 1 >= a.i0) cc += t; else xm += t; }
+                if (i < f.Nx - 1) { t = wk * (double)f.A.xp[k] * (double)f.pw[k + 1]; if (i + 1 <= a.i1) cc += t; else xp += t; }
+                if (j > 0) { t = wk * (double)f.A.ym[k] * (double)f.pw[k - f.Nx]; if (j - 1 >= a.j0) cc += t; else ym += t; }
+                if (j < f.Ny - 1) { t = wk * (double)f.A.yp[k] * (double)f.pw[k + f.Nx]; if (j + 1 <= a.j1) cc += t; else yp += t; }
+                if (l > 0) { t = wk * (double)f.A.zm[k] * (double)f.pw[k - NxNy]; if (l - 1 >= a.l0) cc += t; else zm += t; }
+                if (l < f.Nz - 1) { t = wk * (double)f.A.zp[k] * (double)f.pw[k + NxNy]; if (l + 1 <= a.l1) cc += t; else zp += t; }
+            }
+    // an aggregate of identity rows only carries no equation: identity row.
+    // A zero pivot is replaced by 1 as the reference smoother does (:1939-1940).
+    double sc = cc;
+    if (cc == 0. && xm == 0. && xp == 0. && ym == 0. && yp == 0. && zm == 0. && zp == 0.) sc = 1.0;
+    if (!(sc != 0.0) || !isfinite(sc)) sc = 1.0;
+    double is = 1.0 / sc;
+    c.rs[K] = sc;
+    c.A.xm[K] = (T)(xm * is); c.A.xp[K] = (T)(xp * is);
+    c.A.ym[K] = (T)(ym * is); c.A.yp[K] = (T)(yp * is);
+    c.A.zm[K] = (T)(zm * is); c.A.zp[K] = (T)(zp * is);
+}
+template <class T> void mg_galerkin(hipStream_t s, const MGLevel<T> &f, MGLevel<T> &c) {
+    k_galerkin<T><<<sd_nblocks(c.N), SD_BLOCK, 0, s>>>(f, c);
+}
+
+// Red-black Gauss-Seidel half sweep on the nodes of one colour
+// ((i+j+l) & 1 == color).  zero != 0: the sweep starts from x = 0 (every node
+// is written: the colour's nodes get b/d, the others 0).
+template <class T>
+__global__ void __launch_bounds__(SD_BLOCK) k_rbgs(MGLevel<T> v, const T *b, T *x, int color, int zero) {
+    const int Nx = v.Nx, Ny = v.Ny, Nz = v.Nz, NxNy = Nx * Ny;
+    if (zero) {
+        int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+        if (k >= v.N) return;
+        int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
+        if (((i + j + l) & 1) != color) { x[k] = (T)0; return; }
+        T d = v.A.c ? v.A.c[k] : (T)1;
+        if (!(d != (T)0)) d = (T)1;
+        x[k] = b[k] / d;
+        return;
+    }
+    int Hx = (Nx + 1) >> 1;
+    int t = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (t >= Hx * Ny * Nz) return;
+    int line = t / Hx, i2 = t - line * Hx, l = line / Ny, j = line - l * Ny;
+    int i = 2 * i2 + ((j + l + color) & 1);
+    if (i >= Nx) return;
+    int k = i + Nx * (j + Ny * l);
+    T s = b[k];
+    if (i > 0) s -= v.A.xm[k] * x[k - 1];
+    if (i < Nx - 1) s -= v.A.xp[k] * x[k + 1];
+    if (j > 0) s -= v.A.ym[k] * x[k - Nx];
+    if (j < Ny - 1) s -= v.A.yp[k] * x[k + Nx];
+    if (l > 0) s -= v.A.zm[k] * x[k - NxNy];
+    if (l < Nz - 1) s -= v.A.zp[k] * x[k + NxNy];
+    T d = v.A.c ? v.A.c[k] : (T)1;
+    if (!(d != (T)0)) d = (T)1;
+    x[k] = s / d;
+}
+template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int color, int zero) {
+    if (zero) k_rbgs<T><<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(v, b, x, color, 1);
+    else {
+        int n = ((v.Nx + 1) >> 1) * v.Ny * v.Nz;
+        k_rbgs<T><<<sd_nblocks(n), SD_BLOCK, 0, s>>>(v, b, x, color, 0);
+    }
+}
+
+// Residual of the fine level summed over each aggregate with the fine row
+// scale (restriction R = P^T W, semiconductor_3d.c:2028-2037), divided by the
+// coarse row scale (the coarse rows are stored with unit diagonal).
+template <class T>
+__global__ void __launch_bounds__(SD_BLOCK) k_resid_restrict(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x) {
+    int K = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (K >= c.N) return;
+    int cNxNy = c.Nx * c.Ny, LL = K / cNxNy, rr = K - LL * cNxNy, J = rr / c.Nx, I = rr - J * c.Nx;
+    Agg a = agg_range(f.Nx, f.Ny, f.Nz, f.cx, f.cy, f.cz, I, J, LL);
+    double acc = 0.0;
+    for (int l = a.l0; l <= a.l1; l++)
+        for (int j = a.j0; j <= a.j1; j++)
+            for (int i = a.i0; i <= a.i1; i++) {
+                int k = i + f.Nx * (j + f.Ny * l);
+                double w = f.rs ? f.rs[k] : 1.0;
+                if (w == 0.0) continue;
+                T r = b[k] - sten_apply<T, T>(f.A, x, k, i, j, l, f.Nx, f.Ny, f.Nz, f.A.c == nullptr);
+                acc += w * (double)r;
+            }
+    c.b[K] = (T)(acc / c.rs[K]);
+}
+template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x) {
+    k_resid_restrict<T><<<sd_nblocks(c.N), SD_BLOCK, 0, s>>>(f, c, b, x);
+}
+
+// x += pw * x_parent (semiconductor_3d.c:2038-2048, MG_ALPHA = 1)
+template <class T>
+__global__ void __launch_bounds__(SD_BLOCK) k_prolong(MGLevel<T> f, MGLevel<T> c, T *x) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= f.N) return;
+    int NxNy = f.Nx * f.Ny, l = k / NxNy, r = k - l * NxNy, j = r / f.Nx, i = r - j * f.Nx;
+    int I = f.cx ? i >> 1 : i, J = f.cy ? j >> 1 : j, L = f.cz ? l >> 1 : l;
+    x[k] += f.pw[k] * c.x[I + c.Nx * (J + c.Ny * L)];
+}
+template <class T> void mg_prolong(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c, T *x) {
+    k_prolong<T><<<sd_nblocks(f.N), SD_BLOCK, 0, s>>>(f, c, x);
+}
+
+// Coarsest level (<= 64 unknowns): dense LU with partial pivoting in LDS
+// (semiconductor_3d.c:2050-2081), then the explicit inverse, one column per
+// lane.  A zero pivot is replaced by 1e-300 as in the reference.
+template <class T>
+__global__ void __launch_bounds__(64) k_coarse_factor(MGLevel<T> v, double *inv) {
+    __shared__ double M[64][65];
+    __shared__ int piv[64];
+    const int n = v.N, t = threadIdx.x, NxNy = v.Nx * v.Ny;
+    for (int r = 0; r < n; r++) if (t < n) M[r][t] = 0.0;
+    __syncthreads();
+    if (t < n) {
+        int k = t, l = k / NxNy, rr = k - l * NxNy, j = rr / v.Nx, i = rr - j * v.Nx;
+        M[k][k] = v.A.c ? (double)v.A.c[k] : 1.0;
+        if (i > 0) M[k][k - 1] = v.A.xm[k];
+        if (i < v.Nx - 1) M[k][k + 1] = v.A.xp[k];
+        if (j > 0) M[k][k - v.Nx] = v.A.ym[k];
+        if (j < v.Ny - 1) M[k][k + v.Nx] = v.A.yp[k];
+        if (l > 0) M[k][k - NxNy] = v.A.zm[k];
+        if (l < v.Nz - 1) M[k][k + NxNy] = v.A.zp[k];
+    }
+    __syncthreads();
+    for (int c = 0; c < n; c++) {
+        if (t == 0) {
+            int p = c; double mx = fabs(M[c][c]);
+            for (int r = c + 1; r < n; r++) if (fabs(M[r][c]) > mx) { mx = fabs(M[r][c]); p = r; }
+            piv[c] = p;
+        }
+        __syncthreads();
+        int p = piv[c];
+        if (p != c && t < n) { double tt = M[c][t]; M[c][t] = M[p][t]; M[p][t] = tt; }
+        __syncthreads();
+        if (t == 0 && !(fabs(M[c][c]) > 1e-300)) M[c][c] = 1e-300;
+        __syncthreads();
+        double dg = M[c][c];
+        if (t > c && t < n) {
+            double fct = M[t][c] / dg;
+            M[t][c] = fct;
+            if (fct != 0.) for (int q = c + 1; q < n; q++) M[t][q] -= fct * M[c][q];
+        }
+        __syncthreads();
+    }
+    // column t of the inverse: solve LU y = P e_t
+    if (t < n) {
+        double y[64];
+        for (int r = 0; r < n; r++) y[r] = (r == t) ? 1.0 : 0.0;
+        for (int c = 0; c < n; c++) { int p = piv[c]; if (p != c) { double tt = y[c]; y[c] = y[p]; y[p] = tt; } }
+        for (int r = 0; r < n; r++) { double s = y[r]; for (int q = 0; q < r; q++) s -= M[r][q] * y[q]; y[r] = s; }
+        for (int r = n - 1; r >= 0; r--) { double s = y[r]; for (int q = r + 1; q < n; q++) s -= M[r][q] * y[q]; y[r] = s / M[r][r]; }
+        for (int r = 0; r < n; r++) inv[r * n + t] = y[r];
+    }
+}
+template <class T> void mg_coarse_factor(hipStream_t s, const MGLevel<T> &v, double *inv) {
+    k_coarse_factor<T><<<1, 64, 0, s>>>(v, inv);
+}
+template <class T>
+__global__ void __launch_bounds__(64) k_coarse_solve(int n, const double *inv, const T *b, T *x) {
+    int r = threadIdx.x;
+    if (r >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < n; c++) s += inv[r * n + c] * (double)b[c];
+    x[r] = (T)s;
+}
+template <class T> void mg_coarse_solve(hipStream_t s, const MGLevel<T> &v, const double *inv, const T *b, T *x) {
+    k_coarse_solve<T><<<1, 64, 0, s>>>(v.N, inv, b, x);
+}
+
+__global__ void k_to_float(const double *a, float *b, int n) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k < n) b[k] = (float)a[k];
+}
+__global__ void k_to_double(const float *a, double *b, int n) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k < n) b[k] = (double)a[k];
+}
+__global__ void k_sten_to_float(Sten7<double> A, Sten7<float> B, int n) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= n) return;
+    B.c[k] = (float)A.c[k]; B.xm[k] = (float)A.xm[k]; B.xp[k] = (float)A.xp[k];
+    B.ym[k] = (float)A.ym[k]; B.yp[k] = (float)A.yp[k]; B.zm[k] = (float)A.zm[k]; B.zp[k] = (float)A.zp[k];
+}
+void mg_to_float(hipStream_t s, const double *a, float *b, int n) { k_to_float<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(a, b, n); }
+void mg_to_double(hipStream_t s, const float *a, double *b, int n) { k_to_double<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(a, b, n); }
+void mg_sten_to_float(hipStream_t s, Sten7<double> A, Sten7<float> B, int n) {
+    k_sten_to_float<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(A, B, n);
+}
+
+#define INST(T)                                                                                                   \
+    template void mg_weights_level<T>(hipStream_t, const MGLevel<T> &, MGLevel<T> &, int, const DevView &, int, double); \
+    template void mg_galerkin<T>(hipStream_t, const MGLevel<T> &, MGLevel<T> &);                                  \
+    template void mg_rbgs<T>(hipStream_t, const MGLevel<T> &, const T *, T *, int, int);                         \
+    template void mg_resid_restrict<T>(hipStream_t, const MGLevel<T> &, const MGLevel<T> &, const T *, const T *); \
+    template void mg_prolong<T>(hipStream_t, const MGLevel<T> &, const MGLevel<T> &, T *);                       \
+    template void mg_coarse_factor<T>(hipStream_t, const MGLevel<T> &, double *);                                \
+    template void mg_coarse_solve<T>(hipStream_t, const MGLevel<T> &, const double *, const T *, T *);           \
+    template void mg_set_pw_one<T>(hipStream_t, MGLevel<T> &);
+INST(double)
+INST(float)

@@ -1,0 +1,647 @@
+// k_physics.hip - box-method drift-diffusion physics on the device (gfx950).
+//
+// One lane per mesh node, SoA FP64 node data, 7-point neighbourhoods read
+// straight from HBM/L2 (x neighbours share cache lines, y/z neighbours are
+// L2 hits for the planes in flight).  The physics follows the reference core
+// (semiconductor_3d.c): poisson_eval :2732-2789, rebuild_np :2826-2834,
+// face_flux :2843-2854, upd_mob_ref :2962-3006, cont_assemble :3067-3106,
+// cont_update :3112-3146, terminal_currents :3598-3623, upd_J :3627-3658,
+// apply_bc/neutral_psi/init_eq :1782-1845, aa_step :3709-3757, gummel's
+// neutral-region predictor :3853-3859, float_fix majority test :3168-3169.
+#include "sd_physics.h"
+#include "sd_reduce.h"
+
+#define IDX3(i, j, l) ((i) + g.Nx * ((j) + g.Ny * (l)))
+
+__device__ __forceinline__ double neutral_psi(const DevView &d, int k, double V, double VT) {
+    double D = d.Nd[k] - d.Na[k];
+    return V + VT * (asinh(D / (2.0 * d.niK[k])) - 0.5 * (d.lncK[k] - d.lnvK[k]));
+}
+
+// ---------------------------------------------------------------- Poisson
+__global__ void __launch_bounds__(SD_BLOCK) k_poisson(DevView d, const double *phi, int build, double *F,
+                                                      Sten7<double> A, double *sc, RedBuf rb, int slot) {
+    const Geo g = d.g;
+    double acc[1] = {0.0};
+    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < g.N; k += gridDim.x * SD_BLOCK) {
+        if (d.con_mask[k]) {
+            F[k] = 0.0;
+            if (build) {
+                A.c[k] = 1.0; A.xm[k] = A.xp[k] = A.ym[k] = A.yp[k] = A.zm[k] = A.zp[k] = 0.0;
+                sc[k] = 1.0;
+            }
+            continue;
+        }
+        int i, j, l;
+        sd_unidx(g, k, i, j, l);
+        double ek = d.eps[k], pk = phi[k], f = 0.0, dg = 0.0, off[6];
+#pragma unroll
+        for (int dir = 0; dir < 6; dir++) {
+            int kb; double h, ar;
+            off[dir] = 0.0;
+            if (!sd_nbr(g, i, j, l, k, dir, kb, h, ar)) continue;
+            double eb = d.eps[kb], cf = (2. * ek * eb / (ek + eb)) * ar / h;
+            f += cf * (phi[kb] - pk);
+            dg += cf;
+            if (!d.con_mask[kb]) off[dir] = -cf;
+        }
+        double vol = g.Lx[i] * g.Ly[j] * g.Lz[l];
+        if (d.kind[k] == K_SEMI) {
+            double VT = d.mats[d.mat[k]].VT;
+            double n = sd_nK(d, k, pk, d.phin[k], VT), p = sd_pK(d, k, pk, d.phip[k], VT);
+            f += vol * SD_Q * (p - n + d.Nd[k] - d.Na[k] + d.Nf[k]);
+            dg += vol * SD_Q * (n + p) / VT;
+        } else f += vol * SD_Q * d.Nf[k];
+        int gm = d.gate_mask[k];
+        if (gm) {                                       // oxide (Robin) gate
+            const ConDev &c = d.cons[gm - 1];
+            double ra = d.robA[k];
+            f += ra * (c.Cox * (c.VGp - pk) + c.qox);
+            dg += c.Cox * ra;
+        }
+        F[k] = f;
+        double s;
+        if (build) {
+            A.c[k] = dg; A.xm[k] = off[0]; A.xp[k] = off[1]; A.ym[k] = off[2];
+            A.yp[k] = off[3]; A.zm[k] = off[4]; A.zp[k] = off[5];
+            sc[k] = dg;
+            s = dg;
+        } else s = sc[k];
+        double r = f / s;
+        acc[0] += r * r;
+    }
+    red_finish<1, false>(acc, rb, slot);
+}
+void ph_poisson(hipStream_t s, const DevView &d, const double *phi, int build, double *F, Sten7<double> A,
+                double *sc, RedBuf rb, int slot) {
+    k_poisson<<<red_grid(d.g.N), SD_BLOCK, 0, s>>>(d, phi, build, F, A, sc, rb, slot);
+}
+
+__global__ void k_update_psi(int N, const double *phi, const double *dx, double lam, double *out) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k < N) out[k] = phi[k] + lam * dx[k];
+}
+void ph_update_psi(hipStream_t s, int N, double *phi, const double *dx, double lam, double *out) {
+    k_update_psi<<<sd_nblocks(N), SD_BLOCK, 0, s>>>(N, phi, dx, lam, out);
+}
+
+// --------------------------------------------------------------- carriers
+__global__ void k_rebuild_np(DevView d) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= d.g.N) return;
+    if (d.kind[k] != K_SEMI) { d.n[k] = d.p[k] = 0.0; return; }
+    double VT = d.mats[d.mat[k]].VT, ps = d.psi[k];
+    d.n[k] = sd_nK(d, k, ps, d.phin[k], VT);
+    d.p[k] = sd_pK(d, k, ps, d.phip[k], VT);
+}
+void ph_rebuild_np(hipStream_t s, const DevView &d) { k_rebuild_np<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d); }
+
+// ------------------------------------------------------- face mobilities
+__global__ void k_upd_mob(DevView d, const double *qn, const double *qp) {
+    const Geo g = d.g;
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= g.N) return;
+    int i, j, l;
+    sd_unidx(g, k, i, j, l);
+    const int sat = (d.models & 4) != 0;
+    const size_t N = g.N;
+    for (int ax = 0; ax < 3; ax++) {
+        int kp; double h, ar;
+        bool ok = d.kind[k] == K_SEMI && sd_nbr(g, i, j, l, k, 2 * ax + 1, kp, h, ar) && d.kind[kp] == K_SEMI;
+        if (!ok) { d.fmu[ax * N + k] = 0.0; d.fmu[(3 + ax) * N + k] = 0.0; continue; }
+        const MatDev &m = d.mats[d.mat[k]];
+        double mn = 0.5 * (d.mu_n[k] + d.mu_n[kp]), mp = 0.5 * (d.mu_p[k] + d.mu_p[kp]);
+        if (ax == d.caxis) {
+            if (m.ani_n > 0.) mn *= m.ani_n;
+            if (m.ani_p > 0.) mp *= m.ani_p;
+        }
+        if (sat) {
+            double wv = 1.0;
+            double dps = d.psi[kp] - d.psi[k];
+            double Fn = sd_heat_field(qn[kp] - qn[k], dps, h), Fp = sd_heat_field(qp[kp] - qp[k], dps, h);
+            double sn = sd_mob_hf(mn, Fn, m.vsat_n, m.beta_n), sp = sd_mob_hf(mp, Fp, m.vsat_p, m.beta_p);
+            mn += wv * (sn - mn);
+            mp += wv * (sp - mp);
+        }
+        d.fmu[ax * N + k] = mn;
+        d.fmu[(3 + ax) * N + k] = mp;
+    }
+}
+void ph_upd_mob(hipStream_t s, const DevView &d, const double *qn, const double *qp) {
+    k_upd_mob<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d, qn, qp);
+}
+
+// ---------------------------------------------- Scharfetter-Gummel continuity
+// Scaled rows: unknown u = c/s (s = current carrier density), row scale
+// 1/(diag*s) so the diagonal is 1; dsc = diag*s restores the conservation
+// form for the multigrid restriction.
+__global__ void k_cont(DevView d, int hole, Sten7<double> A, double *rhs, double *sc, double *dsc) {
+    const Geo g = d.g;
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= g.N) return;
+    const double *car = hole ? d.p : d.n;
+    if (d.kind[k] != K_SEMI || d.con_mask[k]) {
+        A.c[k] = 1.0; A.xm[k] = A.xp[k] = A.ym[k] = A.yp[k] = A.zm[k] = A.zp[k] = 0.0;
+        bool semi = d.kind[k] == K_SEMI;
+        rhs[k] = semi ? 1.0 : 0.0;
+        sc[k] = semi ? car[k] : 0.0;
+        dsc[k] = 0.0;
+        return;
+    }
+    int i, j, l;
+    sd_unidx(g, k, i, j, l);
+    const MatDev &ma = d.mats[d.mat[k]];
+    const double VT = ma.VT, sk = fmax(car[k], SD_NFLOOR), pk = d.psi[k];
+    const double lk = hole ? d.lnvK[k] : d.lncK[k];
+    const size_t N = g.N;
+    const double *fm = d.fmu + (size_t)(hole ? 3 : 0) * N;
+    double dg = 0.0, b = 0.0, off[6];
+#pragma unroll
+    for (int dir = 0; dir < 6; dir++) {
+        int kb; double h, ar;
+        off[dir] = 0.0;
+        if (!sd_nbr(g, i, j, l, k, dir, kb, h, ar) || d.kind[kb] != K_SEMI) continue;
+        double dl = (hole ? d.lnvK[kb] : d.lncK[kb]) - lk;
+        double D = (d.psi[kb] - pk) / VT + (hole ? -dl : dl);
+        int kl = (dir & 1) ? k : kb;
+        double gg = ar * fm[(size_t)(dir >> 1) * N + kl] * VT / h;
+        double bp, bm;
+        sd_bern2(D, bp, bm);
+        double self = hole ? gg * bp : gg * bm;
+        double nb = hole ? gg * bm : gg * bp;
+        dg += self;
+        if (d.con_mask[kb]) b += nb * car[kb];
+        else off[dir] = -nb * fmax(car[kb], SD_NFLOOR);
+    }
+    double R, dRn, dRp;
+    sd_recomb(ma, d.niK[k], sd_tauf(d, k, ma), d.n[k], d.p[k], R, dRn, dRp);
+    double dR = hole ? dRp : dRn;
+    if (dR < 0.) dR = 0.;
+    double vol = g.Lx[i] * g.Ly[j] * g.Lz[l];
+    dg += vol * dR;
+    b += vol * (dR * car[k] - R);
+    if (d.gii) b += d.gii[k];
+    double w = 1.0 / (dg * sk);
+    A.c[k] = 1.0;
+    A.xm[k] = off[0] * w; A.xp[k] = off[1] * w; A.ym[k] = off[2] * w;
+    A.yp[k] = off[3] * w; A.zm[k] = off[4] * w; A.zp[k] = off[5] * w;
+    rhs[k] = b * w;
+    sc[k] = sk;
+    dsc[k] = dg * sk;
+}
+void ph_cont(hipStream_t s, const DevView &d, int hole, Sten7<double> A, double *rhs, double *sc, double *dsc) {
+    k_cont<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d, hole, A, rhs, sc, dsc);
+}
+
+// c = s*u -> quasi-Fermi potential clamped to [lo, hi]; weighted max change
+__global__ void __launch_bounds__(SD_BLOCK) k_cont_update(DevView d, int hole, const double *sol, const double *sc,
+                                                          double *dq, double lo, double hi, RedBuf rb, int slot) {
+    double acc[1] = {0.0};
+    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < d.g.N; k += gridDim.x * SD_BLOCK) {
+        if (d.kind[k] != K_SEMI || d.con_mask[k]) { dq[k] = 0.0; continue; }
+        double VT = d.mats[d.mat[k]].VT, ps = d.psi[k];
+        double u = sol[k];
+        if (!(u > 1e-30)) u = 1e-30; else if (u > 1e30) u = 1e30;
+        double c = sc[k] * u, q;
+        if (hole) q = ps + VT * (log(c) - d.lnvK[k]);
+        else q = ps - VT * (log(c) - d.lncK[k]);
+        if (q < lo) q = lo; else if (q > hi) q = hi;
+        double *qf = hole ? d.phip : d.phin;
+        double ch = q - qf[k];
+        qf[k] = q;
+        dq[k] = ch;
+        double cn = hole ? sd_pK(d, k, ps, q, VT) : sd_nK(d, k, ps, q, VT);
+        if (hole) d.p[k] = cn; else d.n[k] = cn;
+        double scale = d.n[k] + d.p[k] + fabs(d.Nd[k] - d.Na[k]);
+        double r = scale > 0. ? cn / scale : 0.;
+        if (r > 1e-6) {
+            double w = r >= 1e-3 ? 1. : r * 1e3, a = w * fabs(ch);
+            acc[0] = sd_nanmax(acc[0], a);
+        }
+    }
+    red_finish<1, true>(acc, rb, slot);
+}
+void ph_cont_update(hipStream_t s, const DevView &d, int hole, const double *sol, const double *sc, double *dq,
+                    double lo, double hi, RedBuf rb, int slot) {
+    k_cont_update<<<red_grid(d.g.N), SD_BLOCK, 0, s>>>(d, hole, sol, sc, dq, lo, hi, rb, slot);
+}
+
+__global__ void k_clamp_qf(DevView d, double lo, double hi) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= d.g.N) return;
+    double a = d.phin[k], b = d.phip[k];
+    if (a < lo) a = lo; else if (a > hi) a = hi;
+    if (b < lo) b = lo; else if (b > hi) b = hi;
+    d.phin[k] = a;
+    d.phip[k] = b;
+}
+void ph_clamp_qf(hipStream_t s, const DevView &d, double lo, double hi) {
+    k_clamp_qf<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d, lo, hi);
+}
+
+// Dirichlet values of contact nodes at the applied voltages
+__global__ void k_apply_bc(DevView d, int have_solution) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= d.g.N) return;
+    int c = d.con_mask[k];
+    if (!c) return;
+    const ConDev &cn = d.cons[c - 1];
+    double V = cn.V;
+    if (d.kind[k] == K_METAL) {
+        double pm = cn.phi_m > 0. ? cn.phi_m : d.Cref;
+        d.psi[k] = V + d.Cref - pm;
+        d.phin[k] = d.phip[k] = V;
+        d.n[k] = d.p[k] = 0.0;
+        return;
+    }
+    double VT = d.mats[d.mat[k]].VT;
+    if (cn.bc == 1) {
+        double pm = cn.phi_m > 0. ? cn.phi_m : d.Cref;
+        d.psi[k] = V + d.Cref - pm;
+        if (cn.te && have_solution) {
+            d.n[k] = sd_nK(d, k, d.psi[k], d.phin[k], VT);
+            d.p[k] = sd_pK(d, k, d.psi[k], d.phip[k], VT);
+            return;
+        }
+    } else d.psi[k] = neutral_psi(d, k, V, VT);
+    d.phin[k] = d.phip[k] = V;
+    d.n[k] = sd_nK(d, k, d.psi[k], V, VT);
+    d.p[k] = sd_pK(d, k, d.psi[k], V, VT);
+}
+void ph_apply_bc(hipStream_t s, const DevView &d, int have_solution) {
+    k_apply_bc<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d, have_solution);
+}
+
+__global__ void k_init_eq(DevView d) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= d.g.N) return;
+    d.phin[k] = d.phip[k] = 0.0;
+    if (d.kind[k] == K_SEMI) {
+        double VT = d.mats[d.mat[k]].VT;
+        double ps = neutral_psi(d, k, 0.0, VT);
+        d.psi[k] = ps;
+        d.n[k] = sd_nK(d, k, ps, 0.0, VT);
+        d.p[k] = sd_pK(d, k, ps, 0.0, VT);
+    } else {
+        d.psi[k] = 0.0;
+        d.n[k] = d.p[k] = 0.0;
+    }
+}
+void ph_init_eq(hipStream_t s, const DevView &d) { k_init_eq<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d); }
+
+// quasi-neutral nodes follow their quasi-Fermi change (gummel :3853-3859)
+__global__ void k_neutral_pred(DevView d, const double *dqn, const double *dqp) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= d.g.N) return;
+    if (d.kind[k] != K_SEMI || d.con_mask[k]) return;
+    double n = d.n[k], p = d.p[k], D = fabs(d.Nd[k] - d.Na[k]);
+    double den = fmax(n + p, D);
+    if (!(den > 0.)) return;
+    d.psi[k] += (n * dqn[k] + p * dqp[k]) / den;
+}
+void ph_neutral_pred(hipStream_t s, const DevView &d, const double *dqn, const double *dqp) {
+    k_neutral_pred<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d, dqn, dqp);
+}
+
+// ---------------------------------------------------- SG edge flux (A)
+__device__ __forceinline__ void face_flux(const DevView &d, int ka, int kb, double h, double ar, double &Jn,
+                                          double &Jp, double &Sn, double &Sp) {
+    const double VT = d.mats[d.mat[ka]].VT;
+    double dps = (d.psi[kb] - d.psi[ka]) / VT;
+    double Dn = dps + (d.lncK[kb] - d.lncK[ka]), Dp = dps - (d.lnvK[kb] - d.lnvK[ka]);
+    int df = ka < kb ? kb - ka : ka - kb;
+    int ax = df == 1 ? 0 : (df == d.g.Nx ? 1 : 2);
+    int kl = ka < kb ? ka : kb;
+    size_t N = d.g.N;
+    double gn = SD_Q * ar * d.fmu[ax * N + kl] * VT / h;
+    double gp = SD_Q * ar * d.fmu[(3 + ax) * N + kl] * VT / h;
+    double bpn, bmn, bpp, bmp;
+    sd_bern2(Dn, bpn, bmn);
+    sd_bern2(Dp, bpp, bmp);
+    double a1 = gn * d.n[kb] * bpn, a2 = gn * d.n[ka] * bmn;
+    double b1 = gp * d.p[ka] * bpp, b2 = gp * d.p[kb] * bmp;
+    Jn = a1 - a2; Jp = b1 - b2; Sn = a1 + a2; Sp = b1 + b2;
+}
+
+// terminal currents: one workgroup per contact, fixed-order tree sums
+__global__ void __launch_bounds__(SD_BLOCK) k_currents(DevView d, CurEdges e, RedBuf rb) {
+    __shared__ double lds[3][SD_BLOCK / 64];
+    int c = blockIdx.x;
+    double a0 = 0., a1 = 0., a2 = 0.;
+    for (int q = e.off[c] + threadIdx.x; q < e.off[c + 1]; q += SD_BLOCK) {
+        double Jn, Jp, Sn, Sp;
+        face_flux(d, e.k[q], e.kb[q], e.h[q], e.ar[q], Jn, Jp, Sn, Sp);
+        a0 += Jn + Jp; a1 += Sn + Sp; a2 += Jn;
+    }
+    a0 = sd_wave_sum(a0); a1 = sd_wave_sum(a1); a2 = sd_wave_sum(a2);
+    int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    if (ln == 0) { lds[0][w] = a0; lds[1][w] = a1; lds[2][w] = a2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int v = 0; v < 3; v++) {
+            double s = lds[v][0];
+            for (int q = 1; q < SD_BLOCK / 64; q++) s += lds[v][q];
+            rb.scal[S_CUR + 3 * c + v] = s;
+        }
+    }
+}
+void ph_currents(hipStream_t s, const DevView &d, CurEdges e, RedBuf rb) {
+    if (e.ncon > 0) k_currents<<<e.ncon, SD_BLOCK, 0, s>>>(d, e, rb);
+}
+
+__global__ void k_extrap(int N, double *d0, double *d1, double *d2, const double *a0, const double *a1,
+                         const double *a2, const double *b0, const double *b1, const double *b2, double r) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= N) return;
+    double x1 = a0[k], x2 = b0[k]; d0[k] = x1 + r * (x1 - x2);
+    x1 = a1[k]; x2 = b1[k]; d1[k] = x1 + r * (x1 - x2);
+    x1 = a2[k]; x2 = b2[k]; d2[k] = x1 + r * (x1 - x2);
+}
+void ph_extrap(hipStream_t s, int N, double *dst0, double *dst1, double *dst2, const double *a0, const double *a1,
+               const double *a2, const double *b0, const double *b1, const double *b2, double r) {
+    k_extrap<<<sd_nblocks(N), SD_BLOCK, 0, s>>>(N, dst0, dst1, dst2, a0, a1, a2, b0, b1, b2, r);
+}
+
+// --------------------------------------------- floating majority regions
+__global__ void k_maj_flags(DevView d, int hole, const double *u, const double *s, unsigned char *flag) {
+    const Geo g = d.g;
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= g.N) return;
+    const double *oth = hole ? d.n : d.p;
+    auto maj = [&](int q) -> bool {
+        double c = s[q] * u[q], D = 0.5 * fabs(d.Nd[q] - d.Na[q]);
+        return d.kind[q] == K_SEMI && !d.con_mask[q] && c >= oth[q] && c >= D && s[q] >= D && c > 10. * d.niK[q];
+    };
+    if (!maj(k)) { flag[k] = 0; return; }
+    int i, j, l;
+    sd_unidx(g, k, i, j, l);
+    unsigned char f = 1;
+    for (int dir = 0; dir < 6; dir++) {
+        int kb; double h, ar;
+        if (!sd_nbr(g, i, j, l, k, dir, kb, h, ar) || d.kind[kb] != K_SEMI) continue;
+        if (d.con_mask[kb]) f = 2;
+    }
+    flag[k] = f;
+}
+void ph_maj_flags(hipStream_t s, const DevView &d, int hole, const double *u, const double *sc, unsigned char *flag) {
+    k_maj_flags<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d, hole, u, sc, flag);
+}
+// anchoring travels along whole lines of one axis per launch (forward, then back)
+__global__ void k_maj_prop(Geo g, unsigned char *flag, int axis, int *changed) {
+    int t = blockIdx.x * SD_BLOCK + threadIdx.x;
+    int n, stride, k0;
+    if (axis == 0) {
+        if (t >= g.Ny * g.Nz) return;
+        n = g.Nx; stride = 1; k0 = t * g.Nx;
+    } else if (axis == 1) {
+        if (t >= g.Nx * g.Nz) return;
+        int i = t % g.Nx, l = t / g.Nx;
+        n = g.Ny; stride = g.Nx; k0 = i + g.Nx * g.Ny * l;
+    } else {
+        if (t >= g.Nx * g.Ny) return;
+        n = g.Nz; stride = g.Nx * g.Ny; k0 = t;
+    }
+    int ch = 0;
+    unsigned char prev = 0;
+    for (int q = 0; q < n; q++) {
+        int k = k0 + q * stride;
+        unsigned char f = flag[k];
+        if (f == 1 && prev == 2) { f = 2; flag[k] = 2; ch = 1; }
+        prev = f;
+    }
+    prev = 0;
+    for (int q = n - 1; q >= 0; q--) {
+        int k = k0 + q * stride;
+        unsigned char f = flag[k];
+        if (f == 1 && prev == 2) { f = 2; flag[k] = 2; ch = 1; }
+        prev = f;
+    }
+    if (ch) *changed = 1;
+}
+void ph_maj_propagate(hipStream_t s, const DevView &d, unsigned char *flag, int axis, int *changed) {
+    const Geo &g = d.g;
+    int n = axis == 0 ? g.Ny * g.Nz : (axis == 1 ? g.Nx * g.Nz : g.Nx * g.Ny);
+    k_maj_prop<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(g, flag, axis, changed);
+}
+__global__ void __launch_bounds__(SD_BLOCK) k_maj_count(int N, const unsigned char *flag, RedBuf rb, int slot) {
+    double acc[1] = {0.0};
+    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < N; k += gridDim.x * SD_BLOCK) acc[0] += flag[k] == 1;
+    red_finish<1, false>(acc, rb, slot);
+}
+void ph_maj_count(hipStream_t s, int N, const unsigned char *flag, RedBuf rb, int slot) {
+    k_maj_count<<<red_grid(N), SD_BLOCK, 0, s>>>(N, flag, rb, slot);
+}
+
+// ------------------------------------------------------------- Anderson
+__device__ __forceinline__ bool aa_sig(const DevView &d, int k, int hole) {
+    if (d.kind[k] != K_SEMI || d.con_mask[k]) return false;
+    double c = hole ? d.p[k] : d.n[k];
+    return c > 1e-6 * (d.n[k] + d.p[k] + fabs(d.Nd[k] - d.Na[k]));
+}
+__global__ void k_aa_diff(DevView d, const double *x0, const double *x1, const double *x2, const double *g0,
+                          const double *g1, const double *g2, double *f, double *gp, float *dF, float *dG,
+                          int have_prev) {
+    int N = d.g.N;
+    int t = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (t >= 3 * N) return;
+    int b = t / N, k = t - b * N;
+    const double *xb = b == 0 ? x0 : (b == 1 ? x1 : x2);
+    const double *gb = b == 0 ? g0 : (b == 1 ? g1 : g2);
+    double gk = gb[k];
+    double fk = (b == 0 || aa_sig(d, k, b - 1)) ? gk - xb[k] : 0.0;
+    if (have_prev) {
+        dF[t] = (float)(fk - f[t]);
+        dG[t] = (float)(gk - gp[t]);
+    }
+    f[t] = fk;
+    gp[t] = gk;
+}
+void ph_aa_diff(hipStream_t s, const DevView &d, const double *x0, const double *x1, const double *x2,
+                const double *g0, const double *g1, const double *g2, double *f, double *gp, float *dF, float *dG,
+                int have_prev) {
+    k_aa_diff<<<sd_nblocks(3L * d.g.N), SD_BLOCK, 0, s>>>(d, x0, x1, x2, g0, g1, g2, f, gp, dF, dG, have_prev);
+}
+struct VecList { float *v[12]; };
+// dots of up to 12 float vectors with one vector (float y or double yd)
+__global__ void __launch_bounds__(SD_BLOCK) k_aa_dots(int L, VecList vl, int nv, const float *y, const double *yd,
+                                                      RedBuf rb, int slot0) {
+    double acc[12];
+#pragma unroll
+    for (int q = 0; q < 12; q++) acc[q] = 0.0;
+    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < L; k += gridDim.x * SD_BLOCK) {
+        double yk = y ? (double)y[k] : yd[k];
+#pragma unroll
+        for (int q = 0; q < 12; q++)
+            if (q < nv) acc[q] += (double)vl.v[q][k] * yk;
+    }
+    red_finish<12, false>(acc, rb, slot0);
+}
+void ph_aa_dots(hipStream_t s, int L, float *const *vecs, int nv, const float *y, const double *yd, RedBuf rb,
+                int slot0) {
+    VecList vl;
+    for (int q = 0; q < 12; q++) vl.v[q] = q < nv ? vecs[q] : nullptr;
+    k_aa_dots<<<red_grid(L), SD_BLOCK, 0, s>>>(L, vl, nv, y, yd, rb, slot0);
+}
+struct GamList { float *v[12]; double gam[12]; };
+__global__ void k_aa_mix(DevView d, double *g0, double *g1, double *g2, GamList gl, int m) {
+    int N = d.g.N;
+    int t = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (t >= 3 * N) return;
+    int b = t / N, k = t - b * N;
+    if (b == 0 ? d.con_mask[k] != 0 : d.con_mask[k] != 0) return;     // psi and carriers fixed on contacts
+    if (b > 0 && !aa_sig(d, k, b - 1)) return;
+    double *gb = b == 0 ? g0 : (b == 1 ? g1 : g2);
+    double s = gb[k];
+    for (int i = 0; i < m; i++) s -= gl.gam[i] * (double)gl.v[i][t];
+    gb[k] = s;
+}
+void ph_aa_mix(hipStream_t s, const DevView &d, double *g0, double *g1, double *g2, float *const *dG, const double *gam,
+               int m) {
+    GamList gl;
+    for (int q = 0; q < 12; q++) { gl.v[q] = q < m ? dG[q] : nullptr; gl.gam[q] = q < m ? gam[q] : 0.0; }
+    k_aa_mix<<<sd_nblocks(3L * d.g.N), SD_BLOCK, 0, s>>>(d, g0, g1, g2, gl, m);
+}
+
+// ------------------------------------------------------ display fields
+__global__ void k_upd_J(DevView d, double *Jx, double *Jy, double *Jz, double *Rt) {
+    const Geo g = d.g;
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= g.N) return;
+    if (d.kind[k] != K_SEMI) { Jx[k] = Jy[k] = Jz[k] = 0.; Rt[k] = 0.; return; }
+    int i, j, l;
+    sd_unidx(g, k, i, j, l);
+    double *Jo[3] = {Jx, Jy, Jz};
+    for (int ax = 0; ax < 3; ax++) {
+        double s = 0.; int cnt = 0, kb; double h, ar, Jn, Jp, Sn, Sp;
+        if (sd_nbr(g, i, j, l, k, 2 * ax, kb, h, ar) && d.kind[kb] == K_SEMI) {
+            face_flux(d, kb, k, h, ar, Jn, Jp, Sn, Sp); s += (Jn + Jp) / ar; cnt++;
+        }
+        if (sd_nbr(g, i, j, l, k, 2 * ax + 1, kb, h, ar) && d.kind[kb] == K_SEMI) {
+            face_flux(d, k, kb, h, ar, Jn, Jp, Sn, Sp); s += (Jn + Jp) / ar; cnt++;
+        }
+        Jo[ax][k] = cnt ? s / cnt : 0.;
+    }
+    const MatDev &m = d.mats[d.mat[k]];
+    double R, a, b;
+    sd_recomb(m, d.niK[k], sd_tauf(d, k, m), d.n[k], d.p[k], R, a, b);
+    Rt[k] = R;
+}
+void ph_upd_J(hipStream_t s, const DevView &d, double *Jx, double *Jy, double *Jz, double *R) {
+    k_upd_J<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d, Jx, Jy, Jz, R);
+}
+__global__ void k_mob_display(DevView d, double *mun, double *mup) {
+    const Geo g = d.g;
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= g.N) return;
+    if (d.kind[k] != K_SEMI) { mun[k] = d.mu_n[k]; mup[k] = d.mu_p[k]; return; }
+    int i, j, l;
+    sd_unidx(g, k, i, j, l);
+    size_t N = g.N;
+    double sn = 0., sp = 0.; int c = 0;
+    for (int dir = 0; dir < 6; dir++) {
+        int kb; double h, ar;
+        if (!sd_nbr(g, i, j, l, k, dir, kb, h, ar) || d.kind[kb] != K_SEMI) continue;
+        int kl = k < kb ? k : kb, ax = dir / 2;
+        double fn = d.fmu[ax * N + kl], fp = d.fmu[(3 + ax) * N + kl];
+        if (ax == d.caxis) {
+            const MatDev &m = d.mats[d.mat[kl]];
+            if (m.ani_n > 0.) fn /= m.ani_n;
+            if (m.ani_p > 0.) fp /= m.ani_p;
+        }
+        sn += fn; sp += fp; c++;
+    }
+    if (c) { mun[k] = sn / c; mup[k] = sp / c; }
+    else { mun[k] = d.mu_n[k]; mup[k] = d.mu_p[k]; }
+}
+void ph_mob_display(hipStream_t s, const DevView &d, double *mun, double *mup) {
+    k_mob_display<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d, mun, mup);
+}
+__global__ void k_node_Jc(DevView d, int ax, int hole, double *out) {
+    const Geo g = d.g;
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= g.N) return;
+    if (d.kind[k] != K_SEMI) { out[k] = 0.; return; }
+    int i, j, l;
+    sd_unidx(g, k, i, j, l);
+    double s = 0.; int cnt = 0, kb; double h, ar, Jn, Jp, Sn, Sp;
+    if (sd_nbr(g, i, j, l, k, 2 * ax, kb, h, ar) && d.kind[kb] == K_SEMI) {
+        face_flux(d, kb, k, h, ar, Jn, Jp, Sn, Sp); s += (hole ? Jp : Jn) / ar; cnt++;
+    }
+    if (sd_nbr(g, i, j, l, k, 2 * ax + 1, kb, h, ar) && d.kind[kb] == K_SEMI) {
+        face_flux(d, k, kb, h, ar, Jn, Jp, Sn, Sp); s += (hole ? Jp : Jn) / ar; cnt++;
+    }
+    out[k] = cnt ? s / cnt : 0.;
+}
+void ph_node_Jc(hipStream_t s, const DevView &d, int ax, int hole, double *out) {
+    k_node_Jc<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d, ax, hole, out);
+}
+__global__ void k_face_J(DevView d, int ax, int carrier, double *out) {
+    const Geo g = d.g;
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= g.N) return;
+    int i, j, l, kb;
+    double h, ar, Jn, Jp, Sn, Sp;
+    sd_unidx(g, k, i, j, l);
+    out[k] = 0.;
+    if (d.kind[k] != K_SEMI || !sd_nbr(g, i, j, l, k, 2 * ax + 1, kb, h, ar) || d.kind[kb] != K_SEMI) return;
+    face_flux(d, k, kb, h, ar, Jn, Jp, Sn, Sp);
+    out[k] = (carrier ? Jn : Jn + Jp) / ar;
+}
+void ph_face_J(hipStream_t s, const DevView &d, int ax, int carrier, double *out) {
+    k_face_J<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d, ax, carrier, out);
+}
+
+// ------------------------------------------------ impact ionisation (bit 8)
+__device__ __forceinline__ double ii_alpha(const MatDev &m, double F, int c) {
+    if (!(F > 1e5)) return 0.;
+    int r = F >= m.ii_E0[c];
+    double x = m.ii_g * m.ii_b[c][r] / F;
+    if (m.ii_m[c] != 1.) x = pow(x, m.ii_m[c]);
+    if (x > 700.) return 0.;
+    return m.ii_g * m.ii_a[c][r] * exp(-x);
+}
+__device__ __forceinline__ double grad_semi(const DevView &d, int i, int j, int l, int k, int ax) {
+    const Geo &g = d.g;
+    int km = 0, kp = 0; double hm = 1., hp = 1., ar;
+    bool a = sd_nbr(g, i, j, l, k, 2 * ax, km, hm, ar) && d.kind[km] == K_SEMI;
+    bool b = sd_nbr(g, i, j, l, k, 2 * ax + 1, kp, hp, ar) && d.kind[kp] == K_SEMI;
+    if (a && b) return (d.psi[kp] - d.psi[km]) / (hm + hp);
+    if (b) return (d.psi[kp] - d.psi[k]) / hp;
+    if (a) return (d.psi[k] - d.psi[km]) / hm;
+    return 0.;
+}
+// G = (alpha_n |J_n| + alpha_p |J_p|)/q * vol, semiconductor_3d.c:4486-4523
+__global__ void k_ii_update(DevView d) {
+    const Geo g = d.g;
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= g.N) return;
+    d.gii[k] = 0.;
+    if (d.kind[k] != K_SEMI || d.con_mask[k]) return;
+    const MatDev &m = d.mats[d.mat[k]];
+    if (!m.ii_on) return;
+    int i, j, l;
+    sd_unidx(g, k, i, j, l);
+    double Jn[3], Jp[3], E[3];
+    for (int ax = 0; ax < 3; ax++) {
+        double sn = 0., sp = 0., h, ar, a, b, c2, e; int cnt = 0, kb;
+        if (sd_nbr(g, i, j, l, k, 2 * ax, kb, h, ar) && d.kind[kb] == K_SEMI) {
+            face_flux(d, kb, k, h, ar, a, b, c2, e); sn += a / ar; sp += b / ar; cnt++;
+        }
+        if (sd_nbr(g, i, j, l, k, 2 * ax + 1, kb, h, ar) && d.kind[kb] == K_SEMI) {
+            face_flux(d, k, kb, h, ar, a, b, c2, e); sn += a / ar; sp += b / ar; cnt++;
+        }
+        Jn[ax] = cnt ? sn / cnt : 0.;
+        Jp[ax] = cnt ? sp / cnt : 0.;
+        E[ax] = -grad_semi(d, i, j, l, k, ax);
+    }
+    double vol = g.Lx[i] * g.Ly[j] * g.Lz[l], G = 0.;
+    for (int c = 0; c < 2; c++) {
+        const double *J = c ? Jp : Jn;
+        double jj = sqrt(J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
+        if (!(jj > 0.)) continue;
+        double F = fabs(E[0] * J[0] + E[1] * J[1] + E[2] * J[2]) / jj;
+        G += ii_alpha(m, F, c) * jj;
+    }
+    d.gii[k] = G / SD_Q * vol;
+}
+void ph_ii_update(hipStream_t s, const DevView &d) { k_ii_update<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d); }

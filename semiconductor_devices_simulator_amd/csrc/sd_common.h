@@ -1,0 +1,201 @@
+// sd_common.h - device data layout and per-node physics of the MI355X
+// drift-diffusion core.  Everything here is shared by the kernel files and
+// the host engine.
+//
+// Layout: every node quantity is a structure-of-arrays FP64 vector over the
+// tensor mesh, node index k = i + Nx*(j + Ny*l) (x fastest, as the reference
+// IDX macro, semiconductor_3d.c:114).  Matrices are 7-point stencils stored as
+// 7 diagonal vectors (c, xm, xp, ym, yp, zm, zp), semiconductor_3d.c:210.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cmath>
+
+#define SD_Q      1.602176634e-19
+#define SD_KB     1.380649e-23
+#define SD_EPS0   8.8541878128e-12
+#define SD_EPS_OX (3.9 * SD_EPS0)
+#define SD_LN_MIN (-230.0)
+#define SD_LN_MAX 150.0
+#define SD_NFLOOR 1e-100
+#define SD_NM_MAX 16
+#define SD_NC_MAX 32
+#define SD_BLOCK  256
+
+enum { K_SEMI = 0, K_INS = 1, K_METAL = 2 };
+
+// Material parameters the hot path needs (subset of the reference Mat,
+// semiconductor_3d.c:127-188), in SI units.
+struct MatDev {
+    double VT, lnc, lnv, ni, Eg, chi;
+    double tau_n0, tau_p0, Cn, Cp, Brad, tau_Nref;
+    double vsat_n, vsat_p, beta_n, beta_p;
+    double ani_n, ani_p;
+    double ii_a[2][2], ii_b[2][2], ii_E0[2], ii_m[2], ii_g;
+    int insulator, ii_on;
+};
+
+// Per-contact data read by kernels (semiconductor_3d.c:191-208).
+struct ConDev {
+    double V;        // applied voltage of the current (sub)step
+    double phi_m;    // work function (eV), 0 = reference
+    double Cox;      // EPS_OX / tox
+    double VGp;      // V + Cref - phi_m (Robin gate)
+    double qox;      // fixed oxide charge (C/m^2)
+    int bc, te;
+};
+
+// Tensor mesh geometry (device pointers, Nx+Ny+Nz doubles each).
+struct Geo {
+    int Nx, Ny, Nz, N;
+    const double *xs, *ys, *zs, *Lx, *Ly, *Lz;
+};
+
+// Everything the physics kernels read or write, as raw device pointers.
+struct DevView {
+    Geo g;
+    const unsigned char *kind;
+    const int *con_mask, *gate_mask;
+    const unsigned char *mat;
+    const double *Nd, *Na, *Nf, *eps, *bgn, *robA;
+    const double *lncK, *lnvK, *niK;     // per-node ln Nc_eff, ln Nv_eff, n_i,eff
+    double *psi, *phin, *phip, *n, *p;
+    double *mu_n, *mu_p;                 // low-field node mobility
+    double *fmu;                         // 6 face mobilities, block q at fmu + q*N
+    double *gii;                         // impact-ionisation generation (1/s per node)
+    const MatDev *mats;
+    const ConDev *cons;
+    int models, caxis, ref_mat;
+    double Cref;
+};
+
+// 7-point stencil (level 0 in FP64, coarse levels in T).
+template <class T> struct Sten7 {
+    T *c, *xm, *xp, *ym, *yp, *zm, *zp;
+};
+
+__device__ __forceinline__ void sd_unidx(const Geo &g, int k, int &i, int &j, int &l) {
+    int NxNy = g.Nx * g.Ny;
+    l = k / NxNy;
+    int r = k - l * NxNy;
+    j = r / g.Nx;
+    i = r - j * g.Nx;
+}
+
+// Bernoulli pair B(x) = x/(e^x - 1) and B(-x) = B(x) + x, evaluated from the
+// smaller of the two (one expm1 per edge, no cancellation).  Branches follow
+// the reference B(), semiconductor_3d.c:670-675: series below 1e-6, 0 above
+// 700.
+__device__ __forceinline__ void sd_bern2(double x, double &bp, double &bm) {
+    double a = fabs(x), s;
+    if (a < 1e-6) {
+        bp = 1.0 - x * 0.5 + x * x / 12.0;
+        bm = 1.0 + x * 0.5 + x * x / 12.0;
+        return;
+    }
+    s = (a > 700.0) ? 0.0 : a / expm1(a);     // B(|x|)
+    if (x > 0) { bp = s; bm = s + a; }
+    else       { bm = s; bp = s + a; }
+}
+
+__device__ __forceinline__ double sd_clampe(double e) {
+    return e < SD_LN_MIN ? SD_LN_MIN : (e > SD_LN_MAX ? SD_LN_MAX : e);
+}
+
+// SRH (midgap) + Auger + radiative recombination and its derivatives,
+// semiconductor_3d.c:686-697.
+__device__ __forceinline__ void sd_recomb(const MatDev &m, double ni, double tf, double n, double p,
+                                          double &R, double &dRn, double &dRp) {
+    double ni2 = ni * ni, np = n * p - ni2;
+    double den = tf * (m.tau_p0 * (n + ni) + m.tau_n0 * (p + ni)) + 1e-300;
+    double Rs = np / den;
+    double dsn = (p * den - np * tf * m.tau_p0) / (den * den);
+    double dsp = (n * den - np * tf * m.tau_n0) / (den * den);
+    double Au = m.Cn * n + m.Cp * p;
+    R = Rs + Au * np + m.Brad * np;
+    dRn = dsn + m.Cn * np + Au * p + m.Brad * p;
+    dRp = dsp + m.Cp * np + Au * n + m.Brad * n;
+}
+
+// SRH lifetime doping factor, semiconductor_3d.c:787-791.
+__device__ __forceinline__ double sd_tauf(const DevView &d, int k, const MatDev &m) {
+    if (!(d.models & 2) || !(m.tau_Nref > 0.)) return 1.0;
+    return 1.0 / (1.0 + (d.Nd[k] + d.Na[k]) / m.tau_Nref);
+}
+
+// Caughey-Thomas velocity saturation, semiconductor_3d.c:664-668.
+__device__ __forceinline__ double sd_mob_hf(double mu0, double E, double vsat, double beta) {
+    double r = mu0 * fabs(E) / vsat;
+    if (r < 1e-12) return mu0;
+    return mu0 / pow(1.0 + pow(r, beta), 1.0 / beta);
+}
+
+// Driving field of an edge: harmonic mean of the quasi-Fermi and
+// electrostatic drops when they have the same sign, semiconductor_3d.c:2882-2886.
+__device__ __forceinline__ double sd_heat_field(double dq, double dps, double h) {
+    if (dq * dps <= 0.) return 0.;
+    double a = fabs(dq), b = fabs(dps);
+    return 2. * a * b / ((a + b) * h);
+}
+
+// carrier densities at node k (qcn = qcp = 0: no quantum confinement)
+__device__ __forceinline__ double sd_nK(const DevView &d, int k, double psi, double qn, double VT) {
+    return exp(sd_clampe(d.lncK[k] + (psi - qn) / VT));
+}
+__device__ __forceinline__ double sd_pK(const DevView &d, int k, double psi, double qp, double VT) {
+    return exp(sd_clampe(d.lnvK[k] + (qp - psi) / VT));
+}
+
+// neighbour geometry along direction dir (0 x-,1 x+,2 y-,3 y+,4 z-,5 z+),
+// semiconductor_3d.c:703-713.  Returns false outside the domain.
+__device__ __forceinline__ bool sd_nbr(const Geo &g, int i, int j, int l, int k, int dir,
+                                       int &kb, double &h, double &ar) {
+    switch (dir) {
+    case 0: if (i == 0) return false; kb = k - 1; h = g.xs[i] - g.xs[i - 1]; ar = g.Ly[j] * g.Lz[l]; return true;
+    case 1: if (i == g.Nx - 1) return false; kb = k + 1; h = g.xs[i + 1] - g.xs[i]; ar = g.Ly[j] * g.Lz[l]; return true;
+    case 2: if (j == 0) return false; kb = k - g.Nx; h = g.ys[j] - g.ys[j - 1]; ar = g.Lx[i] * g.Lz[l]; return true;
+    case 3: if (j == g.Ny - 1) return false; kb = k + g.Nx; h = g.ys[j + 1] - g.ys[j]; ar = g.Lx[i] * g.Lz[l]; return true;
+    case 4: if (l == 0) return false; kb = k - g.Nx * g.Ny; h = g.zs[l] - g.zs[l - 1]; ar = g.Lx[i] * g.Ly[j]; return true;
+    default: if (l == g.Nz - 1) return false; kb = k + g.Nx * g.Ny; h = g.zs[l + 1] - g.zs[l]; ar = g.Lx[i] * g.Ly[j]; return true;
+    }
+}
+
+// deterministic block reduction helpers (wavefront shuffle, then LDS)
+template <class T>
+__device__ __forceinline__ T sd_wave_sum(T v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+// NaN-propagating maximum (a NaN anywhere in a max reduction survives)
+template <class T>
+__device__ __forceinline__ T sd_nanmax(T a, T b) { return (b != b || b > a) ? b : a; }
+template <class T>
+__device__ __forceinline__ T sd_wave_max(T v) {
+    for (int o = 32; o > 0; o >>= 1) { T w = __shfl_down(v, o, 64); v = sd_nanmax(v, w); }
+    return v;
+}
+// sum over a 256-thread block; result valid in thread 0
+template <class T>
+__device__ __forceinline__ T sd_block_sum(T v, T *lds) {
+    v = sd_wave_sum(v);
+    int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    if (ln == 0) lds[w] = v;
+    __syncthreads();
+    T s = 0;
+    if (threadIdx.x == 0) { for (int q = 0; q < (int)(blockDim.x >> 6); q++) s += lds[q]; }
+    __syncthreads();
+    return s;
+}
+template <class T>
+__device__ __forceinline__ T sd_block_max(T v, T *lds) {
+    v = sd_wave_max(v);
+    int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    if (ln == 0) lds[w] = v;
+    __syncthreads();
+    T s = lds[0];
+    if (threadIdx.x == 0) { for (int q = 1; q < (int)(blockDim.x >> 6); q++) s = sd_nanmax(s, lds[q]); }
+    __syncthreads();
+    return s;
+}
+
+inline int sd_nblocks(long n, int bs = SD_BLOCK) { return (int)((n + bs - 1) / bs); }

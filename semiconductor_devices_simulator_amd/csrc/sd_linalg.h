@@ -1,0 +1,84 @@
+// sd_linalg.h - launchers of the linear-algebra kernels (k_linalg.hip).
+#pragma once
+#include "sd_common.h"
+
+// Deterministic reductions: every launch uses a fixed grid for a given n;
+// block partials are summed in a fixed order by the last block to finish, so
+// results are bit-identical from run to run.
+struct RedBuf {
+    double *part;      // [SD_RED_MAXV][SD_RED_MAXB]
+    unsigned *cnt;     // completion counter (zero between launches)
+    double *scal;      // result slots
+};
+#define SD_RED_MAXB 1024
+#define SD_RED_MAXV 16
+
+// Scalar slots used by the Krylov drivers.
+enum {
+    S_RZ = 0, S_RZ0, S_PQ, S_AL, S_BE, S_RHO, S_RHO1, S_RV, S_OM, S_SS, S_TT, S_TS, S_RR,
+    S_FL, S_TMP0, S_TMP1, S_TMP2, S_TMP3, S_NRM, S_MAX0, S_MAX1, S_MAX2,
+    S_AA0 = 32,             // 32..(32+2*12) Anderson dots
+    S_CUR = 64,             // 64..(64+3*32) terminal currents I, S, In
+    S_NSLOT = 192
+};
+
+// Multigrid level in value type T.  Level 0 optionally aliases the FP64
+// operator; coarse levels are row-scaled to unit diagonal (c not stored) with
+// the FP64 row scale rs (the coarse operator is diag(rs) * Ahat).
+template <class T> struct MGLevel {
+    int Nx, Ny, Nz, N;
+    int cx, cy, cz;
+    Sten7<T> A;          // level 0: full stencil; coarse: c == nullptr (unit diagonal)
+    double *rs;          // row scale used by restriction into the next level (nullptr = 1)
+    T *pw;               // prolongation weight of each node to its parent
+    double *ph;          // Slotboom reference potential / VT (coarse levels)
+    T *x, *b;            // level vectors
+};
+
+// vector / reduction launchers (stream-ordered, no host sync)
+void sd_dot(hipStream_t s, const double *a, const double *b, int n, RedBuf rb, int slot);
+void sd_dot2(hipStream_t s, const double *a, const double *b, const double *c, int n, RedBuf rb,
+             int slot_ab, int slot_ac);          // a.b and a.c
+void sd_nrm2sq(hipStream_t s, const double *a, int n, RedBuf rb, int slot);
+void sd_maxabs(hipStream_t s, const double *a, int n, RedBuf rb, int slot);
+void sd_maxabsdiff(hipStream_t s, const double *a, const double *b, int n, RedBuf rb, int slot);
+void sd_reduce_raw(hipStream_t s, int nblocks, int nv, RedBuf rb, int slot0, int is_max);
+void sd_fill(hipStream_t s, double *a, double v, int n);
+void sd_copy(hipStream_t s, double *dst, const double *src, int n);
+void sd_axpy(hipStream_t s, double *y, const double *x, double a, int n);     // y += a x
+
+// y = A x, or y = b - A x (b != nullptr), FP64 level-0 operator
+void sd_spmv(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, const double *b, double *y);
+// bicgstab rounding floor sum_k (sum_j |A_kj x_j| + |b_k|)^2 -> slot
+void sd_floor(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, const double *b,
+              RedBuf rb, int slot);
+
+// Krylov fused updates (scalars read from rb.scal on the device)
+void sd_pcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q, int n, RedBuf rb);
+void sd_pcg_update_p(hipStream_t s, double *p, const double *z, int n, RedBuf rb);
+void sd_scalar_op(hipStream_t s, int op, RedBuf rb);
+enum { SOP_PCG_AL = 0, SOP_PCG_BE, SOP_BI_BE, SOP_BI_AL, SOP_BI_OM, SOP_BI_NEXT, SOP_BI_RESTART };
+void sd_bi_p(hipStream_t s, double *p, const double *r, const double *v, int n, RedBuf rb);
+void sd_bi_r1(hipStream_t s, double *r, const double *v, int n, RedBuf rb);   // r -= al v; SS
+void sd_bi_x1(hipStream_t s, double *x, const double *ph, int n, RedBuf rb);   // x += al ph
+void sd_bi_xr2(hipStream_t s, double *x, double *r, const double *ph, const double *sh, const double *t,
+               int n, RedBuf rb);                                              // RR
+void sd_bi_init(hipStream_t s, double *r, double *rh, double *p, double *v, const double *b,
+                const double *t, int n, RedBuf rb);                             // RR
+void sd_bi_restart(hipStream_t s, double *rh, double *p, double *v, const double *r, int n);
+
+// multigrid (T = double or float)
+template <class T> void mg_weights_level(hipStream_t s, const MGLevel<T> &f, MGLevel<T> &c, int L,
+                                         const DevView &d, int qsign, double VT);
+template <class T> void mg_galerkin(hipStream_t s, const MGLevel<T> &f, MGLevel<T> &c);
+template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int color, int zero);
+template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c,
+                                          const T *b, const T *x);
+template <class T> void mg_prolong(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c, T *x);
+template <class T> void mg_coarse_factor(hipStream_t s, const MGLevel<T> &v, double *inv);
+template <class T> void mg_coarse_solve(hipStream_t s, const MGLevel<T> &v, const double *inv, const T *b, T *x);
+template <class T> void mg_set_pw_one(hipStream_t s, MGLevel<T> &v);
+void mg_to_float(hipStream_t s, const double *a, float *b, int n);
+void mg_to_double(hipStream_t s, const float *a, double *b, int n);
+void mg_sten_to_float(hipStream_t s, Sten7<double> A, Sten7<float> B, int n);
+template <class T> void sd_spmv_T(hipStream_t s, const Geo &g, Sten7<T> A, const T *x, const T *b, T *y);

@@ -1,0 +1,272 @@
+"""Synthetic devices of the drift-diffusion workloads, and the api3_* call
+sequence that loads them into a solver core.
+
+The mesh, doping and contact helpers restate the reference front end's rules
+so that a device built here reaches any api3 core (the HIP core or the
+reference OpenMP core) exactly as the reference GUI would send it:
+
+  graded_grid   main.py:316-358   Gaussian-density CDF inversion
+  span_idx      main.py:363-384   half-open physical index ranges
+  Implant.conc  main.py:785-798   Gaussian / erfc implant with erf lateral mask edges
+  Contact.to_idx6  main.py:666-692  contact ranges placed by physical position
+  load_device   main.py:3286-3361 setup_device (node-data path)
+
+Workloads (sizes in nodes):
+  pn_diode      16x16x64 Si abrupt p-n diode, 1x1x4 um, forward 0-0.8 V
+  mosfet        planar n-MOSFET on an Nx x Ny x Nz tensor mesh (96x64x64,
+                192x128x128), Id-Vg 0-1.2 V at Vd 0.05 V and 1.0 V
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .core import dptr, iptr
+
+Q = 1.602176634e-19
+MAT_SI, MAT_SIO2 = 0, 8          # reference material ids (semiconductor_3d.c:431, 600)
+
+
+def graded_grid(L_nm, N, dense_nm, ratio=8.0, sigma_nm=None):
+    """N nodes in [0, L_nm] nm (returned in m), about `ratio` times denser
+    near the points dense_nm (main.py:316-358)."""
+    L = L_nm * 1e-9
+    if not dense_nm or N < 3:
+        return np.linspace(0, L, N)
+    pts = sorted(set(dense_nm))
+    if sigma_nm is None:
+        sig = []
+        for p in pts:
+            others = [q for q in pts if q != p]
+            dmin = L_nm * 0.10 if not others else min(abs(p - q) for q in others) * 0.5
+            sig.append(min(dmin, L_nm * 0.05) * 1e-9)
+    else:
+        sig = [sigma_nm * 1e-9] * len(pts)
+    sig = [max(s, L / N * 1.5) for s in sig]
+    xt = np.linspace(0, L, 20000)
+    dens = np.ones(20000)
+    for d, s in zip(pts, sig):
+        dens += (ratio - 1.0) * np.exp(-0.5 * ((xt - d * 1e-9) / s) ** 2)
+    cdf = np.cumsum(dens)
+    cdf -= cdf[0]
+    cdf /= cdf[-1]
+    xs = np.interp(np.linspace(0, 1, N), cdf, xt)
+    xs[0] = 0.0
+    xs[-1] = L
+    for i in range(1, N):
+        if xs[i] <= xs[i - 1]:
+            xs[i] = xs[i - 1] + 1e-13
+    return xs
+
+
+def span_idx(c, a, b):
+    """Inclusive index range of the half-open physical interval [a, b)
+    (main.py:363-384)."""
+    c = np.asarray(c)
+    N = len(c)
+    tol = 1e-9 * max(c[-1] - c[0], 1e-30)
+    i0 = 0 if a <= c[0] + tol else int(np.searchsorted(c, a - tol, side="left"))
+    i1 = N - 1 if b >= c[-1] - tol else int(np.searchsorted(c, b - tol, side="left")) - 1
+    i0 = max(0, min(i0, N - 1))
+    i1 = max(-1, min(i1, N - 1))
+    if i1 < i0:
+        k = int(np.argmin(np.abs(c - 0.5 * (a + b))))
+        i0 = i1 = k
+    return i0, i1
+
+
+@dataclass
+class Implant:
+    """Doping profile added on semiconductor nodes (main.py:750-798).
+    kind 'gauss': peak*exp(-((d-Rp)/Lv)^2); 'erfc': peak*erfc(d/Lv); the mask
+    window [a0,a1] x [b0,b1] (nm) on the entry face has erf edges of width
+    lat*Lv.  Concentrations in cm^-3, lengths in nm."""
+    dtype: str
+    peak: float
+    face: str = "y0"
+    a0: float = 0.0
+    a1: float = 1e9
+    b0: float = 0.0
+    b1: float = 1e9
+    kind: str = "gauss"
+    Lv: float = 100.0
+    Rp: float = 0.0
+    lat: float = 0.8
+
+    def axes(self):
+        q = "xyz".index(self.face[0])
+        o = [a for a in range(3) if a != q]
+        return q, o[0], o[1]
+
+    def conc(self, P, L3):
+        P = np.asarray(P, float).reshape(-1, 3)
+        q, ia, ib = self.axes()
+        d = P[:, q] if self.face[1] == "0" else L3[q] - P[:, q]
+        Lv = max(self.Lv, 1e-6)
+        if self.kind == "erfc":
+            v = self.peak * np.vectorize(math.erfc)(np.maximum(d, 0.0) / Lv)
+        else:
+            v = self.peak * np.exp(-((d - self.Rp) / Lv) ** 2)
+        w = max(self.lat * Lv, 1e-9)
+        erf = np.vectorize(math.erf)
+        for ax, lo, hi in ((ia, self.a0, self.a1), (ib, self.b0, self.b1)):
+            if lo <= 0.0 and hi >= L3[ax]:
+                continue
+            s = P[:, ax]
+            fl = erf((s - lo) / w) if lo > 0.0 else 1.0
+            fh = erf((hi - s) / w) if hi < L3[ax] else 1.0
+            v = v * 0.5 * (fl + fh)
+        return v
+
+
+@dataclass
+class Contact:
+    """A contact on face 0..5 (x-,x+,y-,y+,z-,z+) over fractional in-plane
+    ranges (main.py:640-692).  bc 0 ohmic, 1 Schottky, 2 gate (Robin on
+    semiconductor); pm work function (eV), tox (nm)."""
+    face: int
+    i0p: float
+    i1p: float
+    j0p: float
+    j1p: float
+    bc: int = 0
+    V: float = 0.0
+    pm: float = 4.05
+    tox: float = 10.0
+    nox: float = 0.0
+    label: str = ""
+
+    def to_idx6(self, xs, ys, zs):
+        ax = {0: (ys, zs), 1: (ys, zs), 2: (xs, zs), 3: (xs, zs), 4: (xs, ys), 5: (xs, ys)}[self.face]
+        c1, c2 = np.asarray(ax[0]), np.asarray(ax[1])
+        i0, i1 = span_idx(c1, c1[0] + self.i0p * (c1[-1] - c1[0]), c1[0] + self.i1p * (c1[-1] - c1[0]))
+        j0, j1 = span_idx(c2, c2[0] + self.j0p * (c2[-1] - c2[0]), c2[0] + self.j1p * (c2[-1] - c2[0]))
+        return i0, i1, j0, j1, 0, 0
+
+
+@dataclass
+class DeviceSpec:
+    name: str
+    L3: tuple                      # domain (nm)
+    xs: np.ndarray                 # node coordinates (m)
+    ys: np.ndarray
+    zs: np.ndarray
+    mat: np.ndarray                # per-node index into `materials` (int32, x fastest)
+    Nd: np.ndarray                 # m^-3
+    Na: np.ndarray
+    contacts: list
+    materials: list = field(default_factory=lambda: [MAT_SI])
+    models: int = 7
+    T: float = 300.0
+    solver: tuple = (2000, 1000, 500, 1e-10, 1e-9, 1.0, 1.0)
+    tau: tuple = None              # (tau_n0, tau_p0) of material 0, or None
+    sweep: list = field(default_factory=list)   # [(contact id, V start, V end, npts, {cid: V})]
+
+    @property
+    def shape(self):
+        return len(self.xs), len(self.ys), len(self.zs)
+
+    @property
+    def N(self):
+        return len(self.xs) * len(self.ys) * len(self.zs)
+
+    def bias_points(self):
+        return sum(s[3] for s in self.sweep)
+
+
+def node_positions_nm(xs, ys, zs):
+    Z, Y, X = np.meshgrid(np.asarray(zs) * 1e9, np.asarray(ys) * 1e9, np.asarray(xs) * 1e9, indexing="ij")
+    return np.c_[X.ravel(), Y.ravel(), Z.ravel()]
+
+
+def pn_diode(Nx=16, Ny=16, Nz=64):
+    """Si abrupt p-n diode in a 1x1x4 um box: N_A 1e17 cm^-3 for z < 2 um,
+    N_D 1e17 above; ohmic anode on z = 0, cathode on z = 4 um; SRH tau 1e-7 s;
+    forward sweep 0 -> 0.8 V in 0.05 V steps.  Models 0 (no BGN, no tau(N),
+    no velocity saturation)."""
+    xs = np.linspace(0, 1e-6, Nx)
+    ys = np.linspace(0, 1e-6, Ny)
+    zs = graded_grid(4000.0, Nz, [2000.0])
+    N = Nx * Ny * Nz
+    mat = np.zeros(N, np.int32)
+    Nd = np.zeros(N)
+    Na = np.zeros(N)
+    l0, l1 = span_idx(zs, 0.0, 2e-6)
+    zl = np.repeat(np.arange(Nz), Nx * Ny)
+    Na[(zl >= l0) & (zl <= l1)] = 1e23
+    Nd[zl > l1] = 1e23
+    cons = [Contact(4, 0.0, 1.0, 0.0, 1.0, 0, 0.0, label="Anode"),
+            Contact(5, 0.0, 1.0, 0.0, 1.0, 0, 0.0, label="Cathode")]
+    return DeviceSpec("pn_diode_%dx%dx%d" % (Nx, Ny, Nz), (1000.0, 1000.0, 4000.0), xs, ys, zs, mat, Nd, Na,
+                      cons, models=0, tau=(1e-7, 1e-7), sweep=[(0, 0.0, 0.8, 17, {})])
+
+
+def mosfet(Nx=96, Ny=64, Nz=64, vds=(0.05, 1.0), npts=25):
+    """Planar n-MOSFET, x = channel (0.6 um), y = depth (0.3 um, top y = 0),
+    z = width (0.2 um).  p-substrate 1e17 cm^-3; n+ source/drain Gaussian
+    implants (peak 1e20 cm^-3 at the surface, Lv 40 nm, lateral 0.8 Lv)
+    through windows x < 200 nm and x > 400 nm; a 0.2 um gate over 2 nm SiO2
+    (Robin face gate, work function 4.65 eV: about midgap, V_th ~ 0.5 V);
+    ohmic source and drain on the top face, ohmic body on the bottom.
+    Id-Vg 0 -> 1.2 V (npts points) at every drain voltage in vds."""
+    Lx, Ly, Lz = 600.0, 300.0, 200.0
+    xs = graded_grid(Lx, Nx, [200.0, 400.0])
+    ys = graded_grid(Ly, Ny, [0.0, 40.0])
+    zs = np.linspace(0, Lz * 1e-9, Nz)
+    N = Nx * Ny * Nz
+    mat = np.zeros(N, np.int32)
+    Na = np.full(N, 1e23)
+    P = node_positions_nm(xs, ys, zs)
+    L3 = (Lx, Ly, Lz)
+    nd = Implant("n", 1e20, "y0", 0.0, 200.0, 0.0, 1e9, "gauss", 40.0, 0.0, 0.8).conc(P, L3)
+    nd += Implant("n", 1e20, "y0", 400.0, 1e9, 0.0, 1e9, "gauss", 40.0, 0.0, 0.8).conc(P, L3)
+    Nd = nd * 1e6
+    cons = [Contact(2, 0.0, 1.0 / 3.0, 0.0, 1.0, 0, 0.0, label="Source"),
+            Contact(2, 2.0 / 3.0, 1.0, 0.0, 1.0, 0, vds[0], label="Drain"),
+            Contact(2, 1.0 / 3.0, 2.0 / 3.0, 0.0, 1.0, 2, 0.0, pm=4.65, tox=2.0, label="Gate"),
+            Contact(3, 0.0, 1.0, 0.0, 1.0, 0, 0.0, label="Body")]
+    sweep = [(2, 0.0, 1.2, npts, {1: vd}) for vd in vds]
+    return DeviceSpec("mosfet_%dx%dx%d" % (Nx, Ny, Nz), L3, xs, ys, zs, mat, Nd, Na, cons,
+                      models=7, sweep=sweep)
+
+
+WORKLOADS = {
+    "pn_diode": lambda: pn_diode(),
+    "mosfet_96x64x64": lambda: mosfet(96, 64, 64),
+    "mosfet_192x128x128": lambda: mosfet(192, 128, 128),
+}
+
+
+def load_device(lib, spec, solver=None):
+    """Send the device to an api3 core (main.py:3286-3361, node-data path).
+    Returns the contact ids."""
+    lib.api3_init()
+    mids = [lib.api3_add_material(m, spec.T) for m in spec.materials]
+    if min(mids) < 0:
+        raise ValueError("too many materials")
+    lib.api3_set_solver(*(solver or spec.solver))
+    xs = np.ascontiguousarray(spec.xs, np.float64)
+    ys = np.ascontiguousarray(spec.ys, np.float64)
+    zs = np.ascontiguousarray(spec.zs, np.float64)
+    Nx, Ny, Nz = spec.shape
+    if not lib.api3_set_grid_coords(Nx, Ny, Nz, dptr(xs), dptr(ys), dptr(zs)):
+        raise MemoryError(f"could not allocate {Nx}x{Ny}x{Nz} nodes")
+    lib.api3_set_models(spec.models)
+    lib.api3_set_caxis(1)
+    mat = np.ascontiguousarray(np.asarray(mids, np.int32)[spec.mat], np.int32)
+    Nd = np.ascontiguousarray(spec.Nd, np.float64)
+    Na = np.ascontiguousarray(spec.Na, np.float64)
+    Nf = np.zeros(spec.N)
+    lib.api3_set_node_data(iptr(mat), dptr(Nd), dptr(Na), dptr(Nf))
+    lib.api3_clear_contacts()
+    cids = []
+    for c in spec.contacts:
+        i0, i1, j0, j1, k0, k1 = c.to_idx6(spec.xs, spec.ys, spec.zs)
+        cid = lib.api3_add_contact_ex(c.face, i0, i1, j0, j1, k0, k1, c.bc, c.V, c.pm,
+                                      c.tox * 1e-9, c.nox * 1e4)
+        if cid < 0:
+            raise ValueError("too many contacts")
+        cids.append(cid)
+    if spec.tau is not None:
+        lib.api3_set_mat_tau(mids[0], spec.tau[0], spec.tau[1])
+    return cids
