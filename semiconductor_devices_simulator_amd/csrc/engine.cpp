@@ -349,6 +349,30 @@ int eng_prepare() {
     }
     eoff[G.n_cons] = (int)ek.size();
     G.n_edges = (int)ek.size();
+    // line-smoother axes: the two axes with the largest geometric coupling
+    // sum_edges area/h (weaker of the two first); point red-black GS if asked
+    // for (SEMISIM_MG_SMOOTH=rb) or if a line is longer than the kernel takes
+    {
+        double sx = 0., sy = 0., sz = 0., SLx = 0., SLy = 0., SLz = 0.;
+        for (double v : G.Lx) SLx += v;
+        for (double v : G.Ly) SLy += v;
+        for (double v : G.Lz) SLz += v;
+        for (int i = 0; i + 1 < G.Nx; i++) sx += 1. / (G.xs[i + 1] - G.xs[i]);
+        for (int j = 0; j + 1 < G.Ny; j++) sy += 1. / (G.ys[j + 1] - G.ys[j]);
+        for (int l = 0; l + 1 < G.Nz; l++) sz += 1. / (G.zs[l + 1] - G.zs[l]);
+        double w[3] = {sx * SLy * SLz, sy * SLx * SLz, sz * SLx * SLy};
+        int ord[3] = {0, 1, 2};
+        std::sort(ord, ord + 3, [&](int a, int b) { return w[a] < w[b]; });
+        const char *mode = getenv("SEMISIM_MG_SMOOTH");
+        int want = 2;
+        if (mode && !strcmp(mode, "rb")) want = 0;
+        else if (mode && !strcmp(mode, "lines3")) want = 3;
+        else if (mode && !strcmp(mode, "lines1")) want = 1;
+        int maxlen = std::max(G.Nx, std::max(G.Ny, G.Nz));
+        if (maxlen > 1024) want = 0;
+        G.n_axes = want;
+        for (int a = 0; a < want; a++) G.mg_axes[a] = ord[3 - want + a];
+    }
     // upload
     std::vector<double> geo;
     geo.insert(geo.end(), G.xs.begin(), G.xs.end()); geo.insert(geo.end(), G.ys.begin(), G.ys.end());
@@ -453,6 +477,36 @@ static void mg_setup() {
     G.t_mg += wtime() - t0;
 }
 
+// Smoother: zebra line Gauss-Seidel along the mesh axes with the strongest
+// couplings (G.mg_axes, chosen in prepare), or red-black point Gauss-Seidel
+// (G.n_axes == 0).  Pre-smoothing starts from x = 0; returns the restriction
+// skip code of the rows the last sweep solved exactly.
+template <class T> static int smooth_pre(const MGLevel<T> &v, const T *b, T *x) {
+    hipStream_t s = G.st;
+    if (G.n_axes == 0) {
+        mg_rbgs(s, v, b, x, 0, 1);
+        mg_rbgs(s, v, b, x, 1, 0);
+        return 1;
+    }
+    int first = 1;
+    for (int a = 0; a < G.n_axes; a++)
+        for (int color = 0; color < 2; color++) { mg_line(s, v, b, x, G.mg_axes[a], color, first); first = 0; }
+    return 2 + G.mg_axes[G.n_axes - 1];
+}
+// Post-smoothing: the pre-smoother's sweeps in reverse order (symmetric
+// V-cycle for CG).  With c != nullptr the first sweep applies the coarse
+// correction x += pw * x_parent to the neighbours it reads; the sweeps then
+// overwrite every node, so the correction is never stored.
+template <class T> static void smooth_post(const MGLevel<T> &v, const T *b, T *x, const MGLevel<T> *c) {
+    hipStream_t s = G.st;
+    if (G.n_axes == 0) {
+        mg_rbgs(s, v, b, x, 1, 0, c);
+        mg_rbgs(s, v, b, x, 0, 0);
+        return;
+    }
+    for (int a = G.n_axes - 1; a >= 0; a--)
+        for (int color = 1; color >= 0; color--) { mg_line(s, v, b, x, G.mg_axes[a], color, 0, c); c = nullptr; }
+}
 template <class T> static void vcycle_T(int L, const T *b, T *x) {
     Hier<T> &H = hier<T>();
     hipStream_t s = G.st;
@@ -460,20 +514,16 @@ template <class T> static void vcycle_T(int L, const T *b, T *x) {
     MGLevel<T> &v = H.lev[L];
     if (L == nl - 1) { mg_coarse_solve(s, v, H.inv, b, x); return; }
     MGLevel<T> &c = H.lev[L + 1];
-    mg_rbgs(s, v, b, x, 0, 1);          // pre-smoothing from x = 0: red, black
-    mg_rbgs(s, v, b, x, 1, 0);
-    mg_resid_restrict(s, v, c, b, x);   // coarse right-hand side
+    int skip = smooth_pre(v, b, x);
+    mg_resid_restrict(s, v, c, b, x, skip);   // coarse right-hand side
     vcycle_T<T>(L + 1, c.b, c.x);
-    mg_prolong(s, v, c, x);
-    mg_rbgs(s, v, b, x, 1, 0);          // post-smoothing: black, red (symmetric V-cycle)
-    mg_rbgs(s, v, b, x, 0, 0);
+    smooth_post(v, b, x, &c);
 }
-// smoother only (the reference's ILU-only retry): one symmetric sweep from 0
+// smoother only (the reference's ILU-only retry): one symmetric smoothing from 0
 template <class T> static void smooth_only_T(const T *b, T *x) {
     Hier<T> &H = hier<T>();
-    mg_rbgs(G.st, H.lev[0], b, x, 0, 1);
-    mg_rbgs(G.st, H.lev[0], b, x, 1, 0);
-    mg_rbgs(G.st, H.lev[0], b, x, 0, 0);
+    smooth_pre(H.lev[0], b, x);
+    smooth_post<T>(H.lev[0], b, x, nullptr);
 }
 // z = M^-1 r
 static void precond(const double *r, double *z) {
@@ -510,12 +560,11 @@ static int pcg(const double *b, double *x, int maxit, double rtol) {
     sd_copy(s, p, z, N);
     int it;
     for (it = 1; it <= maxit; it++) {
-        matvec(p, q);
-        sd_dot(s, p, q, N, G.rb, S_PQ);
-        sd_scalar_op(s, SOP_PCG_AL, G.rb);
+        G.n_spmv++;
+        sd_spmv_dots(s, g, G.A, p, q, p, nullptr, 1, G.rb, S_PQ, SOP_PCG_AL);
         sd_pcg_update_xr(s, x, r, p, q, N, G.rb);
         precond(r, z);
-        sd_dot(s, r, z, N, G.rb, S_TMP0);
+        sd_dot(s, r, z, N, G.rb, S_TMP0, SOP_PCG_BE);
         rd(S_RZ, S_TMP0 + 1);
         double pq = S(S_PQ), rzn = S(S_TMP0);
         if (!(pq > 0.)) {
@@ -523,7 +572,6 @@ static int pcg(const double *b, double *x, int maxit, double rtol) {
             break;
         }
         if (!(rzn > rtol * rtol * rz0)) break;
-        sd_scalar_op(s, SOP_PCG_BE, G.rb);
         sd_pcg_update_p(s, p, z, N, G.rb);
     }
     if (G.verbose > 3) fprintf(stderr, "      pcg its=%d rz0=%.3e\n", it, rz0);
@@ -552,22 +600,20 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
     double rbest = r0;
     int stall = 0, it, restarted = 0;
     for (it = 1; it <= maxit; it++) {
-        sd_dot(s, rh, r, N, G.rb, S_RHO1);
+        sd_dot(s, rh, r, N, G.rb, S_RHO1, SOP_BI_BE);
     again:
-        sd_scalar_op(s, SOP_BI_BE, G.rb);
         sd_bi_p(s, p, r, v, N, G.rb);
         precond(p, ph);
-        matvec(ph, v);
-        sd_dot(s, rh, v, N, G.rb, S_RV);
-        sd_scalar_op(s, SOP_BI_AL, G.rb);
+        G.n_spmv++;
+        sd_spmv_dots(s, g, G.A, ph, v, rh, nullptr, 1, G.rb, S_RV, SOP_BI_AL);
         sd_bi_r1(s, r, v, N, G.rb);
         rd(S_AL, S_SS + 1);
         double rho1 = S(S_RHO1), rv = S(S_RV), ss = S(S_SS);
         if (fabs(rho1) < 1e-300 && !restarted) {
             // breakdown: restart the shadow residual (al was ~0, r unchanged)
             sd_bi_restart(s, rh, p, v, r, N);
-            sd_dot(s, r, r, N, G.rb, S_RHO1);
             sd_scalar_op(s, SOP_BI_RESTART, G.rb);
+            sd_dot(s, r, r, N, G.rb, S_RHO1, SOP_BI_BE);
             rd(S_RHO1, S_RHO1 + 1);
             if (!(S(S_RHO1) > 0.)) break;
             restarted = 1;
@@ -581,9 +627,8 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
             break;
         }
         precond(r, sh);
-        matvec(sh, t);
-        sd_dot2(s, t, t, r, N, G.rb, S_TT, S_TS);
-        sd_scalar_op(s, SOP_BI_OM, G.rb);
+        G.n_spmv++;
+        sd_spmv_dots(s, g, G.A, sh, t, nullptr, r, 2, G.rb, S_TT, SOP_BI_OM);
         sd_bi_xr2(s, x, r, ph, sh, t, N, G.rb);
         rd(S_OM, S_RR + 1);
         double om = S(S_OM);
@@ -593,7 +638,6 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
         if (rn <= tgt || om == 0.) break;
         if (rn < 0.99 * rbest) { rbest = rn; stall = 0; }
         else if (++stall > 40) break;
-        sd_scalar_op(s, SOP_BI_NEXT, G.rb);
     }
     if (it > maxit) it = maxit;
     G.bicg_its += it;

@@ -89,6 +89,7 @@ struct Engine {
     const double *mg_w = nullptr;
     int mg_qsign = 0;
     int mg_off = 0;
+    int n_axes = 2, mg_axes[3] = {1, 2, 0};   // line-smoother axes (0: point red-black GS)
 
     // Anderson
     int aa_m = 12, aa_n = 0, aa_pos = 0, aa_have = 0;

@@ -18,7 +18,7 @@
 __global__ void __launch_bounds__(SD_BLOCK) k_dot(const double *a, const double *b, int n, RedBuf rb, int slot) {
     double acc[1] = {0.0};
     for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < n; k += gridDim.x * SD_BLOCK) acc[0] += b ? a[k] * b[k] : a[k];
-    red_finish<1, false>(acc, rb, slot);
+    red_partial<1, false>(acc, rb);
 }
 __global__ void __launch_bounds__(SD_BLOCK) k_dot2(const double *a, const double *b, const double *c, int n,
                                                    RedBuf rb, int slot) {
@@ -28,7 +28,7 @@ __global__ void __launch_bounds__(SD_BLOCK) k_dot2(const double *a, const double
         acc[0] += x * b[k];
         acc[1] += x * c[k];
     }
-    red_finish<2, false>(acc, rb, slot);
+    red_partial<2, false>(acc, rb);
 }
 __global__ void __launch_bounds__(SD_BLOCK) k_maxabs(const double *a, const double *b, int n, RedBuf rb, int slot) {
     double acc[1] = {0.0};
@@ -36,23 +36,31 @@ __global__ void __launch_bounds__(SD_BLOCK) k_maxabs(const double *a, const doub
         double x = b ? fabs(a[k] - b[k]) : fabs(a[k]);
         acc[0] = sd_nanmax(acc[0], x);
     }
-    red_finish<1, true>(acc, rb, slot);
+    red_partial<1, true>(acc, rb);
 }
 
-void sd_dot(hipStream_t s, const double *a, const double *b, int n, RedBuf rb, int slot) {
-    k_dot<<<red_grid(n), SD_BLOCK, 0, s>>>(a, b, n, rb, slot);
+void sd_dot(hipStream_t s, const double *a, const double *b, int n, RedBuf rb, int slot, int post) {
+    int nb = red_grid(n);
+    k_dot<<<nb, SD_BLOCK, 0, s>>>(a, b, n, rb, slot);
+    red_final(s, rb, 1, nb, slot, 0, post);
 }
-void sd_dot2(hipStream_t s, const double *a, const double *b, const double *c, int n, RedBuf rb, int sab, int sac) {
+void sd_dot2(hipStream_t s, const double *a, const double *b, const double *c, int n, RedBuf rb, int sab, int sac,
+             int post) {
     // slots must be adjacent: sac == sab + 1
     (void)sac;
-    k_dot2<<<red_grid(n), SD_BLOCK, 0, s>>>(a, b, c, n, rb, sab);
+    int nb = red_grid(n);
+    k_dot2<<<nb, SD_BLOCK, 0, s>>>(a, b, c, n, rb, sab);
+    red_final(s, rb, 2, nb, sab, 0, post);
 }
-void sd_nrm2sq(hipStream_t s, const double *a, int n, RedBuf rb, int slot) { sd_dot(s, a, a, n, rb, slot); }
 void sd_maxabs(hipStream_t s, const double *a, int n, RedBuf rb, int slot) {
-    k_maxabs<<<red_grid(n), SD_BLOCK, 0, s>>>(a, nullptr, n, rb, slot);
+    int nb = red_grid(n);
+    k_maxabs<<<nb, SD_BLOCK, 0, s>>>(a, nullptr, n, rb, slot);
+    red_final(s, rb, 1, nb, slot, 1);
 }
 void sd_maxabsdiff(hipStream_t s, const double *a, const double *b, int n, RedBuf rb, int slot) {
-    k_maxabs<<<red_grid(n), SD_BLOCK, 0, s>>>(a, b, n, rb, slot);
+    int nb = red_grid(n);
+    k_maxabs<<<nb, SD_BLOCK, 0, s>>>(a, b, n, rb, slot);
+    red_final(s, rb, 1, nb, slot, 1);
 }
 
 // -------------------------------------------------------- vector kernels
@@ -109,6 +117,29 @@ template <class T> void sd_spmv_T(hipStream_t s, const Geo &g, Sten7<T> A, const
 template void sd_spmv_T<double>(hipStream_t, const Geo &, Sten7<double>, const double *, const double *, double *);
 template void sd_spmv_T<float>(hipStream_t, const Geo &, Sten7<float>, const float *, const float *, float *);
 
+// y = A x fused with the dots y.(u0 or y) and y.u1 (Krylov: q = A p with p.q;
+// v = A ph with rh.v; t = A sh with t.t and t.r)
+__global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, Sten7<double> A, const double *x,
+                                                        double *y, const double *u0, const double *u1, int nv,
+                                                        RedBuf rb) {
+    int N = Nx * Ny * Nz, NxNy = Nx * Ny;
+    double acc[2] = {0.0, 0.0};
+    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < N; k += gridDim.x * SD_BLOCK) {
+        int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
+        double s = sten_apply<double, double>(A, x, k, i, j, l, Nx, Ny, Nz, false);
+        y[k] = s;
+        acc[0] += s * (u0 ? u0[k] : s);
+        if (nv > 1) acc[1] += s * u1[k];
+    }
+    red_partial<2, false>(acc, rb);
+}
+void sd_spmv_dots(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, double *y, const double *u0,
+                  const double *u1, int nv, RedBuf rb, int slot, int post) {
+    int nb = red_grid(g.N);
+    k_spmv_dots<<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb);
+    red_final(s, rb, nv, nb, slot, 0, post);
+}
+
 // rounding floor of a continuity solve, semiconductor_3d.c:2243-2265
 __global__ void __launch_bounds__(SD_BLOCK) k_floor(int Nx, int Ny, int Nz, Sten7<double> A, const double *x,
                                                     const double *b, RedBuf rb, int slot) {
@@ -125,15 +156,16 @@ __global__ void __launch_bounds__(SD_BLOCK) k_floor(int Nx, int Ny, int Nz, Sten
         if (l < Nz - 1) s += fabs(A.zp[k] * x[k + NxNy]);
         acc[0] += s * s;
     }
-    red_finish<1, false>(acc, rb, slot);
+    red_partial<1, false>(acc, rb);
 }
 void sd_floor(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, const double *b, RedBuf rb, int slot) {
-    k_floor<<<red_grid(g.N), SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, b, rb, slot);
+    int nb = red_grid(g.N);
+    k_floor<<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, b, rb, slot);
+    red_final(s, rb, 1, nb, slot, 0);
 }
 
 // ------------------------------------------------------- Krylov updates
-__global__ void k_scalar_op(int op, double *sc) {
-    if (threadIdx.x != 0) return;
+__device__ __forceinline__ void scalar_op(int op, double *sc) {
     switch (op) {
     case SOP_PCG_AL: sc[S_AL] = (sc[S_PQ] > 0.) ? sc[S_RZ] / sc[S_PQ] : 0.0; break;
     case SOP_PCG_BE: sc[S_BE] = sc[S_TMP0] / sc[S_RZ]; sc[S_RZ] = sc[S_TMP0]; break;
@@ -144,7 +176,35 @@ __global__ void k_scalar_op(int op, double *sc) {
     case SOP_BI_RESTART: sc[S_RHO] = sc[S_AL] = sc[S_OM] = 1.0; break;
     }
 }
+__global__ void k_scalar_op(int op, double *sc) {
+    if (threadIdx.x == 0) scalar_op(op, sc);
+}
 void sd_scalar_op(hipStream_t s, int op, RedBuf rb) { k_scalar_op<<<1, 64, 0, s>>>(op, rb.scal); }
+
+__global__ void __launch_bounds__(SD_BLOCK) k_red_final(RedBuf rb, int nv, int nb, int slot0, int is_max, int post) {
+    __shared__ double lds[SD_BLOCK / 64];
+    int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    for (int v = 0; v < nv; v++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += SD_BLOCK) {
+            double x = rb.part[v * SD_RED_MAXB + b];
+            s = is_max ? sd_nanmax(s, x) : s + x;
+        }
+        s = is_max ? sd_wave_max(s) : sd_wave_sum(s);
+        if (ln == 0) lds[w] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = lds[0];
+            for (int q = 1; q < SD_BLOCK / 64; q++) r = is_max ? sd_nanmax(r, lds[q]) : r + lds[q];
+            rb.scal[slot0 + v] = r;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && post >= 0) scalar_op(post, rb.scal);
+}
+void red_final(hipStream_t s, RedBuf rb, int nv, int nb, int slot0, int is_max, int post) {
+    k_red_final<<<1, SD_BLOCK, 0, s>>>(rb, nv, nb, slot0, is_max, post);
+}
 
 __global__ void k_pcg_xr(double *x, double *r, const double *p, const double *q, int n, const double *sc) {
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
@@ -179,7 +239,7 @@ __global__ void __launch_bounds__(SD_BLOCK) k_bi_r1(double *r, const double *v, 
         r[k] = x;
         acc[0] += x * x;
     }
-    red_finish<1, false>(acc, rb, S_SS);
+    red_partial<1, false>(acc, rb);
 }
 __global__ void k_bi_x1(double *x, const double *ph, int n, const double *sc) {
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
@@ -195,7 +255,7 @@ __global__ void __launch_bounds__(SD_BLOCK) k_bi_xr2(double *x, double *r, const
         r[k] = y;
         acc[0] += y * y;
     }
-    red_finish<1, false>(acc, rb, S_RR);
+    red_partial<1, false>(acc, rb);
 }
 __global__ void __launch_bounds__(SD_BLOCK) k_bi_init(double *r, double *rh, double *p, double *v, const double *b,
                                                       const double *t, int n, RedBuf rb) {
@@ -205,7 +265,7 @@ __global__ void __launch_bounds__(SD_BLOCK) k_bi_init(double *r, double *rh, dou
         r[k] = y; rh[k] = y; p[k] = 0.; v[k] = 0.;
         acc[0] += y * y;
     }
-    red_finish<1, false>(acc, rb, S_RR);
+    red_partial<1, false>(acc, rb);
 }
 __global__ void k_bi_restart(double *rh, double *p, double *v, const double *r, int n) {
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
@@ -215,18 +275,24 @@ void sd_bi_p(hipStream_t s, double *p, const double *r, const double *v, int n, 
     k_bi_p<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(p, r, v, n, rb.scal);
 }
 void sd_bi_r1(hipStream_t s, double *r, const double *v, int n, RedBuf rb) {
-    k_bi_r1<<<red_grid(n), SD_BLOCK, 0, s>>>(r, v, n, rb);
+    int nb = red_grid(n);
+    k_bi_r1<<<nb, SD_BLOCK, 0, s>>>(r, v, n, rb);
+    red_final(s, rb, 1, nb, S_SS, 0);
 }
 void sd_bi_x1(hipStream_t s, double *x, const double *ph, int n, RedBuf rb) {
     k_bi_x1<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(x, ph, n, rb.scal);
 }
 void sd_bi_xr2(hipStream_t s, double *x, double *r, const double *ph, const double *sh, const double *t, int n,
                RedBuf rb) {
-    k_bi_xr2<<<red_grid(n), SD_BLOCK, 0, s>>>(x, r, ph, sh, t, n, rb);
+    int nb = red_grid(n);
+    k_bi_xr2<<<nb, SD_BLOCK, 0, s>>>(x, r, ph, sh, t, n, rb);
+    red_final(s, rb, 1, nb, S_RR, 0, SOP_BI_NEXT);
 }
 void sd_bi_init(hipStream_t s, double *r, double *rh, double *p, double *v, const double *b, const double *t, int n,
                 RedBuf rb) {
-    k_bi_init<<<red_grid(n), SD_BLOCK, 0, s>>>(r, rh, p, v, b, t, n, rb);
+    int nb = red_grid(n);
+    k_bi_init<<<nb, SD_BLOCK, 0, s>>>(r, rh, p, v, b, t, n, rb);
+    red_final(s, rb, 1, nb, S_RR, 0);
 }
 void sd_bi_restart(hipStream_t s, double *rh, double *p, double *v, const double *r, int n) {
     k_bi_restart<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(rh, p, v, r, n);
@@ -332,7 +398,8 @@ template <class T> void mg_galerkin(hipStream_t s, const MGLevel<T> &f, MGLevel<
 // ((i+j+l) & 1 == color).  zero != 0: the sweep starts from x = 0 (every node
 // is written: the colour's nodes get b/d, the others 0).
 template <class T>
-__global__ void __launch_bounds__(SD_BLOCK) k_rbgs(MGLevel<T> v, const T *b, T *x, int color, int zero) {
+__global__ void __launch_bounds__(SD_BLOCK) k_rbgs(MGLevel<T> v, const T *b, T *x, int color, int zero,
+                                                 MGLevel<T> c, int pro) {
     const int Nx = v.Nx, Ny = v.Ny, Nz = v.Nz, NxNy = Nx * Ny;
     if (zero) {
         int k = blockIdx.x * SD_BLOCK + threadIdx.x;
@@ -352,29 +419,164 @@ __global__ void __launch_bounds__(SD_BLOCK) k_rbgs(MGLevel<T> v, const T *b, T *
     if (i >= Nx) return;
     int k = i + Nx * (j + Ny * l);
     T s = b[k];
-    if (i > 0) s -= v.A.xm[k] * x[k - 1];
-    if (i < Nx - 1) s -= v.A.xp[k] * x[k + 1];
-    if (j > 0) s -= v.A.ym[k] * x[k - Nx];
-    if (j < Ny - 1) s -= v.A.yp[k] * x[k + Nx];
-    if (l > 0) s -= v.A.zm[k] * x[k - NxNy];
-    if (l < Nz - 1) s -= v.A.zp[k] * x[k + NxNy];
+    if (pro) {
+        // neighbours carry the coarse correction x += pw * x_parent, applied on the fly
+        auto xc = [&](int ii, int jj, int ll, int kk) -> T {
+            int I = v.cx ? ii >> 1 : ii, J = v.cy ? jj >> 1 : jj, L = v.cz ? ll >> 1 : ll;
+            return x[kk] + v.pw[kk] * c.x[I + c.Nx * (J + c.Ny * L)];
+        };
+        if (i > 0) s -= v.A.xm[k] * xc(i - 1, j, l, k - 1);
+        if (i < Nx - 1) s -= v.A.xp[k] * xc(i + 1, j, l, k + 1);
+        if (j > 0) s -= v.A.ym[k] * xc(i, j - 1, l, k - Nx);
+        if (j < Ny - 1) s -= v.A.yp[k] * xc(i, j + 1, l, k + Nx);
+        if (l > 0) s -= v.A.zm[k] * xc(i, j, l - 1, k - NxNy);
+        if (l < Nz - 1) s -= v.A.zp[k] * xc(i, j, l + 1, k + NxNy);
+    } else {
+        if (i > 0) s -= v.A.xm[k] * x[k - 1];
+        if (i < Nx - 1) s -= v.A.xp[k] * x[k + 1];
+        if (j > 0) s -= v.A.ym[k] * x[k - Nx];
+        if (j < Ny - 1) s -= v.A.yp[k] * x[k + Nx];
+        if (l > 0) s -= v.A.zm[k] * x[k - NxNy];
+        if (l < Nz - 1) s -= v.A.zp[k] * x[k + NxNy];
+    }
     T d = v.A.c ? v.A.c[k] : (T)1;
     if (!(d != (T)0)) d = (T)1;
     x[k] = s / d;
 }
-template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int color, int zero) {
-    if (zero) k_rbgs<T><<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(v, b, x, color, 1);
+template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int color, int zero,
+                                 const MGLevel<T> *c) {
+    MGLevel<T> cc = c ? *c : v;
+    if (zero) k_rbgs<T><<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(v, b, x, color, 1, cc, 0);
     else {
         int n = ((v.Nx + 1) >> 1) * v.Ny * v.Nz;
-        k_rbgs<T><<<sd_nblocks(n), SD_BLOCK, 0, s>>>(v, b, x, color, 0);
+        k_rbgs<T><<<sd_nblocks(n), SD_BLOCK, 0, s>>>(v, b, x, color, 0, cc, c != nullptr);
     }
+}
+
+// Zebra line Gauss-Seidel: every line along axis `ax` whose two other indices
+// have parity `color` is solved exactly (tridiagonal in-line couplings, the
+// off-line neighbours taken from x).  One workgroup owns NL lines of one
+// colour; the tridiagonal systems are solved by parallel cyclic reduction in
+// LDS (log2 n steps, one lane per equation).  zero: the sweep starts from
+// x = 0 (off-line terms vanish).  pro: the off-line neighbours carry the coarse
+// correction x + pw * x_parent (first post-smoothing sweep).
+#define LINE_MAXE 4
+#define LINE_MAXEQ (SD_BLOCK * LINE_MAXE)
+template <class T>
+__global__ void __launch_bounds__(SD_BLOCK) k_line_gs(MGLevel<T> v, const T *b, T *x, int ax, int color, int zero,
+                                                      MGLevel<T> c, int pro, int NL) {
+    __shared__ T sa[LINE_MAXEQ], sb[LINE_MAXEQ], sc[LINE_MAXEQ], sd[LINE_MAXEQ];
+    const int Nx = v.Nx, Ny = v.Ny, Nz = v.Nz, NxNy = Nx * Ny;
+    int n, Ntd, s_ax, s_td, s_3;
+    if (ax == 0) { n = Nx; Ntd = Ny; s_ax = 1; s_td = Nx; s_3 = NxNy; }
+    else if (ax == 1) { n = Ny; Ntd = Nx; s_ax = Nx; s_td = 1; s_3 = NxNy; }
+    else { n = Nz; Ntd = Nx; s_ax = NxNy; s_td = 1; s_3 = Nx; }
+    const int ntile = (Ntd + 2 * NL - 1) / (2 * NL);
+    const int p3 = blockIdx.x / ntile, t0 = (blockIdx.x - p3 * ntile) * 2 * NL;
+    const int sh = (t0 + p3 + color) & 1;
+    const int neq = NL * n;
+    T ra[LINE_MAXE], rb_[LINE_MAXE], rc[LINE_MAXE], rd[LINE_MAXE];
+    for (int m = 0; m < LINE_MAXE; m++) {
+        int e = threadIdx.x + SD_BLOCK * m;
+        if (e >= neq) break;
+        int q = e / NL, L = e - q * NL, td = t0 + 2 * L + sh;
+        T A_ = 0, B_ = 1, C_ = 0, D_ = 0;
+        if (td < Ntd) {
+            int k = td * s_td + q * s_ax + p3 * s_3;
+            int i, j, l;
+            if (ax == 0) { i = q; j = td; l = p3; } else if (ax == 1) { i = td; j = q; l = p3; } else { i = td; j = p3; l = q; }
+            T dg = v.A.c ? v.A.c[k] : (T)1;
+            if (!(dg != (T)0)) dg = (T)1;
+            B_ = dg;
+            const T *am = ax == 0 ? v.A.xm : (ax == 1 ? v.A.ym : v.A.zm);
+            const T *ap = ax == 0 ? v.A.xp : (ax == 1 ? v.A.yp : v.A.zp);
+            if (q > 0) A_ = am[k];
+            if (q < n - 1) C_ = ap[k];
+            T s = b[k];
+            if (!zero) {
+                auto xv = [&](int ii, int jj, int ll, int kk) -> T {
+                    if (!pro) return x[kk];
+                    int I = v.cx ? ii >> 1 : ii, J = v.cy ? jj >> 1 : jj, LL = v.cz ? ll >> 1 : ll;
+                    return x[kk] + v.pw[kk] * c.x[I + c.Nx * (J + c.Ny * LL)];
+                };
+                if (ax != 0) {
+                    if (i > 0) s -= v.A.xm[k] * xv(i - 1, j, l, k - 1);
+                    if (i < Nx - 1) s -= v.A.xp[k] * xv(i + 1, j, l, k + 1);
+                }
+                if (ax != 1) {
+                    if (j > 0) s -= v.A.ym[k] * xv(i, j - 1, l, k - Nx);
+                    if (j < Ny - 1) s -= v.A.yp[k] * xv(i, j + 1, l, k + Nx);
+                }
+                if (ax != 2) {
+                    if (l > 0) s -= v.A.zm[k] * xv(i, j, l - 1, k - NxNy);
+                    if (l < Nz - 1) s -= v.A.zp[k] * xv(i, j, l + 1, k + NxNy);
+                }
+            }
+            D_ = s;
+        }
+        sa[e] = A_; sb[e] = B_; sc[e] = C_; sd[e] = D_;
+    }
+    __syncthreads();
+    for (int st = 1; st < n; st <<= 1) {
+        for (int m = 0; m < LINE_MAXE; m++) {
+            int e = threadIdx.x + SD_BLOCK * m;
+            if (e >= neq) break;
+            int q = e / NL, L = e - q * NL;
+            T a0 = sa[e], b0 = sb[e], c0 = sc[e], d0 = sd[e];
+            T na = 0, nc = 0, nb = b0, nd = d0;
+            if (q - st >= 0) {
+                int em = (q - st) * NL + L;
+                T k1 = a0 / sb[em];
+                na = -sa[em] * k1;
+                nb -= sc[em] * k1;
+                nd -= sd[em] * k1;
+            }
+            if (q + st < n) {
+                int ep = (q + st) * NL + L;
+                T k2 = c0 / sb[ep];
+                nc = -sc[ep] * k2;
+                nb -= sa[ep] * k2;
+                nd -= sd[ep] * k2;
+            }
+            ra[m] = na; rb_[m] = nb; rc[m] = nc; rd[m] = nd;
+        }
+        __syncthreads();
+        for (int m = 0; m < LINE_MAXE; m++) {
+            int e = threadIdx.x + SD_BLOCK * m;
+            if (e >= neq) break;
+            sa[e] = ra[m]; sb[e] = rb_[m]; sc[e] = rc[m]; sd[e] = rd[m];
+        }
+        __syncthreads();
+    }
+    for (int m = 0; m < LINE_MAXE; m++) {
+        int e = threadIdx.x + SD_BLOCK * m;
+        if (e >= neq) break;
+        int q = e / NL, L = e - q * NL, td = t0 + 2 * L + sh;
+        if (td < Ntd) x[td * s_td + q * s_ax + p3 * s_3] = sd[e] / sb[e];
+    }
+}
+template <class T>
+int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero, const MGLevel<T> *c) {
+    int n = ax == 0 ? v.Nx : (ax == 1 ? v.Ny : v.Nz);
+    int Ntd = ax == 0 ? v.Ny : v.Nx, N3 = ax == 2 ? v.Ny : v.Nz;
+    if (n > LINE_MAXEQ) return 0;
+    int NL = LINE_MAXEQ / n;
+    if (NL > 8) NL = 8;
+    int half = (Ntd + 1) / 2;
+    if (NL > half) NL = half;
+    if (NL < 1) NL = 1;
+    int ntile = (Ntd + 2 * NL - 1) / (2 * NL);
+    MGLevel<T> cc = c ? *c : v;
+    k_line_gs<T><<<ntile * N3, SD_BLOCK, 0, s>>>(v, b, x, ax, color, zero, cc, c != nullptr, NL);
+    return 1;
 }
 
 // Residual of the fine level summed over each aggregate with the fine row
 // scale (restriction R = P^T W, semiconductor_3d.c:2028-2037), divided by the
 // coarse row scale (the coarse rows are stored with unit diagonal).
 template <class T>
-__global__ void __launch_bounds__(SD_BLOCK) k_resid_restrict(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x) {
+__global__ void __launch_bounds__(SD_BLOCK) k_resid_restrict(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x,
+                                                             int skip) {
     int K = blockIdx.x * SD_BLOCK + threadIdx.x;
     if (K >= c.N) return;
     int cNxNy = c.Nx * c.Ny, LL = K / cNxNy, rr = K - LL * cNxNy, J = rr / c.Nx, I = rr - J * c.Nx;
@@ -383,6 +585,13 @@ __global__ void __launch_bounds__(SD_BLOCK) k_resid_restrict(MGLevel<T> f, MGLev
     for (int l = a.l0; l <= a.l1; l++)
         for (int j = a.j0; j <= a.j1; j++)
             for (int i = a.i0; i <= a.i1; i++) {
+                // rows solved exactly by the last smoothing sweep have no residual:
+                // skip 1 = point black nodes, 2+a = colour-1 lines along axis a
+                if (skip == 1 && ((i + j + l) & 1)) continue;
+                if (skip >= 2) {
+                    int a = skip - 2, par = a == 0 ? (j + l) : (a == 1 ? (i + l) : (i + j));
+                    if (par & 1) continue;
+                }
                 int k = i + f.Nx * (j + f.Ny * l);
                 double w = f.rs ? f.rs[k] : 1.0;
                 if (w == 0.0) continue;
@@ -391,8 +600,9 @@ __global__ void __launch_bounds__(SD_BLOCK) k_resid_restrict(MGLevel<T> f, MGLev
             }
     c.b[K] = (T)(acc / c.rs[K]);
 }
-template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x) {
-    k_resid_restrict<T><<<sd_nblocks(c.N), SD_BLOCK, 0, s>>>(f, c, b, x);
+template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x,
+                                          int skip) {
+    k_resid_restrict<T><<<sd_nblocks(c.N), SD_BLOCK, 0, s>>>(f, c, b, x, skip);
 }
 
 // x += pw * x_parent (semiconductor_3d.c:2038-2048, MG_ALPHA = 1)
@@ -497,8 +707,9 @@ void mg_sten_to_float(hipStream_t s, Sten7<double> A, Sten7<float> B, int n) {
 #define INST(T)                                                                                                   \
     template void mg_weights_level<T>(hipStream_t, const MGLevel<T> &, MGLevel<T> &, int, const DevView &, int, double); \
     template void mg_galerkin<T>(hipStream_t, const MGLevel<T> &, MGLevel<T> &);                                  \
-    template void mg_rbgs<T>(hipStream_t, const MGLevel<T> &, const T *, T *, int, int);                         \
-    template void mg_resid_restrict<T>(hipStream_t, const MGLevel<T> &, const MGLevel<T> &, const T *, const T *); \
+    template void mg_rbgs<T>(hipStream_t, const MGLevel<T> &, const T *, T *, int, int, const MGLevel<T> *);                         \
+    template void mg_resid_restrict<T>(hipStream_t, const MGLevel<T> &, const MGLevel<T> &, const T *, const T *, int); \
+    template int mg_line<T>(hipStream_t, const MGLevel<T> &, const T *, T *, int, int, int, const MGLevel<T> *); \
     template void mg_prolong<T>(hipStream_t, const MGLevel<T> &, const MGLevel<T> &, T *);                       \
     template void mg_coarse_factor<T>(hipStream_t, const MGLevel<T> &, double *);                                \
     template void mg_coarse_solve<T>(hipStream_t, const MGLevel<T> &, const double *, const T *, T *);           \

@@ -70,11 +70,13 @@ __global__ void __launch_bounds__(SD_BLOCK) k_poisson(DevView d, const double *p
         double r = f / s;
         acc[0] += r * r;
     }
-    red_finish<1, false>(acc, rb, slot);
+    red_partial<1, false>(acc, rb);
 }
 void ph_poisson(hipStream_t s, const DevView &d, const double *phi, int build, double *F, Sten7<double> A,
                 double *sc, RedBuf rb, int slot) {
-    k_poisson<<<red_grid(d.g.N), SD_BLOCK, 0, s>>>(d, phi, build, F, A, sc, rb, slot);
+    int nb = red_grid(d.g.N);
+    k_poisson<<<nb, SD_BLOCK, 0, s>>>(d, phi, build, F, A, sc, rb, slot);
+    red_final(s, rb, 1, nb, slot, 0);
 }
 
 __global__ void k_update_psi(int N, const double *phi, const double *dx, double lam, double *out) {
@@ -219,11 +221,13 @@ __global__ void __launch_bounds__(SD_BLOCK) k_cont_update(DevView d, int hole, c
             acc[0] = sd_nanmax(acc[0], a);
         }
     }
-    red_finish<1, true>(acc, rb, slot);
+    red_partial<1, true>(acc, rb);
 }
 void ph_cont_update(hipStream_t s, const DevView &d, int hole, const double *sol, const double *sc, double *dq,
                     double lo, double hi, RedBuf rb, int slot) {
-    k_cont_update<<<red_grid(d.g.N), SD_BLOCK, 0, s>>>(d, hole, sol, sc, dq, lo, hi, rb, slot);
+    int nb = red_grid(d.g.N);
+    k_cont_update<<<nb, SD_BLOCK, 0, s>>>(d, hole, sol, sc, dq, lo, hi, rb, slot);
+    red_final(s, rb, 1, nb, slot, 1);
 }
 
 __global__ void k_clamp_qf(DevView d, double lo, double hi) {
@@ -426,10 +430,12 @@ void ph_maj_propagate(hipStream_t s, const DevView &d, unsigned char *flag, int 
 __global__ void __launch_bounds__(SD_BLOCK) k_maj_count(int N, const unsigned char *flag, RedBuf rb, int slot) {
     double acc[1] = {0.0};
     for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < N; k += gridDim.x * SD_BLOCK) acc[0] += flag[k] == 1;
-    red_finish<1, false>(acc, rb, slot);
+    red_partial<1, false>(acc, rb);
 }
 void ph_maj_count(hipStream_t s, int N, const unsigned char *flag, RedBuf rb, int slot) {
-    k_maj_count<<<red_grid(N), SD_BLOCK, 0, s>>>(N, flag, rb, slot);
+    int nb = red_grid(N);
+    k_maj_count<<<nb, SD_BLOCK, 0, s>>>(N, flag, rb, slot);
+    red_final(s, rb, 1, nb, slot, 0);
 }
 
 // ------------------------------------------------------------- Anderson
@@ -474,13 +480,15 @@ __global__ void __launch_bounds__(SD_BLOCK) k_aa_dots(int L, VecList vl, int nv,
         for (int q = 0; q < 12; q++)
             if (q < nv) acc[q] += (double)vl.v[q][k] * yk;
     }
-    red_finish<12, false>(acc, rb, slot0);
+    red_partial<12, false>(acc, rb);
 }
 void ph_aa_dots(hipStream_t s, int L, float *const *vecs, int nv, const float *y, const double *yd, RedBuf rb,
                 int slot0) {
     VecList vl;
     for (int q = 0; q < 12; q++) vl.v[q] = q < nv ? vecs[q] : nullptr;
-    k_aa_dots<<<red_grid(L), SD_BLOCK, 0, s>>>(L, vl, nv, y, yd, rb, slot0);
+    int nb = red_grid(L);
+    k_aa_dots<<<nb, SD_BLOCK, 0, s>>>(L, vl, nv, y, yd, rb, slot0);
+    red_final(s, rb, nv, nb, slot0, 0);
 }
 struct GamList { float *v[12]; double gam[12]; };
 __global__ void k_aa_mix(DevView d, double *g0, double *g1, double *g2, GamList gl, int m) {

@@ -36,19 +36,20 @@ template <class T> struct MGLevel {
 };
 
 // vector / reduction launchers (stream-ordered, no host sync)
-void sd_dot(hipStream_t s, const double *a, const double *b, int n, RedBuf rb, int slot);
+void sd_dot(hipStream_t s, const double *a, const double *b, int n, RedBuf rb, int slot, int post = -1);
 void sd_dot2(hipStream_t s, const double *a, const double *b, const double *c, int n, RedBuf rb,
-             int slot_ab, int slot_ac);          // a.b and a.c
-void sd_nrm2sq(hipStream_t s, const double *a, int n, RedBuf rb, int slot);
+             int slot_ab, int slot_ac, int post = -1);   // a.b and a.c
 void sd_maxabs(hipStream_t s, const double *a, int n, RedBuf rb, int slot);
 void sd_maxabsdiff(hipStream_t s, const double *a, const double *b, int n, RedBuf rb, int slot);
-void sd_reduce_raw(hipStream_t s, int nblocks, int nv, RedBuf rb, int slot0, int is_max);
 void sd_fill(hipStream_t s, double *a, double v, int n);
 void sd_copy(hipStream_t s, double *dst, const double *src, int n);
 void sd_axpy(hipStream_t s, double *y, const double *x, double a, int n);     // y += a x
 
 // y = A x, or y = b - A x (b != nullptr), FP64 level-0 operator
 void sd_spmv(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, const double *b, double *y);
+// y = A x fused with dots: slot <- y.(u0 or y), slot+1 <- y.u1 (nv = 1 or 2); post: SOP_*
+void sd_spmv_dots(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, double *y, const double *u0,
+                  const double *u1, int nv, RedBuf rb, int slot, int post = -1);
 // bicgstab rounding floor sum_k (sum_j |A_kj x_j| + |b_k|)^2 -> slot
 void sd_floor(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, const double *b,
               RedBuf rb, int slot);
@@ -71,9 +72,13 @@ void sd_bi_restart(hipStream_t s, double *rh, double *p, double *v, const double
 template <class T> void mg_weights_level(hipStream_t s, const MGLevel<T> &f, MGLevel<T> &c, int L,
                                          const DevView &d, int qsign, double VT);
 template <class T> void mg_galerkin(hipStream_t s, const MGLevel<T> &f, MGLevel<T> &c);
-template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int color, int zero);
+template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int color, int zero,
+                                const MGLevel<T> *c = nullptr);
 template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c,
-                                          const T *b, const T *x);
+                                          const T *b, const T *x, int skip);
+// zebra line Gauss-Seidel along axis ax on lines of one colour (0 = cannot: line too long)
+template <class T> int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero,
+                               const MGLevel<T> *c = nullptr);
 template <class T> void mg_prolong(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c, T *x);
 template <class T> void mg_coarse_factor(hipStream_t s, const MGLevel<T> &v, double *inv);
 template <class T> void mg_coarse_solve(hipStream_t s, const MGLevel<T> &v, const double *inv, const T *b, T *x);

@@ -2,49 +2,28 @@
 #pragma once
 #include "sd_linalg.h"
 
-// Last-block-done reduction: block partials -> rb.part, the last block to
-// arrive sums them in index order (bit-reproducible for a fixed grid).
+// Two-stage deterministic reduction: every block writes its partial (NV
+// values) to rb.part; a one-workgroup finalize kernel (red_final) sums them
+// in index order and can apply a scalar update of the Krylov coefficients.
+// No device-scope fences or atomics: the kernel boundary orders the stages.
 template <int NV, bool MAXR>
-__device__ __forceinline__ void red_finish(double (&acc)[NV], RedBuf rb, int slot0) {
+__device__ __forceinline__ void red_partial(double (&acc)[NV], RedBuf rb) {
     __shared__ double lds[NV][SD_BLOCK / 64];
-    __shared__ int amlast;
     int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
     for (int v = 0; v < NV; v++) {
         double x = MAXR ? sd_wave_max(acc[v]) : sd_wave_sum(acc[v]);
         if (ln == 0) lds[v][w] = x;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int v = 0; v < NV; v++) {
-            double s = lds[v][0];
-            for (int q = 1; q < SD_BLOCK / 64; q++) s = MAXR ? sd_nanmax(s, lds[v][q]) : s + lds[v][q];
-            rb.part[v * SD_RED_MAXB + blockIdx.x] = s;
-        }
-        __threadfence();
-        unsigned t = __hip_atomic_fetch_add(rb.cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        amlast = (t == gridDim.x - 1);
+    if (threadIdx.x < NV) {
+        int v = threadIdx.x;
+        double s = lds[v][0];
+        for (int q = 1; q < SD_BLOCK / 64; q++) s = MAXR ? sd_nanmax(s, lds[v][q]) : s + lds[v][q];
+        rb.part[v * SD_RED_MAXB + blockIdx.x] = s;
     }
-    __syncthreads();
-    if (!amlast) return;
-    __threadfence();
-    for (int v = 0; v < NV; v++) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += SD_BLOCK) {
-            double x = __hip_atomic_load(&rb.part[v * SD_RED_MAXB + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s = MAXR ? sd_nanmax(s, x) : s + x;
-        }
-        s = MAXR ? sd_wave_max(s) : sd_wave_sum(s);
-        __syncthreads();
-        if (ln == 0) lds[v][w] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double r = lds[v][0];
-            for (int q = 1; q < SD_BLOCK / 64; q++) r = MAXR ? sd_nanmax(r, lds[v][q]) : r + lds[v][q];
-            rb.scal[slot0 + v] = r;
-        }
-    }
-    if (threadIdx.x == 0) __hip_atomic_store(rb.cnt, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
+// finalize nv values over nb blocks into rb.scal[slot0..]; then scalar op `post` (SOP_*, -1 none)
+void red_final(hipStream_t s, RedBuf rb, int nv, int nb, int slot0, int is_max, int post = -1);
 
 static inline int red_grid(int n) {
     int nb = sd_nblocks(n);
