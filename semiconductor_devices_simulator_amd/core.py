@@ -84,6 +84,23 @@ EXT_SIGNATURES = [
     ("sdx_device_name", I, [ctypes.c_char_p, I]),
     ("sdx_kernel_stats", I, [DP, I]),
     ("sdx_reset_stats", None, []),
+    ("sdx_test_prepare", I, []),
+    ("sdx_test_set_state", None, [DP, DP, DP]),
+    ("sdx_test_get_node", None, [I, DP]),
+    ("sdx_test_poisson", D, [I, DP, DP, DP, DP]),
+    ("sdx_test_upd_mob", None, [DP]),
+    ("sdx_test_set_fmu", None, [DP]),
+    ("sdx_test_cont", None, [I, DP, DP, DP, DP]),
+    ("sdx_test_load_op", None, [DP, I, DP]),
+    ("sdx_test_matvec", None, [DP, DP]),
+    ("sdx_test_mg_setup", I, []),
+    ("sdx_test_level_dims", None, [I, IP]),
+    ("sdx_test_level_op", None, [I, DP]),
+    ("sdx_test_precond", None, [DP, DP]),
+    ("sdx_test_pcg", I, [DP, DP, I, D]),
+    ("sdx_test_bicgstab", D, [DP, DP, I, D, D, IP]),
+    ("sdx_test_currents", None, [DP, DP, DP, DP]),
+    ("sdx_test_cont_update", D, [I, DP, DP, DP, D, D]),
 ]
 
 

@@ -186,7 +186,7 @@ def scalar_fixture(lib):
     print("wrote scalars")
 
 
-def sweep_fixture(lib, spec, name, fields=True):
+def sweep_fixture(lib, spec, name, fields=True, sample=None):
     """End-to-end reference sweep: currents and floors of every bias point,
     the final psi/phi_n/phi_p and solver statistics."""
     lib.api3_set_threads(8)
@@ -194,11 +194,15 @@ def sweep_fixture(lib, spec, name, fields=True):
     assert lib.api3_solve() == 1
     R = sweep.run_sweep(lib, spec, cids)
     out = dict(V=R["V"], I=R["I"], F=R["F"], conv=R["conv"], gummel=R["gummel"])
-    if fields:
+    if fields or sample:
         N = spec.N
         psi, phin, phip = np.zeros(N), np.zeros(N), np.zeros(N)
         lib.api3_get_state(core.dptr(psi), core.dptr(phin), core.dptr(phip))
-        out.update(psi=psi, phin=phin, phip=phip)
+        if sample:      # large meshes: a fixed random sample of the nodes
+            idx = np.sort(np.random.default_rng(99).choice(N, sample, replace=False))
+            out.update(idx=idx, psi=psi[idx], phin=phin[idx], phip=phip[idx])
+        else:
+            out.update(psi=psi, phin=phin, phip=phip)
     out["stats"] = np.array([lib.api3_get_stat(i) for i in range(4)])
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
     print("wrote", name, "points", len(R["V"]), "all converged", bool(np.all(R["conv"])))
@@ -206,6 +210,9 @@ def sweep_fixture(lib, spec, name, fields=True):
 
 def main():
     lib = load_harness()
+    if "--mosfet96" in sys.argv:       # the full 96x64x64 Id-Vg sweep (about 5 minutes on 8 cores)
+        sweep_fixture(lib, devices.mosfet(96, 64, 64), "e2e_mosfet_96x64x64", fields=False, sample=4096)
+        return
     rng = np.random.default_rng(1234)
     scalar_fixture(lib)
     kernel_fixture(lib, devices.mosfet(12, 8, 6), "k_mosfet_12x8x6", [0.0, 0.5, 0.8, 0.0], rng)
