@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Bias points per second of the n-MOSFET Id-Vg sweep on one MI355X.
+
+Workload: the planar n-MOSFET on a 96x64x64 tensor mesh (393,216 nodes),
+Id-Vg from 0 to 1.2 V in 0.05 V steps at Vd = 0.05 V and 1.0 V (50 bias
+points), synthetic device (no external data).  A step is one bias point:
+api3_resolve continuing the sweep from the previous point, converged to the
+reference's tolerances (tol_p 1e-10 V, tol_c 1e-9 relative current), inputs
+already resident on the device.  W warmup points are solved untimed, then K
+points are timed (W + K <= 50).
+
+Two solver configurations run back to back on the same sweep: the plain FP64
+method (FP64 multigrid) and the mixed-precision method (FP32 multigrid,
+smoothers and preconditioner SpMV inside FP64 Krylov/Newton).  `value` is the
+mixed-precision rate; the plain FP64 rate and the speedup are reported beside
+it.  The terminal currents of every timed point are checked against the
+reference core's sweep (tests/golden/e2e_mosfet_96x64x64.npz).
+
+roofline: the dominant kernel (the level-0 zebra line Gauss-Seidel sweep of
+the multigrid smoother), its algorithmic bytes per launch divided by its
+average launch time, measured with HIP events on the solver stream during the
+timed region (every 8th launch).  cpu_baseline: the reference C core
+(oracle/_ref/libsemisim_ref.so, built from the reference sources) on the host
+cores, on a sample of the same sweep: at four positions it resumes from the
+GPU's converged state and is timed on three consecutive bias points.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "mosfet_96x64x64 Id-Vg sweep (0-1.2 V x 0.05 V at Vd 0.05 and 1.0 V)"
+
+
+def sweep_points(spec):
+    pts = []
+    for tgt, vs, ve, n, hold in spec.sweep:
+        for v in np.linspace(vs, ve, n):
+            pts.append((tgt, float(v), hold))
+    return pts
+
+
+def solve_points(lib, cids, pts, i0, i1):
+    I = []
+    for tgt, v, hold in pts[i0:i1]:
+        for c, hv in hold.items():
+            lib.api3_set_contact_V(cids[c], hv)
+        lib.api3_set_contact_V(cids[tgt], v)
+        ok = lib.api3_resolve()
+        if not ok:
+            raise RuntimeError(f"bias point V={v} did not converge")
+        I.append([lib.api3_get_contact_I(cids[c]) for c in range(len(cids))])
+    return np.array(I)
+
+
+def run_gpu(lib, spec, precision, warmup, steps, ktimer, keep=()):
+    """Solve the sweep's first warmup points untimed, then time `steps`
+    points; returns (seconds, currents, kernel timer, iteration counts,
+    {index: (psi, phi_n, phi_p)} of the converged states listed in keep, read
+    outside the timed calls)."""
+    from semiconductor_devices_simulator_amd import devices
+    from semiconductor_devices_simulator_amd.core import dptr
+    cids = devices.load_device(lib, spec)
+    assert lib.sdx_set_precision(precision) == 1
+    if not lib.api3_solve():
+        raise RuntimeError("equilibrium solve failed")
+    pts = sweep_points(spec)
+    solve_points(lib, cids, pts, 0, warmup)
+    g0 = [lib.api3_get_stat(i) for i in range(4)]
+    if ktimer:
+        lib.sdx_ktimer_enable(0, 8)
+    dt, I, states = 0.0, [], {}
+    for i in range(warmup, warmup + steps):
+        t0 = time.perf_counter()
+        I.append(solve_points(lib, cids, pts, i, i + 1)[0])
+        dt += time.perf_counter() - t0
+        if i in keep:
+            st = tuple(np.zeros(spec.N) for _ in range(3))
+            lib.api3_get_state(dptr(st[0]), dptr(st[1]), dptr(st[2]))
+            states[i] = st
+    kt = np.zeros(4)
+    if ktimer:
+        lib.sdx_ktimer_read(kt.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    g1 = [lib.api3_get_stat(i) for i in range(4)]
+    its = dict(pcg=g1[0] - g0[0], bicgstab=g1[1] - g0[1], newton=g1[2] - g0[2], gummel=g1[3] - g0[3])
+    return dt, np.array(I), kt, its, states
+
+
+def parity(I, warmup, steps):
+    path = os.path.join(ROOT, "tests", "golden", "e2e_mosfet_96x64x64.npz")
+    if not os.path.isfile(path):
+        return None
+    g = np.load(path)
+    Ir, Fr = g["I"][warmup:warmup + steps], g["F"][warmup:warmup + steps]
+    d = np.abs(I - Ir)
+    rel = d[:, 1] / np.abs(Ir[:, 1])
+    ok = d <= 1e-8 * np.abs(Ir) + 2 * Fr
+    return dict(max_rel_drain_current_err=float(rel.max()), points_within_tolerance=int(ok.all(axis=1).sum()),
+                points=int(len(I)), criterion="|dI| <= 1e-8|I_ref| + 2 floor_ref, every terminal")
+
+
+# CPU-baseline sample: the reference resumes from the GPU's converged state
+# at each window start (api3_set_state), solves the next point untimed (it
+# re-arms the sweep predictor), then is timed on the following three points.
+CPU_WINDOWS = (10, 18, 35, 43)
+CPU_TIMED = 3
+
+
+def cpu_baseline(states):
+    ref = os.path.join(ROOT, "oracle", "_ref", "libsemisim_ref.so")
+    if not os.path.isfile(ref) or not states:
+        return None
+    threads = min(os.cpu_count() or 1, 16)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "states.npz")
+        np.savez(path, idx=np.array(sorted(states)), **{f"s{i}": np.stack(states[i]) for i in states})
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker", path],
+                           env=env, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr)
+        return None
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    return dict(value=round(out["points_per_s"], 4), unit="bias points/s", cores=threads, kind="reference",
+                sample=f"{out['points']} bias points ({CPU_TIMED} consecutive points after each of the sweep "
+                       f"positions {list(CPU_WINDOWS)}, the reference resuming from the GPU's converged state "
+                       f"there), {out['seconds']:.1f} s; reference C core built from its sources, OpenMP")
+
+
+def cpu_baseline_worker(path):
+    from semiconductor_devices_simulator_amd import core, devices
+    from semiconductor_devices_simulator_amd.core import dptr
+    lib = core.bind(ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libsemisim_ref.so")))
+    spec = devices.mosfet(96, 64, 64)
+    cids = devices.load_device(lib, spec)
+    pts = sweep_points(spec)
+    data = np.load(path)
+    dt, n = 0.0, 0
+    for i in data["idx"]:
+        i = int(i)
+        st = data[f"s{i}"]
+        tgt, v, hold = pts[i]
+        for c, hv in hold.items():
+            lib.api3_set_contact_V(cids[c], hv)
+        lib.api3_set_contact_V(cids[tgt], v)
+        psi, phin, phip = (np.ascontiguousarray(a) for a in st)
+        assert lib.api3_set_state(dptr(psi), dptr(phin), dptr(phip)) == 1
+        solve_points(lib, cids, pts, i + 1, i + 2)
+        t0 = time.perf_counter()
+        solve_points(lib, cids, pts, i + 2, i + 2 + CPU_TIMED)
+        dt += time.perf_counter() - t0
+        n += CPU_TIMED
+    print(json.dumps(dict(points_per_s=n / dt, seconds=dt, points=n)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=48)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-baseline-worker", default="")
+    a = ap.parse_args()
+    if a.cpu_baseline_worker:
+        cpu_baseline_worker(a.cpu_baseline_worker)
+        return 0
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); the sweep is not sharded "
+              "across GPUs", file=sys.stderr)
+        return 2
+    from semiconductor_devices_simulator_amd import core, devices
+    lib = core.load_core()
+    spec = devices.mosfet(96, 64, 64)
+    npts = spec.bias_points()
+    W = max(0, min(a.warmup, npts - 1))
+    K = max(1, min(a.steps, npts - W))
+    buf = ctypes.create_string_buffer(256)
+    lib.sdx_device_name(buf, 256)
+    print(f"[bench] device {buf.value.decode()}; {WORKLOAD}; warmup {W} timed {K}", file=sys.stderr, flush=True)
+    keep = [i for i in CPU_WINDOWS if W <= i and i + 2 + CPU_TIMED <= npts]
+    t64, I64, _, its64, states = run_gpu(lib, spec, 0, W, K, ktimer=False, keep=keep)
+    print(f"[bench] plain FP64: {K / t64:.3f} points/s", file=sys.stderr, flush=True)
+    t32, I32, kt, its32, _ = run_gpu(lib, spec, 1, W, K, ktimer=True)
+    print(f"[bench] FP32 multigrid: {K / t32:.3f} points/s", file=sys.stderr, flush=True)
+    roof = None
+    if kt[0] > 0:
+        avg_s = kt[1] / kt[0] * 1e-3
+        ach = kt[2] / avg_s / 1e9
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="k_line_gs<float> level 0 (zebra line Gauss-Seidel sweep, one colour)",
+                    bytes_per_launch=kt[2], avg_launch_us=round(avg_s * 1e6, 2), launches_timed=int(kt[0]))
+    cpu = None if a.no_cpu_baseline else cpu_baseline(states)
+    value = K / t32
+    out = {
+        "metric": "bias points/s at 1e-8 rel current err, learned+FP32-AMG vs plain FP64 on GPU",
+        "value": round(value, 4),
+        "unit": "bias points/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(1e3 * t32 / K, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (Krylov, Newton, assembly) + f32 multigrid",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD, "nodes": spec.N, "mesh": list(spec.shape), "method": "FP32-AMG",
+                   "learned_warm_start": False},
+        "plain_fp64_points_per_s": round(K / t64, 4),
+        "speedup_vs_plain_fp64": round(t64 / t32, 3),
+        "iterations": {"fp32_mg": its32, "fp64": its64},
+        "parity": {"fp32_mg": parity(I32, W, K), "fp64": parity(I64, W, K)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

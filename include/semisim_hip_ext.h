@@ -20,6 +20,20 @@ int  sdx_device_name(char *buf, int n);
 int  sdx_kernel_stats(double *out, int n);
 void sdx_reset_stats(void);
 
+/* sampled HIP-event timer on the solver stream: kind 0 = level-0 smoother
+   sweep, 1 = Krylov SpMV; every n-th launch is timed.  read: out[0] timed
+   launches, out[1] their total ms, out[2] algorithmic bytes per launch,
+   out[3] launches seen.  Reading stops the timer. */
+void sdx_ktimer_enable(int kind, int every);
+int  sdx_ktimer_read(double *out);
+
+/* kernel microbenchmark on the solved device: ms per launch of `what`
+   (0/1/2 line sweep x/y/z, 3 point GS sweep, 4 SpMV+dot, 5 residual+restriction,
+   6 V-cycle, 7 copy of N doubles, 8 Poisson assembly, 9 continuity assembly) */
+double sdx_microbench(int what, int iters);
+void   sdx_tune(int what, int value);      /* 0: lines per line-smoother workgroup, 1: line-smoothed levels,
+                                              2: tail sweeps */
+
 /* ---- kernel-level hooks on the prepared device (tests) ----
    A7: 7 blocks of N doubles (c, xm, xp, ym, yp, zm, zp). */
 int    sdx_test_prepare(void);

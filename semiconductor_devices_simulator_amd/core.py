@@ -84,6 +84,10 @@ EXT_SIGNATURES = [
     ("sdx_device_name", I, [ctypes.c_char_p, I]),
     ("sdx_kernel_stats", I, [DP, I]),
     ("sdx_reset_stats", None, []),
+    ("sdx_microbench", D, [I, I]),
+    ("sdx_tune", None, [I, I]),
+    ("sdx_ktimer_enable", None, [I, I]),
+    ("sdx_ktimer_read", I, [DP]),
     ("sdx_test_prepare", I, []),
     ("sdx_test_set_state", None, [DP, DP, DP]),
     ("sdx_test_get_node", None, [I, DP]),

@@ -75,6 +75,16 @@ void eng_free() {
 
 // Multigrid hierarchy shape: 2x2x2 aggregation of every axis longer than 2
 // until N <= 64 (semiconductor_3d.c:1894-1921).
+// the coarse end (levels of at most G.tail_n nodes, not level 0) runs as one
+// single-workgroup kernel; its level descriptors are copied to the device
+template <class T> static void hier_tail(Hier<T> &H) {
+    H.tail = -1;
+    int nl = (int)H.lev.size();
+    for (int L = 1; L < nl; L++) if (H.lev[L].N <= G.tail_n) { H.tail = L; break; }
+    if (H.tail == nl - 1) H.tail = -1;            // a lone coarsest level: the dense solve kernel
+    if (H.tail > 0 && H.d_tail)
+        HC(hipMemcpy(H.d_tail, &H.lev[H.tail], sizeof(MGLevel<T>) * (nl - H.tail), hipMemcpyHostToDevice));
+}
 template <class T> static int hier_alloc(Hier<T> &H, bool level0_own) {
     H.lev.clear();
     MGLevel<T> f{};
@@ -111,9 +121,13 @@ template <class T> static int hier_alloc(Hier<T> &H, bool level0_own) {
             v.b = dalloc<T>(n);
         }
         v.pw = dalloc<T>(n);
+        for (int a = 0; a < 3; a++)
+            for (int q = 0; q < 3; q++) v.lu[a][q] = dalloc<T>(n);
     }
     int nz = H.lev.back().N;
     H.inv = dalloc<double>((size_t)nz * nz);
+    H.d_tail = dalloc<MGLevel<T>>(H.lev.size());
+    hier_tail(H);
     H.alloc = !g_err;
     H.pw_one = false;
     return H.alloc;
@@ -466,6 +480,8 @@ template <class T> static void mg_setup_T() {
     }
     H.lev[0].rs = const_cast<double *>(G.mg_w);
     for (int L = 0; L < nl - 1; L++) mg_galerkin(s, H.lev[L], H.lev[L + 1]);
+    for (int L = 0; L < nl - 1; L++)
+        for (int a = 0; a < G.n_axes; a++) mg_line_factor(s, H.lev[L], G.mg_axes[a]);
     mg_coarse_factor(s, H.lev[nl - 1], H.inv);
 }
 static void mg_setup() {
@@ -481,31 +497,86 @@ static void mg_setup() {
 // couplings (G.mg_axes, chosen in prepare), or red-black point Gauss-Seidel
 // (G.n_axes == 0).  Pre-smoothing starts from x = 0; returns the restriction
 // skip code of the rows the last sweep solved exactly.
-template <class T> static int smooth_pre(const MGLevel<T> &v, const T *b, T *x) {
+// ------------------------------------------------ sampled kernel timer
+// HIP events on the solver stream around every 8th level-0 launch of one
+// kernel class (0: smoother sweep, 1: Krylov SpMV), for live roofline
+// figures; off unless sdx_ktimer_enable is called.
+struct KTimer {
+    int on = 0, kind = 0, every = 8;
+    long long count = 0, sampled = 0;
+    double total_ms = 0.;
+    std::vector<hipEvent_t> pool, pend;
+};
+static KTimer KT;
+static hipEvent_t kt_event() {
+    hipEvent_t e;
+    if (!KT.pool.empty()) { e = KT.pool.back(); KT.pool.pop_back(); return e; }
+    HC(hipEventCreate(&e));
+    return e;
+}
+static void kt_flush() {
+    for (size_t q = 0; q + 1 < KT.pend.size(); q += 2) {
+        float ms = 0.f;
+        HC(hipEventSynchronize(KT.pend[q + 1]));
+        HC(hipEventElapsedTime(&ms, KT.pend[q], KT.pend[q + 1]));
+        KT.total_ms += ms;
+        KT.sampled++;
+        KT.pool.push_back(KT.pend[q]);
+        KT.pool.push_back(KT.pend[q + 1]);
+    }
+    KT.pend.clear();
+}
+static bool kt_begin(int kind) {
+    if (!KT.on || KT.kind != kind) return false;
+    if ((KT.count++ % KT.every) != 0) return false;
+    hipEvent_t e = kt_event();
+    HC(hipEventRecord(e, G.st));
+    KT.pend.push_back(e);
+    return true;
+}
+static void kt_end(bool started) {
+    if (!started) return;
+    hipEvent_t e = kt_event();
+    HC(hipEventRecord(e, G.st));
+    KT.pend.push_back(e);
+    if (KT.pend.size() >= 512) kt_flush();
+}
+
+template <class T> static int smooth_pre(const MGLevel<T> &v, const T *b, T *x, int L = 0) {
     hipStream_t s = G.st;
-    if (G.n_axes == 0) {
+    if (G.n_axes == 0 || L >= G.line_levels) {
         mg_rbgs(s, v, b, x, 0, 1);
         mg_rbgs(s, v, b, x, 1, 0);
         return 1;
     }
     int first = 1;
     for (int a = 0; a < G.n_axes; a++)
-        for (int color = 0; color < 2; color++) { mg_line(s, v, b, x, G.mg_axes[a], color, first); first = 0; }
+        for (int color = 0; color < 2; color++) {
+            bool t = v.N == G.N && kt_begin(0);
+            mg_line(s, v, b, x, G.mg_axes[a], color, first);
+            kt_end(t);
+            first = 0;
+        }
     return 2 + G.mg_axes[G.n_axes - 1];
 }
 // Post-smoothing: the pre-smoother's sweeps in reverse order (symmetric
 // V-cycle for CG).  With c != nullptr the first sweep applies the coarse
 // correction x += pw * x_parent to the neighbours it reads; the sweeps then
 // overwrite every node, so the correction is never stored.
-template <class T> static void smooth_post(const MGLevel<T> &v, const T *b, T *x, const MGLevel<T> *c) {
+template <class T> static void smooth_post(const MGLevel<T> &v, const T *b, T *x, const MGLevel<T> *c, int L = 0) {
     hipStream_t s = G.st;
-    if (G.n_axes == 0) {
+    if (G.n_axes == 0 || L >= G.line_levels) {
         mg_rbgs(s, v, b, x, 1, 0, c);
         mg_rbgs(s, v, b, x, 0, 0);
         return;
     }
     for (int a = G.n_axes - 1; a >= 0; a--)
-        for (int color = 1; color >= 0; color--) { mg_line(s, v, b, x, G.mg_axes[a], color, 0, c); c = nullptr; }
+        for (int color = 1; color >= 0; color--) {
+            bool t = v.N == G.N && kt_begin(0);
+            mg_line(s, v, b, x, G.mg_axes[a], color, 0, c);
+            kt_end(t);
+            c = nullptr;
+        }
 }
 template <class T> static void vcycle_T(int L, const T *b, T *x) {
     Hier<T> &H = hier<T>();
@@ -513,11 +584,12 @@ template <class T> static void vcycle_T(int L, const T *b, T *x) {
     int nl = (int)H.lev.size();
     MGLevel<T> &v = H.lev[L];
     if (L == nl - 1) { mg_coarse_solve(s, v, H.inv, b, x); return; }
+    if (L == H.tail) { mg_tail(s, H.d_tail, nl - L, H.inv, G.tail_sweeps); return; }   // b, x are lev[L].b, .x
     MGLevel<T> &c = H.lev[L + 1];
-    int skip = smooth_pre(v, b, x);
+    int skip = smooth_pre(v, b, x, L);
     mg_resid_restrict(s, v, c, b, x, skip);   // coarse right-hand side
     vcycle_T<T>(L + 1, c.b, c.x);
-    smooth_post(v, b, x, &c);
+    smooth_post(v, b, x, &c, L);
 }
 // smoother only (the reference's ILU-only retry): one symmetric smoothing from 0
 template <class T> static void smooth_only_T(const T *b, T *x) {
@@ -561,7 +633,9 @@ static int pcg(const double *b, double *x, int maxit, double rtol) {
     int it;
     for (it = 1; it <= maxit; it++) {
         G.n_spmv++;
-        sd_spmv_dots(s, g, G.A, p, q, p, nullptr, 1, G.rb, S_PQ, SOP_PCG_AL);
+        { bool kt = kt_begin(1);
+          sd_spmv_dots(s, g, G.A, p, q, p, nullptr, 1, G.rb, S_PQ, SOP_PCG_AL);
+          kt_end(kt); }
         sd_pcg_update_xr(s, x, r, p, q, N, G.rb);
         precond(r, z);
         sd_dot(s, r, z, N, G.rb, S_TMP0, SOP_PCG_BE);
@@ -605,7 +679,9 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
         sd_bi_p(s, p, r, v, N, G.rb);
         precond(p, ph);
         G.n_spmv++;
-        sd_spmv_dots(s, g, G.A, ph, v, rh, nullptr, 1, G.rb, S_RV, SOP_BI_AL);
+        { bool kt = kt_begin(1);
+          sd_spmv_dots(s, g, G.A, ph, v, rh, nullptr, 1, G.rb, S_RV, SOP_BI_AL);
+          kt_end(kt); }
         sd_bi_r1(s, r, v, N, G.rb);
         rd(S_AL, S_SS + 1);
         double rho1 = S(S_RHO1), rv = S(S_RV), ss = S(S_SS);
@@ -628,7 +704,9 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
         }
         precond(r, sh);
         G.n_spmv++;
-        sd_spmv_dots(s, g, G.A, sh, t, nullptr, r, 2, G.rb, S_TT, SOP_BI_OM);
+        { bool kt = kt_begin(1);
+          sd_spmv_dots(s, g, G.A, sh, t, nullptr, r, 2, G.rb, S_TT, SOP_BI_OM);
+          kt_end(kt); }
         sd_bi_xr2(s, x, r, ph, sh, t, N, G.rb);
         rd(S_OM, S_RR + 1);
         double om = S(S_OM);
@@ -1262,6 +1340,9 @@ void eng_read_env() {
     if ((e = getenv("SEMISIM_GROW"))) { double g = atof(e); if (g >= 1. && g <= 4.) GROW_SLOW = g; }
     if ((e = getenv("SEMISIM_CATOL"))) CONT_ATOL = atof(e);
     if ((e = getenv("SEMISIM_MG_FP32"))) G.precision = atoi(e) ? 1 : 0;
+    if ((e = getenv("SEMISIM_MG_LINELEVELS"))) G.line_levels = atoi(e);
+    if ((e = getenv("SEMISIM_MG_TAILN"))) G.tail_n = atoi(e);
+    if ((e = getenv("SEMISIM_MG_TAILSW"))) { int q = atoi(e); G.tail_sweeps = q > 0 ? q : 1; }
 }
 int eng_aa_depth() { return G.aa_m < AA_DEPTH ? G.aa_m : AA_DEPTH; }
 void eng_set_predictor(int on) { PRED_ON = on != 0; }
@@ -1294,6 +1375,89 @@ template <class T> static void level_op_T(Hier<T> &H, int L, double *A7) {
 }
 extern "C" {
 int sdx_test_prepare(void) { return eng_prepare() && eng_prepare_precision(); }
+// Kernel microbenchmark on the current device and multigrid hierarchy (after
+// a solve): `iters` back-to-back launches timed with HIP events on the solver
+// stream.  what: 0/1/2 line sweep along x/y/z (level 0, one colour), 3 point
+// red-black sweep, 4 Krylov SpMV+dot, 5 residual+restriction level 0->1,
+// 6 full V-cycle, 7 copy of N doubles, 8 Poisson assembly, 9 continuity
+// assembly.  out[0] = ms per launch.
+double sdx_microbench(int what, int iters) {
+    if (!G.allocated) return -1.;
+    hipStream_t s = G.st;
+    hipEvent_t e0, e1;
+    HC(hipEventCreate(&e0)); HC(hipEventCreate(&e1));
+    DevView d = eng_view();
+    auto one = [&]() {
+        if (G.precision == 1) {
+            MGLevel<float> &v = G.H32.lev[0];
+            switch (what) {
+            case 0: case 1: case 2: mg_line(s, v, G.d_b32, G.d_x32, what, 0, 0); break;
+            case 3: mg_rbgs(s, v, G.d_b32, G.d_x32, 1, 0); break;
+            case 5: mg_resid_restrict(s, v, G.H32.lev[1], G.d_b32, G.d_x32, 0); break;
+            case 6: vcycle_T<float>(0, G.d_b32, G.d_x32); break;
+            default: break;
+            }
+        } else {
+            MGLevel<double> &v = G.H64.lev[0];
+            switch (what) {
+            case 0: case 1: case 2: mg_line(s, v, G.d_wk[0], G.d_wk[1], what, 0, 0); break;
+            case 3: mg_rbgs(s, v, G.d_wk[0], G.d_wk[1], 1, 0); break;
+            case 5: mg_resid_restrict(s, v, G.H64.lev[1], G.d_wk[0], G.d_wk[1], 0); break;
+            case 6: vcycle_T<double>(0, G.d_wk[0], G.d_wk[1]); break;
+            default: break;
+            }
+        }
+        switch (what) {
+        case 4: sd_spmv_dots(s, d.g, G.A, G.d_wk[2], G.d_wk[3], G.d_wk[4], nullptr, 1, G.rb, S_TMP1, -1); break;
+        case 7: sd_copy_bw(s, G.d_wk[4], G.d_wk[5], G.N); break;
+        case 8: ph_poisson(s, d, G.d_psi, 1, G.d_wk[5], G.A, G.d_sc, G.rb, S_TMP1); break;
+        case 9: ph_cont(s, d, 0, G.A, G.d_wk[5], G.d_wk[6], G.d_dsc); break;
+        default: break;
+        }
+    };
+    for (int w = 0; w < 3; w++) one();
+    HC(hipEventRecord(e0, s));
+    for (int it = 0; it < iters; it++) one();
+    HC(hipEventRecord(e1, s));
+    HC(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HC(hipEventElapsedTime(&ms, e0, e1));
+    HC(hipEventDestroy(e0)); HC(hipEventDestroy(e1));
+    return (double)ms / iters;
+}
+void sdx_tune(int what, int value) {
+    if (what == 0) g_line_nl_max = value;
+    if (what == 1) G.line_levels = value;
+    if (what == 2) G.tail_sweeps = value > 0 ? value : 1;
+    if (what == 3) {
+        G.tail_n = value;
+        if (G.H64.alloc) hier_tail(G.H64);
+        if (G.H32.alloc) hier_tail(G.H32);
+    }
+}
+void sdx_ktimer_enable(int kind, int every) {
+    kt_flush();
+    KT.on = 1; KT.kind = kind; KT.every = every > 0 ? every : 8;
+    KT.count = KT.sampled = 0; KT.total_ms = 0.;
+}
+// out: [sampled launches, total ms of them, algorithmic bytes per launch, launches seen]
+int sdx_ktimer_read(double *out) {
+    eng_sync();
+    kt_flush();
+    double w = G.precision == 1 ? 4.0 : 8.0;
+    double bytes;
+    if (KT.kind == 0) {
+        // one colour of zebra line GS: for the lines' N/2 nodes 3 in-line
+        // coefficients, 4 off-line couplings, b, the x written, plus the other
+        // colour's N/2 x values read (point GS: 6 couplings + diag + b + x + x_nb)
+        bytes = G.n_axes ? 0.5 * G.N * 10.0 * w : 0.5 * G.N * 10.0 * w;
+    } else {
+        bytes = (double)G.N * 11.0 * 8.0;   // 7 diagonals + x + dot vector(s) + y (FP64)
+    }
+    out[0] = (double)KT.sampled; out[1] = KT.total_ms; out[2] = bytes; out[3] = (double)KT.count;
+    KT.on = 0;
+    return 1;
+}
 void sdx_test_set_state(const double *psi, const double *phin, const double *phip) {
     eng_upload(G.d_psi, psi, G.N); eng_upload(G.d_phin, phin, G.N); eng_upload(G.d_phip, phip, G.N);
     eng_rebuild_np();

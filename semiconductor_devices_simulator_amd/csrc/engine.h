@@ -23,6 +23,8 @@ template <class T> struct Hier {
     double *inv = nullptr;         // coarsest-level inverse (row-major)
     bool alloc = false;
     bool pw_one = false;
+    int tail = -1;                 // first level handled by the single-workgroup tail kernel (-1 none)
+    MGLevel<T> *d_tail = nullptr;  // device copies of the tail levels' descriptors
 };
 
 struct Engine {
@@ -90,6 +92,9 @@ struct Engine {
     int mg_qsign = 0;
     int mg_off = 0;
     int n_axes = 2, mg_axes[3] = {1, 2, 0};   // line-smoother axes (0: point red-black GS)
+    int line_levels = 2;                      // levels smoothed with lines (the finest ones)
+    int tail_sweeps = 1;                      // red-black sweeps per side in the tail kernel
+    int tail_n = 1024;                        // levels of at most this many nodes run in the tail kernel
 
     // Anderson
     int aa_m = 12, aa_n = 0, aa_pos = 0, aa_have = 0;

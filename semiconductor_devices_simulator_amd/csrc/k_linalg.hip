@@ -396,31 +396,29 @@ template <class T> void mg_galerkin(hipStream_t s, const MGLevel<T> &f, MGLevel<
 
 // Red-black Gauss-Seidel half sweep on the nodes of one colour
 // ((i+j+l) & 1 == color).  zero != 0: the sweep starts from x = 0 (every node
-// is written: the colour's nodes get b/d, the others 0).
+// is written: the colour's nodes get b/d, the others 0).  pro: the neighbours
+// carry the coarse correction x += pw * x_parent, applied on the fly.
 template <class T>
-__global__ void __launch_bounds__(SD_BLOCK) k_rbgs(MGLevel<T> v, const T *b, T *x, int color, int zero,
-                                                 MGLevel<T> c, int pro) {
+__device__ __forceinline__ void rb_zero_node(const MGLevel<T> &v, const T *b, T *x, int color, int k) {
+    const int Nx = v.Nx, NxNy = Nx * v.Ny;
+    int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
+    if (((i + j + l) & 1) != color) { x[k] = (T)0; return; }
+    T d = v.A.c ? v.A.c[k] : (T)1;
+    if (!(d != (T)0)) d = (T)1;
+    x[k] = b[k] / d;
+}
+// t indexes the colour's nodes: ceil(Nx/2) per x-line
+template <class T>
+__device__ __forceinline__ void rb_node(const MGLevel<T> &v, const T *b, T *x, int color, int t,
+                                        const MGLevel<T> &c, int pro) {
     const int Nx = v.Nx, Ny = v.Ny, Nz = v.Nz, NxNy = Nx * Ny;
-    if (zero) {
-        int k = blockIdx.x * SD_BLOCK + threadIdx.x;
-        if (k >= v.N) return;
-        int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
-        if (((i + j + l) & 1) != color) { x[k] = (T)0; return; }
-        T d = v.A.c ? v.A.c[k] : (T)1;
-        if (!(d != (T)0)) d = (T)1;
-        x[k] = b[k] / d;
-        return;
-    }
     int Hx = (Nx + 1) >> 1;
-    int t = blockIdx.x * SD_BLOCK + threadIdx.x;
-    if (t >= Hx * Ny * Nz) return;
     int line = t / Hx, i2 = t - line * Hx, l = line / Ny, j = line - l * Ny;
     int i = 2 * i2 + ((j + l + color) & 1);
     if (i >= Nx) return;
     int k = i + Nx * (j + Ny * l);
     T s = b[k];
     if (pro) {
-        // neighbours carry the coarse correction x += pw * x_parent, applied on the fly
         auto xc = [&](int ii, int jj, int ll, int kk) -> T {
             int I = v.cx ? ii >> 1 : ii, J = v.cy ? jj >> 1 : jj, L = v.cz ? ll >> 1 : ll;
             return x[kk] + v.pw[kk] * c.x[I + c.Nx * (J + c.Ny * L)];
@@ -443,6 +441,16 @@ __global__ void __launch_bounds__(SD_BLOCK) k_rbgs(MGLevel<T> v, const T *b, T *
     if (!(d != (T)0)) d = (T)1;
     x[k] = s / d;
 }
+template <class T>
+__global__ void __launch_bounds__(SD_BLOCK) k_rbgs(MGLevel<T> v, const T *b, T *x, int color, int zero,
+                                                 MGLevel<T> c, int pro) {
+    int t = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (zero) {
+        if (t < v.N) rb_zero_node(v, b, x, color, t);
+        return;
+    }
+    if (t < ((v.Nx + 1) >> 1) * v.Ny * v.Nz) rb_node(v, b, x, color, t, c, pro);
+}
 template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int color, int zero,
                                  const MGLevel<T> *c) {
     MGLevel<T> cc = c ? *c : v;
@@ -455,43 +463,85 @@ template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, 
 
 // Zebra line Gauss-Seidel: every line along axis `ax` whose two other indices
 // have parity `color` is solved exactly (tridiagonal in-line couplings, the
-// off-line neighbours taken from x).  One workgroup owns NL lines of one
-// colour; the tridiagonal systems are solved by parallel cyclic reduction in
-// LDS (log2 n steps, one lane per equation).  zero: the sweep starts from
-// x = 0 (off-line terms vanish).  pro: the off-line neighbours carry the coarse
-// correction x + pw * x_parent (first post-smoothing sweep).
+// off-line neighbours taken from x).  The LU factors of every line depend only
+// on the operator, so mg_line_factor computes them once per multigrid setup:
+// for line element q, iw = 1/w_q, ai = a_q/w_q, cp = c_q/w_q with
+// w_q = d_q - a_q cp_{q-1}.  A sweep is then a short recurrence per line:
+//   y_q = r_q iw_q - ai_q y_{q-1};  x_q = y_q - cp_q x_{q+1}.
+// One workgroup owns NL lines of one colour: all lanes compute the line
+// right-hand sides r_q = b - (off-line couplings) x with coalesced loads and
+// stage them with the factors in LDS, NL lanes run the recurrences, all lanes
+// store x.  zero: the sweep starts from x = 0 (off-line terms vanish).  pro:
+// the off-line neighbours carry the coarse correction x + pw * x_parent (first
+// post-smoothing sweep).
 #define LINE_MAXE 4
 #define LINE_MAXEQ (SD_BLOCK * LINE_MAXE)
+struct LineGeom { int n, Ntd, s_ax, s_td, s_3; };
+__device__ __forceinline__ LineGeom line_geom(int Nx, int Ny, int Nz, int ax) {
+    LineGeom g;
+    if (ax == 0) { g.n = Nx; g.Ntd = Ny; g.s_ax = 1; g.s_td = Nx; g.s_3 = Nx * Ny; }
+    else if (ax == 1) { g.n = Ny; g.Ntd = Nx; g.s_ax = Nx; g.s_td = 1; g.s_3 = Nx * Ny; }
+    else { g.n = Nz; g.Ntd = Nx; g.s_ax = Nx * Ny; g.s_td = 1; g.s_3 = Nx; }
+    return g;
+}
+// one lane per line (both colours), sequential over the line
 template <class T>
-__global__ void __launch_bounds__(SD_BLOCK) k_line_gs(MGLevel<T> v, const T *b, T *x, int ax, int color, int zero,
-                                                      MGLevel<T> c, int pro, int NL) {
-    __shared__ T sa[LINE_MAXEQ], sb[LINE_MAXEQ], sc[LINE_MAXEQ], sd[LINE_MAXEQ];
+__global__ void __launch_bounds__(SD_BLOCK) k_line_factor(MGLevel<T> v, int ax) {
+    LineGeom g = line_geom(v.Nx, v.Ny, v.Nz, ax);
+    int N3 = ax == 2 ? v.Ny : v.Nz;
+    int t = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (t >= g.Ntd * N3) return;
+    int td = t % g.Ntd, p3 = t / g.Ntd;
+    int k0 = td * g.s_td + p3 * g.s_3;
+    const T *am = ax == 0 ? v.A.xm : (ax == 1 ? v.A.ym : v.A.zm);
+    const T *ap = ax == 0 ? v.A.xp : (ax == 1 ? v.A.yp : v.A.zp);
+    T *ai = v.lu[ax][0], *iw = v.lu[ax][1], *cp = v.lu[ax][2];
+    double cprev = 0.0;
+    for (int q = 0; q < g.n; q++) {
+        int k = k0 + q * g.s_ax;
+        double dg = v.A.c ? (double)v.A.c[k] : 1.0;
+        if (!(dg != 0.0)) dg = 1.0;
+        double a = q > 0 ? (double)am[k] : 0.0, c = q < g.n - 1 ? (double)ap[k] : 0.0;
+        double w = dg - a * cprev;
+        if (!(w != 0.0)) w = dg;
+        double iwq = 1.0 / w;
+        cprev = c * iwq;
+        ai[k] = (T)(a * iwq); iw[k] = (T)iwq; cp[k] = (T)cprev;
+    }
+}
+template <class T> void mg_line_factor(hipStream_t s, const MGLevel<T> &v, int ax) {
+    int N3 = ax == 2 ? v.Ny : v.Nz, Ntd = ax == 0 ? v.Ny : v.Nx;
+    k_line_factor<T><<<sd_nblocks(Ntd * N3), SD_BLOCK, 0, s>>>(v, ax);
+}
+
+template <class T>
+__global__ void __launch_bounds__(SD_BLOCK) k_line_gs(MGLevel<T> v, const T *__restrict__ b, T *__restrict__ x, int ax,
+                                                      int color, int zero, MGLevel<T> c, int pro, int NL) {
+    __shared__ T sr[LINE_MAXEQ], sa[LINE_MAXEQ], sw[LINE_MAXEQ], sp[LINE_MAXEQ], sy[LINE_MAXEQ];
     const int Nx = v.Nx, Ny = v.Ny, Nz = v.Nz, NxNy = Nx * Ny;
-    int n, Ntd, s_ax, s_td, s_3;
-    if (ax == 0) { n = Nx; Ntd = Ny; s_ax = 1; s_td = Nx; s_3 = NxNy; }
-    else if (ax == 1) { n = Ny; Ntd = Nx; s_ax = Nx; s_td = 1; s_3 = NxNy; }
-    else { n = Nz; Ntd = Nx; s_ax = NxNy; s_td = 1; s_3 = Nx; }
-    const int ntile = (Ntd + 2 * NL - 1) / (2 * NL);
+    const LineGeom g = line_geom(Nx, Ny, Nz, ax);
+    const int n = g.n;
+    const int ntile = (g.Ntd + 2 * NL - 1) / (2 * NL);
     const int p3 = blockIdx.x / ntile, t0 = (blockIdx.x - p3 * ntile) * 2 * NL;
     const int sh = (t0 + p3 + color) & 1;
     const int neq = NL * n;
-    T ra[LINE_MAXE], rb_[LINE_MAXE], rc[LINE_MAXE], rd[LINE_MAXE];
+    const T *__restrict__ AI = v.lu[ax][0];
+    const T *__restrict__ IW = v.lu[ax][1];
+    const T *__restrict__ CP = v.lu[ax][2];
+    // phase 1: every lane gathers the right-hand sides of up to LINE_MAXE line
+    // elements; the loop is unrolled and predicated so that all loads issue
+    // before any of them is consumed
+#pragma unroll
     for (int m = 0; m < LINE_MAXE; m++) {
-        int e = threadIdx.x + SD_BLOCK * m;
-        if (e >= neq) break;
-        int q = e / NL, L = e - q * NL, td = t0 + 2 * L + sh;
-        T A_ = 0, B_ = 1, C_ = 0, D_ = 0;
-        if (td < Ntd) {
-            int k = td * s_td + q * s_ax + p3 * s_3;
+        const int e = threadIdx.x + SD_BLOCK * m;
+        const int q = e / NL, L = e - q * NL, td = t0 + 2 * L + sh;
+        const bool ok = e < neq && td < g.Ntd;
+        T r = 0, ai = 0, iw = 1, cp = 0;
+        if (ok) {
+            const int k = td * g.s_td + q * g.s_ax + p3 * g.s_3;
             int i, j, l;
             if (ax == 0) { i = q; j = td; l = p3; } else if (ax == 1) { i = td; j = q; l = p3; } else { i = td; j = p3; l = q; }
-            T dg = v.A.c ? v.A.c[k] : (T)1;
-            if (!(dg != (T)0)) dg = (T)1;
-            B_ = dg;
-            const T *am = ax == 0 ? v.A.xm : (ax == 1 ? v.A.ym : v.A.zm);
-            const T *ap = ax == 0 ? v.A.xp : (ax == 1 ? v.A.yp : v.A.zp);
-            if (q > 0) A_ = am[k];
-            if (q < n - 1) C_ = ap[k];
+            ai = AI[k]; iw = IW[k]; cp = CP[k];
             T s = b[k];
             if (!zero) {
                 auto xv = [&](int ii, int jj, int ll, int kk) -> T {
@@ -512,56 +562,46 @@ __global__ void __launch_bounds__(SD_BLOCK) k_line_gs(MGLevel<T> v, const T *b, 
                     if (l < Nz - 1) s -= v.A.zp[k] * xv(i, j, l + 1, k + NxNy);
                 }
             }
-            D_ = s;
+            r = s;
         }
-        sa[e] = A_; sb[e] = B_; sc[e] = C_; sd[e] = D_;
+        if (e < neq) { sr[e] = r * iw; sa[e] = ai; sp[e] = cp; }
     }
     __syncthreads();
-    for (int st = 1; st < n; st <<= 1) {
-        for (int m = 0; m < LINE_MAXE; m++) {
-            int e = threadIdx.x + SD_BLOCK * m;
-            if (e >= neq) break;
-            int q = e / NL, L = e - q * NL;
-            T a0 = sa[e], b0 = sb[e], c0 = sc[e], d0 = sd[e];
-            T na = 0, nc = 0, nb = b0, nd = d0;
-            if (q - st >= 0) {
-                int em = (q - st) * NL + L;
-                T k1 = a0 / sb[em];
-                na = -sa[em] * k1;
-                nb -= sc[em] * k1;
-                nd -= sd[em] * k1;
-            }
-            if (q + st < n) {
-                int ep = (q + st) * NL + L;
-                T k2 = c0 / sb[ep];
-                nc = -sc[ep] * k2;
-                nb -= sa[ep] * k2;
-                nd -= sd[ep] * k2;
-            }
-            ra[m] = na; rb_[m] = nb; rc[m] = nc; rd[m] = nd;
+    // phase 2: one lane per line runs the two recurrences (separate LDS
+    // arrays in and out, so the next elements' loads can run ahead)
+    if (threadIdx.x < NL) {
+        const int L = threadIdx.x;
+        T y = 0;
+#pragma unroll 8
+        for (int q = 0; q < n; q++) {
+            const int e = q * NL + L;
+            y = sr[e] - sa[e] * y;
+            sy[e] = y;
         }
-        __syncthreads();
-        for (int m = 0; m < LINE_MAXE; m++) {
-            int e = threadIdx.x + SD_BLOCK * m;
-            if (e >= neq) break;
-            sa[e] = ra[m]; sb[e] = rb_[m]; sc[e] = rc[m]; sd[e] = rd[m];
+        T xq = 0;
+#pragma unroll 8
+        for (int q = n - 1; q >= 0; q--) {
+            const int e = q * NL + L;
+            xq = sy[e] - sp[e] * xq;
+            sw[e] = xq;
         }
-        __syncthreads();
     }
+    __syncthreads();
+#pragma unroll
     for (int m = 0; m < LINE_MAXE; m++) {
-        int e = threadIdx.x + SD_BLOCK * m;
-        if (e >= neq) break;
-        int q = e / NL, L = e - q * NL, td = t0 + 2 * L + sh;
-        if (td < Ntd) x[td * s_td + q * s_ax + p3 * s_3] = sd[e] / sb[e];
+        const int e = threadIdx.x + SD_BLOCK * m;
+        const int q = e / NL, L = e - q * NL, td = t0 + 2 * L + sh;
+        if (e < neq && td < g.Ntd) x[td * g.s_td + q * g.s_ax + p3 * g.s_3] = sw[e];
     }
 }
+int g_line_nl_max = 8;       // lines per workgroup cap (tuning)
 template <class T>
 int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero, const MGLevel<T> *c) {
     int n = ax == 0 ? v.Nx : (ax == 1 ? v.Ny : v.Nz);
     int Ntd = ax == 0 ? v.Ny : v.Nx, N3 = ax == 2 ? v.Ny : v.Nz;
-    if (n > LINE_MAXEQ) return 0;
+    if (n > LINE_MAXEQ || !v.lu[ax][0]) return 0;
     int NL = LINE_MAXEQ / n;
-    if (NL > 8) NL = 8;
+    if (NL > g_line_nl_max) NL = g_line_nl_max;
     int half = (Ntd + 1) / 2;
     if (NL > half) NL = half;
     if (NL < 1) NL = 1;
@@ -573,23 +613,21 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
 
 // Residual of the fine level summed over each aggregate with the fine row
 // scale (restriction R = P^T W, semiconductor_3d.c:2028-2037), divided by the
-// coarse row scale (the coarse rows are stored with unit diagonal).
+// coarse row scale (the coarse rows are stored with unit diagonal).  Rows
+// solved exactly by the last smoothing sweep have no residual and are
+// skipped: skip 1 = point black nodes, 2+a = colour-1 lines along axis a.
 template <class T>
-__global__ void __launch_bounds__(SD_BLOCK) k_resid_restrict(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x,
-                                                             int skip) {
-    int K = blockIdx.x * SD_BLOCK + threadIdx.x;
-    if (K >= c.N) return;
+__device__ __forceinline__ void restrict_node(const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x,
+                                              int skip, int K) {
     int cNxNy = c.Nx * c.Ny, LL = K / cNxNy, rr = K - LL * cNxNy, J = rr / c.Nx, I = rr - J * c.Nx;
     Agg a = agg_range(f.Nx, f.Ny, f.Nz, f.cx, f.cy, f.cz, I, J, LL);
     double acc = 0.0;
     for (int l = a.l0; l <= a.l1; l++)
         for (int j = a.j0; j <= a.j1; j++)
             for (int i = a.i0; i <= a.i1; i++) {
-                // rows solved exactly by the last smoothing sweep have no residual:
-                // skip 1 = point black nodes, 2+a = colour-1 lines along axis a
                 if (skip == 1 && ((i + j + l) & 1)) continue;
                 if (skip >= 2) {
-                    int a = skip - 2, par = a == 0 ? (j + l) : (a == 1 ? (i + l) : (i + j));
+                    int ax = skip - 2, par = ax == 0 ? (j + l) : (ax == 1 ? (i + l) : (i + j));
                     if (par & 1) continue;
                 }
                 int k = i + f.Nx * (j + f.Ny * l);
@@ -600,9 +638,63 @@ __global__ void __launch_bounds__(SD_BLOCK) k_resid_restrict(MGLevel<T> f, MGLev
             }
     c.b[K] = (T)(acc / c.rs[K]);
 }
+template <class T>
+__global__ void __launch_bounds__(SD_BLOCK) k_resid_restrict(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x,
+                                                             int skip) {
+    int K = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (K < c.N) restrict_node(f, c, b, x, skip, K);
+}
 template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x,
                                           int skip) {
     k_resid_restrict<T><<<sd_nblocks(c.N), SD_BLOCK, 0, s>>>(f, c, b, x, skip);
+}
+
+// The coarse end of the V-cycle in one workgroup: every level from lv[0]
+// (whose b holds the restricted residual) to the coarsest lv[nt-1] (dense
+// inverse), with nsw red-black Gauss-Seidel sweeps before and after each
+// coarse correction, the same arithmetic as the per-level kernels, separated
+// by workgroup barriers instead of kernel launches.  Result in lv[0].x.
+#define TAIL_BLOCK 1024
+template <class T>
+__global__ void __launch_bounds__(TAIL_BLOCK) k_tail(const MGLevel<T> *lv, int nt, const double *inv, int nsw) {
+    const int tid = threadIdx.x;
+    for (int L = 0; L < nt - 1; L++) {
+        const MGLevel<T> v = lv[L];
+        const int half = ((v.Nx + 1) >> 1) * v.Ny * v.Nz;
+        for (int sw = 0; sw < nsw; sw++) {
+            if (sw == 0) { for (int k = tid; k < v.N; k += TAIL_BLOCK) rb_zero_node(v, v.b, v.x, 0, k); }
+            else { for (int t = tid; t < half; t += TAIL_BLOCK) rb_node(v, v.b, v.x, 0, t, v, 0); }
+            __syncthreads();
+            for (int t = tid; t < half; t += TAIL_BLOCK) rb_node(v, v.b, v.x, 1, t, v, 0);
+            __syncthreads();
+        }
+        const MGLevel<T> c = lv[L + 1];
+        for (int K = tid; K < c.N; K += TAIL_BLOCK) restrict_node(v, c, v.b, v.x, 1, K);
+        __syncthreads();
+    }
+    {
+        const MGLevel<T> z = lv[nt - 1];
+        const int n = z.N;
+        for (int r = tid; r < n; r += TAIL_BLOCK) {
+            double s = 0.0;
+            for (int q = 0; q < n; q++) s += inv[r * n + q] * (double)z.b[q];
+            z.x[r] = (T)s;
+        }
+        __syncthreads();
+    }
+    for (int L = nt - 2; L >= 0; L--) {
+        const MGLevel<T> v = lv[L], c = lv[L + 1];
+        const int half = ((v.Nx + 1) >> 1) * v.Ny * v.Nz;
+        for (int sw = 0; sw < nsw; sw++) {
+            for (int t = tid; t < half; t += TAIL_BLOCK) rb_node(v, v.b, v.x, 1, t, c, sw == 0);
+            __syncthreads();
+            for (int t = tid; t < half; t += TAIL_BLOCK) rb_node(v, v.b, v.x, 0, t, v, 0);
+            __syncthreads();
+        }
+    }
+}
+template <class T> void mg_tail(hipStream_t s, const MGLevel<T> *d_levels, int nt, const double *inv, int nsw) {
+    k_tail<T><<<1, TAIL_BLOCK, 0, s>>>(d_levels, nt, inv, nsw);
 }
 
 // x += pw * x_parent (semiconductor_3d.c:2038-2048, MG_ALPHA = 1)
@@ -684,6 +776,13 @@ template <class T> void mg_coarse_solve(hipStream_t s, const MGLevel<T> &v, cons
     k_coarse_solve<T><<<1, 64, 0, s>>>(v.N, inv, b, x);
 }
 
+__global__ void k_copy_bw(const double *a, double *b, long n) {
+    long k = (long)blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k < n) b[k] = a[k];
+}
+void sd_copy_bw(hipStream_t s, const double *a, double *b, long n) {
+    k_copy_bw<<<(int)((n + SD_BLOCK - 1) / SD_BLOCK), SD_BLOCK, 0, s>>>(a, b, n);
+}
 __global__ void k_to_float(const double *a, float *b, int n) {
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
     if (k < n) b[k] = (float)a[k];
@@ -710,6 +809,8 @@ void mg_sten_to_float(hipStream_t s, Sten7<double> A, Sten7<float> B, int n) {
     template void mg_rbgs<T>(hipStream_t, const MGLevel<T> &, const T *, T *, int, int, const MGLevel<T> *);                         \
     template void mg_resid_restrict<T>(hipStream_t, const MGLevel<T> &, const MGLevel<T> &, const T *, const T *, int); \
     template int mg_line<T>(hipStream_t, const MGLevel<T> &, const T *, T *, int, int, int, const MGLevel<T> *); \
+    template void mg_line_factor<T>(hipStream_t, const MGLevel<T> &, int); \
+    template void mg_tail<T>(hipStream_t, const MGLevel<T> *, int, const double *, int); \
     template void mg_prolong<T>(hipStream_t, const MGLevel<T> &, const MGLevel<T> &, T *);                       \
     template void mg_coarse_factor<T>(hipStream_t, const MGLevel<T> &, double *);                                \
     template void mg_coarse_solve<T>(hipStream_t, const MGLevel<T> &, const double *, const T *, T *);           \

@@ -33,6 +33,7 @@ template <class T> struct MGLevel {
     T *pw;               // prolongation weight of each node to its parent
     double *ph;          // Slotboom reference potential / VT (coarse levels)
     T *x, *b;            // level vectors
+    T *lu[3][3];         // line LU factors per axis: a/w, 1/w, c/w (nullptr: axis not factored)
 };
 
 // vector / reduction launchers (stream-ordered, no host sync)
@@ -76,6 +77,10 @@ template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, 
                                 const MGLevel<T> *c = nullptr);
 template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c,
                                           const T *b, const T *x, int skip);
+// levels d_levels[0..nt-1] (device copies; the last one the coarsest) in one workgroup
+template <class T> void mg_tail(hipStream_t s, const MGLevel<T> *d_levels, int nt, const double *inv, int nsw);
+// LU factors of the lines along axis ax (once per multigrid setup)
+template <class T> void mg_line_factor(hipStream_t s, const MGLevel<T> &v, int ax);
 // zebra line Gauss-Seidel along axis ax on lines of one colour (0 = cannot: line too long)
 template <class T> int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero,
                                const MGLevel<T> *c = nullptr);
@@ -83,6 +88,8 @@ template <class T> void mg_prolong(hipStream_t s, const MGLevel<T> &f, const MGL
 template <class T> void mg_coarse_factor(hipStream_t s, const MGLevel<T> &v, double *inv);
 template <class T> void mg_coarse_solve(hipStream_t s, const MGLevel<T> &v, const double *inv, const T *b, T *x);
 template <class T> void mg_set_pw_one(hipStream_t s, MGLevel<T> &v);
+extern int g_line_nl_max;
+void sd_copy_bw(hipStream_t s, const double *a, double *b, long n);   // plain copy (bandwidth reference)
 void mg_to_float(hipStream_t s, const double *a, float *b, int n);
 void mg_to_double(hipStream_t s, const float *a, double *b, int n);
 void mg_sten_to_float(hipStream_t s, Sten7<double> A, Sten7<float> B, int n);

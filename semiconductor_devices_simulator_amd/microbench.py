@@ -1,0 +1,63 @@
+"""Kernel microbenchmark of the HIP core on a solved device.
+
+Solves a workload device at one bias point, then times back-to-back launches
+of one kernel on the solver stream (HIP events, sdx_microbench) and prints
+ms/launch with its algorithmic bytes and GB/s.  Used for the roofline and
+rocprofv3 counter runs:
+
+    python -m semiconductor_devices_simulator_amd.microbench --mesh 96 64 64 \\
+        --precision 1 --kernel line_y --iters 200
+
+kernels: line_x line_y line_z (one colour of a level-0 zebra line
+Gauss-Seidel sweep), rbgs (one colour of a point red-black sweep), spmv
+(Krylov SpMV fused with a dot), restrict (residual + restriction level 0 to
+1), vcycle, copy (N doubles, a bandwidth reference), poisson (Poisson
+assembly), cont (continuity assembly).
+"""
+import argparse
+import json
+import sys
+
+from . import core, devices
+
+KERNELS = ["line_x", "line_y", "line_z", "rbgs", "spmv", "restrict", "vcycle", "copy", "poisson", "cont"]
+
+
+def algorithmic_bytes(kernel, N, precision):
+    w = 4 if precision == 1 else 8
+    return {
+        "line_x": N / 2 * 10 * w, "line_y": N / 2 * 10 * w, "line_z": N / 2 * 10 * w,
+        "rbgs": N / 2 * 10 * w, "spmv": N * 10 * 8, "restrict": N / 2 * 9 * w + N / 8 * w,
+        "vcycle": None, "copy": N * 16, "poisson": N * 161, "cont": N * 190,
+    }[kernel]
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--mesh", type=int, nargs=3, default=[96, 64, 64])
+    ap.add_argument("--precision", type=int, default=1)
+    ap.add_argument("--kernel", default="line_y", choices=KERNELS + ["all"])
+    ap.add_argument("--iters", type=int, default=200)
+    a = ap.parse_args(argv)
+    lib = core.load_core()
+    spec = devices.mosfet(*a.mesh)
+    cids = devices.load_device(lib, spec)
+    lib.sdx_set_precision(a.precision)
+    if not lib.api3_solve():
+        raise RuntimeError("equilibrium solve failed")
+    for c, v in ((1, 1.0), (2, 0.6)):
+        lib.api3_set_contact_V(cids[c], v)
+    if not lib.api3_resolve():
+        raise RuntimeError("bias point solve failed")
+    names = KERNELS if a.kernel == "all" else [a.kernel]
+    for name in names:
+        ms = lib.sdx_microbench(KERNELS.index(name), a.iters)
+        by = algorithmic_bytes(name, spec.N, a.precision)
+        out = dict(kernel=name, mesh=a.mesh, precision=a.precision, us_per_launch=round(ms * 1e3, 3),
+                   bytes_per_launch=by, GBps=None if by is None else round(by / ms / 1e6, 1))
+        print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
