@@ -29,7 +29,9 @@ int  sdx_ktimer_read(double *out);
 
 /* kernel microbenchmark on the solved device: ms per launch of `what`
    (0/1/2 line sweep x/y/z, 3 point GS sweep, 4 SpMV+dot, 5 residual+restriction,
-   6 V-cycle, 7 copy of N doubles, 8 Poisson assembly, 9 continuity assembly) */
+   6 V-cycle, 7 copy of N doubles, 8 Poisson assembly, 9 continuity assembly,
+   10 copy of 8M doubles, 11 preconditioner application as the solvers run it,
+   12 dot product read back by the host, timed by the host clock) */
 double sdx_microbench(int what, int iters);
 void   sdx_tune(int what, int value);      /* 0: lines per line-smoother workgroup, 1: line-smoothed levels,
                                               2: tail sweeps */

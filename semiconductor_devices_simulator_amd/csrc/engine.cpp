@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <algorithm>
 #include <queue>
+#include <unordered_map>
 
 Engine G;
 volatile int g_stop = 0;
@@ -60,12 +61,26 @@ static void rd(int a, int b) {
 }
 static double S(int slot) { return G.h_scal[slot]; }
 
+// Captured preconditioner launch sequences (see precond): they bake in the
+// hierarchy's pointers, shapes and smoother choices, so every change of
+// those drops them.
+struct VGraph { int prec, off; const double *r; double *z; hipGraphExec_t ex; };
+static std::vector<VGraph> g_vgraphs;
+static void vgraph_clear() {
+    for (auto &v : g_vgraphs) hipGraphExecDestroy(v.ex);
+    g_vgraphs.clear();
+}
+
 template <class T> static void hier_free(Hier<T> &H) { H.lev.clear(); H.inv = nullptr; H.alloc = false; }
 
 void eng_free() {
     if (G.st) hipStreamSynchronize(G.st);
+    vgraph_clear();
     for (void *p : G.dev_allocs) hipFree(p);
     G.dev_allocs.clear();
+    if (G.d_mrec) hipFree(G.d_mrec);
+    G.d_mrec = nullptr;
+    G.mrec_cap = 0;
     hier_free(G.H64);
     hier_free(G.H32);
     G.d_b32 = G.d_x32 = nullptr;
@@ -78,6 +93,7 @@ void eng_free() {
 // the coarse end (levels of at most G.tail_n nodes, not level 0) runs as one
 // single-workgroup kernel; its level descriptors are copied to the device
 template <class T> static void hier_tail(Hier<T> &H) {
+    vgraph_clear();
     H.tail = -1;
     int nl = (int)H.lev.size();
     for (int L = 1; L < nl; L++) if (H.lev[L].N <= G.tail_n) { H.tail = L; break; }
@@ -137,6 +153,17 @@ int eng_alloc(int Nx, int Ny, int Nz) {
     eng_free();
     g_err = 0;
     if (!G.st) {
+        // The solvers read a few device scalars back after most reductions,
+        // so the host's wait latency is on the critical path: wait by
+        // spinning unless SEMISIM_SYNC asks for yield, blocking or auto.
+        // Applies only before the process's HIP context exists.
+        const char *e = getenv("SEMISIM_SYNC");
+        unsigned fl = hipDeviceScheduleSpin;
+        if (e && !strcmp(e, "yield")) fl = hipDeviceScheduleYield;
+        if (e && !strcmp(e, "blocking")) fl = hipDeviceScheduleBlockingSync;
+        if (e && !strcmp(e, "auto")) fl = hipDeviceScheduleAuto;
+        (void)hipSetDeviceFlags(fl);
+        (void)hipGetLastError();
         if (hipStreamCreateWithFlags(&G.st, hipStreamNonBlocking) != hipSuccess) return 0;
     }
     size_t N = (size_t)Nx * Ny * Nz;
@@ -145,6 +172,7 @@ int eng_alloc(int Nx, int Ny, int Nz) {
     G.d_geo = dalloc<double>(2 * (Nx + Ny + Nz));
     G.d_kind = dalloc<unsigned char>(N); G.d_mat = dalloc<unsigned char>(N); G.d_flag = dalloc<unsigned char>(N);
     G.d_con_mask = dalloc<int>(N); G.d_gate_mask = dalloc<int>(N); G.d_changed = dalloc<int>(4);
+    G.d_mlist = dalloc<int>(N);
     double **nd[] = {&G.d_Nd, &G.d_Na, &G.d_Nf, &G.d_eps, &G.d_bgn, &G.d_robA, &G.d_lncK, &G.d_lnvK, &G.d_niK,
                      &G.d_psi, &G.d_phin, &G.d_phip, &G.d_n, &G.d_p, &G.d_mun, &G.d_mup, &G.d_gii,
                      &G.A.c, &G.A.xm, &G.A.xp, &G.A.ym, &G.A.yp, &G.A.zm, &G.A.zp,
@@ -384,6 +412,9 @@ int eng_prepare() {
         else if (mode && !strcmp(mode, "lines1")) want = 1;
         int maxlen = std::max(G.Nx, std::max(G.Ny, G.Nz));
         if (maxlen > 1024) want = 0;
+        int same = G.n_axes == want;
+        for (int a = 0; a < want; a++) same = same && G.mg_axes[a] == ord[3 - want + a];
+        if (!same) vgraph_clear();
         G.n_axes = want;
         for (int a = 0; a < want; a++) G.mg_axes[a] = ord[3 - want + a];
     }
@@ -503,7 +534,8 @@ static void mg_setup() {
 // figures; off unless sdx_ktimer_enable is called.
 struct KTimer {
     int on = 0, kind = 0, every = 8;
-    long long count = 0, sampled = 0;
+    bool sampling = false;             // kind 0: the running V-cycle is timed
+    long long count = 0, sampled = 0, vcycles = 0;
     double total_ms = 0.;
     std::vector<hipEvent_t> pool, pend;
 };
@@ -528,7 +560,8 @@ static void kt_flush() {
 }
 static bool kt_begin(int kind) {
     if (!KT.on || KT.kind != kind) return false;
-    if ((KT.count++ % KT.every) != 0) return false;
+    KT.count++;
+    if (kind == 0 ? !KT.sampling : ((KT.count - 1) % KT.every) != 0) return false;
     hipEvent_t e = kt_event();
     HC(hipEventRecord(e, G.st));
     KT.pend.push_back(e);
@@ -598,8 +631,7 @@ template <class T> static void smooth_only_T(const T *b, T *x) {
     smooth_post<T>(H.lev[0], b, x, nullptr);
 }
 // z = M^-1 r
-static void precond(const double *r, double *z) {
-    G.n_vcycles++;
+static void precond_launch(const double *r, double *z) {
     if (G.precision == 1) {
         mg_to_float(G.st, r, G.d_b32, G.N);
         if (G.mg_off || G.H32.lev.size() < 2) smooth_only_T<float>(G.d_b32, G.d_x32);
@@ -609,6 +641,47 @@ static void precond(const double *r, double *z) {
         if (G.mg_off || G.H64.lev.size() < 2) smooth_only_T<double>(r, z);
         else vcycle_T<double>(0, r, z);
     }
+}
+// The V-cycle is 20-40 short kernels whose launch sequence is fixed between
+// hierarchy changes.  It is captured into a HIP graph once per (precision,
+// smoother-only mode, r, z) and replayed with one hipGraphLaunch, so the host
+// no longer paces the GPU kernel by kernel.  The graph holds kernels only (no
+// memset or copy nodes).  With the level-0 smoother timer on, every n-th
+// V-cycle runs as plain launches bracketed by HIP events.
+static void precond(const double *r, double *z) {
+    G.n_vcycles++;
+    bool sample = KT.on && KT.kind == 0 && (KT.vcycles++ % KT.every) == 0;
+    if (!G.use_graphs || sample) {
+        KT.sampling = sample;
+        precond_launch(r, z);
+        KT.sampling = false;
+        return;
+    }
+    for (auto &v : g_vgraphs)
+        if (v.prec == G.precision && v.off == G.mg_off && v.r == r && v.z == z) {
+            HC(hipGraphLaunch(v.ex, G.st));
+            return;
+        }
+    hipGraph_t gr = nullptr;
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        precond_launch(r, z);
+        e = hipStreamEndCapture(G.st, &gr);
+    }
+    if (e == hipSuccess && gr) e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    if (gr) hipGraphDestroy(gr);
+    if (e != hipSuccess || !ex) {
+        // not a numerical failure: report it and keep plain launches
+        fprintf(stderr, "semisim-hip: V-cycle graph capture failed (%s); using plain launches\n",
+                hipGetErrorString(e));
+        (void)hipGetLastError();
+        G.use_graphs = 0;
+        precond_launch(r, z);
+        return;
+    }
+    g_vgraphs.push_back(VGraph{G.precision, G.mg_off, r, z, ex});
+    HC(hipGraphLaunch(ex, G.st));
 }
 static inline void matvec(const double *x, double *y) {
     G.n_spmv++;
@@ -786,10 +859,13 @@ static int poisson_newton(double ntol, int maxit) {
 }
 
 // ------------------------------------------------------------ continuity
-// Floating majority regions (semiconductor_3d.c:3159-3228): the device finds
-// whether any majority block lacks a contact; only then the block balance
-// runs on the host.
-static int float_fix_host(int hole);
+// Floating majority regions (semiconductor_3d.c:3159-3228).  The device marks
+// majority nodes, spreads "anchored" from the contact neighbours along
+// majority connectivity, and lists the nodes left unanchored; only those are
+// gathered for the host, which labels their blocks in the reference's order
+// (ascending start node, neighbour order x-,x+,y-,y+,z-,z+) and solves each
+// block's balance in ln(lambda).
+static int float_fix_host(int hole, int m);
 static int float_fix(int hole) {
     DevView d = eng_view();
     hipStream_t s = G.st;
@@ -807,68 +883,77 @@ static int float_fix(int hole) {
         memcpy(&ch, &G.h_scal[S_TMP3], 4);
         if (!ch) break;
     }
-    ph_maj_count(s, G.N, G.d_flag, G.rb, S_TMP2);
-    rd(S_TMP2, S_TMP2 + 1);
-    if (S(S_TMP2) < 0.5) return 0;
-    return float_fix_host(hole);
+    HC(hipMemsetAsync(G.d_changed + 2, 0, 4, s));
+    ph_maj_list(s, G.N, G.d_flag, G.d_changed + 2, G.d_mlist, G.N);
+    HC(hipMemcpyAsync(&G.h_scal[S_TMP3], G.d_changed + 2, 4, hipMemcpyDeviceToHost, s));
+    eng_sync();
+    int m = 0;
+    memcpy(&m, &G.h_scal[S_TMP3], 4);
+    if (m <= 0 || g_err) return 0;
+    return float_fix_host(hole, m > G.N ? G.N : m);
 }
 
-// host block balance, only when a floating block exists
-static int float_fix_host(int hole) {
+// block balance of the m listed (unanchored majority) nodes
+static int float_fix_host(int hole, int m) {
     const int N = G.N;
-    std::vector<double> u(N), sc(N), oth(N), psi(N), fmu(6 * (size_t)N), car(N);
-    eng_download(u.data(), G.d_sol, N);
-    eng_download(sc.data(), G.d_sc, N);
-    eng_download(oth.data(), hole ? G.d_n : G.d_p, N);
-    eng_download(car.data(), hole ? G.d_p : G.d_n, N);
-    eng_download(psi.data(), G.d_psi, N);
-    eng_download(fmu.data(), G.d_fmu, 6 * (size_t)N);
-    std::vector<double> nn(N), pp(N);
-    eng_download(nn.data(), G.d_n, N);
-    eng_download(pp.data(), G.d_p, N);
-    auto dnet = [&](int k) { return fabs(G.Nd[k] - G.Na[k]); };
-    auto MAJ = [&](int k) {
-        double c = sc[k] * u[k];
-        return G.kind[k] == K_SEMI && !G.con_mask[k] && c >= oth[k] && c >= 0.5 * dnet(k) && sc[k] >= 0.5 * dnet(k) &&
-               c > 10. * G.niK[k];
-    };
-    std::vector<int> lab(N, 0), q(N);
+    hipStream_t s = G.st;
+    if ((size_t)m * SD_MAJ_REC > G.mrec_cap) {
+        if (G.d_mrec) HC(hipFree(G.d_mrec));
+        G.mrec_cap = (size_t)m * SD_MAJ_REC;
+        if (hipMalloc(&G.d_mrec, G.mrec_cap * 8) != hipSuccess) { G.d_mrec = nullptr; G.mrec_cap = 0; g_err = 1; return 0; }
+    }
+    std::vector<int> list(m);
+    std::vector<double> rec((size_t)m * SD_MAJ_REC);
+    ph_maj_gather(s, eng_view(), hole, G.d_sol, G.d_sc, G.d_mlist, m, G.d_mrec);
+    HC(hipMemcpyAsync(list.data(), G.d_mlist, (size_t)m * 4, hipMemcpyDeviceToHost, s));
+    HC(hipMemcpyAsync(rec.data(), G.d_mrec, rec.size() * 8, hipMemcpyDeviceToHost, s));
+    eng_sync();
+    if (g_err) return 0;
+    std::unordered_map<int, int> pos;          // node -> row of list/rec
+    pos.reserve((size_t)m * 2);
+    for (int t = 0; t < m; t++) pos[list[t]] = t;
+    std::vector<int> order(list);
+    std::sort(order.begin(), order.end());
+    std::vector<int> lab(m, 0), q(m);
+    std::vector<double> lamv(m, 1.0);
+    auto R = [&](int k) { return &rec[(size_t)pos[k] * SD_MAJ_REC]; };
     int nfix = 0, blk = 0;
-    for (int k0 = 0; k0 < N; k0++) {
-        if (lab[k0] || !MAJ(k0)) continue;
+    for (int k0 : order) {
+        int t0 = pos[k0];
+        if (lab[t0]) continue;
         blk++;
         int head = 0, tail = 0;
         q[tail++] = k0;
-        lab[k0] = blk;
-        int anch = 0;
+        lab[t0] = blk;
         while (head < tail) {
             int k = q[head++], i, j, l;
             host_unidx(k, i, j, l);
             for (int dir = 0; dir < 6; dir++) {
                 int kb; double h, ar;
-                if (!host_nbr(i, j, l, k, dir, kb, h, ar) || G.kind[kb] != K_SEMI) continue;
-                if (G.con_mask[kb]) { anch = 1; continue; }
-                if (!lab[kb] && MAJ(kb)) { lab[kb] = blk; q[tail++] = kb; }
+                if (!host_nbr(i, j, l, k, dir, kb, h, ar) || G.kind[kb] != K_SEMI || G.con_mask[kb]) continue;
+                auto it = pos.find(kb);
+                if (it != pos.end() && !lab[it->second]) { lab[it->second] = blk; q[tail++] = kb; }
             }
         }
-        if (anch) continue;
         double Aout = 0., Bin = 0.;
         for (int t = 0; t < tail; t++) {
             int k = q[t], i, j, l;
             host_unidx(k, i, j, l);
+            const double *r = R(k);
             const HostMat &ma = G.mats[G.mat8[k]];
             double VT = ma.VT;
             for (int dir = 0; dir < 6; dir++) {
                 int kb; double h, ar;
-                if (!host_nbr(i, j, l, k, dir, kb, h, ar) || G.kind[kb] != K_SEMI || lab[kb] == blk) continue;
+                if (!host_nbr(i, j, l, k, dir, kb, h, ar) || G.kind[kb] != K_SEMI) continue;
+                auto it = pos.find(kb);
+                if (it != pos.end() && lab[it->second] == blk) continue;
+                const double *nb3 = r + 4 + 3 * dir;
                 double dl = hole ? -(G.lnvK[kb] - G.lnvK[k]) : (G.lncK[kb] - G.lncK[k]);
-                double D = (psi[kb] - psi[k]) / VT + dl;
-                int kl = k < kb ? k : kb;
-                double gg = ar * fmu[(size_t)((hole ? 3 : 0) + dir / 2) * N + kl] * VT / h;
+                double D = (nb3[0] - r[3]) / VT + dl;
+                double gg = ar * nb3[1] * VT / h;
                 double self = hole ? gg * host_B(D) : gg * host_B(-D), nb = hole ? gg * host_B(-D) : gg * host_B(D);
-                double cb = G.con_mask[kb] ? car[kb] : sc[kb] * u[kb];
-                Aout += self * sc[k] * u[k];
-                Bin += nb * cb;
+                Aout += self * r[1] * r[0];
+                Bin += nb * nb3[2];
             }
         }
         double sl = 0.;
@@ -877,12 +962,13 @@ static int float_fix_host(int hole) {
             for (int t = 0; t < tail; t++) {
                 int k = q[t], i, j, l;
                 host_unidx(k, i, j, l);
-                const HostMat &m = G.mats[G.mat8[k]];
-                double c = lam * sc[k] * u[k], R, dRn, dRp;
-                double tf = ((G.models & 2) && m.tau_Nref > 0.) ? 1. / (1. + (G.Nd[k] + G.Na[k]) / m.tau_Nref) : 1.;
-                host_recomb(m, G.niK[k], tf, hole ? oth[k] : c, hole ? c : oth[k], &R, &dRn, &dRp);
+                const double *r = R(k);
+                const HostMat &mt = G.mats[G.mat8[k]];
+                double c = lam * r[1] * r[0], Rr, dRn, dRp;
+                double tf = ((G.models & 2) && mt.tau_Nref > 0.) ? 1. / (1. + (G.Nd[k] + G.Na[k]) / mt.tau_Nref) : 1.;
+                host_recomb(mt, G.niK[k], tf, hole ? r[2] : c, hole ? c : r[2], &Rr, &dRn, &dRp);
                 double vol = G.Lx[i] * G.Ly[j] * G.Lz[l];
-                Gv += vol * R;
+                Gv += vol * Rr;
                 dG += vol * (hole ? dRp : dRn) * c;
             }
             if (!(dG > 0.)) break;
@@ -893,12 +979,19 @@ static int float_fix_host(int hole) {
         }
         if (sl == sl && fabs(sl) > 1e-15 && fabs(sl) < 200.) {
             double lam = exp(sl);
-            for (int t = 0; t < tail; t++) u[q[t]] *= lam;
+            for (int t = 0; t < tail; t++) lamv[pos[q[t]]] = lam;
             nfix++;
         }
     }
-    (void)nn; (void)pp;
-    if (nfix) eng_upload(G.d_sol, u.data(), N);
+    if (nfix) {
+        double *d_lam = G.d_wk[6];             // free after the Krylov solve
+        HC(hipMemcpyAsync(d_lam, lamv.data(), (size_t)m * 8, hipMemcpyHostToDevice, s));
+        ph_scale_list(s, G.d_sol, G.d_mlist, d_lam, m);
+        eng_sync();                             // lamv is pageable host memory
+    }
+    (void)N;
+    if (G.verbose > 2) fprintf(stderr, "    float_fix(%c): %d unanchored nodes, %d blocks, %d rescaled\n",
+                               hole ? 'p' : 'n', m, blk, nfix);
     return nfix;
 }
 
@@ -1343,6 +1436,7 @@ void eng_read_env() {
     if ((e = getenv("SEMISIM_MG_LINELEVELS"))) G.line_levels = atoi(e);
     if ((e = getenv("SEMISIM_MG_TAILN"))) G.tail_n = atoi(e);
     if ((e = getenv("SEMISIM_MG_TAILSW"))) { int q = atoi(e); G.tail_sweeps = q > 0 ? q : 1; }
+    G.use_graphs = (e = getenv("SEMISIM_GRAPHS")) ? atoi(e) != 0 : 0;
 }
 int eng_aa_depth() { return G.aa_m < AA_DEPTH ? G.aa_m : AA_DEPTH; }
 void eng_set_predictor(int on) { PRED_ON = on != 0; }
@@ -1387,6 +1481,12 @@ double sdx_microbench(int what, int iters) {
     hipEvent_t e0, e1;
     HC(hipEventCreate(&e0)); HC(hipEventCreate(&e1));
     DevView d = eng_view();
+    // what 10: copy of 8M doubles (64 MB each way), larger than the summed L2
+    // of all XCDs, as the calibration reference of the HBM byte counters
+    const long BIG_N = 8L << 20;
+    double *big = nullptr;
+    if (what == 10) HC(hipMalloc(&big, 2 * BIG_N * sizeof(double)));
+    if (what == 10) HC(hipMemsetAsync(big, 0, 2 * BIG_N * sizeof(double), s));
     auto one = [&]() {
         if (G.precision == 1) {
             MGLevel<float> &v = G.H32.lev[0];
@@ -1410,22 +1510,30 @@ double sdx_microbench(int what, int iters) {
         switch (what) {
         case 4: sd_spmv_dots(s, d.g, G.A, G.d_wk[2], G.d_wk[3], G.d_wk[4], nullptr, 1, G.rb, S_TMP1, -1); break;
         case 7: sd_copy_bw(s, G.d_wk[4], G.d_wk[5], G.N); break;
+        case 10: sd_copy_bw(s, big, big + BIG_N, BIG_N); break;
+        case 11: precond(G.d_wk[0], G.d_wk[1]); break;
+        case 12: sd_dot(s, G.d_wk[4], G.d_wk[4], G.N, G.rb, S_TMP1); rd(S_TMP1, S_TMP1 + 1); break;
         case 8: ph_poisson(s, d, G.d_psi, 1, G.d_wk[5], G.A, G.d_sc, G.rb, S_TMP1); break;
         case 9: ph_cont(s, d, 0, G.A, G.d_wk[5], G.d_wk[6], G.d_dsc); break;
         default: break;
         }
     };
     for (int w = 0; w < 3; w++) one();
+    HC(hipStreamSynchronize(s));
+    double h0 = wtime();
     HC(hipEventRecord(e0, s));
     for (int it = 0; it < iters; it++) one();
     HC(hipEventRecord(e1, s));
     HC(hipEventSynchronize(e1));
     float ms = 0.f;
     HC(hipEventElapsedTime(&ms, e0, e1));
+    if (what == 12) ms = (float)((wtime() - h0) * 1e3);   // host round trip: wall clock
     HC(hipEventDestroy(e0)); HC(hipEventDestroy(e1));
+    if (big) HC(hipFree(big));
     return (double)ms / iters;
 }
 void sdx_tune(int what, int value) {
+    vgraph_clear();
     if (what == 0) g_line_nl_max = value;
     if (what == 1) G.line_levels = value;
     if (what == 2) G.tail_sweeps = value > 0 ? value : 1;

@@ -67,6 +67,9 @@ struct Engine {
     double *d_geo = nullptr;           // xs|ys|zs|Lx|Ly|Lz
     unsigned char *d_kind = nullptr, *d_mat = nullptr, *d_flag = nullptr;
     int *d_con_mask = nullptr, *d_gate_mask = nullptr, *d_changed = nullptr;
+    int *d_mlist = nullptr;            // unanchored majority nodes (floating blocks)
+    double *d_mrec = nullptr;          // their gathered records, grown on demand
+    size_t mrec_cap = 0;
     double *d_Nd, *d_Na, *d_Nf, *d_eps, *d_bgn, *d_robA, *d_lncK, *d_lnvK, *d_niK;
     double *d_psi, *d_phin, *d_phip, *d_n, *d_p, *d_mun, *d_mup, *d_fmu, *d_gii;
     Sten7<double> A;
@@ -95,6 +98,7 @@ struct Engine {
     int line_levels = 2;                      // levels smoothed with lines (the finest ones)
     int tail_sweeps = 1;                      // red-black sweeps per side in the tail kernel
     int tail_n = 1024;                        // levels of at most this many nodes run in the tail kernel
+    int use_graphs = 0;                       // replay the captured V-cycle (SEMISIM_GRAPHS=1)
 
     // Anderson
     int aa_m = 12, aa_n = 0, aa_pos = 0, aa_have = 0;

@@ -427,6 +427,51 @@ void ph_maj_propagate(hipStream_t s, const DevView &d, unsigned char *flag, int 
     int n = axis == 0 ? g.Ny * g.Nz : (axis == 1 ? g.Nx * g.Nz : g.Nx * g.Ny);
     k_maj_prop<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(g, flag, axis, changed);
 }
+// indices of the majority nodes whose block reached no contact (flag 1);
+// the order of the list is arbitrary, the host sorts it
+__global__ void k_maj_list(int N, const unsigned char *flag, int *cnt, int *list, int cap) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= N || flag[k] != 1) return;
+    int pos = atomicAdd(cnt, 1);
+    if (pos < cap) list[pos] = k;
+}
+void ph_maj_list(hipStream_t s, int N, const unsigned char *flag, int *cnt, int *list, int cap) {
+    k_maj_list<<<sd_nblocks(N), SD_BLOCK, 0, s>>>(N, flag, cnt, list, cap);
+}
+// per listed node the values the block balance reads: u, s, the other
+// carrier, psi, then per direction psi of the neighbour, the face mobility
+// and the neighbour's carrier (its contact value on a contact node, else s*u)
+__global__ void k_maj_gather(DevView d, int hole, const double *u, const double *s, const int *list, int m,
+                             double *out) {
+    const Geo g = d.g;
+    int t = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (t >= m) return;
+    int k = list[t], i, j, l;
+    sd_unidx(g, k, i, j, l);
+    const double *oth = hole ? d.n : d.p, *car = hole ? d.p : d.n;
+    double *o = out + (size_t)t * SD_MAJ_REC;
+    o[0] = u[k]; o[1] = s[k]; o[2] = oth[k]; o[3] = d.psi[k];
+    for (int dir = 0; dir < 6; dir++) {
+        int kb; double h, ar;
+        double *q = o + 4 + 3 * dir;
+        if (!sd_nbr(g, i, j, l, k, dir, kb, h, ar)) { q[0] = q[1] = q[2] = 0.; continue; }
+        int kl = k < kb ? k : kb;
+        q[0] = d.psi[kb];
+        q[1] = d.fmu[(size_t)((hole ? 3 : 0) + dir / 2) * g.N + kl];
+        q[2] = d.con_mask[kb] ? car[kb] : s[kb] * u[kb];
+    }
+}
+void ph_maj_gather(hipStream_t s, const DevView &d, int hole, const double *u, const double *sc, const int *list,
+                   int m, double *out) {
+    k_maj_gather<<<sd_nblocks(m), SD_BLOCK, 0, s>>>(d, hole, u, sc, list, m, out);
+}
+__global__ void k_scale_list(double *u, const int *list, const double *lam, int m) {
+    int t = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (t < m) u[list[t]] *= lam[t];
+}
+void ph_scale_list(hipStream_t s, double *u, const int *list, const double *lam, int m) {
+    k_scale_list<<<sd_nblocks(m), SD_BLOCK, 0, s>>>(u, list, lam, m);
+}
 __global__ void __launch_bounds__(SD_BLOCK) k_maj_count(int N, const unsigned char *flag, RedBuf rb, int slot) {
     double acc[1] = {0.0};
     for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < N; k += gridDim.x * SD_BLOCK) acc[0] += flag[k] == 1;

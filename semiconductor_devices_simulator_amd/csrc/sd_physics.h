@@ -33,6 +33,11 @@ void ph_update_psi(hipStream_t s, int N, double *phi, const double *dx, double l
 void ph_maj_flags(hipStream_t s, const DevView &d, int hole, const double *u, const double *sc, unsigned char *flag);
 void ph_maj_propagate(hipStream_t s, const DevView &d, unsigned char *flag, int axis, int *changed);
 void ph_maj_count(hipStream_t s, int N, const unsigned char *flag, RedBuf rb, int slot);
+#define SD_MAJ_REC 22   // doubles per node record of ph_maj_gather
+void ph_maj_list(hipStream_t s, int N, const unsigned char *flag, int *cnt, int *list, int cap);
+void ph_maj_gather(hipStream_t s, const DevView &d, int hole, const double *u, const double *sc, const int *list,
+                   int m, double *out);
+void ph_scale_list(hipStream_t s, double *u, const int *list, const double *lam, int m);
 
 // Anderson mixing of x = (psi, phi_n, phi_p)
 void ph_aa_diff(hipStream_t s, const DevView &d, const double *x0, const double *x1, const double *x2,

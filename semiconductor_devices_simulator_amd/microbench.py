@@ -12,7 +12,12 @@ kernels: line_x line_y line_z (one colour of a level-0 zebra line
 Gauss-Seidel sweep), rbgs (one colour of a point red-black sweep), spmv
 (Krylov SpMV fused with a dot), restrict (residual + restriction level 0 to
 1), vcycle, copy (N doubles, a bandwidth reference), poisson (Poisson
-assembly), cont (continuity assembly).
+assembly), cont (continuity assembly), copy_big (8M doubles, 64 MB each way:
+larger than the L2 of all XCDs together, the calibration reference of the
+rocprofv3 FETCH_SIZE / WRITE_SIZE counters), precond (the preconditioner as
+the Krylov solvers call it: precision conversions and the V-cycle, replayed
+from its captured HIP graph), roundtrip (a dot product whose value the host
+reads back, host wall clock per round trip).
 """
 import argparse
 import json
@@ -20,7 +25,7 @@ import sys
 
 from . import core, devices
 
-KERNELS = ["line_x", "line_y", "line_z", "rbgs", "spmv", "restrict", "vcycle", "copy", "poisson", "cont"]
+KERNELS = ["line_x", "line_y", "line_z", "rbgs", "spmv", "restrict", "vcycle", "copy", "poisson", "cont", "copy_big", "precond", "roundtrip"]
 
 
 def algorithmic_bytes(kernel, N, precision):
@@ -28,7 +33,7 @@ def algorithmic_bytes(kernel, N, precision):
     return {
         "line_x": N / 2 * 10 * w, "line_y": N / 2 * 10 * w, "line_z": N / 2 * 10 * w,
         "rbgs": N / 2 * 10 * w, "spmv": N * 10 * 8, "restrict": N / 2 * 9 * w + N / 8 * w,
-        "vcycle": None, "copy": N * 16, "poisson": N * 161, "cont": N * 190,
+        "vcycle": None, "copy": N * 16, "poisson": N * 161, "cont": N * 190, "copy_big": (8 << 20) * 16, "precond": None, "roundtrip": None,
     }[kernel]
 
 
