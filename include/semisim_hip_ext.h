@@ -33,8 +33,9 @@ int  sdx_ktimer_read(double *out);
    10 copy of 8M doubles, 11 preconditioner application as the solvers run it,
    12 dot product read back by the host, timed by the host clock) */
 double sdx_microbench(int what, int iters);
-void   sdx_tune(int what, int value);      /* 0: lines per line-smoother workgroup, 1: line-smoothed levels,
-                                              2: tail sweeps */
+void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS line kernel, 1: line-smoothed
+                                              levels, 2: tail sweeps, 3: tail size, 4: wavefront-scan line
+                                              kernel on (1) or off (0) */
 
 /* ---- kernel-level hooks on the prepared device (tests) ----
    A7: 7 blocks of N doubles (c, xm, xp, ym, yp, zm, zp). */

@@ -594,12 +594,148 @@ __global__ void __launch_bounds__(SD_BLOCK) k_line_gs(MGLevel<T> v, const T *__r
         if (e < neq && td < g.Ntd) x[td * g.s_td + q * g.s_ax + p3 * g.s_3] = sw[e];
     }
 }
-int g_line_nl_max = 8;       // lines per workgroup cap (tuning)
+
+// The same zebra sweep with the two recurrences as wavefront scans, for lines
+// of up to 64*P elements.  Each wavefront owns LW = floor(64 P / n) whole
+// lines packed end to end, P consecutive line elements per lane.  The forward
+// recurrence y_e = r_e iw_e - ai_e y_{e-1} composes the affine maps
+// y -> -ai_e y + r_e iw_e: every lane composes its P maps, a 6-step shuffle
+// scan composes the lanes' maps across the wavefront, and every lane replays
+// its P elements from the value that enters it.  The back substitution
+// x_e = y_e - cp_e x_{e+1} runs the same way from the other end.  ai = 0 at
+// the first element of a line and cp = 0 at its last cut the chain between
+// packed lines, so the scan needs no segment flags.  The right-hand sides are
+// gathered as in k_line_gs (consecutive threads along the line for x-lines,
+// across lines otherwise) and staged in LDS.
+#define LS_WAVES 8
+#define LS_THREADS (64 * LS_WAVES)
+template <class T, int P>
+__global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T *__restrict__ b, T *__restrict__ x,
+                                                          int ax, int color, int zero, MGLevel<T> c, int pro, int LW) {
+    __shared__ T s_r[LS_WAVES * 64 * P], s_a[LS_WAVES * 64 * P], s_c[LS_WAVES * 64 * P];
+    const int Nx = v.Nx, Ny = v.Ny, Nz = v.Nz, NxNy = Nx * Ny;
+    const LineGeom g = line_geom(Nx, Ny, Nz, ax);
+    const int n = g.n;
+    const int NLW = LS_WAVES * LW;                   // lines per workgroup
+    const int ntile = (((g.Ntd + 1) >> 1) + NLW - 1) / NLW;
+    const int p3 = blockIdx.x / ntile, h0 = (blockIdx.x - p3 * ntile) * NLW;
+    const int sh = (p3 + color) & 1;
+    const int E = NLW * n, used = LW * n, pad = 64 * P - used;
+    const T *__restrict__ AI = v.lu[ax][0];
+    const T *__restrict__ IW = v.lu[ax][1];
+    const T *__restrict__ CP = v.lu[ax][2];
+    // slots past the packed lines of each wavefront carry the identity
+    for (int t = threadIdx.x; t < LS_WAVES * pad; t += LS_THREADS) {
+        int q = (t / pad) * 64 * P + used + t % pad;
+        s_r[q] = 0; s_a[q] = 0; s_c[q] = 0;
+    }
+#pragma unroll
+    for (int m = 0; m < P; m++) {
+        const int e = threadIdx.x + LS_THREADS * m;
+        if (e >= E) break;
+        int L, q;
+        if (ax == 0) { L = e / n; q = e - L * n; } else { q = e / NLW; L = e - q * NLW; }
+        const int td = 2 * (h0 + L) + sh;
+        T r = 0, ai = 0, iw = 0, cp = 0;
+        if (td < g.Ntd) {
+            const int k = td * g.s_td + q * g.s_ax + p3 * g.s_3;
+            int i, j, l;
+            if (ax == 0) { i = q; j = td; l = p3; } else if (ax == 1) { i = td; j = q; l = p3; } else { i = td; j = p3; l = q; }
+            ai = AI[k]; iw = IW[k]; cp = CP[k];
+            T sr = b[k];
+            if (!zero) {
+                auto xv = [&](int ii, int jj, int ll, int kk) -> T {
+                    if (!pro) return x[kk];
+                    int I = v.cx ? ii >> 1 : ii, J = v.cy ? jj >> 1 : jj, LL = v.cz ? ll >> 1 : ll;
+                    return x[kk] + v.pw[kk] * c.x[I + c.Nx * (J + c.Ny * LL)];
+                };
+                if (ax != 0) {
+                    if (i > 0) sr -= v.A.xm[k] * xv(i - 1, j, l, k - 1);
+                    if (i < Nx - 1) sr -= v.A.xp[k] * xv(i + 1, j, l, k + 1);
+                }
+                if (ax != 1) {
+                    if (j > 0) sr -= v.A.ym[k] * xv(i, j - 1, l, k - Nx);
+                    if (j < Ny - 1) sr -= v.A.yp[k] * xv(i, j + 1, l, k + Nx);
+                }
+                if (ax != 2) {
+                    if (l > 0) sr -= v.A.zm[k] * xv(i, j, l - 1, k - NxNy);
+                    if (l < Nz - 1) sr -= v.A.zp[k] * xv(i, j, l + 1, k + NxNy);
+                }
+            }
+            r = sr;
+        }
+        const int q0 = (L / LW) * 64 * P + (L % LW) * n + q;
+        s_r[q0] = r * iw; s_a[q0] = ai; s_c[q0] = cp;
+    }
+    __syncthreads();
+    {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        T *R = s_r + w * 64 * P;
+        const T *Aa = s_a + w * 64 * P, *Cc = s_c + w * 64 * P;
+        T ra[P], aa[P], ca[P];
+#pragma unroll
+        for (int p = 0; p < P; p++) { ra[p] = R[lane * P + p]; aa[p] = Aa[lane * P + p]; ca[p] = Cc[lane * P + p]; }
+        T A = 1, B = 0;
+#pragma unroll
+        for (int p = 0; p < P; p++) { A = -aa[p] * A; B = ra[p] - aa[p] * B; }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            T Ap = __shfl_up(A, d, 64), Bp = __shfl_up(B, d, 64);
+            if (lane >= d) { B = B + A * Bp; A = A * Ap; }
+        }
+        T y = __shfl_up(B, 1, 64);
+        if (lane == 0) y = 0;
+#pragma unroll
+        for (int p = 0; p < P; p++) { y = ra[p] - aa[p] * y; ra[p] = y; }
+        A = 1; B = 0;
+#pragma unroll
+        for (int p = P - 1; p >= 0; p--) { A = -ca[p] * A; B = ra[p] - ca[p] * B; }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            T Al = __shfl_down(A, d, 64), Bl = __shfl_down(B, d, 64);
+            if (lane + d < 64) { B = B + A * Bl; A = A * Al; }
+        }
+        T xq = __shfl_down(B, 1, 64);
+        if (lane == 63) xq = 0;
+#pragma unroll
+        for (int p = P - 1; p >= 0; p--) { xq = ra[p] - ca[p] * xq; ra[p] = xq; }
+#pragma unroll
+        for (int p = 0; p < P; p++) R[lane * P + p] = ra[p];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < P; m++) {
+        const int e = threadIdx.x + LS_THREADS * m;
+        if (e >= E) break;
+        int L, q;
+        if (ax == 0) { L = e / n; q = e - L * n; } else { q = e / NLW; L = e - q * NLW; }
+        const int td = 2 * (h0 + L) + sh;
+        if (td < g.Ntd) x[td * g.s_td + q * g.s_ax + p3 * g.s_3] = s_r[(L / LW) * 64 * P + (L % LW) * n + q];
+    }
+}
+
+int g_line_nl_max = 8;       // lines per workgroup cap of k_line_gs (tuning)
+int g_line_scan = 1;         // 1: scan kernel for lines of up to 256 elements
 template <class T>
 int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero, const MGLevel<T> *c) {
     int n = ax == 0 ? v.Nx : (ax == 1 ? v.Ny : v.Nz);
     int Ntd = ax == 0 ? v.Ny : v.Nx, N3 = ax == 2 ? v.Ny : v.Nz;
-    if (n > LINE_MAXEQ || !v.lu[ax][0]) return 0;
+    if (!v.lu[ax][0]) return 0;
+    const int P = (n + 63) / 64;
+    if (g_line_scan && P <= 4) {
+        const int LW = 64 * P / n, NLW = LS_WAVES * LW;
+        const int ntile = ((Ntd + 1) / 2 + NLW - 1) / NLW;
+        MGLevel<T> cc = c ? *c : v;
+        const int pro = c != nullptr, nb = ntile * N3;
+        switch (P) {
+        case 1: k_line_scan<T, 1><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW); break;
+        case 2: k_line_scan<T, 2><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW); break;
+        case 3: k_line_scan<T, 3><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW); break;
+        default: k_line_scan<T, 4><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW); break;
+        }
+        return 1;
+    }
+    if (n > LINE_MAXEQ) return 0;
     int NL = LINE_MAXEQ / n;
     if (NL > g_line_nl_max) NL = g_line_nl_max;
     int half = (Ntd + 1) / 2;
@@ -616,37 +752,48 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
 // coarse row scale (the coarse rows are stored with unit diagonal).  Rows
 // solved exactly by the last smoothing sweep have no residual and are
 // skipped: skip 1 = point black nodes, 2+a = colour-1 lines along axis a.
+// One lane per fine node: the eight lanes of an aggregate are adjacent (lane
+// bits 0, 1, 2 = x, y, z offset in the aggregate), each evaluates its node's
+// weighted residual, and three xor shuffles sum the aggregate, so every lane
+// issues its loads at once.  t indexes lanes 8 K + sub; every lane of the
+// wavefront must call it (the shuffles), lanes past 8 c.N contribute nothing.
 template <class T>
-__device__ __forceinline__ void restrict_node(const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x,
-                                              int skip, int K) {
-    int cNxNy = c.Nx * c.Ny, LL = K / cNxNy, rr = K - LL * cNxNy, J = rr / c.Nx, I = rr - J * c.Nx;
-    Agg a = agg_range(f.Nx, f.Ny, f.Nz, f.cx, f.cy, f.cz, I, J, LL);
-    double acc = 0.0;
-    for (int l = a.l0; l <= a.l1; l++)
-        for (int j = a.j0; j <= a.j1; j++)
-            for (int i = a.i0; i <= a.i1; i++) {
-                if (skip == 1 && ((i + j + l) & 1)) continue;
-                if (skip >= 2) {
-                    int ax = skip - 2, par = ax == 0 ? (j + l) : (ax == 1 ? (i + l) : (i + j));
-                    if (par & 1) continue;
-                }
-                int k = i + f.Nx * (j + f.Ny * l);
-                double w = f.rs ? f.rs[k] : 1.0;
-                if (w == 0.0) continue;
+__device__ __forceinline__ void restrict8(const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x, int skip,
+                                          int t) {
+    const int K = t >> 3, sub = t & 7;
+    double v = 0.0;
+    if (K < c.N) {
+        const int cNxNy = c.Nx * c.Ny, LL = K / cNxNy, rr = K - LL * cNxNy, J = rr / c.Nx, I = rr - J * c.Nx;
+        const int di = sub & 1, dj = (sub >> 1) & 1, dl = sub >> 2;
+        const int i = f.cx ? 2 * I + di : I, j = f.cy ? 2 * J + dj : J, l = f.cz ? 2 * LL + dl : LL;
+        bool ok = (f.cx || !di) && (f.cy || !dj) && (f.cz || !dl) && i < f.Nx && j < f.Ny && l < f.Nz;
+        if (ok && skip == 1 && ((i + j + l) & 1)) ok = false;
+        if (ok && skip >= 2) {
+            const int ax = skip - 2, par = ax == 0 ? (j + l) : (ax == 1 ? (i + l) : (i + j));
+            if (par & 1) ok = false;
+        }
+        if (ok) {
+            const int k = i + f.Nx * (j + f.Ny * l);
+            const double w = f.rs ? f.rs[k] : 1.0;
+            if (w != 0.0) {
                 T r = b[k] - sten_apply<T, T>(f.A, x, k, i, j, l, f.Nx, f.Ny, f.Nz, f.A.c == nullptr);
-                acc += w * (double)r;
+                v = w * (double)r;
             }
-    c.b[K] = (T)(acc / c.rs[K]);
+        }
+    }
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    if (sub == 0 && K < c.N) c.b[K] = (T)(v / c.rs[K]);
 }
 template <class T>
-__global__ void __launch_bounds__(SD_BLOCK) k_resid_restrict(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x,
-                                                             int skip) {
-    int K = blockIdx.x * SD_BLOCK + threadIdx.x;
-    if (K < c.N) restrict_node(f, c, b, x, skip, K);
+__global__ void __launch_bounds__(SD_BLOCK) k_resid_restrict8(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x,
+                                                              int skip) {
+    restrict8(f, c, b, x, skip, blockIdx.x * SD_BLOCK + threadIdx.x);
 }
 template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x,
                                           int skip) {
-    k_resid_restrict<T><<<sd_nblocks(c.N), SD_BLOCK, 0, s>>>(f, c, b, x, skip);
+    k_resid_restrict8<T><<<sd_nblocks(8 * c.N), SD_BLOCK, 0, s>>>(f, c, b, x, skip);
 }
 
 // The coarse end of the V-cycle in one workgroup: every level from lv[0]
@@ -669,7 +816,7 @@ __global__ void __launch_bounds__(TAIL_BLOCK) k_tail(const MGLevel<T> *lv, int n
             __syncthreads();
         }
         const MGLevel<T> c = lv[L + 1];
-        for (int K = tid; K < c.N; K += TAIL_BLOCK) restrict_node(v, c, v.b, v.x, 1, K);
+        for (int t0 = 0; t0 < 8 * c.N; t0 += TAIL_BLOCK) restrict8(v, c, v.b, v.x, 1, t0 + tid);
         __syncthreads();
     }
     {

@@ -43,8 +43,13 @@ def main(argv=None):
     ap.add_argument("--precision", type=int, default=1)
     ap.add_argument("--kernel", default="line_y", choices=KERNELS + ["all"])
     ap.add_argument("--iters", type=int, default=200)
+    ap.add_argument("--tune", action="append", default=[],
+                    help="WHAT=VALUE passed to sdx_tune before the solve (4=0: the k_line_gs line kernel)")
     a = ap.parse_args(argv)
     lib = core.load_core()
+    for t in a.tune:
+        w, val = (int(z) for z in t.split("="))
+        lib.sdx_tune(w, val)
     spec = devices.mosfet(*a.mesh)
     cids = devices.load_device(lib, spec)
     lib.sdx_set_precision(a.precision)
