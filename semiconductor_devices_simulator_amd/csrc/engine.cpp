@@ -1536,6 +1536,7 @@ void sdx_tune(int what, int value) {
     vgraph_clear();
     if (what == 0) g_line_nl_max = value;
     if (what == 4) g_line_scan = value;
+    if (what == 5) g_line_slots = value >= 4 ? 4 : (value >= 2 ? 2 : 1);
     if (what == 1) G.line_levels = value;
     if (what == 2) G.tail_sweeps = value > 0 ? value : 1;
     if (what == 3) {

@@ -596,41 +596,43 @@ __global__ void __launch_bounds__(SD_BLOCK) k_line_gs(MGLevel<T> v, const T *__r
 }
 
 // The same zebra sweep with the two recurrences as wavefront scans, for lines
-// of up to 64*P elements.  Each wavefront owns LW = floor(64 P / n) whole
-// lines packed end to end, P consecutive line elements per lane.  The forward
-// recurrence y_e = r_e iw_e - ai_e y_{e-1} composes the affine maps
-// y -> -ai_e y + r_e iw_e: every lane composes its P maps, a 6-step shuffle
-// scan composes the lanes' maps across the wavefront, and every lane replays
-// its P elements from the value that enters it.  The back substitution
-// x_e = y_e - cp_e x_{e+1} runs the same way from the other end.  ai = 0 at
-// the first element of a line and cp = 0 at its last cut the chain between
-// packed lines, so the scan needs no segment flags.  The right-hand sides are
-// gathered as in k_line_gs (consecutive threads along the line for x-lines,
-// across lines otherwise) and staged in LDS.
+// of up to 64*P elements.  A wavefront owns S slots of 64*P elements; each
+// slot holds LW = floor(64 P / n) whole lines packed end to end, P consecutive
+// line elements per lane.  The forward recurrence
+// y_e = r_e iw_e - ai_e y_{e-1} composes the affine maps y -> -ai_e y + r_e iw_e:
+// every lane composes its P maps, a 6-step shuffle scan composes the lanes'
+// maps across the wavefront, and every lane replays its P elements from the
+// value that enters it.  The back substitution x_e = y_e - cp_e x_{e+1} runs
+// the same way from the other end.  ai = 0 at the first element of a line and
+// cp = 0 at its last cut the chain between packed lines, so the scan needs no
+// segment flags.  The right-hand sides are gathered as in k_line_gs and staged
+// in LDS: along the line for x-lines (contiguous), across the workgroup's
+// lines otherwise, where S > 1 widens the contiguous run each load covers.
 #define LS_WAVES 8
 #define LS_THREADS (64 * LS_WAVES)
-template <class T, int P>
+template <class T, int P, int S>
 __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T *__restrict__ b, T *__restrict__ x,
                                                           int ax, int color, int zero, MGLevel<T> c, int pro, int LW) {
-    __shared__ T s_r[LS_WAVES * 64 * P], s_a[LS_WAVES * 64 * P], s_c[LS_WAVES * 64 * P];
+    constexpr int SL = 64 * P;                       // elements per slot
+    __shared__ T s_r[LS_WAVES * S * SL], s_a[LS_WAVES * S * SL], s_c[LS_WAVES * S * SL];
     const int Nx = v.Nx, Ny = v.Ny, Nz = v.Nz, NxNy = Nx * Ny;
     const LineGeom g = line_geom(Nx, Ny, Nz, ax);
     const int n = g.n;
-    const int NLW = LS_WAVES * LW;                   // lines per workgroup
+    const int NLW = LS_WAVES * S * LW;               // lines per workgroup
     const int ntile = (((g.Ntd + 1) >> 1) + NLW - 1) / NLW;
     const int p3 = blockIdx.x / ntile, h0 = (blockIdx.x - p3 * ntile) * NLW;
     const int sh = (p3 + color) & 1;
-    const int E = NLW * n, used = LW * n, pad = 64 * P - used;
+    const int E = NLW * n, used = LW * n, pad = SL - used;
     const T *__restrict__ AI = v.lu[ax][0];
     const T *__restrict__ IW = v.lu[ax][1];
     const T *__restrict__ CP = v.lu[ax][2];
-    // slots past the packed lines of each wavefront carry the identity
-    for (int t = threadIdx.x; t < LS_WAVES * pad; t += LS_THREADS) {
-        int q = (t / pad) * 64 * P + used + t % pad;
+    // slot elements past the packed lines carry the identity
+    for (int t = threadIdx.x; t < LS_WAVES * S * pad; t += LS_THREADS) {
+        int q = (t / pad) * SL + used + t % pad;
         s_r[q] = 0; s_a[q] = 0; s_c[q] = 0;
     }
 #pragma unroll
-    for (int m = 0; m < P; m++) {
+    for (int m = 0; m < P * S; m++) {
         const int e = threadIdx.x + LS_THREADS * m;
         if (e >= E) break;
         int L, q;
@@ -664,14 +666,14 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
             }
             r = sr;
         }
-        const int q0 = (L / LW) * 64 * P + (L % LW) * n + q;
+        const int q0 = (L / LW) * SL + (L % LW) * n + q;
         s_r[q0] = r * iw; s_a[q0] = ai; s_c[q0] = cp;
     }
     __syncthreads();
-    {
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-        T *R = s_r + w * 64 * P;
-        const T *Aa = s_a + w * 64 * P, *Cc = s_c + w * 64 * P;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int si = 0; si < S; si++) {
+        T *R = s_r + (w * S + si) * SL;
+        const T *Aa = s_a + (w * S + si) * SL, *Cc = s_c + (w * S + si) * SL;
         T ra[P], aa[P], ca[P];
 #pragma unroll
         for (int p = 0; p < P; p++) { ra[p] = R[lane * P + p]; aa[p] = Aa[lane * P + p]; ca[p] = Cc[lane * P + p]; }
@@ -704,18 +706,35 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
     }
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < P; m++) {
+    for (int m = 0; m < P * S; m++) {
         const int e = threadIdx.x + LS_THREADS * m;
         if (e >= E) break;
         int L, q;
         if (ax == 0) { L = e / n; q = e - L * n; } else { q = e / NLW; L = e - q * NLW; }
         const int td = 2 * (h0 + L) + sh;
-        if (td < g.Ntd) x[td * g.s_td + q * g.s_ax + p3 * g.s_3] = s_r[(L / LW) * 64 * P + (L % LW) * n + q];
+        if (td < g.Ntd) x[td * g.s_td + q * g.s_ax + p3 * g.s_3] = s_r[(L / LW) * SL + (L % LW) * n + q];
     }
+}
+// slots per wavefront allowed by the 64 KB of LDS a workgroup may hold
+template <class T> constexpr int ls_smax(int P) {
+    return 3 * LS_WAVES * 64 * P * 4 * (int)sizeof(T) <= 65536 ? 4
+         : (3 * LS_WAVES * 64 * P * 2 * (int)sizeof(T) <= 65536 ? 2 : 1);
+}
+template <class T, int P>
+static void ls_launch(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero,
+                      const MGLevel<T> &cc, int pro, int LW) {
+    constexpr int SM = ls_smax<T>(P);
+    if (S >= 4 && SM >= 4)
+        k_line_scan<T, P, (SM >= 4 ? 4 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW);
+    else if (S >= 2 && SM >= 2)
+        k_line_scan<T, P, (SM >= 2 ? 2 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW);
+    else
+        k_line_scan<T, P, 1><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW);
 }
 
 int g_line_nl_max = 8;       // lines per workgroup cap of k_line_gs (tuning)
 int g_line_scan = 1;         // 1: scan kernel for lines of up to 256 elements
+int g_line_slots = 4;        // most line slots per wavefront of the scan kernel (1, 2, 4)
 template <class T>
 int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero, const MGLevel<T> *c) {
     int n = ax == 0 ? v.Nx : (ax == 1 ? v.Ny : v.Nz);
@@ -723,15 +742,28 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
     if (!v.lu[ax][0]) return 0;
     const int P = (n + 63) / 64;
     if (g_line_scan && P <= 4) {
-        const int LW = 64 * P / n, NLW = LS_WAVES * LW;
-        const int ntile = ((Ntd + 1) / 2 + NLW - 1) / NLW;
+        const int LW = 64 * P / n;
+        // lines across the workgroup (y- and z-lines): widen the contiguous
+        // runs with more slots per wavefront while at least 512 workgroups
+        // remain (two per CU)
+        int S = 1;
+        if (ax != 0)
+            for (int sc = g_line_slots; sc > 1; sc >>= 1) {
+                int nlw = LS_WAVES * sc * LW;
+                if ((long)(((Ntd + 1) / 2 + nlw - 1) / nlw) * N3 >= 512) { S = sc; break; }
+            }
+        // at most the slots the LDS allows (ls_launch makes the same choice)
+        const int SM = P == 1 ? ls_smax<T>(1) : (P == 2 ? ls_smax<T>(2) : (P == 3 ? ls_smax<T>(3) : ls_smax<T>(4)));
+        int Srun = S;
+        while (Srun > SM) Srun >>= 1;
+        const int NLWr = LS_WAVES * Srun * LW, nb = ((Ntd + 1) / 2 + NLWr - 1) / NLWr * N3;
         MGLevel<T> cc = c ? *c : v;
-        const int pro = c != nullptr, nb = ntile * N3;
+        const int pro = c != nullptr;
         switch (P) {
-        case 1: k_line_scan<T, 1><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW); break;
-        case 2: k_line_scan<T, 2><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW); break;
-        case 3: k_line_scan<T, 3><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW); break;
-        default: k_line_scan<T, 4><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW); break;
+        case 1: ls_launch<T, 1>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
+        case 2: ls_launch<T, 2>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
+        case 3: ls_launch<T, 3>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
+        default: ls_launch<T, 4>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
         }
         return 1;
     }
@@ -757,43 +789,61 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
 // weighted residual, and three xor shuffles sum the aggregate, so every lane
 // issues its loads at once.  t indexes lanes 8 K + sub; every lane of the
 // wavefront must call it (the shuffles), lanes past 8 c.N contribute nothing.
+// weighted residual of fine node (2I+di, 2J+dj, 2L+dl) of coarse node K
+// (0 outside the aggregate or on a skipped row)
+template <class T>
+__device__ __forceinline__ double agg_resid(const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x, int skip,
+                                            int K, int di, int dj, int dl) {
+    const int cNxNy = c.Nx * c.Ny, LL = K / cNxNy, rr = K - LL * cNxNy, J = rr / c.Nx, I = rr - J * c.Nx;
+    const int i = f.cx ? 2 * I + di : I, j = f.cy ? 2 * J + dj : J, l = f.cz ? 2 * LL + dl : LL;
+    bool ok = (f.cx || !di) && (f.cy || !dj) && (f.cz || !dl) && i < f.Nx && j < f.Ny && l < f.Nz;
+    if (ok && skip == 1 && ((i + j + l) & 1)) ok = false;
+    if (ok && skip >= 2) {
+        const int ax = skip - 2, par = ax == 0 ? (j + l) : (ax == 1 ? (i + l) : (i + j));
+        if (par & 1) ok = false;
+    }
+    if (!ok) return 0.0;
+    const int k = i + f.Nx * (j + f.Ny * l);
+    const double w = f.rs ? f.rs[k] : 1.0;
+    if (w == 0.0) return 0.0;
+    T r = b[k] - sten_apply<T, T>(f.A, x, k, i, j, l, f.Nx, f.Ny, f.Nz, f.A.c == nullptr);
+    return w * (double)r;
+}
 template <class T>
 __device__ __forceinline__ void restrict8(const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x, int skip,
                                           int t) {
     const int K = t >> 3, sub = t & 7;
-    double v = 0.0;
-    if (K < c.N) {
-        const int cNxNy = c.Nx * c.Ny, LL = K / cNxNy, rr = K - LL * cNxNy, J = rr / c.Nx, I = rr - J * c.Nx;
-        const int di = sub & 1, dj = (sub >> 1) & 1, dl = sub >> 2;
-        const int i = f.cx ? 2 * I + di : I, j = f.cy ? 2 * J + dj : J, l = f.cz ? 2 * LL + dl : LL;
-        bool ok = (f.cx || !di) && (f.cy || !dj) && (f.cz || !dl) && i < f.Nx && j < f.Ny && l < f.Nz;
-        if (ok && skip == 1 && ((i + j + l) & 1)) ok = false;
-        if (ok && skip >= 2) {
-            const int ax = skip - 2, par = ax == 0 ? (j + l) : (ax == 1 ? (i + l) : (i + j));
-            if (par & 1) ok = false;
-        }
-        if (ok) {
-            const int k = i + f.Nx * (j + f.Ny * l);
-            const double w = f.rs ? f.rs[k] : 1.0;
-            if (w != 0.0) {
-                T r = b[k] - sten_apply<T, T>(f.A, x, k, i, j, l, f.Nx, f.Ny, f.Nz, f.A.c == nullptr);
-                v = w * (double)r;
-            }
-        }
-    }
+    double v = K < c.N ? agg_resid(f, c, b, x, skip, K, sub & 1, (sub >> 1) & 1, sub >> 2) : 0.0;
     v += __shfl_xor(v, 1, 64);
     v += __shfl_xor(v, 2, 64);
     v += __shfl_xor(v, 4, 64);
     if (sub == 0 && K < c.N) c.b[K] = (T)(v / c.rs[K]);
 }
+// Level-wide restriction: a workgroup of four wavefronts covers 32
+// consecutive coarse nodes; wavefront w takes the fine rows (dj, dl) =
+// (w & 1, w >> 1) and lane pairs the two x offsets, so each load of a
+// wavefront spans 64 consecutive fine nodes of one row.  Pair sums by
+// shuffle, then the four rows through LDS.
 template <class T>
-__global__ void __launch_bounds__(SD_BLOCK) k_resid_restrict8(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x,
-                                                              int skip) {
-    restrict8(f, c, b, x, skip, blockIdx.x * SD_BLOCK + threadIdx.x);
+__global__ void __launch_bounds__(256) k_resid_restrict(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x, int skip) {
+    __shared__ double part[4][32];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int K = blockIdx.x * 32 + (lane >> 1);
+    double v = K < c.N ? agg_resid(f, c, b, x, skip, K, lane & 1, w & 1, w >> 1) : 0.0;
+    v += __shfl_xor(v, 1, 64);
+    if (!(lane & 1)) part[w][lane >> 1] = v;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        const int K2 = blockIdx.x * 32 + threadIdx.x;
+        if (K2 < c.N) {
+            double sum = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+            c.b[K2] = (T)(sum / c.rs[K2]);
+        }
+    }
 }
 template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x,
                                           int skip) {
-    k_resid_restrict8<T><<<sd_nblocks(8 * c.N), SD_BLOCK, 0, s>>>(f, c, b, x, skip);
+    k_resid_restrict<T><<<(c.N + 31) / 32, 256, 0, s>>>(f, c, b, x, skip);
 }
 
 // The coarse end of the V-cycle in one workgroup: every level from lv[0]
