@@ -1,30 +1,34 @@
 #!/usr/bin/env python3
 """Bias points per second of the n-MOSFET Id-Vg sweep on one MI355X.
 
-Workload: the planar n-MOSFET on a 96x64x64 tensor mesh (393,216 nodes),
+Workload: the planar n-MOSFET on a 192x128x128 tensor mesh (3,145,728 nodes),
 Id-Vg from 0 to 1.2 V in 0.05 V steps at Vd = 0.05 V and 1.0 V (50 bias
 points), synthetic device (no external data).  A step is one bias point:
 api3_resolve continuing the sweep from the previous point, converged to the
 reference's tolerances (tol_p 1e-10 V, tol_c 1e-9 relative current), inputs
 already resident on the device.  W warmup points are solved untimed, then K
-points are timed (W + K <= 50).
+points are timed (W + K <= 50).  --mesh 96 64 64 runs the 393,216-node mesh
+instead.
 
 Two solver configurations run back to back on the same sweep: the plain FP64
 method (FP64 multigrid) and the mixed-precision method (FP32 multigrid,
 smoothers and preconditioner SpMV inside FP64 Krylov/Newton).  `value` is the
 mixed-precision rate; the plain FP64 rate and the speedup are reported beside
-it.  The terminal currents of every timed point are checked against the
-reference core's sweep (tests/golden/e2e_mosfet_96x64x64.npz).
+it.  Parity: the terminal currents of every timed point of each method against
+the reference core's sweep of the same mesh when its fixture is present
+(tests/golden/e2e_mosfet_<mesh>.npz), and the mixed-precision currents and
+final fields against the plain FP64 GPU run.
 
 roofline: the dominant kernel (the level-0 zebra line Gauss-Seidel sweep of
 the multigrid smoother), its algorithmic bytes per launch divided by its
 average launch time, measured with HIP events on the solver stream during the
-timed region (every 8th launch).  cpu_baseline: the reference C core
-(oracle/_ref/libsemisim_ref.so, built from the reference sources) on the host
-cores, on a sample of the same sweep: at four positions it resumes from the
-GPU's converged state and is timed on three consecutive bias points.
+timed region (every 8th V-cycle runs with its level-0 sweeps bracketed).
+cpu_baseline: the reference C core (oracle/_ref/libsemisim_ref.so, built from
+the reference sources) on the host cores, on a sample of the same sweep: at
+each sample position it resumes from the GPU's converged state and is timed
+on the points that follow.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--mesh NX NY NZ]
 """
 import argparse
 import ctypes
@@ -41,7 +45,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-WORKLOAD = "mosfet_96x64x64 Id-Vg sweep (0-1.2 V x 0.05 V at Vd 0.05 and 1.0 V)"
+SWEEP = "Id-Vg sweep (0-1.2 V x 0.05 V at Vd 0.05 and 1.0 V)"
 
 
 def sweep_points(spec):
@@ -52,7 +56,9 @@ def sweep_points(spec):
     return pts
 
 
-def solve_points(lib, cids, pts, i0, i1):
+def solve_points(lib, cids, pts, i0, i1, floors=None):
+    """Solve sweep points i0..i1-1; returns their terminal currents (and
+    appends the current floors to `floors` when given)."""
     I = []
     for tgt, v, hold in pts[i0:i1]:
         for c, hv in hold.items():
@@ -62,14 +68,16 @@ def solve_points(lib, cids, pts, i0, i1):
         if not ok:
             raise RuntimeError(f"bias point V={v} did not converge")
         I.append([lib.api3_get_contact_I(cids[c]) for c in range(len(cids))])
+        if floors is not None:
+            floors.append([lib.api3_get_contact_Ifloor(cids[c]) for c in range(len(cids))])
     return np.array(I)
 
 
 def run_gpu(lib, spec, precision, warmup, steps, ktimer, keep=()):
     """Solve the sweep's first warmup points untimed, then time `steps`
-    points; returns (seconds, currents, kernel timer, iteration counts,
-    {index: (psi, phi_n, phi_p)} of the converged states listed in keep, read
-    outside the timed calls)."""
+    points; returns (seconds, currents, current floors, kernel timer,
+    iteration counts, {index: (psi, phi_n, phi_p)} of the converged states
+    listed in keep, read outside the timed calls)."""
     from semiconductor_devices_simulator_amd import devices
     from semiconductor_devices_simulator_amd.core import dptr
     cids = devices.load_device(lib, spec)
@@ -81,10 +89,10 @@ def run_gpu(lib, spec, precision, warmup, steps, ktimer, keep=()):
     g0 = [lib.api3_get_stat(i) for i in range(4)]
     if ktimer:
         lib.sdx_ktimer_enable(0, 8)
-    dt, I, states = 0.0, [], {}
+    dt, I, F, states = 0.0, [], [], {}
     for i in range(warmup, warmup + steps):
         t0 = time.perf_counter()
-        I.append(solve_points(lib, cids, pts, i, i + 1)[0])
+        I.append(solve_points(lib, cids, pts, i, i + 1, F)[0])
         dt += time.perf_counter() - t0
         if i in keep:
             st = tuple(np.zeros(spec.N) for _ in range(3))
@@ -95,11 +103,12 @@ def run_gpu(lib, spec, precision, warmup, steps, ktimer, keep=()):
         lib.sdx_ktimer_read(kt.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     g1 = [lib.api3_get_stat(i) for i in range(4)]
     its = dict(pcg=g1[0] - g0[0], bicgstab=g1[1] - g0[1], newton=g1[2] - g0[2], gummel=g1[3] - g0[3])
-    return dt, np.array(I), kt, its, states
+    return dt, np.array(I), np.array(F), kt, its, states
 
 
-def parity(I, warmup, steps):
-    path = os.path.join(ROOT, "tests", "golden", "e2e_mosfet_96x64x64.npz")
+def parity(I, warmup, steps, mesh):
+    """Currents of the timed points against the reference core's sweep."""
+    path = os.path.join(ROOT, "tests", "golden", "e2e_mosfet_%dx%dx%d.npz" % tuple(mesh))
     if not os.path.isfile(path):
         return None
     g = np.load(path)
@@ -107,18 +116,34 @@ def parity(I, warmup, steps):
     d = np.abs(I - Ir)
     rel = d[:, 1] / np.abs(Ir[:, 1])
     ok = d <= 1e-8 * np.abs(Ir) + 2 * Fr
-    return dict(max_rel_drain_current_err=float(rel.max()), points_within_tolerance=int(ok.all(axis=1).sum()),
-                points=int(len(I)), criterion="|dI| <= 1e-8|I_ref| + 2 floor_ref, every terminal")
+    return dict(against="reference C core (FP64)", max_rel_drain_current_err=float(rel.max()),
+                points_within_tolerance=int(ok.all(axis=1).sum()), points=int(len(I)),
+                criterion="|dI| <= 1e-8|I_ref| + 2 floor_ref, every terminal")
+
+
+def mixed_vs_fp64(I32, I64, F64, st32, st64):
+    """Mixed-precision method against the plain FP64 GPU method: currents of
+    every timed point and the fields of the last one."""
+    d = np.abs(I32 - I64)
+    ok = d <= 1e-8 * np.abs(I64) + 2 * F64
+    rel = d[:, 1] / np.abs(I64[:, 1])
+    out = dict(against="plain FP64 GPU method", max_rel_drain_current_err=float(rel.max()),
+               points_within_tolerance=int(ok.all(axis=1).sum()), points=int(len(I32)),
+               criterion="|dI| <= 1e-8|I_fp64| + 2 floor_fp64, every terminal")
+    if st32 is not None and st64 is not None:
+        out["last_point_max_abs_diff_V"] = {k: float(np.abs(a - b).max())
+                                           for k, a, b in zip(("psi", "phi_n", "phi_p"), st32, st64)}
+    return out
 
 
 # CPU-baseline sample: the reference resumes from the GPU's converged state
 # at each window start (api3_set_state), solves the next point untimed (it
-# re-arms the sweep predictor), then is timed on the following three points.
-CPU_WINDOWS = (10, 18, 35, 43)
-CPU_TIMED = 3
+# re-arms the sweep predictor), then is timed on the following points.
+CPU_WINDOWS = {96: (10, 18, 35, 43), 192: (18, 43)}
+CPU_TIMED = {96: 3, 192: 1}
 
 
-def cpu_baseline(states):
+def cpu_baseline(states, mesh):
     ref = os.path.join(ROOT, "oracle", "_ref", "libsemisim_ref.so")
     if not os.path.isfile(ref) or not states:
         return None
@@ -126,7 +151,8 @@ def cpu_baseline(states):
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
     with tempfile.TemporaryDirectory() as td:
         path = os.path.join(td, "states.npz")
-        np.savez(path, idx=np.array(sorted(states)), **{f"s{i}": np.stack(states[i]) for i in states})
+        np.savez(path, idx=np.array(sorted(states)), mesh=np.array(mesh),
+                 **{f"s{i}": np.stack(states[i]) for i in states})
         r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker", path],
                            env=env, capture_output=True, text=True, timeout=900)
     if r.returncode != 0:
@@ -134,19 +160,23 @@ def cpu_baseline(states):
         return None
     out = json.loads(r.stdout.strip().splitlines()[-1])
     return dict(value=round(out["points_per_s"], 4), unit="bias points/s", cores=threads, kind="reference",
-                sample=f"{out['points']} bias points ({CPU_TIMED} consecutive points after each of the sweep "
-                       f"positions {list(CPU_WINDOWS)}, the reference resuming from the GPU's converged state "
-                       f"there), {out['seconds']:.1f} s; reference C core built from its sources, OpenMP")
+                sample=f"{out['points']} bias points ({CPU_TIMED.get(mesh[0], CPU_TIMED[96])} "
+                       f"consecutive points after each of the sweep positions "
+                       f"{list(CPU_WINDOWS.get(mesh[0], CPU_WINDOWS[96]))}, the reference resuming from the GPU's "
+                       f"converged state there), {out['seconds']:.1f} s; reference C core built from its sources, "
+                       f"OpenMP")
 
 
 def cpu_baseline_worker(path):
     from semiconductor_devices_simulator_amd import core, devices
     from semiconductor_devices_simulator_amd.core import dptr
     lib = core.bind(ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libsemisim_ref.so")))
-    spec = devices.mosfet(96, 64, 64)
+    data = np.load(path)
+    mesh = [int(v) for v in data["mesh"]]
+    timed = CPU_TIMED.get(mesh[0], CPU_TIMED[96])
+    spec = devices.mosfet(*mesh)
     cids = devices.load_device(lib, spec)
     pts = sweep_points(spec)
-    data = np.load(path)
     dt, n = 0.0, 0
     for i in data["idx"]:
         i = int(i)
@@ -159,9 +189,9 @@ def cpu_baseline_worker(path):
         assert lib.api3_set_state(dptr(psi), dptr(phin), dptr(phip)) == 1
         solve_points(lib, cids, pts, i + 1, i + 2)
         t0 = time.perf_counter()
-        solve_points(lib, cids, pts, i + 2, i + 2 + CPU_TIMED)
+        solve_points(lib, cids, pts, i + 2, i + 2 + timed)
         dt += time.perf_counter() - t0
-        n += CPU_TIMED
+        n += timed
     print(json.dumps(dict(points_per_s=n / dt, seconds=dt, points=n)))
 
 
@@ -170,6 +200,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=48)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mesh", type=int, nargs=3, default=[192, 128, 128])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-worker", default="")
     a = ap.parse_args()
@@ -182,17 +213,21 @@ def main():
         return 2
     from semiconductor_devices_simulator_amd import core, devices
     lib = core.load_core()
-    spec = devices.mosfet(96, 64, 64)
+    spec = devices.mosfet(*a.mesh)
     npts = spec.bias_points()
     W = max(0, min(a.warmup, npts - 1))
     K = max(1, min(a.steps, npts - W))
+    workload = "mosfet_%dx%dx%d %s" % (*a.mesh, SWEEP)
     buf = ctypes.create_string_buffer(256)
     lib.sdx_device_name(buf, 256)
-    print(f"[bench] device {buf.value.decode()}; {WORKLOAD}; warmup {W} timed {K}", file=sys.stderr, flush=True)
-    keep = [i for i in CPU_WINDOWS if W <= i and i + 2 + CPU_TIMED <= npts]
-    t64, I64, _, its64, states = run_gpu(lib, spec, 0, W, K, ktimer=False, keep=keep)
+    print(f"[bench] device {buf.value.decode()}; {workload}; warmup {W} timed {K}", file=sys.stderr, flush=True)
+    windows = CPU_WINDOWS.get(a.mesh[0], CPU_WINDOWS[96])
+    timed = CPU_TIMED.get(a.mesh[0], CPU_TIMED[96])
+    last = W + K - 1
+    keep = [i for i in windows if W <= i and i + 2 + timed <= npts] + [last]
+    t64, I64, F64, _, its64, states = run_gpu(lib, spec, 0, W, K, ktimer=False, keep=keep)
     print(f"[bench] plain FP64: {K / t64:.3f} points/s", file=sys.stderr, flush=True)
-    t32, I32, kt, its32, _ = run_gpu(lib, spec, 1, W, K, ktimer=True)
+    t32, I32, _, kt, its32, st32 = run_gpu(lib, spec, 1, W, K, ktimer=True, keep=[last])
     print(f"[bench] FP32 multigrid: {K / t32:.3f} points/s", file=sys.stderr, flush=True)
     roof = None
     if kt[0] > 0:
@@ -200,9 +235,10 @@ def main():
         ach = kt[2] / avg_s / 1e9
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_line_gs<float> level 0 (zebra line Gauss-Seidel sweep, one colour)",
+                    kernel="k_line_scan<float> level 0 (zebra line Gauss-Seidel sweep, one colour)",
                     bytes_per_launch=kt[2], avg_launch_us=round(avg_s * 1e6, 2), launches_timed=int(kt[0]))
-    cpu = None if a.no_cpu_baseline else cpu_baseline(states)
+    cpu_states = {i: st for i, st in states.items() if i != last or i in windows}
+    cpu = None if a.no_cpu_baseline else cpu_baseline(cpu_states, a.mesh)
     value = K / t32
     out = {
         "metric": "bias points/s at 1e-8 rel current err, learned+FP32-AMG vs plain FP64 on GPU",
@@ -217,12 +253,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 (Krylov, Newton, assembly) + f32 multigrid",
         "data": "synthetic",
-        "config": {"workload": WORKLOAD, "nodes": spec.N, "mesh": list(spec.shape), "method": "FP32-AMG",
+        "config": {"workload": workload, "nodes": spec.N, "mesh": list(spec.shape), "method": "FP32-AMG",
                    "learned_warm_start": False},
         "plain_fp64_points_per_s": round(K / t64, 4),
         "speedup_vs_plain_fp64": round(t64 / t32, 3),
         "iterations": {"fp32_mg": its32, "fp64": its64},
-        "parity": {"fp32_mg": parity(I32, W, K), "fp64": parity(I64, W, K)},
+        "parity": {"fp32_mg": parity(I32, W, K, a.mesh), "fp64": parity(I64, W, K, a.mesh),
+                   "fp32_mg_vs_fp64": mixed_vs_fp64(I32, I64, F64, st32.get(last), states.get(last))},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -612,16 +612,24 @@ __global__ void __launch_bounds__(SD_BLOCK) k_line_gs(MGLevel<T> v, const T *__r
 #define LS_THREADS (64 * LS_WAVES)
 template <class T, int P, int S>
 __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T *__restrict__ b, T *__restrict__ x,
-                                                          int ax, int color, int zero, MGLevel<T> c, int pro, int LW) {
+                                                          int ax, int color, int zero, MGLevel<T> c, int pro, int LW,
+                                                          int plane) {
     constexpr int SL = 64 * P;                       // elements per slot
     __shared__ T s_r[LS_WAVES * S * SL], s_a[LS_WAVES * S * SL], s_c[LS_WAVES * S * SL];
     const int Nx = v.Nx, Ny = v.Ny, Nz = v.Nz, NxNy = Nx * Ny;
     const LineGeom g = line_geom(Nx, Ny, Nz, ax);
     const int n = g.n;
     const int NLW = LS_WAVES * S * LW;               // lines per workgroup
-    const int ntile = (((g.Ntd + 1) >> 1) + NLW - 1) / NLW;
-    const int p3 = blockIdx.x / ntile, h0 = (blockIdx.x - p3 * ntile) * NLW;
-    const int sh = (p3 + color) & 1;
+    // zebra: the colour's lines of plane p3 are every other td; plane
+    // colouring: every line of the planes p3 = colour (mod 2)
+    int p3, h0, sh, tstep;
+    if (plane) {
+        const int ntile = (g.Ntd + NLW - 1) / NLW, pp = blockIdx.x / ntile;
+        p3 = 2 * pp + color; h0 = (blockIdx.x - pp * ntile) * NLW; sh = 0; tstep = 1;
+    } else {
+        const int ntile = (((g.Ntd + 1) >> 1) + NLW - 1) / NLW;
+        p3 = blockIdx.x / ntile; h0 = (blockIdx.x - p3 * ntile) * NLW; sh = (p3 + color) & 1; tstep = 2;
+    }
     const int E = NLW * n, used = LW * n, pad = SL - used;
     const T *__restrict__ AI = v.lu[ax][0];
     const T *__restrict__ IW = v.lu[ax][1];
@@ -637,7 +645,7 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
         if (e >= E) break;
         int L, q;
         if (ax == 0) { L = e / n; q = e - L * n; } else { q = e / NLW; L = e - q * NLW; }
-        const int td = 2 * (h0 + L) + sh;
+        const int td = tstep * (h0 + L) + sh;
         T r = 0, ai = 0, iw = 0, cp = 0;
         if (td < g.Ntd) {
             const int k = td * g.s_td + q * g.s_ax + p3 * g.s_3;
@@ -711,7 +719,7 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
         if (e >= E) break;
         int L, q;
         if (ax == 0) { L = e / n; q = e - L * n; } else { q = e / NLW; L = e - q * NLW; }
-        const int td = 2 * (h0 + L) + sh;
+        const int td = tstep * (h0 + L) + sh;
         if (td < g.Ntd) x[td * g.s_td + q * g.s_ax + p3 * g.s_3] = s_r[(L / LW) * SL + (L % LW) * n + q];
     }
 }
@@ -722,19 +730,20 @@ template <class T> constexpr int ls_smax(int P) {
 }
 template <class T, int P>
 static void ls_launch(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero,
-                      const MGLevel<T> &cc, int pro, int LW) {
+                      const MGLevel<T> &cc, int pro, int LW, int plane) {
     constexpr int SM = ls_smax<T>(P);
     if (S >= 4 && SM >= 4)
-        k_line_scan<T, P, (SM >= 4 ? 4 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW);
+        k_line_scan<T, P, (SM >= 4 ? 4 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW, plane);
     else if (S >= 2 && SM >= 2)
-        k_line_scan<T, P, (SM >= 2 ? 2 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW);
+        k_line_scan<T, P, (SM >= 2 ? 2 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW, plane);
     else
-        k_line_scan<T, P, 1><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW);
+        k_line_scan<T, P, 1><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW, plane);
 }
 
 int g_line_nl_max = 8;       // lines per workgroup cap of k_line_gs (tuning)
 int g_line_scan = 1;         // 1: scan kernel for lines of up to 256 elements
 int g_line_slots = 4;        // most line slots per wavefront of the scan kernel (1, 2, 4)
+int g_line_plane = 0;        // 1: y- and z-lines coloured by plane parity (whole planes per colour)
 template <class T>
 int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero, const MGLevel<T> *c) {
     int n = ax == 0 ? v.Nx : (ax == 1 ? v.Ny : v.Nz);
@@ -743,6 +752,9 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
     const int P = (n + 63) / 64;
     if (g_line_scan && P <= 4) {
         const int LW = 64 * P / n;
+        const int plane = ax != 0 && g_line_plane;
+        // lines per plane and colour, planes per colour
+        const int lpp = plane ? Ntd : (Ntd + 1) / 2, npl = plane ? (N3 - color + 1) / 2 : N3;
         // lines across the workgroup (y- and z-lines): widen the contiguous
         // runs with more slots per wavefront while at least 512 workgroups
         // remain (two per CU)
@@ -750,22 +762,23 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
         if (ax != 0)
             for (int sc = g_line_slots; sc > 1; sc >>= 1) {
                 int nlw = LS_WAVES * sc * LW;
-                if ((long)(((Ntd + 1) / 2 + nlw - 1) / nlw) * N3 >= 512) { S = sc; break; }
+                if ((long)((lpp + nlw - 1) / nlw) * npl >= 512) { S = sc; break; }
             }
         // at most the slots the LDS allows (ls_launch makes the same choice)
         const int SM = P == 1 ? ls_smax<T>(1) : (P == 2 ? ls_smax<T>(2) : (P == 3 ? ls_smax<T>(3) : ls_smax<T>(4)));
         int Srun = S;
         while (Srun > SM) Srun >>= 1;
-        const int NLWr = LS_WAVES * Srun * LW, nb = ((Ntd + 1) / 2 + NLWr - 1) / NLWr * N3;
+        const int NLWr = LS_WAVES * Srun * LW, nb = (lpp + NLWr - 1) / NLWr * npl;
+        if (nb == 0) return plane ? 2 : 1;
         MGLevel<T> cc = c ? *c : v;
         const int pro = c != nullptr;
         switch (P) {
-        case 1: ls_launch<T, 1>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
-        case 2: ls_launch<T, 2>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
-        case 3: ls_launch<T, 3>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
-        default: ls_launch<T, 4>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
+        case 1: ls_launch<T, 1>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW, plane); break;
+        case 2: ls_launch<T, 2>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW, plane); break;
+        case 3: ls_launch<T, 3>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW, plane); break;
+        default: ls_launch<T, 4>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW, plane); break;
         }
-        return 1;
+        return plane ? 2 : 1;
     }
     if (n > LINE_MAXEQ) return 0;
     int NL = LINE_MAXEQ / n;
@@ -783,7 +796,8 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
 // scale (restriction R = P^T W, semiconductor_3d.c:2028-2037), divided by the
 // coarse row scale (the coarse rows are stored with unit diagonal).  Rows
 // solved exactly by the last smoothing sweep have no residual and are
-// skipped: skip 1 = point black nodes, 2+a = colour-1 lines along axis a.
+// skipped: skip 1 = point black nodes, 2+a = colour-1 lines along axis a,
+// 5+a = the same for lines coloured by plane parity.
 // One lane per fine node: the eight lanes of an aggregate are adjacent (lane
 // bits 0, 1, 2 = x, y, z offset in the aggregate), each evaluates its node's
 // weighted residual, and three xor shuffles sum the aggregate, so every lane
@@ -798,8 +812,12 @@ __device__ __forceinline__ double agg_resid(const MGLevel<T> &f, const MGLevel<T
     const int i = f.cx ? 2 * I + di : I, j = f.cy ? 2 * J + dj : J, l = f.cz ? 2 * LL + dl : LL;
     bool ok = (f.cx || !di) && (f.cy || !dj) && (f.cz || !dl) && i < f.Nx && j < f.Ny && l < f.Nz;
     if (ok && skip == 1 && ((i + j + l) & 1)) ok = false;
-    if (ok && skip >= 2) {
+    if (ok && skip >= 2 && skip <= 4) {
         const int ax = skip - 2, par = ax == 0 ? (j + l) : (ax == 1 ? (i + l) : (i + j));
+        if (par & 1) ok = false;
+    }
+    if (ok && skip >= 5) {                           // plane-coloured lines
+        const int ax = skip - 5, par = ax == 0 ? (j + l) : (ax == 1 ? l : j);
         if (par & 1) ok = false;
     }
     if (!ok) return 0.0;

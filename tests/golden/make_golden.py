@@ -213,6 +213,9 @@ def main():
     if "--mosfet96" in sys.argv:       # the full 96x64x64 Id-Vg sweep (about 5 minutes on 8 cores)
         sweep_fixture(lib, devices.mosfet(96, 64, 64), "e2e_mosfet_96x64x64", fields=False, sample=4096)
         return
+    if "--mosfet192" in sys.argv:      # the full 192x128x128 Id-Vg sweep (under an hour on 8 cores)
+        sweep_fixture(lib, devices.mosfet(192, 128, 128), "e2e_mosfet_192x128x128", fields=False, sample=4096)
+        return
     rng = np.random.default_rng(1234)
     scalar_fixture(lib)
     kernel_fixture(lib, devices.mosfet(12, 8, 6), "k_mosfet_12x8x6", [0.0, 0.5, 0.8, 0.0], rng)
