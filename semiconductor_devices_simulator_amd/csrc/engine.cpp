@@ -583,16 +583,15 @@ template <class T> static int smooth_pre(const MGLevel<T> &v, const T *b, T *x, 
         return 1;
     }
     int first = 1;
-    int mode = 1;
     for (int a = 0; a < G.n_axes; a++)
         for (int color = 0; color < 2; color++) {
             bool t = v.N == G.N && kt_begin(0);
-            mode = mg_line(s, v, b, x, G.mg_axes[a], color, first);
+            mg_line(s, v, b, x, G.mg_axes[a], color, first);
             kt_end(t);
             first = 0;
         }
     // rows of the last sweep's colour-1 lines have no residual
-    return (mode == 2 ? 5 : 2) + G.mg_axes[G.n_axes - 1];
+    return 2 + G.mg_axes[G.n_axes - 1];
 }
 // Post-smoothing: the pre-smoother's sweeps in reverse order (symmetric
 // V-cycle for CG).  With c != nullptr the first sweep applies the coarse
@@ -1438,7 +1437,6 @@ void eng_read_env() {
     if ((e = getenv("SEMISIM_MG_LINELEVELS"))) G.line_levels = atoi(e);
     if ((e = getenv("SEMISIM_MG_TAILN"))) G.tail_n = atoi(e);
     if ((e = getenv("SEMISIM_MG_TAILSW"))) { int q = atoi(e); G.tail_sweeps = q > 0 ? q : 1; }
-    if ((e = getenv("SEMISIM_MG_PLANE"))) g_line_plane = atoi(e) != 0;
     G.use_graphs = (e = getenv("SEMISIM_GRAPHS")) ? atoi(e) != 0 : 0;
 }
 int eng_aa_depth() { return G.aa_m < AA_DEPTH ? G.aa_m : AA_DEPTH; }
@@ -1540,7 +1538,6 @@ void sdx_tune(int what, int value) {
     if (what == 0) g_line_nl_max = value;
     if (what == 4) g_line_scan = value;
     if (what == 5) g_line_slots = value >= 4 ? 4 : (value >= 2 ? 2 : 1);
-    if (what == 6) g_line_plane = value != 0;
     if (what == 1) G.line_levels = value;
     if (what == 2) G.tail_sweeps = value > 0 ? value : 1;
     if (what == 3) {

@@ -612,76 +612,100 @@ __global__ void __launch_bounds__(SD_BLOCK) k_line_gs(MGLevel<T> v, const T *__r
 #define LS_THREADS (64 * LS_WAVES)
 template <class T, int P, int S>
 __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T *__restrict__ b, T *__restrict__ x,
-                                                          int ax, int color, int zero, MGLevel<T> c, int pro, int LW,
-                                                          int plane) {
+                                                          int ax, int color, int zero, MGLevel<T> c, int pro, int LW) {
     constexpr int SL = 64 * P;                       // elements per slot
-    __shared__ T s_r[LS_WAVES * S * SL], s_a[LS_WAVES * S * SL], s_c[LS_WAVES * S * SL];
+    constexpr int SP = SL + 1;                       // slot pitch in LDS: odd, so that the gathers
+                                                     // across lines (consecutive lanes, one slot
+                                                     // apart) fall in different banks
+    __shared__ T s_r[LS_WAVES * S * SP], s_a[LS_WAVES * S * SP], s_c[LS_WAVES * S * SP];
     const int Nx = v.Nx, Ny = v.Ny, Nz = v.Nz, NxNy = Nx * Ny;
     const LineGeom g = line_geom(Nx, Ny, Nz, ax);
     const int n = g.n;
     const int NLW = LS_WAVES * S * LW;               // lines per workgroup
-    // zebra: the colour's lines of plane p3 are every other td; plane
-    // colouring: every line of the planes p3 = colour (mod 2)
-    int p3, h0, sh, tstep;
-    if (plane) {
-        const int ntile = (g.Ntd + NLW - 1) / NLW, pp = blockIdx.x / ntile;
-        p3 = 2 * pp + color; h0 = (blockIdx.x - pp * ntile) * NLW; sh = 0; tstep = 1;
-    } else {
-        const int ntile = (((g.Ntd + 1) >> 1) + NLW - 1) / NLW;
-        p3 = blockIdx.x / ntile; h0 = (blockIdx.x - p3 * ntile) * NLW; sh = (p3 + color) & 1; tstep = 2;
-    }
+    // the colour's lines of plane p3 are every other td: td = 2 h + sh
+    const int ntile = (((g.Ntd + 1) >> 1) + NLW - 1) / NLW;
+    const int p3 = blockIdx.x / ntile, h0 = (blockIdx.x - p3 * ntile) * NLW, sh = (p3 + color) & 1;
     const int E = NLW * n, used = LW * n, pad = SL - used;
     const T *__restrict__ AI = v.lu[ax][0];
     const T *__restrict__ IW = v.lu[ax][1];
     const T *__restrict__ CP = v.lu[ax][2];
     // slot elements past the packed lines carry the identity
     for (int t = threadIdx.x; t < LS_WAVES * S * pad; t += LS_THREADS) {
-        int q = (t / pad) * SL + used + t % pad;
+        int q = (t / pad) * SP + used + t % pad;
         s_r[q] = 0; s_a[q] = 0; s_c[q] = 0;
     }
+    // Gather, first pass: every load of this thread's M elements is issued
+    // before any of them is used.  Elements outside the level and missing
+    // neighbours read the element itself (clamped index) and are masked.
+    // The two off-line axes in the order the terms are subtracted: for x-lines
+    // y then z, for y-lines x then z, for z-lines x then y.
+    constexpr int M = P * S;
+    const int s1 = ax == 0 ? Nx : 1, s2 = ax == 2 ? Nx : NxNy;
+    const T *o0 = ax == 0 ? v.A.ym : v.A.xm, *o1 = ax == 0 ? v.A.yp : v.A.xp;
+    const T *o2 = ax == 2 ? v.A.ym : v.A.zm, *o3 = ax == 2 ? v.A.yp : v.A.zp;
+    T vb[M], vai[M], viw[M], vcp[M], cf[M][4], xn[M][4], pwn[M][4], xcn[M][4];
+    int msk[M];
 #pragma unroll
-    for (int m = 0; m < P * S; m++) {
+    for (int m = 0; m < M; m++) {
+        const int e = threadIdx.x + LS_THREADS * m;
+        int L, q;
+        if (ax == 0) { L = e / n; q = e - L * n; } else { q = e / NLW; L = e - q * NLW; }
+        const int td = 2 * (h0 + L) + sh;
+        const bool ok = e < E && td < g.Ntd;
+        const int k = ok ? td * g.s_td + q * g.s_ax + p3 * g.s_3 : 0;
+        int i, j, l;
+        if (ax == 0) { i = q; j = td; l = p3; } else if (ax == 1) { i = td; j = q; l = p3; } else { i = td; j = p3; l = q; }
+        const int u1 = ax == 0 ? j : i, n1 = ax == 0 ? Ny : Nx;
+        const int u2 = ax == 2 ? j : l, n2 = ax == 2 ? Ny : Nz;
+        const bool hm1 = ok && u1 > 0, hp1 = ok && u1 < n1 - 1, hm2 = ok && u2 > 0, hp2 = ok && u2 < n2 - 1;
+        const int nk[4] = {hm1 ? k - s1 : k, hp1 ? k + s1 : k, hm2 ? k - s2 : k, hp2 ? k + s2 : k};
+        msk[m] = (int)ok | ((int)hm1 << 1) | ((int)hp1 << 2) | ((int)hm2 << 3) | ((int)hp2 << 4);
+        vb[m] = b[k];
+        vai[m] = AI[k]; viw[m] = IW[k]; vcp[m] = CP[k];
+        if (!zero) { cf[m][0] = o0[k]; cf[m][1] = o1[k]; cf[m][2] = o2[k]; cf[m][3] = o3[k]; }
+        if (!zero) {
+#pragma unroll
+            for (int t = 0; t < 4; t++) xn[m][t] = x[nk[t]];
+        }
+        if (pro) {
+            // the neighbours' coarse parents (first post-smoothing sweep)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int d = (t & 1) ? 1 : -1, on = (msk[m] >> (t + 1)) & 1;
+                int ii = i, jj = j, ll = l;
+                if (t < 2) { if (ax == 0) jj += on * d; else ii += on * d; }
+                else { if (ax == 2) jj += on * d; else ll += on * d; }
+                const int I = v.cx ? ii >> 1 : ii, J = v.cy ? jj >> 1 : jj, LL = v.cz ? ll >> 1 : ll;
+                pwn[m][t] = v.pw[nk[t]];
+                xcn[m][t] = c.x[ok ? I + c.Nx * (J + c.Ny * LL) : 0];
+            }
+        }
+    }
+    // second pass: right-hand sides into LDS
+#pragma unroll
+    for (int m = 0; m < M; m++) {
         const int e = threadIdx.x + LS_THREADS * m;
         if (e >= E) break;
         int L, q;
         if (ax == 0) { L = e / n; q = e - L * n; } else { q = e / NLW; L = e - q * NLW; }
-        const int td = tstep * (h0 + L) + sh;
         T r = 0, ai = 0, iw = 0, cp = 0;
-        if (td < g.Ntd) {
-            const int k = td * g.s_td + q * g.s_ax + p3 * g.s_3;
-            int i, j, l;
-            if (ax == 0) { i = q; j = td; l = p3; } else if (ax == 1) { i = td; j = q; l = p3; } else { i = td; j = p3; l = q; }
-            ai = AI[k]; iw = IW[k]; cp = CP[k];
-            T sr = b[k];
+        if (msk[m] & 1) {
+            T sr = vb[m];
             if (!zero) {
-                auto xv = [&](int ii, int jj, int ll, int kk) -> T {
-                    if (!pro) return x[kk];
-                    int I = v.cx ? ii >> 1 : ii, J = v.cy ? jj >> 1 : jj, LL = v.cz ? ll >> 1 : ll;
-                    return x[kk] + v.pw[kk] * c.x[I + c.Nx * (J + c.Ny * LL)];
-                };
-                if (ax != 0) {
-                    if (i > 0) sr -= v.A.xm[k] * xv(i - 1, j, l, k - 1);
-                    if (i < Nx - 1) sr -= v.A.xp[k] * xv(i + 1, j, l, k + 1);
-                }
-                if (ax != 1) {
-                    if (j > 0) sr -= v.A.ym[k] * xv(i, j - 1, l, k - Nx);
-                    if (j < Ny - 1) sr -= v.A.yp[k] * xv(i, j + 1, l, k + Nx);
-                }
-                if (ax != 2) {
-                    if (l > 0) sr -= v.A.zm[k] * xv(i, j, l - 1, k - NxNy);
-                    if (l < Nz - 1) sr -= v.A.zp[k] * xv(i, j, l + 1, k + NxNy);
-                }
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    if ((msk[m] >> (t + 1)) & 1) sr -= cf[m][t] * (pro ? xn[m][t] + pwn[m][t] * xcn[m][t] : xn[m][t]);
             }
-            r = sr;
+            r = sr; ai = vai[m]; iw = viw[m]; cp = vcp[m];
         }
-        const int q0 = (L / LW) * SL + (L % LW) * n + q;
+        const int q0 = (L / LW) * SP + (L % LW) * n + q;
         s_r[q0] = r * iw; s_a[q0] = ai; s_c[q0] = cp;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int si = 0; si < S; si++) {
-        T *R = s_r + (w * S + si) * SL;
-        const T *Aa = s_a + (w * S + si) * SL, *Cc = s_c + (w * S + si) * SL;
+        T *R = s_r + (w * S + si) * SP;
+        const T *Aa = s_a + (w * S + si) * SP, *Cc = s_c + (w * S + si) * SP;
         T ra[P], aa[P], ca[P];
 #pragma unroll
         for (int p = 0; p < P; p++) { ra[p] = R[lane * P + p]; aa[p] = Aa[lane * P + p]; ca[p] = Cc[lane * P + p]; }
@@ -719,31 +743,30 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
         if (e >= E) break;
         int L, q;
         if (ax == 0) { L = e / n; q = e - L * n; } else { q = e / NLW; L = e - q * NLW; }
-        const int td = tstep * (h0 + L) + sh;
-        if (td < g.Ntd) x[td * g.s_td + q * g.s_ax + p3 * g.s_3] = s_r[(L / LW) * SL + (L % LW) * n + q];
+        const int td = 2 * (h0 + L) + sh;
+        if (td < g.Ntd) x[td * g.s_td + q * g.s_ax + p3 * g.s_3] = s_r[(L / LW) * SP + (L % LW) * n + q];
     }
 }
 // slots per wavefront allowed by the 64 KB of LDS a workgroup may hold
 template <class T> constexpr int ls_smax(int P) {
-    return 3 * LS_WAVES * 64 * P * 4 * (int)sizeof(T) <= 65536 ? 4
-         : (3 * LS_WAVES * 64 * P * 2 * (int)sizeof(T) <= 65536 ? 2 : 1);
+    return 3 * LS_WAVES * (64 * P + 1) * 4 * (int)sizeof(T) <= 65536 ? 4
+         : (3 * LS_WAVES * (64 * P + 1) * 2 * (int)sizeof(T) <= 65536 ? 2 : 1);
 }
 template <class T, int P>
 static void ls_launch(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero,
-                      const MGLevel<T> &cc, int pro, int LW, int plane) {
+                      const MGLevel<T> &cc, int pro, int LW) {
     constexpr int SM = ls_smax<T>(P);
     if (S >= 4 && SM >= 4)
-        k_line_scan<T, P, (SM >= 4 ? 4 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW, plane);
+        k_line_scan<T, P, (SM >= 4 ? 4 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW);
     else if (S >= 2 && SM >= 2)
-        k_line_scan<T, P, (SM >= 2 ? 2 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW, plane);
+        k_line_scan<T, P, (SM >= 2 ? 2 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW);
     else
-        k_line_scan<T, P, 1><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW, plane);
+        k_line_scan<T, P, 1><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW);
 }
 
 int g_line_nl_max = 8;       // lines per workgroup cap of k_line_gs (tuning)
 int g_line_scan = 1;         // 1: scan kernel for lines of up to 256 elements
 int g_line_slots = 4;        // most line slots per wavefront of the scan kernel (1, 2, 4)
-int g_line_plane = 0;        // 1: y- and z-lines coloured by plane parity (whole planes per colour)
 template <class T>
 int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero, const MGLevel<T> *c) {
     int n = ax == 0 ? v.Nx : (ax == 1 ? v.Ny : v.Nz);
@@ -752,9 +775,7 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
     const int P = (n + 63) / 64;
     if (g_line_scan && P <= 4) {
         const int LW = 64 * P / n;
-        const int plane = ax != 0 && g_line_plane;
-        // lines per plane and colour, planes per colour
-        const int lpp = plane ? Ntd : (Ntd + 1) / 2, npl = plane ? (N3 - color + 1) / 2 : N3;
+        const int lpp = (Ntd + 1) / 2, npl = N3;      // lines per plane and colour, planes
         // lines across the workgroup (y- and z-lines): widen the contiguous
         // runs with more slots per wavefront while at least 512 workgroups
         // remain (two per CU)
@@ -769,16 +790,16 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
         int Srun = S;
         while (Srun > SM) Srun >>= 1;
         const int NLWr = LS_WAVES * Srun * LW, nb = (lpp + NLWr - 1) / NLWr * npl;
-        if (nb == 0) return plane ? 2 : 1;
+        if (nb == 0) return 1;
         MGLevel<T> cc = c ? *c : v;
         const int pro = c != nullptr;
         switch (P) {
-        case 1: ls_launch<T, 1>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW, plane); break;
-        case 2: ls_launch<T, 2>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW, plane); break;
-        case 3: ls_launch<T, 3>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW, plane); break;
-        default: ls_launch<T, 4>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW, plane); break;
+        case 1: ls_launch<T, 1>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
+        case 2: ls_launch<T, 2>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
+        case 3: ls_launch<T, 3>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
+        default: ls_launch<T, 4>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
         }
-        return plane ? 2 : 1;
+        return 1;
     }
     if (n > LINE_MAXEQ) return 0;
     int NL = LINE_MAXEQ / n;
@@ -796,8 +817,7 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
 // scale (restriction R = P^T W, semiconductor_3d.c:2028-2037), divided by the
 // coarse row scale (the coarse rows are stored with unit diagonal).  Rows
 // solved exactly by the last smoothing sweep have no residual and are
-// skipped: skip 1 = point black nodes, 2+a = colour-1 lines along axis a,
-// 5+a = the same for lines coloured by plane parity.
+// skipped: skip 1 = point black nodes, 2+a = colour-1 lines along axis a.
 // One lane per fine node: the eight lanes of an aggregate are adjacent (lane
 // bits 0, 1, 2 = x, y, z offset in the aggregate), each evaluates its node's
 // weighted residual, and three xor shuffles sum the aggregate, so every lane
@@ -812,12 +832,8 @@ __device__ __forceinline__ double agg_resid(const MGLevel<T> &f, const MGLevel<T
     const int i = f.cx ? 2 * I + di : I, j = f.cy ? 2 * J + dj : J, l = f.cz ? 2 * LL + dl : LL;
     bool ok = (f.cx || !di) && (f.cy || !dj) && (f.cz || !dl) && i < f.Nx && j < f.Ny && l < f.Nz;
     if (ok && skip == 1 && ((i + j + l) & 1)) ok = false;
-    if (ok && skip >= 2 && skip <= 4) {
+    if (ok && skip >= 2) {
         const int ax = skip - 2, par = ax == 0 ? (j + l) : (ax == 1 ? (i + l) : (i + j));
-        if (par & 1) ok = false;
-    }
-    if (ok && skip >= 5) {                           // plane-coloured lines
-        const int ax = skip - 5, par = ax == 0 ? (j + l) : (ax == 1 ? l : j);
         if (par & 1) ok = false;
     }
     if (!ok) return 0.0;
