@@ -1,0 +1,186 @@
+"""CPU restatement of the warm-start network - TEST INFRASTRUCTURE ONLY.
+
+The checker of the device network in semiconductor_devices_simulator_amd/csrc
+(k_nn.hip, nn.cpp): only tests/ may use it.  The network has no counterpart
+in the reference (the learned warm start is new); this module restates the
+same architecture and parameter layout in numpy:
+
+  * voxel v = x + G (y + G z) of a G^3 grid, arrays indexed [z, y, x, c];
+  * conv3x3x3 with zero padding, tap t = (dx+1) + 3 (dy+1) + 9 (dz+1),
+    y[v, n] = b[n] + sum_t sum_k W[t, n, k] x[v + off(t), k];
+  * the U-Net of nn.cpp: e0..e3 encoder (2x2x2 average pooling), u2..u0
+    decoder (nearest upsampling plus skip), ReLU after every convolution but
+    the last, MSE over the 3 output channels;
+  * parameters: per layer W[27][N][K] then b[N], layers 0..7 in order.
+
+quant=True rounds weights and every stored activation and gradient buffer to
+BF16 (round to nearest even) where the device stores BF16, so the device's
+results can be compared at BF16 accuracy.  quant=False is plain float64, the
+form the finite-difference test pins.
+"""
+import numpy as np
+
+
+def bf16(a):
+    """Round to the nearest BF16 value (ties to even), returned as float64."""
+    f = np.asarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def layer_shapes(c):
+    """(Kr, Nr) of the eight convolutions for channels c = (c0, c1, c2, c3)."""
+    c0, c1, c2, c3 = c
+    return [(5, c0), (c0, c1), (c1, c2), (c2, c3), (c2, c1), (c1, c0), (c0, c0), (c0, 3)]
+
+
+def split_params(p, c):
+    out, o = [], 0
+    for K, N in layer_shapes(c):
+        W = p[o:o + 27 * N * K].reshape(27, N, K); o += 27 * N * K
+        b = p[o:o + N]; o += N
+        out.append((W, b))
+    assert o == len(p)
+    return out
+
+
+def nparams(c):
+    return sum(27 * N * K + N for K, N in layer_shapes(c))
+
+
+def _taps():
+    return [(t % 3 - 1, (t // 3) % 3 - 1, t // 9 - 1) for t in range(27)]
+
+
+def conv(x, W, b):
+    """x [G, G, G, K] -> [G, G, G, N]."""
+    G = x.shape[0]
+    xp = np.zeros((G + 2, G + 2, G + 2, x.shape[3]))
+    xp[1:-1, 1:-1, 1:-1] = x
+    y = np.zeros((G, G, G, W.shape[1])) + b
+    for t, (dx, dy, dz) in enumerate(_taps()):
+        sl = xp[1 + dz:1 + dz + G, 1 + dy:1 + dy + G, 1 + dx:1 + dx + G]
+        y += sl @ W[t].T
+    return y
+
+
+def conv_wgrad(dy, x):
+    """dW[t, n, k] = sum_v dy[v, n] x[v + off(t), k]."""
+    G = x.shape[0]
+    xp = np.zeros((G + 2, G + 2, G + 2, x.shape[3]))
+    xp[1:-1, 1:-1, 1:-1] = x
+    out = np.zeros((27, dy.shape[3], x.shape[3]))
+    d2 = dy.reshape(-1, dy.shape[3])
+    for t, (dx, dyy, dz) in enumerate(_taps()):
+        sl = xp[1 + dz:1 + dz + G, 1 + dyy:1 + dyy + G, 1 + dx:1 + dx + G].reshape(-1, x.shape[3])
+        out[t] = d2.T @ sl
+    return out
+
+
+def conv_dgrad(dy, W):
+    """dx[u, k] = sum_t sum_n W[t, n, k] dy[u - off(t), n]."""
+    G = dy.shape[0]
+    dp = np.zeros((G + 2, G + 2, G + 2, dy.shape[3]))
+    dp[1:-1, 1:-1, 1:-1] = dy
+    dx = np.zeros((G, G, G, W.shape[2]))
+    for t, (ox, oy, oz) in enumerate(_taps()):
+        sl = dp[1 - oz:1 - oz + G, 1 - oy:1 - oy + G, 1 - ox:1 - ox + G]
+        dx += sl @ W[t]
+    return dx
+
+
+def pool(x):
+    G = x.shape[0] // 2
+    return x.reshape(G, 2, G, 2, G, 2, -1).mean(axis=(1, 3, 5))
+
+
+def pool_bwd(g):
+    return np.repeat(np.repeat(np.repeat(g, 2, 0), 2, 1), 2, 2) / 8.0
+
+
+def up(x):
+    return np.repeat(np.repeat(np.repeat(x, 2, 0), 2, 1), 2, 2)
+
+
+def up_bwd(g):
+    G = g.shape[0] // 2
+    return g.reshape(G, 2, G, 2, G, 2, -1).sum(axis=(1, 3, 5))
+
+
+def forward(p, c, x5, quant=False):
+    """x5 [G, G, G, 5] -> (output [G, G, G, 3], cache)."""
+    q = bf16 if quant else (lambda a: a)
+    L = [(q(W), b) for W, b in split_params(p, c)]
+    relu = lambda a: np.maximum(a, 0.0)
+    x = q(x5)
+    e0 = q(relu(conv(x, *L[0])))
+    p0 = q(pool(e0))
+    e1 = q(relu(conv(p0, *L[1])))
+    p1 = q(pool(e1))
+    e2 = q(relu(conv(p1, *L[2])))
+    p2 = q(pool(e2))
+    e3 = q(relu(conv(p2, *L[3])))
+    u2 = q(up(e3) + e2)
+    d2 = q(relu(conv(u2, *L[4])))
+    u1 = q(up(d2) + e1)
+    d1 = q(relu(conv(u1, *L[5])))
+    u0 = q(up(d1) + e0)
+    d0 = q(relu(conv(u0, *L[6])))
+    y = conv(d0, *L[7])
+    cache = dict(L=L, x=x, e0=e0, p0=p0, e1=e1, p1=p1, e2=e2, p2=p2, e3=e3, u2=u2, d2=d2, u1=u1, d1=d1, u0=u0,
+                 d0=d0)
+    return y, cache
+
+
+def loss_grad(p, c, x5, tgt, quant=False):
+    """MSE loss and the gradient of every parameter (same layout as p)."""
+    q = bf16 if quant else (lambda a: a)
+    y, C = forward(p, c, x5, quant)
+    L = C["L"]
+    n = y.size
+    loss = float(np.mean((y - tgt) ** 2))
+    gy = q(2.0 * (y - tgt) / n)
+    grads = [None] * 8
+
+    def back(i, dy, inp, want_dx=True):
+        W, _ = L[i]
+        grads[i] = (conv_wgrad(dy, inp), dy.reshape(-1, dy.shape[3]).sum(0))
+        return q(conv_dgrad(dy, W)) if want_dx else None
+
+    mask = lambda a: (a > 0).astype(np.float64)
+    gd0 = back(7, gy, C["d0"])
+    gd0 = q(gd0 * mask(C["d0"]))
+    gu0 = back(6, gd0, C["u0"])
+    gd1 = q(up_bwd(gu0))
+    gd1 = q(gd1 * mask(C["d1"]))
+    gu1 = back(5, gd1, C["u1"])
+    gd2 = q(up_bwd(gu1))
+    gd2 = q(gd2 * mask(C["d2"]))
+    gu2 = back(4, gd2, C["u2"])
+    ge3 = q(up_bwd(gu2))
+    ge3 = q(ge3 * mask(C["e3"]))
+    gp2 = back(3, ge3, C["p2"])
+    ge2 = q(pool_bwd(gp2))
+    ge2 = q((ge2 + gu2) * mask(C["e2"]))
+    gp1 = back(2, ge2, C["p1"])
+    ge1 = q(pool_bwd(gp1))
+    ge1 = q((ge1 + gu1) * mask(C["e1"]))
+    gp0 = back(1, ge1, C["p0"])
+    ge0 = q(pool_bwd(gp0))
+    ge0 = q((ge0 + gu0) * mask(C["e0"]))
+    back(0, ge0, C["x"], want_dx=False)
+    flat = []
+    for gW, gb in grads:
+        flat.append(gW.ravel())
+        flat.append(gb)
+    return loss, np.concatenate(flat)
+
+
+def planar_to_grid(a, G, C):
+    """[C][G^3] planar (voxel x + G (y + G z)) -> [G, G, G, C]."""
+    return np.asarray(a, dtype=np.float64).reshape(C, G, G, G).transpose(1, 2, 3, 0)
+
+
+def grid_to_planar(a):
+    return np.ascontiguousarray(a.transpose(3, 0, 1, 2)).reshape(a.shape[3], -1)

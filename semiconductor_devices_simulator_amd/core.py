@@ -21,6 +21,7 @@ I = ctypes.c_int
 LL = ctypes.c_longlong
 DP = ctypes.POINTER(D)
 IP = ctypes.POINTER(I)
+FP = ctypes.POINTER(ctypes.c_float)
 
 # (name, restype, argtypes): the reference's hard and soft bindings
 # (main.py:172-217, 3485, 3494-3513; validate.py:31-49 for the test hooks).
@@ -88,6 +89,23 @@ EXT_SIGNATURES = [
     ("sdx_tune", None, [I, I]),
     ("sdx_ktimer_enable", None, [I, I]),
     ("sdx_ktimer_read", I, [DP]),
+    ("sdx_ws_mode", None, [I]),
+    ("sdx_ws_stats", None, [DP]),
+    ("sdx_ws_reset_stats", None, []),
+    ("sdx_nn_init", I, [I, I, I, I, I, ctypes.c_uint]),
+    ("sdx_nn_free", None, []),
+    ("sdx_nn_nparams", I, []),
+    ("sdx_nn_get_params", I, [FP, I]),
+    ("sdx_nn_set_params", I, [FP, I]),
+    ("sdx_nn_set_scale", None, [D]),
+    ("sdx_nn_get_scale", D, []),
+    ("sdx_nn_forward_host", I, [FP, FP]),
+    ("sdx_nn_grad_host", D, [FP, FP, FP]),
+    ("sdx_nn_dataset_alloc", I, [I]),
+    ("sdx_nn_dataset_size", I, []),
+    ("sdx_nn_dataset_get", I, [I, FP, FP]),
+    ("sdx_nn_train", D, [I, I, D, ctypes.c_uint, FP]),
+    ("sdx_nn_eval", D, [I, I]),
     ("sdx_test_prepare", I, []),
     ("sdx_test_set_state", None, [DP, DP, DP]),
     ("sdx_test_get_node", None, [I, DP]),
@@ -144,3 +162,7 @@ def dptr(a):
 
 def iptr(a):
     return a.ctypes.data_as(IP)
+
+
+def fptr(a):
+    return a.ctypes.data_as(FP)

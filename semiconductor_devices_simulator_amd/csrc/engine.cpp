@@ -8,6 +8,7 @@
 // work runs on the device (k_physics.hip, k_linalg.hip); the host keeps the
 // scalar decisions, reading a handful of reduction results per iteration.
 #include "engine.h"
+#include "sd_nn.h"
 #include <chrono>
 #include <cstdlib>
 #include <algorithm>
@@ -84,6 +85,8 @@ void eng_free() {
     hier_free(G.H64);
     hier_free(G.H32);
     G.d_b32 = G.d_x32 = nullptr;
+    G.d_ws[0] = G.d_ws[1] = G.d_ws[2] = nullptr;
+    G.ws_have = 0;
     G.allocated = 0;
     G.have_solution = 0;
 }
@@ -1282,6 +1285,59 @@ static double gummel_tol() {
     return g;
 }
 
+// ------------------------------------------------------------ learned warm start
+// The plain continuation's predicted state of a bias point (secant or region
+// shift, boundary values applied) is the network's input; its output is the
+// correction toward the converged state.  Mode 1 records (prediction,
+// converged state) pairs as training samples; mode 2 applies the correction
+// and keeps it only when it lowers the Poisson residual of the starting
+// state (otherwise the plain prediction is restored).
+static void ws_save_pred() {
+    if (!G.d_ws[0])
+        for (int a = 0; a < 3; a++) {
+            G.d_ws[a] = dalloc<double>(G.N);
+            if (!G.d_ws[a]) return;
+        }
+    sd_copy(G.st, G.d_ws[0], G.d_psi, G.N);
+    sd_copy(G.st, G.d_ws[1], G.d_phin, G.N);
+    sd_copy(G.st, G.d_ws[2], G.d_phip, G.N);
+    G.ws_have = 1;
+}
+static void ws_before(const double *Va, double dvn) {
+    if (!nn_ready()) return;
+    hipStream_t s = G.st;
+    const int Gg = nn_grid();
+    ws_save_pred();
+    if (!G.ws_have) return;
+    nn_mesh_to_grid(s, Gg, G.Nx, G.Ny, G.Nz, G.d_psi, G.d_phin, G.d_phip, G.d_Nd, G.d_Na, (float)dvn, NN_CIN,
+                    nn_input_buf());
+    if (G.ws_mode != 2) return;
+    G.ws_stat[0]++;
+    double r0 = poisson_eval(G.d_psi, 0, G.d_Ft);
+    const float *y = nn_infer(s);
+    nn_grid_to_mesh_add(s, Gg, G.Nx, G.Ny, G.Nz, y, NN_COUT, nn_target_scale(), G.d_kind, G.d_con_mask, G.d_psi,
+                        G.d_phin, G.d_phip);
+    eng_clamp_qf();
+    apply_bc(Va);
+    eng_rebuild_np();
+    double r1 = poisson_eval(G.d_psi, 0, G.d_Ft);
+    if (G.verbose > 1) fprintf(stderr, "semisim-hip: warm start residual %.3e (plain %.3e)\n", r1, r0);
+    if (r1 < r0) { G.ws_stat[1]++; return; }
+    G.ws_stat[2]++;
+    sd_copy(s, G.d_psi, G.d_ws[0], G.N);
+    sd_copy(s, G.d_phin, G.d_ws[1], G.N);
+    sd_copy(s, G.d_phip, G.d_ws[2], G.N);
+    eng_rebuild_np();
+}
+static void ws_fill_target(hipStream_t s, float *tgt, void *) {
+    nn_target_to_grid(s, nn_grid(), G.Nx, G.Ny, G.Nz, G.d_ws[0], G.d_ws[1], G.d_ws[2], G.d_psi, G.d_phin, G.d_phip,
+                      1.0f / nn_target_scale(), tgt);
+}
+static void ws_capture() {
+    if (!nn_ready()) return;
+    if (nn_capture(G.st, ws_fill_target, nullptr)) G.ws_stat[3]++;
+}
+
 static int cont_path() {
     const int nc = G.n_cons, N = G.N;
     double Vf[SD_NC_MAX], Vt[SD_NC_MAX], Va[SD_NC_MAX], dV[SD_NC_MAX];
@@ -1311,6 +1367,10 @@ static int cont_path() {
     }
     int first = 1;
     double t = 0., tp = -1., dt = span > 0. ? fmin(1., 0.1 / span) : 1.;
+    // bias step of the contact that moves most, in units of 50 mV (warm-start input)
+    double dvn = 0.;
+    for (int c = 0; c < nc; c++)
+        if (fabs(Vt[c] - Vf[c]) >= span && span > 0.) dvn = (Vt[c] - Vf[c]) / 0.05;
     int ok_all = 0, its = 0, fails = 0, cfail = 0, steps = 0;
     double res = 0.;
     int max_cfail = (G.models & 256) ? 3 : (1 << 30);
@@ -1342,6 +1402,7 @@ static int cont_path() {
         eng_clamp_qf();
         apply_bc(Va);
         eng_rebuild_np();
+        if (guard == 0 && tp < 0. && span > 0. && G.ws_mode) ws_before(Va, dvn);
         int ok = gummel(Va, maxg, gtol, &its, &res);
         G.conv_iter = its;
         G.conv_res = res;
@@ -1363,6 +1424,9 @@ static int cont_path() {
             dt *= 0.3;
         }
     }
+    // a bias point reached in one step from the saved prediction: training sample
+    if (ok_all && span > 0. && G.ws_mode == 1 && steps == 1 && fails == 0 && G.ws_have) ws_capture();
+    G.ws_have = 0;
     if (ok_all && span > 0.) {
         for (int c = 0; c < nc; c++) { G.Ve2[c] = Vf[c]; G.Ve1[c] = Vt[c]; }
         G.e_n = G.e_n < 2 ? G.e_n + 1 : 2;
@@ -1546,6 +1610,11 @@ void sdx_tune(int what, int value) {
         if (G.H32.alloc) hier_tail(G.H32);
     }
 }
+// learned warm start: 0 off, 1 record training samples, 2 apply the network
+void sdx_ws_mode(int mode) { G.ws_mode = mode == 1 || mode == 2 ? mode : 0; }
+// out[0] predictions tried, [1] kept, [2] rejected (plain prediction used), [3] samples recorded
+void sdx_ws_stats(double *out) { for (int i = 0; i < 4; i++) out[i] = (double)G.ws_stat[i]; }
+void sdx_ws_reset_stats(void) { for (int i = 0; i < 4; i++) G.ws_stat[i] = 0; }
 void sdx_ktimer_enable(int kind, int every) {
     kt_flush();
     KT.on = 1; KT.kind = kind; KT.every = every > 0 ? every : 8;

@@ -99,6 +99,10 @@ struct Engine {
     int tail_sweeps = 1;                      // red-black sweeps per side in the tail kernel
     int tail_n = 1024;                        // levels of at most this many nodes run in the tail kernel
     int use_graphs = 0;                       // replay the captured V-cycle (SEMISIM_GRAPHS=1)
+    // learned warm start: mode, saved prediction of the running bias point
+    int ws_mode = 0, ws_have = 0;
+    double *d_ws[3] = {nullptr, nullptr, nullptr};
+    long long ws_stat[4] = {0, 0, 0, 0};
 
     // Anderson
     int aa_m = 12, aa_n = 0, aa_pos = 0, aa_have = 0;
