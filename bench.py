@@ -10,11 +10,14 @@ already resident on the device.  W warmup points are solved untimed, then K
 points are timed (W + K <= 50).  --mesh 96 64 64 runs the 393,216-node mesh
 instead.
 
-Two solver configurations run back to back on the same sweep: the plain FP64
-method (FP64 multigrid) and the mixed-precision method (FP32 multigrid,
-smoothers and preconditioner SpMV inside FP64 Krylov/Newton).  `value` is the
-mixed-precision rate; the plain FP64 rate and the speedup are reported beside
-it.  Parity: the terminal currents of every timed point of each method against
+Up to three solver configurations run back to back on the same sweep: the
+plain FP64 method (FP64 multigrid), the mixed-precision method (FP32
+multigrid, smoothers and preconditioner SpMV inside FP64 Krylov/Newton), and
+the mixed-precision method seeded by the learned warm start when the trained
+network (semiconductor_devices_simulator_amd/warmstart_unet64.npz) is present.
+`value` is the rate of the last (the headline metric's method), or of the
+mixed-precision method without a trained network; the others and the speedup
+over plain FP64 are reported beside it.  Parity: the terminal currents of every timed point of each method against
 the reference core's sweep of the same mesh when its fixture is present
 (tests/golden/e2e_mosfet_<mesh>.npz), and the mixed-precision currents and
 final fields against the plain FP64 GPU run.
@@ -73,7 +76,7 @@ def solve_points(lib, cids, pts, i0, i1, floors=None):
     return np.array(I)
 
 
-def run_gpu(lib, spec, precision, warmup, steps, ktimer, keep=()):
+def run_gpu(lib, spec, precision, warmup, steps, ktimer, keep=(), ws=0):
     """Solve the sweep's first warmup points untimed, then time `steps`
     points; returns (seconds, currents, current floors, kernel timer,
     iteration counts, {index: (psi, phi_n, phi_p)} of the converged states
@@ -82,8 +85,11 @@ def run_gpu(lib, spec, precision, warmup, steps, ktimer, keep=()):
     from semiconductor_devices_simulator_amd.core import dptr
     cids = devices.load_device(lib, spec)
     assert lib.sdx_set_precision(precision) == 1
+    lib.sdx_ws_mode(0)
     if not lib.api3_solve():
         raise RuntimeError("equilibrium solve failed")
+    lib.sdx_ws_mode(ws)
+    lib.sdx_ws_reset_stats()
     pts = sweep_points(spec)
     solve_points(lib, cids, pts, 0, warmup)
     g0 = [lib.api3_get_stat(i) for i in range(4)]
@@ -102,6 +108,7 @@ def run_gpu(lib, spec, precision, warmup, steps, ktimer, keep=()):
     if ktimer:
         lib.sdx_ktimer_read(kt.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     g1 = [lib.api3_get_stat(i) for i in range(4)]
+    lib.sdx_ws_mode(0)
     its = dict(pcg=g1[0] - g0[0], bicgstab=g1[1] - g0[1], newton=g1[2] - g0[2], gummel=g1[3] - g0[3])
     return dt, np.array(I), np.array(F), kt, its, states
 
@@ -202,6 +209,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mesh", type=int, nargs=3, default=[192, 128, 128])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-baseline-worker", default="")
     a = ap.parse_args()
     if a.cpu_baseline_worker:
@@ -211,7 +219,7 @@ def main():
         print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); the sweep is not sharded "
               "across GPUs", file=sys.stderr)
         return 2
-    from semiconductor_devices_simulator_amd import core, devices
+    from semiconductor_devices_simulator_amd import core, devices, warmstart
     lib = core.load_core()
     spec = devices.mosfet(*a.mesh)
     npts = spec.bias_points()
@@ -229,6 +237,16 @@ def main():
     print(f"[bench] plain FP64: {K / t64:.3f} points/s", file=sys.stderr, flush=True)
     t32, I32, _, kt, its32, st32 = run_gpu(lib, spec, 1, W, K, ktimer=True, keep=[last])
     print(f"[bench] FP32 multigrid: {K / t32:.3f} points/s", file=sys.stderr, flush=True)
+    learned = None
+    if os.path.isfile(warmstart.MODEL) and not a.no_learned:
+        meta = warmstart.load(lib, warmstart.MODEL)
+        tL, IL, _, _, itsL, stL = run_gpu(lib, spec, 1, W, K, ktimer=False, keep=[last], ws=2)
+        learned = dict(points_per_s=round(K / tL, 4), seconds=tL, iterations=itsL, warm_start=warmstart.ws_stats(lib),
+                       model=dict(file=os.path.relpath(warmstart.MODEL, ROOT), samples=meta.get("samples"),
+                                  heldout_loss=meta.get("heldout_loss_after")),
+                       I=IL, last=stL.get(last))
+        lib.sdx_nn_free()
+        print(f"[bench] learned warm start + FP32 multigrid: {K / tL:.3f} points/s", file=sys.stderr, flush=True)
     roof = None
     if kt[0] > 0:
         avg_s = kt[1] / kt[0] * 1e-3
@@ -239,7 +257,10 @@ def main():
                     bytes_per_launch=kt[2], avg_launch_us=round(avg_s * 1e6, 2), launches_timed=int(kt[0]))
     cpu_states = {i: st for i, st in states.items() if i != last or i in windows}
     cpu = None if a.no_cpu_baseline else cpu_baseline(cpu_states, a.mesh)
-    value = K / t32
+    # the metric's method: learned warm start + FP32 multigrid when the
+    # trained network is present, else FP32 multigrid alone
+    t_val = learned["seconds"] if learned else t32
+    value = K / t_val
     out = {
         "metric": "bias points/s at 1e-8 rel current err, learned+FP32-AMG vs plain FP64 on GPU",
         "value": round(value, 4),
@@ -247,19 +268,25 @@ def main():
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": round(1e3 * t32 / K, 2),
+        "ms_per_step": round(1e3 * t_val / K, 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64 (Krylov, Newton, assembly) + f32 multigrid",
         "data": "synthetic",
-        "config": {"workload": workload, "nodes": spec.N, "mesh": list(spec.shape), "method": "FP32-AMG",
-                   "learned_warm_start": False},
+        "config": {"workload": workload, "nodes": spec.N, "mesh": list(spec.shape),
+                   "method": "learned warm start + FP32-AMG" if learned else "FP32-AMG",
+                   "learned_warm_start": bool(learned)},
         "plain_fp64_points_per_s": round(K / t64, 4),
-        "speedup_vs_plain_fp64": round(t64 / t32, 3),
-        "iterations": {"fp32_mg": its32, "fp64": its64},
+        "fp32_mg_points_per_s": round(K / t32, 4),
+        "speedup_vs_plain_fp64": round(t64 / t_val, 3),
+        "iterations": {"fp32_mg": its32, "fp64": its64, "learned_fp32_mg": learned["iterations"] if learned else None},
+        "warm_start": ({k: learned[k] for k in ("warm_start", "model")} if learned else None),
         "parity": {"fp32_mg": parity(I32, W, K, a.mesh), "fp64": parity(I64, W, K, a.mesh),
-                   "fp32_mg_vs_fp64": mixed_vs_fp64(I32, I64, F64, st32.get(last), states.get(last))},
+                   "learned_fp32_mg": parity(learned["I"], W, K, a.mesh) if learned else None,
+                   "fp32_mg_vs_fp64": mixed_vs_fp64(I32, I64, F64, st32.get(last), states.get(last)),
+                   "learned_fp32_mg_vs_fp64": (mixed_vs_fp64(learned["I"], I64, F64, learned["last"], states.get(last))
+                                               if learned else None)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -35,13 +35,16 @@ int  sdx_ktimer_read(double *out);
 double sdx_microbench(int what, int iters);
 void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS line kernel, 1: line-smoothed
                                               levels, 2: tail sweeps, 3: tail size, 4: wavefront-scan line
-                                              kernel on (1) or off (0) */
+                                              kernel on (1) or off (0), 5: most line slots per wavefront,
+                                              7: fewest workgroups a wider slot choice may leave */
 
 /* ---- learned warm start ----
    A 3-D convolutional U-Net on a G^3 grid resampled from the mesh (index
-   space), BF16 matrix-core kernels with FP32 accumulation.  Input: the plain
-   continuation's predicted psi, phi_n, phi_p, signed doping log-magnitude and
-   bias step; output: the correction toward the converged state.
+   space), BF16 matrix-core kernels with FP32 accumulation.  Input (8
+   channels): the plain continuation's predicted psi, phi_n, phi_p, signed
+   doping log-magnitude, bias step, and the predicted change of psi, phi_n,
+   phi_p from the last converged state; output: the correction toward the
+   converged state.
    sdx_ws_mode: 0 off, 1 record (prediction, converged) training samples at
    each bias point reached in one step, 2 apply the network's correction and
    keep it only if it lowers the starting Poisson residual.
@@ -57,11 +60,11 @@ int    sdx_nn_get_params(float *out, int n);            /* per layer W[27][N][K]
 int    sdx_nn_set_params(const float *in, int n);
 void   sdx_nn_set_scale(double volts);                  /* target scale: output x scale = correction */
 double sdx_nn_get_scale(void);
-int    sdx_nn_forward_host(const float *in5, float *out3);          /* planar [c][x + G (y + G z)] */
-double sdx_nn_grad_host(const float *in5, const float *tgt3, float *grad);   /* returns the MSE loss */
+int    sdx_nn_forward_host(const float *in8, float *out3);          /* planar [c][x + G (y + G z)] */
+double sdx_nn_grad_host(const float *in8, const float *tgt3, float *grad);   /* returns the MSE loss */
 int    sdx_nn_dataset_alloc(int nmax);
 int    sdx_nn_dataset_size(void);
-int    sdx_nn_dataset_get(int i, float *in5, float *tgt3);
+int    sdx_nn_dataset_get(int i, float *in8, float *tgt3);
 double sdx_nn_train(int steps, int n_train, double lr, unsigned seed, float *loss);  /* Adam; mean loss */
 double sdx_nn_eval(int i0, int i1);                     /* mean loss over samples [i0, i1) */
 

@@ -20,6 +20,8 @@ form the finite-difference test pins.
 """
 import numpy as np
 
+NIN = 8            # input channels
+
 
 def bf16(a):
     """Round to the nearest BF16 value (ties to even), returned as float64."""
@@ -32,7 +34,7 @@ def bf16(a):
 def layer_shapes(c):
     """(Kr, Nr) of the eight convolutions for channels c = (c0, c1, c2, c3)."""
     c0, c1, c2, c3 = c
-    return [(5, c0), (c0, c1), (c1, c2), (c2, c3), (c2, c1), (c1, c0), (c0, c0), (c0, 3)]
+    return [(NIN, c0), (c0, c1), (c1, c2), (c2, c3), (c2, c1), (c1, c0), (c0, c0), (c0, 3)]
 
 
 def split_params(p, c):
@@ -109,7 +111,7 @@ def up_bwd(g):
 
 
 def forward(p, c, x5, quant=False):
-    """x5 [G, G, G, 5] -> (output [G, G, G, 3], cache)."""
+    """x5 [G, G, G, NIN] -> (output [G, G, G, 3], cache)."""
     q = bf16 if quant else (lambda a: a)
     L = [(q(W), b) for W, b in split_params(p, c)]
     relu = lambda a: np.maximum(a, 0.0)

@@ -1309,8 +1309,9 @@ static void ws_before(const double *Va, double dvn) {
     const int Gg = nn_grid();
     ws_save_pred();
     if (!G.ws_have) return;
-    nn_mesh_to_grid(s, Gg, G.Nx, G.Ny, G.Nz, G.d_psi, G.d_phin, G.d_phip, G.d_Nd, G.d_Na, (float)dvn, NN_CIN,
-                    nn_input_buf());
+    // d_h1: the last converged state (saved at the start of the bias point)
+    nn_mesh_to_grid(s, Gg, G.Nx, G.Ny, G.Nz, G.d_psi, G.d_phin, G.d_phip, G.d_Nd, G.d_Na, (float)dvn, G.d_h1[0],
+                    G.d_h1[1], G.d_h1[2], NN_CIN, nn_input_buf());
     if (G.ws_mode != 2) return;
     G.ws_stat[0]++;
     double r0 = poisson_eval(G.d_psi, 0, G.d_Ft);
@@ -1602,6 +1603,7 @@ void sdx_tune(int what, int value) {
     if (what == 0) g_line_nl_max = value;
     if (what == 4) g_line_scan = value;
     if (what == 5) g_line_slots = value >= 4 ? 4 : (value >= 2 ? 2 : 1);
+    if (what == 7) g_line_minwg = value > 0 ? value : 1;
     if (what == 1) G.line_levels = value;
     if (what == 2) G.tail_sweeps = value > 0 ? value : 1;
     if (what == 3) {

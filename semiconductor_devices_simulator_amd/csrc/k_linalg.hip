@@ -767,6 +767,7 @@ static void ls_launch(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T
 int g_line_nl_max = 8;       // lines per workgroup cap of k_line_gs (tuning)
 int g_line_scan = 1;         // 1: scan kernel for lines of up to 256 elements
 int g_line_slots = 4;        // most line slots per wavefront of the scan kernel (1, 2, 4)
+int g_line_minwg = 512;      // fewest workgroups a wider slot choice may leave
 template <class T>
 int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero, const MGLevel<T> *c) {
     int n = ax == 0 ? v.Nx : (ax == 1 ? v.Ny : v.Nz);
@@ -777,13 +778,13 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
         const int LW = 64 * P / n;
         const int lpp = (Ntd + 1) / 2, npl = N3;      // lines per plane and colour, planes
         // lines across the workgroup (y- and z-lines): widen the contiguous
-        // runs with more slots per wavefront while at least 512 workgroups
-        // remain (two per CU)
+        // runs with more slots per wavefront while at least g_line_minwg
+        // workgroups remain
         int S = 1;
         if (ax != 0)
             for (int sc = g_line_slots; sc > 1; sc >>= 1) {
                 int nlw = LS_WAVES * sc * LW;
-                if ((long)((lpp + nlw - 1) / nlw) * npl >= 512) { S = sc; break; }
+                if ((long)((lpp + nlw - 1) / nlw) * npl >= g_line_minwg) { S = sc; break; }
             }
         // at most the slots the LDS allows (ls_launch makes the same choice)
         const int SM = P == 1 ? ls_smax<T>(1) : (P == 2 ? ls_smax<T>(2) : (P == 3 ? ls_smax<T>(3) : ls_smax<T>(4)));

@@ -303,8 +303,11 @@ __device__ __forceinline__ void nn_lin(float p, int n, int &i0, int &i1, float &
 }
 // network input channels of one sample (C channels, channel-last BF16):
 // 0 psi, 1 phi_n, 2 phi_p (V), 3 signed doping log-magnitude, 4 bias step
+// (units of 50 mV), 5-7 (psi, phi_n, phi_p) - h (the last converged state),
+// in units of 50 mV (0 without h)
 __global__ void k_mesh_to_grid(int G, int Nx, int Ny, int Nz, const double *psi, const double *phin,
-                               const double *phip, const double *Nd, const double *Na, float dv, int C, __bf16 *in) {
+                               const double *phip, const double *Nd, const double *Na, float dv, const double *h0,
+                               const double *h1, const double *h2, int C, __bf16 *in) {
     const int v = blockIdx.x * 256 + threadIdx.x;
     if (v >= G * G * G) return;
     const int a = v % G, b = (v / G) % G, c = v / (G * G);
@@ -313,7 +316,7 @@ __global__ void k_mesh_to_grid(int G, int Nx, int Ny, int Nz, const double *psi,
     nn_lin((a + 0.5f) * Nx / G - 0.5f, Nx, i0, i1, wx);
     nn_lin((b + 0.5f) * Ny / G - 0.5f, Ny, j0, j1, wy);
     nn_lin((c + 0.5f) * Nz / G - 0.5f, Nz, l0, l1, wz);
-    float f[4] = {0.f, 0.f, 0.f, 0.f};
+    float f[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int d = 0; d < 8; d++) {
         const int i = (d & 1) ? i1 : i0, j = (d & 2) ? j1 : j0, l = (d & 4) ? l1 : l0;
         const float w = ((d & 1) ? wx : 1.f - wx) * ((d & 2) ? wy : 1.f - wy) * ((d & 4) ? wz : 1.f - wz);
@@ -322,14 +325,22 @@ __global__ void k_mesh_to_grid(int G, int Nx, int Ny, int Nz, const double *psi,
         const double lg = log10(fabs(net) * 1e-6 + 1.0);       // cm^-3
         f[0] += w * (float)psi[k]; f[1] += w * (float)phin[k]; f[2] += w * (float)phip[k];
         f[3] += w * (float)(net >= 0. ? lg : -lg) / 20.f;
+        if (h0) {
+            f[4] += w * (float)((psi[k] - h0[k]) * 20.0);
+            f[5] += w * (float)((phin[k] - h1[k]) * 20.0);
+            f[6] += w * (float)((phip[k] - h2[k]) * 20.0);
+        }
     }
     __bf16 *o = in + (size_t)v * C;
     for (int q = 0; q < C; q++) o[q] = (__bf16)0.f;
     o[0] = (__bf16)f[0]; o[1] = (__bf16)f[1]; o[2] = (__bf16)f[2]; o[3] = (__bf16)f[3]; o[4] = (__bf16)dv;
+    o[5] = (__bf16)f[4]; o[6] = (__bf16)f[5]; o[7] = (__bf16)f[6];
 }
 void nn_mesh_to_grid(hipStream_t s, int G, int Nx, int Ny, int Nz, const double *psi, const double *phin,
-                     const double *phip, const double *Nd, const double *Na, float dv, int C, __bf16 *in) {
-    k_mesh_to_grid<<<(G * G * G + 255) / 256, 256, 0, s>>>(G, Nx, Ny, Nz, psi, phin, phip, Nd, Na, dv, C, in);
+                     const double *phip, const double *Nd, const double *Na, float dv, const double *h0,
+                     const double *h1, const double *h2, int C, __bf16 *in) {
+    k_mesh_to_grid<<<(G * G * G + 255) / 256, 256, 0, s>>>(G, Nx, Ny, Nz, psi, phin, phip, Nd, Na, dv, h0, h1, h2,
+                                                            C, in);
 }
 // training target: (x1 - x0) / scale resampled to the grid, 3 FP32 channels
 __global__ void k_target_to_grid(int G, int Nx, int Ny, int Nz, const double *a0, const double *a1, const double *a2,

@@ -1,10 +1,12 @@
 // nn.cpp - the warm-start network: a 3-D convolutional U-Net on a G^3 grid
 // (four levels G, G/2, G/4, G/8 with c0..c3 channels), trained and run on the
 // device with the kernels of k_nn.hip (BF16 matrix-core convolutions, FP32
-// accumulation, FP32 master weights and Adam).  Input: psi, phi_n, phi_p, the
-// signed doping log-magnitude and the bias step of the plain continuation's
-// predicted state; output: the correction (delta psi, delta phi_n, delta
-// phi_p) / scale that takes the predicted state toward the converged one.
+// accumulation, FP32 master weights and Adam).  Input (NN_IN channels): psi,
+// phi_n, phi_p of the plain continuation's predicted state, the signed doping
+// log-magnitude, the bias step, and the predicted change of psi, phi_n, phi_p
+// from the last converged state; output: the correction (delta psi, delta
+// phi_n, delta phi_p) / scale that takes the predicted state toward the
+// converged one.
 //
 //   e0 = relu(conv0(x))        G    c0      u2 = up(e3) + e2   G/4  c2
 //   e1 = relu(conv1(pool e0))  G/2  c1      d2 = relu(conv4(u2))  c1
@@ -54,7 +56,7 @@ struct Net {
     size_t part_n = 0;
     int lblk = 256;
     float *tgt = nullptr;          // target of a host-supplied sample (V x 3)
-    float *planar = nullptr;       // host-layout staging (5 V or 3 V floats)
+    float *planar = nullptr;       // host-layout staging (NN_IN V floats)
     // dataset on the device
     int ds_max = 0, ds_n = 0;
     __bf16 *ds_x = nullptr;
@@ -215,7 +217,7 @@ int sdx_nn_init(int G, int c0, int c1, int c2, int c3, unsigned seed) {
     if (hipStreamCreateWithFlags(&n.st, hipStreamNonBlocking) != hipSuccess) { delete np; return 0; }
     const int G0 = G, G1 = G / 2, G2 = G / 4, G3 = G / 8;
     // G, Kr, K, Nr, N, relu
-    const int spec[8][6] = {{G0, 5, NN_CIN, c0, c0, 1}, {G1, c0, c0, c1, c1, 1}, {G2, c1, c1, c2, c2, 1},
+    const int spec[8][6] = {{G0, NN_IN, NN_CIN, c0, c0, 1}, {G1, c0, c0, c1, c1, 1}, {G2, c1, c1, c2, c2, 1},
                             {G3, c2, c2, c3, c3, 1},    {G2, c2, c2, c1, c1, 1}, {G1, c1, c1, c0, c0, 1},
                             {G0, c0, c0, c0, c0, 1},    {G0, c0, c0, 3, NN_COUT, 0}};
     size_t ofs = 0;
@@ -262,7 +264,7 @@ int sdx_nn_init(int G, int c0, int c1, int c2, int c3, unsigned seed) {
     n.part = nalloc<float>(n, n.part_n);
     n.lpart = nalloc<float>(n, n.lblk);
     n.tgt = nalloc<float>(n, V0 * 3);
-    n.planar = nalloc<float>(n, V0 * 5);
+    n.planar = nalloc<float>(n, V0 * NN_IN);
     // He initialisation (the output layer scaled down)
     std::mt19937 rng(seed);
     std::vector<float> h(nalloc_p, 0.f);
@@ -301,14 +303,14 @@ int sdx_nn_set_params(const float *in, int n) {
 void sdx_nn_set_scale(double s) { if (g_net && s > 0.) g_net->scale = (float)s; }
 double sdx_nn_get_scale(void) { return g_net ? g_net->scale : 0.; }
 
-// forward on a host sample: in5 = 5 planar channels of G^3 floats (voxel
+// forward on a host sample: in = NN_IN planar channels of G^3 floats (voxel
 // x + G (y + G z)); out3 = the 3 output channels, planar
-int sdx_nn_forward_host(const float *in5, float *out3) {
+int sdx_nn_forward_host(const float *in, float *out3) {
     if (!g_net) return 0;
     Net &n = *g_net;
     const size_t V = vox(n.G);
-    NC(hipMemcpyAsync(n.planar, in5, 5 * V * sizeof(float), hipMemcpyHostToDevice, n.st));
-    nn_planar_to_cl(n.st, (int)V, 5, NN_CIN, n.planar, n.x);
+    NC(hipMemcpyAsync(n.planar, in, NN_IN * V * sizeof(float), hipMemcpyHostToDevice, n.st));
+    nn_planar_to_cl(n.st, (int)V, NN_IN, NN_CIN, n.planar, n.x);
     forward(n, n.x);
     std::vector<float> y(V * NN_COUT);
     NC(hipMemcpyAsync(y.data(), n.yf, y.size() * sizeof(float), hipMemcpyDeviceToHost, n.st));
@@ -318,16 +320,16 @@ int sdx_nn_forward_host(const float *in5, float *out3) {
     return !g_nn_err;
 }
 // loss and parameter gradients (no update) for a host sample; tgt3 planar
-double sdx_nn_grad_host(const float *in5, const float *tgt3, float *grad) {
+double sdx_nn_grad_host(const float *in, const float *tgt3, float *grad) {
     if (!g_net) return -1.;
     Net &n = *g_net;
     const size_t V = vox(n.G);
     std::vector<float> t(V * 3);
     for (int c = 0; c < 3; c++)
         for (size_t v = 0; v < V; v++) t[v * 3 + c] = tgt3[c * V + v];
-    NC(hipMemcpyAsync(n.planar, in5, 5 * V * sizeof(float), hipMemcpyHostToDevice, n.st));
+    NC(hipMemcpyAsync(n.planar, in, NN_IN * V * sizeof(float), hipMemcpyHostToDevice, n.st));
     NC(hipMemcpyAsync(n.tgt, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice, n.st));
-    nn_planar_to_cl(n.st, (int)V, 5, NN_CIN, n.planar, n.x);
+    nn_planar_to_cl(n.st, (int)V, NN_IN, NN_CIN, n.planar, n.x);
     float loss = loss_grad(n, n.x, n.tgt);
     if (grad) NC(hipMemcpy(grad, n.Gr, n.np * sizeof(float), hipMemcpyDeviceToHost));
     return g_nn_err ? -1. : (double)loss;
@@ -350,7 +352,7 @@ int sdx_nn_dataset_alloc(int nmax) {
 }
 int sdx_nn_dataset_size(void) { return g_net ? g_net->ds_n : 0; }
 // read sample i back (planar, 5 input and 3 target channels) for inspection
-int sdx_nn_dataset_get(int i, float *in5, float *tgt3) {
+int sdx_nn_dataset_get(int i, float *in, float *tgt3) {
     if (!g_net || i < 0 || i >= g_net->ds_n) return 0;
     Net &n = *g_net;
     const size_t V = vox(n.G);
@@ -359,7 +361,7 @@ int sdx_nn_dataset_get(int i, float *in5, float *tgt3) {
     NC(hipMemcpy(x.data(), n.ds_x + (size_t)i * V * NN_CIN, x.size() * sizeof(__bf16), hipMemcpyDeviceToHost));
     NC(hipMemcpy(t.data(), n.ds_t + (size_t)i * V * 3, t.size() * sizeof(float), hipMemcpyDeviceToHost));
     for (size_t v = 0; v < V; v++) {
-        for (int c = 0; c < 5; c++) if (in5) in5[c * V + v] = (float)x[v * NN_CIN + c];
+        for (int c = 0; c < NN_IN; c++) if (in) in[c * V + v] = (float)x[v * NN_CIN + c];
         for (int c = 0; c < 3; c++) if (tgt3) tgt3[c * V + v] = t[v * 3 + c];
     }
     return !g_nn_err;

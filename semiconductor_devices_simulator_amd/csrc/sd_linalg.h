@@ -91,6 +91,7 @@ template <class T> void mg_set_pw_one(hipStream_t s, MGLevel<T> &v);
 extern int g_line_nl_max;
 extern int g_line_scan;
 extern int g_line_slots;
+extern int g_line_minwg;
 void sd_copy_bw(hipStream_t s, const double *a, double *b, long n);   // plain copy (bandwidth reference)
 void mg_to_float(hipStream_t s, const double *a, float *b, int n);
 void mg_to_double(hipStream_t s, const float *a, double *b, int n);

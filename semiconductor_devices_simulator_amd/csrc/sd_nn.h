@@ -20,7 +20,8 @@ void nn_relu_bwd(hipStream_t s, int n, const __bf16 *y, __bf16 *g, const __bf16 
 void nn_mse(hipStream_t s, int V, const float *out, int N, const float *tgt, int C3, __bf16 *dy, float *part,
             int nblk);
 void nn_mesh_to_grid(hipStream_t s, int G, int Nx, int Ny, int Nz, const double *psi, const double *phin,
-                     const double *phip, const double *Nd, const double *Na, float dv, int C, __bf16 *in);
+                     const double *phip, const double *Nd, const double *Na, float dv, const double *h0,
+                     const double *h1, const double *h2, int C, __bf16 *in);
 void nn_target_to_grid(hipStream_t s, int G, int Nx, int Ny, int Nz, const double *a0, const double *a1,
                        const double *a2, const double *b0, const double *b1, const double *b2, float iscale,
                        float *tgt);
@@ -29,7 +30,8 @@ void nn_grid_to_mesh_add(hipStream_t s, int G, int Nx, int Ny, int Nz, const flo
 void nn_planar_to_cl(hipStream_t s, int V, int Cr, int C, const float *src, __bf16 *dst);
 
 // ---- network host API (nn.cpp), used by the engine's warm-start hook
-#define NN_CIN 16              // padded input channels (5 used)
+#define NN_IN 8                // input channels used
+#define NN_CIN 16              // padded input channels
 #define NN_COUT 32             // padded output channels (3 used)
 bool nn_ready();               // a network exists
 int nn_grid();                 // its grid size G

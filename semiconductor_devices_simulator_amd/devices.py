@@ -201,33 +201,49 @@ def pn_diode(Nx=16, Ny=16, Nz=64):
                       cons, models=0, tau=(1e-7, 1e-7), sweep=[(0, 0.0, 0.8, 17, {})])
 
 
-def mosfet(Nx=96, Ny=64, Nz=64, vds=(0.05, 1.0), npts=25):
-    """Planar n-MOSFET, x = channel (0.6 um), y = depth (0.3 um, top y = 0),
-    z = width (0.2 um).  p-substrate 1e17 cm^-3; n+ source/drain Gaussian
-    implants (peak 1e20 cm^-3 at the surface, Lv 40 nm, lateral 0.8 Lv)
-    through windows x < 200 nm and x > 400 nm; a 0.2 um gate over 2 nm SiO2
-    (Robin face gate, work function 4.65 eV: about midgap, V_th ~ 0.5 V);
-    ohmic source and drain on the top face, ohmic body on the bottom.
-    Id-Vg 0 -> 1.2 V (npts points) at every drain voltage in vds."""
-    Lx, Ly, Lz = 600.0, 300.0, 200.0
-    xs = graded_grid(Lx, Nx, [200.0, 400.0])
+def mosfet(Nx=96, Ny=64, Nz=64, vds=(0.05, 1.0), npts=25, nsub=1e17, nsd=1e20, sigma=40.0, tox=2.0, lg=200.0):
+    """Planar n-MOSFET, x = channel, y = depth (0.3 um, top y = 0), z = width
+    (0.2 um).  p-substrate nsub cm^-3; n+ source/drain Gaussian implants (peak
+    nsd cm^-3 at the surface, vertical sigma `sigma` nm, lateral 0.8 of it)
+    through 200 nm windows on either side of a gate of length lg nm over tox
+    nm of SiO2 (Robin face gate, work function 4.65 eV: about midgap, V_th ~
+    0.5 V at the defaults); ohmic source and drain on the top face, ohmic body
+    on the bottom.  Id-Vg 0 -> 1.2 V (npts points) at every drain voltage in
+    vds.  The defaults are the 0.6 um device of the benchmark."""
+    Lx, Ly, Lz = 400.0 + lg, 300.0, 200.0
+    xs = graded_grid(Lx, Nx, [200.0, 200.0 + lg])
     ys = graded_grid(Ly, Ny, [0.0, 40.0])
     zs = np.linspace(0, Lz * 1e-9, Nz)
     N = Nx * Ny * Nz
     mat = np.zeros(N, np.int32)
-    Na = np.full(N, 1e23)
+    Na = np.full(N, nsub * 1e6)
     P = node_positions_nm(xs, ys, zs)
     L3 = (Lx, Ly, Lz)
-    nd = Implant("n", 1e20, "y0", 0.0, 200.0, 0.0, 1e9, "gauss", 40.0, 0.0, 0.8).conc(P, L3)
-    nd += Implant("n", 1e20, "y0", 400.0, 1e9, 0.0, 1e9, "gauss", 40.0, 0.0, 0.8).conc(P, L3)
+    nd = Implant("n", nsd, "y0", 0.0, 200.0, 0.0, 1e9, "gauss", sigma, 0.0, 0.8).conc(P, L3)
+    nd += Implant("n", nsd, "y0", 200.0 + lg, 1e9, 0.0, 1e9, "gauss", sigma, 0.0, 0.8).conc(P, L3)
     Nd = nd * 1e6
-    cons = [Contact(2, 0.0, 1.0 / 3.0, 0.0, 1.0, 0, 0.0, label="Source"),
-            Contact(2, 2.0 / 3.0, 1.0, 0.0, 1.0, 0, vds[0], label="Drain"),
-            Contact(2, 1.0 / 3.0, 2.0 / 3.0, 0.0, 1.0, 2, 0.0, pm=4.65, tox=2.0, label="Gate"),
+    a, b = 200.0 / Lx, (200.0 + lg) / Lx
+    cons = [Contact(2, 0.0, a, 0.0, 1.0, 0, 0.0, label="Source"),
+            Contact(2, b, 1.0, 0.0, 1.0, 0, vds[0], label="Drain"),
+            Contact(2, a, b, 0.0, 1.0, 2, 0.0, pm=4.65, tox=tox, label="Gate"),
             Contact(3, 0.0, 1.0, 0.0, 1.0, 0, 0.0, label="Body")]
     sweep = [(2, 0.0, 1.2, npts, {1: vd}) for vd in vds]
     return DeviceSpec("mosfet_%dx%dx%d" % (Nx, Ny, Nz), L3, xs, ys, zs, mat, Nd, Na, cons,
                       models=7, sweep=sweep)
+
+
+def random_mosfet(seed, Nx=96, Ny=64, Nz=64, npts=25):
+    """A MOSFET of the warm-start training family, drawn with
+    numpy.random.default_rng(seed): substrate log-uniform 1e15-1e18 cm^-3,
+    source/drain peak log-uniform 1e19-1e20 cm^-3 with sigma uniform 20-60 nm,
+    oxide uniform 1.5-5 nm, gate length uniform 0.1-0.5 um."""
+    rng = np.random.default_rng(seed)
+    p = dict(nsub=10 ** rng.uniform(15, 18), nsd=10 ** rng.uniform(19, 20), sigma=rng.uniform(20, 60),
+             tox=rng.uniform(1.5, 5.0), lg=rng.uniform(100, 500))
+    spec = mosfet(Nx, Ny, Nz, npts=npts, **p)
+    spec.name = "random_mosfet_%d" % seed
+    spec.params = p
+    return spec
 
 
 WORKLOADS = {

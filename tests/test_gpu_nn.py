@@ -18,7 +18,7 @@ C = (32, 32, 64, 64)
 
 def _sample(G, seed):
     rng = np.random.default_rng(seed)
-    x = rng.standard_normal((G, G, G, 5)) * 0.5
+    x = rng.standard_normal((G, G, G, NO.NIN)) * 0.5
     t = rng.standard_normal((G, G, G, 3)) * 0.1
     return x, t
 

@@ -50,7 +50,7 @@ def test_gradient_finite_differences():
     rng = np.random.default_rng(3)
     c, G = (2, 3, 2, 2), 8
     p = rng.standard_normal(NO.nparams(c)) * 0.3
-    x = rng.standard_normal((G, G, G, 5))
+    x = rng.standard_normal((G, G, G, NO.NIN))
     t = rng.standard_normal((G, G, G, 3))
     loss, g = NO.loss_grad(p, c, x, t)
     idx = rng.choice(len(p), 40, replace=False)
