@@ -36,7 +36,8 @@ double sdx_microbench(int what, int iters);
 void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS line kernel, 1: line-smoothed
                                               levels, 2: tail sweeps, 3: tail size, 4: wavefront-scan line
                                               kernel on (1) or off (0), 5: most line slots per wavefront,
-                                              7: fewest workgroups a wider slot choice may leave */
+                                              7: fewest workgroups a wider slot choice may leave,
+                                              8: LDS-resident coarse tail kernel on (1) or off (0) */
 
 /* ---- learned warm start ----
    A 3-D convolutional U-Net on a G^3 grid resampled from the mesh (index
@@ -47,11 +48,15 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
    converged state.
    sdx_ws_mode: 0 off, 1 record (prediction, converged) training samples at
    each bias point reached in one step, 2 apply the network's correction and
-   keep it only if it lowers the starting Poisson residual.
+   keep it only if it lowers the starting Poisson residual (sdx_ws_accept),
+   3 apply it always.
    stats: [0] corrections tried, [1] kept, [2] rejected, [3] samples recorded. */
 void   sdx_ws_mode(int mode);
 void   sdx_ws_stats(double *out);
 void   sdx_ws_reset_stats(void);
+void   sdx_ws_config(int chmask, double thr);   /* fields corrected: psi 1, phi_n 2, phi_p 4; smallest correction (V) */
+void   sdx_ws_accept(double ratio);              /* mode 2 keeps a correction whose starting Poisson residual is
+                                                    below ratio x the plain prediction's (default 0.5) */
 /* network: G a power of two >= 8, channels multiples of 32 with c3 == c2 */
 int    sdx_nn_init(int G, int c0, int c1, int c2, int c3, unsigned seed);
 void   sdx_nn_free(void);

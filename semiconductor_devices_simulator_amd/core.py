@@ -92,6 +92,8 @@ EXT_SIGNATURES = [
     ("sdx_ws_mode", None, [I]),
     ("sdx_ws_stats", None, [DP]),
     ("sdx_ws_reset_stats", None, []),
+    ("sdx_ws_config", None, [I, D]),
+    ("sdx_ws_accept", None, [D]),
     ("sdx_nn_init", I, [I, I, I, I, I, ctypes.c_uint]),
     ("sdx_nn_free", None, []),
     ("sdx_nn_nparams", I, []),

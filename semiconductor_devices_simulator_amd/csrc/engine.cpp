@@ -621,7 +621,7 @@ template <class T> static void vcycle_T(int L, const T *b, T *x) {
     int nl = (int)H.lev.size();
     MGLevel<T> &v = H.lev[L];
     if (L == nl - 1) { mg_coarse_solve(s, v, H.inv, b, x); return; }
-    if (L == H.tail) { mg_tail(s, H.d_tail, nl - L, H.inv, G.tail_sweeps); return; }   // b, x are lev[L].b, .x
+    if (L == H.tail) { mg_tail(s, H.d_tail, &H.lev[L], nl - L, H.inv, G.tail_sweeps); return; }   // lev[L].b -> .x
     MGLevel<T> &c = H.lev[L + 1];
     int skip = smooth_pre(v, b, x, L);
     mg_resid_restrict(s, v, c, b, x, skip);   // coarse right-hand side
@@ -1291,7 +1291,8 @@ static double gummel_tol() {
 // correction toward the converged state.  Mode 1 records (prediction,
 // converged state) pairs as training samples; mode 2 applies the correction
 // and keeps it only when it lowers the Poisson residual of the starting
-// state (otherwise the plain prediction is restored).
+// state below half of the plain prediction's (otherwise the plain prediction
+// is restored).
 static void ws_save_pred() {
     if (!G.d_ws[0])
         for (int a = 0; a < 3; a++) {
@@ -1312,18 +1313,18 @@ static void ws_before(const double *Va, double dvn) {
     // d_h1: the last converged state (saved at the start of the bias point)
     nn_mesh_to_grid(s, Gg, G.Nx, G.Ny, G.Nz, G.d_psi, G.d_phin, G.d_phip, G.d_Nd, G.d_Na, (float)dvn, G.d_h1[0],
                     G.d_h1[1], G.d_h1[2], NN_CIN, nn_input_buf());
-    if (G.ws_mode != 2) return;
+    if (G.ws_mode < 2) return;
     G.ws_stat[0]++;
     double r0 = poisson_eval(G.d_psi, 0, G.d_Ft);
     const float *y = nn_infer(s);
     nn_grid_to_mesh_add(s, Gg, G.Nx, G.Ny, G.Nz, y, NN_COUT, nn_target_scale(), G.d_kind, G.d_con_mask, G.d_psi,
-                        G.d_phin, G.d_phip);
+                        G.d_phin, G.d_phip, G.ws_chmask, (float)G.ws_thr);
     eng_clamp_qf();
     apply_bc(Va);
     eng_rebuild_np();
     double r1 = poisson_eval(G.d_psi, 0, G.d_Ft);
     if (G.verbose > 1) fprintf(stderr, "semisim-hip: warm start residual %.3e (plain %.3e)\n", r1, r0);
-    if (r1 < r0) { G.ws_stat[1]++; return; }
+    if (r1 < G.ws_accept * r0 || G.ws_mode == 3) { G.ws_stat[1]++; return; }
     G.ws_stat[2]++;
     sd_copy(s, G.d_psi, G.d_ws[0], G.N);
     sd_copy(s, G.d_phin, G.d_ws[1], G.N);
@@ -1604,6 +1605,7 @@ void sdx_tune(int what, int value) {
     if (what == 4) g_line_scan = value;
     if (what == 5) g_line_slots = value >= 4 ? 4 : (value >= 2 ? 2 : 1);
     if (what == 7) g_line_minwg = value > 0 ? value : 1;
+    if (what == 8) g_tail_lds = value != 0;
     if (what == 1) G.line_levels = value;
     if (what == 2) G.tail_sweeps = value > 0 ? value : 1;
     if (what == 3) {
@@ -1613,10 +1615,17 @@ void sdx_tune(int what, int value) {
     }
 }
 // learned warm start: 0 off, 1 record training samples, 2 apply the network
-void sdx_ws_mode(int mode) { G.ws_mode = mode == 1 || mode == 2 ? mode : 0; }
+// (kept only if it lowers the starting Poisson residual), 3 apply it always
+void sdx_ws_mode(int mode) { G.ws_mode = mode >= 1 && mode <= 3 ? mode : 0; }
 // out[0] predictions tried, [1] kept, [2] rejected (plain prediction used), [3] samples recorded
 void sdx_ws_stats(double *out) { for (int i = 0; i < 4; i++) out[i] = (double)G.ws_stat[i]; }
 void sdx_ws_reset_stats(void) { for (int i = 0; i < 4; i++) G.ws_stat[i] = 0; }
+// which of psi (1), phi_n (2), phi_p (4) the correction changes, and the
+// smallest correction applied (V)
+void sdx_ws_config(int chmask, double thr) { G.ws_chmask = chmask & 7; G.ws_thr = thr > 0. ? thr : 0.; }
+// mode 2 keeps a correction when it lowers the starting Poisson residual below
+// ratio x the plain prediction's (default 0.5)
+void sdx_ws_accept(double ratio) { G.ws_accept = ratio > 0. ? ratio : 0.5; }
 void sdx_ktimer_enable(int kind, int every) {
     kt_flush();
     KT.on = 1; KT.kind = kind; KT.every = every > 0 ? every : 8;

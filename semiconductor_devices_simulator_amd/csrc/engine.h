@@ -103,6 +103,9 @@ struct Engine {
     int ws_mode = 0, ws_have = 0;
     double *d_ws[3] = {nullptr, nullptr, nullptr};
     long long ws_stat[4] = {0, 0, 0, 0};
+    int ws_chmask = 7;                 // fields the correction changes (psi 1, phi_n 2, phi_p 4)
+    double ws_thr = 0.;                // smallest correction applied (V)
+    double ws_accept = 0.5;            // residual ratio a kept correction must reach
 
     // Anderson
     int aa_m = 12, aa_n = 0, aa_pos = 0, aa_have = 0;

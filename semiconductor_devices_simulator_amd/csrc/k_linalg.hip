@@ -925,8 +925,136 @@ __global__ void __launch_bounds__(TAIL_BLOCK) k_tail(const MGLevel<T> *lv, int n
         }
     }
 }
-template <class T> void mg_tail(hipStream_t s, const MGLevel<T> *d_levels, int nt, const double *inv, int nsw) {
-    k_tail<T><<<1, TAIL_BLOCK, 0, s>>>(d_levels, nt, inv, nsw);
+// The same coarse end with every level's x and b held in LDS and each
+// thread's row of every level (node t of level L) in registers, loaded once:
+// after the first loads each phase is LDS work between barriers.  The
+// prolongation is applied explicitly before the post-smoothing sweeps (the
+// same values the fused form reads).  Up to TL_MAXT levels, the first of at
+// most TAIL_BLOCK nodes, TL_MAXN nodes in all, the coarsest of at most 64.
+#define TL_MAXT 4
+#define TL_MAXN 2048
+int g_tail_lds = 1;          // 1: LDS-resident tail kernel where the levels fit
+template <class T>
+__global__ void __launch_bounds__(TAIL_BLOCK) k_tail_lds(const MGLevel<T> *lv, int nt, const double *inv, int nsw) {
+    __shared__ T sx[TL_MAXN], sb[TL_MAXN];
+    __shared__ double sr[TAIL_BLOCK];
+    const int t = threadIdx.x;
+    int off[TL_MAXT + 1];
+    off[0] = 0;
+    for (int L = 0; L < nt; L++) off[L + 1] = off[L] + lv[L].N;
+    T a[TL_MAXT][6], pw[TL_MAXT];
+    double rs[TL_MAXT];
+#pragma unroll
+    for (int L = 0; L < TL_MAXT; L++) {
+        pw[L] = 0; rs[L] = 1.0;
+        for (int q = 0; q < 6; q++) a[L][q] = 0;
+        if (L < nt && t < lv[L].N) {
+            const MGLevel<T> &v = lv[L];
+            if (L < nt - 1) {
+                a[L][0] = v.A.xm[t]; a[L][1] = v.A.xp[t]; a[L][2] = v.A.ym[t];
+                a[L][3] = v.A.yp[t]; a[L][4] = v.A.zm[t]; a[L][5] = v.A.zp[t];
+            }
+            pw[L] = v.pw[t];
+            rs[L] = v.rs ? v.rs[t] : 1.0;
+        }
+    }
+    if (t < lv[0].N) sb[t] = lv[0].b[t];
+    __syncthreads();
+    // x of node t of level L from its neighbours in LDS, colour `color`
+    auto sweep = [&](int L, int color, int zero) {
+        const MGLevel<T> &v = lv[L];
+        if (t < v.N) {
+            const int Nx = v.Nx, NxNy = v.Nx * v.Ny;
+            const int l = t / NxNy, r = t - l * NxNy, j = r / Nx, i = r - j * Nx;
+            T *x = sx + off[L];
+            if (((i + j + l) & 1) == color) {
+                T s = sb[off[L] + t];
+                if (!zero) {
+                    if (i > 0) s -= a[L][0] * x[t - 1];
+                    if (i < Nx - 1) s -= a[L][1] * x[t + 1];
+                    if (j > 0) s -= a[L][2] * x[t - Nx];
+                    if (j < v.Ny - 1) s -= a[L][3] * x[t + Nx];
+                    if (l > 0) s -= a[L][4] * x[t - NxNy];
+                    if (l < v.Nz - 1) s -= a[L][5] * x[t + NxNy];
+                }
+                x[t] = s;
+            } else if (zero) {
+                x[t] = 0;
+            }
+        }
+        __syncthreads();
+    };
+    for (int L = 0; L < nt - 1; L++) {
+        const MGLevel<T> &v = lv[L], &c = lv[L + 1];
+        for (int sw = 0; sw < nsw; sw++) {
+            sweep(L, 0, sw == 0);
+            sweep(L, 1, 0);
+        }
+        // weighted residual of the rows the last sweep did not solve (colour 0)
+        if (t < v.N) {
+            const int Nx = v.Nx, NxNy = v.Nx * v.Ny;
+            const int l = t / NxNy, r = t - l * NxNy, j = r / Nx, i = r - j * Nx;
+            double w = 0.0;
+            if (!((i + j + l) & 1)) {
+                const T *x = sx + off[L];
+                T s = sb[off[L] + t] - x[t];
+                if (i > 0) s -= a[L][0] * x[t - 1];
+                if (i < Nx - 1) s -= a[L][1] * x[t + 1];
+                if (j > 0) s -= a[L][2] * x[t - Nx];
+                if (j < v.Ny - 1) s -= a[L][3] * x[t + Nx];
+                if (l > 0) s -= a[L][4] * x[t - NxNy];
+                if (l < v.Nz - 1) s -= a[L][5] * x[t + NxNy];
+                w = rs[L] * (double)s;
+            }
+            sr[t] = w;
+        }
+        __syncthreads();
+        if (t < c.N) {
+            const int cNxNy = c.Nx * c.Ny, LL = t / cNxNy, rr = t - LL * cNxNy, J = rr / c.Nx, I = rr - J * c.Nx;
+            Agg g = agg_range(v.Nx, v.Ny, v.Nz, v.cx, v.cy, v.cz, I, J, LL);
+            double acc = 0.0;
+            for (int l = g.l0; l <= g.l1; l++)
+                for (int j = g.j0; j <= g.j1; j++)
+                    for (int i = g.i0; i <= g.i1; i++) acc += sr[i + v.Nx * (j + v.Ny * l)];
+            sb[off[L + 1] + t] = (T)(acc / rs[L + 1]);
+        }
+        __syncthreads();
+    }
+    {
+        const MGLevel<T> &z = lv[nt - 1];
+        const int n = z.N;
+        if (t < n) {
+            double s = 0.0;
+            for (int q = 0; q < n; q++) s += inv[t * n + q] * (double)sb[off[nt - 1] + q];
+            sx[off[nt - 1] + t] = (T)s;
+        }
+        __syncthreads();
+    }
+    for (int L = nt - 2; L >= 0; L--) {
+        const MGLevel<T> &v = lv[L], &c = lv[L + 1];
+        if (t < v.N) {
+            const int Nx = v.Nx, NxNy = v.Nx * v.Ny;
+            const int l = t / NxNy, r = t - l * NxNy, j = r / Nx, i = r - j * Nx;
+            const int I = v.cx ? i >> 1 : i, J = v.cy ? j >> 1 : j, LL = v.cz ? l >> 1 : l;
+            sx[off[L] + t] += pw[L] * sx[off[L + 1] + I + c.Nx * (J + c.Ny * LL)];
+        }
+        __syncthreads();
+        for (int sw = 0; sw < nsw; sw++) {
+            sweep(L, 1, 0);
+            sweep(L, 0, 0);
+        }
+    }
+    if (t < lv[0].N) lv[0].x[t] = sx[t];
+}
+template <class T> void mg_tail(hipStream_t s, const MGLevel<T> *d_levels, const MGLevel<T> *h_levels, int nt,
+                                const double *inv, int nsw) {
+    int tot = 0;
+    for (int L = 0; L < nt; L++) tot += h_levels[L].N;
+    bool lds = g_tail_lds && nt <= TL_MAXT && nt >= 2 && tot <= TL_MAXN && h_levels[0].N <= TAIL_BLOCK &&
+               h_levels[nt - 1].N <= 64;
+    for (int L = 0; L + 1 < nt && lds; L++) lds = h_levels[L].A.c == nullptr;     // unit-diagonal levels
+    if (lds) k_tail_lds<T><<<1, TAIL_BLOCK, 0, s>>>(d_levels, nt, inv, nsw);
+    else k_tail<T><<<1, TAIL_BLOCK, 0, s>>>(d_levels, nt, inv, nsw);
 }
 
 // x += pw * x_parent (semiconductor_3d.c:2038-2048, MG_ALPHA = 1)
@@ -1042,7 +1170,7 @@ void mg_sten_to_float(hipStream_t s, Sten7<double> A, Sten7<float> B, int n) {
     template void mg_resid_restrict<T>(hipStream_t, const MGLevel<T> &, const MGLevel<T> &, const T *, const T *, int); \
     template int mg_line<T>(hipStream_t, const MGLevel<T> &, const T *, T *, int, int, int, const MGLevel<T> *); \
     template void mg_line_factor<T>(hipStream_t, const MGLevel<T> &, int); \
-    template void mg_tail<T>(hipStream_t, const MGLevel<T> *, int, const double *, int); \
+    template void mg_tail<T>(hipStream_t, const MGLevel<T> *, const MGLevel<T> *, int, const double *, int); \
     template void mg_prolong<T>(hipStream_t, const MGLevel<T> &, const MGLevel<T> &, T *);                       \
     template void mg_coarse_factor<T>(hipStream_t, const MGLevel<T> &, double *);                                \
     template void mg_coarse_solve<T>(hipStream_t, const MGLevel<T> &, const double *, const T *, T *);           \

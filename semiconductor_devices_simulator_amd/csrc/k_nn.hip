@@ -372,7 +372,7 @@ void nn_target_to_grid(hipStream_t s, int G, int Nx, int Ny, int Nz, const doubl
 // phi_n / phi_p on non-contact semiconductor nodes
 __global__ void k_grid_to_mesh_add(int G, int Nx, int Ny, int Nz, const float *out, int N, float scale,
                                    const unsigned char *kind, const int *con_mask, double *psi, double *phin,
-                                   double *phip) {
+                                   double *phip, int chmask, float thr) {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= Nx * Ny * Nz || con_mask[k]) return;
     const int i = k % Nx, j = (k / Nx) % Ny, l = k / (Nx * Ny);
@@ -388,16 +388,21 @@ __global__ void k_grid_to_mesh_add(int G, int Nx, int Ny, int Nz, const float *o
         const size_t v = (size_t)a + (size_t)G * (b + (size_t)G * c);
         f[0] += w * out[v * N + 0]; f[1] += w * out[v * N + 1]; f[2] += w * out[v * N + 2];
     }
-    psi[k] += (double)(f[0] * scale);
+    // chmask: which of psi, phi_n, phi_p receive the correction; corrections
+    // smaller than thr (V) are dropped
+    for (int q = 0; q < 3; q++) f[q] = fabsf(f[q] * scale) >= thr ? f[q] * scale : 0.f;
+    if (chmask & 1) psi[k] += (double)f[0];
     if (kind[k] == K_SEMI) {
-        phin[k] += (double)(f[1] * scale);
-        phip[k] += (double)(f[2] * scale);
+        if (chmask & 2) phin[k] += (double)f[1];
+        if (chmask & 4) phip[k] += (double)f[2];
     }
 }
 void nn_grid_to_mesh_add(hipStream_t s, int G, int Nx, int Ny, int Nz, const float *out, int N, float scale,
-                         const unsigned char *kind, const int *con_mask, double *psi, double *phin, double *phip) {
+                         const unsigned char *kind, const int *con_mask, double *psi, double *phin, double *phip,
+                         int chmask, float thr) {
     const int n = Nx * Ny * Nz;
-    k_grid_to_mesh_add<<<(n + 255) / 256, 256, 0, s>>>(G, Nx, Ny, Nz, out, N, scale, kind, con_mask, psi, phin, phip);
+    k_grid_to_mesh_add<<<(n + 255) / 256, 256, 0, s>>>(G, Nx, Ny, Nz, out, N, scale, kind, con_mask, psi, phin, phip,
+                                                       chmask, thr);
 }
 // host-layout conversions for tests: in5 [C][V] float -> channel-last BF16
 __global__ void k_planar_to_cl(int V, int Cr, int C, const float *src, __bf16 *dst) {

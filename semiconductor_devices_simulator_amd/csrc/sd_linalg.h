@@ -78,7 +78,8 @@ template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, 
 template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c,
                                           const T *b, const T *x, int skip);
 // levels d_levels[0..nt-1] (device copies; the last one the coarsest) in one workgroup
-template <class T> void mg_tail(hipStream_t s, const MGLevel<T> *d_levels, int nt, const double *inv, int nsw);
+template <class T> void mg_tail(hipStream_t s, const MGLevel<T> *d_levels, const MGLevel<T> *h_levels, int nt,
+                                const double *inv, int nsw);
 // LU factors of the lines along axis ax (once per multigrid setup)
 template <class T> void mg_line_factor(hipStream_t s, const MGLevel<T> &v, int ax);
 // zebra line Gauss-Seidel along axis ax on lines of one colour (0 = cannot: line too long)
@@ -92,6 +93,7 @@ extern int g_line_nl_max;
 extern int g_line_scan;
 extern int g_line_slots;
 extern int g_line_minwg;
+extern int g_tail_lds;
 void sd_copy_bw(hipStream_t s, const double *a, double *b, long n);   // plain copy (bandwidth reference)
 void mg_to_float(hipStream_t s, const double *a, float *b, int n);
 void mg_to_double(hipStream_t s, const float *a, double *b, int n);

@@ -26,7 +26,8 @@ void nn_target_to_grid(hipStream_t s, int G, int Nx, int Ny, int Nz, const doubl
                        const double *a2, const double *b0, const double *b1, const double *b2, float iscale,
                        float *tgt);
 void nn_grid_to_mesh_add(hipStream_t s, int G, int Nx, int Ny, int Nz, const float *out, int N, float scale,
-                         const unsigned char *kind, const int *con_mask, double *psi, double *phin, double *phip);
+                         const unsigned char *kind, const int *con_mask, double *psi, double *phin, double *phip,
+                         int chmask, float thr);
 void nn_planar_to_cl(hipStream_t s, int V, int Cr, int C, const float *src, __bf16 *dst);
 
 // ---- network host API (nn.cpp), used by the engine's warm-start hook

@@ -135,6 +135,10 @@ def main(argv=None):
     e.add_argument("--seeds", type=int, nargs="*", default=[10000, 10001])
     e.add_argument("--mesh", type=int, nargs=3, default=[96, 64, 64])
     e.add_argument("--bench-mesh", type=int, nargs=3, default=None)
+    e.add_argument("--modes", type=int, nargs="*", default=[2],
+                   help="2: keep a correction only if it lowers the starting residual, 3: always keep it")
+    e.add_argument("--config", nargs="*", default=["7,0"],
+                   help="FIELDS,THR pairs: fields corrected (psi 1, phi_n 2, phi_p 4), smallest correction in V")
     a = ap.parse_args(argv)
     lib = core.load_core()
     log = lambda m: print(m, flush=True)
@@ -162,14 +166,23 @@ def main(argv=None):
     specs = [devices.random_mosfet(s, *a.mesh) for s in a.seeds]
     if a.bench_mesh:
         specs.append(devices.mosfet(*a.bench_mesh))
+    variants = [tuple(float(v) for v in c.split(",")) for c in a.config]
     for spec in specs:
         r0 = sweep_with(lib, spec, 0)
-        r1 = sweep_with(lib, spec, 2)
-        d = np.abs(r1["I"] - r0["I"])
-        ok = bool(np.all(d <= 1e-8 * np.abs(r0["I"]) + 2 * r0["F"]))
-        log("[ws] " + json.dumps(dict(device=spec.name, plain=r0["its"], learned=r1["its"], ws=r1["ws"],
-                                      seconds=[round(r0["seconds"], 2), round(r1["seconds"], 2)],
-                                      currents_agree=ok)))
+        for mode in a.modes:
+            for chm, thr in variants:
+                lib.sdx_ws_config(int(chm), thr)
+                r1 = sweep_with(lib, spec, mode)
+                d = np.abs(r1["I"] - r0["I"])
+                ok = bool(np.all(d <= 1e-8 * np.abs(r0["I"]) + r0["F"] + r1["F"]))
+                # relative deviation where the current stands well above its floors
+                big = np.abs(r0["I"]) > 100.0 * (r0["F"] + r1["F"])
+                dev = float(np.max(np.where(big, d / np.where(big, np.abs(r0["I"]), 1.0), 0.0)))
+                log("[ws] " + json.dumps(dict(device=spec.name, mode=mode, fields=int(chm), thr=thr, plain=r0["its"],
+                                              learned=r1["its"], ws=r1["ws"],
+                                              seconds=[round(r0["seconds"], 2), round(r1["seconds"], 2)],
+                                              currents_agree=ok, max_rel_current_dev=dev)))
+    lib.sdx_ws_config(7, 0.0)
     return 0
 
 
