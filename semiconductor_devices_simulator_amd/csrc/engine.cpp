@@ -72,7 +72,29 @@ static void vgraph_clear() {
     g_vgraphs.clear();
 }
 
-template <class T> static void hier_free(Hier<T> &H) { H.lev.clear(); H.inv = nullptr; H.alloc = false; }
+template <class T> static void hier_free(Hier<T> &H) {
+    H.lev.clear(); H.inv = nullptr; H.alloc = false;
+    H.A0 = Sten7<T>{}; H.rs0 = nullptr; H.b0 = H.x0 = nullptr;
+}
+// Storage order of each level: the line-smoothed levels above the coarse tail
+// are stored parity-split along x (MGLevel::px) when their lines fit the scan
+// kernel, so that a zebra sweep across x reads and writes its colour's lines
+// contiguously.  The coarse levels (red-black sweeps, tail kernel, coarsest
+// solve) keep the natural order.
+template <class T> static void hier_layout(Hier<T> &H) {
+    const int nl = (int)H.lev.size();
+    bool changed = false;
+    for (int L = 0; L < nl; L++) {
+        MGLevel<T> &v = H.lev[L];
+        int px = G.mg_perm && G.n_axes > 0 && L < G.line_levels && L < nl - 1 && (H.tail < 0 || L < H.tail);
+        for (int a = 0; a < G.n_axes && px; a++) {
+            int len = G.mg_axes[a] == 0 ? v.Nx : (G.mg_axes[a] == 1 ? v.Ny : v.Nz);
+            if (len > 256) px = 0;
+        }
+        if (v.px != px) { v.px = px; changed = true; }
+    }
+    if (changed) vgraph_clear();
+}
 
 void eng_free() {
     if (G.st) hipStreamSynchronize(G.st);
@@ -142,6 +164,16 @@ template <class T> static int hier_alloc(Hier<T> &H, bool level0_own) {
         v.pw = dalloc<T>(n);
         for (int a = 0; a < 3; a++)
             for (int q = 0; q < 3; q++) v.lu[a][q] = dalloc<T>(n);
+    }
+    {
+        const size_t n = H.lev[0].N;
+        if (level0_own) H.A0 = H.lev[0].A;
+        else {
+            H.A0.c = dalloc<T>(n); H.A0.xm = dalloc<T>(n); H.A0.xp = dalloc<T>(n); H.A0.ym = dalloc<T>(n);
+            H.A0.yp = dalloc<T>(n); H.A0.zm = dalloc<T>(n); H.A0.zp = dalloc<T>(n);
+            H.b0 = dalloc<T>(n); H.x0 = dalloc<T>(n);
+        }
+        H.rs0 = dalloc<double>(n);
     }
     int nz = H.lev.back().N;
     H.inv = dalloc<double>((size_t)nz * nz);
@@ -502,6 +534,21 @@ template <class T> static void mg_setup_T() {
     hipStream_t s = G.st;
     DevView d = eng_view();
     int nl = (int)H.lev.size();
+    hier_layout(H);
+    // level 0: the FP64 operator itself (natural FP64 storage) or a copy in
+    // the level's storage order and precision; restriction weights likewise
+    MGLevel<T> &v0 = H.lev[0];
+    const bool own = v0.px || sizeof(T) != sizeof(double);
+    if (own) {
+        mg_sten_in(s, G.A, v0, H.A0);
+        v0.A = H.A0;
+    } else {
+        v0.A = *reinterpret_cast<Sten7<T> *>(&G.A);
+    }
+    if (G.mg_w && v0.px) {
+        mg_rs_in(s, G.mg_w, v0, H.rs0);
+        v0.rs = H.rs0;
+    } else v0.rs = const_cast<double *>(G.mg_w);
     if (G.mg_qsign == 0) {
         if (!H.pw_one) {
             for (int L = 0; L < nl; L++) mg_set_pw_one(s, H.lev[L]);
@@ -512,7 +559,6 @@ template <class T> static void mg_setup_T() {
         for (int L = 0; L < nl - 1; L++) mg_weights_level(s, H.lev[L], H.lev[L + 1], L, d, G.mg_qsign, VT);
         H.pw_one = false;
     }
-    H.lev[0].rs = const_cast<double *>(G.mg_w);
     for (int L = 0; L < nl - 1; L++) mg_galerkin(s, H.lev[L], H.lev[L + 1]);
     for (int L = 0; L < nl - 1; L++)
         for (int a = 0; a < G.n_axes; a++) mg_line_factor(s, H.lev[L], G.mg_axes[a]);
@@ -520,10 +566,8 @@ template <class T> static void mg_setup_T() {
 }
 static void mg_setup() {
     double t0 = wtime();
-    if (G.precision == 1) {
-        mg_sten_to_float(G.st, G.A, G.H32.lev[0].A, G.N);
-        mg_setup_T<float>();
-    } else mg_setup_T<double>();
+    if (G.precision == 1) mg_setup_T<float>();
+    else mg_setup_T<double>();
     G.t_mg += wtime() - t0;
 }
 
@@ -637,10 +681,17 @@ template <class T> static void smooth_only_T(const T *b, T *x) {
 // z = M^-1 r
 static void precond_launch(const double *r, double *z) {
     if (G.precision == 1) {
-        mg_to_float(G.st, r, G.d_b32, G.N);
+        const MGLevel<float> &v0 = G.H32.lev[0];
+        mg_vec_in(G.st, r, v0, G.d_b32);
         if (G.mg_off || G.H32.lev.size() < 2) smooth_only_T<float>(G.d_b32, G.d_x32);
         else vcycle_T<float>(0, G.d_b32, G.d_x32);
-        mg_to_double(G.st, G.d_x32, z, G.N);
+        mg_vec_out(G.st, G.d_x32, v0, z);
+    } else if (G.H64.lev[0].px) {
+        const MGLevel<double> &v0 = G.H64.lev[0];
+        mg_vec_in(G.st, r, v0, G.H64.b0);
+        if (G.mg_off || G.H64.lev.size() < 2) smooth_only_T<double>(G.H64.b0, G.H64.x0);
+        else vcycle_T<double>(0, G.H64.b0, G.H64.x0);
+        mg_vec_out(G.st, G.H64.x0, v0, z);
     } else {
         if (G.mg_off || G.H64.lev.size() < 2) smooth_only_T<double>(r, z);
         else vcycle_T<double>(0, r, z);
@@ -1504,6 +1555,7 @@ void eng_read_env() {
     if ((e = getenv("SEMISIM_MG_TAILN"))) G.tail_n = atoi(e);
     if ((e = getenv("SEMISIM_MG_TAILSW"))) { int q = atoi(e); G.tail_sweeps = q > 0 ? q : 1; }
     G.use_graphs = (e = getenv("SEMISIM_GRAPHS")) ? atoi(e) != 0 : 0;
+    G.mg_perm = (e = getenv("SEMISIM_MG_PERM")) ? atoi(e) != 0 : 1;
 }
 int eng_aa_depth() { return G.aa_m < AA_DEPTH ? G.aa_m : AA_DEPTH; }
 void eng_set_predictor(int on) { PRED_ON = on != 0; }
@@ -1527,11 +1579,17 @@ template <class T> static void level_op_T(Hier<T> &H, int L, double *A7) {
     std::vector<double> rs(n, 1.0);
     if (L > 0) eng_download(rs.data(), v.rs, n);
     T *a[7] = {v.A.c, v.A.xm, v.A.xp, v.A.ym, v.A.yp, v.A.zm, v.A.zp};
+    // storage order -> natural order
+    std::vector<size_t> pos(n);
+    for (size_t k = 0; k < n; k++) {
+        int i = (int)(k % v.Nx), j = (int)((k / v.Nx) % v.Ny), l = (int)(k / ((size_t)v.Nx * v.Ny));
+        pos[k] = (size_t)mg_idx(v, i, j, l);
+    }
     for (int q = 0; q < 7; q++) {
-        if (!a[q]) { for (size_t k = 0; k < n; k++) A7[q * n + k] = rs[k]; continue; }
+        if (!a[q]) { for (size_t k = 0; k < n; k++) A7[q * n + k] = rs[pos[k]]; continue; }
         HC(hipMemcpyAsync(tmp.data(), a[q], n * sizeof(T), hipMemcpyDeviceToHost, G.st));
         eng_sync();
-        for (size_t k = 0; k < n; k++) A7[q * n + k] = (double)tmp[k] * rs[k];
+        for (size_t k = 0; k < n; k++) A7[q * n + k] = (double)tmp[pos[k]] * rs[pos[k]];
     }
 }
 extern "C" {
@@ -1606,6 +1664,7 @@ void sdx_tune(int what, int value) {
     if (what == 5) g_line_slots = value >= 4 ? 4 : (value >= 2 ? 2 : 1);
     if (what == 7) g_line_minwg = value > 0 ? value : 1;
     if (what == 8) g_tail_lds = value != 0;
+    if (what == 9) G.mg_perm = value != 0;
     if (what == 1) G.line_levels = value;
     if (what == 2) G.tail_sweeps = value > 0 ? value : 1;
     if (what == 3) {

@@ -25,6 +25,11 @@ template <class T> struct Hier {
     bool pw_one = false;
     int tail = -1;                 // first level handled by the single-workgroup tail kernel (-1 none)
     MGLevel<T> *d_tail = nullptr;  // device copies of the tail levels' descriptors
+    // level-0 storage of the hierarchy's own (operator, restriction weights,
+    // vectors), used when level 0 is stored parity-split or in FP32
+    Sten7<T> A0{};
+    double *rs0 = nullptr;
+    T *b0 = nullptr, *x0 = nullptr;
 };
 
 struct Engine {
@@ -99,6 +104,7 @@ struct Engine {
     int tail_sweeps = 1;                      // red-black sweeps per side in the tail kernel
     int tail_n = 1024;                        // levels of at most this many nodes run in the tail kernel
     int use_graphs = 0;                       // replay the captured V-cycle (SEMISIM_GRAPHS=1)
+    int mg_perm = 1;                          // line-smoothed levels stored parity-split along x (SEMISIM_MG_PERM)
     // learned warm start: mode, saved prediction of the running bias point
     int ws_mode = 0, ws_have = 0;
     double *d_ws[3] = {nullptr, nullptr, nullptr};

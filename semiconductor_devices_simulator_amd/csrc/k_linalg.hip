@@ -324,16 +324,16 @@ __global__ void __launch_bounds__(SD_BLOCK) k_mg_weights(MGLevel<T> f, MGLevel<T
     for (int l = a.l0; l <= a.l1; l++)
         for (int j = a.j0; j <= a.j1; j++)
             for (int i = a.i0; i <= a.i1; i++) {
-                int k = i + f.Nx * (j + f.Ny * l);
+                const int k = mg_idx(f, i, j, l), kn = i + f.Nx * (j + f.Ny * l);
                 double p;
                 if (L == 0) {
-                    if (d.kind[k] != K_SEMI || d.con_mask[k]) p = PH_NONE;
-                    else p = qsign > 0 ? d.phin[k] / VT : -d.phip[k] / VT;
+                    if (d.kind[kn] != K_SEMI || d.con_mask[kn]) p = PH_NONE;
+                    else p = qsign > 0 ? d.phin[kn] / VT : -d.phip[kn] / VT;
                 } else p = f.ph[k];
                 if (p > mx) mx = p;
                 pv[m] = p; kk[m] = k; m++;
             }
-    c.ph[K] = mx;
+    c.ph[mg_idx(c, I, J, LL)] = mx;
     for (int q = 0; q < m; q++) {
         double e = (pv[q] <= 0.5 * PH_NONE) ? -800. : pv[q] - mx;
         f.pw[kk[q]] = (T)(e < -700. ? 0. : exp(e));
@@ -367,13 +367,13 @@ __global__ void __launch_bounds__(SD_BLOCK) k_galerkin(MGLevel<T> f, MGLevel<T> 
     for (int l = a.l0; l <= a.l1; l++)
         for (int j = a.j0; j <= a.j1; j++)
             for (int i = a.i0; i <= a.i1; i++) {
-                int k = i + f.Nx * (j + f.Ny * l);
+                const int k = mg_idx(f, i, j, l), kx = k - mg_xpos(f, i);
                 double wk = f.rs ? f.rs[k] : 1.0, t;
                 if (wk == 0.0) continue;
                 double ac = f.A.c ? (double)f.A.c[k] : 1.0;
                 cc += wk * ac * (double)f.pw[k];
-                if (i > 0) { t = wk * (double)f.A.xm[k] * (double)f.pw[k - 1]; if (i - 1 >= a.i0) cc += t; else xm += t; }
-                if (i < f.Nx - 1) { t = wk * (double)f.A.xp[k] * (double)f.pw[k + 1]; if (i + 1 <= a.i1) cc += t; else xp += t; }
+                if (i > 0) { t = wk * (double)f.A.xm[k] * (double)f.pw[kx + mg_xpos(f, i - 1)]; if (i - 1 >= a.i0) cc += t; else xm += t; }
+                if (i < f.Nx - 1) { t = wk * (double)f.A.xp[k] * (double)f.pw[kx + mg_xpos(f, i + 1)]; if (i + 1 <= a.i1) cc += t; else xp += t; }
                 if (j > 0) { t = wk * (double)f.A.ym[k] * (double)f.pw[k - f.Nx]; if (j - 1 >= a.j0) cc += t; else ym += t; }
                 if (j < f.Ny - 1) { t = wk * (double)f.A.yp[k] * (double)f.pw[k + f.Nx]; if (j + 1 <= a.j1) cc += t; else yp += t; }
                 if (l > 0) { t = wk * (double)f.A.zm[k] * (double)f.pw[k - NxNy]; if (l - 1 >= a.l0) cc += t; else zm += t; }
@@ -385,10 +385,11 @@ __global__ void __launch_bounds__(SD_BLOCK) k_galerkin(MGLevel<T> f, MGLevel<T> 
     if (cc == 0. && xm == 0. && xp == 0. && ym == 0. && yp == 0. && zm == 0. && zp == 0.) sc = 1.0;
     if (!(sc != 0.0) || !isfinite(sc)) sc = 1.0;
     double is = 1.0 / sc;
-    c.rs[K] = sc;
-    c.A.xm[K] = (T)(xm * is); c.A.xp[K] = (T)(xp * is);
-    c.A.ym[K] = (T)(ym * is); c.A.yp[K] = (T)(yp * is);
-    c.A.zm[K] = (T)(zm * is); c.A.zp[K] = (T)(zp * is);
+    const int Kc = mg_idx(c, I, J, LL);
+    c.rs[Kc] = sc;
+    c.A.xm[Kc] = (T)(xm * is); c.A.xp[Kc] = (T)(xp * is);
+    c.A.ym[Kc] = (T)(ym * is); c.A.yp[Kc] = (T)(yp * is);
+    c.A.zm[Kc] = (T)(zm * is); c.A.zp[Kc] = (T)(zp * is);
 }
 template <class T> void mg_galerkin(hipStream_t s, const MGLevel<T> &f, MGLevel<T> &c) {
     k_galerkin<T><<<sd_nblocks(c.N), SD_BLOCK, 0, s>>>(f, c);
@@ -492,13 +493,12 @@ __global__ void __launch_bounds__(SD_BLOCK) k_line_factor(MGLevel<T> v, int ax) 
     int t = blockIdx.x * SD_BLOCK + threadIdx.x;
     if (t >= g.Ntd * N3) return;
     int td = t % g.Ntd, p3 = t / g.Ntd;
-    int k0 = td * g.s_td + p3 * g.s_3;
     const T *am = ax == 0 ? v.A.xm : (ax == 1 ? v.A.ym : v.A.zm);
     const T *ap = ax == 0 ? v.A.xp : (ax == 1 ? v.A.yp : v.A.zp);
     T *ai = v.lu[ax][0], *iw = v.lu[ax][1], *cp = v.lu[ax][2];
     double cprev = 0.0;
     for (int q = 0; q < g.n; q++) {
-        int k = k0 + q * g.s_ax;
+        const int k = ax == 0 ? mg_idx(v, q, td, p3) : (ax == 1 ? mg_idx(v, td, q, p3) : mg_idx(v, td, p3, q));
         double dg = v.A.c ? (double)v.A.c[k] : 1.0;
         if (!(dg != 0.0)) dg = 1.0;
         double a = q > 0 ? (double)am[k] : 0.0, c = q < g.n - 1 ? (double)ap[k] : 0.0;
@@ -652,13 +652,16 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
         if (ax == 0) { L = e / n; q = e - L * n; } else { q = e / NLW; L = e - q * NLW; }
         const int td = 2 * (h0 + L) + sh;
         const bool ok = e < E && td < g.Ntd;
-        const int k = ok ? td * g.s_td + q * g.s_ax + p3 * g.s_3 : 0;
         int i, j, l;
         if (ax == 0) { i = q; j = td; l = p3; } else if (ax == 1) { i = td; j = q; l = p3; } else { i = td; j = p3; l = q; }
+        const int k = ok ? mg_idx(v, i, j, l) : 0;
         const int u1 = ax == 0 ? j : i, n1 = ax == 0 ? Ny : Nx;
         const int u2 = ax == 2 ? j : l, n2 = ax == 2 ? Ny : Nz;
         const bool hm1 = ok && u1 > 0, hp1 = ok && u1 < n1 - 1, hm2 = ok && u2 > 0, hp2 = ok && u2 < n2 - 1;
-        const int nk[4] = {hm1 ? k - s1 : k, hp1 ? k + s1 : k, hm2 ? k - s2 : k, hp2 ? k + s2 : k};
+        // off-line neighbours: along x through the storage order, else strided
+        const int kx = k - mg_xpos(v, i);
+        const int m1 = ax == 0 ? k - s1 : kx + mg_xpos(v, i - 1), p1 = ax == 0 ? k + s1 : kx + mg_xpos(v, i + 1);
+        const int nk[4] = {hm1 ? m1 : k, hp1 ? p1 : k, hm2 ? k - s2 : k, hp2 ? k + s2 : k};
         msk[m] = (int)ok | ((int)hm1 << 1) | ((int)hp1 << 2) | ((int)hm2 << 3) | ((int)hp2 << 4);
         vb[m] = b[k];
         vai[m] = AI[k]; viw[m] = IW[k]; vcp[m] = CP[k];
@@ -677,7 +680,7 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
                 else { if (ax == 2) jj += on * d; else ll += on * d; }
                 const int I = v.cx ? ii >> 1 : ii, J = v.cy ? jj >> 1 : jj, LL = v.cz ? ll >> 1 : ll;
                 pwn[m][t] = v.pw[nk[t]];
-                xcn[m][t] = c.x[ok ? I + c.Nx * (J + c.Ny * LL) : 0];
+                xcn[m][t] = c.x[ok ? mg_idx(c, I, J, LL) : 0];
             }
         }
     }
@@ -744,7 +747,10 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
         int L, q;
         if (ax == 0) { L = e / n; q = e - L * n; } else { q = e / NLW; L = e - q * NLW; }
         const int td = 2 * (h0 + L) + sh;
-        if (td < g.Ntd) x[td * g.s_td + q * g.s_ax + p3 * g.s_3] = s_r[(L / LW) * SP + (L % LW) * n + q];
+        if (td < g.Ntd) {
+            const int k = ax == 0 ? mg_idx(v, q, td, p3) : (ax == 1 ? mg_idx(v, td, q, p3) : mg_idx(v, td, p3, q));
+            x[k] = s_r[(L / LW) * SP + (L % LW) * n + q];
+        }
     }
 }
 // slots per wavefront allowed by the 64 KB of LDS a workgroup may hold
@@ -838,11 +844,24 @@ __device__ __forceinline__ double agg_resid(const MGLevel<T> &f, const MGLevel<T
         if (par & 1) ok = false;
     }
     if (!ok) return 0.0;
-    const int k = i + f.Nx * (j + f.Ny * l);
+    const int k = mg_idx(f, i, j, l), kx = k - mg_xpos(f, i);
     const double w = f.rs ? f.rs[k] : 1.0;
     if (w == 0.0) return 0.0;
-    T r = b[k] - sten_apply<T, T>(f.A, x, k, i, j, l, f.Nx, f.Ny, f.Nz, f.A.c == nullptr);
+    const int NxNy = f.Nx * f.Ny;
+    T s = f.A.c ? f.A.c[k] * x[k] : x[k];
+    if (i > 0) s += f.A.xm[k] * x[kx + mg_xpos(f, i - 1)];
+    if (i < f.Nx - 1) s += f.A.xp[k] * x[kx + mg_xpos(f, i + 1)];
+    if (j > 0) s += f.A.ym[k] * x[k - f.Nx];
+    if (j < f.Ny - 1) s += f.A.yp[k] * x[k + f.Nx];
+    if (l > 0) s += f.A.zm[k] * x[k - NxNy];
+    if (l < f.Nz - 1) s += f.A.zp[k] * x[k + NxNy];
+    T r = b[k] - s;
     return w * (double)r;
+}
+// storage position of coarse node K (natural numbering)
+template <class T> __device__ __forceinline__ int mg_kpos(const MGLevel<T> &c, int K) {
+    const int cNxNy = c.Nx * c.Ny, LL = K / cNxNy, rr = K - LL * cNxNy, J = rr / c.Nx, I = rr - J * c.Nx;
+    return mg_idx(c, I, J, LL);
 }
 template <class T>
 __device__ __forceinline__ void restrict8(const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x, int skip,
@@ -852,7 +871,7 @@ __device__ __forceinline__ void restrict8(const MGLevel<T> &f, const MGLevel<T> 
     v += __shfl_xor(v, 1, 64);
     v += __shfl_xor(v, 2, 64);
     v += __shfl_xor(v, 4, 64);
-    if (sub == 0 && K < c.N) c.b[K] = (T)(v / c.rs[K]);
+    if (sub == 0 && K < c.N) { const int Kc = mg_kpos(c, K); c.b[Kc] = (T)(v / c.rs[Kc]); }
 }
 // Level-wide restriction: a workgroup of four wavefronts covers 32
 // consecutive coarse nodes; wavefront w takes the fine rows (dj, dl) =
@@ -872,7 +891,8 @@ __global__ void __launch_bounds__(256) k_resid_restrict(MGLevel<T> f, MGLevel<T>
         const int K2 = blockIdx.x * 32 + threadIdx.x;
         if (K2 < c.N) {
             double sum = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
-            c.b[K2] = (T)(sum / c.rs[K2]);
+            const int Kc = mg_kpos(c, K2);
+            c.b[Kc] = (T)(sum / c.rs[Kc]);
         }
     }
 }
@@ -1064,7 +1084,8 @@ __global__ void __launch_bounds__(SD_BLOCK) k_prolong(MGLevel<T> f, MGLevel<T> c
     if (k >= f.N) return;
     int NxNy = f.Nx * f.Ny, l = k / NxNy, r = k - l * NxNy, j = r / f.Nx, i = r - j * f.Nx;
     int I = f.cx ? i >> 1 : i, J = f.cy ? j >> 1 : j, L = f.cz ? l >> 1 : l;
-    x[k] += f.pw[k] * c.x[I + c.Nx * (J + c.Ny * L)];
+    const int kf = mg_idx(f, i, j, l);
+    x[kf] += f.pw[kf] * c.x[mg_idx(c, I, J, L)];
 }
 template <class T> void mg_prolong(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c, T *x) {
     k_prolong<T><<<sd_nblocks(f.N), SD_BLOCK, 0, s>>>(f, c, x);
@@ -1143,24 +1164,45 @@ __global__ void k_copy_bw(const double *a, double *b, long n) {
 void sd_copy_bw(hipStream_t s, const double *a, double *b, long n) {
     k_copy_bw<<<(int)((n + SD_BLOCK - 1) / SD_BLOCK), SD_BLOCK, 0, s>>>(a, b, n);
 }
-__global__ void k_to_float(const double *a, float *b, int n) {
+// natural-order FP64 node data into (and out of) the level's storage order
+// and precision; one thread per x-row pair of positions keeps both the
+// natural reads and the split writes contiguous
+template <class T> __global__ void k_vec_in(const double *r, MGLevel<T> v, T *b) {
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
-    if (k < n) b[k] = (float)a[k];
+    if (k >= v.N) return;
+    const int i = k % v.Nx;
+    b[k - i + mg_xpos(v, i)] = (T)r[k];
 }
-__global__ void k_to_double(const float *a, double *b, int n) {
+template <class T> __global__ void k_vec_out(const T *x, MGLevel<T> v, double *z) {
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
-    if (k < n) b[k] = (double)a[k];
+    if (k >= v.N) return;
+    const int i = k % v.Nx;
+    z[k] = (double)x[k - i + mg_xpos(v, i)];
 }
-__global__ void k_sten_to_float(Sten7<double> A, Sten7<float> B, int n) {
+template <class T> __global__ void k_sten_in(Sten7<double> A, MGLevel<T> v, Sten7<T> B) {
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
-    if (k >= n) return;
-    B.c[k] = (float)A.c[k]; B.xm[k] = (float)A.xm[k]; B.xp[k] = (float)A.xp[k];
-    B.ym[k] = (float)A.ym[k]; B.yp[k] = (float)A.yp[k]; B.zm[k] = (float)A.zm[k]; B.zp[k] = (float)A.zp[k];
+    if (k >= v.N) return;
+    const int i = k % v.Nx, q = k - i + mg_xpos(v, i);
+    B.c[q] = (T)A.c[k]; B.xm[q] = (T)A.xm[k]; B.xp[q] = (T)A.xp[k];
+    B.ym[q] = (T)A.ym[k]; B.yp[q] = (T)A.yp[k]; B.zm[q] = (T)A.zm[k]; B.zp[q] = (T)A.zp[k];
 }
-void mg_to_float(hipStream_t s, const double *a, float *b, int n) { k_to_float<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(a, b, n); }
-void mg_to_double(hipStream_t s, const float *a, double *b, int n) { k_to_double<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(a, b, n); }
-void mg_sten_to_float(hipStream_t s, Sten7<double> A, Sten7<float> B, int n) {
-    k_sten_to_float<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(A, B, n);
+template <class T> __global__ void k_rs_in(const double *w, MGLevel<T> v, double *rs) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= v.N) return;
+    const int i = k % v.Nx;
+    rs[k - i + mg_xpos(v, i)] = w[k];
+}
+template <class T> void mg_vec_in(hipStream_t s, const double *r, const MGLevel<T> &v, T *b) {
+    k_vec_in<T><<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(r, v, b);
+}
+template <class T> void mg_vec_out(hipStream_t s, const T *x, const MGLevel<T> &v, double *z) {
+    k_vec_out<T><<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(x, v, z);
+}
+template <class T> void mg_sten_in(hipStream_t s, Sten7<double> A, const MGLevel<T> &v, Sten7<T> B) {
+    k_sten_in<T><<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(A, v, B);
+}
+template <class T> void mg_rs_in(hipStream_t s, const double *w, const MGLevel<T> &v, double *rs) {
+    k_rs_in<T><<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(w, v, rs);
 }
 
 #define INST(T)                                                                                                   \
@@ -1173,6 +1215,10 @@ void mg_sten_to_float(hipStream_t s, Sten7<double> A, Sten7<float> B, int n) {
     template void mg_tail<T>(hipStream_t, const MGLevel<T> *, const MGLevel<T> *, int, const double *, int); \
     template void mg_prolong<T>(hipStream_t, const MGLevel<T> &, const MGLevel<T> &, T *);                       \
     template void mg_coarse_factor<T>(hipStream_t, const MGLevel<T> &, double *);                                \
+    template void mg_vec_in<T>(hipStream_t, const double *, const MGLevel<T> &, T *);                             \
+    template void mg_vec_out<T>(hipStream_t, const T *, const MGLevel<T> &, double *);                            \
+    template void mg_sten_in<T>(hipStream_t, Sten7<double>, const MGLevel<T> &, Sten7<T>);                       \
+    template void mg_rs_in<T>(hipStream_t, const double *, const MGLevel<T> &, double *);                        \
     template void mg_coarse_solve<T>(hipStream_t, const MGLevel<T> &, const double *, const T *, T *);           \
     template void mg_set_pw_one<T>(hipStream_t, MGLevel<T> &);
 INST(double)

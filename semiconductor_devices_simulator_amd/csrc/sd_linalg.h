@@ -28,6 +28,11 @@ enum {
 template <class T> struct MGLevel {
     int Nx, Ny, Nz, N;
     int cx, cy, cz;
+    // storage of the level's node arrays: px = 0 natural order k = i + Nx (j +
+    // Ny l); px = 1 each x-row split by parity (even i first: position
+    // (i & 1) * ((Nx + 1) / 2) + i / 2), so that the lines of one zebra colour
+    // across x are contiguous
+    int px;
     Sten7<T> A;          // level 0: full stencil; coarse: c == nullptr (unit diagonal)
     double *rs;          // row scale used by restriction into the next level (nullptr = 1)
     T *pw;               // prolongation weight of each node to its parent
@@ -35,6 +40,14 @@ template <class T> struct MGLevel {
     T *x, *b;            // level vectors
     T *lu[3][3];         // line LU factors per axis: a/w, 1/w, c/w (nullptr: axis not factored)
 };
+
+// storage position of node (i, j, l) of a level
+template <class T> __host__ __device__ inline int mg_xpos(const MGLevel<T> &v, int i) {
+    return v.px ? (i & 1) * ((v.Nx + 1) >> 1) + (i >> 1) : i;
+}
+template <class T> __host__ __device__ inline int mg_idx(const MGLevel<T> &v, int i, int j, int l) {
+    return mg_xpos(v, i) + v.Nx * (j + v.Ny * l);
+}
 
 // vector / reduction launchers (stream-ordered, no host sync)
 void sd_dot(hipStream_t s, const double *a, const double *b, int n, RedBuf rb, int slot, int post = -1);
@@ -95,7 +108,10 @@ extern int g_line_slots;
 extern int g_line_minwg;
 extern int g_tail_lds;
 void sd_copy_bw(hipStream_t s, const double *a, double *b, long n);   // plain copy (bandwidth reference)
-void mg_to_float(hipStream_t s, const double *a, float *b, int n);
-void mg_to_double(hipStream_t s, const float *a, double *b, int n);
-void mg_sten_to_float(hipStream_t s, Sten7<double> A, Sten7<float> B, int n);
+// level-0 data in the level's storage order and precision: the right-hand
+// side in, the correction out, the operator and the restriction weights
+template <class T> void mg_vec_in(hipStream_t s, const double *r, const MGLevel<T> &v, T *b);
+template <class T> void mg_vec_out(hipStream_t s, const T *x, const MGLevel<T> &v, double *z);
+template <class T> void mg_sten_in(hipStream_t s, Sten7<double> A, const MGLevel<T> &v, Sten7<T> B);
+template <class T> void mg_rs_in(hipStream_t s, const double *w, const MGLevel<T> &v, double *rs);
 template <class T> void sd_spmv_T(hipStream_t s, const Geo &g, Sten7<T> A, const T *x, const T *b, T *y);
