@@ -37,7 +37,9 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
                                               levels, 2: tail sweeps, 3: tail size, 4: wavefront-scan line
                                               kernel on (1) or off (0), 5: most line slots per wavefront,
                                               7: fewest workgroups a wider slot choice may leave,
-                                              8: LDS-resident coarse tail kernel on (1) or off (0) */
+                                              8: LDS-resident coarse tail kernel on (1) or off (0),
+                                              9: parity-split storage of the line levels, 10: XCD-contiguous
+                                              line tiles */
 
 /* ---- learned warm start ----
    A 3-D convolutional U-Net on a G^3 grid resampled from the mesh (index

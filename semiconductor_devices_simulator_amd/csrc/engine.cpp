@@ -1665,6 +1665,7 @@ void sdx_tune(int what, int value) {
     if (what == 7) g_line_minwg = value > 0 ? value : 1;
     if (what == 8) g_tail_lds = value != 0;
     if (what == 9) G.mg_perm = value != 0;
+    if (what == 10) g_line_xcd = value != 0;
     if (what == 1) G.line_levels = value;
     if (what == 2) G.tail_sweeps = value > 0 ? value : 1;
     if (what == 3) {

@@ -612,7 +612,8 @@ __global__ void __launch_bounds__(SD_BLOCK) k_line_gs(MGLevel<T> v, const T *__r
 #define LS_THREADS (64 * LS_WAVES)
 template <class T, int P, int S>
 __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T *__restrict__ b, T *__restrict__ x,
-                                                          int ax, int color, int zero, MGLevel<T> c, int pro, int LW) {
+                                                          int ax, int color, int zero, MGLevel<T> c, int pro, int LW,
+                                                          int xcd) {
     constexpr int SL = 64 * P;                       // elements per slot
     constexpr int SP = SL + 1;                       // slot pitch in LDS: odd, so that the gathers
                                                      // across lines (consecutive lanes, one slot
@@ -622,9 +623,13 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
     const LineGeom g = line_geom(Nx, Ny, Nz, ax);
     const int n = g.n;
     const int NLW = LS_WAVES * S * LW;               // lines per workgroup
-    // the colour's lines of plane p3 are every other td: td = 2 h + sh
+    // the colour's lines of plane p3 are every other td: td = 2 h + sh.
+    // Workgroup b runs on XCD b % 8; with xcd set, each XCD takes a
+    // contiguous range of tiles, so that neighbouring tiles (which share
+    // cache lines) and neighbouring planes meet in the same L2.
     const int ntile = (((g.Ntd + 1) >> 1) + NLW - 1) / NLW;
-    const int p3 = blockIdx.x / ntile, h0 = (blockIdx.x - p3 * ntile) * NLW, sh = (p3 + color) & 1;
+    const int bid = xcd ? (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+    const int p3 = bid / ntile, h0 = (bid - p3 * ntile) * NLW, sh = (p3 + color) & 1;
     const int E = NLW * n, used = LW * n, pad = SL - used;
     const T *__restrict__ AI = v.lu[ax][0];
     const T *__restrict__ IW = v.lu[ax][1];
@@ -760,20 +765,21 @@ template <class T> constexpr int ls_smax(int P) {
 }
 template <class T, int P>
 static void ls_launch(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero,
-                      const MGLevel<T> &cc, int pro, int LW) {
+                      const MGLevel<T> &cc, int pro, int LW, int xcd) {
     constexpr int SM = ls_smax<T>(P);
     if (S >= 4 && SM >= 4)
-        k_line_scan<T, P, (SM >= 4 ? 4 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW);
+        k_line_scan<T, P, (SM >= 4 ? 4 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW, xcd);
     else if (S >= 2 && SM >= 2)
-        k_line_scan<T, P, (SM >= 2 ? 2 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW);
+        k_line_scan<T, P, (SM >= 2 ? 2 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW, xcd);
     else
-        k_line_scan<T, P, 1><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW);
+        k_line_scan<T, P, 1><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW, xcd);
 }
 
 int g_line_nl_max = 8;       // lines per workgroup cap of k_line_gs (tuning)
 int g_line_scan = 1;         // 1: scan kernel for lines of up to 256 elements
 int g_line_slots = 4;        // most line slots per wavefront of the scan kernel (1, 2, 4)
 int g_line_minwg = 512;      // fewest workgroups a wider slot choice may leave
+int g_line_xcd = 1;          // XCD-contiguous tile order
 template <class T>
 int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero, const MGLevel<T> *c) {
     int n = ax == 0 ? v.Nx : (ax == 1 ? v.Ny : v.Nz);
@@ -798,13 +804,14 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
         while (Srun > SM) Srun >>= 1;
         const int NLWr = LS_WAVES * Srun * LW, nb = (lpp + NLWr - 1) / NLWr * npl;
         if (nb == 0) return 1;
+        const int xcd = g_line_xcd && (nb & 7) == 0;      // the remap is a bijection for nb % 8 == 0
         MGLevel<T> cc = c ? *c : v;
         const int pro = c != nullptr;
         switch (P) {
-        case 1: ls_launch<T, 1>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
-        case 2: ls_launch<T, 2>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
-        case 3: ls_launch<T, 3>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
-        default: ls_launch<T, 4>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW); break;
+        case 1: ls_launch<T, 1>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW, xcd); break;
+        case 2: ls_launch<T, 2>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW, xcd); break;
+        case 3: ls_launch<T, 3>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW, xcd); break;
+        default: ls_launch<T, 4>(s, Srun, nb, v, b, x, ax, color, zero, cc, pro, LW, xcd); break;
         }
         return 1;
     }

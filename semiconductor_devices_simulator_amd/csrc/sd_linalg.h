@@ -106,6 +106,7 @@ extern int g_line_nl_max;
 extern int g_line_scan;
 extern int g_line_slots;
 extern int g_line_minwg;
+extern int g_line_xcd;
 extern int g_tail_lds;
 void sd_copy_bw(hipStream_t s, const double *a, double *b, long n);   // plain copy (bandwidth reference)
 // level-0 data in the level's storage order and precision: the right-hand

@@ -47,11 +47,11 @@ def main(argv=None):
                     help="WHAT=VALUE passed to sdx_tune before the solve (4=0: the k_line_gs line kernel)")
     a = ap.parse_args(argv)
     lib = core.load_core()
-    for t in a.tune:
-        w, val = (int(z) for z in t.split("="))
-        lib.sdx_tune(w, val)
     spec = devices.mosfet(*a.mesh)
     cids = devices.load_device(lib, spec)
+    for t in a.tune:              # after api3_init, which reads the environment knobs
+        w, val = (int(z) for z in t.split("="))
+        lib.sdx_tune(w, val)
     lib.sdx_set_precision(a.precision)
     if not lib.api3_solve():
         raise RuntimeError("equilibrium solve failed")
