@@ -102,7 +102,7 @@ __device__ __forceinline__ V sten_apply(const Sten7<T> &A, const V *x, int k, in
 template <class T>
 __global__ void __launch_bounds__(SD_BLOCK) k_spmv(int Nx, int Ny, int Nz, Sten7<T> A, const T *x, const T *b, T *y) {
     int N = Nx * Ny * Nz;
-    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    int k = sd_tile() * SD_BLOCK + threadIdx.x;
     if (k >= N) return;
     int NxNy = Nx * Ny, l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
     T s = sten_apply<T, T>(A, x, k, i, j, l, Nx, Ny, Nz, false);
@@ -124,7 +124,9 @@ __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, 
                                                         RedBuf rb) {
     int N = Nx * Ny * Nz, NxNy = Nx * Ny;
     double acc[2] = {0.0, 0.0};
-    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < N; k += gridDim.x * SD_BLOCK) {
+    int k0, k1;
+    sd_tile_range(N, k0, k1);
+    for (int k = k0 + threadIdx.x; k < k1; k += SD_BLOCK) {
         int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
         double s = sten_apply<double, double>(A, x, k, i, j, l, Nx, Ny, Nz, false);
         y[k] = s;
@@ -145,7 +147,9 @@ __global__ void __launch_bounds__(SD_BLOCK) k_floor(int Nx, int Ny, int Nz, Sten
                                                     const double *b, RedBuf rb, int slot) {
     int N = Nx * Ny * Nz, NxNy = Nx * Ny;
     double acc[1] = {0.0};
-    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < N; k += gridDim.x * SD_BLOCK) {
+    int k0, k1;
+    sd_tile_range(N, k0, k1);
+    for (int k = k0 + threadIdx.x; k < k1; k += SD_BLOCK) {
         int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
         double s = fabs(A.c[k] * x[k]) + fabs(b[k]);
         if (i > 0) s += fabs(A.xm[k] * x[k - 1]);
@@ -445,7 +449,7 @@ __device__ __forceinline__ void rb_node(const MGLevel<T> &v, const T *b, T *x, i
 template <class T>
 __global__ void __launch_bounds__(SD_BLOCK) k_rbgs(MGLevel<T> v, const T *b, T *x, int color, int zero,
                                                  MGLevel<T> c, int pro) {
-    int t = blockIdx.x * SD_BLOCK + threadIdx.x;
+    int t = sd_tile() * SD_BLOCK + threadIdx.x;
     if (zero) {
         if (t < v.N) rb_zero_node(v, b, x, color, t);
         return;
@@ -888,14 +892,14 @@ __device__ __forceinline__ void restrict8(const MGLevel<T> &f, const MGLevel<T> 
 template <class T>
 __global__ void __launch_bounds__(256) k_resid_restrict(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x, int skip) {
     __shared__ double part[4][32];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int K = blockIdx.x * 32 + (lane >> 1);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, tb = sd_tile();
+    const int K = tb * 32 + (lane >> 1);
     double v = K < c.N ? agg_resid(f, c, b, x, skip, K, lane & 1, w & 1, w >> 1) : 0.0;
     v += __shfl_xor(v, 1, 64);
     if (!(lane & 1)) part[w][lane >> 1] = v;
     __syncthreads();
     if (threadIdx.x < 32) {
-        const int K2 = blockIdx.x * 32 + threadIdx.x;
+        const int K2 = tb * 32 + threadIdx.x;
         if (K2 < c.N) {
             double sum = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
             const int Kc = mg_kpos(c, K2);

@@ -23,7 +23,9 @@ __global__ void __launch_bounds__(SD_BLOCK) k_poisson(DevView d, const double *p
                                                       Sten7<double> A, double *sc, RedBuf rb, int slot) {
     const Geo g = d.g;
     double acc[1] = {0.0};
-    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < g.N; k += gridDim.x * SD_BLOCK) {
+    int k0, k1;
+    sd_tile_range(g.N, k0, k1);
+    for (int k = k0 + threadIdx.x; k < k1; k += SD_BLOCK) {
         if (d.con_mask[k]) {
             F[k] = 0.0;
             if (build) {
@@ -101,7 +103,7 @@ void ph_rebuild_np(hipStream_t s, const DevView &d) { k_rebuild_np<<<sd_nblocks(
 // ------------------------------------------------------- face mobilities
 __global__ void k_upd_mob(DevView d, const double *qn, const double *qp) {
     const Geo g = d.g;
-    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    int k = sd_tile() * SD_BLOCK + threadIdx.x;
     if (k >= g.N) return;
     int i, j, l;
     sd_unidx(g, k, i, j, l);
@@ -139,7 +141,7 @@ void ph_upd_mob(hipStream_t s, const DevView &d, const double *qn, const double 
 // form for the multigrid restriction.
 __global__ void k_cont(DevView d, int hole, Sten7<double> A, double *rhs, double *sc, double *dsc) {
     const Geo g = d.g;
-    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    int k = sd_tile() * SD_BLOCK + threadIdx.x;
     if (k >= g.N) return;
     const double *car = hole ? d.p : d.n;
     if (d.kind[k] != K_SEMI || d.con_mask[k]) {

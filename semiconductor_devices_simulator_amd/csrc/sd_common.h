@@ -199,3 +199,19 @@ __device__ __forceinline__ T sd_block_max(T v, T *lds) {
 }
 
 inline int sd_nblocks(long n, int bs = SD_BLOCK) { return (int)((n + bs - 1) / bs); }
+
+// Workgroup b runs on XCD b % 8, and each of the eight XCDs has its own L2.
+// sd_tile() renumbers the workgroups so that each XCD takes a contiguous
+// range of tiles: stencil neighbours read across tile boundaries (the next
+// row, the next plane) then mostly come from the same L2.  Identity when the
+// grid is not a multiple of 8.
+__device__ __forceinline__ int sd_tile() {
+    return (gridDim.x & 7) ? (int)blockIdx.x : (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
+}
+// contiguous node range [k0, k1) of this workgroup's tile when a grid of
+// gridDim.x workgroups covers n nodes (for loops over k0 + tid, step SD_BLOCK)
+__device__ __forceinline__ void sd_tile_range(int n, int &k0, int &k1) {
+    const int per = ((n + (int)gridDim.x - 1) / (int)gridDim.x + SD_BLOCK - 1) / SD_BLOCK * SD_BLOCK;
+    k0 = sd_tile() * per;
+    k1 = k0 + per < n ? k0 + per : n;
+}
