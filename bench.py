@@ -202,6 +202,20 @@ def cpu_baseline_worker(path):
     print(json.dumps(dict(points_per_s=n / dt, seconds=dt, points=n)))
 
 
+def pmc_traffic(mesh):
+    """HBM bytes per launch of the level-0 sweep from the committed counter
+    summary of this mesh (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE,
+    calibrated against a copy kernel), or (None, None) when there is none."""
+    name = "k_line_scan_level0_mosfet_%dx%dx%d_pmc.md" % tuple(mesh)
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.isfile(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("Level-0 sweep") and "MB per launch" in line:
+            mb = float(line.rsplit("=", 1)[1].split("MB")[0])
+            return round(mb * 1e6), "profiles/" + name
+    return None, None
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,11 +262,12 @@ def main():
         lib.sdx_nn_free()
         print(f"[bench] learned warm start + FP32 multigrid: {K / tL:.3f} points/s", file=sys.stderr, flush=True)
     roof = None
+    traffic, traffic_src = pmc_traffic(a.mesh)
     if kt[0] > 0:
         avg_s = kt[1] / kt[0] * 1e-3
         ach = kt[2] / avg_s / 1e9
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_src,
                     kernel="k_line_scan<float> level 0 (zebra line Gauss-Seidel sweep, one colour)",
                     bytes_per_launch=kt[2], avg_launch_us=round(avg_s * 1e6, 2), launches_timed=int(kt[0]))
     cpu_states = {i: st for i, st in states.items() if i != last or i in windows}

@@ -39,7 +39,12 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
                                               7: fewest workgroups a wider slot choice may leave,
                                               8: LDS-resident coarse tail kernel on (1) or off (0),
                                               9: parity-split storage of the line levels, 10: XCD-contiguous
-                                              line tiles */
+                                              line tiles, 11: tile order of the other stencil kernels (0
+                                              contiguous node range per workgroup, 1 XCD-contiguous tiles
+                                              with grid-stride node loops, 2 launch order; -1 default),
+                                              12: restriction with lanes only for rows not solved exactly
+                                              (1) or for every fine node (0), 13: BiCGSTAB computes the
+                                              next rh.r with its x/r update (1) or separately (0) */
 
 /* ---- learned warm start ----
    A 3-D convolutional U-Net on a G^3 grid resampled from the mesh (index

@@ -25,24 +25,31 @@ SOURCES = ["k_linalg.hip", "k_physics.hip", "k_nn.hip", "engine.cpp", "nn.cpp", 
 def build(verbose=False, jobs=4):
     os.makedirs(LIB_DIR, exist_ok=True)
     objs, procs = [], []
-    for src in SOURCES:
-        obj = os.path.join(LIB_DIR, os.path.splitext(src)[0] + ".o")
-        lang = [] if src.endswith(".hip") else ["-x", "hip"]
-        cmd = [HIPCC] + FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
-        objs.append(obj)
-        if len(procs) >= jobs:
+    try:
+        for src in SOURCES:
+            obj = os.path.join(LIB_DIR, os.path.splitext(src)[0] + ".o")
+            lang = [] if src.endswith(".hip") else ["-x", "hip"]
+            cmd = [HIPCC] + FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+            objs.append(obj)
+            if len(procs) >= jobs:
+                _wait(procs.pop(0))
+        while procs:
             _wait(procs.pop(0))
-    for p in procs:
-        _wait(p)
-    tmp = OUT + ".tmp"
-    link = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
-    _run(link, "link")
-    os.replace(tmp, OUT)
-    for o in objs:
-        os.remove(o)
+        tmp = OUT + ".tmp"
+        link = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+        _run(link, "link")
+        os.replace(tmp, OUT)
+    finally:
+        # a failed compile must not leave the other compilers running or
+        # their partial outputs behind
+        for _, p in procs:
+            p.communicate()
+        for f in os.listdir(LIB_DIR):
+            if f.endswith(".o") or ".o.tmp" in f:
+                os.remove(os.path.join(LIB_DIR, f))
     return OUT
 
 

@@ -571,10 +571,6 @@ static void mg_setup() {
     G.t_mg += wtime() - t0;
 }
 
-// Smoother: zebra line Gauss-Seidel along the mesh axes with the strongest
-// couplings (G.mg_axes, chosen in prepare), or red-black point Gauss-Seidel
-// (G.n_axes == 0).  Pre-smoothing starts from x = 0; returns the restriction
-// skip code of the rows the last sweep solved exactly.
 // ------------------------------------------------ sampled kernel timer
 // HIP events on the solver stream around every 8th level-0 launch of one
 // kernel class (0: smoother sweep, 1: Krylov SpMV), for live roofline
@@ -622,6 +618,10 @@ static void kt_end(bool started) {
     if (KT.pend.size() >= 512) kt_flush();
 }
 
+// Smoother: zebra line Gauss-Seidel along the mesh axes with the strongest
+// couplings (G.mg_axes, chosen in prepare), or red-black point Gauss-Seidel
+// (G.n_axes == 0).  Pre-smoothing starts from x = 0; returns the restriction
+// skip code of the rows the last sweep solved exactly.
 template <class T> static int smooth_pre(const MGLevel<T> &v, const T *b, T *x, int L = 0) {
     hipStream_t s = G.st;
     if (G.n_axes == 0 || L >= G.line_levels) {
@@ -802,7 +802,9 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
     double rbest = r0;
     int stall = 0, it, restarted = 0;
     for (it = 1; it <= maxit; it++) {
-        sd_dot(s, rh, r, N, G.rb, S_RHO1, SOP_BI_BE);
+        // rho1 = rh.r and beta: here on the first iteration, later by the
+        // previous iteration's x/r update
+        if (it == 1 || !G.bi_fuse) sd_dot(s, rh, r, N, G.rb, S_RHO1, SOP_BI_BE);
     again:
         sd_bi_p(s, p, r, v, N, G.rb);
         precond(p, ph);
@@ -835,7 +837,7 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
         { bool kt = kt_begin(1);
           sd_spmv_dots(s, g, G.A, sh, t, nullptr, r, 2, G.rb, S_TT, SOP_BI_OM);
           kt_end(kt); }
-        sd_bi_xr2(s, x, r, ph, sh, t, N, G.rb);
+        sd_bi_xr2(s, x, r, ph, sh, t, G.bi_fuse ? rh : nullptr, N, G.rb);
         rd(S_OM, S_RR + 1);
         double om = S(S_OM);
         rr = S(S_RR);
@@ -1666,6 +1668,14 @@ void sdx_tune(int what, int value) {
     if (what == 8) g_tail_lds = value != 0;
     if (what == 9) G.mg_perm = value != 0;
     if (what == 10) g_line_xcd = value != 0;
+    if (what == 12) g_restrict_act = value != 0;
+    if (what == 13) G.bi_fuse = value != 0;
+    if (what == 11) {
+        const int m = value >= 0 && value <= 2 ? value : SD_TILE_MODE_DEFAULT;
+        hipDeviceSynchronize();
+        sd_set_tile_mode_linalg(m);
+        ph_set_tile_mode(m);
+    }
     if (what == 1) G.line_levels = value;
     if (what == 2) G.tail_sweeps = value > 0 ? value : 1;
     if (what == 3) {

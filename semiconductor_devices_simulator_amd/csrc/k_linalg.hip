@@ -124,9 +124,9 @@ __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, 
                                                         RedBuf rb) {
     int N = Nx * Ny * Nz, NxNy = Nx * Ny;
     double acc[2] = {0.0, 0.0};
-    int k0, k1;
-    sd_tile_range(N, k0, k1);
-    for (int k = k0 + threadIdx.x; k < k1; k += SD_BLOCK) {
+    int k0, st, k1;
+    sd_node_loop(N, k0, st, k1);
+    for (int k = k0; k < k1; k += st) {
         int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
         double s = sten_apply<double, double>(A, x, k, i, j, l, Nx, Ny, Nz, false);
         y[k] = s;
@@ -142,14 +142,19 @@ void sd_spmv_dots(hipStream_t s, const Geo &g, Sten7<double> A, const double *x,
     red_final(s, rb, nv, nb, slot, 0, post);
 }
 
+void sd_set_tile_mode_linalg(int m) {
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_tile_mode), &m, sizeof(int)) != hipSuccess)
+        fprintf(stderr, "semisim-hip: setting the tile order failed\n");
+}
+
 // rounding floor of a continuity solve, semiconductor_3d.c:2243-2265
 __global__ void __launch_bounds__(SD_BLOCK) k_floor(int Nx, int Ny, int Nz, Sten7<double> A, const double *x,
                                                     const double *b, RedBuf rb, int slot) {
     int N = Nx * Ny * Nz, NxNy = Nx * Ny;
     double acc[1] = {0.0};
-    int k0, k1;
-    sd_tile_range(N, k0, k1);
-    for (int k = k0 + threadIdx.x; k < k1; k += SD_BLOCK) {
+    int k0, st, k1;
+    sd_node_loop(N, k0, st, k1);
+    for (int k = k0; k < k1; k += st) {
         int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
         double s = fabs(A.c[k] * x[k]) + fabs(b[k]);
         if (i > 0) s += fabs(A.xm[k] * x[k - 1]);
@@ -177,6 +182,12 @@ __device__ __forceinline__ void scalar_op(int op, double *sc) {
     case SOP_BI_AL: sc[S_AL] = (fabs(sc[S_RV]) < 1e-300) ? 0.0 : sc[S_RHO1] / sc[S_RV]; break;
     case SOP_BI_OM: sc[S_OM] = sc[S_TT] > 0. ? sc[S_TS] / sc[S_TT] : 0.; break;
     case SOP_BI_NEXT: sc[S_RHO] = sc[S_RHO1]; break;
+    case SOP_BI_NEXT_BE:       // r.r and rh.r of the new r, then the next beta
+        sc[S_RR] = sc[S_TMP2];
+        sc[S_RHO] = sc[S_RHO1];
+        sc[S_RHO1] = sc[S_TMP3];
+        sc[S_BE] = (sc[S_RHO1] / sc[S_RHO]) * (sc[S_AL] / sc[S_OM]);
+        break;
     case SOP_BI_RESTART: sc[S_RHO] = sc[S_AL] = sc[S_OM] = 1.0; break;
     }
 }
@@ -249,17 +260,20 @@ __global__ void k_bi_x1(double *x, const double *ph, int n, const double *sc) {
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
     if (k < n) x[k] += sc[S_AL] * ph[k];
 }
+// x += al ph + om sh, r -= om t; r.r and (with rh) the next iteration's rh.r
+template <int NV>
 __global__ void __launch_bounds__(SD_BLOCK) k_bi_xr2(double *x, double *r, const double *ph, const double *sh,
-                                                     const double *t, int n, RedBuf rb) {
+                                                     const double *t, const double *rh, int n, RedBuf rb) {
     double al = rb.scal[S_AL], om = rb.scal[S_OM];
-    double acc[1] = {0.0};
+    double acc[NV] = {};
     for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < n; k += gridDim.x * SD_BLOCK) {
         x[k] += al * ph[k] + om * sh[k];
         double y = r[k] - om * t[k];
         r[k] = y;
         acc[0] += y * y;
+        if constexpr (NV == 2) acc[1] += rh[k] * y;
     }
-    red_partial<1, false>(acc, rb);
+    red_partial<NV, false>(acc, rb);
 }
 __global__ void __launch_bounds__(SD_BLOCK) k_bi_init(double *r, double *rh, double *p, double *v, const double *b,
                                                       const double *t, int n, RedBuf rb) {
@@ -286,11 +300,16 @@ void sd_bi_r1(hipStream_t s, double *r, const double *v, int n, RedBuf rb) {
 void sd_bi_x1(hipStream_t s, double *x, const double *ph, int n, RedBuf rb) {
     k_bi_x1<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(x, ph, n, rb.scal);
 }
-void sd_bi_xr2(hipStream_t s, double *x, double *r, const double *ph, const double *sh, const double *t, int n,
-               RedBuf rb) {
+void sd_bi_xr2(hipStream_t s, double *x, double *r, const double *ph, const double *sh, const double *t,
+               const double *rh, int n, RedBuf rb) {
     int nb = red_grid(n);
-    k_bi_xr2<<<nb, SD_BLOCK, 0, s>>>(x, r, ph, sh, t, n, rb);
-    red_final(s, rb, 1, nb, S_RR, 0, SOP_BI_NEXT);
+    if (rh) {
+        k_bi_xr2<2><<<nb, SD_BLOCK, 0, s>>>(x, r, ph, sh, t, rh, n, rb);
+        red_final(s, rb, 2, nb, S_TMP2, 0, SOP_BI_NEXT_BE);
+    } else {
+        k_bi_xr2<1><<<nb, SD_BLOCK, 0, s>>>(x, r, ph, sh, t, rh, n, rb);
+        red_final(s, rb, 1, nb, S_RR, 0, SOP_BI_NEXT);
+    }
 }
 void sd_bi_init(hipStream_t s, double *r, double *rh, double *p, double *v, const double *b, const double *t, int n,
                 RedBuf rb) {
@@ -614,10 +633,14 @@ __global__ void __launch_bounds__(SD_BLOCK) k_line_gs(MGLevel<T> v, const T *__r
 // lines otherwise, where S > 1 widens the contiguous run each load covers.
 #define LS_WAVES 8
 #define LS_THREADS (64 * LS_WAVES)
-template <class T, int P, int S>
+// MD: 0 the sweep starts from x = 0 (no neighbour reads), 1 plain sweep, 2
+// first post-smoothing sweep (adds the coarse correction to the neighbours it
+// reads).  A template argument, so that the plain sweep holds no registers for
+// the other two.
+template <class T, int P, int S, int MD>
 __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T *__restrict__ b, T *__restrict__ x,
-                                                          int ax, int color, int zero, MGLevel<T> c, int pro, int LW,
-                                                          int xcd) {
+                                                          int ax, int color, MGLevel<T> c, int LW, int xcd) {
+    constexpr bool zero = MD == 0, pro = MD == 2;
     constexpr int SL = 64 * P;                       // elements per slot
     constexpr int SP = SL + 1;                       // slot pitch in LDS: odd, so that the gathers
                                                      // across lines (consecutive lanes, one slot
@@ -652,7 +675,7 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
     const int s1 = ax == 0 ? Nx : 1, s2 = ax == 2 ? Nx : NxNy;
     const T *o0 = ax == 0 ? v.A.ym : v.A.xm, *o1 = ax == 0 ? v.A.yp : v.A.xp;
     const T *o2 = ax == 2 ? v.A.ym : v.A.zm, *o3 = ax == 2 ? v.A.yp : v.A.zp;
-    T vb[M], vai[M], viw[M], vcp[M], cf[M][4], xn[M][4], pwn[M][4], xcn[M][4];
+    T vb[M], vai[M], viw[M], vcp[M], cf[M][4], xn[M][4], pwn[pro ? M : 1][4], xcn[pro ? M : 1][4];
     int msk[M];
 #pragma unroll
     for (int m = 0; m < M; m++) {
@@ -674,12 +697,12 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
         msk[m] = (int)ok | ((int)hm1 << 1) | ((int)hp1 << 2) | ((int)hm2 << 3) | ((int)hp2 << 4);
         vb[m] = b[k];
         vai[m] = AI[k]; viw[m] = IW[k]; vcp[m] = CP[k];
-        if (!zero) { cf[m][0] = o0[k]; cf[m][1] = o1[k]; cf[m][2] = o2[k]; cf[m][3] = o3[k]; }
-        if (!zero) {
+        if constexpr (!zero) {
+            cf[m][0] = o0[k]; cf[m][1] = o1[k]; cf[m][2] = o2[k]; cf[m][3] = o3[k];
 #pragma unroll
             for (int t = 0; t < 4; t++) xn[m][t] = x[nk[t]];
         }
-        if (pro) {
+        if constexpr (pro) {
             // the neighbours' coarse parents (first post-smoothing sweep)
 #pragma unroll
             for (int t = 0; t < 4; t++) {
@@ -703,10 +726,13 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
         T r = 0, ai = 0, iw = 0, cp = 0;
         if (msk[m] & 1) {
             T sr = vb[m];
-            if (!zero) {
+            if constexpr (!zero) {
 #pragma unroll
-                for (int t = 0; t < 4; t++)
-                    if ((msk[m] >> (t + 1)) & 1) sr -= cf[m][t] * (pro ? xn[m][t] + pwn[m][t] * xcn[m][t] : xn[m][t]);
+                for (int t = 0; t < 4; t++) {
+                    if (!((msk[m] >> (t + 1)) & 1)) continue;
+                    if constexpr (pro) sr -= cf[m][t] * (xn[m][t] + pwn[m][t] * xcn[m][t]);
+                    else sr -= cf[m][t] * xn[m][t];
+                }
             }
             r = sr; ai = vai[m]; iw = viw[m]; cp = vcp[m];
         }
@@ -767,16 +793,23 @@ template <class T> constexpr int ls_smax(int P) {
     return 3 * LS_WAVES * (64 * P + 1) * 4 * (int)sizeof(T) <= 65536 ? 4
          : (3 * LS_WAVES * (64 * P + 1) * 2 * (int)sizeof(T) <= 65536 ? 2 : 1);
 }
+template <class T, int P, int MD>
+static void ls_launch_md(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color,
+                         const MGLevel<T> &cc, int LW, int xcd) {
+    constexpr int SM = ls_smax<T>(P);
+    if (S >= 4 && SM >= 4)
+        k_line_scan<T, P, (SM >= 4 ? 4 : 1), MD><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, cc, LW, xcd);
+    else if (S >= 2 && SM >= 2)
+        k_line_scan<T, P, (SM >= 2 ? 2 : 1), MD><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, cc, LW, xcd);
+    else
+        k_line_scan<T, P, 1, MD><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, cc, LW, xcd);
+}
 template <class T, int P>
 static void ls_launch(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero,
                       const MGLevel<T> &cc, int pro, int LW, int xcd) {
-    constexpr int SM = ls_smax<T>(P);
-    if (S >= 4 && SM >= 4)
-        k_line_scan<T, P, (SM >= 4 ? 4 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW, xcd);
-    else if (S >= 2 && SM >= 2)
-        k_line_scan<T, P, (SM >= 2 ? 2 : 1)><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW, xcd);
-    else
-        k_line_scan<T, P, 1><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, zero, cc, pro, LW, xcd);
+    if (zero) ls_launch_md<T, P, 0>(s, S, nb, v, b, x, ax, color, cc, LW, xcd);
+    else if (pro) ls_launch_md<T, P, 2>(s, S, nb, v, b, x, ax, color, cc, LW, xcd);
+    else ls_launch_md<T, P, 1>(s, S, nb, v, b, x, ax, color, cc, LW, xcd);
 }
 
 int g_line_nl_max = 8;       // lines per workgroup cap of k_line_gs (tuning)
@@ -907,9 +940,45 @@ __global__ void __launch_bounds__(256) k_resid_restrict(MGLevel<T> f, MGLevel<T>
         }
     }
 }
+// The same restriction when the last pre-smoothing sweep solved half of every
+// aggregate exactly (skip != 0) and all three axes are coarsened: only the 4
+// fine nodes of an aggregate whose residual is not 0 get a lane.  Wavefront w
+// takes the w-th of these offsets (di, dj, dl) for 64 consecutive coarse
+// nodes, so every lane works and each load spans 64 consecutive fine nodes of
+// one parity (contiguous in the parity-split order).  The four partial sums
+// meet in LDS and are added in offset order.
+__device__ __forceinline__ void act_offset(int skip, int w, int &di, int &dj, int &dl) {
+    const int lo = w & 1, hi = w >> 1;
+    switch (skip) {
+    case 1: di = lo ^ hi; dj = lo; dl = hi; break;    // (i + j + l) even
+    case 2: di = lo; dj = hi; dl = hi; break;         // x lines: (j + l) even
+    case 3: di = hi; dj = lo; dl = hi; break;         // y lines: (i + l) even
+    default: di = hi; dj = hi; dl = lo; break;        // z lines: (i + j) even
+    }
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_resid_restrict_act(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x,
+                                                            int skip) {
+    __shared__ double part[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, tb = sd_tile();
+    const int K = tb * 64 + lane;
+    int di, dj, dl;
+    act_offset(skip, w, di, dj, dl);
+    part[w][lane] = K < c.N ? agg_resid(f, c, b, x, skip, K, di, dj, dl) : 0.0;
+    __syncthreads();
+    if (threadIdx.x < 64 && K < c.N) {
+        const double sum = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+        const int Kc = mg_kpos(c, K);
+        c.b[Kc] = (T)(sum / c.rs[Kc]);
+    }
+}
+int g_restrict_act = 1;      // 1: active-lane restriction where it applies
 template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x,
                                           int skip) {
-    k_resid_restrict<T><<<(c.N + 31) / 32, 256, 0, s>>>(f, c, b, x, skip);
+    if (g_restrict_act && skip >= 1 && skip <= 4 && f.cx && f.cy && f.cz)
+        k_resid_restrict_act<T><<<(c.N + 63) / 64, 256, 0, s>>>(f, c, b, x, skip);
+    else
+        k_resid_restrict<T><<<(c.N + 31) / 32, 256, 0, s>>>(f, c, b, x, skip);
 }
 
 // The coarse end of the V-cycle in one workgroup: every level from lv[0]

@@ -18,14 +18,19 @@ __device__ __forceinline__ double neutral_psi(const DevView &d, int k, double V,
     return V + VT * (asinh(D / (2.0 * d.niK[k])) - 0.5 * (d.lncK[k] - d.lnvK[k]));
 }
 
+void ph_set_tile_mode(int m) {
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_tile_mode), &m, sizeof(int)) != hipSuccess)
+        fprintf(stderr, "semisim-hip: setting the tile order failed\n");
+}
+
 // ---------------------------------------------------------------- Poisson
 __global__ void __launch_bounds__(SD_BLOCK) k_poisson(DevView d, const double *phi, int build, double *F,
                                                       Sten7<double> A, double *sc, RedBuf rb, int slot) {
     const Geo g = d.g;
     double acc[1] = {0.0};
-    int k0, k1;
-    sd_tile_range(g.N, k0, k1);
-    for (int k = k0 + threadIdx.x; k < k1; k += SD_BLOCK) {
+    int k0, st, k1;
+    sd_node_loop(g.N, k0, st, k1);
+    for (int k = k0; k < k1; k += st) {
         if (d.con_mask[k]) {
             F[k] = 0.0;
             if (build) {

@@ -198,6 +198,9 @@ __device__ __forceinline__ T sd_block_max(T v, T *lds) {
     return s;
 }
 
+#ifndef SD_TILE_MODE_DEFAULT
+#define SD_TILE_MODE_DEFAULT 1
+#endif
 inline int sd_nblocks(long n, int bs = SD_BLOCK) { return (int)((n + bs - 1) / bs); }
 
 // Workgroup b runs on XCD b % 8, and each of the eight XCDs has its own L2.
@@ -205,13 +208,28 @@ inline int sd_nblocks(long n, int bs = SD_BLOCK) { return (int)((n + bs - 1) / b
 // range of tiles: stencil neighbours read across tile boundaries (the next
 // row, the next plane) then mostly come from the same L2.  Identity when the
 // grid is not a multiple of 8.
+//
+// c_tile_mode (one copy per source file, set by sd_set_tile_mode_*):
+//   0  renumbered tiles; node loops cover one contiguous range per workgroup
+//   1  renumbered tiles; node loops are grid-stride over the renumbered tiles
+//   2  launch order; node loops grid-stride
+static __constant__ int c_tile_mode = SD_TILE_MODE_DEFAULT;
 __device__ __forceinline__ int sd_tile() {
-    return (gridDim.x & 7) ? (int)blockIdx.x : (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
+    return (c_tile_mode == 2 || (gridDim.x & 7)) ? (int)blockIdx.x
+                                                 : (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
 }
-// contiguous node range [k0, k1) of this workgroup's tile when a grid of
-// gridDim.x workgroups covers n nodes (for loops over k0 + tid, step SD_BLOCK)
-__device__ __forceinline__ void sd_tile_range(int n, int &k0, int &k1) {
-    const int per = ((n + (int)gridDim.x - 1) / (int)gridDim.x + SD_BLOCK - 1) / SD_BLOCK * SD_BLOCK;
-    k0 = sd_tile() * per;
-    k1 = k0 + per < n ? k0 + per : n;
+// this thread's first node, stride and end for a loop over n nodes by a grid
+// of gridDim.x workgroups of SD_BLOCK threads
+__device__ __forceinline__ void sd_node_loop(int n, int &k0, int &step, int &k1) {
+    if (c_tile_mode == 0) {
+        const int per = ((n + (int)gridDim.x - 1) / (int)gridDim.x + SD_BLOCK - 1) / SD_BLOCK * SD_BLOCK;
+        const int t0 = sd_tile() * per;
+        k0 = t0 + (int)threadIdx.x;
+        k1 = t0 + per < n ? t0 + per : n;
+        step = SD_BLOCK;
+    } else {
+        k0 = sd_tile() * SD_BLOCK + (int)threadIdx.x;
+        k1 = n;
+        step = (int)gridDim.x * SD_BLOCK;
+    }
 }

@@ -72,12 +72,14 @@ void sd_floor(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, con
 void sd_pcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q, int n, RedBuf rb);
 void sd_pcg_update_p(hipStream_t s, double *p, const double *z, int n, RedBuf rb);
 void sd_scalar_op(hipStream_t s, int op, RedBuf rb);
-enum { SOP_PCG_AL = 0, SOP_PCG_BE, SOP_BI_BE, SOP_BI_AL, SOP_BI_OM, SOP_BI_NEXT, SOP_BI_RESTART };
+enum { SOP_PCG_AL = 0, SOP_PCG_BE, SOP_BI_BE, SOP_BI_AL, SOP_BI_OM, SOP_BI_NEXT, SOP_BI_RESTART, SOP_BI_NEXT_BE };
 void sd_bi_p(hipStream_t s, double *p, const double *r, const double *v, int n, RedBuf rb);
 void sd_bi_r1(hipStream_t s, double *r, const double *v, int n, RedBuf rb);   // r -= al v; SS
 void sd_bi_x1(hipStream_t s, double *x, const double *ph, int n, RedBuf rb);   // x += al ph
+// x += al ph + om sh; r -= om t; RR = r.r.  With rh: also rho = rho1, rho1 =
+// rh.r and the next iteration's beta (the dot of its first step)
 void sd_bi_xr2(hipStream_t s, double *x, double *r, const double *ph, const double *sh, const double *t,
-               int n, RedBuf rb);                                              // RR
+               const double *rh, int n, RedBuf rb);
 void sd_bi_init(hipStream_t s, double *r, double *rh, double *p, double *v, const double *b,
                 const double *t, int n, RedBuf rb);                             // RR
 void sd_bi_restart(hipStream_t s, double *rh, double *p, double *v, const double *r, int n);
@@ -108,6 +110,8 @@ extern int g_line_slots;
 extern int g_line_minwg;
 extern int g_line_xcd;
 extern int g_tail_lds;
+extern int g_restrict_act;      // active-lane restriction after an exact half sweep
+void sd_set_tile_mode_linalg(int m);   // c_tile_mode of k_linalg.hip
 void sd_copy_bw(hipStream_t s, const double *a, double *b, long n);   // plain copy (bandwidth reference)
 // level-0 data in the level's storage order and precision: the right-hand
 // side in, the correction out, the operator and the restriction weights

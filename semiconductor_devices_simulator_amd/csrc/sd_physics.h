@@ -12,6 +12,7 @@ struct CurEdges {
     int ncon;
 };
 
+void ph_set_tile_mode(int m);          // c_tile_mode of k_physics.hip
 void ph_poisson(hipStream_t s, const DevView &d, const double *phi, int build, double *F, Sten7<double> A,
                 double *sc, RedBuf rb, int slot);
 void ph_rebuild_np(hipStream_t s, const DevView &d);
