@@ -31,7 +31,8 @@ int  sdx_ktimer_read(double *out);
    (0/1/2 line sweep x/y/z, 3 point GS sweep, 4 SpMV+dot, 5 residual+restriction,
    6 V-cycle, 7 copy of N doubles, 8 Poisson assembly, 9 continuity assembly,
    10 copy of 8M doubles, 11 preconditioner application as the solvers run it,
-   12 dot product read back by the host, timed by the host clock) */
+   12 dot product read back by the host, timed by the host clock, 13/14/15 line
+   sweep x/y/z on level 1) */
 double sdx_microbench(int what, int iters);
 void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS line kernel, 1: line-smoothed
                                               levels, 2: tail sweeps, 3: tail size, 4: wavefront-scan line
@@ -44,7 +45,9 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
                                               with grid-stride node loops, 2 launch order; -1 default),
                                               12: restriction with lanes only for rows not solved exactly
                                               (1) or for every fine node (0), 13: BiCGSTAB computes the
-                                              next rh.r with its x/r update (1) or separately (0) */
+                                              next rh.r with its x/r update (1) or separately (0),
+                                              14: Krylov kernels stage the V-cycle's input and output in
+                                              level-0 storage (1) or the V-cycle converts them (0) */
 
 /* ---- learned warm start ----
    A 3-D convolutional U-Net on a G^3 grid resampled from the mesh (index

@@ -560,7 +560,9 @@ template <class T> static void mg_setup_T() {
         H.pw_one = false;
     }
     for (int L = 0; L < nl - 1; L++) mg_galerkin(s, H.lev[L], H.lev[L + 1]);
-    for (int L = 0; L < nl - 1; L++)
+    // line factors of the levels the line smoother runs on (the others use
+    // point Gauss-Seidel)
+    for (int L = 0; L < nl - 1 && L < G.line_levels; L++)
         for (int a = 0; a < G.n_axes; a++) mg_line_factor(s, H.lev[L], G.mg_axes[a]);
     mg_coarse_factor(s, H.lev[nl - 1], H.inv);
 }
@@ -678,20 +680,37 @@ template <class T> static void smooth_only_T(const T *b, T *x) {
     smooth_pre(H.lev[0], b, x);
     smooth_post<T>(H.lev[0], b, x, nullptr);
 }
-// z = M^-1 r
-static void precond_launch(const double *r, double *z) {
+// The level-0 staging of the current hierarchy (off when level 0 works on the
+// Krylov vectors directly, or when V-cycles replay as graphs, which convert
+// their input and output themselves)
+static Stage0 stage0() {
+    Stage0 st{nullptr, nullptr, 0, 0, 0};
+    if (!G.stage_on || G.use_graphs) return st;
+    if (G.precision == 1 && !G.H32.lev.empty()) {
+        const MGLevel<float> &v0 = G.H32.lev[0];
+        st = Stage0{G.d_b32, G.d_x32, v0.Nx, v0.px, 1};
+    } else if (G.precision == 0 && !G.H64.lev.empty() && G.H64.lev[0].px) {
+        const MGLevel<double> &v0 = G.H64.lev[0];
+        st = Stage0{G.H64.b0, G.H64.x0, v0.Nx, v0.px, 0};
+    }
+    return st;
+}
+// z = M^-1 r.  sin: r is already staged in the level-0 b (r unused); sout:
+// the result stays in the level-0 x (z unused), for a consumer that reads it
+// from there.  Both need stage0() on.
+static void precond_launch(const double *r, double *z, bool sin = false, bool sout = false) {
     if (G.precision == 1) {
         const MGLevel<float> &v0 = G.H32.lev[0];
-        mg_vec_in(G.st, r, v0, G.d_b32);
+        if (!sin) mg_vec_in(G.st, r, v0, G.d_b32);
         if (G.mg_off || G.H32.lev.size() < 2) smooth_only_T<float>(G.d_b32, G.d_x32);
         else vcycle_T<float>(0, G.d_b32, G.d_x32);
-        mg_vec_out(G.st, G.d_x32, v0, z);
+        if (!sout) mg_vec_out(G.st, G.d_x32, v0, z);
     } else if (G.H64.lev[0].px) {
         const MGLevel<double> &v0 = G.H64.lev[0];
-        mg_vec_in(G.st, r, v0, G.H64.b0);
+        if (!sin) mg_vec_in(G.st, r, v0, G.H64.b0);
         if (G.mg_off || G.H64.lev.size() < 2) smooth_only_T<double>(G.H64.b0, G.H64.x0);
         else vcycle_T<double>(0, G.H64.b0, G.H64.x0);
-        mg_vec_out(G.st, G.H64.x0, v0, z);
+        if (!sout) mg_vec_out(G.st, G.H64.x0, v0, z);
     } else {
         if (G.mg_off || G.H64.lev.size() < 2) smooth_only_T<double>(r, z);
         else vcycle_T<double>(0, r, z);
@@ -703,12 +722,12 @@ static void precond_launch(const double *r, double *z) {
 // no longer paces the GPU kernel by kernel.  The graph holds kernels only (no
 // memset or copy nodes).  With the level-0 smoother timer on, every n-th
 // V-cycle runs as plain launches bracketed by HIP events.
-static void precond(const double *r, double *z) {
+static void precond(const double *r, double *z, bool sin = false, bool sout = false) {
     G.n_vcycles++;
     bool sample = KT.on && KT.kind == 0 && (KT.vcycles++ % KT.every) == 0;
-    if (!G.use_graphs || sample) {
+    if (!G.use_graphs || sample || sin || sout) {
         KT.sampling = sample;
-        precond_launch(r, z);
+        precond_launch(r, z, sin, sout);
         KT.sampling = false;
         return;
     }
@@ -758,15 +777,18 @@ static int pcg(const double *b, double *x, int maxit, double rtol) {
     double rz0 = S(S_RZ);
     if (!(rz0 > 0.)) return 0;
     sd_copy(s, p, z, N);
+    const Stage0 st = stage0();
+    const bool sg = st.b != nullptr;
     int it;
     for (it = 1; it <= maxit; it++) {
         G.n_spmv++;
         { bool kt = kt_begin(1);
           sd_spmv_dots(s, g, G.A, p, q, p, nullptr, 1, G.rb, S_PQ, SOP_PCG_AL);
           kt_end(kt); }
-        sd_pcg_update_xr(s, x, r, p, q, N, G.rb);
-        precond(r, z);
-        sd_dot(s, r, z, N, G.rb, S_TMP0, SOP_PCG_BE);
+        sd_pcg_update_xr(s, x, r, p, q, N, G.rb, st);      // r, staged for the V-cycle
+        precond(r, z, sg, sg);
+        if (sg) sd_dot_stage(s, r, z, st, N, G.rb, S_TMP0, SOP_PCG_BE);   // z from the staged result
+        else sd_dot(s, r, z, N, G.rb, S_TMP0, SOP_PCG_BE);
         rd(S_RZ, S_TMP0 + 1);
         double pq = S(S_PQ), rzn = S(S_TMP0);
         if (!(pq > 0.)) {
@@ -801,18 +823,22 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
     sd_scalar_op(s, SOP_BI_RESTART, G.rb);       // rho = al = om = 1
     double rbest = r0;
     int stall = 0, it, restarted = 0;
+    // the V-cycles read p and s staged by the kernels that update them, and
+    // the SpMV after each reads the V-cycle's result where it lies
+    const Stage0 st = stage0();
+    const bool sg = st.b != nullptr;
     for (it = 1; it <= maxit; it++) {
         // rho1 = rh.r and beta: here on the first iteration, later by the
         // previous iteration's x/r update
         if (it == 1 || !G.bi_fuse) sd_dot(s, rh, r, N, G.rb, S_RHO1, SOP_BI_BE);
     again:
-        sd_bi_p(s, p, r, v, N, G.rb);
-        precond(p, ph);
+        sd_bi_p(s, p, r, v, N, G.rb, st);
+        precond(p, ph, sg, sg);
         G.n_spmv++;
         { bool kt = kt_begin(1);
-          sd_spmv_dots(s, g, G.A, ph, v, rh, nullptr, 1, G.rb, S_RV, SOP_BI_AL);
+          sd_spmv_dots(s, g, G.A, ph, v, rh, nullptr, 1, G.rb, S_RV, SOP_BI_AL, st);   // also writes ph
           kt_end(kt); }
-        sd_bi_r1(s, r, v, N, G.rb);
+        sd_bi_r1(s, r, v, N, G.rb, st);
         rd(S_AL, S_SS + 1);
         double rho1 = S(S_RHO1), rv = S(S_RV), ss = S(S_SS);
         if (fabs(rho1) < 1e-300 && !restarted) {
@@ -832,10 +858,10 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
             rr = ss;
             break;
         }
-        precond(r, sh);
+        precond(r, sh, sg, sg);
         G.n_spmv++;
         { bool kt = kt_begin(1);
-          sd_spmv_dots(s, g, G.A, sh, t, nullptr, r, 2, G.rb, S_TT, SOP_BI_OM);
+          sd_spmv_dots(s, g, G.A, sh, t, nullptr, r, 2, G.rb, S_TT, SOP_BI_OM, st);    // also writes sh
           kt_end(kt); }
         sd_bi_xr2(s, x, r, ph, sh, t, G.bi_fuse ? rh : nullptr, N, G.rb);
         rd(S_OM, S_RR + 1);
@@ -1622,6 +1648,9 @@ double sdx_microbench(int what, int iters) {
             case 3: mg_rbgs(s, v, G.d_b32, G.d_x32, 1, 0); break;
             case 5: mg_resid_restrict(s, v, G.H32.lev[1], G.d_b32, G.d_x32, 0); break;
             case 6: vcycle_T<float>(0, G.d_b32, G.d_x32); break;
+            case 13: case 14: case 15:
+                if (G.H32.lev.size() > 1) mg_line(s, G.H32.lev[1], G.H32.lev[1].b, G.H32.lev[1].x, what - 13, 0, 0);
+                break;
             default: break;
             }
         } else {
@@ -1631,6 +1660,9 @@ double sdx_microbench(int what, int iters) {
             case 3: mg_rbgs(s, v, G.d_wk[0], G.d_wk[1], 1, 0); break;
             case 5: mg_resid_restrict(s, v, G.H64.lev[1], G.d_wk[0], G.d_wk[1], 0); break;
             case 6: vcycle_T<double>(0, G.d_wk[0], G.d_wk[1]); break;
+            case 13: case 14: case 15:
+                if (G.H64.lev.size() > 1) mg_line(s, G.H64.lev[1], G.H64.lev[1].b, G.H64.lev[1].x, what - 13, 0, 0);
+                break;
             default: break;
             }
         }
@@ -1670,6 +1702,7 @@ void sdx_tune(int what, int value) {
     if (what == 10) g_line_xcd = value != 0;
     if (what == 12) g_restrict_act = value != 0;
     if (what == 13) G.bi_fuse = value != 0;
+    if (what == 14) G.stage_on = value != 0;
     if (what == 11) {
         const int m = value >= 0 && value <= 2 ? value : SD_TILE_MODE_DEFAULT;
         hipDeviceSynchronize();

@@ -105,6 +105,7 @@ struct Engine {
     int tail_n = 1024;                        // levels of at most this many nodes run in the tail kernel
     int use_graphs = 0;                       // replay the captured V-cycle (SEMISIM_GRAPHS=1)
     int bi_fuse = 1;                          // BiCGSTAB: rh.r computed with the x/r update
+    int stage_on = 1;                         // Krylov kernels stage the V-cycle's input/output
     int mg_perm = 1;                          // line-smoothed levels stored parity-split along x (SEMISIM_MG_PERM)
     // learned warm start: mode, saved prediction of the running bias point
     int ws_mode = 0, ws_have = 0;

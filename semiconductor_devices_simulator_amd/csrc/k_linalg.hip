@@ -63,6 +63,36 @@ void sd_maxabsdiff(hipStream_t s, const double *a, const double *b, int n, RedBu
     red_final(s, rb, 1, nb, slot, 1);
 }
 
+// ------------------------------------------------ level-0 staging
+// storage position of natural node k in the staged level-0 arrays
+__device__ __forceinline__ int st_pos(const Stage0 &st, int k) {
+    if (!st.px) return k;
+    const int i = k % st.Nx;
+    return k - i + (i & 1) * ((st.Nx + 1) >> 1) + (i >> 1);
+}
+__device__ __forceinline__ void st_put(const Stage0 &st, int k, double v) {
+    const int q = st_pos(st, k);
+    if (st.f32) ((float *)st.b)[q] = (float)v;
+    else ((double *)st.b)[q] = v;
+}
+__device__ __forceinline__ double st_get(const Stage0 &st, int q) {
+    return st.f32 ? (double)((const float *)st.x)[q] : ((const double *)st.x)[q];
+}
+__global__ void __launch_bounds__(SD_BLOCK) k_dot_stage(const double *r, double *z, Stage0 st, int n, RedBuf rb) {
+    double acc[1] = {0.0};
+    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < n; k += gridDim.x * SD_BLOCK) {
+        const double zk = st_get(st, st_pos(st, k));
+        z[k] = zk;
+        acc[0] += r[k] * zk;
+    }
+    red_partial<1, false>(acc, rb);
+}
+void sd_dot_stage(hipStream_t s, const double *r, double *z, Stage0 st, int n, RedBuf rb, int slot, int post) {
+    int nb = red_grid(n);
+    k_dot_stage<<<nb, SD_BLOCK, 0, s>>>(r, z, st, n, rb);
+    red_final(s, rb, 1, nb, slot, 0, post);
+}
+
 // -------------------------------------------------------- vector kernels
 __global__ void k_fill(double *a, double v, int n) {
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
@@ -119,16 +149,36 @@ template void sd_spmv_T<float>(hipStream_t, const Geo &, Sten7<float>, const flo
 
 // y = A x fused with the dots y.(u0 or y) and y.u1 (Krylov: q = A p with p.q;
 // v = A ph with rh.v; t = A sh with t.t and t.r)
+// STG: x is the staged level-0 solution `in` (its storage order and
+// precision), copied to xw in FP64 natural order on the way
+template <bool STG>
 __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, Sten7<double> A, const double *x,
                                                         double *y, const double *u0, const double *u1, int nv,
-                                                        RedBuf rb) {
+                                                        RedBuf rb, Stage0 in, double *xw) {
     int N = Nx * Ny * Nz, NxNy = Nx * Ny;
     double acc[2] = {0.0, 0.0};
     int k0, st, k1;
     sd_node_loop(N, k0, st, k1);
     for (int k = k0; k < k1; k += st) {
         int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
-        double s = sten_apply<double, double>(A, x, k, i, j, l, Nx, Ny, Nz, false);
+        double s;
+        if constexpr (STG) {
+            // row start is the same in both orders; x positions within the row differ
+            const int h = (Nx + 1) >> 1, row = k - i;
+            auto xp = [&](int ii) { return in.px ? (ii & 1) * h + (ii >> 1) : ii; };
+            const int q = row + xp(i);
+            const double xk = st_get(in, q);
+            xw[k] = xk;
+            s = A.c[k] * xk;
+            if (i > 0) s += A.xm[k] * st_get(in, row + xp(i - 1));
+            if (i < Nx - 1) s += A.xp[k] * st_get(in, row + xp(i + 1));
+            if (j > 0) s += A.ym[k] * st_get(in, q - Nx);
+            if (j < Ny - 1) s += A.yp[k] * st_get(in, q + Nx);
+            if (l > 0) s += A.zm[k] * st_get(in, q - NxNy);
+            if (l < Nz - 1) s += A.zp[k] * st_get(in, q + NxNy);
+        } else {
+            s = sten_apply<double, double>(A, x, k, i, j, l, Nx, Ny, Nz, false);
+        }
         y[k] = s;
         acc[0] += s * (u0 ? u0[k] : s);
         if (nv > 1) acc[1] += s * u1[k];
@@ -136,9 +186,13 @@ __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, 
     red_partial<2, false>(acc, rb);
 }
 void sd_spmv_dots(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, double *y, const double *u0,
-                  const double *u1, int nv, RedBuf rb, int slot, int post) {
+                  const double *u1, int nv, RedBuf rb, int slot, int post, Stage0 in) {
     int nb = red_grid(g.N);
-    k_spmv_dots<<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb);
+    if (in.x)
+        k_spmv_dots<true><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in,
+                                                    const_cast<double *>(x));
+    else
+        k_spmv_dots<false><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, nullptr);
     red_final(s, rb, nv, nb, slot, 0, post);
 }
 
@@ -221,37 +275,44 @@ void red_final(hipStream_t s, RedBuf rb, int nv, int nb, int slot0, int is_max, 
     k_red_final<<<1, SD_BLOCK, 0, s>>>(rb, nv, nb, slot0, is_max, post);
 }
 
-__global__ void k_pcg_xr(double *x, double *r, const double *p, const double *q, int n, const double *sc) {
+__global__ void k_pcg_xr(double *x, double *r, const double *p, const double *q, int n, const double *sc,
+                         Stage0 st) {
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
     if (k >= n || !(sc[S_PQ] > 0.)) return;
     double al = sc[S_AL];
     x[k] += al * p[k];
-    r[k] -= al * q[k];
+    const double rk = r[k] - al * q[k];
+    r[k] = rk;
+    if (st.b) st_put(st, k, rk);
 }
 __global__ void k_pcg_p(double *p, const double *z, int n, const double *sc) {
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
     if (k >= n) return;
     p[k] = z[k] + sc[S_BE] * p[k];
 }
-void sd_pcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q, int n, RedBuf rb) {
-    k_pcg_xr<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(x, r, p, q, n, rb.scal);
+void sd_pcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q, int n, RedBuf rb,
+                      Stage0 st) {
+    k_pcg_xr<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(x, r, p, q, n, rb.scal, st);
 }
 void sd_pcg_update_p(hipStream_t s, double *p, const double *z, int n, RedBuf rb) {
     k_pcg_p<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(p, z, n, rb.scal);
 }
 
-__global__ void k_bi_p(double *p, const double *r, const double *v, int n, const double *sc) {
+__global__ void k_bi_p(double *p, const double *r, const double *v, int n, const double *sc, Stage0 st) {
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
     if (k >= n) return;
     double be = sc[S_BE], om = sc[S_OM];
-    p[k] = r[k] + be * (p[k] - om * v[k]);
+    const double pk = r[k] + be * (p[k] - om * v[k]);
+    p[k] = pk;
+    if (st.b) st_put(st, k, pk);
 }
-__global__ void __launch_bounds__(SD_BLOCK) k_bi_r1(double *r, const double *v, int n, RedBuf rb) {
+__global__ void __launch_bounds__(SD_BLOCK) k_bi_r1(double *r, const double *v, int n, RedBuf rb, Stage0 st) {
     double al = rb.scal[S_AL];
     double acc[1] = {0.0};
     for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < n; k += gridDim.x * SD_BLOCK) {
         double x = r[k] - al * v[k];
         r[k] = x;
+        if (st.b) st_put(st, k, x);
         acc[0] += x * x;
     }
     red_partial<1, false>(acc, rb);
@@ -289,12 +350,12 @@ __global__ void k_bi_restart(double *rh, double *p, double *v, const double *r, 
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
     if (k < n) { rh[k] = r[k]; p[k] = 0.; v[k] = 0.; }
 }
-void sd_bi_p(hipStream_t s, double *p, const double *r, const double *v, int n, RedBuf rb) {
-    k_bi_p<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(p, r, v, n, rb.scal);
+void sd_bi_p(hipStream_t s, double *p, const double *r, const double *v, int n, RedBuf rb, Stage0 st) {
+    k_bi_p<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(p, r, v, n, rb.scal, st);
 }
-void sd_bi_r1(hipStream_t s, double *r, const double *v, int n, RedBuf rb) {
+void sd_bi_r1(hipStream_t s, double *r, const double *v, int n, RedBuf rb, Stage0 st) {
     int nb = red_grid(n);
-    k_bi_r1<<<nb, SD_BLOCK, 0, s>>>(r, v, n, rb);
+    k_bi_r1<<<nb, SD_BLOCK, 0, s>>>(r, v, n, rb, st);
     red_final(s, rb, 1, nb, S_SS, 0);
 }
 void sd_bi_x1(hipStream_t s, double *x, const double *ph, int n, RedBuf rb) {
@@ -508,33 +569,56 @@ __device__ __forceinline__ LineGeom line_geom(int Nx, int Ny, int Nz, int ax) {
     else { g.n = Nz; g.Ntd = Nx; g.s_ax = Nx * Ny; g.s_td = 1; g.s_3 = Nx; }
     return g;
 }
-// one lane per line (both colours), sequential over the line
+// One lane per line (both colours), sequential over the line.  The
+// coefficients of LF_CHUNK consecutive elements are loaded before any of them
+// is used, so a line waits for memory once per chunk rather than once per
+// element; 64-lane workgroups spread the few thousand lines over all CUs.
+#define LF_CHUNK 16
 template <class T>
-__global__ void __launch_bounds__(SD_BLOCK) k_line_factor(MGLevel<T> v, int ax) {
+__global__ void __launch_bounds__(64) k_line_factor(MGLevel<T> v, int ax) {
     LineGeom g = line_geom(v.Nx, v.Ny, v.Nz, ax);
     int N3 = ax == 2 ? v.Ny : v.Nz;
-    int t = blockIdx.x * SD_BLOCK + threadIdx.x;
+    int t = blockIdx.x * 64 + threadIdx.x;
     if (t >= g.Ntd * N3) return;
     int td = t % g.Ntd, p3 = t / g.Ntd;
     const T *am = ax == 0 ? v.A.xm : (ax == 1 ? v.A.ym : v.A.zm);
     const T *ap = ax == 0 ? v.A.xp : (ax == 1 ? v.A.yp : v.A.zp);
     T *ai = v.lu[ax][0], *iw = v.lu[ax][1], *cp = v.lu[ax][2];
+    auto node = [&](int q) {
+        return ax == 0 ? mg_idx(v, q, td, p3) : (ax == 1 ? mg_idx(v, td, q, p3) : mg_idx(v, td, p3, q));
+    };
     double cprev = 0.0;
-    for (int q = 0; q < g.n; q++) {
-        const int k = ax == 0 ? mg_idx(v, q, td, p3) : (ax == 1 ? mg_idx(v, td, q, p3) : mg_idx(v, td, p3, q));
-        double dg = v.A.c ? (double)v.A.c[k] : 1.0;
-        if (!(dg != 0.0)) dg = 1.0;
-        double a = q > 0 ? (double)am[k] : 0.0, c = q < g.n - 1 ? (double)ap[k] : 0.0;
-        double w = dg - a * cprev;
-        if (!(w != 0.0)) w = dg;
-        double iwq = 1.0 / w;
-        cprev = c * iwq;
-        ai[k] = (T)(a * iwq); iw[k] = (T)iwq; cp[k] = (T)cprev;
+    for (int q0 = 0; q0 < g.n; q0 += LF_CHUNK) {
+        T dv[LF_CHUNK], av[LF_CHUNK], cv[LF_CHUNK];
+        int kk[LF_CHUNK];
+#pragma unroll
+        for (int u = 0; u < LF_CHUNK; u++) {
+            const int q = q0 + u < g.n ? q0 + u : g.n - 1;
+            const int k = node(q);
+            kk[u] = k;
+            dv[u] = v.A.c ? v.A.c[k] : (T)1;
+            av[u] = am[k];
+            cv[u] = ap[k];
+        }
+#pragma unroll
+        for (int u = 0; u < LF_CHUNK; u++) {
+            const int q = q0 + u;
+            if (q >= g.n) break;
+            double dg = (double)dv[u];
+            if (!(dg != 0.0)) dg = 1.0;
+            double a = q > 0 ? (double)av[u] : 0.0, c = q < g.n - 1 ? (double)cv[u] : 0.0;
+            double w = dg - a * cprev;
+            if (!(w != 0.0)) w = dg;
+            double iwq = 1.0 / w;
+            cprev = c * iwq;
+            const int k = kk[u];
+            ai[k] = (T)(a * iwq); iw[k] = (T)iwq; cp[k] = (T)cprev;
+        }
     }
 }
 template <class T> void mg_line_factor(hipStream_t s, const MGLevel<T> &v, int ax) {
     int N3 = ax == 2 ? v.Ny : v.Nz, Ntd = ax == 0 ? v.Ny : v.Nx;
-    k_line_factor<T><<<sd_nblocks(Ntd * N3), SD_BLOCK, 0, s>>>(v, ax);
+    k_line_factor<T><<<sd_nblocks(Ntd * N3, 64), 64, 0, s>>>(v, ax);
 }
 
 template <class T>

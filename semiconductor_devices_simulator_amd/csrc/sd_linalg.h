@@ -59,22 +59,40 @@ void sd_fill(hipStream_t s, double *a, double v, int n);
 void sd_copy(hipStream_t s, double *dst, const double *src, int n);
 void sd_axpy(hipStream_t s, double *y, const double *x, double a, int n);     // y += a x
 
+// Level-0 staging of the preconditioner's input and output.  The Krylov
+// kernels that produce the V-cycle's right-hand side also write it into the
+// hierarchy's level-0 b (its precision and x order), and the kernels that
+// consume the V-cycle's result read it from the level-0 x, so the V-cycle
+// needs no conversion passes.  b == nullptr: staging off.
+struct Stage0 {
+    void *b;            // level-0 right-hand side (float when f32, else double)
+    const void *x;      // level-0 solution, same storage
+    int Nx, px, f32;    // x-row length, parity-split x order, precision
+};
+
 // y = A x, or y = b - A x (b != nullptr), FP64 level-0 operator
 void sd_spmv(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, const double *b, double *y);
 // y = A x fused with dots: slot <- y.(u0 or y), slot+1 <- y.u1 (nv = 1 or 2); post: SOP_*
+// in.x != nullptr: x is read from the staged level-0 solution instead, and
+// copied to x (FP64, natural order) on the way
 void sd_spmv_dots(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, double *y, const double *u0,
-                  const double *u1, int nv, RedBuf rb, int slot, int post = -1);
+                  const double *u1, int nv, RedBuf rb, int slot, int post = -1, Stage0 in = Stage0{});
+// z = the staged level-0 solution (FP64, natural order) and slot <- r.z
+void sd_dot_stage(hipStream_t s, const double *r, double *z, Stage0 st, int n, RedBuf rb, int slot, int post);
 // bicgstab rounding floor sum_k (sum_j |A_kj x_j| + |b_k|)^2 -> slot
 void sd_floor(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, const double *b,
               RedBuf rb, int slot);
 
 // Krylov fused updates (scalars read from rb.scal on the device)
-void sd_pcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q, int n, RedBuf rb);
+void sd_pcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q, int n, RedBuf rb,
+                      Stage0 st = Stage0{});                                   // stages r
 void sd_pcg_update_p(hipStream_t s, double *p, const double *z, int n, RedBuf rb);
 void sd_scalar_op(hipStream_t s, int op, RedBuf rb);
 enum { SOP_PCG_AL = 0, SOP_PCG_BE, SOP_BI_BE, SOP_BI_AL, SOP_BI_OM, SOP_BI_NEXT, SOP_BI_RESTART, SOP_BI_NEXT_BE };
-void sd_bi_p(hipStream_t s, double *p, const double *r, const double *v, int n, RedBuf rb);
-void sd_bi_r1(hipStream_t s, double *r, const double *v, int n, RedBuf rb);   // r -= al v; SS
+void sd_bi_p(hipStream_t s, double *p, const double *r, const double *v, int n, RedBuf rb,
+             Stage0 st = Stage0{});                                            // stages p
+void sd_bi_r1(hipStream_t s, double *r, const double *v, int n, RedBuf rb,
+              Stage0 st = Stage0{});                                           // r -= al v; SS; stages r
 void sd_bi_x1(hipStream_t s, double *x, const double *ph, int n, RedBuf rb);   // x += al ph
 // x += al ph + om sh; r -= om t; RR = r.r.  With rh: also rho = rho1, rho1 =
 // rh.r and the next iteration's beta (the dot of its first step)

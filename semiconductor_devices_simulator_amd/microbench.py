@@ -25,7 +25,8 @@ import sys
 
 from . import core, devices
 
-KERNELS = ["line_x", "line_y", "line_z", "rbgs", "spmv", "restrict", "vcycle", "copy", "poisson", "cont", "copy_big", "precond", "roundtrip"]
+KERNELS = ["line_x", "line_y", "line_z", "rbgs", "spmv", "restrict", "vcycle", "copy", "poisson", "cont", "copy_big",
+           "precond", "roundtrip", "line_x_l1", "line_y_l1", "line_z_l1"]
 
 
 def algorithmic_bytes(kernel, N, precision):
@@ -34,6 +35,7 @@ def algorithmic_bytes(kernel, N, precision):
         "line_x": N / 2 * 10 * w, "line_y": N / 2 * 10 * w, "line_z": N / 2 * 10 * w,
         "rbgs": N / 2 * 10 * w, "spmv": N * 10 * 8, "restrict": N / 2 * 9 * w + N / 8 * w,
         "vcycle": None, "copy": N * 16, "poisson": N * 161, "cont": N * 190, "copy_big": (8 << 20) * 16, "precond": None, "roundtrip": None,
+        "line_x_l1": N / 16 * 10 * w, "line_y_l1": N / 16 * 10 * w, "line_z_l1": N / 16 * 10 * w,
     }[kernel]
 
 

@@ -174,6 +174,35 @@ def test_vcycle_kernel_variants_agree(hip, dev, op):
         close(z, ref, 1e-11, scale=np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_krylov_staging_matches_plain(hip, dev, precision):
+    """PCG and BiCGSTAB with the V-cycle's input and output staged by the
+    Krylov kernels (tune 14) take the same iterates as with separate
+    conversion passes: same iteration counts, solutions equal to rounding."""
+    g, N = dev
+    hip.sdx_set_precision(precision)
+    res = {}
+    try:
+        for on in (0, 1):
+            hip.sdx_tune(14, on)
+            hip.sdx_test_load_op(dptr(np.ascontiguousarray(g["poisson_A"].ravel())), 0, None)
+            x = np.zeros(N)
+            its_p = hip.sdx_test_pcg(dptr(np.ascontiguousarray(g["poisson_F"])), dptr(x), 200, 1e-10)
+            hip.sdx_test_load_op(dptr(np.ascontiguousarray(g["contn_A"].ravel())), 1,
+                                 dptr(np.ascontiguousarray(g["contn_dsc"])))
+            y = np.ones(N)
+            its = np.zeros(1, np.int32)
+            hip.sdx_test_bicgstab(dptr(np.ascontiguousarray(g["contn_rhs"])), dptr(y), 200, 1e-10, 0.0, iptr(its))
+            res[on] = (its_p, int(its[0]), x, y)
+    finally:
+        hip.sdx_tune(14, 1)
+        hip.sdx_set_precision(0)
+    a, b = res[0], res[1]
+    assert a[0] == b[0] and a[1] == b[1]
+    close(b[2], a[2], 1e-12)
+    close(b[3], a[3], 1e-12)
+
+
 def test_pcg_solves_poisson(hip, dev):
     g, N = dev
     A7 = np.ascontiguousarray(g["poisson_A"].ravel())

@@ -1,0 +1,54 @@
+"""Kernel variants that change only how work is distributed, or the order of
+floating-point sums, on a mesh large enough for every variant to apply
+(64x64x64 n-MOSFET: 256 line tiles per sweep, so the XCD tile ranges are in
+use): the active-lane restriction (tune 12) and the tile orders (tune 11)
+give the same V-cycle to rounding.
+"""
+import numpy as np
+import pytest
+
+from semiconductor_devices_simulator_amd import devices
+from semiconductor_devices_simulator_amd.core import dptr
+
+pytestmark = pytest.mark.gpu
+
+DEFAULTS = {12: 1, 11: -1}
+
+
+@pytest.fixture(scope="module")
+def solved(hip):
+    print("\n[gpu variants] mosfet 64x64x64 equilibrium", flush=True)
+    spec = devices.mosfet(64, 64, 64)
+    devices.load_device(hip, spec)
+    assert hip.sdx_set_precision(1) == 1
+    assert hip.api3_solve() == 1
+    yield spec.N
+    for w, v in DEFAULTS.items():
+        hip.sdx_tune(w, v)
+    hip.sdx_set_precision(0)
+
+
+def _vcycle(hip, N, r, tunes):
+    for w, v in DEFAULTS.items():
+        hip.sdx_tune(w, v)
+    for w, v in tunes.items():
+        hip.sdx_tune(w, v)
+    z = np.zeros(N)
+    hip.sdx_test_precond(dptr(r), dptr(z))
+    return z
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_vcycle_variants(hip, solved, precision):
+    N = solved
+    hip.sdx_set_precision(precision)
+    assert hip.sdx_test_mg_setup() >= 4
+    r = np.random.default_rng(3).standard_normal(N)
+    base = _vcycle(hip, N, r, {})
+    assert np.all(np.isfinite(base)) and np.max(np.abs(base)) > 0
+    tol = 1e-5 if precision == 1 else 1e-11
+    scale = np.max(np.abs(base))
+    for tunes in ({12: 0}, {11: 0}, {11: 1}, {11: 2}, {12: 0, 11: 0}):
+        z = _vcycle(hip, N, r, tunes)
+        err = np.max(np.abs(z - base))
+        assert err <= tol * scale, (tunes, err, scale)
