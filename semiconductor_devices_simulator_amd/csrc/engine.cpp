@@ -949,23 +949,33 @@ static int poisson_newton(double ntol, int maxit) {
 // (ascending start node, neighbour order x-,x+,y-,y+,z-,z+) and solves each
 // block's balance in ln(lambda).
 static int float_fix_host(int hole, int m);
-static int float_fix(int hole) {
+// Spread "anchored" (flag 2) through the majority nodes (flag 1) along the
+// mesh axes until nothing changes: afterwards a majority node is anchored
+// exactly when its 6-connected block of majority nodes touches a contact.
+// Returns the rounds used.
+static int maj_closure() {
     DevView d = eng_view();
     hipStream_t s = G.st;
-    ph_maj_flags(s, d, hole, G.d_sol, G.d_sc, G.d_flag);
-    for (int round = 0; round < 64; round++) {
-        HC(hipMemsetAsync(G.d_changed, 0, 4, s));
+    // two passes over the axes per round; the round ends the propagation when
+    // its second pass changed nothing (the flags are then the closure)
+    int round;
+    for (round = 0; round < 64; round++) {
+        HC(hipMemsetAsync(G.d_changed, 0, 8, s));
         for (int rep = 0; rep < 2; rep++)
-            for (int ax = 0; ax < 3; ax++) ph_maj_propagate(s, d, G.d_flag, ax, G.d_changed);
+            for (int ax = 0; ax < 3; ax++) ph_maj_propagate(s, d, G.d_flag, ax, G.d_changed + rep);
         int ch = 0;
-        HC(hipMemsetAsync(G.d_changed + 1, 0, 4, s));
-        // second pass over the axes must report no change
-        for (int ax = 0; ax < 3; ax++) ph_maj_propagate(s, d, G.d_flag, ax, G.d_changed + 1);
         HC(hipMemcpyAsync(&G.h_scal[S_TMP3], G.d_changed + 1, 4, hipMemcpyDeviceToHost, s));
         eng_sync();
         memcpy(&ch, &G.h_scal[S_TMP3], 4);
         if (!ch) break;
     }
+    return round + 1;
+}
+static int float_fix(int hole) {
+    DevView d = eng_view();
+    hipStream_t s = G.st;
+    ph_maj_flags(s, d, hole, G.d_sol, G.d_sc, G.d_flag);
+    maj_closure();
     HC(hipMemsetAsync(G.d_changed + 2, 0, 4, s));
     ph_maj_list(s, G.N, G.d_flag, G.d_changed + 2, G.d_mlist, G.N);
     HC(hipMemcpyAsync(&G.h_scal[S_TMP3], G.d_changed + 2, 4, hipMemcpyDeviceToHost, s));
@@ -1751,6 +1761,19 @@ int sdx_ktimer_read(double *out) {
     out[0] = (double)KT.sampled; out[1] = KT.total_ms; out[2] = bytes; out[3] = (double)KT.count;
     KT.on = 0;
     return 1;
+}
+// majority-node flags of carrier `hole` for the quantities u, sc: before
+// (0 none, 1 majority, 2 majority next to a contact) and after the closure;
+// returns the propagation rounds
+int sdx_test_maj_flags(int hole, const double *u, const double *sc, unsigned char *f0, unsigned char *f1) {
+    eng_upload(G.d_sol, u, G.N);
+    eng_upload(G.d_sc, sc, G.N);
+    ph_maj_flags(G.st, eng_view(), hole, G.d_sol, G.d_sc, G.d_flag);
+    HC(hipMemcpyAsync(f0, G.d_flag, G.N, hipMemcpyDeviceToHost, G.st));
+    int rounds = maj_closure();
+    HC(hipMemcpyAsync(f1, G.d_flag, G.N, hipMemcpyDeviceToHost, G.st));
+    eng_sync();
+    return rounds;
 }
 void sdx_test_set_state(const double *psi, const double *phin, const double *phip) {
     eng_upload(G.d_psi, psi, G.N); eng_upload(G.d_phin, phin, G.N); eng_upload(G.d_phip, phip, G.N);

@@ -397,9 +397,12 @@ __global__ void k_maj_flags(DevView d, int hole, const double *u, const double *
 void ph_maj_flags(hipStream_t s, const DevView &d, int hole, const double *u, const double *sc, unsigned char *flag) {
     k_maj_flags<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d, hole, u, sc, flag);
 }
-// anchoring travels along whole lines of one axis per launch (forward, then back)
-__global__ void k_maj_prop(Geo g, unsigned char *flag, int axis, int *changed) {
-    int t = blockIdx.x * SD_BLOCK + threadIdx.x;
+// anchoring travels along whole lines of one axis per launch (forward, then
+// back), one lane per line.  The flags of MP_CHUNK consecutive elements are
+// loaded before any is examined, so a lane waits for memory once per chunk.
+#define MP_CHUNK 16
+__global__ void __launch_bounds__(64) k_maj_prop(Geo g, unsigned char *flag, int axis, int *changed) {
+    int t = blockIdx.x * 64 + threadIdx.x;
     int n, stride, k0;
     if (axis == 0) {
         if (t >= g.Ny * g.Nz) return;
@@ -413,26 +416,33 @@ __global__ void k_maj_prop(Geo g, unsigned char *flag, int axis, int *changed) {
         n = g.Nz; stride = g.Nx * g.Ny; k0 = t;
     }
     int ch = 0;
-    unsigned char prev = 0;
-    for (int q = 0; q < n; q++) {
-        int k = k0 + q * stride;
-        unsigned char f = flag[k];
-        if (f == 1 && prev == 2) { f = 2; flag[k] = 2; ch = 1; }
-        prev = f;
-    }
-    prev = 0;
-    for (int q = n - 1; q >= 0; q--) {
-        int k = k0 + q * stride;
-        unsigned char f = flag[k];
-        if (f == 1 && prev == 2) { f = 2; flag[k] = 2; ch = 1; }
-        prev = f;
+    for (int dir = 0; dir < 2; dir++) {
+        unsigned char prev = 0;
+        for (int c0 = 0; c0 < n; c0 += MP_CHUNK) {
+            unsigned char f[MP_CHUNK];
+#pragma unroll
+            for (int u = 0; u < MP_CHUNK; u++) {
+                const int q = c0 + u < n ? c0 + u : n - 1;
+                f[u] = flag[k0 + (dir ? n - 1 - q : q) * stride];
+            }
+#pragma unroll
+            for (int u = 0; u < MP_CHUNK; u++) {
+                if (c0 + u >= n) break;
+                if (f[u] == 1 && prev == 2) {
+                    f[u] = 2;
+                    flag[k0 + (dir ? n - 1 - (c0 + u) : c0 + u) * stride] = 2;
+                    ch = 1;
+                }
+                prev = f[u];
+            }
+        }
     }
     if (ch) *changed = 1;
 }
 void ph_maj_propagate(hipStream_t s, const DevView &d, unsigned char *flag, int axis, int *changed) {
     const Geo &g = d.g;
     int n = axis == 0 ? g.Ny * g.Nz : (axis == 1 ? g.Nx * g.Nz : g.Nx * g.Ny);
-    k_maj_prop<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(g, flag, axis, changed);
+    k_maj_prop<<<sd_nblocks(n, 64), 64, 0, s>>>(g, flag, axis, changed);
 }
 // indices of the majority nodes whose block reached no contact (flag 1);
 // the order of the list is arbitrary, the host sorts it

@@ -256,3 +256,38 @@ def test_continuity_update(hip, dev, hole):
 
 def _dims(g):
     return len(g["xs"]), len(g["ys"]), len(g["zs"])
+
+
+@pytest.mark.parametrize("hole", [0, 1])
+def test_majority_anchoring_closure(hip, dev, hole):
+    """float_fix's anchoring (semiconductor_3d.c:3159-3228): after the
+    propagation a majority node is anchored (2) exactly when its 6-connected
+    block of majority nodes holds a node next to a contact, checked against a
+    breadth-first flood fill on the CPU over a random majority pattern."""
+    import ctypes
+    from collections import deque
+    g, N = dev
+    rng = np.random.default_rng(7 + hole)
+    # s * u far above every threshold on a random 60% of the nodes, 0 elsewhere
+    u = np.ones(N)
+    sc = np.where(rng.random(N) < 0.6, 1e40, 0.0)
+    f0 = np.zeros(N, np.uint8)
+    f1 = np.zeros(N, np.uint8)
+    U8 = ctypes.POINTER(ctypes.c_ubyte)
+    rounds = hip.sdx_test_maj_flags(hole, dptr(u), dptr(sc), f0.ctypes.data_as(U8), f1.ctypes.data_as(U8))
+    assert 1 <= rounds < 64
+    assert set(np.unique(f0)) <= {0, 1, 2} and (f0 == 1).any() and (f0 == 2).any()
+    Nx, Ny, Nz = _dims(g)
+    ref = f0.copy()
+    q = deque(np.flatnonzero(f0 == 2).tolist())
+    while q:
+        k = q.popleft()
+        i, j, l = k % Nx, (k // Nx) % Ny, k // (Nx * Ny)
+        for di, dj, dl in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            a, b, c = i + di, j + dj, l + dl
+            if 0 <= a < Nx and 0 <= b < Ny and 0 <= c < Nz:
+                kb = a + Nx * (b + Ny * c)
+                if ref[kb] == 1:
+                    ref[kb] = 2
+                    q.append(kb)
+    assert np.array_equal(f1, ref)
