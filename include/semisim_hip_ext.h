@@ -21,7 +21,8 @@ int  sdx_kernel_stats(double *out, int n);
 void sdx_reset_stats(void);
 
 /* sampled HIP-event timer on the solver stream: kind 0 = level-0 smoother
-   sweep, 1 = Krylov SpMV; every n-th launch is timed.  read: out[0] timed
+   sweep, 1 = Krylov SpMV, 2 = whole preconditioner application (V-cycle);
+   every n-th launch is timed.  read: out[0] timed
    launches, out[1] their total ms, out[2] algorithmic bytes per launch,
    out[3] launches seen.  Reading stops the timer. */
 void sdx_ktimer_enable(int kind, int every);
@@ -32,7 +33,7 @@ int  sdx_ktimer_read(double *out);
    6 V-cycle, 7 copy of N doubles, 8 Poisson assembly, 9 continuity assembly,
    10 copy of 8M doubles, 11 preconditioner application as the solvers run it,
    12 dot product read back by the host, timed by the host clock, 13/14/15 line
-   sweep x/y/z on level 1) */
+   sweep x/y/z on level 1, 16 SpMV+dot reading the staged level-0 solution) */
 double sdx_microbench(int what, int iters);
 void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS line kernel, 1: line-smoothed
                                               levels, 2: tail sweeps, 3: tail size, 4: wavefront-scan line
@@ -47,7 +48,10 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
                                               (1) or for every fine node (0), 13: BiCGSTAB computes the
                                               next rh.r with its x/r update (1) or separately (0),
                                               14: Krylov kernels stage the V-cycle's input and output in
-                                              level-0 storage (1) or the V-cycle converts them (0) */
+                                              level-0 storage (1) or the V-cycle converts them (0),
+                                              16: coarse-correction scale above the tail in percent
+                                              (100 = none), 17: BF16 level-0 smoother coefficients in the
+                                              FP32 method (1) or FP32 (0) */
 
 /* ---- learned warm start ----
    A 3-D convolutional U-Net on a G^3 grid resampled from the mesh (index

@@ -8,6 +8,7 @@
 // work runs on the device (k_physics.hip, k_linalg.hip); the host keeps the
 // scalar decisions, reading a handful of reduction results per iteration.
 #include "engine.h"
+#include <type_traits>
 #include "sd_nn.h"
 #include <chrono>
 #include <cstdlib>
@@ -74,7 +75,7 @@ static void vgraph_clear() {
 
 template <class T> static void hier_free(Hier<T> &H) {
     H.lev.clear(); H.inv = nullptr; H.alloc = false;
-    H.A0 = Sten7<T>{}; H.rs0 = nullptr; H.b0 = H.x0 = nullptr;
+    H.A0 = Sten7<T>{}; H.rs0 = nullptr; H.b0 = H.x0 = nullptr; H.c16 = nullptr;
 }
 // Storage order of each level: the line-smoothed levels above the coarse tail
 // are stored parity-split along x (MGLevel::px) when their lines fit the scan
@@ -529,6 +530,25 @@ template <class T> static Hier<T> &hier();
 template <> Hier<double> &hier<double>() { return G.H64; }
 template <> Hier<float> &hier<float>() { return G.H32; }
 
+// BF16 copies of level 0's off-diagonals and line factors for the smoother
+// (FP32 hierarchy, when enabled); else the smoother reads the FP32 arrays
+static void level0_bf16(Hier<float> &H) {
+    MGLevel<float> &v0 = H.lev[0];
+    const bool on = G.mg_bf16 && G.n_axes > 0 && G.line_levels > 0 && H.lev.size() > 1;
+    if (on && !H.c16) {
+        H.c16 = dalloc<__bf16>((size_t)15 * v0.N);
+        if (g_err) { H.c16 = nullptr; g_err = 0; G.mg_bf16 = 0; return level0_bf16(H); }
+    }
+    const bool was = v0.o16[0] != nullptr;
+    for (int q = 0; q < 6; q++) v0.o16[q] = on ? H.c16 + (size_t)q * v0.N : nullptr;
+    for (int a = 0; a < 3; a++)
+        for (int q = 0; q < 3; q++) v0.lu16[a][q] = on ? H.c16 + (size_t)(6 + 3 * a + q) * v0.N : nullptr;
+    if (was != on) vgraph_clear();
+    if (!on) return;
+    mg_o16(G.st, v0);
+    for (int a = 0; a < G.n_axes; a++) mg_lu16(G.st, v0, G.mg_axes[a]);
+}
+
 template <class T> static void mg_setup_T() {
     Hier<T> &H = hier<T>();
     hipStream_t s = G.st;
@@ -564,6 +584,7 @@ template <class T> static void mg_setup_T() {
     // point Gauss-Seidel)
     for (int L = 0; L < nl - 1 && L < G.line_levels; L++)
         for (int a = 0; a < G.n_axes; a++) mg_line_factor(s, H.lev[L], G.mg_axes[a]);
+    if constexpr (std::is_same_v<T, float>) level0_bf16(H);
     mg_coarse_factor(s, H.lev[nl - 1], H.inv);
 }
 static void mg_setup() {
@@ -670,7 +691,7 @@ template <class T> static void vcycle_T(int L, const T *b, T *x) {
     if (L == H.tail) { mg_tail(s, H.d_tail, &H.lev[L], nl - L, H.inv, G.tail_sweeps); return; }   // lev[L].b -> .x
     MGLevel<T> &c = H.lev[L + 1];
     int skip = smooth_pre(v, b, x, L);
-    mg_resid_restrict(s, v, c, b, x, skip);   // coarse right-hand side
+    mg_resid_restrict(s, v, c, b, x, skip, G.mg_alpha);   // coarse right-hand side
     vcycle_T<T>(L + 1, c.b, c.x);
     smooth_post(v, b, x, &c, L);
 }
@@ -727,7 +748,9 @@ static void precond(const double *r, double *z, bool sin = false, bool sout = fa
     bool sample = KT.on && KT.kind == 0 && (KT.vcycles++ % KT.every) == 0;
     if (!G.use_graphs || sample || sin || sout) {
         KT.sampling = sample;
+        const bool whole = kt_begin(2);          // kind 2: the whole preconditioner application
         precond_launch(r, z, sin, sout);
+        kt_end(whole);
         KT.sampling = false;
         return;
     }
@@ -1594,6 +1617,7 @@ void eng_read_env() {
     if ((e = getenv("SEMISIM_MG_TAILSW"))) { int q = atoi(e); G.tail_sweeps = q > 0 ? q : 1; }
     G.use_graphs = (e = getenv("SEMISIM_GRAPHS")) ? atoi(e) != 0 : 0;
     G.mg_perm = (e = getenv("SEMISIM_MG_PERM")) ? atoi(e) != 0 : 1;
+    G.mg_bf16 = (e = getenv("SEMISIM_MG_BF16")) ? atoi(e) != 0 : G.mg_bf16;
 }
 int eng_aa_depth() { return G.aa_m < AA_DEPTH ? G.aa_m : AA_DEPTH; }
 void eng_set_predictor(int on) { PRED_ON = on != 0; }
@@ -1650,13 +1674,16 @@ double sdx_microbench(int what, int iters) {
     double *big = nullptr;
     if (what == 10) HC(hipMalloc(&big, 2 * BIG_N * sizeof(double)));
     if (what == 10) HC(hipMemsetAsync(big, 0, 2 * BIG_N * sizeof(double), s));
+    // the restriction as the V-cycle runs it on level 0: after a line sweep
+    // only the rows the last sweep did not solve carry a residual
+    const int skip0 = G.n_axes > 0 && G.line_levels > 0 ? 2 + G.mg_axes[G.n_axes - 1] : 1;
     auto one = [&]() {
         if (G.precision == 1) {
             MGLevel<float> &v = G.H32.lev[0];
             switch (what) {
             case 0: case 1: case 2: mg_line(s, v, G.d_b32, G.d_x32, what, 0, 0); break;
             case 3: mg_rbgs(s, v, G.d_b32, G.d_x32, 1, 0); break;
-            case 5: mg_resid_restrict(s, v, G.H32.lev[1], G.d_b32, G.d_x32, 0); break;
+            case 5: mg_resid_restrict(s, v, G.H32.lev[1], G.d_b32, G.d_x32, skip0); break;
             case 6: vcycle_T<float>(0, G.d_b32, G.d_x32); break;
             case 13: case 14: case 15:
                 if (G.H32.lev.size() > 1) mg_line(s, G.H32.lev[1], G.H32.lev[1].b, G.H32.lev[1].x, what - 13, 0, 0);
@@ -1668,7 +1695,7 @@ double sdx_microbench(int what, int iters) {
             switch (what) {
             case 0: case 1: case 2: mg_line(s, v, G.d_wk[0], G.d_wk[1], what, 0, 0); break;
             case 3: mg_rbgs(s, v, G.d_wk[0], G.d_wk[1], 1, 0); break;
-            case 5: mg_resid_restrict(s, v, G.H64.lev[1], G.d_wk[0], G.d_wk[1], 0); break;
+            case 5: mg_resid_restrict(s, v, G.H64.lev[1], G.d_wk[0], G.d_wk[1], skip0); break;
             case 6: vcycle_T<double>(0, G.d_wk[0], G.d_wk[1]); break;
             case 13: case 14: case 15:
                 if (G.H64.lev.size() > 1) mg_line(s, G.H64.lev[1], G.H64.lev[1].b, G.H64.lev[1].x, what - 13, 0, 0);
@@ -1678,6 +1705,7 @@ double sdx_microbench(int what, int iters) {
         }
         switch (what) {
         case 4: sd_spmv_dots(s, d.g, G.A, G.d_wk[2], G.d_wk[3], G.d_wk[4], nullptr, 1, G.rb, S_TMP1, -1); break;
+        case 16: sd_spmv_dots(s, d.g, G.A, G.d_wk[2], G.d_wk[3], G.d_wk[4], nullptr, 1, G.rb, S_TMP1, -1, stage0()); break;
         case 7: sd_copy_bw(s, G.d_wk[4], G.d_wk[5], G.N); break;
         case 10: sd_copy_bw(s, big, big + BIG_N, BIG_N); break;
         case 11: precond(G.d_wk[0], G.d_wk[1]); break;
@@ -1713,6 +1741,8 @@ void sdx_tune(int what, int value) {
     if (what == 12) g_restrict_act = value != 0;
     if (what == 13) G.bi_fuse = value != 0;
     if (what == 14) G.stage_on = value != 0;
+    if (what == 16) G.mg_alpha = value > 0 ? value / 100.0 : 1.0;
+    if (what == 17) G.mg_bf16 = value != 0;
     if (what == 11) {
         const int m = value >= 0 && value <= 2 ? value : SD_TILE_MODE_DEFAULT;
         hipDeviceSynchronize();
@@ -1755,6 +1785,8 @@ int sdx_ktimer_read(double *out) {
         // coefficients, 4 off-line couplings, b, the x written, plus the other
         // colour's N/2 x values read (point GS: 6 couplings + diag + b + x + x_nb)
         bytes = G.n_axes ? 0.5 * G.N * 10.0 * w : 0.5 * G.N * 10.0 * w;
+    } else if (KT.kind == 2) {
+        bytes = 0.;                              // a whole V-cycle: time only
     } else {
         bytes = (double)G.N * 11.0 * 8.0;   // 7 diagonals + x + dot vector(s) + y (FP64)
     }

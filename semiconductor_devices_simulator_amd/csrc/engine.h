@@ -30,6 +30,9 @@ template <class T> struct Hier {
     Sten7<T> A0{};
     double *rs0 = nullptr;
     T *b0 = nullptr, *x0 = nullptr;
+    // BF16 smoother coefficients of level 0 (FP32 hierarchy; allocated on
+    // first use): 6 off-diagonals, then 3 factors for each of the 3 axes
+    __bf16 *c16 = nullptr;
 };
 
 struct Engine {
@@ -106,6 +109,8 @@ struct Engine {
     int use_graphs = 0;                       // replay the captured V-cycle (SEMISIM_GRAPHS=1)
     int bi_fuse = 1;                          // BiCGSTAB: rh.r computed with the x/r update
     int stage_on = 1;                         // Krylov kernels stage the V-cycle's input/output
+    double mg_alpha = 1.0;                    // coarse-correction scale above the tail
+    int mg_bf16 = 1;                          // FP32 method: BF16 level-0 smoother coefficients
     int mg_perm = 1;                          // line-smoothed levels stored parity-split along x (SEMISIM_MG_PERM)
     // learned warm start: mode, saved prediction of the running bias point
     int ws_mode = 0, ws_have = 0;

@@ -9,6 +9,7 @@
 // pcg :2182-2210, bicgstab_core :2234-2320.  The chunked D-ILU smoother of the
 // reference is replaced by red-black Gauss-Seidel (symmetric ordering in the
 // V-cycle), which changes iteration counts but not the solution.
+#include <type_traits>
 #include "sd_reduce.h"
 
 #define PH_NONE (-1e300)
@@ -75,13 +76,16 @@ __device__ __forceinline__ void st_put(const Stage0 &st, int k, double v) {
     if (st.f32) ((float *)st.b)[q] = (float)v;
     else ((double *)st.b)[q] = v;
 }
-__device__ __forceinline__ double st_get(const Stage0 &st, int q) {
-    return st.f32 ? (double)((const float *)st.x)[q] : ((const double *)st.x)[q];
+// the staged solution's storage type is a template argument, so that no
+// load is issued for the type not in use
+template <class TS> __device__ __forceinline__ double st_get(const Stage0 &st, int q) {
+    return (double)((const TS *)st.x)[q];
 }
+template <class TS>
 __global__ void __launch_bounds__(SD_BLOCK) k_dot_stage(const double *r, double *z, Stage0 st, int n, RedBuf rb) {
     double acc[1] = {0.0};
     for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < n; k += gridDim.x * SD_BLOCK) {
-        const double zk = st_get(st, st_pos(st, k));
+        const double zk = st_get<TS>(st, st_pos(st, k));
         z[k] = zk;
         acc[0] += r[k] * zk;
     }
@@ -89,7 +93,8 @@ __global__ void __launch_bounds__(SD_BLOCK) k_dot_stage(const double *r, double 
 }
 void sd_dot_stage(hipStream_t s, const double *r, double *z, Stage0 st, int n, RedBuf rb, int slot, int post) {
     int nb = red_grid(n);
-    k_dot_stage<<<nb, SD_BLOCK, 0, s>>>(r, z, st, n, rb);
+    if (st.f32) k_dot_stage<float><<<nb, SD_BLOCK, 0, s>>>(r, z, st, n, rb);
+    else k_dot_stage<double><<<nb, SD_BLOCK, 0, s>>>(r, z, st, n, rb);
     red_final(s, rb, 1, nb, slot, 0, post);
 }
 
@@ -149,9 +154,9 @@ template void sd_spmv_T<float>(hipStream_t, const Geo &, Sten7<float>, const flo
 
 // y = A x fused with the dots y.(u0 or y) and y.u1 (Krylov: q = A p with p.q;
 // v = A ph with rh.v; t = A sh with t.t and t.r)
-// STG: x is the staged level-0 solution `in` (its storage order and
-// precision), copied to xw in FP64 natural order on the way
-template <bool STG>
+// TS != void: x is the staged level-0 solution `in` (storage order of the
+// level, type TS), copied to xw in FP64 natural order on the way
+template <class TS>
 __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, Sten7<double> A, const double *x,
                                                         double *y, const double *u0, const double *u1, int nv,
                                                         RedBuf rb, Stage0 in, double *xw) {
@@ -162,20 +167,20 @@ __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, 
     for (int k = k0; k < k1; k += st) {
         int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
         double s;
-        if constexpr (STG) {
+        if constexpr (!std::is_void_v<TS>) {
             // row start is the same in both orders; x positions within the row differ
             const int h = (Nx + 1) >> 1, row = k - i;
             auto xp = [&](int ii) { return in.px ? (ii & 1) * h + (ii >> 1) : ii; };
             const int q = row + xp(i);
-            const double xk = st_get(in, q);
+            const double xk = st_get<TS>(in, q);
             xw[k] = xk;
             s = A.c[k] * xk;
-            if (i > 0) s += A.xm[k] * st_get(in, row + xp(i - 1));
-            if (i < Nx - 1) s += A.xp[k] * st_get(in, row + xp(i + 1));
-            if (j > 0) s += A.ym[k] * st_get(in, q - Nx);
-            if (j < Ny - 1) s += A.yp[k] * st_get(in, q + Nx);
-            if (l > 0) s += A.zm[k] * st_get(in, q - NxNy);
-            if (l < Nz - 1) s += A.zp[k] * st_get(in, q + NxNy);
+            if (i > 0) s += A.xm[k] * st_get<TS>(in, row + xp(i - 1));
+            if (i < Nx - 1) s += A.xp[k] * st_get<TS>(in, row + xp(i + 1));
+            if (j > 0) s += A.ym[k] * st_get<TS>(in, q - Nx);
+            if (j < Ny - 1) s += A.yp[k] * st_get<TS>(in, q + Nx);
+            if (l > 0) s += A.zm[k] * st_get<TS>(in, q - NxNy);
+            if (l < Nz - 1) s += A.zp[k] * st_get<TS>(in, q + NxNy);
         } else {
             s = sten_apply<double, double>(A, x, k, i, j, l, Nx, Ny, Nz, false);
         }
@@ -188,11 +193,13 @@ __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, 
 void sd_spmv_dots(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, double *y, const double *u0,
                   const double *u1, int nv, RedBuf rb, int slot, int post, Stage0 in) {
     int nb = red_grid(g.N);
-    if (in.x)
-        k_spmv_dots<true><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in,
-                                                    const_cast<double *>(x));
+    double *xw = const_cast<double *>(x);
+    if (in.x && in.f32)
+        k_spmv_dots<float><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, xw);
+    else if (in.x)
+        k_spmv_dots<double><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, xw);
     else
-        k_spmv_dots<false><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, nullptr);
+        k_spmv_dots<void><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, nullptr);
     red_final(s, rb, nv, nb, slot, 0, post);
 }
 
@@ -721,7 +728,9 @@ __global__ void __launch_bounds__(SD_BLOCK) k_line_gs(MGLevel<T> v, const T *__r
 // first post-smoothing sweep (adds the coarse correction to the neighbours it
 // reads).  A template argument, so that the plain sweep holds no registers for
 // the other two.
-template <class T, int P, int S, int MD>
+// H16: the off-line couplings and line factors come from the level's BF16
+// copies (o16, lu16), converted to T on load.
+template <class T, int P, int S, int MD, bool H16>
 __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T *__restrict__ b, T *__restrict__ x,
                                                           int ax, int color, MGLevel<T> c, int LW, int xcd) {
     constexpr bool zero = MD == 0, pro = MD == 2;
@@ -745,6 +754,9 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
     const T *__restrict__ AI = v.lu[ax][0];
     const T *__restrict__ IW = v.lu[ax][1];
     const T *__restrict__ CP = v.lu[ax][2];
+    const __bf16 *__restrict__ AI16 = v.lu16[ax][0];
+    const __bf16 *__restrict__ IW16 = v.lu16[ax][1];
+    const __bf16 *__restrict__ CP16 = v.lu16[ax][2];
     // slot elements past the packed lines carry the identity
     for (int t = threadIdx.x; t < LS_WAVES * S * pad; t += LS_THREADS) {
         int q = (t / pad) * SP + used + t % pad;
@@ -759,6 +771,8 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
     const int s1 = ax == 0 ? Nx : 1, s2 = ax == 2 ? Nx : NxNy;
     const T *o0 = ax == 0 ? v.A.ym : v.A.xm, *o1 = ax == 0 ? v.A.yp : v.A.xp;
     const T *o2 = ax == 2 ? v.A.ym : v.A.zm, *o3 = ax == 2 ? v.A.yp : v.A.zp;
+    const __bf16 *h0p = ax == 0 ? v.o16[2] : v.o16[0], *h1p = ax == 0 ? v.o16[3] : v.o16[1];
+    const __bf16 *h2p = ax == 2 ? v.o16[2] : v.o16[4], *h3p = ax == 2 ? v.o16[3] : v.o16[5];
     T vb[M], vai[M], viw[M], vcp[M], cf[M][4], xn[M][4], pwn[pro ? M : 1][4], xcn[pro ? M : 1][4];
     int msk[M];
 #pragma unroll
@@ -780,9 +794,17 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
         const int nk[4] = {hm1 ? m1 : k, hp1 ? p1 : k, hm2 ? k - s2 : k, hp2 ? k + s2 : k};
         msk[m] = (int)ok | ((int)hm1 << 1) | ((int)hp1 << 2) | ((int)hm2 << 3) | ((int)hp2 << 4);
         vb[m] = b[k];
-        vai[m] = AI[k]; viw[m] = IW[k]; vcp[m] = CP[k];
+        if constexpr (H16) {
+            vai[m] = (T)AI16[k]; viw[m] = (T)IW16[k]; vcp[m] = (T)CP16[k];
+        } else {
+            vai[m] = AI[k]; viw[m] = IW[k]; vcp[m] = CP[k];
+        }
         if constexpr (!zero) {
-            cf[m][0] = o0[k]; cf[m][1] = o1[k]; cf[m][2] = o2[k]; cf[m][3] = o3[k];
+            if constexpr (H16) {
+                cf[m][0] = (T)h0p[k]; cf[m][1] = (T)h1p[k]; cf[m][2] = (T)h2p[k]; cf[m][3] = (T)h3p[k];
+            } else {
+                cf[m][0] = o0[k]; cf[m][1] = o1[k]; cf[m][2] = o2[k]; cf[m][3] = o3[k];
+            }
 #pragma unroll
             for (int t = 0; t < 4; t++) xn[m][t] = x[nk[t]];
         }
@@ -877,16 +899,28 @@ template <class T> constexpr int ls_smax(int P) {
     return 3 * LS_WAVES * (64 * P + 1) * 4 * (int)sizeof(T) <= 65536 ? 4
          : (3 * LS_WAVES * (64 * P + 1) * 2 * (int)sizeof(T) <= 65536 ? 2 : 1);
 }
+template <class T, int P, int MD, bool H>
+static void ls_launch_h(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color,
+                        const MGLevel<T> &cc, int LW, int xcd) {
+    constexpr int SM = ls_smax<T>(P);
+    if (S >= 4 && SM >= 4)
+        k_line_scan<T, P, (SM >= 4 ? 4 : 1), MD, H><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, cc, LW, xcd);
+    else if (S >= 2 && SM >= 2)
+        k_line_scan<T, P, (SM >= 2 ? 2 : 1), MD, H><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, cc, LW, xcd);
+    else
+        k_line_scan<T, P, 1, MD, H><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, cc, LW, xcd);
+}
 template <class T, int P, int MD>
 static void ls_launch_md(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color,
                          const MGLevel<T> &cc, int LW, int xcd) {
-    constexpr int SM = ls_smax<T>(P);
-    if (S >= 4 && SM >= 4)
-        k_line_scan<T, P, (SM >= 4 ? 4 : 1), MD><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, cc, LW, xcd);
-    else if (S >= 2 && SM >= 2)
-        k_line_scan<T, P, (SM >= 2 ? 2 : 1), MD><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, cc, LW, xcd);
-    else
-        k_line_scan<T, P, 1, MD><<<nb, LS_THREADS, 0, s>>>(v, b, x, ax, color, cc, LW, xcd);
+    // BF16 coefficients exist only for FP32 levels (and lines up to 128 long)
+    if constexpr (std::is_same_v<T, float> && P <= 2) {
+        if (v.o16[0] && v.lu16[ax][0]) {
+            ls_launch_h<T, P, MD, true>(s, S, nb, v, b, x, ax, color, cc, LW, xcd);
+            return;
+        }
+    }
+    ls_launch_h<T, P, MD, false>(s, S, nb, v, b, x, ax, color, cc, LW, xcd);
 }
 template <class T, int P>
 static void ls_launch(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero,
@@ -1007,7 +1041,8 @@ __device__ __forceinline__ void restrict8(const MGLevel<T> &f, const MGLevel<T> 
 // wavefront spans 64 consecutive fine nodes of one row.  Pair sums by
 // shuffle, then the four rows through LDS.
 template <class T>
-__global__ void __launch_bounds__(256) k_resid_restrict(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x, int skip) {
+__global__ void __launch_bounds__(256) k_resid_restrict(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x, int skip,
+                                                        double alpha) {
     __shared__ double part[4][32];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, tb = sd_tile();
     const int K = tb * 32 + (lane >> 1);
@@ -1020,7 +1055,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(MGLevel<T> f, MGLevel<T>
         if (K2 < c.N) {
             double sum = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
             const int Kc = mg_kpos(c, K2);
-            c.b[Kc] = (T)(sum / c.rs[Kc]);
+            c.b[Kc] = (T)(alpha * sum / c.rs[Kc]);
         }
     }
 }
@@ -1042,7 +1077,7 @@ __device__ __forceinline__ void act_offset(int skip, int w, int &di, int &dj, in
 }
 template <class T>
 __global__ void __launch_bounds__(256) k_resid_restrict_act(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x,
-                                                            int skip) {
+                                                            int skip, double alpha) {
     __shared__ double part[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, tb = sd_tile();
     const int K = tb * 64 + lane;
@@ -1053,16 +1088,16 @@ __global__ void __launch_bounds__(256) k_resid_restrict_act(MGLevel<T> f, MGLeve
     if (threadIdx.x < 64 && K < c.N) {
         const double sum = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
         const int Kc = mg_kpos(c, K);
-        c.b[Kc] = (T)(sum / c.rs[Kc]);
+        c.b[Kc] = (T)(alpha * sum / c.rs[Kc]);
     }
 }
 int g_restrict_act = 1;      // 1: active-lane restriction where it applies
 template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x,
-                                          int skip) {
+                                          int skip, double alpha) {
     if (g_restrict_act && skip >= 1 && skip <= 4 && f.cx && f.cy && f.cz)
-        k_resid_restrict_act<T><<<(c.N + 63) / 64, 256, 0, s>>>(f, c, b, x, skip);
+        k_resid_restrict_act<T><<<(c.N + 63) / 64, 256, 0, s>>>(f, c, b, x, skip, alpha);
     else
-        k_resid_restrict<T><<<(c.N + 31) / 32, 256, 0, s>>>(f, c, b, x, skip);
+        k_resid_restrict<T><<<(c.N + 31) / 32, 256, 0, s>>>(f, c, b, x, skip, alpha);
 }
 
 // The coarse end of the V-cycle in one workgroup: every level from lv[0]
@@ -1343,6 +1378,23 @@ template <class T> __global__ void k_vec_out(const T *x, MGLevel<T> v, double *z
     const int i = k % v.Nx;
     z[k] = (double)x[k - i + mg_xpos(v, i)];
 }
+__global__ void k_o16(MGLevel<float> v) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= v.N) return;
+    const float *src[6] = {v.A.xm, v.A.xp, v.A.ym, v.A.yp, v.A.zm, v.A.zp};
+#pragma unroll
+    for (int q = 0; q < 6; q++) v.o16[q][k] = (__bf16)src[q][k];
+}
+__global__ void k_lu16(MGLevel<float> v, int ax) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= v.N) return;
+#pragma unroll
+    for (int q = 0; q < 3; q++) v.lu16[ax][q][k] = (__bf16)v.lu[ax][q][k];
+}
+void mg_o16(hipStream_t s, const MGLevel<float> &v) { k_o16<<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(v); }
+void mg_lu16(hipStream_t s, const MGLevel<float> &v, int ax) {
+    k_lu16<<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(v, ax);
+}
 template <class T> __global__ void k_sten_in(Sten7<double> A, MGLevel<T> v, Sten7<T> B) {
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
     if (k >= v.N) return;
@@ -1373,7 +1425,7 @@ template <class T> void mg_rs_in(hipStream_t s, const double *w, const MGLevel<T
     template void mg_weights_level<T>(hipStream_t, const MGLevel<T> &, MGLevel<T> &, int, const DevView &, int, double); \
     template void mg_galerkin<T>(hipStream_t, const MGLevel<T> &, MGLevel<T> &);                                  \
     template void mg_rbgs<T>(hipStream_t, const MGLevel<T> &, const T *, T *, int, int, const MGLevel<T> *);                         \
-    template void mg_resid_restrict<T>(hipStream_t, const MGLevel<T> &, const MGLevel<T> &, const T *, const T *, int); \
+    template void mg_resid_restrict<T>(hipStream_t, const MGLevel<T> &, const MGLevel<T> &, const T *, const T *, int, double); \
     template int mg_line<T>(hipStream_t, const MGLevel<T> &, const T *, T *, int, int, int, const MGLevel<T> *); \
     template void mg_line_factor<T>(hipStream_t, const MGLevel<T> &, int); \
     template void mg_tail<T>(hipStream_t, const MGLevel<T> *, const MGLevel<T> *, int, const double *, int); \

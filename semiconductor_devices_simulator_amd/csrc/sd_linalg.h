@@ -39,6 +39,11 @@ template <class T> struct MGLevel {
     double *ph;          // Slotboom reference potential / VT (coarse levels)
     T *x, *b;            // level vectors
     T *lu[3][3];         // line LU factors per axis: a/w, 1/w, c/w (nullptr: axis not factored)
+    // BF16 copies of the off-diagonals (xm, xp, ym, yp, zm, zp) and of the
+    // line factors, read by the line smoother instead of the T arrays when
+    // set (FP32 level 0 with BF16 smoother coefficients)
+    __bf16 *o16[6];
+    __bf16 *lu16[3][3];
 };
 
 // storage position of node (i, j, l) of a level
@@ -108,8 +113,10 @@ template <class T> void mg_weights_level(hipStream_t s, const MGLevel<T> &f, MGL
 template <class T> void mg_galerkin(hipStream_t s, const MGLevel<T> &f, MGLevel<T> &c);
 template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int color, int zero,
                                 const MGLevel<T> *c = nullptr);
+// alpha scales the coarse right-hand side, and so the coarse correction
+// (over-correction of the aggregation coarse space; 1 = plain)
 template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c,
-                                          const T *b, const T *x, int skip);
+                                          const T *b, const T *x, int skip, double alpha = 1.0);
 // levels d_levels[0..nt-1] (device copies; the last one the coarsest) in one workgroup
 template <class T> void mg_tail(hipStream_t s, const MGLevel<T> *d_levels, const MGLevel<T> *h_levels, int nt,
                                 const double *inv, int nsw);
@@ -136,5 +143,8 @@ void sd_copy_bw(hipStream_t s, const double *a, double *b, long n);   // plain c
 template <class T> void mg_vec_in(hipStream_t s, const double *r, const MGLevel<T> &v, T *b);
 template <class T> void mg_vec_out(hipStream_t s, const T *x, const MGLevel<T> &v, double *z);
 template <class T> void mg_sten_in(hipStream_t s, Sten7<double> A, const MGLevel<T> &v, Sten7<T> B);
+// BF16 copies of a level's off-diagonals and of the line factors of axis ax
+void mg_o16(hipStream_t s, const MGLevel<float> &v);
+void mg_lu16(hipStream_t s, const MGLevel<float> &v, int ax);
 template <class T> void mg_rs_in(hipStream_t s, const double *w, const MGLevel<T> &v, double *rs);
 template <class T> void sd_spmv_T(hipStream_t s, const Geo &g, Sten7<T> A, const T *x, const T *b, T *y);

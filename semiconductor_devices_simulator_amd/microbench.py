@@ -26,7 +26,7 @@ import sys
 from . import core, devices
 
 KERNELS = ["line_x", "line_y", "line_z", "rbgs", "spmv", "restrict", "vcycle", "copy", "poisson", "cont", "copy_big",
-           "precond", "roundtrip", "line_x_l1", "line_y_l1", "line_z_l1"]
+           "precond", "roundtrip", "line_x_l1", "line_y_l1", "line_z_l1", "spmv_staged"]
 
 
 def algorithmic_bytes(kernel, N, precision):
@@ -36,6 +36,7 @@ def algorithmic_bytes(kernel, N, precision):
         "rbgs": N / 2 * 10 * w, "spmv": N * 10 * 8, "restrict": N / 2 * 9 * w + N / 8 * w,
         "vcycle": None, "copy": N * 16, "poisson": N * 161, "cont": N * 190, "copy_big": (8 << 20) * 16, "precond": None, "roundtrip": None,
         "line_x_l1": N / 16 * 10 * w, "line_y_l1": N / 16 * 10 * w, "line_z_l1": N / 16 * 10 * w,
+        "spmv_staged": N * (10 * 8 + w),
     }[kernel]
 
 
