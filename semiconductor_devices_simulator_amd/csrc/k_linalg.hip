@@ -724,6 +724,47 @@ __global__ void __launch_bounds__(SD_BLOCK) k_line_gs(MGLevel<T> v, const T *__r
 // lines otherwise, where S > 1 widens the contiguous run each load covers.
 #define LS_WAVES 8
 #define LS_THREADS (64 * LS_WAVES)
+// The wavefront scans of one sweep: wavefront w solves its S slots of 64 P
+// elements in LDS (right-hand side r iw in s_r, ai in s_a, cp in s_c); the
+// solution replaces s_r.
+template <class T, int P, int S>
+__device__ __forceinline__ void line_scan_slots(T *s_r, const T *s_a, const T *s_c) {
+    constexpr int SP = 64 * P + 1;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int si = 0; si < S; si++) {
+        T *R = s_r + (w * S + si) * SP;
+        const T *Aa = s_a + (w * S + si) * SP, *Cc = s_c + (w * S + si) * SP;
+        T ra[P], aa[P], ca[P];
+#pragma unroll
+        for (int p = 0; p < P; p++) { ra[p] = R[lane * P + p]; aa[p] = Aa[lane * P + p]; ca[p] = Cc[lane * P + p]; }
+        T A = 1, B = 0;
+#pragma unroll
+        for (int p = 0; p < P; p++) { A = -aa[p] * A; B = ra[p] - aa[p] * B; }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            T Ap = __shfl_up(A, d, 64), Bp = __shfl_up(B, d, 64);
+            if (lane >= d) { B = B + A * Bp; A = A * Ap; }
+        }
+        T y = __shfl_up(B, 1, 64);
+        if (lane == 0) y = 0;
+#pragma unroll
+        for (int p = 0; p < P; p++) { y = ra[p] - aa[p] * y; ra[p] = y; }
+        A = 1; B = 0;
+#pragma unroll
+        for (int p = P - 1; p >= 0; p--) { A = -ca[p] * A; B = ra[p] - ca[p] * B; }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            T Al = __shfl_down(A, d, 64), Bl = __shfl_down(B, d, 64);
+            if (lane + d < 64) { B = B + A * Bl; A = A * Al; }
+        }
+        T xq = __shfl_down(B, 1, 64);
+        if (lane == 63) xq = 0;
+#pragma unroll
+        for (int p = P - 1; p >= 0; p--) { xq = ra[p] - ca[p] * xq; ra[p] = xq; }
+#pragma unroll
+        for (int p = 0; p < P; p++) R[lane * P + p] = ra[p];
+    }
+}
 // MD: 0 the sweep starts from x = 0 (no neighbour reads), 1 plain sweep, 2
 // first post-smoothing sweep (adds the coarse correction to the neighbours it
 // reads).  A template argument, so that the plain sweep holds no registers for
@@ -846,40 +887,7 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
         s_r[q0] = r * iw; s_a[q0] = ai; s_c[q0] = cp;
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int si = 0; si < S; si++) {
-        T *R = s_r + (w * S + si) * SP;
-        const T *Aa = s_a + (w * S + si) * SP, *Cc = s_c + (w * S + si) * SP;
-        T ra[P], aa[P], ca[P];
-#pragma unroll
-        for (int p = 0; p < P; p++) { ra[p] = R[lane * P + p]; aa[p] = Aa[lane * P + p]; ca[p] = Cc[lane * P + p]; }
-        T A = 1, B = 0;
-#pragma unroll
-        for (int p = 0; p < P; p++) { A = -aa[p] * A; B = ra[p] - aa[p] * B; }
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            T Ap = __shfl_up(A, d, 64), Bp = __shfl_up(B, d, 64);
-            if (lane >= d) { B = B + A * Bp; A = A * Ap; }
-        }
-        T y = __shfl_up(B, 1, 64);
-        if (lane == 0) y = 0;
-#pragma unroll
-        for (int p = 0; p < P; p++) { y = ra[p] - aa[p] * y; ra[p] = y; }
-        A = 1; B = 0;
-#pragma unroll
-        for (int p = P - 1; p >= 0; p--) { A = -ca[p] * A; B = ra[p] - ca[p] * B; }
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            T Al = __shfl_down(A, d, 64), Bl = __shfl_down(B, d, 64);
-            if (lane + d < 64) { B = B + A * Bl; A = A * Al; }
-        }
-        T xq = __shfl_down(B, 1, 64);
-        if (lane == 63) xq = 0;
-#pragma unroll
-        for (int p = P - 1; p >= 0; p--) { xq = ra[p] - ca[p] * xq; ra[p] = xq; }
-#pragma unroll
-        for (int p = 0; p < P; p++) R[lane * P + p] = ra[p];
-    }
+    line_scan_slots<T, P, S>(s_r, s_a, s_c);
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < P * S; m++) {
@@ -893,6 +901,133 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
             x[k] = s_r[(L / LW) * SP + (L % LW) * n + q];
         }
     }
+}
+// The same sweep for one whole line per slot (LW = 1: lines of 64 (P-1) + 1
+// to 64 P elements), with the axis a template argument: lines per workgroup
+// is then a compile-time power of two, the indices of every element are
+// computed once and kept for the three phases, and the x positions of a
+// line and of its neighbours across x follow from the line's parity without
+// division.
+template <class T, int P, int S, int MD, bool H16, int AX>
+__global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T *__restrict__ b, T *__restrict__ x,
+                                                           int color, MGLevel<T> c, int xcd) {
+    constexpr bool zero = MD == 0, pro = MD == 2;
+    constexpr int SL = 64 * P, SP = SL + 1, NLW = LS_WAVES * S, M = P * S;
+    __shared__ T s_r[NLW * SP], s_a[NLW * SP], s_c[NLW * SP];
+    const int Nx = v.Nx, Ny = v.Ny, Nz = v.Nz, NxNy = Nx * Ny;
+    const int n = AX == 0 ? Nx : (AX == 1 ? Ny : Nz);        // line length
+    const int Ntd = AX == 0 ? Ny : Nx;                       // lines of both colours per plane
+    const int ntile = (((Ntd + 1) >> 1) + NLW - 1) / NLW;
+    const int bid = xcd ? (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+    const int p3 = bid / ntile, h0 = (bid - p3 * ntile) * NLW, sh = (p3 + color) & 1;
+    const int E = NLW * n, pad = SL - n, hx = (Nx + 1) >> 1;
+    for (int t = threadIdx.x; t < NLW * pad; t += LS_THREADS) {
+        const int q = (t / pad) * SP + n + t % pad;
+        s_r[q] = 0; s_a[q] = 0; s_c[q] = 0;
+    }
+    constexpr int a_lu = AX;
+    const T *o0 = AX == 0 ? v.A.ym : v.A.xm, *o1 = AX == 0 ? v.A.yp : v.A.xp;
+    const T *o2 = AX == 2 ? v.A.ym : v.A.zm, *o3 = AX == 2 ? v.A.yp : v.A.zp;
+    const __bf16 *h0p = AX == 0 ? v.o16[2] : v.o16[0], *h1p = AX == 0 ? v.o16[3] : v.o16[1];
+    const __bf16 *h2p = AX == 2 ? v.o16[2] : v.o16[4], *h3p = AX == 2 ? v.o16[3] : v.o16[5];
+    const float rn = 1.0f / (float)n;
+    int kk[M], qq[M], msk[M];
+    T vb[M], vai[M], viw[M], vcp[M], cf[M][4], xn[M][4], pwn[pro ? M : 1][4], xcn[pro ? M : 1][4];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const int e = threadIdx.x + LS_THREADS * m;
+        int L, q;
+        if constexpr (AX == 0) {
+            // e / n exactly: (e + 1/2) / n is at least 1/(2n) away from an integer
+            L = (int)(((float)e + 0.5f) * rn);
+            q = e - L * n;
+        } else {
+            q = e / NLW;
+            L = e % NLW;
+        }
+        const int a = h0 + L, td = 2 * a + sh;
+        const bool ok = e < E && td < Ntd;
+        int i, j, l;
+        if constexpr (AX == 0) { i = q; j = td; l = p3; }
+        else if constexpr (AX == 1) { i = td; j = q; l = p3; }
+        else { i = td; j = p3; l = q; }
+        const int row = Nx * (j + Ny * l);
+        int k, km1, kp1, km2, kp2;
+        bool hm1, hp1, hm2, hp2;
+        if constexpr (AX == 0) {
+            k = row + (v.px ? (q & 1) * hx + (q >> 1) : q);
+            hm1 = j > 0; hp1 = j < Ny - 1; km1 = k - Nx; kp1 = k + Nx;
+            hm2 = l > 0; hp2 = l < Nz - 1; km2 = k - NxNy; kp2 = k + NxNy;
+        } else {
+            // x = td = 2 a + sh; x - 1 and x + 1 have the other parity
+            const int xi = v.px ? sh * hx + a : td;
+            const int xm = v.px ? (1 - sh) * hx + a - 1 + sh : td - 1;
+            const int xp = v.px ? (1 - sh) * hx + a + sh : td + 1;
+            k = row + xi; km1 = row + xm; kp1 = row + xp;
+            hm1 = td > 0; hp1 = td < Nx - 1;
+            if constexpr (AX == 1) { hm2 = l > 0; hp2 = l < Nz - 1; km2 = k - NxNy; kp2 = k + NxNy; }
+            else { hm2 = j > 0; hp2 = j < Ny - 1; km2 = k - Nx; kp2 = k + Nx; }
+        }
+        hm1 = hm1 && ok; hp1 = hp1 && ok; hm2 = hm2 && ok; hp2 = hp2 && ok;
+        if (!ok) k = 0;
+        const int nk[4] = {hm1 ? km1 : k, hp1 ? kp1 : k, hm2 ? km2 : k, hp2 ? kp2 : k};
+        msk[m] = (int)ok | ((int)hm1 << 1) | ((int)hp1 << 2) | ((int)hm2 << 3) | ((int)hp2 << 4);
+        kk[m] = k;
+        qq[m] = L * SP + q;
+        vb[m] = b[k];
+        if constexpr (H16) {
+            vai[m] = (T)v.lu16[a_lu][0][k]; viw[m] = (T)v.lu16[a_lu][1][k]; vcp[m] = (T)v.lu16[a_lu][2][k];
+        } else {
+            vai[m] = v.lu[a_lu][0][k]; viw[m] = v.lu[a_lu][1][k]; vcp[m] = v.lu[a_lu][2][k];
+        }
+        if constexpr (!zero) {
+            if constexpr (H16) {
+                cf[m][0] = (T)h0p[k]; cf[m][1] = (T)h1p[k]; cf[m][2] = (T)h2p[k]; cf[m][3] = (T)h3p[k];
+            } else {
+                cf[m][0] = o0[k]; cf[m][1] = o1[k]; cf[m][2] = o2[k]; cf[m][3] = o3[k];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) xn[m][t] = x[nk[t]];
+        }
+        if constexpr (pro) {
+            // the neighbours' coarse parents (first post-smoothing sweep)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int d = (t & 1) ? 1 : -1, on = (msk[m] >> (t + 1)) & 1;
+                int ii = i, jj = j, ll = l;
+                if (t < 2) { if (AX == 0) jj += on * d; else ii += on * d; }
+                else { if (AX == 2) jj += on * d; else ll += on * d; }
+                const int I = v.cx ? ii >> 1 : ii, J = v.cy ? jj >> 1 : jj, LL = v.cz ? ll >> 1 : ll;
+                pwn[m][t] = v.pw[nk[t]];
+                xcn[m][t] = c.x[ok ? mg_idx(c, I, J, LL) : 0];
+            }
+        }
+    }
+    // right-hand sides into LDS
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        if (threadIdx.x + LS_THREADS * m >= E) break;
+        T r = 0, ai = 0, iw = 0, cp = 0;
+        if (msk[m] & 1) {
+            T sr = vb[m];
+            if constexpr (!zero) {
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    if (!((msk[m] >> (t + 1)) & 1)) continue;
+                    if constexpr (pro) sr -= cf[m][t] * (xn[m][t] + pwn[m][t] * xcn[m][t]);
+                    else sr -= cf[m][t] * xn[m][t];
+                }
+            }
+            r = sr; ai = vai[m]; iw = viw[m]; cp = vcp[m];
+        }
+        s_r[qq[m]] = r * iw; s_a[qq[m]] = ai; s_c[qq[m]] = cp;
+    }
+    __syncthreads();
+    line_scan_slots<T, P, S>(s_r, s_a, s_c);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < M; m++)
+        if (msk[m] & 1) x[kk[m]] = s_r[qq[m]];
 }
 // slots per wavefront allowed by the 64 KB of LDS a workgroup may hold
 template <class T> constexpr int ls_smax(int P) {
@@ -922,9 +1057,47 @@ static void ls_launch_md(hipStream_t s, int S, int nb, const MGLevel<T> &v, cons
     }
     ls_launch_h<T, P, MD, false>(s, S, nb, v, b, x, ax, color, cc, LW, xcd);
 }
+int g_line_scan1 = 1;         // 1: axis-specialised kernel for one line per slot
+// launch of k_line_scan1 (one line per slot): slots per wavefront S, sweep
+// kind MD, BF16 coefficients H, axis from the runtime value
+template <class T, int P, int MD, bool H, int AX>
+static void ls1_launch_ax(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int color,
+                          const MGLevel<T> &cc, int xcd) {
+    constexpr int SM = ls_smax<T>(P);
+    if (S >= 4 && SM >= 4)
+        k_line_scan1<T, P, (SM >= 4 ? 4 : 1), MD, H, AX><<<nb, LS_THREADS, 0, s>>>(v, b, x, color, cc, xcd);
+    else if (S >= 2 && SM >= 2)
+        k_line_scan1<T, P, (SM >= 2 ? 2 : 1), MD, H, AX><<<nb, LS_THREADS, 0, s>>>(v, b, x, color, cc, xcd);
+    else
+        k_line_scan1<T, P, 1, MD, H, AX><<<nb, LS_THREADS, 0, s>>>(v, b, x, color, cc, xcd);
+}
+template <class T, int P, int MD, bool H>
+static void ls1_launch_h(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color,
+                         const MGLevel<T> &cc, int xcd) {
+    if (ax == 0) ls1_launch_ax<T, P, MD, H, 0>(s, S, nb, v, b, x, color, cc, xcd);
+    else if (ax == 1) ls1_launch_ax<T, P, MD, H, 1>(s, S, nb, v, b, x, color, cc, xcd);
+    else ls1_launch_ax<T, P, MD, H, 2>(s, S, nb, v, b, x, color, cc, xcd);
+}
+template <class T, int P, int MD>
+static void ls1_launch_md(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color,
+                          const MGLevel<T> &cc, int xcd) {
+    if constexpr (std::is_same_v<T, float> && P <= 2) {
+        if (v.o16[0] && v.lu16[ax][0]) {
+            ls1_launch_h<T, P, MD, true>(s, S, nb, v, b, x, ax, color, cc, xcd);
+            return;
+        }
+    }
+    ls1_launch_h<T, P, MD, false>(s, S, nb, v, b, x, ax, color, cc, xcd);
+}
 template <class T, int P>
 static void ls_launch(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero,
                       const MGLevel<T> &cc, int pro, int LW, int xcd) {
+    if (LW == 1 && g_line_scan1) {
+        if (zero) ls1_launch_md<T, P, 0>(s, S, nb, v, b, x, ax, color, cc, xcd);
+        else if (pro) ls1_launch_md<T, P, 2>(s, S, nb, v, b, x, ax, color, cc, xcd);
+        else ls1_launch_md<T, P, 1>(s, S, nb, v, b, x, ax, color, cc, xcd);
+        return;
+    }
     if (zero) ls_launch_md<T, P, 0>(s, S, nb, v, b, x, ax, color, cc, LW, xcd);
     else if (pro) ls_launch_md<T, P, 2>(s, S, nb, v, b, x, ax, color, cc, LW, xcd);
     else ls_launch_md<T, P, 1>(s, S, nb, v, b, x, ax, color, cc, LW, xcd);
