@@ -724,6 +724,88 @@ __global__ void __launch_bounds__(SD_BLOCK) k_line_gs(MGLevel<T> v, const T *__r
 // lines otherwise, where S > 1 widens the contiguous run each load covers.
 #define LS_WAVES 8
 #define LS_THREADS (64 * LS_WAVES)
+// Wavefront scans of affine maps y -> A y + B.  Lane i holds the composition
+// of its own elements' maps; the scan leaves in lane i the composition of
+// lanes 0..i (forward, the later map applied last) or of lanes i..63
+// (backward).  FP32: four DPP row shifts give the scan within each row of 16
+// lanes, then the row totals, read with readlane, carry it across rows.
+// FP64: shuffles, log2(64) steps.
+#define DPP_ROW_SHL(n) (0x100 + (n))
+#define DPP_ROW_SHR(n) (0x110 + (n))
+#define DPP_WAVE_SHL1 0x130
+#define DPP_WAVE_SHR1 0x138
+// lane move; lanes without a source lane get `old`
+template <int CTRL> __device__ __forceinline__ float dpp_mov(float old, float src) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+template <int CTRL> __device__ __forceinline__ void scan_step(float &A, float &B) {
+    const float Ap = dpp_mov<CTRL>(1.0f, A), Bp = dpp_mov<CTRL>(0.0f, B);
+    B = B + A * Bp;
+    A = A * Ap;
+}
+template <class T> __device__ __forceinline__ void wave_scan_fwd(T &A, T &B, int lane) {
+    if constexpr (std::is_same_v<T, float>) {
+        scan_step<DPP_ROW_SHR(1)>(A, B);
+        scan_step<DPP_ROW_SHR(2)>(A, B);
+        scan_step<DPP_ROW_SHR(4)>(A, B);
+        scan_step<DPP_ROW_SHR(8)>(A, B);
+        // rows 0..2 end at lanes 15, 31, 47; the map entering row r composes them
+        const float a0 = lane_f(A, 15), b0 = lane_f(B, 15), a1 = lane_f(A, 31), b1 = lane_f(B, 31);
+        const float a2 = lane_f(A, 47), b2 = lane_f(B, 47);
+        const float e2a = a1 * a0, e2b = b1 + a1 * b0;
+        const float e3a = a2 * e2a, e3b = b2 + a2 * e2b;
+        const int r = lane >> 4;
+        if (r > 0) {
+            const float ea = r == 1 ? a0 : (r == 2 ? e2a : e3a), eb = r == 1 ? b0 : (r == 2 ? e2b : e3b);
+            B = B + A * eb;
+            A = A * ea;
+        }
+    } else {
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            T Ap = __shfl_up(A, d, 64), Bp = __shfl_up(B, d, 64);
+            if (lane >= d) { B = B + A * Bp; A = A * Ap; }
+        }
+    }
+}
+template <class T> __device__ __forceinline__ void wave_scan_bwd(T &A, T &B, int lane) {
+    if constexpr (std::is_same_v<T, float>) {
+        scan_step<DPP_ROW_SHL(1)>(A, B);
+        scan_step<DPP_ROW_SHL(2)>(A, B);
+        scan_step<DPP_ROW_SHL(4)>(A, B);
+        scan_step<DPP_ROW_SHL(8)>(A, B);
+        // rows 1..3 start at lanes 16, 32, 48
+        const float a1 = lane_f(A, 16), b1 = lane_f(B, 16), a2 = lane_f(A, 32), b2 = lane_f(B, 32);
+        const float a3 = lane_f(A, 48), b3 = lane_f(B, 48);
+        const float e1a = a2 * a3, e1b = b2 + a2 * b3;
+        const float e0a = a1 * e1a, e0b = b1 + a1 * e1b;
+        const int r = lane >> 4;
+        if (r < 3) {
+            const float ea = r == 2 ? a3 : (r == 1 ? e1a : e0a), eb = r == 2 ? b3 : (r == 1 ? e1b : e0b);
+            B = B + A * eb;
+            A = A * ea;
+        }
+    } else {
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            T Al = __shfl_down(A, d, 64), Bl = __shfl_down(B, d, 64);
+            if (lane + d < 64) { B = B + A * Bl; A = A * Al; }
+        }
+    }
+}
+// B of lane i - 1 (forward) or i + 1 (backward); 0 past the ends
+template <class T> __device__ __forceinline__ T wave_prev(T B, int lane) {
+    if constexpr (std::is_same_v<T, float>) return dpp_mov<DPP_WAVE_SHR1>(0.0f, B);
+    else { T y = __shfl_up(B, 1, 64); return lane == 0 ? (T)0 : y; }
+}
+template <class T> __device__ __forceinline__ T wave_next(T B, int lane) {
+    if constexpr (std::is_same_v<T, float>) return dpp_mov<DPP_WAVE_SHL1>(0.0f, B);
+    else { T y = __shfl_down(B, 1, 64); return lane == 63 ? (T)0 : y; }
+}
+
 // The wavefront scans of one sweep: wavefront w solves its S slots of 64 P
 // elements in LDS (right-hand side r iw in s_r, ai in s_a, cp in s_c); the
 // solution replaces s_r.
@@ -740,25 +822,15 @@ __device__ __forceinline__ void line_scan_slots(T *s_r, const T *s_a, const T *s
         T A = 1, B = 0;
 #pragma unroll
         for (int p = 0; p < P; p++) { A = -aa[p] * A; B = ra[p] - aa[p] * B; }
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            T Ap = __shfl_up(A, d, 64), Bp = __shfl_up(B, d, 64);
-            if (lane >= d) { B = B + A * Bp; A = A * Ap; }
-        }
-        T y = __shfl_up(B, 1, 64);
-        if (lane == 0) y = 0;
+        wave_scan_fwd<T>(A, B, lane);
+        T y = wave_prev<T>(B, lane);
 #pragma unroll
         for (int p = 0; p < P; p++) { y = ra[p] - aa[p] * y; ra[p] = y; }
         A = 1; B = 0;
 #pragma unroll
         for (int p = P - 1; p >= 0; p--) { A = -ca[p] * A; B = ra[p] - ca[p] * B; }
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            T Al = __shfl_down(A, d, 64), Bl = __shfl_down(B, d, 64);
-            if (lane + d < 64) { B = B + A * Bl; A = A * Al; }
-        }
-        T xq = __shfl_down(B, 1, 64);
-        if (lane == 63) xq = 0;
+        wave_scan_bwd<T>(A, B, lane);
+        T xq = wave_next<T>(B, lane);
 #pragma unroll
         for (int p = P - 1; p >= 0; p--) { xq = ra[p] - ca[p] * xq; ra[p] = xq; }
 #pragma unroll
@@ -931,12 +1003,8 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
     const __bf16 *h0p = AX == 0 ? v.o16[2] : v.o16[0], *h1p = AX == 0 ? v.o16[3] : v.o16[1];
     const __bf16 *h2p = AX == 2 ? v.o16[2] : v.o16[4], *h3p = AX == 2 ? v.o16[3] : v.o16[5];
     const float rn = 1.0f / (float)n;
-    int kk[M], qq[M], msk[M];
-    T vb[M], vai[M], viw[M], vcp[M], cf[M][4], xn[M][4], pwn[pro ? M : 1][4], xcn[pro ? M : 1][4];
-#pragma unroll
-    for (int m = 0; m < M; m++) {
-        const int e = threadIdx.x + LS_THREADS * m;
-        int L, q;
+    // line L of the tile and position q along it of element e
+    auto split = [&](int e, int &L, int &q) {
         if constexpr (AX == 0) {
             // e / n exactly: (e + 1/2) / n is at least 1/(2n) away from an integer
             L = (int)(((float)e + 0.5f) * rn);
@@ -945,6 +1013,14 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
             q = e / NLW;
             L = e % NLW;
         }
+    };
+    int kk[M], msk[M];
+    T vb[M], vai[M], viw[M], vcp[M], cf[M][4], xn[M][4], pwn[pro ? M : 1][4], xcn[pro ? M : 1][4];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const int e = threadIdx.x + LS_THREADS * m;
+        int L, q;
+        split(e, L, q);
         const int a = h0 + L, td = 2 * a + sh;
         const bool ok = e < E && td < Ntd;
         int i, j, l;
@@ -973,7 +1049,6 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
         const int nk[4] = {hm1 ? km1 : k, hp1 ? kp1 : k, hm2 ? km2 : k, hp2 ? kp2 : k};
         msk[m] = (int)ok | ((int)hm1 << 1) | ((int)hp1 << 2) | ((int)hm2 << 3) | ((int)hp2 << 4);
         kk[m] = k;
-        qq[m] = L * SP + q;
         vb[m] = b[k];
         if constexpr (H16) {
             vai[m] = (T)v.lu16[a_lu][0][k]; viw[m] = (T)v.lu16[a_lu][1][k]; vcp[m] = (T)v.lu16[a_lu][2][k];
@@ -1006,7 +1081,10 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
     // right-hand sides into LDS
 #pragma unroll
     for (int m = 0; m < M; m++) {
-        if (threadIdx.x + LS_THREADS * m >= E) break;
+        const int e = threadIdx.x + LS_THREADS * m;
+        if (e >= E) break;
+        int L, q;
+        split(e, L, q);
         T r = 0, ai = 0, iw = 0, cp = 0;
         if (msk[m] & 1) {
             T sr = vb[m];
@@ -1020,14 +1098,19 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
             }
             r = sr; ai = vai[m]; iw = viw[m]; cp = vcp[m];
         }
-        s_r[qq[m]] = r * iw; s_a[qq[m]] = ai; s_c[qq[m]] = cp;
+        const int q0 = L * SP + q;
+        s_r[q0] = r * iw; s_a[q0] = ai; s_c[q0] = cp;
     }
     __syncthreads();
     line_scan_slots<T, P, S>(s_r, s_a, s_c);
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < M; m++)
-        if (msk[m] & 1) x[kk[m]] = s_r[qq[m]];
+    for (int m = 0; m < M; m++) {
+        if (!(msk[m] & 1)) continue;
+        int L, q;
+        split(threadIdx.x + LS_THREADS * m, L, q);
+        x[kk[m]] = s_r[L * SP + q];
+    }
 }
 // slots per wavefront allowed by the 64 KB of LDS a workgroup may hold
 template <class T> constexpr int ls_smax(int P) {
