@@ -211,7 +211,7 @@ def pmc_traffic(mesh):
     if not os.path.isfile(path):
         return None, None
     for line in open(path):
-        if line.startswith("Level-0 sweep") and "MB per launch" in line:
+        if line.startswith("Level-0") and "MB per launch" in line:
             mb = float(line.rsplit("=", 1)[1].split("MB")[0])
             return round(mb * 1e6), "profiles/" + name
     return None, None

@@ -164,8 +164,9 @@ __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, 
     double acc[2] = {0.0, 0.0};
     int k0, st, k1;
     sd_node_loop(N, k0, st, k1);
-    for (int k = k0; k < k1; k += st) {
-        int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
+    NodeWalk nw(k0, st, Nx, Ny);
+    for (int k = k0; k < k1; k += st, nw.next()) {
+        const int i = nw.i, j = nw.j, l = nw.l;
         double s;
         if constexpr (!std::is_void_v<TS>) {
             // row start is the same in both orders; x positions within the row differ
@@ -215,8 +216,9 @@ __global__ void __launch_bounds__(SD_BLOCK) k_floor(int Nx, int Ny, int Nz, Sten
     double acc[1] = {0.0};
     int k0, st, k1;
     sd_node_loop(N, k0, st, k1);
-    for (int k = k0; k < k1; k += st) {
-        int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
+    NodeWalk nw(k0, st, Nx, Ny);
+    for (int k = k0; k < k1; k += st, nw.next()) {
+        const int i = nw.i, j = nw.j, l = nw.l;
         double s = fabs(A.c[k] * x[k]) + fabs(b[k]);
         if (i > 0) s += fabs(A.xm[k] * x[k - 1]);
         if (i < Nx - 1) s += fabs(A.xp[k] * x[k + 1]);
@@ -1252,8 +1254,7 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
 // (0 outside the aggregate or on a skipped row)
 template <class T>
 __device__ __forceinline__ double agg_resid(const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x, int skip,
-                                            int K, int di, int dj, int dl) {
-    const int cNxNy = c.Nx * c.Ny, LL = K / cNxNy, rr = K - LL * cNxNy, J = rr / c.Nx, I = rr - J * c.Nx;
+                                            int I, int J, int LL, int di, int dj, int dl) {
     const int i = f.cx ? 2 * I + di : I, j = f.cy ? 2 * J + dj : J, l = f.cz ? 2 * LL + dl : LL;
     bool ok = (f.cx || !di) && (f.cy || !dj) && (f.cz || !dl) && i < f.Nx && j < f.Ny && l < f.Nz;
     if (ok && skip == 1 && ((i + j + l) & 1)) ok = false;
@@ -1275,6 +1276,12 @@ __device__ __forceinline__ double agg_resid(const MGLevel<T> &f, const MGLevel<T
     if (l < f.Nz - 1) s += f.A.zp[k] * x[k + NxNy];
     T r = b[k] - s;
     return w * (double)r;
+}
+template <class T>
+__device__ __forceinline__ double agg_resid(const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x, int skip,
+                                            int K, int di, int dj, int dl) {
+    const int cNxNy = c.Nx * c.Ny, LL = K / cNxNy, rr = K - LL * cNxNy, J = rr / c.Nx, I = rr - J * c.Nx;
+    return agg_resid(f, c, b, x, skip, I, J, LL, di, dj, dl);
 }
 // storage position of coarse node K (natural numbering)
 template <class T> __device__ __forceinline__ int mg_kpos(const MGLevel<T> &c, int K) {
@@ -1339,11 +1346,19 @@ __global__ void __launch_bounds__(256) k_resid_restrict_act(MGLevel<T> f, MGLeve
     const int K = tb * 64 + lane;
     int di, dj, dl;
     act_offset(skip, w, di, dj, dl);
-    part[w][lane] = K < c.N ? agg_resid(f, c, b, x, skip, K, di, dj, dl) : 0.0;
+    // coarse coordinates: the workgroup's first node by division, then a
+    // carry along x and y for the lane
+    const int cNxNy = c.Nx * c.Ny, K0 = tb * 64, L0 = K0 / cNxNy, r0 = K0 - L0 * cNxNy, J0 = r0 / c.Nx;
+    int I = r0 - J0 * c.Nx + lane, J = J0, LL = L0;
+    while (I >= c.Nx) {
+        I -= c.Nx;
+        if (++J == c.Ny) { J = 0; LL++; }
+    }
+    part[w][lane] = K < c.N ? agg_resid(f, c, b, x, skip, I, J, LL, di, dj, dl) : 0.0;
     __syncthreads();
     if (threadIdx.x < 64 && K < c.N) {
         const double sum = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
-        const int Kc = mg_kpos(c, K);
+        const int Kc = mg_idx(c, I, J, LL);
         c.b[Kc] = (T)(alpha * sum / c.rs[Kc]);
     }
 }

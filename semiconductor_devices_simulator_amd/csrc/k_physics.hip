@@ -30,7 +30,8 @@ __global__ void __launch_bounds__(SD_BLOCK) k_poisson(DevView d, const double *p
     double acc[1] = {0.0};
     int k0, st, k1;
     sd_node_loop(g.N, k0, st, k1);
-    for (int k = k0; k < k1; k += st) {
+    NodeWalk nw(k0, st, g.Nx, g.Ny);
+    for (int k = k0; k < k1; k += st, nw.next()) {
         if (d.con_mask[k]) {
             F[k] = 0.0;
             if (build) {
@@ -39,8 +40,7 @@ __global__ void __launch_bounds__(SD_BLOCK) k_poisson(DevView d, const double *p
             }
             continue;
         }
-        int i, j, l;
-        sd_unidx(g, k, i, j, l);
+        const int i = nw.i, j = nw.j, l = nw.l;
         double ek = d.eps[k], pk = phi[k], f = 0.0, dg = 0.0, off[6];
 #pragma unroll
         for (int dir = 0; dir < 6; dir++) {

@@ -218,6 +218,27 @@ __device__ __forceinline__ int sd_tile() {
     return (c_tile_mode == 2 || (gridDim.x & 7)) ? (int)blockIdx.x
                                                  : (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
 }
+// coordinates (i, j, l) of the node of a strided node loop, advanced without
+// a division per node: the stride is split into (di, dj, dl) once
+struct NodeWalk {
+    int i, j, l, di, dj, dl, Nx, Ny;
+    __device__ __forceinline__ NodeWalk(int k, int step, int nx, int ny) : Nx(nx), Ny(ny) {
+        const int NxNy = nx * ny;
+        l = k / NxNy;
+        int r = k - l * NxNy;
+        j = r / nx;
+        i = r - j * nx;
+        dl = step / NxNy;
+        r = step - dl * NxNy;
+        dj = r / nx;
+        di = r - dj * nx;
+    }
+    __device__ __forceinline__ void next() {
+        i += di; j += dj; l += dl;
+        if (i >= Nx) { i -= Nx; j++; }
+        if (j >= Ny) { j -= Ny; l++; }
+    }
+};
 // this thread's first node, stride and end for a loop over n nodes by a grid
 // of gridDim.x workgroups of SD_BLOCK threads
 __device__ __forceinline__ void sd_node_loop(int n, int &k0, int &step, int &k1) {
