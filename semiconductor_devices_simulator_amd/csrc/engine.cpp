@@ -1784,8 +1784,10 @@ int sdx_ktimer_read(double *out) {
     if (KT.kind == 0) {
         // one colour of zebra line GS: for the lines' N/2 nodes 3 in-line
         // coefficients, 4 off-line couplings, b, the x written, plus the other
-        // colour's N/2 x values read (point GS: 6 couplings + diag + b + x + x_nb)
-        bytes = G.n_axes ? 0.5 * G.N * 10.0 * w : 0.5 * G.N * 10.0 * w;
+        // colour's N/2 x values read (point GS: 6 couplings + diag + b + x + x_nb);
+        // with BF16 coefficients (FP32 level 0) the 7 coefficients take 2 B each
+        const bool h16 = G.precision == 1 && !G.H32.lev.empty() && G.H32.lev[0].o16[0];
+        bytes = 0.5 * G.N * (h16 ? 3.0 * w + 7.0 * 2.0 : 10.0 * w);
     } else if (KT.kind == 2) {
         bytes = 0.;                              // a whole V-cycle: time only
     } else {

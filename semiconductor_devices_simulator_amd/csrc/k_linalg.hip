@@ -164,30 +164,53 @@ __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, 
     double acc[2] = {0.0, 0.0};
     int k0, st, k1;
     sd_node_loop(N, k0, st, k1);
-    NodeWalk nw(k0, st, Nx, Ny);
-    for (int k = k0; k < k1; k += st, nw.next()) {
-        const int i = nw.i, j = nw.j, l = nw.l;
-        double s;
+    // (A x)_k, and x_k itself
+    auto row = [&](int k, int i, int j, int l, double &xk) -> double {
         if constexpr (!std::is_void_v<TS>) {
             // row start is the same in both orders; x positions within the row differ
-            const int h = (Nx + 1) >> 1, row = k - i;
+            const int h = (Nx + 1) >> 1, r0 = k - i;
             auto xp = [&](int ii) { return in.px ? (ii & 1) * h + (ii >> 1) : ii; };
-            const int q = row + xp(i);
-            const double xk = st_get<TS>(in, q);
-            xw[k] = xk;
-            s = A.c[k] * xk;
-            if (i > 0) s += A.xm[k] * st_get<TS>(in, row + xp(i - 1));
-            if (i < Nx - 1) s += A.xp[k] * st_get<TS>(in, row + xp(i + 1));
+            const int q = r0 + xp(i);
+            xk = st_get<TS>(in, q);
+            double s = A.c[k] * xk;
+            if (i > 0) s += A.xm[k] * st_get<TS>(in, r0 + xp(i - 1));
+            if (i < Nx - 1) s += A.xp[k] * st_get<TS>(in, r0 + xp(i + 1));
             if (j > 0) s += A.ym[k] * st_get<TS>(in, q - Nx);
             if (j < Ny - 1) s += A.yp[k] * st_get<TS>(in, q + Nx);
             if (l > 0) s += A.zm[k] * st_get<TS>(in, q - NxNy);
             if (l < Nz - 1) s += A.zp[k] * st_get<TS>(in, q + NxNy);
+            return s;
         } else {
-            s = sten_apply<double, double>(A, x, k, i, j, l, Nx, Ny, Nz, false);
+            xk = 0.0;
+            return sten_apply<double, double>(A, x, k, i, j, l, Nx, Ny, Nz, false);
         }
-        y[k] = s;
-        acc[0] += s * (u0 ? u0[k] : s);
-        if (nv > 1) acc[1] += s * u1[k];
+    };
+    // two nodes per iteration, both computed before either is stored, so
+    // that the loads of both are in flight together
+    NodeWalk nw(k0, st, Nx, Ny);
+    for (int k = k0; k < k1; k += 2 * st) {
+        const int kb = k + st;
+        NodeWalk nb = nw;
+        nb.next();
+        double xa, xb = 0.0, sb = 0.0;
+        const double sa = row(k, nw.i, nw.j, nw.l, xa);
+        const bool hb = kb < k1;
+        if (hb) sb = row(kb, nb.i, nb.j, nb.l, xb);
+        const double ua = u0 ? u0[k] : sa, va = nv > 1 ? u1[k] : 0.0;
+        double ub = 0.0, vb = 0.0;
+        if (hb) { ub = u0 ? u0[kb] : sb; vb = nv > 1 ? u1[kb] : 0.0; }
+        y[k] = sa;
+        if constexpr (!std::is_void_v<TS>) xw[k] = xa;
+        acc[0] += sa * ua;
+        if (nv > 1) acc[1] += sa * va;
+        if (hb) {
+            y[kb] = sb;
+            if constexpr (!std::is_void_v<TS>) xw[kb] = xb;
+            acc[0] += sb * ub;
+            if (nv > 1) acc[1] += sb * vb;
+        }
+        nb.next();
+        nw = nb;
     }
     red_partial<2, false>(acc, rb);
 }

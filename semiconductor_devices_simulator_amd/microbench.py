@@ -20,6 +20,7 @@ from its captured HIP graph), roundtrip (a dot product whose value the host
 reads back, host wall clock per round trip).
 """
 import argparse
+import os
 import json
 import sys
 
@@ -31,8 +32,11 @@ KERNELS = ["line_x", "line_y", "line_z", "rbgs", "spmv", "restrict", "vcycle", "
 
 def algorithmic_bytes(kernel, N, precision):
     w = 4 if precision == 1 else 8
+    # level-0 line sweep: b, the other colour's x and the x written in T, 4
+    # couplings and 3 line factors in T, or in BF16 in the FP32 method
+    lw = 3 * w + 7 * (2 if precision == 1 and os.environ.get("SEMISIM_MG_BF16", "1") != "0" else w)
     return {
-        "line_x": N / 2 * 10 * w, "line_y": N / 2 * 10 * w, "line_z": N / 2 * 10 * w,
+        "line_x": N / 2 * lw, "line_y": N / 2 * lw, "line_z": N / 2 * lw,
         "rbgs": N / 2 * 10 * w, "spmv": N * 10 * 8, "restrict": N / 2 * 9 * w + N / 8 * w,
         "vcycle": None, "copy": N * 16, "poisson": N * 161, "cont": N * 190, "copy_big": (8 << 20) * 16, "precond": None, "roundtrip": None,
         "line_x_l1": N / 16 * 10 * w, "line_y_l1": N / 16 * 10 * w, "line_z_l1": N / 16 * 10 * w,
