@@ -51,7 +51,10 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
                                               level-0 storage (1) or the V-cycle converts them (0),
                                               16: coarse-correction scale above the tail in percent
                                               (100 = none), 17: BF16 level-0 smoother coefficients in the
-                                              FP32 method (1) or FP32 (0) */
+                                              FP32 method (1) or FP32 (0), 18: axis-specialised line
+                                              kernel for one line per slot (1) or the generic one (0),
+                                              19: non-temporal loads of the operators streamed once per
+                                              use (Krylov SpMV, level-0 restriction) (1) or plain (0) */
 
 /* ---- learned warm start ----
    A 3-D convolutional U-Net on a G^3 grid resampled from the mesh (index

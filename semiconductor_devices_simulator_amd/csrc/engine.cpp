@@ -1744,6 +1744,7 @@ void sdx_tune(int what, int value) {
     if (what == 16) G.mg_alpha = value > 0 ? value / 100.0 : 1.0;
     if (what == 17) G.mg_bf16 = value != 0;
     if (what == 18) g_line_scan1 = value != 0;
+    if (what == 19) g_spmv_nt = value != 0;
     if (what == 11) {
         const int m = value >= 0 && value <= 2 ? value : SD_TILE_MODE_DEFAULT;
         hipDeviceSynchronize();

@@ -64,6 +64,8 @@ void sd_maxabsdiff(hipStream_t s, const double *a, const double *b, int n, RedBu
     red_final(s, rb, 1, nb, slot, 1);
 }
 
+int g_spmv_nt = 1;           // 1: the Krylov SpMV streams the operator with non-temporal loads
+
 // ------------------------------------------------ level-0 staging
 // storage position of natural node k in the staged level-0 arrays
 __device__ __forceinline__ int st_pos(const Stage0 &st, int k) {
@@ -156,7 +158,9 @@ template void sd_spmv_T<float>(hipStream_t, const Geo &, Sten7<float>, const flo
 // v = A ph with rh.v; t = A sh with t.t and t.r)
 // TS != void: x is the staged level-0 solution `in` (storage order of the
 // level, type TS), copied to xw in FP64 natural order on the way
-template <class TS>
+// NT: the operator is read with non-temporal loads (streamed once per
+// product, so it need not displace the multigrid data from the caches)
+template <class TS, bool NT>
 __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, Sten7<double> A, const double *x,
                                                         double *y, const double *u0, const double *u1, int nv,
                                                         RedBuf rb, Stage0 in, double *xw) {
@@ -164,6 +168,10 @@ __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, 
     double acc[2] = {0.0, 0.0};
     int k0, st, k1;
     sd_node_loop(N, k0, st, k1);
+    auto ld = [](const double *p, int k) -> double {
+        if constexpr (NT) return __builtin_nontemporal_load(p + k);
+        else return p[k];
+    };
     // (A x)_k, and x_k itself
     auto row = [&](int k, int i, int j, int l, double &xk) -> double {
         if constexpr (!std::is_void_v<TS>) {
@@ -172,17 +180,24 @@ __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, 
             auto xp = [&](int ii) { return in.px ? (ii & 1) * h + (ii >> 1) : ii; };
             const int q = r0 + xp(i);
             xk = st_get<TS>(in, q);
-            double s = A.c[k] * xk;
-            if (i > 0) s += A.xm[k] * st_get<TS>(in, r0 + xp(i - 1));
-            if (i < Nx - 1) s += A.xp[k] * st_get<TS>(in, r0 + xp(i + 1));
-            if (j > 0) s += A.ym[k] * st_get<TS>(in, q - Nx);
-            if (j < Ny - 1) s += A.yp[k] * st_get<TS>(in, q + Nx);
-            if (l > 0) s += A.zm[k] * st_get<TS>(in, q - NxNy);
-            if (l < Nz - 1) s += A.zp[k] * st_get<TS>(in, q + NxNy);
+            double s = ld(A.c, k) * xk;
+            if (i > 0) s += ld(A.xm, k) * st_get<TS>(in, r0 + xp(i - 1));
+            if (i < Nx - 1) s += ld(A.xp, k) * st_get<TS>(in, r0 + xp(i + 1));
+            if (j > 0) s += ld(A.ym, k) * st_get<TS>(in, q - Nx);
+            if (j < Ny - 1) s += ld(A.yp, k) * st_get<TS>(in, q + Nx);
+            if (l > 0) s += ld(A.zm, k) * st_get<TS>(in, q - NxNy);
+            if (l < Nz - 1) s += ld(A.zp, k) * st_get<TS>(in, q + NxNy);
             return s;
         } else {
             xk = 0.0;
-            return sten_apply<double, double>(A, x, k, i, j, l, Nx, Ny, Nz, false);
+            double s = ld(A.c, k) * x[k];
+            if (i > 0) s += ld(A.xm, k) * x[k - 1];
+            if (i < Nx - 1) s += ld(A.xp, k) * x[k + 1];
+            if (j > 0) s += ld(A.ym, k) * x[k - Nx];
+            if (j < Ny - 1) s += ld(A.yp, k) * x[k + Nx];
+            if (l > 0) s += ld(A.zm, k) * x[k - NxNy];
+            if (l < Nz - 1) s += ld(A.zp, k) * x[k + NxNy];
+            return s;
         }
     };
     // two nodes per iteration, both computed before either is stored, so
@@ -218,12 +233,17 @@ void sd_spmv_dots(hipStream_t s, const Geo &g, Sten7<double> A, const double *x,
                   const double *u1, int nv, RedBuf rb, int slot, int post, Stage0 in) {
     int nb = red_grid(g.N);
     double *xw = const_cast<double *>(x);
-    if (in.x && in.f32)
-        k_spmv_dots<float><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, xw);
-    else if (in.x)
-        k_spmv_dots<double><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, xw);
-    else
-        k_spmv_dots<void><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, nullptr);
+    auto go = [&](auto nt) {
+        constexpr bool NT = decltype(nt)::value;
+        if (in.x && in.f32)
+            k_spmv_dots<float, NT><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, xw);
+        else if (in.x)
+            k_spmv_dots<double, NT><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, xw);
+        else
+            k_spmv_dots<void, NT><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, nullptr);
+    };
+    if (g_spmv_nt) go(std::true_type{});
+    else go(std::false_type{});
     red_final(s, rb, nv, nb, slot, 0, post);
 }
 
@@ -1275,7 +1295,9 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
 // wavefront must call it (the shuffles), lanes past 8 c.N contribute nothing.
 // weighted residual of fine node (2I+di, 2J+dj, 2L+dl) of coarse node K
 // (0 outside the aggregate or on a skipped row)
-template <class T>
+// NT: the operator and row scales are read with non-temporal loads (level 0
+// with BF16 smoother coefficients, where nothing else reads them in the V-cycle)
+template <class T, bool NT = false>
 __device__ __forceinline__ double agg_resid(const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x, int skip,
                                             int I, int J, int LL, int di, int dj, int dl) {
     const int i = f.cx ? 2 * I + di : I, j = f.cy ? 2 * J + dj : J, l = f.cz ? 2 * LL + dl : LL;
@@ -1286,17 +1308,21 @@ __device__ __forceinline__ double agg_resid(const MGLevel<T> &f, const MGLevel<T
         if (par & 1) ok = false;
     }
     if (!ok) return 0.0;
+    auto ld = [](const auto *p, int k) {
+        if constexpr (NT) return __builtin_nontemporal_load(p + k);
+        else return p[k];
+    };
     const int k = mg_idx(f, i, j, l), kx = k - mg_xpos(f, i);
-    const double w = f.rs ? f.rs[k] : 1.0;
+    const double w = f.rs ? ld(f.rs, k) : 1.0;
     if (w == 0.0) return 0.0;
     const int NxNy = f.Nx * f.Ny;
-    T s = f.A.c ? f.A.c[k] * x[k] : x[k];
-    if (i > 0) s += f.A.xm[k] * x[kx + mg_xpos(f, i - 1)];
-    if (i < f.Nx - 1) s += f.A.xp[k] * x[kx + mg_xpos(f, i + 1)];
-    if (j > 0) s += f.A.ym[k] * x[k - f.Nx];
-    if (j < f.Ny - 1) s += f.A.yp[k] * x[k + f.Nx];
-    if (l > 0) s += f.A.zm[k] * x[k - NxNy];
-    if (l < f.Nz - 1) s += f.A.zp[k] * x[k + NxNy];
+    T s = f.A.c ? ld(f.A.c, k) * x[k] : x[k];
+    if (i > 0) s += ld(f.A.xm, k) * x[kx + mg_xpos(f, i - 1)];
+    if (i < f.Nx - 1) s += ld(f.A.xp, k) * x[kx + mg_xpos(f, i + 1)];
+    if (j > 0) s += ld(f.A.ym, k) * x[k - f.Nx];
+    if (j < f.Ny - 1) s += ld(f.A.yp, k) * x[k + f.Nx];
+    if (l > 0) s += ld(f.A.zm, k) * x[k - NxNy];
+    if (l < f.Nz - 1) s += ld(f.A.zp, k) * x[k + NxNy];
     T r = b[k] - s;
     return w * (double)r;
 }
@@ -1361,7 +1387,7 @@ __device__ __forceinline__ void act_offset(int skip, int w, int &di, int &dj, in
     default: di = hi; dj = hi; dl = lo; break;        // z lines: (i + j) even
     }
 }
-template <class T>
+template <class T, bool NT>
 __global__ void __launch_bounds__(256) k_resid_restrict_act(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x,
                                                             int skip, double alpha) {
     __shared__ double part[4][64];
@@ -1377,7 +1403,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict_act(MGLevel<T> f, MGLeve
         I -= c.Nx;
         if (++J == c.Ny) { J = 0; LL++; }
     }
-    part[w][lane] = K < c.N ? agg_resid(f, c, b, x, skip, I, J, LL, di, dj, dl) : 0.0;
+    part[w][lane] = K < c.N ? agg_resid<T, NT>(f, c, b, x, skip, I, J, LL, di, dj, dl) : 0.0;
     __syncthreads();
     if (threadIdx.x < 64 && K < c.N) {
         const double sum = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
@@ -1388,8 +1414,12 @@ __global__ void __launch_bounds__(256) k_resid_restrict_act(MGLevel<T> f, MGLeve
 int g_restrict_act = 1;      // 1: active-lane restriction where it applies
 template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x,
                                           int skip, double alpha) {
-    if (g_restrict_act && skip >= 1 && skip <= 4 && f.cx && f.cy && f.cz)
-        k_resid_restrict_act<T><<<(c.N + 63) / 64, 256, 0, s>>>(f, c, b, x, skip, alpha);
+    if (g_restrict_act && skip >= 1 && skip <= 4 && f.cx && f.cy && f.cz) {
+        if (f.o16[0] && g_spmv_nt)
+            k_resid_restrict_act<T, true><<<(c.N + 63) / 64, 256, 0, s>>>(f, c, b, x, skip, alpha);
+        else
+            k_resid_restrict_act<T, false><<<(c.N + 63) / 64, 256, 0, s>>>(f, c, b, x, skip, alpha);
+    }
     else
         k_resid_restrict<T><<<(c.N + 31) / 32, 256, 0, s>>>(f, c, b, x, skip, alpha);
 }

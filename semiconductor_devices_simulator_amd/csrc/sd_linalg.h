@@ -136,6 +136,7 @@ extern int g_line_minwg;
 extern int g_line_xcd;
 extern int g_tail_lds;
 extern int g_line_scan1;        // axis-specialised line kernel for one line per slot
+extern int g_spmv_nt;            // non-temporal operator loads in the Krylov SpMV
 extern int g_restrict_act;      // active-lane restriction after an exact half sweep
 void sd_set_tile_mode_linalg(int m);   // c_tile_mode of k_linalg.hip
 void sd_copy_bw(hipStream_t s, const double *a, double *b, long n);   // plain copy (bandwidth reference)
