@@ -109,6 +109,7 @@ int    sdx_test_pcg(const double *b, double *x, int maxit, double rtol);
 double sdx_test_bicgstab(const double *b, double *x, int maxit, double rtol, double atol, int *its);
 void   sdx_test_currents(const double *Va, double *I, double *S, double *In);
 double sdx_test_cont_update(int hole, const double *u, const double *sc, double *dq, double lo, double hi);
+void   sdx_test_ii_update(double *gii);             /* impact-ionisation generation, 1/s per node */
 int    sdx_test_maj_flags(int hole, const double *u, const double *sc, unsigned char *f0, unsigned char *f1);
                                                    /* majority flags before/after the anchoring closure */
 

@@ -9,6 +9,7 @@
  *   mg_setup/vcycle semiconductor_3d.c:2126-2165
  *   terminal_currents semiconductor_3d.c:3598-3623
  *   cont_update    semiconductor_3d.c:3112-3146
+ *   ii_update      semiconductor_3d.c:4486-4523
  * Built by oracle/Makefile into oracle/_ref/libsemisim_refh.so.
  */
 #include REF_SRC
@@ -119,3 +120,11 @@ void refh_mat(int mi, double *o){
     memcpy(o,v,sizeof v);
 }
 int refh_threads(void){ return gdev.nthreads; }
+/* impact-ionisation generation at the current state and face mobilities,
+   evaluated as with models bit 8 set (the device's own bits are restored) */
+void refh_ii_update(double *gii){
+    int m=gdev.models; gdev.models|=256;
+    ii_update(&gdev);
+    gdev.models=m;
+    memcpy(gii,gdev.gii,NN(&gdev)*8);
+}

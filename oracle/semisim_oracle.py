@@ -13,7 +13,8 @@ reference function it restates.  Pinning (see tests/test_oracle_*.py):
     identities, and the reference harness (oracle/_ref/libsemisim_refh.so)
     point values;
   * prepare, poisson_eval, face_mobility, cont_assemble, matvec,
-    mg_hierarchy/galerkin/restrict/prolong, terminal_currents, cont_update:
+    mg_hierarchy/galerkin/restrict/prolong, terminal_currents, cont_update,
+    ii_update:
     compared with the reference harness on small devices (golden fixtures
     tests/golden/*.npz written by tests/golden/make_golden.py) and with
     invariants (Poisson Jacobian symmetric on free rows, row sums of the
@@ -66,6 +67,13 @@ def silicon(T=300.0):
     m["Cn"], m["Cp"], m["Brad"], m["tau_Nref"] = 2.8e-43, 9.9e-44, 4.73e-21, 1e22
     m["bgn"] = (6.92e-3, 1.3e23, 0.5)
     m["ni"] = math.sqrt(m["Nc"] * m["Nv"]) * math.exp(-m["Eg"] / (2 * m["VT"]))
+    # van Overstraeten-de Man impact ionisation (:470-472): a, b per carrier
+    # and field range (below / above E0), optical-phonon energy hw (:626)
+    m["ii_a"] = ((7.03e7, 7.03e7), (1.582e8, 6.71e7))
+    m["ii_b"] = ((1.231e8, 1.231e8), (2.036e8, 1.693e8))
+    m["ii_E0"], m["ii_m"] = (4e7, 4e7), (1.0, 1.0)
+    hw = 0.063
+    m["ii_g"] = math.tanh(hw / (2.0 * KB * 300.0 / Q)) / math.tanh(hw / (2.0 * m["VT"]))
     return m
 
 
@@ -507,6 +515,70 @@ def face_flux(dev, ka, kb, h, ar, psi, n, p, fmu, ax):
     a1, a2 = gn * n[kb] * bernoulli(Dn), gn * n[ka] * bernoulli(-Dn)
     b1, b2 = gp * p[ka] * bernoulli(Dp), gp * p[kb] * bernoulli(-Dp)
     return a1 - a2, b1 - b2, a1 + a2, b1 + b2
+
+
+def ii_alpha(m, F, c):
+    """Ionisation coefficient (1/m) of carrier c (0 electrons, 1 holes) at
+    field F (V/m), Chynoweth form, ii_alpha semiconductor_3d.c:4478-4485."""
+    F = np.asarray(F, dtype=np.float64)
+    hi = F >= m["ii_E0"][c]
+    a = np.where(hi, m["ii_a"][c][1], m["ii_a"][c][0])
+    b = np.where(hi, m["ii_b"][c][1], m["ii_b"][c][0])
+    with np.errstate(divide="ignore", over="ignore"):
+        x = m["ii_g"] * b / np.where(F > 0, F, 1.0)
+        if m["ii_m"][c] != 1.0:
+            x = x ** m["ii_m"][c]
+        out = m["ii_g"] * a * np.exp(-np.minimum(x, 800.0))
+    return np.where((F > 1e5) & (x <= 700.0), out, 0.0)
+
+
+def node_field_current(dev, psi, n, p, fmu):
+    """Node current-density vectors of both carriers (mean of the node's
+    semiconductor faces per axis, A/m^2) and the field E = -grad psi
+    (central difference over semiconductor neighbours, one-sided at their
+    ends), as ii_update semiconductor_3d.c:4500-4512 with grad_semi :2909-2917."""
+    N = dev.N
+    Jn, Jp, E = np.zeros((3, N)), np.zeros((3, N)), np.zeros((3, N))
+    semi = dev.kind == K_SEMI
+    for ax in range(3):
+        k, kp, h, ar = dev.edges(ax)
+        s = semi[k] & semi[kp]
+        k, kp, h, ar = k[s], kp[s], h[s], ar[s]
+        a, b, _, _ = face_flux(dev, k, kp, h, ar, psi, n, p, fmu, ax)
+        sn, sp, cnt = np.zeros(N), np.zeros(N), np.zeros(N)
+        for node in (k, kp):                 # the face flux k -> kp seen from both ends
+            np.add.at(sn, node, a / ar)
+            np.add.at(sp, node, b / ar)
+            np.add.at(cnt, node, 1.0)
+        Jn[ax] = np.where(cnt > 0, sn / np.maximum(cnt, 1.0), 0.0)
+        Jp[ax] = np.where(cnt > 0, sp / np.maximum(cnt, 1.0), 0.0)
+        # grad_semi: neighbours on both sides -> central, else one-sided
+        hm, hp = np.zeros(N), np.zeros(N)
+        pm, pp = np.zeros(N), np.zeros(N)
+        has_m, has_p = np.zeros(N, bool), np.zeros(N, bool)
+        has_p[k], hp[k], pp[k] = True, h, psi[kp]
+        has_m[kp], hm[kp], pm[kp] = True, h, psi[k]
+        both = has_m & has_p
+        g = np.where(both, (pp - pm) / np.where(both, hm + hp, 1.0), 0.0)
+        g = np.where(has_p & ~has_m, (pp - psi) / np.where(has_p, hp, 1.0), g)
+        g = np.where(has_m & ~has_p, (psi - pm) / np.where(has_m, hm, 1.0), g)
+        E[ax] = -g
+    return Jn, Jp, E
+
+
+def ii_update(dev, psi, n, p, fmu):
+    """Impact-ionisation pair generation per node (1/s over the control
+    volume): G = (alpha_n(F_n)|J_n| + alpha_p(F_p)|J_p|)/q * vol with
+    F = |E.J|/|J|, on semiconductor nodes that are not contacts,
+    ii_update semiconductor_3d.c:4486-4523."""
+    Jn, Jp, E = node_field_current(dev, psi, n, p, fmu)
+    G = np.zeros(dev.N)
+    for c, J in enumerate((Jn, Jp)):
+        jj = np.sqrt((J * J).sum(axis=0))
+        F = np.abs((E * J).sum(axis=0)) / np.where(jj > 0, jj, 1.0)
+        G += np.where(jj > 0, ii_alpha(dev.m, F, c) * jj, 0.0)
+    on = (dev.kind == K_SEMI) & (dev.con_mask == 0)
+    return np.where(on, G / Q * dev.volume(), 0.0)
 
 
 def terminal_currents(dev, psi, n, p, fmu):

@@ -1886,6 +1886,15 @@ void sdx_test_currents(const double *Va, double *I, double *Sv, double *In) {
     terminal_currents(I, Sv);
     for (int c = 0; c < G.n_cons; c++) In[c] = G.cons[c].In;
 }
+// impact-ionisation generation at the present state and face mobilities,
+// evaluated as with models bit 8 set (the device's own bits are restored)
+void sdx_test_ii_update(double *gii) {
+    int m = G.models;
+    G.models |= 256;
+    ph_ii_update(G.st, eng_view());
+    G.models = m;
+    eng_download(gii, G.d_gii, G.N);
+}
 double sdx_test_cont_update(int hole, const double *u, const double *sc, double *dq, double lo, double hi) {
     eng_upload(G.d_sol, u, G.N); eng_upload(G.d_sc, sc, G.N);
     ph_cont_update(G.st, eng_view(), hole, G.d_sol, G.d_sc, G.d_wk[0], lo, hi, G.rb, S_MAX1);

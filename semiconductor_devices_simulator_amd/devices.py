@@ -201,6 +201,18 @@ def pn_diode(Nx=16, Ny=16, Nz=64):
                       cons, models=0, tau=(1e-7, 1e-7), sweep=[(0, 0.0, 0.8, 17, {})])
 
 
+def avalanche_diode(Nx=8, Ny=8, Nz=48, V1=-19.0, npts=20):
+    """The pn diode of pn_diode() in reverse bias with impact ionisation on
+    (models bit 8, van Overstraeten-de Man coefficients): anode 0 -> V1 in
+    npts points.  The junction breaks down near -20 V; from about -13 V the
+    current shows avalanche multiplication."""
+    spec = pn_diode(Nx, Ny, Nz)
+    spec.name = "avalanche_diode_%dx%dx%d" % (Nx, Ny, Nz)
+    spec.models = 256
+    spec.sweep = [(0, 0.0, V1, npts, {})]
+    return spec
+
+
 def mosfet(Nx=96, Ny=64, Nz=64, vds=(0.05, 1.0), npts=25, nsub=1e17, nsd=1e20, sigma=40.0, tox=2.0, lg=200.0):
     """Planar n-MOSFET, x = channel, y = depth (0.3 um, top y = 0), z = width
     (0.2 um).  p-substrate nsub cm^-3; n+ source/drain Gaussian implants (peak

@@ -41,7 +41,7 @@ def load_harness():
         "refh_cont_update": (D, [I, DP, DP]), "refh_B": (D, [D]), "refh_mob_low": (D, [I, D, I]),
         "refh_bgn_half": (D, [I, D]), "refh_recomb": (None, [I, D, D, D, D, DP]), "refh_mat": (None, [I, DP]),
         "refh_neutral_psi": (D, [ctypes.c_longlong, D]), "refh_set_fmu": (None, [DP]),
-        "refh_apply_bc": (None, [DP]),
+        "refh_apply_bc": (None, [DP]), "refh_ii_update": (None, [DP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -208,8 +208,52 @@ def sweep_fixture(lib, spec, name, fields=True, sample=None):
     print("wrote", name, "points", len(R["V"]), "all converged", bool(np.all(R["conv"])))
 
 
+def ii_fixture(lib, spec, name, Va, rng, perturb=1e-3):
+    """Impact-ionisation generation (ii_update) of the reference at a
+    perturbed reverse-bias state: the state, face mobilities and gii."""
+    cids = devices.load_device(lib, spec)
+    for c, v in enumerate(Va):
+        lib.api3_set_contact_V(cids[c], v)
+    assert lib.api3_solve() == 1, "reference solve failed"
+    N = spec.N
+    psi, phin, phip = np.zeros(N), np.zeros(N), np.zeros(N)
+    lib.api3_get_state(core.dptr(psi), core.dptr(phin), core.dptr(phip))
+    assert lib.refh_prepare() == 1
+    kind = np.zeros(N, np.uint8)
+    lib.refh_get_kind(kind.ctypes.data)
+    cm = np.zeros(N, np.int32)
+    lib.refh_get_conmask(core.iptr(cm))
+    free = (cm == 0) & (kind == 0)
+    psi = psi + perturb * free * rng.standard_normal(N)
+    phin = phin + 0.3 * perturb * free * rng.standard_normal(N)
+    phip = phip + 0.3 * perturb * free * rng.standard_normal(N)
+    lib.refh_set_raw_state(core.dptr(psi), core.dptr(phin), core.dptr(phip))
+    lib.refh_mob0()
+    fmu = np.zeros(6 * N)
+    lib.refh_upd_mob(core.dptr(fmu))
+    gii = np.zeros(N)
+    lib.refh_ii_update(core.dptr(gii))
+    nodes = {}
+    for w, key in ((3, "n"), (4, "p")):
+        a = np.zeros(N)
+        lib.refh_get_node(w, core.dptr(a))
+        nodes[key] = a
+    out = dict(xs=spec.xs, ys=spec.ys, zs=spec.zs, Nd=spec.Nd, Na=spec.Na, models=spec.models,
+               tau=np.asarray(spec.tau if spec.tau else (0.0, 0.0)), Va=np.asarray(Va, np.float64),
+               contacts=json.dumps(contacts_of(spec)), psi=psi, phin=phin, phip=phip, kind=kind, con_mask=cm,
+               fmu=fmu.reshape(6, N), gii=gii, **nodes)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print("wrote", name, N, "nodes; total generation", gii.sum(), "1/s; nonzero nodes", int((gii > 0).sum()))
+
+
 def main():
     lib = load_harness()
+    if "--ii" in sys.argv:             # impact ionisation: kernel fixture
+        ii_fixture(lib, devices.pn_diode(6, 5, 16), "k_ii_diode_6x5x16", [-8.0, 0.0], np.random.default_rng(4321))
+        ii_fixture(lib, devices.mosfet(20, 14, 9), "k_ii_mosfet_20x14x9", [0.0, 3.0, 1.0, 0.0],
+                   np.random.default_rng(4322))
+        sweep_fixture(lib, devices.avalanche_diode(8, 8, 48), "e2e_ii_diode_8x8x48")
+        return
     if "--mosfet96" in sys.argv:       # the full 96x64x64 Id-Vg sweep (about 5 minutes on 8 cores)
         sweep_fixture(lib, devices.mosfet(96, 64, 64), "e2e_mosfet_96x64x64", fields=False, sample=4096)
         return
