@@ -54,7 +54,9 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
                                               FP32 method (1) or FP32 (0), 18: axis-specialised line
                                               kernel for one line per slot (1) or the generic one (0),
                                               19: non-temporal loads of the operators streamed once per
-                                              use (Krylov SpMV, level-0 restriction) (1) or plain (0) */
+                                              use (Krylov SpMV, level-0 restriction) (1) or plain (0),
+                                              20: continuity BiCGSTAB products skip the unit diagonal
+                                              (1) or read it (0) */
 
 /* ---- learned warm start ----
    A 3-D convolutional U-Net on a G^3 grid resampled from the mesh (index

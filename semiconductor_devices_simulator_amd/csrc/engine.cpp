@@ -832,6 +832,10 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
     double *r = G.d_wk[0], *rh = G.d_wk[1], *p = G.d_wk[2], *v = G.d_wk[3], *ph = G.d_wk[4], *sh = G.d_wk[5],
            *t = G.d_wk[6];
     Geo g = eng_view().g;
+    // the products inside the loop skip the diagonal when it is known to be 1
+    // (multiplying by 1 is exact, so the iterates are the same)
+    Sten7<double> Ak = G.A;
+    if (G.unit_diag) Ak.c = nullptr;
     if (!G.mg_off) mg_setup();
     matvec(x, t);
     sd_bi_init(s, r, rh, p, v, b, t, N, G.rb);
@@ -859,7 +863,7 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
         precond(p, ph, sg, sg);
         G.n_spmv++;
         { bool kt = kt_begin(1);
-          sd_spmv_dots(s, g, G.A, ph, v, rh, nullptr, 1, G.rb, S_RV, SOP_BI_AL, st);   // also writes ph
+          sd_spmv_dots(s, g, Ak, ph, v, rh, nullptr, 1, G.rb, S_RV, SOP_BI_AL, st);   // also writes ph
           kt_end(kt); }
         sd_bi_r1(s, r, v, N, G.rb, st);
         rd(S_AL, S_SS + 1);
@@ -884,7 +888,7 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
         precond(r, sh, sg, sg);
         G.n_spmv++;
         { bool kt = kt_begin(1);
-          sd_spmv_dots(s, g, G.A, sh, t, nullptr, r, 2, G.rb, S_TT, SOP_BI_OM, st);    // also writes sh
+          sd_spmv_dots(s, g, Ak, sh, t, nullptr, r, 2, G.rb, S_TT, SOP_BI_OM, st);    // also writes sh
           kt_end(kt); }
         sd_bi_xr2(s, x, r, ph, sh, t, G.bi_fuse ? rh : nullptr, N, G.rb);
         rd(S_OM, S_RR + 1);
@@ -1112,6 +1116,7 @@ static int float_fix_host(int hole, int m) {
 }
 
 static double CONT_ATOL = 2e-14;
+static int g_unit_diag = 1;            // 0: the continuity products read the stored unit diagonal
 static int solve_cont(int hole, double *dq, double lo, double hi, double *dmax) {
     const int N = G.N;
     hipStream_t s = G.st;
@@ -1122,7 +1127,9 @@ static int solve_cont(int hole, double *dq, double lo, double hi, double *dmax) 
     sd_fill(s, G.d_sol, 1.0, N);
     double atol = CONT_ATOL * G.cont_atol_f;
     int its;
+    G.unit_diag = g_unit_diag;            // k_cont writes every diagonal entry as 1
     double rn = bicgstab(G.d_rhs, G.d_sol, G.max_iter_c > 0 ? G.max_iter_c : 1000, G.bicg_rtol, atol, &its);
+    G.unit_diag = 0;
     if (G.verbose > 2) fprintf(stderr, "    bicgstab(%c) its=%d |r|=%.3e\n", hole ? 'p' : 'n', its, rn);
     int ok = std::isfinite(rn);
     if (ok) float_fix(hole);
@@ -1745,6 +1752,7 @@ void sdx_tune(int what, int value) {
     if (what == 17) G.mg_bf16 = value != 0;
     if (what == 18) g_line_scan1 = value != 0;
     if (what == 19) g_spmv_nt = value != 0;
+    if (what == 20) g_unit_diag = value != 0;
     if (what == 11) {
         const int m = value >= 0 && value <= 2 ? value : SD_TILE_MODE_DEFAULT;
         hipDeviceSynchronize();

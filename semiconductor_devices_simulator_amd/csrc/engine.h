@@ -108,6 +108,8 @@ struct Engine {
     int tail_n = 1024;                        // levels of at most this many nodes run in the tail kernel
     int use_graphs = 0;                       // replay the captured V-cycle (SEMISIM_GRAPHS=1)
     int bi_fuse = 1;                          // BiCGSTAB: rh.r computed with the x/r update
+    int unit_diag = 0;                        // the operator's diagonal is 1 (scaled continuity rows):
+                                              // the Krylov products skip reading it
     int stage_on = 1;                         // Krylov kernels stage the V-cycle's input/output
     double mg_alpha = 1.0;                    // coarse-correction scale above the tail
     int mg_bf16 = 1;                          // FP32 method: BF16 level-0 smoother coefficients

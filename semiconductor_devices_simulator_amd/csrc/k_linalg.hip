@@ -160,6 +160,7 @@ template void sd_spmv_T<float>(hipStream_t, const Geo &, Sten7<float>, const flo
 // level, type TS), copied to xw in FP64 natural order on the way
 // NT: the operator is read with non-temporal loads (streamed once per
 // product, so it need not displace the multigrid data from the caches)
+// A.c == nullptr: unit diagonal (not read)
 template <class TS, bool NT>
 __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, Sten7<double> A, const double *x,
                                                         double *y, const double *u0, const double *u1, int nv,
@@ -180,7 +181,7 @@ __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, 
             auto xp = [&](int ii) { return in.px ? (ii & 1) * h + (ii >> 1) : ii; };
             const int q = r0 + xp(i);
             xk = st_get<TS>(in, q);
-            double s = ld(A.c, k) * xk;
+            double s = A.c ? ld(A.c, k) * xk : xk;
             if (i > 0) s += ld(A.xm, k) * st_get<TS>(in, r0 + xp(i - 1));
             if (i < Nx - 1) s += ld(A.xp, k) * st_get<TS>(in, r0 + xp(i + 1));
             if (j > 0) s += ld(A.ym, k) * st_get<TS>(in, q - Nx);
@@ -190,7 +191,7 @@ __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, 
             return s;
         } else {
             xk = 0.0;
-            double s = ld(A.c, k) * x[k];
+            double s = A.c ? ld(A.c, k) * x[k] : x[k];
             if (i > 0) s += ld(A.xm, k) * x[k - 1];
             if (i < Nx - 1) s += ld(A.xp, k) * x[k + 1];
             if (j > 0) s += ld(A.ym, k) * x[k - Nx];

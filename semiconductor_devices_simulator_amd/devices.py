@@ -15,6 +15,9 @@ Workloads (sizes in nodes):
   pn_diode      16x16x64 Si abrupt p-n diode, 1x1x4 um, forward 0-0.8 V
   mosfet        planar n-MOSFET on an Nx x Ny x Nz tensor mesh (96x64x64,
                 192x128x128), Id-Vg 0-1.2 V at Vd 0.05 V and 1.0 V
+  avalanche_diode  the pn diode in reverse bias with impact ionisation
+  vdmos         55 V VDMOS-like cell (256x128x256), drain sweep 0-60 V at
+                Vgs 0 with impact ionisation, Id-Vds at Vgs 10 V
 """
 import math
 from dataclasses import dataclass, field
@@ -242,6 +245,55 @@ def mosfet(Nx=96, Ny=64, Nz=64, vds=(0.05, 1.0), npts=25, nsub=1e17, nsd=1e20, s
     sweep = [(2, 0.0, 1.2, npts, {1: vd}) for vd in vds]
     return DeviceSpec("mosfet_%dx%dx%d" % (Nx, Ny, Nz), L3, xs, ys, zs, mat, Nd, Na, cons,
                       models=7, sweep=sweep)
+
+
+def vdmos(Nx=256, Ny=128, Nz=256, vd_max=60.0, npts=61, vgs_on=10.0, npts_on=0, vd_on_max=20.0, tol_c=1e-9):
+    """55 V VDMOS-like cell, 20 x 4 x 8 um (x lateral, y depth with the top
+    at y = 0, z width), after the reference's VDMOS template (main.py:984-1059).
+    n- drift 5e15 cm^-3; n+ substrate 1e19 cm^-3 below y = 3.5 um (drain on the
+    bottom face); p-body Gaussian implants (peak 1e17 cm^-3, Lv 0.6 um) under
+    both outer thirds of the top face, leaving a JFET window in the middle;
+    n+ source (1e20 cm^-3, Lv 0.1 um) and p+ body tap (1e19 cm^-3, Lv 0.2 um)
+    under the source contacts at both outer edges; two gates (Robin, 50 nm
+    oxide, work function 4.1 eV) over the channels.  Models 7 | 256
+    (impact ionisation on).  Sweep: drain 0 -> vd_max in npts points at
+    Vgs = 0 (breakdown), then, when npts_on > 0, the drain back at 0 V and
+    Id-Vds 0 -> vd_on_max in npts_on points at Vgs = vgs_on.  tol_c: the
+    Gummel loop's relative current tolerance (api3_set_solver)."""
+    Lx, Ly, Lz = 20000.0, 4000.0, 8000.0
+    tap_w, src_w, body_x = 1500.0, 2500.0, 7000.0      # from each outer edge
+    sub_top = 3500.0
+    xs = graded_grid(Lx, Nx, [tap_w + src_w, body_x, Lx - body_x, Lx - tap_w - src_w])
+    ys = graded_grid(Ly, Ny, [0.0, 1000.0, sub_top])
+    zs = np.linspace(0, Lz * 1e-9, Nz)
+    N = Nx * Ny * Nz
+    mat = np.zeros(N, np.int32)
+    P = node_positions_nm(xs, ys, zs)
+    L3 = (Lx, Ly, Lz)
+    Nd = np.full(N, 5e15) + np.where(P[:, 1] >= sub_top, 1e19, 0.0)
+    Na = np.zeros(N)
+    for a0, a1 in ((0.0, body_x), (Lx - body_x, 1e9)):
+        Na += Implant("p", 1e17, "y0", a0, a1, 0.0, 1e9, "gauss", 600.0, 0.0, 0.8).conc(P, L3)
+    for a0, a1 in ((0.0, tap_w), (Lx - tap_w, 1e9)):
+        Na += Implant("p", 1e19, "y0", a0, a1, 0.0, 1e9, "gauss", 200.0, 0.0, 0.8).conc(P, L3)
+    for a0, a1 in ((tap_w, tap_w + src_w), (Lx - tap_w - src_w, Lx - tap_w)):
+        Nd += Implant("n", 1e20, "y0", a0, a1, 0.0, 1e9, "gauss", 100.0, 0.0, 0.8).conc(P, L3)
+    sf = (tap_w + src_w) / Lx
+    g0, g1 = (tap_w + src_w - 200.0) / Lx, (body_x + 500.0) / Lx
+    cons = [Contact(2, 0.0, sf, 0.0, 1.0, 0, 0.0, label="Source-L"),
+            Contact(2, 1.0 - sf, 1.0, 0.0, 1.0, 0, 0.0, label="Source-R"),
+            Contact(2, g0, g1, 0.0, 1.0, 2, 0.0, pm=4.1, tox=50.0, label="Gate-L"),
+            Contact(2, 1.0 - g1, 1.0 - g0, 0.0, 1.0, 2, 0.0, pm=4.1, tox=50.0, label="Gate-R"),
+            Contact(3, 0.0, 1.0, 0.0, 1.0, 0, 0.0, label="Drain")]
+    sweep = [(4, 0.0, vd_max, npts, {2: 0.0, 3: 0.0})]
+    if npts_on > 0:
+        # the drain returns to 0 V before the gates turn on (the gates move
+        # first within a resolve, and at full drain voltage an on-state
+        # device would be driven into avalanche)
+        sweep.append((4, 0.0, 0.0, 1, {2: 0.0, 3: 0.0}))
+        sweep.append((4, 0.0, vd_on_max, npts_on, {2: vgs_on, 3: vgs_on}))
+    return DeviceSpec("vdmos_%dx%dx%d" % (Nx, Ny, Nz), L3, xs, ys, zs, mat, Nd * 1e6, Na * 1e6, cons,
+                      models=7 | 256, sweep=sweep, solver=(2000, 1000, 500, 1e-10, tol_c, 1.0, 1.0))
 
 
 def random_mosfet(seed, Nx=96, Ny=64, Nz=64, npts=25):

@@ -246,13 +246,43 @@ def ii_fixture(lib, spec, name, Va, rng, perturb=1e-3):
     print("wrote", name, N, "nodes; total generation", gii.sum(), "1/s; nonzero nodes", int((gii > 0).sum()))
 
 
+def ivsweep_fixture(lib, spec, name, cid, Vs, Ve, npts, hold):
+    """The reference's C-side sweeps: api3_iv_sweep2 of contact cid (currents
+    and floors of every contact per point) from equilibrium, then, with cid
+    held at Ve, api3_iv_sweep (current density of the swept contact) of
+    contact 1 from 0.05 V to 1.0 V."""
+    lib.api3_set_threads(8)
+    cids = devices.load_device(lib, spec)
+    assert lib.api3_solve() == 1
+    for c, v in hold.items():
+        lib.api3_set_contact_V(cids[c], v)
+    nc = len(cids)
+    V, I, F, C = np.zeros(npts), np.zeros(npts * nc), np.zeros(npts * nc), np.zeros(npts, np.int32)
+    lib.api3_iv_sweep2(cids[cid], Vs, Ve, npts, core.dptr(V), core.dptr(I), core.dptr(F), core.iptr(C))
+    V1, J1, C1 = np.zeros(npts), np.zeros(npts), np.zeros(npts, np.int32)
+    lib.api3_set_contact_V(cids[cid], Ve)
+    lib.api3_iv_sweep(cids[1], 0.05, 1.0, npts, core.dptr(V1), core.dptr(J1), core.iptr(C1))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), V=V, I=I.reshape(npts, nc), F=F.reshape(npts, nc), C=C,
+                        V1=V1, J1=J1, C1=C1, A=np.array([lib.api3_get_contact_A(cids[c]) for c in range(nc)]))
+    print("wrote", name, "points", npts, "converged", int(C.sum()), int(C1.sum()))
+
+
 def main():
     lib = load_harness()
+    if "--ivsweep" in sys.argv:        # the C-side sweep entry points
+        ivsweep_fixture(lib, devices.mosfet(24, 16, 8, npts=7), "e2e_ivsweep_mosfet_24x16x8", 2, 0.0, 1.2, 7,
+                        {1: 0.05})
+        return
     if "--ii" in sys.argv:             # impact ionisation: kernel fixture
         ii_fixture(lib, devices.pn_diode(6, 5, 16), "k_ii_diode_6x5x16", [-8.0, 0.0], np.random.default_rng(4321))
         ii_fixture(lib, devices.mosfet(20, 14, 9), "k_ii_mosfet_20x14x9", [0.0, 3.0, 1.0, 0.0],
                    np.random.default_rng(4322))
         sweep_fixture(lib, devices.avalanche_diode(8, 8, 48), "e2e_ii_diode_8x8x48")
+        # tol_c 1e-11: the on-state points converge slowly (over 1000 Gummel
+        # iterations), and at the default 1e-9 the reference's own currents
+        # there move by 1e-7 relative when the tolerance is tightened
+        sweep_fixture(lib, devices.vdmos(40, 24, 4, vd_max=50.0, npts=11, npts_on=5, tol_c=1e-11),
+                      "e2e_vdmos_40x24x4")
         return
     if "--mosfet96" in sys.argv:       # the full 96x64x64 Id-Vg sweep (about 5 minutes on 8 cores)
         sweep_fixture(lib, devices.mosfet(96, 64, 64), "e2e_mosfet_96x64x64", fields=False, sample=4096)

@@ -130,3 +130,39 @@ def test_readback_quantities(hip):
     hip.api3_bulk_copy(1, dptr(n))
     hip.api3_bulk_copy(2, dptr(p))
     assert np.all(n > 0) and np.all(p > 0)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_c_side_sweeps_match_reference(hip, precision):
+    """api3_iv_sweep2 (all currents and floors per point) of the gate of the
+    24x16x8 MOSFET from equilibrium at Vd = 0.05 V, then api3_iv_sweep
+    (current density of the swept contact) of the drain from 0.05 to 1.0 V
+    at Vg = 1.2 V, against the reference core's same two calls."""
+    from semiconductor_devices_simulator_amd.core import iptr
+    print(f"\n[gpu e2e] C-side sweeps mosfet 24x16x8 precision {precision}", flush=True)
+    spec = devices.mosfet(24, 16, 8, npts=7)
+    g = load_golden("e2e_ivsweep_mosfet_24x16x8")
+    cids = devices.load_device(hip, spec)
+    assert hip.sdx_set_precision(precision) == 1
+    assert hip.api3_solve() == 1
+    hip.api3_set_contact_V(cids[1], 0.05)
+    n, nc = 7, len(cids)
+    V, I, F, C = np.zeros(n), np.zeros(n * nc), np.zeros(n * nc), np.zeros(n, np.int32)
+    hip.api3_iv_sweep2(cids[2], 0.0, 1.2, n, dptr(V), dptr(I), dptr(F), iptr(C))
+    I, F = I.reshape(n, nc), F.reshape(n, nc)
+    assert np.all(C == 1) and np.all(g["C"] == 1)
+    np.testing.assert_allclose(V, g["V"], rtol=0, atol=1e-15)
+    ok = np.abs(I - g["I"]) <= 1e-8 * np.abs(g["I"]) + g["F"] + F
+    assert ok.all(), (np.argwhere(~ok), I[~ok], g["I"][~ok])
+    V1, J1, C1 = np.zeros(n), np.zeros(n), np.zeros(n, np.int32)
+    hip.api3_set_contact_V(cids[2], 1.2)
+    hip.api3_iv_sweep(cids[1], 0.05, 1.0, n, dptr(V1), dptr(J1), iptr(C1))
+    assert np.all(C1 == 1) and np.all(g["C1"] == 1)
+    np.testing.assert_allclose(V1, g["V1"], rtol=0, atol=1e-15)
+    # J = I/A of the swept contact (drain), compared at 1e-8 relative plus
+    # the largest current floor of the drain in the first sweep, scaled by A
+    A = g["A"][1]
+    assert hip.api3_get_contact_A(cids[1]) == pytest.approx(A, rel=1e-14)
+    assert np.all(np.abs(g["J1"]) > 0)
+    Fl = (g["F"][:, 1].max() + F[:, 1].max()) / A
+    assert np.all(np.abs(J1 - g["J1"]) <= 1e-8 * np.abs(g["J1"]) + Fl), (J1, g["J1"])
