@@ -604,6 +604,7 @@ struct KTimer {
     long long count = 0, sampled = 0, vcycles = 0;
     double total_ms = 0.;
     std::vector<hipEvent_t> pool, pend;
+    std::vector<int> pend_n;           // launches inside each bracketed pair of pend
 };
 static KTimer KT;
 static hipEvent_t kt_event() {
@@ -618,11 +619,12 @@ static void kt_flush() {
         HC(hipEventSynchronize(KT.pend[q + 1]));
         HC(hipEventElapsedTime(&ms, KT.pend[q], KT.pend[q + 1]));
         KT.total_ms += ms;
-        KT.sampled++;
+        KT.sampled += KT.pend_n[q / 2];
         KT.pool.push_back(KT.pend[q]);
         KT.pool.push_back(KT.pend[q + 1]);
     }
     KT.pend.clear();
+    KT.pend_n.clear();
 }
 static bool kt_begin(int kind) {
     if (!KT.on || KT.kind != kind) return false;
@@ -633,11 +635,13 @@ static bool kt_begin(int kind) {
     KT.pend.push_back(e);
     return true;
 }
-static void kt_end(bool started) {
+// n: kernel launches between the two events
+static void kt_end(bool started, int n = 1) {
     if (!started) return;
     hipEvent_t e = kt_event();
     HC(hipEventRecord(e, G.st));
     KT.pend.push_back(e);
+    KT.pend_n.push_back(n);
     if (KT.pend.size() >= 512) kt_flush();
 }
 
@@ -648,18 +652,23 @@ static void kt_end(bool started) {
 template <class T> static int smooth_pre(const MGLevel<T> &v, const T *b, T *x, int L = 0) {
     hipStream_t s = G.st;
     if (G.n_axes == 0 || L >= G.line_levels) {
-        mg_rbgs(s, v, b, x, 0, 1);
-        mg_rbgs(s, v, b, x, 1, 0);
+        if (g_rb_fuse) mg_rbgs(s, v, b, x, 0, 2);        // both colours from x = 0, one launch
+        else {
+            mg_rbgs(s, v, b, x, 0, 1);
+            mg_rbgs(s, v, b, x, 1, 0);
+        }
         return 1;
     }
+    // the level-0 sweeps are timed as one bracket (back-to-back launches, as
+    // in an untimed V-cycle), the time divided over its launches
     int first = 1;
+    bool t = v.N == G.N && kt_begin(0);
     for (int a = 0; a < G.n_axes; a++)
         for (int color = 0; color < 2; color++) {
-            bool t = v.N == G.N && kt_begin(0);
             mg_line(s, v, b, x, G.mg_axes[a], color, first);
-            kt_end(t);
             first = 0;
         }
+    kt_end(t, 2 * G.n_axes);
     // rows of the last sweep's colour-1 lines have no residual
     return 2 + G.mg_axes[G.n_axes - 1];
 }
@@ -674,13 +683,13 @@ template <class T> static void smooth_post(const MGLevel<T> &v, const T *b, T *x
         mg_rbgs(s, v, b, x, 0, 0);
         return;
     }
+    bool t = v.N == G.N && kt_begin(0);
     for (int a = G.n_axes - 1; a >= 0; a--)
         for (int color = 1; color >= 0; color--) {
-            bool t = v.N == G.N && kt_begin(0);
             mg_line(s, v, b, x, G.mg_axes[a], color, 0, c);
-            kt_end(t);
             c = nullptr;
         }
+    kt_end(t, 2 * G.n_axes);
 }
 template <class T> static void vcycle_T(int L, const T *b, T *x) {
     Hier<T> &H = hier<T>();
@@ -1753,6 +1762,7 @@ void sdx_tune(int what, int value) {
     if (what == 18) g_line_scan1 = value != 0;
     if (what == 19) g_spmv_nt = value != 0;
     if (what == 20) g_unit_diag = value != 0;
+    if (what == 21) g_rb_fuse = value != 0;
     if (what == 11) {
         const int m = value >= 0 && value <= 2 ? value : SD_TILE_MODE_DEFAULT;
         hipDeviceSynchronize();

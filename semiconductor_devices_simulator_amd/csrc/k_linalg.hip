@@ -579,20 +579,50 @@ __device__ __forceinline__ void rb_node(const MGLevel<T> &v, const T *b, T *x, i
     if (!(d != (T)0)) d = (T)1;
     x[k] = s / d;
 }
+// Both half sweeps of a red-black sweep from x = 0 in one pass: the first
+// colour's nodes get b/d, and a node of the other colour reads its
+// neighbours' values as b/d directly, so every node is written once.  The
+// operations and their order are those of the two half sweeps, so x is the
+// same to the last bit.
+template <class T>
+__device__ __forceinline__ void rb_zero2_node(const MGLevel<T> &v, const T *b, T *x, int color, int k) {
+    const int Nx = v.Nx, Ny = v.Ny, Nz = v.Nz, NxNy = Nx * Ny;
+    int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
+    auto dg = [&](int m) -> T {
+        T d = v.A.c ? v.A.c[m] : (T)1;
+        return d != (T)0 ? d : (T)1;
+    };
+    if (((i + j + l) & 1) == color) { x[k] = b[k] / dg(k); return; }
+    T s = b[k];
+    if (i > 0) s -= v.A.xm[k] * (b[k - 1] / dg(k - 1));
+    if (i < Nx - 1) s -= v.A.xp[k] * (b[k + 1] / dg(k + 1));
+    if (j > 0) s -= v.A.ym[k] * (b[k - Nx] / dg(k - Nx));
+    if (j < Ny - 1) s -= v.A.yp[k] * (b[k + Nx] / dg(k + Nx));
+    if (l > 0) s -= v.A.zm[k] * (b[k - NxNy] / dg(k - NxNy));
+    if (l < Nz - 1) s -= v.A.zp[k] * (b[k + NxNy] / dg(k + NxNy));
+    x[k] = s / dg(k);
+}
+// zero: 1 the colour's half sweep from x = 0, 2 the whole sweep from x = 0
+// (`color` first)
 template <class T>
 __global__ void __launch_bounds__(SD_BLOCK) k_rbgs(MGLevel<T> v, const T *b, T *x, int color, int zero,
                                                  MGLevel<T> c, int pro) {
     int t = sd_tile() * SD_BLOCK + threadIdx.x;
+    if (zero == 2) {
+        if (t < v.N) rb_zero2_node(v, b, x, color, t);
+        return;
+    }
     if (zero) {
         if (t < v.N) rb_zero_node(v, b, x, color, t);
         return;
     }
     if (t < ((v.Nx + 1) >> 1) * v.Ny * v.Nz) rb_node(v, b, x, color, t, c, pro);
 }
+int g_rb_fuse = 1;            // 1: the pre-smoothing red-black sweep from x = 0 runs as one launch
 template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int color, int zero,
                                  const MGLevel<T> *c) {
     MGLevel<T> cc = c ? *c : v;
-    if (zero) k_rbgs<T><<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(v, b, x, color, 1, cc, 0);
+    if (zero) k_rbgs<T><<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(v, b, x, color, zero, cc, 0);
     else {
         int n = ((v.Nx + 1) >> 1) * v.Ny * v.Nz;
         k_rbgs<T><<<sd_nblocks(n), SD_BLOCK, 0, s>>>(v, b, x, color, 0, cc, c != nullptr);

@@ -111,6 +111,8 @@ void sd_bi_restart(hipStream_t s, double *rh, double *p, double *v, const double
 template <class T> void mg_weights_level(hipStream_t s, const MGLevel<T> &f, MGLevel<T> &c, int L,
                                          const DevView &d, int qsign, double VT);
 template <class T> void mg_galerkin(hipStream_t s, const MGLevel<T> &f, MGLevel<T> &c);
+// red-black Gauss-Seidel: zero 0 one half sweep (colour `color`), 1 that half
+// sweep from x = 0, 2 the whole sweep from x = 0 (`color` first) in one launch
 template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int color, int zero,
                                 const MGLevel<T> *c = nullptr);
 // alpha scales the coarse right-hand side, and so the coarse correction
@@ -138,6 +140,7 @@ extern int g_tail_lds;
 extern int g_line_scan1;        // axis-specialised line kernel for one line per slot
 extern int g_spmv_nt;            // non-temporal operator loads in the Krylov SpMV
 extern int g_restrict_act;      // active-lane restriction after an exact half sweep
+extern int g_rb_fuse;           // pre-smoothing red-black sweep from x = 0 in one launch
 void sd_set_tile_mode_linalg(int m);   // c_tile_mode of k_linalg.hip
 void sd_copy_bw(hipStream_t s, const double *a, double *b, long n);   // plain copy (bandwidth reference)
 // level-0 data in the level's storage order and precision: the right-hand

@@ -166,3 +166,27 @@ def test_c_side_sweeps_match_reference(hip, precision):
     assert np.all(np.abs(g["J1"]) > 0)
     Fl = (g["F"][:, 1].max() + F[:, 1].max()) / A
     assert np.all(np.abs(J1 - g["J1"]) <= 1e-8 * np.abs(g["J1"]) + Fl), (J1, g["J1"])
+
+
+def test_unit_diagonal_products_are_bitwise(hip):
+    """The continuity BiCGSTAB products skip the stored unit diagonal (tune
+    20); multiplying by 1 is exact, so a sweep's currents are bit-identical
+    with the diagonal read."""
+    print("\n[gpu e2e] unit-diagonal products, pn diode", flush=True)
+    spec = devices.pn_diode()
+    seg = [(0, 0.0, 0.3, 4, {})]
+    out = []
+    try:
+        for ud in (1, 0):
+            cids = devices.load_device(hip, spec)
+            hip.sdx_tune(20, ud)
+            assert hip.sdx_set_precision(1) == 1
+            assert hip.api3_solve() == 1
+            R = sweep.run_sweep(hip, spec, cids, segments=seg)
+            assert np.all(R["conv"])
+            out.append((R["I"], state(hip, spec.N)))
+    finally:
+        hip.sdx_tune(20, 1)
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1], out[1][1]):
+        np.testing.assert_array_equal(a, b)

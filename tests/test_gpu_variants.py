@@ -12,7 +12,7 @@ from semiconductor_devices_simulator_amd.core import dptr
 
 pytestmark = pytest.mark.gpu
 
-DEFAULTS = {12: 1, 11: -1}
+DEFAULTS = {12: 1, 11: -1, 21: 1}
 
 
 @pytest.fixture(scope="module")
@@ -52,3 +52,18 @@ def test_vcycle_variants(hip, solved, precision):
         z = _vcycle(hip, N, r, tunes)
         err = np.max(np.abs(z - base))
         assert err <= tol * scale, (tunes, err, scale)
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_fused_zero_start_sweep_is_bitwise(hip, solved, precision):
+    """The pre-smoothing red-black sweep from x = 0 as one launch (tune 21)
+    performs the two half sweeps' operations in their order: the V-cycle's
+    output is bit-identical."""
+    N = solved
+    hip.sdx_set_precision(precision)
+    assert hip.sdx_test_mg_setup() >= 4
+    r = np.random.default_rng(5).standard_normal(N)
+    fused = _vcycle(hip, N, r, {21: 1})
+    split = _vcycle(hip, N, r, {21: 0})
+    assert np.all(np.isfinite(fused)) and np.max(np.abs(fused)) > 0
+    np.testing.assert_array_equal(fused, split)
