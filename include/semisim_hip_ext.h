@@ -49,7 +49,7 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
                                               next rh.r with its x/r update (1) or separately (0),
                                               14: Krylov kernels stage the V-cycle's input and output in
                                               level-0 storage (1) or the V-cycle converts them (0),
-                                              16: coarse-correction scale above the tail in percent
+                                              16: coarse-correction scale above the tail in percent, for both operators
                                               (100 = none), 17: BF16 level-0 smoother coefficients in the
                                               FP32 method (1) or FP32 (0), 18: axis-specialised line
                                               kernel for one line per slot (1) or the generic one (0),

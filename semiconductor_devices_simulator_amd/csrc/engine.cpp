@@ -649,6 +649,11 @@ static void kt_end(bool started, int n = 1) {
 // couplings (G.mg_axes, chosen in prepare), or red-black point Gauss-Seidel
 // (G.n_axes == 0).  Pre-smoothing starts from x = 0; returns the restriction
 // skip code of the rows the last sweep solved exactly.
+// first smoothed axis (index into G.mg_axes) of line level L: coarse line
+// levels smooth at most G.coarse_axes axes, the strongest ones (last in order)
+static inline int line_axis0(int L) {
+    return L > 0 && G.coarse_axes < G.n_axes ? G.n_axes - G.coarse_axes : 0;
+}
 template <class T> static int smooth_pre(const MGLevel<T> &v, const T *b, T *x, int L = 0) {
     hipStream_t s = G.st;
     if (G.n_axes == 0 || L >= G.line_levels) {
@@ -663,12 +668,12 @@ template <class T> static int smooth_pre(const MGLevel<T> &v, const T *b, T *x, 
     // in an untimed V-cycle), the time divided over its launches
     int first = 1;
     bool t = v.N == G.N && kt_begin(0);
-    for (int a = 0; a < G.n_axes; a++)
+    for (int a = line_axis0(L); a < G.n_axes; a++)
         for (int color = 0; color < 2; color++) {
             mg_line(s, v, b, x, G.mg_axes[a], color, first);
             first = 0;
         }
-    kt_end(t, 2 * G.n_axes);
+    kt_end(t, 2 * (G.n_axes - line_axis0(L)));
     // rows of the last sweep's colour-1 lines have no residual
     return 2 + G.mg_axes[G.n_axes - 1];
 }
@@ -684,12 +689,12 @@ template <class T> static void smooth_post(const MGLevel<T> &v, const T *b, T *x
         return;
     }
     bool t = v.N == G.N && kt_begin(0);
-    for (int a = G.n_axes - 1; a >= 0; a--)
+    for (int a = G.n_axes - 1; a >= line_axis0(L); a--)
         for (int color = 1; color >= 0; color--) {
             mg_line(s, v, b, x, G.mg_axes[a], color, 0, c);
             c = nullptr;
         }
-    kt_end(t, 2 * G.n_axes);
+    kt_end(t, 2 * (G.n_axes - line_axis0(L)));
 }
 template <class T> static void vcycle_T(int L, const T *b, T *x) {
     Hier<T> &H = hier<T>();
@@ -700,7 +705,9 @@ template <class T> static void vcycle_T(int L, const T *b, T *x) {
     if (L == H.tail) { mg_tail(s, H.d_tail, &H.lev[L], nl - L, H.inv, G.tail_sweeps); return; }   // lev[L].b -> .x
     MGLevel<T> &c = H.lev[L + 1];
     int skip = smooth_pre(v, b, x, L);
-    mg_resid_restrict(s, v, c, b, x, skip, G.mg_alpha);   // coarse right-hand side
+    // coarse right-hand side; the coarse-correction scale may differ for the
+    // Poisson (CG) and continuity (BiCGSTAB) operators
+    mg_resid_restrict(s, v, c, b, x, skip, G.mg_qsign == 0 ? G.mg_alpha_p : G.mg_alpha);
     vcycle_T<T>(L + 1, c.b, c.x);
     smooth_post(v, b, x, &c, L);
 }
@@ -1631,6 +1638,9 @@ void eng_read_env() {
     if ((e = getenv("SEMISIM_MG_LINELEVELS"))) G.line_levels = atoi(e);
     if ((e = getenv("SEMISIM_MG_TAILN"))) G.tail_n = atoi(e);
     if ((e = getenv("SEMISIM_MG_TAILSW"))) { int q = atoi(e); G.tail_sweeps = q > 0 ? q : 1; }
+    if ((e = getenv("SEMISIM_MG_COARSEAXES"))) { int q = atoi(e); G.coarse_axes = q > 0 ? q : 1; }
+    if ((e = getenv("SEMISIM_MG_ALPHA_P"))) { double q = atof(e); G.mg_alpha_p = q > 0. ? q : 1.; }
+    if ((e = getenv("SEMISIM_MG_ALPHA"))) { double q = atof(e); G.mg_alpha = q > 0. ? q : 1.; }
     G.use_graphs = (e = getenv("SEMISIM_GRAPHS")) ? atoi(e) != 0 : 0;
     G.mg_perm = (e = getenv("SEMISIM_MG_PERM")) ? atoi(e) != 0 : 1;
     G.mg_bf16 = (e = getenv("SEMISIM_MG_BF16")) ? atoi(e) != 0 : G.mg_bf16;
@@ -1757,7 +1767,7 @@ void sdx_tune(int what, int value) {
     if (what == 12) g_restrict_act = value != 0;
     if (what == 13) G.bi_fuse = value != 0;
     if (what == 14) G.stage_on = value != 0;
-    if (what == 16) G.mg_alpha = value > 0 ? value / 100.0 : 1.0;
+    if (what == 16) G.mg_alpha = G.mg_alpha_p = value > 0 ? value / 100.0 : 1.0;   // (defaults 1.1 / 1.3)
     if (what == 17) G.mg_bf16 = value != 0;
     if (what == 18) g_line_scan1 = value != 0;
     if (what == 19) g_spmv_nt = value != 0;
