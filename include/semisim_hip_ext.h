@@ -57,7 +57,9 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
                                               use (Krylov SpMV, level-0 restriction) (1) or plain (0),
                                               20: continuity BiCGSTAB products skip the unit diagonal
                                               (1) or read it (0), 21: the pre-smoothing red-black sweep
-                                              from x = 0 as one launch (1) or two half sweeps (0) */
+                                              from x = 0 as one launch (1) or two half sweeps (0),
+                                              22: the post-smoothing red-black sweep of the coarse
+                                              point-GS levels as one launch (1) or two (0) */
 
 /* ---- learned warm start ----
    A 3-D convolutional U-Net on a G^3 grid resampled from the mesh (index

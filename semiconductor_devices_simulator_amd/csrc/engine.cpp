@@ -160,6 +160,7 @@ template <class T> static int hier_alloc(Hier<T> &H, bool level0_own) {
             v.rs = dalloc<double>(n);
             v.ph = dalloc<double>(n);
             v.x = dalloc<T>(n);
+            v.xs = dalloc<T>(n);
             v.b = dalloc<T>(n);
         }
         v.pw = dalloc<T>(n);
@@ -681,12 +682,18 @@ template <class T> static int smooth_pre(const MGLevel<T> &v, const T *b, T *x, 
 // V-cycle for CG).  With c != nullptr the first sweep applies the coarse
 // correction x += pw * x_parent to the neighbours it reads; the sweeps then
 // overwrite every node, so the correction is never stored.
-template <class T> static void smooth_post(const MGLevel<T> &v, const T *b, T *x, const MGLevel<T> *c, int L = 0) {
+// Returns where the smoothed x lies: x, or the level's second array xs when a
+// coarse point-GS level ran its sweep as one launch.
+template <class T> static T *smooth_post(const MGLevel<T> &v, const T *b, T *x, const MGLevel<T> *c, int L = 0) {
     hipStream_t s = G.st;
     if (G.n_axes == 0 || L >= G.line_levels) {
+        if (g_rb_fuse_post && c && v.xs && x != v.xs) {
+            mg_rbgs_post2(s, v, b, x, v.xs, *c);
+            return v.xs;
+        }
         mg_rbgs(s, v, b, x, 1, 0, c);
         mg_rbgs(s, v, b, x, 0, 0);
-        return;
+        return x;
     }
     bool t = v.N == G.N && kt_begin(0);
     for (int a = G.n_axes - 1; a >= line_axis0(L); a--)
@@ -695,21 +702,24 @@ template <class T> static void smooth_post(const MGLevel<T> &v, const T *b, T *x
             c = nullptr;
         }
     kt_end(t, 2 * (G.n_axes - line_axis0(L)));
+    return x;
 }
-template <class T> static void vcycle_T(int L, const T *b, T *x) {
+// V-cycle from level L; returns where level L's result lies (x, or the
+// level's xs after a one-launch post-smoothing sweep)
+template <class T> static T *vcycle_T(int L, const T *b, T *x) {
     Hier<T> &H = hier<T>();
     hipStream_t s = G.st;
     int nl = (int)H.lev.size();
     MGLevel<T> &v = H.lev[L];
-    if (L == nl - 1) { mg_coarse_solve(s, v, H.inv, b, x); return; }
-    if (L == H.tail) { mg_tail(s, H.d_tail, &H.lev[L], nl - L, H.inv, G.tail_sweeps); return; }   // lev[L].b -> .x
-    MGLevel<T> &c = H.lev[L + 1];
+    if (L == nl - 1) { mg_coarse_solve(s, v, H.inv, b, x); return x; }
+    if (L == H.tail) { mg_tail(s, H.d_tail, &H.lev[L], nl - L, H.inv, G.tail_sweeps); return x; }   // lev[L].b -> .x
+    MGLevel<T> c = H.lev[L + 1];
     int skip = smooth_pre(v, b, x, L);
     // coarse right-hand side; the coarse-correction scale may differ for the
     // Poisson (CG) and continuity (BiCGSTAB) operators
     mg_resid_restrict(s, v, c, b, x, skip, G.mg_qsign == 0 ? G.mg_alpha_p : G.mg_alpha);
-    vcycle_T<T>(L + 1, c.b, c.x);
-    smooth_post(v, b, x, &c, L);
+    c.x = vcycle_T<T>(L + 1, c.b, c.x);
+    return smooth_post(v, b, x, &c, L);
 }
 // smoother only (the reference's ILU-only retry): one symmetric smoothing from 0
 template <class T> static void smooth_only_T(const T *b, T *x) {
@@ -1773,6 +1783,7 @@ void sdx_tune(int what, int value) {
     if (what == 19) g_spmv_nt = value != 0;
     if (what == 20) g_unit_diag = value != 0;
     if (what == 21) g_rb_fuse = value != 0;
+    if (what == 22) g_rb_fuse_post = value != 0;
     if (what == 11) {
         const int m = value >= 0 && value <= 2 ? value : SD_TILE_MODE_DEFAULT;
         hipDeviceSynchronize();

@@ -619,6 +619,54 @@ __global__ void __launch_bounds__(SD_BLOCK) k_rbgs(MGLevel<T> v, const T *b, T *
     if (t < ((v.Nx + 1) >> 1) * v.Ny * v.Nz) rb_node(v, b, x, color, t, c, pro);
 }
 int g_rb_fuse = 1;            // 1: the pre-smoothing red-black sweep from x = 0 runs as one launch
+
+// Both half sweeps of the post-smoothing red-black sweep in one pass.  A
+// colour-1 node computes its new value as the first half sweep does (its
+// neighbours from x plus the coarse correction pw * x_parent); a colour-0
+// node recomputes the new values of its colour-1 neighbours the same way and
+// then its own from them.  The result goes to a second array, since the
+// colour-0 values of x are read by other nodes' recomputations.
+template <class T>
+__global__ void __launch_bounds__(SD_BLOCK) k_rbgs_post2(MGLevel<T> v, const T *b, const T *x, T *xo, MGLevel<T> c) {
+    const int k = sd_tile() * SD_BLOCK + threadIdx.x;
+    if (k >= v.N) return;
+    const int Nx = v.Nx, Ny = v.Ny, Nz = v.Nz, NxNy = Nx * Ny;
+    const int l = k / NxNy, r = k - l * NxNy, j = r / Nx, i = r - j * Nx;
+    auto dg = [&](int m) -> T {
+        T d = v.A.c ? v.A.c[m] : (T)1;
+        return d != (T)0 ? d : (T)1;
+    };
+    // x of a colour-0 node with the coarse correction
+    auto xcor = [&](int ii, int jj, int ll, int kk) -> T {
+        int I = v.cx ? ii >> 1 : ii, J = v.cy ? jj >> 1 : jj, L = v.cz ? ll >> 1 : ll;
+        return x[kk] + v.pw[kk] * c.x[I + c.Nx * (J + c.Ny * L)];
+    };
+    // new value of colour-1 node (ii, jj, ll) = kk
+    auto new1 = [&](int ii, int jj, int ll, int kk) -> T {
+        T s = b[kk];
+        if (ii > 0) s -= v.A.xm[kk] * xcor(ii - 1, jj, ll, kk - 1);
+        if (ii < Nx - 1) s -= v.A.xp[kk] * xcor(ii + 1, jj, ll, kk + 1);
+        if (jj > 0) s -= v.A.ym[kk] * xcor(ii, jj - 1, ll, kk - Nx);
+        if (jj < Ny - 1) s -= v.A.yp[kk] * xcor(ii, jj + 1, ll, kk + Nx);
+        if (ll > 0) s -= v.A.zm[kk] * xcor(ii, jj, ll - 1, kk - NxNy);
+        if (ll < Nz - 1) s -= v.A.zp[kk] * xcor(ii, jj, ll + 1, kk + NxNy);
+        return s / dg(kk);
+    };
+    if ((i + j + l) & 1) { xo[k] = new1(i, j, l, k); return; }
+    T s = b[k];
+    if (i > 0) s -= v.A.xm[k] * new1(i - 1, j, l, k - 1);
+    if (i < Nx - 1) s -= v.A.xp[k] * new1(i + 1, j, l, k + 1);
+    if (j > 0) s -= v.A.ym[k] * new1(i, j - 1, l, k - Nx);
+    if (j < Ny - 1) s -= v.A.yp[k] * new1(i, j + 1, l, k + Nx);
+    if (l > 0) s -= v.A.zm[k] * new1(i, j, l - 1, k - NxNy);
+    if (l < Nz - 1) s -= v.A.zp[k] * new1(i, j, l + 1, k + NxNy);
+    xo[k] = s / dg(k);
+}
+int g_rb_fuse_post = 0;       // 1: the post-smoothing red-black sweep of a coarse level runs as one launch (measured slower: off)
+template <class T> void mg_rbgs_post2(hipStream_t s, const MGLevel<T> &v, const T *b, const T *x, T *xo,
+                                      const MGLevel<T> &c) {
+    k_rbgs_post2<T><<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(v, b, x, xo, c);
+}
 template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int color, int zero,
                                  const MGLevel<T> *c) {
     MGLevel<T> cc = c ? *c : v;
@@ -1780,6 +1828,7 @@ template <class T> void mg_rs_in(hipStream_t s, const double *w, const MGLevel<T
     template void mg_weights_level<T>(hipStream_t, const MGLevel<T> &, MGLevel<T> &, int, const DevView &, int, double); \
     template void mg_galerkin<T>(hipStream_t, const MGLevel<T> &, MGLevel<T> &);                                  \
     template void mg_rbgs<T>(hipStream_t, const MGLevel<T> &, const T *, T *, int, int, const MGLevel<T> *);                         \
+    template void mg_rbgs_post2<T>(hipStream_t, const MGLevel<T> &, const T *, const T *, T *, const MGLevel<T> &);                 \
     template void mg_resid_restrict<T>(hipStream_t, const MGLevel<T> &, const MGLevel<T> &, const T *, const T *, int, double); \
     template int mg_line<T>(hipStream_t, const MGLevel<T> &, const T *, T *, int, int, int, const MGLevel<T> *); \
     template void mg_line_factor<T>(hipStream_t, const MGLevel<T> &, int); \

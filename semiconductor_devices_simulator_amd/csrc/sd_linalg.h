@@ -38,6 +38,8 @@ template <class T> struct MGLevel {
     T *pw;               // prolongation weight of each node to its parent
     double *ph;          // Slotboom reference potential / VT (coarse levels)
     T *x, *b;            // level vectors
+    T *xs;               // second x: the fused post-smoothing sweep of a point-GS level reads x and
+                         // writes here (coarse levels only)
     T *lu[3][3];         // line LU factors per axis: a/w, 1/w, c/w (nullptr: axis not factored)
     // BF16 copies of the off-diagonals (xm, xp, ym, yp, zm, zp) and of the
     // line factors, read by the line smoother instead of the T arrays when
@@ -115,6 +117,11 @@ template <class T> void mg_galerkin(hipStream_t s, const MGLevel<T> &f, MGLevel<
 // sweep from x = 0, 2 the whole sweep from x = 0 (`color` first) in one launch
 template <class T> void mg_rbgs(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int color, int zero,
                                 const MGLevel<T> *c = nullptr);
+// post-smoothing red-black sweep in one launch: colour 1 from its neighbours
+// in x with the coarse correction pw * c.x added, then colour 0 from the new
+// colour-1 values; reads x, writes xo (every node)
+template <class T> void mg_rbgs_post2(hipStream_t s, const MGLevel<T> &v, const T *b, const T *x, T *xo,
+                                      const MGLevel<T> &c);
 // alpha scales the coarse right-hand side, and so the coarse correction
 // (over-correction of the aggregation coarse space; 1 = plain)
 template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c,
@@ -141,6 +148,7 @@ extern int g_line_scan1;        // axis-specialised line kernel for one line per
 extern int g_spmv_nt;            // non-temporal operator loads in the Krylov SpMV
 extern int g_restrict_act;      // active-lane restriction after an exact half sweep
 extern int g_rb_fuse;           // pre-smoothing red-black sweep from x = 0 in one launch
+extern int g_rb_fuse_post;      // post-smoothing red-black sweep in one launch (coarse levels)
 void sd_set_tile_mode_linalg(int m);   // c_tile_mode of k_linalg.hip
 void sd_copy_bw(hipStream_t s, const double *a, double *b, long n);   // plain copy (bandwidth reference)
 // level-0 data in the level's storage order and precision: the right-hand
