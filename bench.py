@@ -8,7 +8,9 @@ api3_resolve continuing the sweep from the previous point, converged to the
 reference's tolerances (tol_p 1e-10 V, tol_c 1e-9 relative current), inputs
 already resident on the device.  W warmup points are solved untimed, then K
 points are timed (W + K <= 50).  --mesh 96 64 64 runs the 393,216-node mesh
-instead.
+instead.  --workload vdmos runs the 55 V VDMOS cell with impact ionisation
+(256x128x256, 8,388,608 nodes; drain 0 -> 60 V in 1 V steps at Vgs = 0) as a
+supplementary line: the headline metric is quoted on the MOSFET.
 
 Up to three solver configurations run back to back on the same sweep: the
 plain FP64 method (FP64 multigrid), the mixed-precision method (FP32
@@ -31,7 +33,7 @@ the reference sources) on the host cores, on a sample of the same sweep: at
 each sample position it resumes from the GPU's converged state and is timed
 on the points that follow.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--mesh NX NY NZ]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--mesh NX NY NZ] [--workload mosfet|vdmos]
 """
 import argparse
 import ctypes
@@ -48,7 +50,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SWEEP = "Id-Vg sweep (0-1.2 V x 0.05 V at Vd 0.05 and 1.0 V)"
+SWEEP = {"mosfet": "Id-Vg sweep (0-1.2 V x 0.05 V at Vd 0.05 and 1.0 V)",
+         "vdmos": "drain sweep (0-60 V x 1 V at Vgs 0, impact ionisation)"}
+DEFAULT_MESH = {"mosfet": [192, 128, 128], "vdmos": [256, 128, 256]}
+DRAIN = {"mosfet": 1, "vdmos": 4}          # contact index of the drain
 
 
 def sweep_points(spec):
@@ -86,8 +91,10 @@ def run_gpu(lib, spec, precision, warmup, steps, ktimer, keep=(), ws=0):
     cids = devices.load_device(lib, spec)
     assert lib.sdx_set_precision(precision) == 1
     lib.sdx_ws_mode(0)
+    t0 = time.perf_counter()
     if not lib.api3_solve():
         raise RuntimeError("equilibrium solve failed")
+    print(f"[bench]   precision {precision}: equilibrium {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     lib.sdx_ws_mode(ws)
     lib.sdx_ws_reset_stats()
     pts = sweep_points(spec)
@@ -100,6 +107,7 @@ def run_gpu(lib, spec, precision, warmup, steps, ktimer, keep=(), ws=0):
         t0 = time.perf_counter()
         I.append(solve_points(lib, cids, pts, i, i + 1, F)[0])
         dt += time.perf_counter() - t0
+        print(f"[bench]   point {i} ({pts[i][1]:.3g} V) {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
         if i in keep:
             st = tuple(np.zeros(spec.N) for _ in range(3))
             lib.api3_get_state(dptr(st[0]), dptr(st[1]), dptr(st[2]))
@@ -113,27 +121,28 @@ def run_gpu(lib, spec, precision, warmup, steps, ktimer, keep=(), ws=0):
     return dt, np.array(I), np.array(F), kt, its, states
 
 
-def parity(I, warmup, steps, mesh):
+def parity(I, warmup, steps, mesh, workload="mosfet"):
     """Currents of the timed points against the reference core's sweep."""
-    path = os.path.join(ROOT, "tests", "golden", "e2e_mosfet_%dx%dx%d.npz" % tuple(mesh))
+    path = os.path.join(ROOT, "tests", "golden", "e2e_%s_%dx%dx%d.npz" % (workload, *mesh))
     if not os.path.isfile(path):
         return None
     g = np.load(path)
     Ir, Fr = g["I"][warmup:warmup + steps], g["F"][warmup:warmup + steps]
     d = np.abs(I - Ir)
-    rel = d[:, 1] / np.abs(Ir[:, 1])
+    dr = DRAIN[workload]
+    rel = d[:, dr] / np.abs(Ir[:, dr])
     ok = d <= 1e-8 * np.abs(Ir) + 2 * Fr
     return dict(against="reference C core (FP64)", max_rel_drain_current_err=float(rel.max()),
                 points_within_tolerance=int(ok.all(axis=1).sum()), points=int(len(I)),
                 criterion="|dI| <= 1e-8|I_ref| + 2 floor_ref, every terminal")
 
 
-def mixed_vs_fp64(I32, I64, F64, st32, st64):
+def mixed_vs_fp64(I32, I64, F64, st32, st64, drain=1):
     """Mixed-precision method against the plain FP64 GPU method: currents of
     every timed point and the fields of the last one."""
     d = np.abs(I32 - I64)
     ok = d <= 1e-8 * np.abs(I64) + 2 * F64
-    rel = d[:, 1] / np.abs(I64[:, 1])
+    rel = d[:, drain] / np.maximum(np.abs(I64[:, drain]), 1e-300)
     out = dict(against="plain FP64 GPU method", max_rel_drain_current_err=float(rel.max()),
                points_within_tolerance=int(ok.all(axis=1).sum()), points=int(len(I32)),
                criterion="|dI| <= 1e-8|I_fp64| + 2 floor_fp64, every terminal")
@@ -202,11 +211,11 @@ def cpu_baseline_worker(path):
     print(json.dumps(dict(points_per_s=n / dt, seconds=dt, points=n)))
 
 
-def pmc_traffic(mesh):
+def pmc_traffic(mesh, workload="mosfet"):
     """HBM bytes per launch of the level-0 sweep from the committed counter
     summary of this mesh (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE,
     calibrated against a copy kernel), or (None, None) when there is none."""
-    name = "k_line_scan_level0_mosfet_%dx%dx%d_pmc.md" % tuple(mesh)
+    name = "k_line_scan_level0_%s_%dx%dx%d_pmc.md" % (workload, *mesh)
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.isfile(path):
         return None, None
@@ -221,7 +230,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=48)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mesh", type=int, nargs=3, default=[192, 128, 128])
+    ap.add_argument("--mesh", type=int, nargs=3, default=None)
+    ap.add_argument("--workload", choices=["mosfet", "vdmos"], default="mosfet")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-baseline-worker", default="")
@@ -235,15 +245,18 @@ def main():
         return 2
     from semiconductor_devices_simulator_amd import core, devices, warmstart
     lib = core.load_core()
-    spec = devices.mosfet(*a.mesh)
+    a.mesh = a.mesh or DEFAULT_MESH[a.workload]
+    spec = devices.mosfet(*a.mesh) if a.workload == "mosfet" else devices.vdmos(*a.mesh)
+    if a.workload != "mosfet":          # the trained network and the CPU sample are MOSFET-only
+        a.no_learned = a.no_cpu_baseline = True
     npts = spec.bias_points()
     W = max(0, min(a.warmup, npts - 1))
     K = max(1, min(a.steps, npts - W))
-    workload = "mosfet_%dx%dx%d %s" % (*a.mesh, SWEEP)
+    workload = "%s_%dx%dx%d %s" % (a.workload, *a.mesh, SWEEP[a.workload])
     buf = ctypes.create_string_buffer(256)
     lib.sdx_device_name(buf, 256)
     print(f"[bench] device {buf.value.decode()}; {workload}; warmup {W} timed {K}", file=sys.stderr, flush=True)
-    windows = CPU_WINDOWS.get(a.mesh[0], CPU_WINDOWS[96])
+    windows = CPU_WINDOWS.get(a.mesh[0], CPU_WINDOWS[96]) if a.workload == "mosfet" else ()
     timed = CPU_TIMED.get(a.mesh[0], CPU_TIMED[96])
     last = W + K - 1
     keep = [i for i in windows if W <= i and i + 2 + timed <= npts] + [last]
@@ -262,7 +275,7 @@ def main():
         lib.sdx_nn_free()
         print(f"[bench] learned warm start + FP32 multigrid: {K / tL:.3f} points/s", file=sys.stderr, flush=True)
     roof = None
-    traffic, traffic_src = pmc_traffic(a.mesh)
+    traffic, traffic_src = pmc_traffic(a.mesh, a.workload)
     if kt[0] > 0:
         avg_s = kt[1] / kt[0] * 1e-3
         ach = kt[2] / avg_s / 1e9
@@ -297,11 +310,11 @@ def main():
         "speedup_vs_plain_fp64": round(t64 / t_val, 3),
         "iterations": {"fp32_mg": its32, "fp64": its64, "learned_fp32_mg": learned["iterations"] if learned else None},
         "warm_start": ({k: learned[k] for k in ("warm_start", "model")} if learned else None),
-        "parity": {"fp32_mg": parity(I32, W, K, a.mesh), "fp64": parity(I64, W, K, a.mesh),
-                   "learned_fp32_mg": parity(learned["I"], W, K, a.mesh) if learned else None,
-                   "fp32_mg_vs_fp64": mixed_vs_fp64(I32, I64, F64, st32.get(last), states.get(last)),
-                   "learned_fp32_mg_vs_fp64": (mixed_vs_fp64(learned["I"], I64, F64, learned["last"], states.get(last))
-                                               if learned else None)},
+        "parity": {"fp32_mg": parity(I32, W, K, a.mesh, a.workload), "fp64": parity(I64, W, K, a.mesh, a.workload),
+                   "learned_fp32_mg": parity(learned["I"], W, K, a.mesh, a.workload) if learned else None,
+                   "fp32_mg_vs_fp64": mixed_vs_fp64(I32, I64, F64, st32.get(last), states.get(last), DRAIN[a.workload]),
+                   "learned_fp32_mg_vs_fp64": (mixed_vs_fp64(learned["I"], I64, F64, learned["last"], states.get(last),
+                                                             DRAIN[a.workload]) if learned else None)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -59,7 +59,10 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
                                               (1) or read it (0), 21: the pre-smoothing red-black sweep
                                               from x = 0 as one launch (1) or two half sweeps (0),
                                               22: the post-smoothing red-black sweep of the coarse
-                                              point-GS levels as one launch (1) or two (0) */
+                                              point-GS levels as one launch (1) or two (0),
+                                              23: Krylov updates read the V-cycle results from the
+                                              level-0 arrays they were left in (1) or from FP64 copies
+                                              written by the products (0) */
 
 /* ---- learned warm start ----
    A 3-D convolutional U-Net on a G^3 grid resampled from the mesh (index

@@ -75,7 +75,7 @@ static void vgraph_clear() {
 
 template <class T> static void hier_free(Hier<T> &H) {
     H.lev.clear(); H.inv = nullptr; H.alloc = false;
-    H.A0 = Sten7<T>{}; H.rs0 = nullptr; H.b0 = H.x0 = nullptr; H.c16 = nullptr;
+    H.A0 = Sten7<T>{}; H.rs0 = nullptr; H.b0 = H.x0 = H.x0b = nullptr; H.c16 = nullptr;
 }
 // Storage order of each level: the line-smoothed levels above the coarse tail
 // are stored parity-split along x (MGLevel::px) when their lines fit the scan
@@ -107,7 +107,7 @@ void eng_free() {
     G.mrec_cap = 0;
     hier_free(G.H64);
     hier_free(G.H32);
-    G.d_b32 = G.d_x32 = nullptr;
+    G.d_b32 = G.d_x32 = G.d_x32b = nullptr;
     G.d_ws[0] = G.d_ws[1] = G.d_ws[2] = nullptr;
     G.ws_have = 0;
     G.allocated = 0;
@@ -173,7 +173,7 @@ template <class T> static int hier_alloc(Hier<T> &H, bool level0_own) {
         else {
             H.A0.c = dalloc<T>(n); H.A0.xm = dalloc<T>(n); H.A0.xp = dalloc<T>(n); H.A0.ym = dalloc<T>(n);
             H.A0.yp = dalloc<T>(n); H.A0.zm = dalloc<T>(n); H.A0.zp = dalloc<T>(n);
-            H.b0 = dalloc<T>(n); H.x0 = dalloc<T>(n);
+            H.b0 = dalloc<T>(n); H.x0 = dalloc<T>(n); H.x0b = dalloc<T>(n);
         }
         H.rs0 = dalloc<double>(n);
     }
@@ -254,6 +254,7 @@ static int ensure_h32() {
     if (!hier_alloc(G.H32, true)) return 0;
     G.d_b32 = dalloc<float>(G.N);
     G.d_x32 = dalloc<float>(G.N);
+    G.d_x32b = dalloc<float>(G.N);
     return !g_err;
 }
 
@@ -731,33 +732,36 @@ template <class T> static void smooth_only_T(const T *b, T *x) {
 // Krylov vectors directly, or when V-cycles replay as graphs, which convert
 // their input and output themselves)
 static Stage0 stage0() {
-    Stage0 st{nullptr, nullptr, 0, 0, 0};
+    Stage0 st{nullptr, nullptr, 0, 0, 0, nullptr};
     if (!G.stage_on || G.use_graphs) return st;
     if (G.precision == 1 && !G.H32.lev.empty()) {
         const MGLevel<float> &v0 = G.H32.lev[0];
-        st = Stage0{G.d_b32, G.d_x32, v0.Nx, v0.px, 1};
+        st = Stage0{G.d_b32, G.d_x32, v0.Nx, v0.px, 1, G.d_x32b};
     } else if (G.precision == 0 && !G.H64.lev.empty() && G.H64.lev[0].px) {
         const MGLevel<double> &v0 = G.H64.lev[0];
-        st = Stage0{G.H64.b0, G.H64.x0, v0.Nx, v0.px, 0};
+        st = Stage0{G.H64.b0, G.H64.x0, v0.Nx, v0.px, 0, G.H64.x0b};
     }
     return st;
 }
 // z = M^-1 r.  sin: r is already staged in the level-0 b (r unused); sout:
 // the result stays in the level-0 x (z unused), for a consumer that reads it
 // from there.  Both need stage0() on.
-static void precond_launch(const double *r, double *z, bool sin = false, bool sout = false) {
+// second: the result goes to the second level-0 solution array (Stage0::x2)
+static void precond_launch(const double *r, double *z, bool sin = false, bool sout = false, bool second = false) {
     if (G.precision == 1) {
         const MGLevel<float> &v0 = G.H32.lev[0];
+        float *x0 = second ? G.d_x32b : G.d_x32;
         if (!sin) mg_vec_in(G.st, r, v0, G.d_b32);
-        if (G.mg_off || G.H32.lev.size() < 2) smooth_only_T<float>(G.d_b32, G.d_x32);
-        else vcycle_T<float>(0, G.d_b32, G.d_x32);
-        if (!sout) mg_vec_out(G.st, G.d_x32, v0, z);
+        if (G.mg_off || G.H32.lev.size() < 2) smooth_only_T<float>(G.d_b32, x0);
+        else vcycle_T<float>(0, G.d_b32, x0);
+        if (!sout) mg_vec_out(G.st, x0, v0, z);
     } else if (G.H64.lev[0].px) {
         const MGLevel<double> &v0 = G.H64.lev[0];
+        double *x0 = second ? G.H64.x0b : G.H64.x0;
         if (!sin) mg_vec_in(G.st, r, v0, G.H64.b0);
-        if (G.mg_off || G.H64.lev.size() < 2) smooth_only_T<double>(G.H64.b0, G.H64.x0);
-        else vcycle_T<double>(0, G.H64.b0, G.H64.x0);
-        if (!sout) mg_vec_out(G.st, G.H64.x0, v0, z);
+        if (G.mg_off || G.H64.lev.size() < 2) smooth_only_T<double>(G.H64.b0, x0);
+        else vcycle_T<double>(0, G.H64.b0, x0);
+        if (!sout) mg_vec_out(G.st, x0, v0, z);
     } else {
         if (G.mg_off || G.H64.lev.size() < 2) smooth_only_T<double>(r, z);
         else vcycle_T<double>(0, r, z);
@@ -769,13 +773,13 @@ static void precond_launch(const double *r, double *z, bool sin = false, bool so
 // no longer paces the GPU kernel by kernel.  The graph holds kernels only (no
 // memset or copy nodes).  With the level-0 smoother timer on, every n-th
 // V-cycle runs as plain launches bracketed by HIP events.
-static void precond(const double *r, double *z, bool sin = false, bool sout = false) {
+static void precond(const double *r, double *z, bool sin = false, bool sout = false, bool second = false) {
     G.n_vcycles++;
     bool sample = KT.on && KT.kind == 0 && (KT.vcycles++ % KT.every) == 0;
     if (!G.use_graphs || sample || sin || sout) {
         KT.sampling = sample;
         const bool whole = kt_begin(2);          // kind 2: the whole preconditioner application
-        precond_launch(r, z, sin, sout);
+        precond_launch(r, z, sin, sout, second);
         kt_end(whole);
         KT.sampling = false;
         return;
@@ -828,6 +832,7 @@ static int pcg(const double *b, double *x, int maxit, double rtol) {
     sd_copy(s, p, z, N);
     const Stage0 st = stage0();
     const bool sg = st.b != nullptr;
+    const bool sz = sg && G.stage_read;       // z is read where the V-cycle left it
     int it;
     for (it = 1; it <= maxit; it++) {
         G.n_spmv++;
@@ -836,7 +841,7 @@ static int pcg(const double *b, double *x, int maxit, double rtol) {
           kt_end(kt); }
         sd_pcg_update_xr(s, x, r, p, q, N, G.rb, st);      // r, staged for the V-cycle
         precond(r, z, sg, sg);
-        if (sg) sd_dot_stage(s, r, z, st, N, G.rb, S_TMP0, SOP_PCG_BE);   // z from the staged result
+        if (sg) sd_dot_stage(s, r, sz ? nullptr : z, st, N, G.rb, S_TMP0, SOP_PCG_BE);   // z from the staged result
         else sd_dot(s, r, z, N, G.rb, S_TMP0, SOP_PCG_BE);
         rd(S_RZ, S_TMP0 + 1);
         double pq = S(S_PQ), rzn = S(S_TMP0);
@@ -845,7 +850,8 @@ static int pcg(const double *b, double *x, int maxit, double rtol) {
             break;
         }
         if (!(rzn > rtol * rtol * rz0)) break;
-        sd_pcg_update_p(s, p, z, N, G.rb);
+        if (sz) sd_pcg_update_p_st(s, p, st, N, G.rb);
+        else sd_pcg_update_p(s, p, z, N, G.rb);
     }
     if (G.verbose > 3) fprintf(stderr, "      pcg its=%d rz0=%.3e\n", it, rz0);
     G.pcg_its += it;
@@ -880,6 +886,11 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
     // the SpMV after each reads the V-cycle's result where it lies
     const Stage0 st = stage0();
     const bool sg = st.b != nullptr;
+    // ph and sh stay where their V-cycles left them (two level-0 arrays): the
+    // x update reads them there, and the products write no FP64 copies
+    const bool sr = sg && G.stage_read;
+    Stage0 st2 = st;
+    st2.x = st.x2;
     for (it = 1; it <= maxit; it++) {
         // rho1 = rh.r and beta: here on the first iteration, later by the
         // previous iteration's x/r update
@@ -889,7 +900,7 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
         precond(p, ph, sg, sg);
         G.n_spmv++;
         { bool kt = kt_begin(1);
-          sd_spmv_dots(s, g, Ak, ph, v, rh, nullptr, 1, G.rb, S_RV, SOP_BI_AL, st);   // also writes ph
+          sd_spmv_dots(s, g, Ak, sr ? nullptr : ph, v, rh, nullptr, 1, G.rb, S_RV, SOP_BI_AL, st);   // (also writes ph)
           kt_end(kt); }
         sd_bi_r1(s, r, v, N, G.rb, st);
         rd(S_AL, S_SS + 1);
@@ -907,16 +918,18 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
         restarted = 0;
         if (fabs(rv) < 1e-300) break;
         if (sqrt(ss) <= tgt) {
-            sd_bi_x1(s, x, ph, N, G.rb);
+            if (sr) sd_bi_x1_st(s, x, st, N, G.rb);
+            else sd_bi_x1(s, x, ph, N, G.rb);
             rr = ss;
             break;
         }
-        precond(r, sh, sg, sg);
+        precond(r, sh, sg, sg, sr);
         G.n_spmv++;
         { bool kt = kt_begin(1);
-          sd_spmv_dots(s, g, Ak, sh, t, nullptr, r, 2, G.rb, S_TT, SOP_BI_OM, st);    // also writes sh
+          sd_spmv_dots(s, g, Ak, sr ? nullptr : sh, t, nullptr, r, 2, G.rb, S_TT, SOP_BI_OM, sr ? st2 : st);
           kt_end(kt); }
-        sd_bi_xr2(s, x, r, ph, sh, t, G.bi_fuse ? rh : nullptr, N, G.rb);
+        if (sr) sd_bi_xr2_st(s, x, r, st, t, G.bi_fuse ? rh : nullptr, N, G.rb);
+        else sd_bi_xr2(s, x, r, ph, sh, t, G.bi_fuse ? rh : nullptr, N, G.rb);
         rd(S_OM, S_RR + 1);
         double om = S(S_OM);
         rr = S(S_RR);
@@ -1784,6 +1797,7 @@ void sdx_tune(int what, int value) {
     if (what == 20) g_unit_diag = value != 0;
     if (what == 21) g_rb_fuse = value != 0;
     if (what == 22) g_rb_fuse_post = value != 0;
+    if (what == 23) G.stage_read = value != 0;
     if (what == 11) {
         const int m = value >= 0 && value <= 2 ? value : SD_TILE_MODE_DEFAULT;
         hipDeviceSynchronize();

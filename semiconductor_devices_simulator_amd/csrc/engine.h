@@ -29,7 +29,7 @@ template <class T> struct Hier {
     // vectors), used when level 0 is stored parity-split or in FP32
     Sten7<T> A0{};
     double *rs0 = nullptr;
-    T *b0 = nullptr, *x0 = nullptr;
+    T *b0 = nullptr, *x0 = nullptr, *x0b = nullptr;
     // BF16 smoother coefficients of level 0 (FP32 hierarchy; allocated on
     // first use): 6 off-diagonals, then 3 factors for each of the 3 axes
     __bf16 *c16 = nullptr;
@@ -99,6 +99,7 @@ struct Engine {
     Hier<double> H64;
     Hier<float> H32;
     float *d_b32 = nullptr, *d_x32 = nullptr;   // level-0 FP32 vectors
+    float *d_x32b = nullptr;                    // second level-0 FP32 solution (BiCGSTAB sh)
     const double *mg_w = nullptr;
     int mg_qsign = 0;
     int mg_off = 0;
@@ -112,6 +113,7 @@ struct Engine {
     int unit_diag = 0;                        // the operator's diagonal is 1 (scaled continuity rows):
                                               // the Krylov products skip reading it
     int stage_on = 1;                         // Krylov kernels stage the V-cycle's input/output
+    int stage_read = 1;                       // Krylov updates read the V-cycle results where they lie
     // coarse-correction scale above the tail: over-correction of the
     // piecewise-constant aggregation coarse space, continuity (BiCGSTAB) and
     // Poisson (CG) operators (measured on the 3.1M-node MOSFET sweep: 7%

@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(SD_BLOCK) k_dot_stage(const double *r, double 
     double acc[1] = {0.0};
     for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < n; k += gridDim.x * SD_BLOCK) {
         const double zk = st_get<TS>(st, st_pos(st, k));
-        z[k] = zk;
+        if (z) z[k] = zk;
         acc[0] += r[k] * zk;
     }
     red_partial<1, false>(acc, rb);
@@ -216,12 +216,12 @@ __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, 
         double ub = 0.0, vb = 0.0;
         if (hb) { ub = u0 ? u0[kb] : sb; vb = nv > 1 ? u1[kb] : 0.0; }
         y[k] = sa;
-        if constexpr (!std::is_void_v<TS>) xw[k] = xa;
+        if constexpr (!std::is_void_v<TS>) { if (xw) xw[k] = xa; }
         acc[0] += sa * ua;
         if (nv > 1) acc[1] += sa * va;
         if (hb) {
             y[kb] = sb;
-            if constexpr (!std::is_void_v<TS>) xw[kb] = xb;
+            if constexpr (!std::is_void_v<TS>) { if (xw) xw[kb] = xb; }
             acc[0] += sb * ub;
             if (nv > 1) acc[1] += sb * vb;
         }
@@ -343,6 +343,17 @@ __global__ void k_pcg_p(double *p, const double *z, int n, const double *sc) {
     if (k >= n) return;
     p[k] = z[k] + sc[S_BE] * p[k];
 }
+// z read where the V-cycle left it (the staged level-0 solution)
+template <class TS>
+__global__ void k_pcg_p_st(double *p, Stage0 st, int n, const double *sc) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= n) return;
+    p[k] = st_get<TS>(st, st_pos(st, k)) + sc[S_BE] * p[k];
+}
+void sd_pcg_update_p_st(hipStream_t s, double *p, Stage0 st, int n, RedBuf rb) {
+    if (st.f32) k_pcg_p_st<float><<<sd_nblocks(n), SD_BLOCK, 0, s>>>(p, st, n, rb.scal);
+    else k_pcg_p_st<double><<<sd_nblocks(n), SD_BLOCK, 0, s>>>(p, st, n, rb.scal);
+}
 void sd_pcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q, int n, RedBuf rb,
                       Stage0 st) {
     k_pcg_xr<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(x, r, p, q, n, rb.scal, st);
@@ -373,6 +384,31 @@ __global__ void __launch_bounds__(SD_BLOCK) k_bi_r1(double *r, const double *v, 
 __global__ void k_bi_x1(double *x, const double *ph, int n, const double *sc) {
     int k = blockIdx.x * SD_BLOCK + threadIdx.x;
     if (k < n) x[k] += sc[S_AL] * ph[k];
+}
+template <class TS>
+__global__ void k_bi_x1_st(double *x, Stage0 st, int n, const double *sc) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k < n) x[k] += sc[S_AL] * st_get<TS>(st, st_pos(st, k));
+}
+// x += al ph + om sh, r -= om t with ph and sh read where the two V-cycles
+// left them (staged level-0 solutions st.x and st.x2): the same values as
+// their FP64 copies, so the same iterates
+template <int NV, class TS>
+__global__ void __launch_bounds__(SD_BLOCK) k_bi_xr2_st(double *x, double *r, Stage0 st, const double *t,
+                                                        const double *rh, int n, RedBuf rb) {
+    double al = rb.scal[S_AL], om = rb.scal[S_OM];
+    double acc[NV] = {};
+    Stage0 s2 = st;
+    s2.x = st.x2;
+    for (int k = blockIdx.x * SD_BLOCK + threadIdx.x; k < n; k += gridDim.x * SD_BLOCK) {
+        const int q = st_pos(st, k);
+        x[k] += al * st_get<TS>(st, q) + om * st_get<TS>(s2, q);
+        double y = r[k] - om * t[k];
+        r[k] = y;
+        acc[0] += y * y;
+        if constexpr (NV == 2) acc[1] += rh[k] * y;
+    }
+    red_partial<NV, false>(acc, rb);
 }
 // x += al ph + om sh, r -= om t; r.r and (with rh) the next iteration's rh.r
 template <int NV>
@@ -413,6 +449,23 @@ void sd_bi_r1(hipStream_t s, double *r, const double *v, int n, RedBuf rb, Stage
 }
 void sd_bi_x1(hipStream_t s, double *x, const double *ph, int n, RedBuf rb) {
     k_bi_x1<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(x, ph, n, rb.scal);
+}
+void sd_bi_x1_st(hipStream_t s, double *x, Stage0 st, int n, RedBuf rb) {
+    if (st.f32) k_bi_x1_st<float><<<sd_nblocks(n), SD_BLOCK, 0, s>>>(x, st, n, rb.scal);
+    else k_bi_x1_st<double><<<sd_nblocks(n), SD_BLOCK, 0, s>>>(x, st, n, rb.scal);
+}
+void sd_bi_xr2_st(hipStream_t s, double *x, double *r, Stage0 st, const double *t, const double *rh, int n,
+                  RedBuf rb) {
+    int nb = red_grid(n);
+    if (rh) {
+        if (st.f32) k_bi_xr2_st<2, float><<<nb, SD_BLOCK, 0, s>>>(x, r, st, t, rh, n, rb);
+        else k_bi_xr2_st<2, double><<<nb, SD_BLOCK, 0, s>>>(x, r, st, t, rh, n, rb);
+        red_final(s, rb, 2, nb, S_TMP2, 0, SOP_BI_NEXT_BE);
+    } else {
+        if (st.f32) k_bi_xr2_st<1, float><<<nb, SD_BLOCK, 0, s>>>(x, r, st, t, rh, n, rb);
+        else k_bi_xr2_st<1, double><<<nb, SD_BLOCK, 0, s>>>(x, r, st, t, rh, n, rb);
+        red_final(s, rb, 1, nb, S_RR, 0, SOP_BI_NEXT);
+    }
 }
 void sd_bi_xr2(hipStream_t s, double *x, double *r, const double *ph, const double *sh, const double *t,
                const double *rh, int n, RedBuf rb) {

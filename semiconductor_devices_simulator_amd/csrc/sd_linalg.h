@@ -75,16 +75,18 @@ struct Stage0 {
     void *b;            // level-0 right-hand side (float when f32, else double)
     const void *x;      // level-0 solution, same storage
     int Nx, px, f32;    // x-row length, parity-split x order, precision
+    const void *x2;     // a second level-0 solution (the second V-cycle of a BiCGSTAB iteration)
 };
 
 // y = A x, or y = b - A x (b != nullptr), FP64 level-0 operator
 void sd_spmv(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, const double *b, double *y);
 // y = A x fused with dots: slot <- y.(u0 or y), slot+1 <- y.u1 (nv = 1 or 2); post: SOP_*
 // in.x != nullptr: x is read from the staged level-0 solution instead, and
-// copied to x (FP64, natural order) on the way
+// copied to x (FP64, natural order) on the way unless x == nullptr
 void sd_spmv_dots(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, double *y, const double *u0,
                   const double *u1, int nv, RedBuf rb, int slot, int post = -1, Stage0 in = Stage0{});
-// z = the staged level-0 solution (FP64, natural order) and slot <- r.z
+// slot <- r.z with z the staged level-0 solution, also copied to z (FP64,
+// natural order) unless z == nullptr
 void sd_dot_stage(hipStream_t s, const double *r, double *z, Stage0 st, int n, RedBuf rb, int slot, int post);
 // bicgstab rounding floor sum_k (sum_j |A_kj x_j| + |b_k|)^2 -> slot
 void sd_floor(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, const double *b,
@@ -94,6 +96,7 @@ void sd_floor(hipStream_t s, const Geo &g, Sten7<double> A, const double *x, con
 void sd_pcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q, int n, RedBuf rb,
                       Stage0 st = Stage0{});                                   // stages r
 void sd_pcg_update_p(hipStream_t s, double *p, const double *z, int n, RedBuf rb);
+void sd_pcg_update_p_st(hipStream_t s, double *p, Stage0 st, int n, RedBuf rb);   // z = staged st.x
 void sd_scalar_op(hipStream_t s, int op, RedBuf rb);
 enum { SOP_PCG_AL = 0, SOP_PCG_BE, SOP_BI_BE, SOP_BI_AL, SOP_BI_OM, SOP_BI_NEXT, SOP_BI_RESTART, SOP_BI_NEXT_BE };
 void sd_bi_p(hipStream_t s, double *p, const double *r, const double *v, int n, RedBuf rb,
@@ -101,10 +104,14 @@ void sd_bi_p(hipStream_t s, double *p, const double *r, const double *v, int n, 
 void sd_bi_r1(hipStream_t s, double *r, const double *v, int n, RedBuf rb,
               Stage0 st = Stage0{});                                           // r -= al v; SS; stages r
 void sd_bi_x1(hipStream_t s, double *x, const double *ph, int n, RedBuf rb);   // x += al ph
+void sd_bi_x1_st(hipStream_t s, double *x, Stage0 st, int n, RedBuf rb);       // ph = staged st.x
 // x += al ph + om sh; r -= om t; RR = r.r.  With rh: also rho = rho1, rho1 =
 // rh.r and the next iteration's beta (the dot of its first step)
 void sd_bi_xr2(hipStream_t s, double *x, double *r, const double *ph, const double *sh, const double *t,
                const double *rh, int n, RedBuf rb);
+// the same with ph and sh read from the staged level-0 solutions st.x and st.x2
+void sd_bi_xr2_st(hipStream_t s, double *x, double *r, Stage0 st, const double *t, const double *rh, int n,
+                  RedBuf rb);
 void sd_bi_init(hipStream_t s, double *r, double *rh, double *p, double *v, const double *b,
                 const double *t, int n, RedBuf rb);                             // RR
 void sd_bi_restart(hipStream_t s, double *rh, double *p, double *v, const double *r, int n);

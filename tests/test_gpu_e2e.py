@@ -190,3 +190,30 @@ def test_unit_diagonal_products_are_bitwise(hip):
     np.testing.assert_array_equal(out[0][0], out[1][0])
     for a, b in zip(out[0][1], out[1][1]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_staged_reads_are_bitwise(hip, precision):
+    """The Krylov updates read the V-cycle results where the V-cycles left
+    them (two level-0 arrays, tune 23) instead of FP64 copies written by the
+    products: the same values, so a sweep's currents and fields are
+    bit-identical."""
+    print(f"\n[gpu e2e] staged reads precision {precision}, mosfet 24x16x8", flush=True)
+    spec = devices.mosfet(24, 16, 8, npts=4)
+    seg = spec.sweep[:1]
+    out = []
+    try:
+        for on in (1, 0):
+            cids = devices.load_device(hip, spec)
+            hip.sdx_tune(23, on)
+            assert hip.sdx_set_precision(precision) == 1
+            assert hip.api3_solve() == 1
+            R = sweep.run_sweep(hip, spec, cids, segments=seg)
+            assert np.all(R["conv"])
+            out.append((R["I"], state(hip, spec.N), [hip.api3_get_stat(i) for i in range(4)]))
+    finally:
+        hip.sdx_tune(23, 1)
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1], out[1][1]):
+        np.testing.assert_array_equal(a, b)
+    assert out[0][2] == out[1][2]
