@@ -267,8 +267,35 @@ def ivsweep_fixture(lib, spec, name, cid, Vs, Ve, npts, hold):
     print("wrote", name, "points", npts, "converged", int(C.sum()), int(C1.sum()))
 
 
+def readback_fixture(lib, spec, name, Va):
+    """The reference's readback at a solved bias point: the state, all 36
+    api3_bulk_copy quantities and api3_face_J of both carriers' sums and
+    electrons along every axis."""
+    lib.api3_set_threads(8)
+    cids = devices.load_device(lib, spec)
+    for c, v in enumerate(Va):
+        lib.api3_set_contact_V(cids[c], v)
+    assert lib.api3_solve() == 1
+    N = spec.N
+    st = [np.zeros(N) for _ in range(3)]
+    lib.api3_get_state(*(core.dptr(a) for a in st))
+    bulk = np.zeros((36, N))
+    for w in range(36):
+        lib.api3_bulk_copy(w, core.dptr(bulk[w]))
+    fJ = np.zeros((3, 2, N))
+    for ax in range(3):
+        for car in range(2):
+            assert lib.api3_face_J(ax, car, core.dptr(fJ[ax, car])) == 1
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), Va=np.asarray(Va, np.float64), psi=st[0], phin=st[1],
+                        phip=st[2], bulk=bulk, faceJ=fJ)
+    print("wrote", name, N, "nodes")
+
+
 def main():
     lib = load_harness()
+    if "--readback" in sys.argv:       # readback quantities at a solved bias point
+        readback_fixture(lib, devices.mosfet(24, 16, 8), "rb_mosfet_24x16x8", [0.0, 0.5, 1.0, 0.0])
+        return
     if "--ivsweep" in sys.argv:        # the C-side sweep entry points
         ivsweep_fixture(lib, devices.mosfet(24, 16, 8, npts=7), "e2e_ivsweep_mosfet_24x16x8", 2, 0.0, 1.2, 7,
                         {1: 0.05})

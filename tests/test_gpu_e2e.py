@@ -217,3 +217,66 @@ def test_staged_reads_are_bitwise(hip, precision):
     for a, b in zip(out[0][1], out[1][1]):
         np.testing.assert_array_equal(a, b)
     assert out[0][2] == out[1][2]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_readback_matches_reference(hip, precision):
+    """All 36 api3_bulk_copy quantities and api3_face_J of the 24x16x8
+    MOSFET solved at Vd = 0.5 V, Vg = 1.0 V against the reference core's at
+    the same bias (rb_mosfet_24x16x8.npz).  Both cores converge to the same
+    state within the solver tolerance (psi to 1e-9 V), so quantities that
+    follow from the state match to the tolerances below; the integer and
+    input-derived ones (kind, doping, contact ids, interface distance, the
+    quantities of physics outside the scope) match exactly."""
+    print(f"\n[gpu e2e] readback parity precision {precision}", flush=True)
+    spec = devices.mosfet(24, 16, 8)
+    g = load_golden("rb_mosfet_24x16x8")
+    cids = devices.load_device(hip, spec)
+    assert hip.sdx_set_precision(precision) == 1
+    for c, v in enumerate(g["Va"]):
+        hip.api3_set_contact_V(cids[c], float(v))
+    assert hip.api3_solve() == 1
+    N = spec.N
+    bulk = np.zeros((36, N))
+    for w in range(36):
+        hip.api3_bulk_copy(w, dptr(bulk[w]))
+    ref = g["bulk"]
+
+    def close(w, rtol, atol_frac=0.0, atol=0.0, scale=None):
+        a, b = bulk[w], ref[w]
+        np.testing.assert_array_equal(np.isnan(a), np.isnan(b), err_msg=f"quantity {w}: NaN pattern")
+        m = ~np.isnan(b)
+        sc = np.max(np.abs(b[m])) if scale is None else scale
+        tol = rtol * np.abs(b[m]) + atol_frac * sc + atol
+        err = np.abs(a[m] - b[m])
+        assert np.all(err <= tol), (w, float(err.max()), float(np.max(err - tol)))
+
+    close(0, 0, atol=1e-9)                                  # psi (V)
+    for w in (1, 2):                                        # n, p
+        close(w, 1e-6)
+    for w in (3, 4):                                        # band edges (eV)
+        close(w, 0, atol=1e-9)
+    for w in (5, 6, 15, 16):                                # quasi-Fermi levels
+        close(w, 0, atol=1e-6)
+    # current densities: 1e-4 relative plus 1e-6 of the largest total current
+    # density.  The solves resolve currents relative to the total current, so
+    # a component across the current flow (Jz of this device) and the
+    # minority (hole) current, 1e-16 of the electron current here, compare
+    # only at that scale
+    sc = max(np.max(np.abs(ref[w])) for w in (7, 8, 9))
+    for w in (7, 8, 9, 20, 21, 22, 23, 24, 25):
+        close(w, 1e-4, atol_frac=1e-6, scale=sc)
+    for w in (10, 11):                                      # mobilities
+        close(w, 1e-6)
+    close(12, 1e-4, atol_frac=1e-6)                         # recombination
+    for w in (13, 14, 17, 18, 27, 28, 29, 31, 32, 33, 34, 35):
+        close(w, 0)                                         # exact
+    for w in (19, 26, 30):                                  # BGN, n_i, interface distance
+        close(w, 1e-12)
+    sc = np.max(np.abs(g["faceJ"][:, 0]))
+    for car in range(2):
+        for ax in range(3):
+            fj = np.zeros(N)
+            assert hip.api3_face_J(ax, car, dptr(fj)) == 1
+            r = g["faceJ"][ax, car]
+            assert np.all(np.abs(fj - r) <= 1e-4 * np.abs(r) + 1e-6 * sc), (ax, car)
