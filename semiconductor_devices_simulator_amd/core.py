@@ -37,6 +37,9 @@ SIGNATURES = [
     ("api3_clear_contacts", None, []),
     ("api3_add_sheet_charge", None, [I, I, I, I, I, I, D]),
     ("api3_add_fixed_charge", None, [I, I, I, I, I, I, D]),
+    ("api3_charge_partition", I, [IP]),
+    ("api3_terminal_charges", I, [I, DP]),
+    ("api3_charge_owner", I, [IP]),
     ("api3_set_models", None, [I]),
     ("api3_set_caxis", None, [I]),
     ("api3_get_caxis", I, []),

@@ -489,9 +489,107 @@ void api3_carrier_sums(double *out) {
     }
     out[0] = sn; out[1] = sp;
 }
-int api3_charge_partition(const int *) { unsupported("terminal charge partition (C-V)"); return -1; }
-int api3_terminal_charges(int, double *) { unsupported("terminal charges (C-V)"); return 0; }
-int api3_charge_owner(int *) { return 0; }
+// ---- terminal charges (quasi-static C-V), api3_charge_partition :5192-5245,
+// api3_terminal_charges :5248-5277, api3_charge_owner :5300-5304.
+// The partition: every contact node belongs to its terminal; from the nodes
+// of ohmic contacts each terminal spreads, breadth first and all at once,
+// through semiconductor nodes of the same majority carrier and through
+// metal; fronts of different terminals that meet are counted (0 = every
+// domain has one terminal).  Semiconductor left over joins the terminal next
+// to it.  A terminal's charge is then the flux of D out of its node set
+// (Gauss's law), plus the oxide flux of its face (Robin) gate nodes.  These
+// are host passes over a readback of the state, as in the reference (serial,
+// the same bits on every run).
+static std::vector<int> g_qown;
+static int g_qct[SD_NC_MAX];
+int api3_charge_partition(const int *cterm) {
+    if (!G.allocated || !G.have_solution || !cterm) return -1;
+    const int N = G.N;
+    std::vector<double> n = dl(G.d_n), p = dl(G.d_p);
+    auto qtype = [&](int k) { return n[k] >= p[k] ? 1 : -1; };
+    g_qown.assign((size_t)N, -1);
+    for (int c = 0; c < SD_NC_MAX; c++) g_qct[c] = c < G.n_cons ? cterm[c] : -1;
+    std::vector<int> qu;
+    qu.reserve((size_t)N);
+    int meet = 0;
+    for (int k = 0; k < N; k++) {
+        const int c = G.con_mask[k] - 1;
+        if (c < 0 || c >= G.n_cons) continue;
+        const int t = cterm[c];
+        g_qown[k] = t;
+        const HostCon &cn = G.cons[c];
+        if (t >= 0 && cn.bc == 0 && !cn.te && (G.kind[k] == K_SEMI || G.kind[k] == K_METAL)) qu.push_back(k);
+    }
+    for (size_t qh = 0; qh < qu.size(); qh++) {
+        const int k = qu[qh], t = g_qown[k];
+        int i, j, l;
+        host_unidx(k, i, j, l);
+        for (int dir = 0; dir < 6; dir++) {
+            int b; double h, ar;
+            if (!host_nbr(i, j, l, k, dir, b, h, ar) || G.con_mask[b]) continue;
+            const bool ok = G.kind[k] == K_METAL ? G.kind[b] == K_METAL
+                          : (G.kind[k] == K_SEMI && G.kind[b] == K_SEMI && qtype(b) == qtype(k));
+            if (!ok) continue;
+            if (g_qown[b] < 0) { g_qown[b] = t; qu.push_back(b); }
+            else if (g_qown[b] != t) meet++;
+        }
+    }
+    // second pass: semiconductor no ohmic front reached joins its neighbour's terminal
+    qu.clear();
+    for (int k = 0; k < N; k++) if (g_qown[k] >= 0 && G.kind[k] == K_SEMI) qu.push_back(k);
+    for (size_t qh = 0; qh < qu.size(); qh++) {
+        const int k = qu[qh], t = g_qown[k];
+        int i, j, l;
+        host_unidx(k, i, j, l);
+        for (int dir = 0; dir < 6; dir++) {
+            int b; double h, ar;
+            if (!host_nbr(i, j, l, k, dir, b, h, ar)) continue;
+            if (G.con_mask[b] || G.kind[b] != K_SEMI || g_qown[b] >= 0) continue;
+            g_qown[b] = t;
+            qu.push_back(b);
+        }
+    }
+    return meet;
+}
+int api3_terminal_charges(int nt, double *Qt) {
+    const int N = G.N;
+    if (!G.allocated || !G.have_solution || (int)g_qown.size() != N || N == 0 || nt <= 0 || !Qt) return 0;
+    for (int t = 0; t <= nt; t++) Qt[t] = 0.;
+    std::vector<double> phi = dl(G.d_psi);
+    for (int k = 0; k < N; k++) {
+        const int t = g_qown[k];
+        if (t < 0 || t >= nt) continue;
+        int i, j, l;
+        host_unidx(k, i, j, l);
+        for (int dir = 0; dir < 6; dir++) {
+            int b; double h, ar;
+            if (!host_nbr(i, j, l, k, dir, b, h, ar)) continue;
+            const int tb = g_qown[b];
+            if (tb == t) continue;
+            const double ek = G.eps[k], eb = G.eps[b];
+            const double f = (2. * ek * eb / (ek + eb)) * ar / h * (phi[k] - phi[b]);
+            Qt[t] += f;
+            if (tb < 0 || tb >= nt) Qt[nt] += f;
+        }
+    }
+    for (int k = 0; k < N; k++) {                 // face gates: D through the oxide
+        if (!(G.robA[k] > 0.) || !G.gate_mask[k]) continue;
+        const int c = G.gate_mask[k] - 1;
+        const HostCon &cn = G.cons[c];
+        const double Cox = SD_EPS_OX / (cn.tox > 0. ? cn.tox : 10e-9);
+        const double pm = cn.phi_m > 0. ? cn.phi_m : G.Cref;
+        const double f = G.robA[k] * Cox * (phi[k] - (cn.V + G.Cref - pm));   // out of the semiconductor
+        const int t = g_qown[k], tg = (c >= 0 && c < SD_NC_MAX) ? g_qct[c] : -1;
+        if (t >= 0 && t < nt) Qt[t] += f;
+        if (tg >= 0 && tg < nt) Qt[tg] -= f;
+    }
+    return 1;
+}
+int api3_charge_owner(int *out) {
+    if ((int)g_qown.size() != G.N || G.N == 0 || !out) return 0;
+    memcpy(out, g_qown.data(), (size_t)G.N * sizeof(int));
+    return 1;
+}
 int api3_face_J(int ax, int carrier, double *out) {
     if (!G.allocated || !G.have_solution || !out || ax < 0 || ax > 2) return 0;
     ph_face_J(G.st, eng_view(), ax, carrier, G.d_wk[0]);

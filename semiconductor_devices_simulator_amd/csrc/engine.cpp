@@ -313,7 +313,7 @@ template <class F> static void for_con_nodes(const HostCon &cn, F body) {
             }
 }
 
-static bool host_nbr(int i, int j, int l, int k, int dir, int &kb, double &h, double &ar) {
+bool host_nbr(int i, int j, int l, int k, int dir, int &kb, double &h, double &ar) {
     switch (dir) {
     case 0: if (i == 0) return false; kb = k - 1; h = G.xs[i] - G.xs[i - 1]; ar = G.Ly[j] * G.Lz[l]; return true;
     case 1: if (i == G.Nx - 1) return false; kb = k + 1; h = G.xs[i + 1] - G.xs[i]; ar = G.Ly[j] * G.Lz[l]; return true;
@@ -323,7 +323,7 @@ static bool host_nbr(int i, int j, int l, int k, int dir, int &kb, double &h, do
     default: if (l == G.Nz - 1) return false; kb = k + G.Nx * G.Ny; h = G.zs[l + 1] - G.zs[l]; ar = G.Lx[i] * G.Ly[j]; return true;
     }
 }
-static void host_unidx(int k, int &i, int &j, int &l) {
+void host_unidx(int k, int &i, int &j, int &l) {
     int NxNy = G.Nx * G.Ny;
     l = k / NxNy; int r = k - l * NxNy; j = r / G.Nx; i = r - j * G.Nx;
 }

@@ -150,6 +150,10 @@ int eng_equilibrium();
 int eng_run_continuation();
 void eng_display();
 DevView eng_view();
+// host geometry: neighbour kb of node k = (i, j, l) in direction dir (0..5:
+// -x +x -y +y -z +z), spacing h and dual face area ar; false at the boundary
+bool host_nbr(int i, int j, int l, int k, int dir, int &kb, double &h, double &ar);
+void host_unidx(int k, int &i, int &j, int &l);
 void eng_sync();
 void eng_download(double *dst, const double *src, size_t n);
 void eng_upload(double *dst, const double *src, size_t n);

@@ -291,8 +291,42 @@ def readback_fixture(lib, spec, name, Va):
     print("wrote", name, N, "nodes")
 
 
+def cv_fixture(lib, spec, name, Va, cterm, tgt, dV=5e-3):
+    """The reference's terminal charges at a solved bias point: the node
+    partition (api3_charge_partition, api3_charge_owner), the terminal
+    charges (api3_terminal_charges) and one column of the quasi-static
+    capacitance matrix by central differences of contact tgt (+-dV, one
+    resolve each), as the front end's cap_matrix forms it (main.py:4034-4087)."""
+    lib.api3_set_threads(8)
+    cids = devices.load_device(lib, spec)
+    for c, v in enumerate(Va):
+        lib.api3_set_contact_V(cids[c], v)
+    assert lib.api3_solve() == 1
+    N, nt = spec.N, max(cterm) + 1
+    ct = np.asarray(cterm, np.int32)
+    meet = lib.api3_charge_partition(core.iptr(ct))
+    own = np.zeros(N, np.int32)
+    assert lib.api3_charge_owner(core.iptr(own)) == 1
+    Q0 = np.zeros(nt + 1)
+    assert lib.api3_terminal_charges(nt, core.dptr(Q0)) == 1
+    Qs = []
+    for sg in (1.0, -1.0):
+        lib.api3_set_contact_V(cids[tgt], Va[tgt] + sg * dV)
+        assert lib.api3_resolve() == 1
+        Q = np.zeros(nt + 1)
+        assert lib.api3_terminal_charges(nt, core.dptr(Q)) == 1
+        Qs.append(Q)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), Va=np.asarray(Va, np.float64), cterm=ct, meet=meet,
+                        owner=own, Q0=Q0, Qp=Qs[0], Qm=Qs[1], tgt=tgt, dV=dV)
+    print("wrote", name, "meet", meet, "Q0", Q0, "K col", (Qs[0] - Qs[1]) / (2 * dV))
+
+
 def main():
     lib = load_harness()
+    if "--cv" in sys.argv:             # terminal charges and a capacitance column
+        cv_fixture(lib, devices.mosfet(24, 16, 8), "cv_mosfet_24x16x8_on", [0.0, 0.5, 1.0, 0.0], [0, 1, 2, 3], 2)
+        cv_fixture(lib, devices.mosfet(24, 16, 8), "cv_mosfet_24x16x8_off", [0.0, 0.0, 0.0, 0.0], [0, 1, 2, 3], 2)
+        return
     if "--readback" in sys.argv:       # readback quantities at a solved bias point
         readback_fixture(lib, devices.mosfet(24, 16, 8), "rb_mosfet_24x16x8", [0.0, 0.5, 1.0, 0.0])
         return
