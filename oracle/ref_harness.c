@@ -128,3 +128,5 @@ void refh_ii_update(double *gii){
     gdev.models=m;
     memcpy(gii,gdev.gii,NN(&gdev)*8);
 }
+/* distance of every node to the nearest interface (walls_tensor, prepare) */
+void refh_get_qd(double *out){ memcpy(out,gdev.qd,NN(&gdev)*8); }

@@ -74,6 +74,9 @@ def silicon(T=300.0):
     m["ii_E0"], m["ii_m"] = (4e7, 4e7), (1.0, 1.0)
     hw = 0.063
     m["ii_g"] = math.tanh(hw / (2.0 * KB * 300.0 / Q)) / math.tanh(hw / (2.0 * m["VT"]))
+    # Lombardi surface mobility (:462-465), cm units, (electrons, holes)
+    m["lbB"], m["lbC"] = (4.75e7, 9.925e6), (580.0, 2947.0)
+    m["lbL"], m["lbD"], m["lbE"] = (0.125, 0.0317), (5.82e14, 2.0546e14), (0.0, 0.0)
     return m
 
 
@@ -323,16 +326,107 @@ def poisson_eval(dev, psi, phin, phip, Va, build=True, sc_prev=None):
     return F, A, sc, float(np.sqrt(np.sum((F / sc) ** 2)))
 
 
+def wall_distance(dev):
+    """Distance (m) of every semiconductor node to the nearest insulator
+    interface or Robin gate face along the mesh lines, 1 where there is
+    none: the qd part of walls_tensor, semiconductor_3d.c:1100-1135."""
+    qd = np.ones(dev.N)
+    Nx, Ny, Nz = dev.Nx, dev.Ny, dev.Nz
+    coords = (dev.xs, dev.ys, dev.zs)
+    for ax in range(3):
+        n, c = (Nx, Ny, Nz)[ax], coords[ax]
+        st = (1, Nx, Nx * Ny)[ax]
+        others = [q for q in range(3) if q != ax]
+        for b in range((Nx, Ny, Nz)[others[1]]):
+            for a in range((Nx, Ny, Nz)[others[0]]):
+                idx = [0, 0, 0]
+                idx[others[0]], idx[others[1]] = a, b
+                k0 = idx[0] + Nx * (idx[1] + Ny * idx[2])
+                for side in (0, 1):
+                    w, have = 0.0, False
+                    for t in range(n):
+                        q = n - 1 - t if side else t
+                        k = k0 + q * st
+                        if dev.kind[k] != K_SEMI:
+                            have = False
+                            continue
+                        qp = q + 1 if side else q - 1
+                        if qp < 0 or qp >= n:
+                            g, have = dev.gate_mask[k], False
+                            if g:
+                                f = dev.contacts[g - 1]["face"]
+                                if f // 2 == ax and f % 2 == side:
+                                    w, have = c[q], True
+                        else:
+                            kp = k0 + qp * st
+                            if dev.kind[kp] == K_INS:
+                                w, have = 0.5 * (c[q] + c[qp]), True
+                            elif dev.kind[kp] != K_SEMI:
+                                have = False
+                        if have:
+                            qd[k] = min(qd[k], abs(c[q] - w))
+    return qd
+
+
+def semi_gradient(dev, psi):
+    """Potential gradient per axis at every node, inside the semiconductor:
+    central over two semiconductor neighbours, one-sided at an interface, 0
+    with none (grad_semi, semiconductor_3d.c:2909-2917); shape (3, N)."""
+    N = dev.N
+    out = np.zeros((3, N))
+    semi = dev.kind == K_SEMI
+    for ax in range(3):
+        k, kp, h, _ = dev.edges(ax)
+        hm, hp, pm, pp = np.ones(N), np.ones(N), np.zeros(N), np.zeros(N)
+        has_m, has_p = np.zeros(N, bool), np.zeros(N, bool)
+        sp = semi[kp]                      # node k has its + neighbour kp in the semiconductor
+        has_p[k[sp]], hp[k[sp]], pp[k[sp]] = True, h[sp], psi[kp[sp]]
+        sm = semi[k]                       # node kp has its - neighbour k in the semiconductor
+        has_m[kp[sm]], hm[kp[sm]], pm[kp[sm]] = True, h[sm], psi[k[sm]]
+        g = np.zeros(N)
+        both = has_m & has_p
+        g = np.where(both, (pp - pm) / (hm + hp), g)
+        g = np.where(has_p & ~has_m, (pp - psi) / hp, g)
+        g = np.where(has_m & ~has_p, (psi - pm) / hm, g)
+        out[ax] = g
+    return out
+
+
+def lombardi(m, mub, Fv, Nt, Dw, hole):
+    """Lombardi surface mobility combined with the bulk mobility mub (m^2/Vs)
+    by Matthiessen's rule, weighted by the interface proximity Dw; Fv normal
+    field (V/m), Nt total doping (m^-3); lombardi semiconductor_3d.c:2924-2934
+    (no Coulomb term: silicon)."""
+    F = np.maximum(Fv * 1e-2, 1.0)
+    N = np.maximum(Nt * 1e-6, 1.0)
+    B, C, L, D, E = (m[key][hole] for key in ("lbB", "lbC", "lbL", "lbD", "lbE"))
+    muac = B / F + C * N ** L / (np.cbrt(F) * (m["T"] / 300.0))
+    musr = 1.0 / (F * F / D + F ** 3 / E) if E > 0 else D / (F * F)
+    return 1.0 / (1.0 / mub + Dw * 1e4 * (1.0 / muac + 1.0 / musr))
+
+
 def face_mobility(dev, psi, qn, qp):
     """Edge mobilities fmu[6][N] (face k -> k+e_ax stored at k), upd_mob_ref
-    semiconductor_3d.c:2962-3006 (no surface mobility, no metal contacts)."""
+    semiconductor_3d.c:2962-3006 (Lombardi surface mobility with models bit
+    16, no metal contacts)."""
     fmu = np.zeros((6, dev.N))
     m = dev.m
+    if dev.models & 16:
+        qd = wall_distance(dev)
+        gr = semi_gradient(dev, psi)
     for ax in range(3):
         k, kp, h, ar = dev.edges(ax)
         ok = (dev.kind[k] == K_SEMI) & (dev.kind[kp] == K_SEMI)
         mn = 0.5 * (dev.mu0n[k] + dev.mu0n[kp])
         mp = 0.5 * (dev.mu0p[k] + dev.mu0p[kp])
+        if dev.models & 16:
+            Dw = np.exp(-0.5 * (qd[k] + qd[kp]) / 1e-8)
+            fn2 = gr[(ax + 1) % 3] ** 2 + gr[(ax + 2) % 3] ** 2
+            F = np.sqrt(0.5 * (fn2[k] + fn2[kp]))
+            Nt = 0.5 * (dev.Nd[k] + dev.Na[k] + dev.Nd[kp] + dev.Na[kp])
+            near = Dw > 1e-8
+            mn = np.where(near, lombardi(m, mn, F, Nt, Dw, 0), mn)
+            mp = np.where(near, lombardi(m, mp, F, Nt, Dw, 1), mp)
         if dev.models & 4:
             dps = psi[kp] - psi[k]
             sn = mob_hf(mn, heat_field(qn[kp] - qn[k], dps, h), m["vsat"][0], m["beta"][0])

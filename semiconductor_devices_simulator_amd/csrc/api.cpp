@@ -285,52 +285,6 @@ void api3_iv_sweep2(int cid, double Vs, double Ve, int Npts, double *Varr, doubl
     G.cons[cid].V = V0;
 }
 
-// Distance of every semiconductor node to the nearest insulator interface or
-// Robin gate face along the mesh lines (1 m where there is none), as the
-// reference's wall scan computes it in prepare (walls_tensor,
-// semiconductor_3d.c:1100-1135; the MLDA factors it also forms belong to
-// out-of-scope physics).  Readback only: computed on demand.
-static void wall_distance(std::vector<double> &qd) {
-    const int Nx = G.Nx, Ny = G.Ny, Nz = G.Nz;
-    qd.assign((size_t)G.N, 1.0);
-    for (int ax = 0; ax < 3; ax++) {
-        const int n = ax == 0 ? Nx : (ax == 1 ? Ny : Nz);
-        const std::vector<double> &c = ax == 0 ? G.xs : (ax == 1 ? G.ys : G.zs);
-        const size_t st = ax == 0 ? 1 : (ax == 1 ? (size_t)Nx : (size_t)Nx * Ny);
-        const int n1 = ax == 0 ? Ny : Nx, n2 = ax == 2 ? Ny : Nz;
-        for (int b = 0; b < n2; b++)
-            for (int a = 0; a < n1; a++) {
-                const size_t k0 = ax == 0 ? (size_t)Nx * (a + (size_t)Ny * b)
-                                          : (ax == 1 ? a + (size_t)Nx * Ny * b : a + (size_t)Nx * b);
-                for (int side = 0; side < 2; side++) {      // wall on the low (0) or high (1) side
-                    double w = 0.;
-                    bool have = false;
-                    for (int t = 0; t < n; t++) {
-                        const int q = side ? n - 1 - t : t;
-                        const size_t k = k0 + (size_t)q * st;
-                        if (G.kind[k] != K_SEMI) { have = false; continue; }
-                        const int qp = side ? q + 1 : q - 1;
-                        if (qp < 0 || qp >= n) {
-                            const int g = G.gate_mask[k];
-                            have = false;
-                            if (g) {
-                                const int f = G.cons[g - 1].face;
-                                if (f / 2 == ax && f % 2 == side) { w = c[q]; have = true; }
-                            }
-                        } else {
-                            const size_t kp = k0 + (size_t)qp * st;
-                            if (G.kind[kp] == K_INS) { w = 0.5 * (c[q] + c[qp]); have = true; }
-                            else if (G.kind[kp] != K_SEMI) have = false;
-                        }
-                        if (!have) continue;
-                        const double zn = fabs(c[q] - w);
-                        if (zn < qd[k]) qd[k] = zn;
-                    }
-                }
-            }
-    }
-}
-
 static std::vector<double> dl(const double *src) {
     std::vector<double> v(G.N);
     eng_download(v.data(), src, G.N);
@@ -460,7 +414,7 @@ void api3_bulk_copy(int which, double *out) {
     case 28: case 29: for (int k = 0; k < N; k++) out[k] = G.kind[k] == K_SEMI ? -0.0 : NAN; return;
     case 30: {
         std::vector<double> qd;
-        wall_distance(qd);
+        eng_wall_distance(qd);
         for (int k = 0; k < N; k++) out[k] = G.kind[k] == K_SEMI ? qd[k] : NAN;
         return;
     }

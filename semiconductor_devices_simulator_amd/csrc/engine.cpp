@@ -215,7 +215,7 @@ int eng_alloc(int Nx, int Ny, int Nz) {
                      &G.A.c, &G.A.xm, &G.A.xp, &G.A.ym, &G.A.yp, &G.A.zm, &G.A.zp,
                      &G.d_rhs, &G.d_sc, &G.d_dsc, &G.d_sol, &G.d_xsave, &G.d_tr, &G.d_Ft,
                      &G.d_xk, &G.d_dqn, &G.d_dqp, &G.d_qn0, &G.d_qp0,
-                     &G.d_Jx, &G.d_Jy, &G.d_Jz, &G.d_Rt, &G.d_mudn, &G.d_mudp};
+                     &G.d_Jx, &G.d_Jy, &G.d_Jz, &G.d_Rt, &G.d_mudn, &G.d_mudp, &G.d_qd};
     for (double **p : nd) *p = dalloc<double>(N);
     for (int q = 0; q < 7; q++) G.d_wk[q] = dalloc<double>(N);
     for (int a = 0; a < 3; a++) { G.d_h1[a] = dalloc<double>(N); G.d_h2[a] = dalloc<double>(N); G.d_e2[a] = dalloc<double>(N); }
@@ -270,6 +270,7 @@ DevView eng_view() {
     d.psi = G.d_psi; d.phin = G.d_phin; d.phip = G.d_phip; d.n = G.d_n; d.p = G.d_p;
     d.mu_n = G.d_mun; d.mu_p = G.d_mup; d.fmu = G.d_fmu;
     d.gii = (G.models & 256) ? G.d_gii : nullptr;
+    d.qd = (G.models & 16) ? G.d_qd : nullptr;
     d.mats = G.d_mats; d.cons = G.d_cons;
     d.models = G.models; d.caxis = G.caxis; d.ref_mat = G.ref_mat; d.Cref = G.Cref;
     return d;
@@ -328,11 +329,60 @@ void host_unidx(int k, int &i, int &j, int &l) {
     l = k / NxNy; int r = k - l * NxNy; j = r / G.Nx; i = r - j * G.Nx;
 }
 
+// Distance of every semiconductor node to the nearest insulator interface or
+// Robin gate face along the mesh lines (1 m where there is none), as the
+// reference's wall scan computes it in prepare (walls_tensor,
+// semiconductor_3d.c:1100-1135; the MLDA factors it also forms belong to
+// out-of-scope physics).  Used by the Lombardi surface mobility (prepare) and
+// by the readback.
+void eng_wall_distance(std::vector<double> &qd) {
+    const int Nx = G.Nx, Ny = G.Ny, Nz = G.Nz;
+    qd.assign((size_t)G.N, 1.0);
+    for (int ax = 0; ax < 3; ax++) {
+        const int n = ax == 0 ? Nx : (ax == 1 ? Ny : Nz);
+        const std::vector<double> &c = ax == 0 ? G.xs : (ax == 1 ? G.ys : G.zs);
+        const size_t st = ax == 0 ? 1 : (ax == 1 ? (size_t)Nx : (size_t)Nx * Ny);
+        const int n1 = ax == 0 ? Ny : Nx, n2 = ax == 2 ? Ny : Nz;
+        for (int b = 0; b < n2; b++)
+            for (int a = 0; a < n1; a++) {
+                const size_t k0 = ax == 0 ? (size_t)Nx * (a + (size_t)Ny * b)
+                                          : (ax == 1 ? a + (size_t)Nx * Ny * b : a + (size_t)Nx * b);
+                for (int side = 0; side < 2; side++) {      // wall on the low (0) or high (1) side
+                    double w = 0.;
+                    bool have = false;
+                    for (int t = 0; t < n; t++) {
+                        const int q = side ? n - 1 - t : t;
+                        const size_t k = k0 + (size_t)q * st;
+                        if (G.kind[k] != K_SEMI) { have = false; continue; }
+                        const int qp = side ? q + 1 : q - 1;
+                        if (qp < 0 || qp >= n) {
+                            const int g = G.gate_mask[k];
+                            have = false;
+                            if (g) {
+                                const int f = G.cons[g - 1].face;
+                                if (f / 2 == ax && f % 2 == side) { w = c[q]; have = true; }
+                            }
+                        } else {
+                            const size_t kp = k0 + (size_t)qp * st;
+                            if (G.kind[kp] == K_INS) { w = 0.5 * (c[q] + c[qp]); have = true; }
+                            else if (G.kind[kp] != K_SEMI) have = false;
+                        }
+                        if (!have) continue;
+                        const double zn = fabs(c[q] - w);
+                        if (zn < qd[k]) qd[k] = zn;
+                    }
+                }
+            }
+    }
+}
+
 int eng_prepare() {
     if (!G.allocated || G.n_mats < 1) return 0;
     {   // physics outside this core is rejected where it would take effect, never approximated
         const char *why = nullptr;
-        if (G.models & (8 | 16 | 32)) why = "quantum confinement / surface mobility / gate tunnelling (models bits 3-5)";
+        if (G.models & (8 | 32)) why = "quantum confinement / gate tunnelling (models bits 3 and 5)";
+        for (int i = 0; i < G.n_mats && !why; i++)
+            if ((G.models & 16) && G.mats[i].id == 4) why = "Coulomb surface mobility of 4H-SiC (models bit 4)";
         for (int c = 0; c < G.n_cons && !why; c++)
             if ((G.models & (64 | 128)) && G.cons[c].bc == 1 && G.cons[c].face <= 5)
                 why = "metal-contact emission on a Schottky face contact (models bits 6-7)";
@@ -470,6 +520,10 @@ int eng_prepare() {
     const std::vector<double> *hv[] = {&G.Nd, &G.Na, &G.Nf, &G.eps, &G.bgn, &G.robA, &G.lncK, &G.lnvK, &G.niK, &G.mu0n, &G.mu0p};
     double *dv[] = {G.d_Nd, G.d_Na, G.d_Nf, G.d_eps, G.d_bgn, G.d_robA, G.d_lncK, G.d_lnvK, G.d_niK, G.d_mun, G.d_mup};
     for (int q = 0; q < 11; q++) HC(hipMemcpyAsync(dv[q], hv[q]->data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
+    if (G.models & 16) {                  // Lombardi: distance of every node to the nearest interface
+        eng_wall_distance(G.qd);
+        HC(hipMemcpyAsync(G.d_qd, G.qd.data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
+    }
     MatDev md[SD_NM_MAX];
     for (int i = 0; i < G.n_mats; i++) md[i] = host_mat_dev(G.mats[i]);
     HC(hipMemcpyAsync(G.d_mats, md, sizeof(MatDev) * G.n_mats, hipMemcpyHostToDevice, s));

@@ -87,6 +87,8 @@ struct Engine {
     float *d_aadF[12], *d_aadG[12];
     double *d_aaf, *d_aag;
     double *d_Jx, *d_Jy, *d_Jz, *d_Rt, *d_mudn, *d_mudp;
+    double *d_qd = nullptr;            // distance to the nearest interface (Lombardi)
+    std::vector<double> qd;            // its host copy (source of the upload)
     MatDev *d_mats = nullptr;
     ConDev *d_cons = nullptr;
     int *d_ek = nullptr, *d_ekb = nullptr, *d_eoff = nullptr;
@@ -154,6 +156,8 @@ DevView eng_view();
 // -x +x -y +y -z +z), spacing h and dual face area ar; false at the boundary
 bool host_nbr(int i, int j, int l, int k, int dir, int &kb, double &h, double &ar);
 void host_unidx(int k, int &i, int &j, int &l);
+// distance of every semiconductor node to the nearest interface (1 m: none)
+void eng_wall_distance(std::vector<double> &qd);
 void eng_sync();
 void eng_download(double *dst, const double *src, size_t n);
 void eng_upload(double *dst, const double *src, size_t n);

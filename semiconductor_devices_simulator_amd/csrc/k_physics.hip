@@ -106,6 +106,37 @@ __global__ void k_rebuild_np(DevView d) {
 void ph_rebuild_np(hipStream_t s, const DevView &d) { k_rebuild_np<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d); }
 
 // ------------------------------------------------------- face mobilities
+// potential gradient along ax inside the semiconductor (one-sided at an
+// interface), semiconductor_3d.c:2909-2917
+__device__ __forceinline__ double grad_semi(const DevView &d, int i, int j, int l, int k, int ax) {
+    const Geo &g = d.g;
+    int km = 0, kp = 0; double hm = 1., hp = 1., ar;
+    bool a = sd_nbr(g, i, j, l, k, 2 * ax, km, hm, ar) && d.kind[km] == K_SEMI;
+    bool b = sd_nbr(g, i, j, l, k, 2 * ax + 1, kp, hp, ar) && d.kind[kp] == K_SEMI;
+    if (a && b) return (d.psi[kp] - d.psi[km]) / (hm + hp);
+    if (b) return (d.psi[kp] - d.psi[k]) / hp;
+    if (a) return (d.psi[k] - d.psi[km]) / hm;
+    return 0.;
+}
+// squared field normal to an edge along ax: the gradients along the two
+// other axes (fnorm2, semiconductor_3d.c:2918-2921)
+__device__ __forceinline__ double fnorm2(const DevView &d, int i, int j, int l, int k, int ax) {
+    const double g1 = grad_semi(d, i, j, l, k, (ax + 1) % 3), g2 = grad_semi(d, i, j, l, k, (ax + 2) % 3);
+    return g1 * g1 + g2 * g2;
+}
+// Lombardi surface mobility combined with the bulk value mub (m^2/Vs) by
+// Matthiessen's rule, weighted by the interface proximity Dw; F normal field
+// (V/m), Nt total doping (m^-3); lombardi, semiconductor_3d.c:2924-2934
+// (without the Coulomb term, which only 4H-SiC carries)
+__device__ __forceinline__ double lombardi(const MatDev &m, double mub, double Fv, double Nt, double Dw, int hole) {
+    double F = Fv * 1e-2; if (F < 1.) F = 1.;                       // V/cm
+    double N = Nt * 1e-6; if (N < 1.) N = 1.;                       // cm^-3
+    const double Bc = m.lbB[hole], Cc = m.lbC[hole], Lc = m.lbL[hole], Dc = m.lbD[hole], Ec = m.lbE[hole];
+    const double muac = Bc / F + Cc * pow(N, Lc) / (cbrt(F) * (m.T / 300.));
+    const double musr = Ec > 0. ? 1. / (F * F / Dc + F * F * F / Ec) : Dc / (F * F);
+    return 1. / (1. / mub + Dw * 1e4 * (1. / muac + 1. / musr));    // cm^2/Vs -> m^2/Vs
+}
+#define SD_LB_LCRIT 1e-8
 __global__ void k_upd_mob(DevView d, const double *qn, const double *qp) {
     const Geo g = d.g;
     int k = sd_tile() * SD_BLOCK + threadIdx.x;
@@ -123,6 +154,16 @@ __global__ void k_upd_mob(DevView d, const double *qn, const double *qp) {
         if (ax == d.caxis) {
             if (m.ani_n > 0.) mn *= m.ani_n;
             if (m.ani_p > 0.) mp *= m.ani_p;
+        }
+        if (d.qd && m.lb_on) {            // Lombardi surface mobility near an interface
+            const double Dw = exp(-0.5 * (d.qd[k] + d.qd[kp]) / SD_LB_LCRIT);
+            if (Dw > 1e-8) {
+                const int ip = i + (ax == 0), jp = j + (ax == 1), lp = l + (ax == 2);
+                const double F = sqrt(0.5 * (fnorm2(d, i, j, l, k, ax) + fnorm2(d, ip, jp, lp, kp, ax)));
+                const double Nt = 0.5 * (d.Nd[k] + d.Na[k] + d.Nd[kp] + d.Na[kp]);
+                if (m.lb_on & 1) mn = lombardi(m, mn, F, Nt, Dw, 0);
+                if (m.lb_on & 2) mp = lombardi(m, mp, F, Nt, Dw, 1);
+            }
         }
         if (sat) {
             double wv = 1.0;
@@ -669,16 +710,6 @@ __device__ __forceinline__ double ii_alpha(const MatDev &m, double F, int c) {
     if (m.ii_m[c] != 1.) x = pow(x, m.ii_m[c]);
     if (x > 700.) return 0.;
     return m.ii_g * m.ii_a[c][r] * exp(-x);
-}
-__device__ __forceinline__ double grad_semi(const DevView &d, int i, int j, int l, int k, int ax) {
-    const Geo &g = d.g;
-    int km = 0, kp = 0; double hm = 1., hp = 1., ar;
-    bool a = sd_nbr(g, i, j, l, k, 2 * ax, km, hm, ar) && d.kind[km] == K_SEMI;
-    bool b = sd_nbr(g, i, j, l, k, 2 * ax + 1, kp, hp, ar) && d.kind[kp] == K_SEMI;
-    if (a && b) return (d.psi[kp] - d.psi[km]) / (hm + hp);
-    if (b) return (d.psi[kp] - d.psi[k]) / hp;
-    if (a) return (d.psi[k] - d.psi[km]) / hm;
-    return 0.;
 }
 // G = (alpha_n |J_n| + alpha_p |J_p|)/q * vol, semiconductor_3d.c:4486-4523
 __global__ void k_ii_update(DevView d) {

@@ -33,6 +33,9 @@ struct MatDev {
     double ani_n, ani_p;
     double ii_a[2][2], ii_b[2][2], ii_E0[2], ii_m[2], ii_g;
     int insulator, ii_on;
+    double T;                            // K
+    int lb_on;                           // Lombardi surface mobility: bit 0 electrons, bit 1 holes
+    double lbB[2], lbC[2], lbL[2], lbD[2], lbE[2];
 };
 
 // Per-contact data read by kernels (semiconductor_3d.c:191-208).
@@ -63,6 +66,7 @@ struct DevView {
     double *mu_n, *mu_p;                 // low-field node mobility
     double *fmu;                         // 6 face mobilities, block q at fmu + q*N
     double *gii;                         // impact-ionisation generation (1/s per node)
+    const double *qd;                    // distance to the nearest interface (m); nullptr: no Lombardi
     const MatDev *mats;
     const ConDev *cons;
     int models, caxis, ref_mat;

@@ -20,6 +20,10 @@ struct HostMat {
     double bgn_E, bgn_N, bgn_C;
     double ani_n, ani_p;
     int ii_on;
+    // Lombardi surface mobility (Sentaurus form, cm units as in the reference's
+    // mat_init :462-465): lb_on bit 0 electrons, bit 1 holes; [0] n, [1] p
+    int lb_on;
+    double lbB[2], lbC[2], lbL[2], lbD[2], lbE[2];
     double ii_a[2][2], ii_b[2][2], ii_E0[2], ii_m[2], ii_hw, ii_g;
     double lnc, lnv;
 };
@@ -64,6 +68,9 @@ inline bool host_mat_init(HostMat *m, int id, double T) {
         m->ii_on = 1; m->ii_hw = 0.063; m->ii_m[0] = m->ii_m[1] = 1.;
         m->ii_a[0][0] = m->ii_a[0][1] = 7.03e7; m->ii_b[0][0] = m->ii_b[0][1] = 1.231e8; m->ii_E0[0] = 4e7;
         m->ii_a[1][0] = 1.582e8; m->ii_b[1][0] = 2.036e8; m->ii_a[1][1] = 6.71e7; m->ii_b[1][1] = 1.693e8; m->ii_E0[1] = 4e7;
+        m->lb_on = 3;                                    // Lombardi (1988): B/F + C N^l / F^(1/3); delta/F^2
+        m->lbB[0] = 4.75e7; m->lbC[0] = 580.; m->lbL[0] = 0.125; m->lbD[0] = 5.82e14;
+        m->lbB[1] = 9.925e6; m->lbC[1] = 2947.; m->lbL[1] = 0.0317; m->lbD[1] = 2.0546e14;
         break;
     case 1:  // Ge
         m->Eg = 0.742 - 4.77e-4 * T * T / (T + 235.); m->chi = 4.00; m->eps_r = 16.2;
@@ -186,5 +193,9 @@ inline MatDev host_mat_dev(const HostMat &m) {
     memcpy(d.ii_a, m.ii_a, sizeof d.ii_a); memcpy(d.ii_b, m.ii_b, sizeof d.ii_b);
     d.ii_E0[0] = m.ii_E0[0]; d.ii_E0[1] = m.ii_E0[1]; d.ii_m[0] = m.ii_m[0]; d.ii_m[1] = m.ii_m[1]; d.ii_g = m.ii_g;
     d.insulator = m.insulator; d.ii_on = m.ii_on;
+    d.T = m.T; d.lb_on = m.lb_on;
+    for (int c = 0; c < 2; c++) {
+        d.lbB[c] = m.lbB[c]; d.lbC[c] = m.lbC[c]; d.lbL[c] = m.lbL[c]; d.lbD[c] = m.lbD[c]; d.lbE[c] = m.lbE[c];
+    }
     return d;
 }

@@ -41,7 +41,7 @@ def load_harness():
         "refh_cont_update": (D, [I, DP, DP]), "refh_B": (D, [D]), "refh_mob_low": (D, [I, D, I]),
         "refh_bgn_half": (D, [I, D]), "refh_recomb": (None, [I, D, D, D, D, DP]), "refh_mat": (None, [I, DP]),
         "refh_neutral_psi": (D, [ctypes.c_longlong, D]), "refh_set_fmu": (None, [DP]),
-        "refh_apply_bc": (None, [DP]), "refh_ii_update": (None, [DP]),
+        "refh_apply_bc": (None, [DP]), "refh_ii_update": (None, [DP]), "refh_get_qd": (None, [DP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -321,8 +321,51 @@ def cv_fixture(lib, spec, name, Va, cterm, tgt, dV=5e-3):
     print("wrote", name, "meet", meet, "Q0", Q0, "K col", (Qs[0] - Qs[1]) / (2 * dV))
 
 
+def lombardi_fixture(lib, spec, name, Va, rng, perturb=1e-3):
+    """Face mobilities with the Lombardi surface mobility (models bit 16) at a
+    perturbed solved state, and the interface distances prepare derives."""
+    cids = devices.load_device(lib, spec)
+    for c, v in enumerate(Va):
+        lib.api3_set_contact_V(cids[c], v)
+    assert lib.api3_solve() == 1, "reference solve failed"
+    N = spec.N
+    psi, phin, phip = np.zeros(N), np.zeros(N), np.zeros(N)
+    lib.api3_get_state(core.dptr(psi), core.dptr(phin), core.dptr(phip))
+    assert lib.refh_prepare() == 1
+    kind = np.zeros(N, np.uint8)
+    lib.refh_get_kind(kind.ctypes.data)
+    cm = np.zeros(N, np.int32)
+    lib.refh_get_conmask(core.iptr(cm))
+    free = (cm == 0) & (kind == 0)
+    psi = psi + perturb * free * rng.standard_normal(N)
+    phin = phin + 0.3 * perturb * free * rng.standard_normal(N)
+    phip = phip + 0.3 * perturb * free * rng.standard_normal(N)
+    lib.refh_set_raw_state(core.dptr(psi), core.dptr(phin), core.dptr(phip))
+    lib.refh_mob0()
+    fmu = np.zeros(6 * N)
+    lib.refh_upd_mob(core.dptr(fmu))
+    qd = np.zeros(N)
+    lib.refh_get_qd(core.dptr(qd))
+    out = dict(xs=spec.xs, ys=spec.ys, zs=spec.zs, Nd=spec.Nd, Na=spec.Na, models=spec.models,
+               tau=np.asarray(spec.tau if spec.tau else (0.0, 0.0)), Va=np.asarray(Va, np.float64),
+               contacts=json.dumps(contacts_of(spec)), psi=psi, phin=phin, phip=phip, kind=kind, con_mask=cm,
+               fmu=fmu.reshape(6, N), qd=qd)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print("wrote", name, N, "nodes; nodes within 10 nm of an interface", int((qd < 1e-8).sum()))
+
+
 def main():
     lib = load_harness()
+    if "--lombardi" in sys.argv:       # surface mobility (models bit 16)
+        for nm, shape, Va, seed in (("k_lb_mosfet_12x8x6", (12, 8, 6), [0.0, 0.5, 0.8, 0.0], 77),
+                                    ("k_lb_mosfet_20x14x9", (20, 14, 9), [0.0, 1.0, 1.1, 0.0], 78)):
+            spec = devices.mosfet(*shape)
+            spec.models = 7 | 16
+            lombardi_fixture(lib, spec, nm, Va, np.random.default_rng(seed))
+        spec = devices.mosfet(24, 16, 8, npts=7)
+        spec.models = 7 | 16
+        sweep_fixture(lib, spec, "e2e_lb_mosfet_24x16x8")
+        return
     if "--cv" in sys.argv:             # terminal charges and a capacitance column
         cv_fixture(lib, devices.mosfet(24, 16, 8), "cv_mosfet_24x16x8_on", [0.0, 0.5, 1.0, 0.0], [0, 1, 2, 3], 2)
         cv_fixture(lib, devices.mosfet(24, 16, 8), "cv_mosfet_24x16x8_off", [0.0, 0.0, 0.0, 0.0], [0, 1, 2, 3], 2)

@@ -103,7 +103,7 @@ def test_state_injection_round_trip(hip):
 def test_out_of_scope_physics_is_rejected(hip):
     spec = devices.pn_diode(8, 8, 32)
     devices.load_device(hip, spec)
-    hip.api3_set_models(16)                          # Lombardi surface mobility
+    hip.api3_set_models(32)                          # gate tunnelling
     assert hip.api3_solve() == 0
     hip.api3_set_models(0)
     assert hip.api3_solve() == 1
