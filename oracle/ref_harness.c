@@ -130,3 +130,5 @@ void refh_ii_update(double *gii){
 }
 /* distance of every node to the nearest interface (walls_tensor, prepare) */
 void refh_get_qd(double *out){ memcpy(out,gdev.qd,NN(&gdev)*8); }
+/* MLDA confinement terms of ln Nc, ln Nv (walls_tensor with models bit 3) */
+void refh_get_qc(double *qcn, double *qcp){ memcpy(qcn,gdev.qcn,NN(&gdev)*8); memcpy(qcp,gdev.qcp,NN(&gdev)*8); }

@@ -77,6 +77,8 @@ def silicon(T=300.0):
     # Lombardi surface mobility (:462-465), cm units, (electrons, holes)
     m["lbB"], m["lbC"] = (4.75e7, 9.925e6), (580.0, 2947.0)
     m["lbL"], m["lbD"], m["lbE"] = (0.125, 0.0317), (5.82e14, 2.0546e14), (0.0, 0.0)
+    # MLDA confinement masses (m0) and valley weights (:457): (e1, e2, w_e1), (h1, h2, w_h1)
+    m["mq_e"], m["mq_h"] = (0.916, 0.19, 1.0 / 3.0), (0.29, 0.20, 0.84)
     return m
 
 
@@ -185,9 +187,14 @@ class Device:
             for k, area in self.contact_nodes(cn):
                 if self.gate_mask[k] == c + 1 and not self.con_mask[k]:
                     self.robA[k] = area
-        self.lncK = self.lnc + self.bgn
-        self.lnvK = self.lnv + self.bgn
-        self.niK = m["ni"] * np.exp(self.bgn)
+        # MLDA (models bit 3): confinement terms of ln Nc, ln Nv near interfaces (:736-738)
+        if models & 8:
+            _, self.qcn, self.qcp = walls(self, mlda=True)
+        else:
+            self.qcn, self.qcp = np.zeros(self.N), np.zeros(self.N)
+        self.lncK = self.lnc + self.bgn + self.qcn
+        self.lnvK = self.lnv + self.bgn + self.qcp
+        self.niK = m["ni"] * np.exp(self.bgn + 0.5 * (self.qcn + self.qcp))
         semi = self.kind == K_SEMI
         self.mu0n = np.where(semi, mob_low(m, self.Nd + self.Na, 0), 0.0)
         self.mu0p = np.where(semi, mob_low(m, self.Nd + self.Na, 1), 0.0)
@@ -326,11 +333,49 @@ def poisson_eval(dev, psi, phin, phip, Va, build=True, sc_prev=None):
     return F, A, sc, float(np.sqrt(np.sum((F / sc) ** 2)))
 
 
+HBAR, M0 = 1.054571817e-34, 9.1093837015e-31
+
+
+def mlda_pop(a, b, lam):
+    """Fraction of the bulk density kept, averaged over the cell [a, b] from
+    a wall, for thermal length lam: 1 - exp(-(z/lam)^2) averaged, mlda_pop
+    semiconductor_3d.c:1073-1081 (series for small cells)."""
+    u, v = a / lam, b / lam
+    if v < 0.05:
+        d = v - u
+        if d < 1e-12:
+            return u * u
+        return ((v ** 3 - u ** 3) / 3.0 - (v ** 5 - u ** 5) / 10.0 + (v ** 7 - u ** 7) / 42.0) / d
+    if v - u < 1e-9 * v:
+        return 1.0 - math.exp(-u * u)
+    return 1.0 - 0.5 * math.sqrt(math.pi) * (math.erf(v) - math.erf(u)) / (v - u)
+
+
+def mlda_factor(m, a, b, hole):
+    """Cell-averaged MLDA factor of two valley masses, mlda_factor :1082-1089."""
+    m1, m2, w = m["mq_h"] if hole else m["mq_e"]
+    kT = KB * m["T"]
+    a = max(a, 0.0)
+    b = max(b, a)
+    f = w * mlda_pop(a, b, HBAR / math.sqrt(2.0 * m1 * M0 * kT))
+    if w < 1.0 and m2 > 0.0:
+        f += (1.0 - w) * mlda_pop(a, b, HBAR / math.sqrt(2.0 * m2 * M0 * kT))
+    return f
+
+
 def wall_distance(dev):
     """Distance (m) of every semiconductor node to the nearest insulator
     interface or Robin gate face along the mesh lines, 1 where there is
     none: the qd part of walls_tensor, semiconductor_3d.c:1100-1135."""
+    return walls(dev)[0]
+
+
+def walls(dev, mlda=False):
+    """The wall scan of walls_tensor (semiconductor_3d.c:1100-1146): qd, and
+    with mlda the sums over the walls within 40 nm of ln(cell-averaged MLDA
+    factor) per carrier (qcn, qcp, clamped at -60)."""
     qd = np.ones(dev.N)
+    qcn, qcp = np.zeros(dev.N), np.zeros(dev.N)
     Nx, Ny, Nz = dev.Nx, dev.Ny, dev.Nz
     coords = (dev.xs, dev.ys, dev.zs)
     for ax in range(3):
@@ -363,9 +408,17 @@ def wall_distance(dev):
                                 w, have = 0.5 * (c[q] + c[qp]), True
                             elif dev.kind[kp] != K_SEMI:
                                 have = False
-                        if have:
-                            qd[k] = min(qd[k], abs(c[q] - w))
-    return qd
+                        if not have:
+                            continue
+                        lo = 0.5 * (c[q - 1] + c[q]) if q > 0 else c[q]
+                        hi = 0.5 * (c[q] + c[q + 1]) if q < n - 1 else c[q]
+                        za, zb = (w - hi, w - lo) if side else (lo - w, hi - w)
+                        za = max(za, 0.0)
+                        qd[k] = min(qd[k], abs(c[q] - w))
+                        if mlda and za < 40e-9:
+                            qcn[k] += math.log(max(mlda_factor(dev.m, za, zb, 0), 1e-30))
+                            qcp[k] += math.log(max(mlda_factor(dev.m, za, zb, 1), 1e-30))
+    return qd, np.maximum(qcn, -60.0), np.maximum(qcp, -60.0)
 
 
 def semi_gradient(dev, psi):

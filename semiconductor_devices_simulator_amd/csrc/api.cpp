@@ -411,7 +411,12 @@ void api3_bulk_copy(int which, double *out) {
     }
     case 26: for (int k = 0; k < N; k++) out[k] = G.kind[k] == K_SEMI ? G.niK[k] : 0.; return;
     case 27: for (int k = 0; k < N; k++) out[k] = 0.; return;
-    case 28: case 29: for (int k = 0; k < N; k++) out[k] = G.kind[k] == K_SEMI ? -0.0 : NAN; return;
+    case 28: case 29: {            // quantum potentials -qc VT (eV); 0 without MLDA
+        const std::vector<double> &qc = which == 28 ? G.qcn : G.qcp;
+        for (int k = 0; k < N; k++)
+            out[k] = G.kind[k] == K_SEMI ? -((size_t)k < qc.size() ? qc[k] : 0.) * mt(k).VT : NAN;
+        return;
+    }
     case 30: {
         std::vector<double> qd;
         eng_wall_distance(qd);

@@ -335,9 +335,38 @@ void host_unidx(int k, int &i, int &j, int &l) {
 // semiconductor_3d.c:1100-1135; the MLDA factors it also forms belong to
 // out-of-scope physics).  Used by the Lombardi surface mobility (prepare) and
 // by the readback.
-void eng_wall_distance(std::vector<double> &qd) {
+// MLDA: fraction of the bulk density kept in the cell [a, b] from a wall
+// (Gaussian population average, thermal length lam), mlda_pop
+// semiconductor_3d.c:1073-1081; series for small cells, no cancellation
+static double mlda_pop(double a, double b, double lam) {
+    const double u = a / lam, v = b / lam;
+    if (v < 0.05) {
+        const double d = v - u;
+        if (d < 1e-12) return u * u;
+        return ((v * v * v - u * u * u) / 3. - (pow(v, 5) - pow(u, 5)) / 10. + (pow(v, 7) - pow(u, 7)) / 42.) / d;
+    }
+    if (v - u < 1e-9 * v) return 1. - exp(-u * u);
+    return 1. - 0.5 * sqrt(M_PI) * (erf(v) - erf(u)) / (v - u);
+}
+// cell-averaged MLDA factor of carrier `hole` for a cell [a, b] from the
+// wall, two valley masses weighted, mlda_factor :1082-1089
+static double mlda_factor(const HostMat &m, double a, double b, int hole) {
+    const double kT = SD_KB * m.T, m1 = hole ? m.mq_h1 : m.mq_e1, m2 = hole ? m.mq_h2 : m.mq_e2;
+    const double w = hole ? m.wq_h1 : m.wq_e1;
+    if (!(m1 > 0.)) return 1.;
+    if (a < 0.) a = 0.;
+    if (b < a) b = a;
+    const double hb = 1.054571817e-34, m0 = 9.1093837015e-31;
+    double f = w * mlda_pop(a, b, hb / sqrt(2. * m1 * m0 * kT));
+    if (w < 1. && m2 > 0.) f += (1. - w) * mlda_pop(a, b, hb / sqrt(2. * m2 * m0 * kT));
+    return f;
+}
+// With qcn/qcp given (MLDA on): also the sum over the walls within 40 nm of
+// ln(cell-averaged factor) per carrier, clamped at -60.
+void eng_walls(std::vector<double> &qd, std::vector<double> *qcn, std::vector<double> *qcp) {
     const int Nx = G.Nx, Ny = G.Ny, Nz = G.Nz;
     qd.assign((size_t)G.N, 1.0);
+    if (qcn) { qcn->assign((size_t)G.N, 0.); qcp->assign((size_t)G.N, 0.); }
     for (int ax = 0; ax < 3; ax++) {
         const int n = ax == 0 ? Nx : (ax == 1 ? Ny : Nz);
         const std::vector<double> &c = ax == 0 ? G.xs : (ax == 1 ? G.ys : G.zs);
@@ -368,19 +397,34 @@ void eng_wall_distance(std::vector<double> &qd) {
                             else if (G.kind[kp] != K_SEMI) have = false;
                         }
                         if (!have) continue;
-                        const double zn = fabs(c[q] - w);
+                        const double lo = q > 0 ? 0.5 * (c[q - 1] + c[q]) : c[q];
+                        const double hi = q < n - 1 ? 0.5 * (c[q] + c[q + 1]) : c[q];
+                        double za = side ? w - hi : lo - w;
+                        const double zb = side ? w - lo : hi - w, zn = fabs(c[q] - w);
+                        if (za < 0.) za = 0.;
                         if (zn < qd[k]) qd[k] = zn;
+                        if (qcn && za < 40e-9) {
+                            const HostMat &m = G.mats[G.mat8[k]];
+                            (*qcn)[k] += log(fmax(mlda_factor(m, za, zb, 0), 1e-30));
+                            (*qcp)[k] += log(fmax(mlda_factor(m, za, zb, 1), 1e-30));
+                        }
                     }
                 }
             }
     }
+    if (qcn)
+        for (int k = 0; k < G.N; k++) {
+            if ((*qcn)[k] < -60.) (*qcn)[k] = -60.;
+            if ((*qcp)[k] < -60.) (*qcp)[k] = -60.;
+        }
 }
+void eng_wall_distance(std::vector<double> &qd) { eng_walls(qd, nullptr, nullptr); }
 
 int eng_prepare() {
     if (!G.allocated || G.n_mats < 1) return 0;
     {   // physics outside this core is rejected where it would take effect, never approximated
         const char *why = nullptr;
-        if (G.models & (8 | 32)) why = "quantum confinement / gate tunnelling (models bits 3 and 5)";
+        if (G.models & 32) why = "gate tunnelling (models bit 5)";
         for (int i = 0; i < G.n_mats && !why; i++)
             if ((G.models & 16) && G.mats[i].id == 4) why = "Coulomb surface mobility of 4H-SiC (models bit 4)";
         for (int c = 0; c < G.n_cons && !why; c++)
@@ -442,14 +486,22 @@ int eng_prepare() {
         });
         cn.A = Ab;
     }
-    // node band data and low-field mobility (constant for the device)
+    // node band data and low-field mobility (constant for the device); with
+    // MLDA (models bit 3) the confinement factors qcn, qcp of the effective
+    // densities of states near interfaces (lncK, lnvK, niK :736-738)
+    if (G.models & 8) {
+        std::vector<double> qd;
+        eng_walls(qd, &G.qcn, &G.qcp);
+    } else {
+        G.qcn.assign(N, 0.); G.qcp.assign(N, 0.);
+    }
     G.lncK.assign(N, -1e300); G.lnvK.assign(N, -1e300); G.niK.assign(N, 0.); G.mu0n.assign(N, 0.); G.mu0p.assign(N, 0.);
     for (int k = 0; k < N; k++) {
         const HostMat &m = G.mats[G.mat8[k]];
         if (m.insulator) { G.lncK[k] = G.lnvK[k] = m.lnc; continue; }
-        G.lncK[k] = m.lnc + G.bgn[k];
-        G.lnvK[k] = m.lnv + G.bgn[k];
-        G.niK[k] = m.ni * exp(G.bgn[k]);
+        G.lncK[k] = m.lnc + G.bgn[k] + G.qcn[k];
+        G.lnvK[k] = m.lnv + G.bgn[k] + G.qcp[k];
+        G.niK[k] = m.ni * exp(G.bgn[k] + 0.5 * (G.qcn[k] + G.qcp[k]));
         if (G.kind[k] == K_SEMI) {
             double Nt = G.Nd[k] + G.Na[k];
             G.mu0n[k] = host_mob_low(m, Nt, 0);

@@ -89,6 +89,7 @@ struct Engine {
     double *d_Jx, *d_Jy, *d_Jz, *d_Rt, *d_mudn, *d_mudp;
     double *d_qd = nullptr;            // distance to the nearest interface (Lombardi)
     std::vector<double> qd;            // its host copy (source of the upload)
+    std::vector<double> qcn, qcp;      // MLDA confinement terms of ln Nc, ln Nv (0 without MLDA)
     MatDev *d_mats = nullptr;
     ConDev *d_cons = nullptr;
     int *d_ek = nullptr, *d_ekb = nullptr, *d_eoff = nullptr;
@@ -158,6 +159,8 @@ bool host_nbr(int i, int j, int l, int k, int dir, int &kb, double &h, double &a
 void host_unidx(int k, int &i, int &j, int &l);
 // distance of every semiconductor node to the nearest interface (1 m: none)
 void eng_wall_distance(std::vector<double> &qd);
+// the same, with the MLDA confinement terms per carrier (qcn, qcp)
+void eng_walls(std::vector<double> &qd, std::vector<double> *qcn, std::vector<double> *qcp);
 void eng_sync();
 void eng_download(double *dst, const double *src, size_t n);
 void eng_upload(double *dst, const double *src, size_t n);

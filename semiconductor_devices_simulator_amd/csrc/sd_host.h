@@ -24,6 +24,8 @@ struct HostMat {
     // mat_init :462-465): lb_on bit 0 electrons, bit 1 holes; [0] n, [1] p
     int lb_on;
     double lbB[2], lbC[2], lbL[2], lbD[2], lbE[2];
+    // MLDA confinement masses (units of m0) and valley weights (:146, :457, :619-623)
+    double mq_e1, mq_e2, wq_e1, mq_h1, mq_h2, wq_h1;
     double ii_a[2][2], ii_b[2][2], ii_E0[2], ii_m[2], ii_hw, ii_g;
     double lnc, lnv;
 };
@@ -71,6 +73,8 @@ inline bool host_mat_init(HostMat *m, int id, double T) {
         m->lb_on = 3;                                    // Lombardi (1988): B/F + C N^l / F^(1/3); delta/F^2
         m->lbB[0] = 4.75e7; m->lbC[0] = 580.; m->lbL[0] = 0.125; m->lbD[0] = 5.82e14;
         m->lbB[1] = 9.925e6; m->lbC[1] = 2947.; m->lbL[1] = 0.0317; m->lbD[1] = 2.0546e14;
+        // MLDA: electrons in the 2 Delta-2 (0.916) and 4 Delta-4 (0.19) valleys, holes heavy / light
+        m->mq_e1 = 0.916; m->mq_e2 = 0.19; m->wq_e1 = 1. / 3.; m->mq_h1 = 0.29; m->mq_h2 = 0.20; m->wq_h1 = 0.84;
         break;
     case 1:  // Ge
         m->Eg = 0.742 - 4.77e-4 * T * T / (T + 235.); m->chi = 4.00; m->eps_r = 16.2;
@@ -132,6 +136,11 @@ inline bool host_mat_init(HostMat *m, int id, double T) {
     case 12: m->insulator = 1; m->Eg = 10.; m->chi = 0.; m->eps_r = 1.0; break;      // void
     case 13: m->insulator = 1; m->Eg = 10.; m->chi = 0.; m->eps_r = 1.0e3; break;    // floating metal inclusion
     default: return false;
+    }
+    if (!m->insulator && !(m->mq_e1 > 0.) && id >= 0 && id < 8) {   // single DOS-like masses
+        static const double me[8] = {0.916, 0.082, 0.067, 0.08, 0.37, 0.20, 0.09, 0.23};
+        static const double mh[8] = {0.29, 0.28, 0.50, 0.60, 1.00, 1.00, 0.55, 1.00};
+        m->mq_e1 = m->mq_e2 = me[id]; m->wq_e1 = 1.; m->mq_h1 = m->mq_h2 = mh[id]; m->wq_h1 = 1.;
     }
     m->ni = m->insulator ? 0. : sqrt(m->Nc * m->Nv) * exp(-m->Eg / (2. * m->VT));
     m->ii_g = m->ii_hw > 0. ? tanh(m->ii_hw / (2. * SD_KB * 300. / SD_Q)) / tanh(m->ii_hw / (2. * m->VT)) : 1.;

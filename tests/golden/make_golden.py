@@ -41,7 +41,7 @@ def load_harness():
         "refh_cont_update": (D, [I, DP, DP]), "refh_B": (D, [D]), "refh_mob_low": (D, [I, D, I]),
         "refh_bgn_half": (D, [I, D]), "refh_recomb": (None, [I, D, D, D, D, DP]), "refh_mat": (None, [I, DP]),
         "refh_neutral_psi": (D, [ctypes.c_longlong, D]), "refh_set_fmu": (None, [DP]),
-        "refh_apply_bc": (None, [DP]), "refh_ii_update": (None, [DP]), "refh_get_qd": (None, [DP]),
+        "refh_apply_bc": (None, [DP]), "refh_ii_update": (None, [DP]), "refh_get_qd": (None, [DP]), "refh_get_qc": (None, [DP, DP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -158,6 +158,10 @@ def kernel_fixture(lib, spec, name, Va, rng, perturb=1e-3):
     mats = np.zeros(19)
     lib.refh_mat(0, core.dptr(mats))
     out["mat0"] = mats
+    if spec.models & 8:                # MLDA confinement terms prepare derived
+        qcn, qcp = np.zeros(N), np.zeros(N)
+        lib.refh_get_qc(core.dptr(qcn), core.dptr(qcp))
+        out["qcn"], out["qcp"] = qcn, qcp
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
     print("wrote", name, N, "nodes")
 
@@ -356,6 +360,16 @@ def lombardi_fixture(lib, spec, name, Va, rng, perturb=1e-3):
 
 def main():
     lib = load_harness()
+    if "--mlda" in sys.argv:           # quantum confinement (models bit 3)
+        rng = np.random.default_rng(4444)
+        for shape, Va in (((12, 8, 6), [0.0, 0.5, 0.8, 0.0]), ((20, 14, 9), [0.0, 1.0, 1.1, 0.0])):
+            spec = devices.mosfet(*shape)
+            spec.models = 7 | 8
+            kernel_fixture(lib, spec, "k_mlda_mosfet_%dx%dx%d" % shape, Va, rng)
+        spec = devices.mosfet(24, 16, 8, npts=7)
+        spec.models = 7 | 8
+        sweep_fixture(lib, spec, "e2e_mlda_mosfet_24x16x8")
+        return
     if "--lombardi" in sys.argv:       # surface mobility (models bit 16)
         for nm, shape, Va, seed in (("k_lb_mosfet_12x8x6", (12, 8, 6), [0.0, 0.5, 0.8, 0.0], 77),
                                     ("k_lb_mosfet_20x14x9", (20, 14, 9), [0.0, 1.0, 1.1, 0.0], 78)):

@@ -1,7 +1,10 @@
-"""Lombardi surface mobility (models bit 16) on the device: the face
-mobilities against the reference core's at perturbed solved states of two
-MOSFETs (golden fixtures k_lb_*), and a MOSFET Id-Vg sweep with the model on
-against the reference core's sweep (e2e_lb_mosfet_24x16x8)."""
+"""MOSFET surface models on the device.  Lombardi surface mobility (models
+bit 16): the face mobilities against the reference core's at perturbed
+solved states of two MOSFETs (golden fixtures k_lb_*), and a MOSFET Id-Vg
+sweep with the model on against the reference core's sweep
+(e2e_lb_mosfet_24x16x8).  MLDA quantum confinement (models bit 3): its
+kernel-level checks run on the k_mlda_* fixtures in test_gpu_kernels; here
+the Id-Vg sweep against the reference's (e2e_mlda_mosfet_24x16x8)."""
 import numpy as np
 import pytest
 
@@ -47,3 +50,26 @@ def test_lombardi_sweep_matches_reference(hip, precision):
     psi = np.zeros(spec.N)
     hip.api3_get_state(dptr(psi), dptr(np.zeros(spec.N)), dptr(np.zeros(spec.N)))
     assert np.max(np.abs(psi - g["psi"])) <= 1e-9
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_mlda_sweep_matches_reference(hip, precision):
+    print(f"\n[gpu mlda] mosfet 24x16x8 models 15 precision {precision}", flush=True)
+    spec = devices.mosfet(24, 16, 8, npts=7)
+    spec.models = 7 | 8
+    g = load_golden("e2e_mlda_mosfet_24x16x8")
+    cids = devices.load_device(hip, spec)
+    assert hip.sdx_set_precision(precision) == 1
+    assert hip.api3_solve() == 1
+    R = sweep.run_sweep(hip, spec, cids)
+    assert np.all(R["conv"])
+    d = np.abs(R["I"] - g["I"])
+    ok = d <= 1e-8 * np.abs(g["I"]) + g["F"] + R["F"]
+    assert ok.all(), (np.argwhere(~ok), d[~ok], g["I"][~ok])
+    psi = np.zeros(spec.N)
+    hip.api3_get_state(dptr(psi), dptr(np.zeros(spec.N)), dptr(np.zeros(spec.N)))
+    assert np.max(np.abs(psi - g["psi"])) <= 1e-9
+    # the quantum potential readback is the confinement term times -VT
+    qn = np.zeros(spec.N)
+    hip.api3_bulk_copy(28, dptr(qn))
+    assert np.nanmin(qn) >= 0.0 and 0.0 < np.nanmax(qn) < 0.05     # eV, positive near the gate only
