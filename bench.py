@@ -246,6 +246,8 @@ def main():
     from semiconductor_devices_simulator_amd import core, devices, warmstart
     lib = core.load_core()
     a.mesh = a.mesh or DEFAULT_MESH[a.workload]
+    if a.workload == "vdmos" and a.mesh == DEFAULT_MESH["vdmos"] and "--steps" not in sys.argv:
+        a.steps = 43          # the blocking sweep to 44 V: this mesh breaks down at 46 V
     spec = devices.mosfet(*a.mesh) if a.workload == "mosfet" else devices.vdmos(*a.mesh)
     if a.workload != "mosfet":          # the trained network and the CPU sample are MOSFET-only
         a.no_learned = a.no_cpu_baseline = True
