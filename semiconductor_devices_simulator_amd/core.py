@@ -74,6 +74,7 @@ SIGNATURES = [
     ("api3_set_node_data", None, [IP, DP, DP, DP]),
     ("api3_set_mat_tau", I, [I, D, D]),
     ("api3_carrier_sums", None, [DP]),
+    ("api3_ins_field_max", D, [DP]),
     ("api3_get_state", I, [DP, DP, DP]),
     ("api3_set_state", I, [DP, DP, DP]),
     ("api3_set_predictor", None, [I]),

@@ -290,8 +290,16 @@ def readback_fixture(lib, spec, name, Va):
     for ax in range(3):
         for car in range(2):
             assert lib.api3_face_J(ax, car, core.dptr(fJ[ax, car])) == 1
+    nc = len(cids)
+    con = np.array([[lib.api3_get_contact_I(cids[c]), lib.api3_get_contact_In(cids[c]), lib.api3_get_contact_Iraw(cids[c]),
+                     lib.api3_get_contact_Ifloor(cids[c]), lib.api3_get_contact_A(cids[c])] for c in range(nc)])
+    sums = np.zeros(2)
+    lib.api3_carrier_sums(core.dptr(sums))
+    pos = np.zeros(3)
+    fins = lib.api3_ins_field_max(core.dptr(pos))
     np.savez_compressed(os.path.join(HERE, name + ".npz"), Va=np.asarray(Va, np.float64), psi=st[0], phin=st[1],
-                        phip=st[2], bulk=bulk, faceJ=fJ)
+                        phip=st[2], bulk=bulk, faceJ=fJ, contacts_IInIrawFA=con, carrier_sums=sums,
+                        Emax=lib.api3_get_Emax(), ins_field_max=fins, ins_field_pos=pos)
     print("wrote", name, N, "nodes")
 
 

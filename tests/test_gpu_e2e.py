@@ -325,3 +325,36 @@ def test_terminal_charges_match_reference(hip, precision, case):
     assert np.all(np.abs(K - Kr) <= 1e-4 * np.max(np.abs(Kr))), (K, Kr)
     # charge neutrality of the column: the terminal charges' changes sum to 0
     assert abs(K[:nt].sum()) <= 1e-3 * np.max(np.abs(K))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_scalar_readback_matches_reference(hip, precision):
+    """Scalar readbacks at the same solved bias point as the node readback
+    (rb_mosfet_24x16x8.npz): every contact's current, electron current, raw
+    flux, floor and area; the carrier numbers; the largest semiconductor
+    field; the largest insulator (gate oxide) field and where it is."""
+    print(f"\n[gpu e2e] scalar readback parity precision {precision}", flush=True)
+    spec = devices.mosfet(24, 16, 8)
+    g = load_golden("rb_mosfet_24x16x8")
+    cids = devices.load_device(hip, spec)
+    assert hip.sdx_set_precision(precision) == 1
+    for c, v in enumerate(g["Va"]):
+        hip.api3_set_contact_V(cids[c], float(v))
+    assert hip.api3_solve() == 1
+    ref = g["contacts_IInIrawFA"]
+    for c in range(len(cids)):
+        I, In, Iraw = (hip.api3_get_contact_I(cids[c]), hip.api3_get_contact_In(cids[c]),
+                       hip.api3_get_contact_Iraw(cids[c]))
+        F, A = hip.api3_get_contact_Ifloor(cids[c]), hip.api3_get_contact_A(cids[c])
+        fl = F + ref[c, 3]
+        for got, want in ((I, ref[c, 0]), (In, ref[c, 1]), (Iraw, ref[c, 2])):
+            assert abs(got - want) <= 1e-8 * abs(want) + fl, (c, got, want)
+        assert F == pytest.approx(ref[c, 3], rel=1e-3, abs=1e-30)
+        assert A == pytest.approx(ref[c, 4], rel=1e-14)
+    sums = np.zeros(2)
+    hip.api3_carrier_sums(dptr(sums))
+    np.testing.assert_allclose(sums, g["carrier_sums"], rtol=1e-6)
+    assert hip.api3_get_Emax() == pytest.approx(float(g["Emax"]), rel=1e-6)
+    pos = np.zeros(3)
+    assert hip.api3_ins_field_max(dptr(pos)) == pytest.approx(float(g["ins_field_max"]), rel=1e-6)
+    np.testing.assert_allclose(pos, g["ins_field_pos"], rtol=0, atol=1e-15)
