@@ -1,0 +1,66 @@
+"""Host logic of the measurement (bench.py): the parity criteria it reports,
+the counter-summary traffic it reads, the sweep points of each workload and
+the one-GPU guard.  CPU only."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+from semiconductor_devices_simulator_amd import devices  # noqa: E402
+
+
+def test_parity_against_the_reference_sweep_fixture():
+    g = np.load(os.path.join(ROOT, "tests", "golden", "e2e_mosfet_96x64x64.npz"))
+    W, K = 2, 10
+    I = g["I"][W:W + K].copy()
+    p = bench.parity(I, W, K, [96, 64, 64])
+    assert p["points_within_tolerance"] == K and p["max_rel_drain_current_err"] == 0.0
+    # a drain current off by 1e-6 relative at a point well above its floor fails there
+    j = int(np.argmax(np.abs(I[:, 1]) / (g["F"][W:W + K, 1] + 1e-300)))
+    I[j, 1] *= 1 + 1e-6
+    p = bench.parity(I, W, K, [96, 64, 64])
+    assert p["points_within_tolerance"] == K - 1
+    assert p["max_rel_drain_current_err"] == pytest.approx(1e-6, rel=1e-6)
+    # no fixture for the mesh: no claim
+    assert bench.parity(I, W, K, [7, 7, 7]) is None
+
+
+def test_mixed_vs_fp64_criterion_and_fields():
+    I64 = np.array([[1e-6, -1e-6], [2e-6, -2e-6]])
+    F64 = np.full_like(I64, 1e-18)
+    st = tuple(np.zeros(4) for _ in range(3))
+    st2 = (np.full(4, 1e-10), np.zeros(4), np.zeros(4))
+    out = bench.mixed_vs_fp64(I64 * (1 + 5e-9), I64, F64, st2, st)
+    assert out["points_within_tolerance"] == 2
+    assert out["last_point_max_abs_diff_V"]["psi"] == pytest.approx(1e-10)
+    out = bench.mixed_vs_fp64(I64 * (1 + 5e-8), I64, F64, st2, st)
+    assert out["points_within_tolerance"] == 0
+
+
+def test_traffic_read_from_the_committed_counter_summary():
+    by, src = bench.pmc_traffic([192, 128, 128])
+    assert src == "profiles/k_line_scan_level0_mosfet_192x128x128_pmc.md"
+    assert 40e6 < by < 160e6
+    assert bench.pmc_traffic([8, 8, 8]) == (None, None)
+
+
+def test_sweep_points_of_the_workloads():
+    pts = bench.sweep_points(devices.mosfet(12, 8, 6))
+    assert len(pts) == 50
+    assert pts[0] == (2, 0.0, {1: 0.05}) and pts[25][2] == {1: 1.0}
+    assert pts[24][1] == pytest.approx(1.2)
+    v = devices.vdmos(8, 6, 4)
+    pts = bench.sweep_points(v)
+    assert len(pts) == 61 and pts[-1][0] == 4 and pts[-1][1] == pytest.approx(60.0)
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU only" in r.stderr
