@@ -14,7 +14,8 @@ reference function it restates.  Pinning (see tests/test_oracle_*.py):
     point values;
   * prepare, poisson_eval, face_mobility, cont_assemble, matvec,
     mg_hierarchy/galerkin/restrict/prolong, terminal_currents, cont_update,
-    ii_update:
+    ii_update, walls (interface distance, MLDA), face_mobility with Lombardi,
+    charge_partition, terminal_charges:
     compared with the reference harness on small devices (golden fixtures
     tests/golden/*.npz written by tests/golden/make_golden.py) and with
     invariants (Poisson Jacobian symmetric on free rows, row sums of the
@@ -772,3 +773,130 @@ def cont_update(dev, hole, psi, n, p, phin, phip, u, sc, lo, hi):
     w = np.where(r >= 1e-3, 1.0, r * 1e3)
     a = np.where(eq & (r > 1e-6), w * np.abs(dq), 0.0)
     return qnew, n2, p2, dq, float(a.max(initial=0.0))
+
+
+# ------------------------------------------------ terminal charges (C-V)
+def _nbr(dev, i, j, l, d):
+    """Neighbour of node (i, j, l) in direction d (0..5: -x +x -y +y -z +z):
+    (kb, h, area) or None at the boundary, nbr() semiconductor_3d.c:703-713."""
+    Nx, Ny, Nz = dev.Nx, dev.Ny, dev.Nz
+    k = i + Nx * (j + Ny * l)
+    if d == 0:
+        return None if i == 0 else (k - 1, dev.xs[i] - dev.xs[i - 1], dev.Ly[j] * dev.Lz[l])
+    if d == 1:
+        return None if i == Nx - 1 else (k + 1, dev.xs[i + 1] - dev.xs[i], dev.Ly[j] * dev.Lz[l])
+    if d == 2:
+        return None if j == 0 else (k - Nx, dev.ys[j] - dev.ys[j - 1], dev.Lx[i] * dev.Lz[l])
+    if d == 3:
+        return None if j == Ny - 1 else (k + Nx, dev.ys[j + 1] - dev.ys[j], dev.Lx[i] * dev.Lz[l])
+    if d == 4:
+        return None if l == 0 else (k - Nx * Ny, dev.zs[l] - dev.zs[l - 1], dev.Lx[i] * dev.Ly[j])
+    return None if l == Nz - 1 else (k + Nx * Ny, dev.zs[l + 1] - dev.zs[l], dev.Lx[i] * dev.Ly[j])
+
+
+def charge_partition(dev, n, p, cterm):
+    """Owner terminal of every node (-1 none) and the number of meeting
+    edges, api3_charge_partition semiconductor_3d.c:5192-5245: fronts from
+    the nodes of ohmic contacts spread breadth first, all at once, through
+    semiconductor nodes of the same majority carrier (n >= p) and through
+    metal; then semiconductor no front reached joins its neighbour's."""
+    from collections import deque
+    N, Nx, Ny = dev.N, dev.Nx, dev.Ny
+    own = np.full(N, -1, np.int64)
+    qtype = np.where(n >= p, 1, -1)
+    qu = deque()
+    for k in range(N):
+        c = dev.con_mask[k] - 1
+        if c < 0 or c >= len(dev.contacts):
+            continue
+        t = cterm[c]
+        own[k] = t
+        if t >= 0 and dev.contacts[c]["bc"] == 0 and dev.kind[k] in (K_SEMI, K_METAL):
+            qu.append(k)
+    meet = 0
+
+    def ijl(k):
+        l, r = divmod(k, Nx * Ny)
+        j, i = divmod(r, Nx)
+        return i, j, l
+
+    while qu:
+        k = qu.popleft()
+        t = own[k]
+        i, j, l = ijl(k)
+        for d in range(6):
+            nb = _nbr(dev, i, j, l, d)
+            if nb is None:
+                continue
+            b = nb[0]
+            if dev.con_mask[b]:
+                continue
+            if dev.kind[k] == K_METAL:
+                ok = dev.kind[b] == K_METAL
+            else:
+                ok = dev.kind[k] == K_SEMI and dev.kind[b] == K_SEMI and qtype[b] == qtype[k]
+            if not ok:
+                continue
+            if own[b] < 0:
+                own[b] = t
+                qu.append(b)
+            elif own[b] != t:
+                meet += 1
+    qu = deque(k for k in range(N) if own[k] >= 0 and dev.kind[k] == K_SEMI)
+    while qu:
+        k = qu.popleft()
+        t = own[k]
+        i, j, l = ijl(k)
+        for d in range(6):
+            nb = _nbr(dev, i, j, l, d)
+            if nb is None:
+                continue
+            b = nb[0]
+            if dev.con_mask[b] or dev.kind[b] != K_SEMI or own[b] >= 0:
+                continue
+            own[b] = t
+            qu.append(b)
+    return own, meet
+
+
+def terminal_charges(dev, psi, own, cterm, Va):
+    """Charge of every terminal (C): the flux of D out of its node set, plus
+    the oxide flux of its face (Robin) gate nodes; the last entry is the
+    flux into the nodes of no terminal.  api3_terminal_charges
+    semiconductor_3d.c:5248-5277; Va: the contact voltages."""
+    nt = int(max(cterm)) + 1
+    Q = np.zeros(nt + 1)
+    Nx, Ny = dev.Nx, dev.Ny
+    for k in range(dev.N):
+        t = own[k]
+        if t < 0 or t >= nt:
+            continue
+        l, r = divmod(k, Nx * Ny)
+        j, i = divmod(r, Nx)
+        for d in range(6):
+            nb = _nbr(dev, i, j, l, d)
+            if nb is None:
+                continue
+            b, h, ar = nb
+            tb = own[b]
+            if tb == t:
+                continue
+            ek, eb = dev.eps[k], dev.eps[b]
+            f = (2.0 * ek * eb / (ek + eb)) * ar / h * (psi[k] - psi[b])
+            Q[t] += f
+            if tb < 0 or tb >= nt:
+                Q[nt] += f
+    for k in np.nonzero(dev.robA > 0)[0]:
+        c = dev.gate_mask[k] - 1
+        if c < 0:
+            continue
+        cn = dev.contacts[c]
+        Cox = EPS_OX / (cn["tox"] if cn["tox"] > 0 else 10e-9)
+        pm = cn["pm"] if cn["pm"] > 0 else dev.Cref
+        f = dev.robA[k] * Cox * (psi[k] - (Va[c] + dev.Cref - pm))
+        t, tg = own[k], cterm[c]
+        if 0 <= t < nt:
+            Q[t] += f
+        if 0 <= tg < nt:
+            Q[tg] -= f
+    return Q

@@ -315,6 +315,8 @@ def cv_fixture(lib, spec, name, Va, cterm, tgt, dV=5e-3):
         lib.api3_set_contact_V(cids[c], v)
     assert lib.api3_solve() == 1
     N, nt = spec.N, max(cterm) + 1
+    st = [np.zeros(N) for _ in range(3)]
+    lib.api3_get_state(*(core.dptr(a) for a in st))
     ct = np.asarray(cterm, np.int32)
     meet = lib.api3_charge_partition(core.iptr(ct))
     own = np.zeros(N, np.int32)
@@ -329,7 +331,9 @@ def cv_fixture(lib, spec, name, Va, cterm, tgt, dV=5e-3):
         assert lib.api3_terminal_charges(nt, core.dptr(Q)) == 1
         Qs.append(Q)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), Va=np.asarray(Va, np.float64), cterm=ct, meet=meet,
-                        owner=own, Q0=Q0, Qp=Qs[0], Qm=Qs[1], tgt=tgt, dV=dV)
+                        owner=own, Q0=Q0, Qp=Qs[0], Qm=Qs[1], tgt=tgt, dV=dV, psi=st[0], phin=st[1], phip=st[2],
+                        xs=spec.xs, ys=spec.ys, zs=spec.zs, Nd=spec.Nd, Na=spec.Na, models=spec.models,
+                        tau=np.asarray(spec.tau if spec.tau else (0.0, 0.0)), contacts=json.dumps(contacts_of(spec)))
     print("wrote", name, "meet", meet, "Q0", Q0, "K col", (Qs[0] - Qs[1]) / (2 * dV))
 
 
