@@ -54,3 +54,40 @@ def test_workload_shapes():
     cons = [c.to_idx6(m.xs, m.ys, m.zs) for c in m.contacts]
     # source, gate and drain windows on the top face do not overlap
     assert cons[0][1] < cons[2][0] and cons[2][1] < cons[1][0]
+
+
+def test_avalanche_diode_is_the_reverse_biased_diode_with_impact_ionisation():
+    d = D.avalanche_diode(8, 8, 48)
+    base = D.pn_diode(8, 8, 48)
+    assert d.models == 256 and d.sweep == [(0, 0.0, -19.0, 20, {})]
+    np.testing.assert_array_equal(d.Nd, base.Nd)
+    np.testing.assert_array_equal(d.Na, base.Na)
+
+
+def test_vdmos_cell_structure():
+    v = D.vdmos(40, 24, 4, vd_max=50.0, npts=11, npts_on=5, tol_c=1e-11)
+    assert v.shape == (40, 24, 4) and v.models == 7 | 256 and v.solver[4] == 1e-11
+    assert v.L3 == (20000.0, 4000.0, 8000.0)
+    # drain sweep at Vgs 0, the drain back to 0 V, then Id-Vds at Vgs 10 V (gates are contacts 2, 3)
+    assert v.sweep[0] == (4, 0.0, 50.0, 11, {2: 0.0, 3: 0.0})
+    assert v.sweep[1] == (4, 0.0, 0.0, 1, {2: 0.0, 3: 0.0})
+    assert v.sweep[2] == (4, 0.0, 20.0, 5, {2: 10.0, 3: 10.0})
+    Nx, Ny, Nz = v.shape
+    Nd = v.Nd.reshape(Nz, Ny, Nx)
+    Na = v.Na.reshape(Nz, Ny, Nx)
+    # n+ substrate (1e19 cm^-3) below 3.5 um, n- drift (5e15) in the middle of the JFET window
+    assert np.all(Nd[:, -1, :] >= 1e25)
+    mid_x, mid_y = Nx // 2, int(np.searchsorted(v.ys, 2.0e-6))
+    assert Nd[0, mid_y, mid_x] == pytest.approx(5e21, rel=1e-6) and Na[0, mid_y, mid_x] < 1e20
+    # p-body under both outer thirds, n+ source at the surface next to the taps
+    y05 = int(np.searchsorted(v.ys, 0.5e-6))
+    assert Na[0, y05, int(np.searchsorted(v.xs, 5e-6))] > 1e22
+    assert Na[0, y05, int(np.searchsorted(v.xs, 15e-6))] > 1e22
+    assert Nd[0, 0, int(np.searchsorted(v.xs, 2.5e-6))] > 1e25
+    assert Nd[0, 0, int(np.searchsorted(v.xs, 17.5e-6))] > 1e25
+    # two sources, two gates (Robin, 50 nm, 4.1 eV) and the drain on the bottom face
+    faces = [(c.face, c.bc) for c in v.contacts]
+    assert faces == [(2, 0), (2, 0), (2, 2), (2, 2), (3, 0)]
+    assert all(c.tox == 50.0 and c.pm == 4.1 for c in v.contacts if c.bc == 2)
+    full = D.vdmos()
+    assert full.N == 8388608 and full.bias_points() == 61
