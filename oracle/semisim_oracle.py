@@ -15,7 +15,7 @@ reference function it restates.  Pinning (see tests/test_oracle_*.py):
   * prepare, poisson_eval, face_mobility, cont_assemble, matvec,
     mg_hierarchy/galerkin/restrict/prolong, terminal_currents, cont_update,
     ii_update, walls (interface distance, MLDA), face_mobility with Lombardi,
-    charge_partition, terminal_charges:
+    charge_partition, terminal_charges, readback:
     compared with the reference harness on small devices (golden fixtures
     tests/golden/*.npz written by tests/golden/make_golden.py) and with
     invariants (Poisson Jacobian symmetric on free rows, row sums of the
@@ -900,3 +900,63 @@ def terminal_charges(dev, psi, own, cterm, Va):
         if 0 <= tg < nt:
             Q[tg] -= f
     return Q
+
+
+# ------------------------------------------------------------- readback
+def readback(dev, which, psi, phin, phip, fmu=None, gii=None):
+    """Node quantity `which` of api3_bulk_copy at a state (node_val,
+    semiconductor_3d.c:5066-5123) for the quantities that follow from the
+    state, the device and (20-25) the face mobilities: 0 psi, 1 n, 2 p,
+    3 Ec, 4 Ev, 5 Efn, 6 Efp (eV, vacuum level -psi), 13 kind, 14 net doping,
+    15 phi_n, 16 phi_p, 17 fixed charge, 18 contact id, 19 band-gap
+    narrowing (eV), 20-25 node current densities (Jn x y z, Jp x y z),
+    26 effective n_i, 27 gate tunnelling (0), 28-29 quantum potentials (eV),
+    30 interface distance, 31-32 traps (0), 33 impact ionisation per volume,
+    34 interface charge (0), 35 ionised net doping.  No traps, tunnelling
+    or incomplete ionisation (outside the scope)."""
+    m, VT = dev.m, dev.m["VT"]
+    semi, metal, ins = dev.kind == K_SEMI, dev.kind == K_METAL, dev.kind == K_INS
+    n, p = dev.carriers(psi, phin, phip)
+    nan = np.full(dev.N, np.nan)
+    if which == 0:
+        return psi.copy()
+    if which in (1, 2):
+        return n if which == 1 else p
+    if which == 3:
+        return np.where(metal, nan, -psi - m["chi"] - dev.bgn * VT)
+    if which == 4:
+        return np.where(metal, nan, -psi - m["chi"] - m["Eg"] + dev.bgn * VT)
+    if which in (5, 6):
+        return np.where(ins, nan, -(phin if which == 5 else phip) - dev.Cref)
+    if which == 13:
+        return dev.kind.astype(np.float64)
+    if which == 14:
+        return dev.Nd - dev.Na
+    if which in (15, 16):
+        return (phin if which == 15 else phip).copy()
+    if which == 17:
+        return dev.Nf.copy()
+    if which == 18:
+        return dev.con_mask.astype(np.float64)
+    if which == 19:
+        return dev.bgn * VT * 2.0
+    if 20 <= which <= 25:
+        Jn, Jp, _ = node_field_current(dev, psi, n, p, fmu)
+        J = Jn if which < 23 else Jp
+        return np.where(semi, J[(which - 20) % 3], 0.0)
+    if which == 26:
+        return np.where(semi, dev.niK, 0.0)
+    if which == 27:
+        return np.zeros(dev.N)
+    if which in (28, 29):
+        return np.where(semi, -(dev.qcn if which == 28 else dev.qcp) * VT, nan)
+    if which == 30:
+        return np.where(semi, wall_distance(dev), nan)
+    if which in (31, 32, 34):
+        return np.where(semi, 0.0, nan)
+    if which == 33:
+        g = np.zeros(dev.N) if gii is None else gii
+        return np.where(semi, g / dev.volume(), 0.0)
+    if which == 35:
+        return np.where(semi, dev.Nd - dev.Na, 0.0)
+    raise ValueError(f"readback {which} is not a state quantity")
