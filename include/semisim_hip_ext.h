@@ -118,6 +118,9 @@ double sdx_test_bicgstab(const double *b, double *x, int maxit, double rtol, dou
 void   sdx_test_currents(const double *Va, double *I, double *S, double *In);
 double sdx_test_cont_update(int hole, const double *u, const double *sc, double *dq, double lo, double hi);
 void   sdx_test_ii_update(double *gii);             /* impact-ionisation generation, 1/s per node */
+void   sdx_test_aa_step(const double *x3, double *g3, int reset);
+                                                   /* one Anderson step: x3, g3 the 3 blocks psi|phin|phip,
+                                                      g3 overwritten by the mixed iterate; reset clears the history */
 int    sdx_test_maj_flags(int hole, const double *u, const double *sc, unsigned char *f0, unsigned char *f1);
                                                    /* majority flags before/after the anchoring closure */
 

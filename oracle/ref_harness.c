@@ -132,3 +132,13 @@ void refh_ii_update(double *gii){
 void refh_get_qd(double *out){ memcpy(out,gdev.qd,NN(&gdev)*8); }
 /* MLDA confinement terms of ln Nc, ln Nv (walls_tensor with models bit 3) */
 void refh_get_qc(double *qcn, double *qcp){ memcpy(qcn,gdev.qcn,NN(&gdev)*8); memcpy(qcp,gdev.qcp,NN(&gdev)*8); }
+/* Anderson mixing (aa_step :3708-3757) on the prepared device: reset the
+   history; one step with the 3 blocks of x (psi, phi_n, phi_p) and g (their
+   images, overwritten with the mixed iterate), N doubles each */
+void refh_aa_reset(void){ aa_reset(&gdev); }
+void refh_aa_step(const double *x3, double *g3){
+    size_t N=NN(&gdev);
+    double *xb[3]={(double*)x3,(double*)x3+N,(double*)x3+2*N}, *gb[3]={g3,g3+N,g3+2*N};
+    aa_step(&gdev,xb,gb);
+}
+int refh_aa_m(void){ return gdev.aa_m; }

@@ -15,7 +15,7 @@ reference function it restates.  Pinning (see tests/test_oracle_*.py):
   * prepare, poisson_eval, face_mobility, cont_assemble, matvec,
     mg_hierarchy/galerkin/restrict/prolong, terminal_currents, cont_update,
     ii_update, walls (interface distance, MLDA), face_mobility with Lombardi,
-    charge_partition, terminal_charges, readback:
+    charge_partition, terminal_charges, readback, Anderson:
     compared with the reference harness on small devices (golden fixtures
     tests/golden/*.npz written by tests/golden/make_golden.py) and with
     invariants (Poisson Jacobian symmetric on free rows, row sums of the
@@ -960,3 +960,86 @@ def readback(dev, which, psi, phin, phip, fmu=None, gii=None):
     if which == 35:
         return np.where(semi, dev.Nd - dev.Na, 0.0)
     raise ValueError(f"readback {which} is not a state quantity")
+
+
+# ------------------------------------------------------------ Anderson mixing
+class Anderson:
+    """Anderson mixing of the Gummel map, aa_step semiconductor_3d.c:3709-3757
+    (reset :3695).  x, g: the three blocks (psi, phi_n, phi_p) stacked, 3N.
+    The history keeps FP32 differences dF, dG of the residual f = g - x and
+    of the image g over the last m steps (a ring), their Gram matrix in FP64;
+    the mixing solves the regularised normal equations (diagonal + 1e-9 trace)
+    by Gaussian elimination with partial pivoting and moves only the free
+    nodes: psi off every contact node, phi_n/phi_p on the nodes where the
+    carrier is significant (aa_sig :3689-3694)."""
+
+    def __init__(self, dev, n, p, m=12):
+        N = dev.N
+        self.m, self.N = m, N
+        te = np.zeros(self.N, bool)
+        for c, cn in enumerate(dev.contacts):
+            if cn.get("te", 0):
+                te |= dev.con_mask == c + 1
+        semi = dev.kind == K_SEMI
+        carfix = (dev.con_mask != 0) & ~(te & semi)                      # car_fix :796-799
+        base = semi & ~carfix
+        tot = n + p + np.abs(dev.Nd - dev.Na)
+        sig_n = base & (n > 1e-6 * tot)
+        sig_p = base & (p > 1e-6 * tot)
+        # the residual's mask (psi everywhere) and the mixing's mask
+        self.fmask = np.concatenate([np.ones(N, bool), sig_n, sig_p])
+        self.mix = np.concatenate([dev.con_mask == 0, sig_n & ~carfix, sig_p & ~carfix])
+        self.reset()
+
+    def reset(self):
+        self.n = self.pos = self.have = 0
+        self.dF = [None] * self.m
+        self.dG = [None] * self.m
+        self.gram = np.zeros((self.m, self.m))
+        self.f = self.gp = None
+
+    def step(self, x, g):
+        """One step: returns the mixed iterate (the new g)."""
+        M = self.m
+        f = np.where(self.fmask, g - x, 0.0)
+        if M > 0 and self.have:
+            s = self.pos
+            self.dF[s] = (f - self.f).astype(np.float32)
+            self.dG[s] = (g - self.gp).astype(np.float32)
+            if self.n < M:
+                self.n += 1
+            dFs = self.dF[s].astype(np.float64)
+            for i in range(self.n):
+                v = float(np.dot(self.dF[i].astype(np.float64), dFs))
+                self.gram[i, s] = self.gram[s, i] = v
+            self.pos = (s + 1) % M
+        self.f, self.gp = f, g.copy()
+        self.have = 1
+        m = self.n
+        if m == 0:
+            return g.copy()
+        A = self.gram[:m, :m].copy()
+        b = np.array([np.dot(self.dF[i].astype(np.float64), f) for i in range(m)])
+        A[np.diag_indices(m)] += 1e-9 * np.trace(A) + 1e-300
+        for c in range(m):                                   # elimination, partial pivoting
+            pv = c + int(np.argmax(np.abs(A[c:, c])))
+            if abs(A[pv, c]) <= abs(A[c, c]):
+                pv = c
+            if pv != c:
+                A[[c, pv]] = A[[pv, c]]
+                b[[c, pv]] = b[[pv, c]]
+            for r in range(c + 1, m):
+                fct = A[r, c] / A[c, c]
+                A[r, c:] -= fct * A[c, c:]
+                b[r] -= fct * b[c]
+        gam = np.zeros(m)
+        for r in range(m - 1, -1, -1):
+            gam[r] = (b[r] - np.dot(A[r, r + 1:], gam[r + 1:])) / A[r, r]
+        if np.any(np.isnan(gam)):
+            return g.copy()
+        out = g.copy()
+        corr = np.zeros_like(g)
+        for i in range(m):
+            corr += gam[i] * self.dG[i].astype(np.float64)
+        out[self.mix] -= corr[self.mix]
+        return out

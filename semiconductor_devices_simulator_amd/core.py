@@ -130,6 +130,7 @@ EXT_SIGNATURES = [
     ("sdx_test_currents", None, [DP, DP, DP, DP]),
     ("sdx_test_cont_update", D, [I, DP, DP, DP, D, D]),
     ("sdx_test_ii_update", None, [DP]),
+    ("sdx_test_aa_step", None, [DP, DP, I]),
     ("sdx_test_maj_flags", I, [I, DP, DP, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(ctypes.c_ubyte)]),
 ]
 

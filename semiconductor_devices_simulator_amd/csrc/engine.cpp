@@ -2059,6 +2059,13 @@ void sdx_test_ii_update(double *gii) {
     G.models = m;
     eng_download(gii, G.d_gii, G.N);
 }
+void sdx_test_aa_step(const double *x3, double *g3, int reset) {
+    const int N = G.N;
+    if (reset) aa_reset();
+    for (int a = 0; a < 3; a++) { eng_upload(G.d_h1[a], x3 + (size_t)a * N, N); eng_upload(G.d_h2[a], g3 + (size_t)a * N, N); }
+    aa_step(G.d_h1, G.d_h2);
+    for (int a = 0; a < 3; a++) eng_download(g3 + (size_t)a * N, G.d_h2[a], N);
+}
 double sdx_test_cont_update(int hole, const double *u, const double *sc, double *dq, double lo, double hi) {
     eng_upload(G.d_sol, u, G.N); eng_upload(G.d_sc, sc, G.N);
     ph_cont_update(G.st, eng_view(), hole, G.d_sol, G.d_sc, G.d_wk[0], lo, hi, G.rb, S_MAX1);

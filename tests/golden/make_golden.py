@@ -42,6 +42,7 @@ def load_harness():
         "refh_bgn_half": (D, [I, D]), "refh_recomb": (None, [I, D, D, D, D, DP]), "refh_mat": (None, [I, DP]),
         "refh_neutral_psi": (D, [ctypes.c_longlong, D]), "refh_set_fmu": (None, [DP]),
         "refh_apply_bc": (None, [DP]), "refh_ii_update": (None, [DP]), "refh_get_qd": (None, [DP]), "refh_get_qc": (None, [DP, DP]),
+        "refh_aa_reset": (None, []), "refh_aa_step": (None, [DP, DP]), "refh_aa_m": (I, []),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -370,8 +371,52 @@ def lombardi_fixture(lib, spec, name, Va, rng, perturb=1e-3):
     print("wrote", name, N, "nodes; nodes within 10 nm of an interface", int((qd < 1e-8).sum()))
 
 
+def anderson_fixture(lib, spec, name, Va, rng, steps=8):
+    """The reference's Anderson mixing on a prepared MOSFET state: a sequence
+    of (x, g) pairs from a synthetic contraction toward a perturbed state
+    (x_{k+1} = the mixed iterate of step k), with every step's inputs and
+    mixed output, so that a restatement can replay the same history."""
+    cids = devices.load_device(lib, spec)
+    for c, v in enumerate(Va):
+        lib.api3_set_contact_V(cids[c], v)
+    assert lib.api3_solve() == 1
+    N = spec.N
+    st = [np.zeros(N) for _ in range(3)]
+    lib.api3_get_state(*(core.dptr(a) for a in st))
+    assert lib.refh_prepare() == 1
+    lib.refh_set_raw_state(*(core.dptr(a) for a in st))
+    kind = np.zeros(N, np.uint8)
+    lib.refh_get_kind(kind.ctypes.data)
+    cm = np.zeros(N, np.int32)
+    lib.refh_get_conmask(core.iptr(cm))
+    free = np.tile((cm == 0) & (kind == 0), 3)
+    x = np.concatenate(st)
+    xs = x + 1e-3 * free * rng.standard_normal(3 * N)            # the fixed point of the map
+    B = 0.02 * rng.standard_normal(3 * N)
+    X, Gin, Gout = [], [], []
+    lib.refh_aa_reset()
+    for _ in range(steps):
+        g = xs + 0.8 * (x - xs) + B * (x - xs) ** 2 * free          # a mildly nonlinear contraction
+        X.append(x.copy())
+        Gin.append(g.copy())
+        lib.refh_aa_step(core.dptr(x), core.dptr(g))
+        Gout.append(g.copy())
+        x = g
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), xs=spec.xs, ys=spec.ys, zs=spec.zs, Nd=spec.Nd,
+                        Na=spec.Na, models=spec.models, tau=np.asarray(spec.tau if spec.tau else (0.0, 0.0)),
+                        Va=np.asarray(Va, np.float64), contacts=json.dumps(contacts_of(spec)), psi=st[0],
+                        phin=st[1], phip=st[2], X=np.array(X), Gin=np.array(Gin), Gout=np.array(Gout),
+                        aa_m=lib.refh_aa_m())
+    print("wrote", name, "steps", steps, "depth", lib.refh_aa_m(),
+          "mixing moved g by", [float(np.max(np.abs(a - b))) for a, b in zip(Gout, Gin)])
+
+
 def main():
     lib = load_harness()
+    if "--anderson" in sys.argv:       # Anderson mixing of the Gummel map
+        anderson_fixture(lib, devices.mosfet(12, 8, 6), "aa_mosfet_12x8x6", [0.0, 0.5, 0.8, 0.0],
+                         np.random.default_rng(5151))
+        return
     if "--mlda" in sys.argv:           # quantum confinement (models bit 3)
         rng = np.random.default_rng(4444)
         for shape, Va in (((12, 8, 6), [0.0, 0.5, 0.8, 0.0]), ((20, 14, 9), [0.0, 1.0, 1.1, 0.0])):
