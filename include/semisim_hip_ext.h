@@ -118,6 +118,10 @@ double sdx_test_bicgstab(const double *b, double *x, int maxit, double rtol, dou
 void   sdx_test_currents(const double *Va, double *I, double *S, double *In);
 double sdx_test_cont_update(int hole, const double *u, const double *sc, double *dq, double lo, double hi);
 void   sdx_test_ii_update(double *gii);             /* impact-ionisation generation, 1/s per node */
+int    sdx_test_tun_paths(int *ncol, int *pk, int *pc, int *pc0, int *pcn, double *pA, int *ck, double *cl,
+                          double *ch);              /* gate tunnelling paths and supply columns */
+int    sdx_test_tun(const double *Va, int hole, double *cK, double *pG, double *tk, double *tg);
+                                                   /* one tunnelling update + the gathered sink/source */
 void   sdx_test_aa_step(const double *x3, double *g3, int reset);
                                                    /* one Anderson step: x3, g3 the 3 blocks psi|phin|phip,
                                                       g3 overwritten by the mixed iterate; reset clears the history */

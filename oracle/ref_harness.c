@@ -142,3 +142,21 @@ void refh_aa_step(const double *x3, double *g3){
     aa_step(&gdev,xb,gb);
 }
 int refh_aa_m(void){ return gdev.aa_m; }
+/* gate tunnelling (tun_build :1213-1247, tun_update/tun_scatter :3368-3380):
+   the paths of the prepared device and one update at the injected state */
+int refh_tun_paths(int *ncol, int *pk, int *pc, int *pc0, int *pcn, double *pA, int *ck, double *cl, double *ch){
+    if(ncol) *ncol=gdev.n_col;
+    for(int p=0;p<gdev.n_tp;p++){
+        if(pk) pk[p]=(int)gdev.tp_k[p]; if(pc) pc[p]=gdev.tp_c[p]; if(pc0) pc0[p]=gdev.tp_c0[p];
+        if(pcn) pcn[p]=gdev.tp_cn[p]; if(pA) pA[p]=gdev.tp_A[p];
+    }
+    for(int q=0;q<gdev.n_col;q++){ if(ck) ck[q]=(int)gdev.col_k[q]; if(cl) cl[q]=gdev.col_l[q]; if(ch) ch[q]=gdev.col_h[q]; }
+    return gdev.n_tp;
+}
+void refh_tun(const double *Va, int hole, double *cK, double *pG, double *tk, double *tg){
+    size_t N=NN(&gdev);
+    tun_update(&gdev,Va);
+    tun_scatter(&gdev,hole);
+    memcpy(cK,gdev.col_K,(size_t)2*gdev.n_col*8); memcpy(pG,gdev.tp_G,(size_t)2*gdev.n_tp*8);
+    memcpy(tk,gdev.tk,N*8); memcpy(tg,gdev.tg,N*8);
+}

@@ -15,7 +15,8 @@ reference function it restates.  Pinning (see tests/test_oracle_*.py):
   * prepare, poisson_eval, face_mobility, cont_assemble, matvec,
     mg_hierarchy/galerkin/restrict/prolong, terminal_currents, cont_update,
     ii_update, walls (interface distance, MLDA), face_mobility with Lombardi,
-    charge_partition, terminal_charges, readback, Anderson:
+    charge_partition, terminal_charges, readback, Anderson, tun_paths,
+    tun_update, tun_gather, tun_gate_currents:
     compared with the reference harness on small devices (golden fixtures
     tests/golden/*.npz written by tests/golden/make_golden.py) and with
     invariants (Poisson Jacobian symmetric on free rows, row sums of the
@@ -1043,3 +1044,205 @@ class Anderson:
             corr += gam[i] * self.dG[i].astype(np.float64)
         out[self.mix] -= corr[self.mix]
         return out
+
+
+# ------------------------------------------------------------ gate tunnelling
+HPLANCK = 6.62607015e-34
+TUN_COL, TUN_NE, TUN_EMAX = 5e-9, 64, 3.0
+# insulators of a gate stack (mat_init semiconductor_3d.c:599-614): eps_r,
+# affinity (eV), gap (eV), tunnelling masses of electrons/holes (m0)
+INSULATORS = {8: dict(eps_r=3.9, chi=0.95, Eg=9.0, mt_e=0.42, mt_h=0.33),       # SiO2
+              9: dict(eps_r=9.0, chi=1.35, Eg=6.8, mt_e=0.35, mt_h=0.35),       # Al2O3
+              10: dict(eps_r=22.0, chi=2.55, Eg=5.8, mt_e=0.18, mt_h=0.20),     # HfO2
+              11: dict(eps_r=7.5, chi=2.05, Eg=5.1, mt_e=0.50, mt_h=0.50)}      # Si3N4
+SI_SUPPLY_MASS = (0.50, 0.50)                                                    # ms_e, ms_h of Si (:458)
+
+
+def tun_paths(dev, stacks):
+    """Paths of the Robin gates with a layer stack (tun_build :1238-1244,
+    tun_push :1172-1210): one per gate node in the order of the gate nodes,
+    each with its supply column, the semiconductor nodes inward from the
+    interface over TUN_COL (stopping at contacts and at the domain edge).
+    stacks: {contact index: [(insulator id, thickness nm), ...]}."""
+    P = dict(pk=[], pc=[], pA=[], pc0=[], pcn=[], ck=[], cl=[], ch=[], layers=[])
+    for c, cn in enumerate(dev.contacts):
+        if cn["bc"] != 2 or cn["face"] > 5 or not stacks.get(c):
+            continue
+        lay = [(INSULATORS[m], t * 1e-9) for m, t in stacks[c]]
+        face = cn["face"]
+        inward, ax = face ^ 1, face // 2
+        for k, area in dev.contact_nodes(cn):
+            if dev.gate_mask[k] != c + 1 or dev.con_mask[k]:
+                continue
+            P["pk"].append(k); P["pc"].append(c); P["pA"].append(area); P["layers"].append(lay)
+            P["pc0"].append(len(P["ck"]))
+            cnt, cur, depth = 0, k, 0.0
+            while True:
+                l, r = divmod(cur, dev.Nx * dev.Ny)
+                j, i = divmod(r, dev.Nx)
+                L = (dev.Lx[i], dev.Ly[j], dev.Lz[l])[ax]
+                nb = _nbr(dev, i, j, l, inward)
+                more = nb is not None and dev.kind[nb[0]] == K_SEMI and not dev.con_mask[nb[0]]
+                P["ck"].append(cur); P["cl"].append(L); P["ch"].append(nb[1] if more else 0.0)
+                cnt += 1
+                depth += L
+                if not more or depth >= TUN_COL:
+                    break
+                cur = nb[0]
+            P["pcn"].append(cnt)
+    for key in ("pk", "pc", "pc0", "pcn", "ck"):
+        P[key] = np.asarray(P[key], np.int64)
+    for key in ("pA", "cl", "ch"):
+        P[key] = np.asarray(P[key], np.float64)
+    return P
+
+
+def _softplus(x):
+    return x + math.log1p(math.exp(-x)) if x > 35.0 else math.log1p(math.exp(x))
+
+
+def tun_wkb(m, L, u0, u1):
+    """2 Int kappa dx over a layer of length L whose barrier above the carrier
+    energy runs linearly from u0 to u1 eV (wkb_seg :3262-3270)."""
+    a, b = max(u0, 0.0), max(u1, 0.0)
+    if a == 0.0 and b == 0.0:
+        return 0.0
+    c = 2.0 * math.sqrt(2.0 * m * M0 * Q) / HBAR
+    du = u1 - u0
+    if abs(du) < 1e-9 * (a + b):
+        I = L * math.sqrt(0.5 * (a + b))
+    else:
+        I = L * (2.0 / 3.0) * (b * math.sqrt(b) - a * math.sqrt(a)) / du
+    return c * I
+
+
+def tun_stack(layers, psik, psim):
+    """Potentials along an all-insulator stack from the node (psik) to the
+    metal (psim): D = (psim - psik) / sum t/eps, psi linear in every layer
+    (tun_stack :3300-3312).  Returns (D, psis, [(mat, t, p0, dp)])."""
+    R = sum(t / (m["eps_r"] * EPS0) for m, t in layers)
+    D = (psim - psik) / R if R > 0 else 0.0
+    out, ps = [], psik
+    for m, t in layers:
+        dp = D * t / (m["eps_r"] * EPS0)
+        out.append((m, t, ps, dp))
+        ps += dp
+    return D, psik, out
+
+
+def tun_T(stack, E, hole):
+    """WKB transmission at energy E (eV) (tun_T :3275-3288)."""
+    s = 0.0
+    for m, t, p0, dp in stack:
+        mt = m["mt_h"] if hole else m["mt_e"]
+        ea, eb = -p0 - m["chi"], -(p0 + dp) - m["chi"]
+        if hole:
+            u0, u1 = E - (ea - m["Eg"]), E - (eb - m["Eg"])
+        else:
+            u0, u1 = ea - E, eb - E
+        s += tun_wkb(mt, t, u0, u1)
+        if s > 745.0:
+            return 0.0
+    return math.exp(-s)
+
+
+def tun_int(stack, E0, EF, kT, hole):
+    """Simpson's rule over 64 intervals of T(E) ln(1 + exp(+-(EF-E)/kT)) from
+    the band edge E0, over min(max(+-(EF-E0), 0) + 25 kT, 3 eV) (tun_int :3291-3303)."""
+    sg = -1.0 if hole else 1.0
+    span = min(max(sg * (EF - E0), 0.0) + 25.0 * kT, TUN_EMAX)
+    h = span / TUN_NE
+    s = 0.0
+    for q in range(TUN_NE + 1):
+        E = E0 + sg * q * h
+        w = 1.0 if q in (0, TUN_NE) else (4.0 if q & 1 else 2.0)
+        s += tun_T(stack, E, hole) * _softplus(sg * (EF - E) / kT) * w
+    return s * h / 3.0
+
+
+def tun_flux(dev, stack, Ecp, Evp, EFn, EFp, EFm):
+    """Electrons s->g, g->s, holes s->g, g->s (1/m^2 s) (tun_flux :3316-3322)."""
+    kT = dev.m["VT"]
+    h3 = HPLANCK ** 3
+    ce = 4.0 * math.pi * SI_SUPPLY_MASS[0] * M0 * Q * Q * kT / h3
+    ch = 4.0 * math.pi * SI_SUPPLY_MASS[1] * M0 * Q * Q * kT / h3
+    return (ce * tun_int(stack, Ecp, EFn, kT, 0), ce * tun_int(stack, Ecp, EFm, kT, 0),
+            ch * tun_int(stack, Evp, EFp, kT, 1), ch * tun_int(stack, Evp, EFm, kT, 1))
+
+
+def tun_update(dev, P, psi, phin, phip, n, p, Va):
+    """Per path (tun_path :3326-3366): the effective supply densities from the
+    column (contact theorem for carriers pressed against the interface by
+    the field, the local density extrapolated to the interface otherwise),
+    the four fluxes, and the linearised exchange: cK (escape per unit density
+    of each column entry, e/h interleaved) and pG (injection, e/h)."""
+    VT = dev.m["VT"]
+    cK = np.zeros(2 * len(P["ck"]))
+    pG = np.zeros(2 * len(P["pk"]))
+    eps_s = dev.m["eps_r"] * EPS0
+    for t in range(len(P["pk"])):
+        k, c = int(P["pk"][t]), int(P["pc"][t])
+        cn = dev.contacts[c]
+        pm = cn["pm"] if cn["pm"] > 0 else dev.Cref
+        D, psis, stack = tun_stack(P["layers"][t], psi[k], Va[c] + dev.Cref - pm)
+        q0, nq = int(P["pc0"][t]), int(P["pcn"][t])
+        Fs = D / eps_s
+        neff = [0.0, 0.0]
+        for hole in (0, 1):
+            sg = -1.0 if hole else 1.0
+            car = p if hole else n
+            if sg * Fs > 0:
+                P0, Fl = 0.0, sg * Fs
+                for q in range(q0, q0 + nq):
+                    kq = P["ck"][q]
+                    Fr = sg * (psi[kq] - psi[P["ck"][q + 1]]) / P["ch"][q] if (q < q0 + nq - 1 and P["ch"][q] > 0) else Fl
+                    w = max(0.5 * (Fl + Fr), 0.0) * P["cl"][q] / VT
+                    cK[2 * q + hole] = w
+                    P0 += w * car[kq]
+                    Fl = Fr
+                kl = P["ck"][q0 + nq - 1]
+                cK[2 * (q0 + nq - 1) + hole] += 1.0
+                P0 += car[kl]
+                neff[hole] = P0
+            else:
+                qc = (dev.qcp if hole else dev.qcn)[k]
+                neff[hole] = car[k] * math.exp(-qc + sg * (psis - psi[k]) / VT)
+                cK[2 * q0 + hole] = neff[hole] / max(car[k], NFLOOR)
+            if not neff[hole] > 1e-100:
+                neff[hole] = 1e-100
+        EFn, EFp, EFm = -phin[k] - dev.Cref, -phip[k] - dev.Cref, -Va[c] - dev.Cref
+        Ecp = EFn - VT * math.log(neff[0] / dev.m["Nc"])
+        Evp = EFp + VT * math.log(neff[1] / dev.m["Nv"])
+        F = tun_flux(dev, stack, Ecp, Evp, EFn, EFp, EFm)
+        A = P["pA"][t]
+        cK[2 * q0:2 * (q0 + nq):2] *= A * F[0] / neff[0]
+        cK[2 * q0 + 1:2 * (q0 + nq):2] *= A * F[2] / neff[1]
+        pG[2 * t], pG[2 * t + 1] = A * F[1], A * F[3]
+    return cK, pG
+
+
+def tun_gather(dev, P, cK, pG, hole):
+    """Per node sink tk (sum of the column entries' escape) and source tg (sum
+    of the injection of the paths starting there) (tun_scatter :3373-3380)."""
+    tk, tg = np.zeros(dev.N), np.zeros(dev.N)
+    np.add.at(tk, P["ck"], cK[hole::2])
+    np.add.at(tg, P["pk"], pG[hole::2])
+    return tk, tg
+
+
+def tun_gate_currents(dev, P, cK, pG, n, p):
+    """Gate current of every contact (A, into the device): electrons leaving
+    to the gate minus their injection, the reverse for holes, with the flux
+    sum S and the electron part (terminal_currents :3614-3621)."""
+    I, S, In = (np.zeros(len(dev.contacts)) for _ in range(3))
+    for t in range(len(P["pk"])):
+        q = slice(int(P["pc0"][t]), int(P["pc0"][t] + P["pcn"][t]))
+        ks = P["ck"][q]
+        se = float(np.sum(cK[0::2][q] * n[ks]))
+        sh = float(np.sum(cK[1::2][q] * p[ks]))
+        c = int(P["pc"][t])
+        Ie, Ih = Q * (se - pG[2 * t]), -Q * (sh - pG[2 * t + 1])
+        I[c] += Ie + Ih
+        S[c] += Q * (se + pG[2 * t] + sh + pG[2 * t + 1])
+        In[c] += Ie
+    return I, S, In

@@ -28,6 +28,8 @@ FP = ctypes.POINTER(ctypes.c_float)
 SIGNATURES = [
     ("api3_init", None, []),
     ("api3_add_material", I, [I, D]),
+    ("api3_set_gate_stack", I, [I, I, IP, DP]),
+    ("api3_get_ntp", I, [I]),
     ("api3_set_grid", None, [I, I, I, D, D, D]),
     ("api3_set_grid_coords", I, [I, I, I, DP, DP, DP]),
     ("api3_set_solver", None, [I, I, I, D, D, D, D]),
@@ -131,6 +133,8 @@ EXT_SIGNATURES = [
     ("sdx_test_cont_update", D, [I, DP, DP, DP, D, D]),
     ("sdx_test_ii_update", None, [DP]),
     ("sdx_test_aa_step", None, [DP, DP, I]),
+    ("sdx_test_tun_paths", I, [IP, IP, IP, IP, IP, DP, IP, DP, DP]),
+    ("sdx_test_tun", I, [DP, I, DP, DP, DP, DP]),
     ("sdx_test_maj_flags", I, [I, DP, DP, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(ctypes.c_ubyte)]),
 ]
 

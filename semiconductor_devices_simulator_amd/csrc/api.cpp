@@ -168,7 +168,10 @@ void api3_clear_contacts(void) { G.n_cons = 0; mark_changed(); }
 int api3_set_gate_stack(int cid, int nl, const int *mat, const double *t) {
     if (cid < 0 || cid >= G.n_cons || nl < 0 || nl > 6) return 0;
     for (int i = 0; i < nl; i++) if (mat[i] < 0 || mat[i] >= G.n_mats || !(t[i] > 0.)) return 0;
-    // the stack only matters for gate tunnelling (models bit 5), which this core rejects
+    HostCon &cn = G.cons[cid];
+    cn.snl = nl;
+    for (int i = 0; i < nl; i++) { cn.smat[i] = mat[i]; cn.st[i] = t[i]; }
+    mark_changed();
     return 1;
 }
 int api3_set_mesh_graph(int, int, const int *, const int *, const double *, const double *, const double *,
@@ -197,7 +200,11 @@ int api3_set_con_nodes(int cid, int n, const int *, const double *) {
     return 0;
 }
 void api3_set_walls(const double *) {}
-int api3_get_ntp(int) { return 0; }
+// tunnelling paths of the last prepared device (all, or those of contact cid)
+int api3_get_ntp(int cid) {
+    if (cid < 0) return (int)G.tp_k.size();
+    return cid < G.n_cons ? G.cons[cid].ntp : 0;
+}
 double api3_tun_eval(int, int, const int *, const double *, double, double, double, double, double, double *out) {
     unsupported("gate tunnelling");
     if (out) for (int i = 0; i < 4; i++) out[i] = NAN;
@@ -410,7 +417,11 @@ void api3_bulk_copy(int which, double *out) {
         return;
     }
     case 26: for (int k = 0; k < N; k++) out[k] = G.kind[k] == K_SEMI ? G.niK[k] : 0.; return;
-    case 27: for (int k = 0; k < N; k++) out[k] = 0.; return;
+    case 27:                       // gate tunnelling current density (A/m^2), 0 off the interface
+        sd_fill(G.st, G.d_wk[0], 0.0, N);
+        ph_tun_J(G.st, d, G.tun, G.d_wk[0]);
+        eng_download(out, G.d_wk[0], N);
+        return;
     case 28: case 29: {            // quantum potentials -qc VT (eV); 0 without MLDA
         const std::vector<double> &qc = which == 28 ? G.qcn : G.qcp;
         for (int k = 0; k < N; k++)

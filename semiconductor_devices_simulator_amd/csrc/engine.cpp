@@ -215,7 +215,8 @@ int eng_alloc(int Nx, int Ny, int Nz) {
                      &G.A.c, &G.A.xm, &G.A.xp, &G.A.ym, &G.A.yp, &G.A.zm, &G.A.zp,
                      &G.d_rhs, &G.d_sc, &G.d_dsc, &G.d_sol, &G.d_xsave, &G.d_tr, &G.d_Ft,
                      &G.d_xk, &G.d_dqn, &G.d_dqp, &G.d_qn0, &G.d_qp0,
-                     &G.d_Jx, &G.d_Jy, &G.d_Jz, &G.d_Rt, &G.d_mudn, &G.d_mudp, &G.d_qd};
+                     &G.d_Jx, &G.d_Jy, &G.d_Jz, &G.d_Rt, &G.d_mudn, &G.d_mudp, &G.d_qd,
+                     &G.d_tk, &G.d_tg, &G.d_qcn, &G.d_qcp};
     for (double **p : nd) *p = dalloc<double>(N);
     for (int q = 0; q < 7; q++) G.d_wk[q] = dalloc<double>(N);
     for (int a = 0; a < 3; a++) { G.d_h1[a] = dalloc<double>(N); G.d_h2[a] = dalloc<double>(N); G.d_e2[a] = dalloc<double>(N); }
@@ -271,6 +272,8 @@ DevView eng_view() {
     d.mu_n = G.d_mun; d.mu_p = G.d_mup; d.fmu = G.d_fmu;
     d.gii = (G.models & 256) ? G.d_gii : nullptr;
     d.qd = (G.models & 16) ? G.d_qd : nullptr;
+    d.tk = G.tun.np ? G.d_tk : nullptr;
+    d.tg = G.tun.np ? G.d_tg : nullptr;
     d.mats = G.d_mats; d.cons = G.d_cons;
     d.models = G.models; d.caxis = G.caxis; d.ref_mat = G.ref_mat; d.Cref = G.Cref;
     return d;
@@ -420,11 +423,123 @@ void eng_walls(std::vector<double> &qd, std::vector<double> *qcn, std::vector<do
 }
 void eng_wall_distance(std::vector<double> &qd) { eng_walls(qd, nullptr, nullptr); }
 
+// ------------------------------------------------------ gate tunnelling
+// Paths of the Robin (oxide-capacitor) gates with a layer stack: one per gate
+// node, in the order of the gate nodes (tun_build :1238-1244, tun_push
+// :1172-1210), each with its supply column: the semiconductor nodes inward
+// from the interface over 5 nm, stopping at contacts and non-semiconductors.
+// Paths through meshed insulator layers to a metal gate (:1214-1237) are not
+// built: such a device is rejected.
+#define TUN_COL 5e-9
+static int tun_build() {
+    G.tp_k.clear(); G.tp_c.clear(); G.tp_c0.clear(); G.tp_cn.clear(); G.col_k.clear();
+    G.tp_A.clear(); G.col_l.clear(); G.col_h.clear();
+    for (int c = 0; c < G.n_cons; c++) G.cons[c].ntp = 0;
+    if (!(G.models & 32)) return 1;
+    for (int k = 0; k < G.N; k++) {
+        if (G.kind[k] != K_SEMI || G.con_mask[k]) continue;
+        int i, j, l;
+        host_unidx(k, i, j, l);
+        for (int dir = 0; dir < 6; dir++) {
+            int kb; double h, ar;
+            if (host_nbr(i, j, l, k, dir, kb, h, ar) && G.kind[kb] == K_INS && !G.con_mask[kb]) {
+                fprintf(stderr, "semisim-hip: gate tunnelling through meshed insulator layers is outside this core: "
+                                "not solved\n");
+                return 0;
+            }
+        }
+    }
+    for (int c = 0; c < G.n_cons; c++) {
+        HostCon &cn = G.cons[c];
+        if (cn.bc != 2 || cn.face > 5 || cn.snl < 1) continue;
+        const int dir = cn.face, in = dir ^ 1, ax = dir / 2;
+        for_con_nodes(cn, [&](int, int, int, int k, double area) {
+            if (G.gate_mask[k] != c + 1 || G.con_mask[k]) return;
+            G.tp_k.push_back(k); G.tp_c.push_back(c); G.tp_A.push_back(area);
+            G.tp_c0.push_back((int)G.col_k.size());
+            int cnt = 0, cur = k;
+            double depth = 0.;
+            for (;;) {
+                int ii, jj, ll;
+                host_unidx(cur, ii, jj, ll);
+                double L = ax == 0 ? G.Lx[ii] : (ax == 1 ? G.Ly[jj] : G.Lz[ll]);
+                int kn; double h, ar;
+                bool more = host_nbr(ii, jj, ll, cur, in, kn, h, ar) && G.kind[kn] == K_SEMI && !G.con_mask[kn];
+                G.col_k.push_back(cur); G.col_l.push_back(L); G.col_h.push_back(more ? h : 0.);
+                cnt++; depth += L;
+                if (!more || depth >= TUN_COL) break;
+                cur = kn;
+            }
+            G.tp_cn.push_back(cnt);
+            cn.ntp++;
+        });
+    }
+    if (G.verbose && !G.tp_k.empty())
+        fprintf(stderr, "semisim-hip: %zu tunnelling paths, %zu column nodes\n", G.tp_k.size(), G.col_k.size());
+    return 1;
+}
+template <class T> static T *dput(const std::vector<T> &v) {
+    T *p = dalloc<T>(v.size());
+    if (p && !v.empty()) HC(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, G.st));
+    return p;
+}
+// device copies of the paths (reallocated per prepare, like the current
+// edges), the per-node gather lists, and the MLDA terms the supply reads
+static void tun_upload() {
+    TunDev &T = G.tun;
+    T = TunDev{};
+    const int np = (int)G.tp_k.size(), N = G.N;
+    if (np == 0) return;
+    std::vector<int> pnl(np), pm((size_t)np * SD_TUN_MAXL, 0), poff(G.n_cons + 1, 0);
+    std::vector<double> pt((size_t)np * SD_TUN_MAXL, 0.);
+    for (int p = 0; p < np; p++) {
+        const HostCon &cn = G.cons[G.tp_c[p]];
+        pnl[p] = cn.snl;
+        for (int i = 0; i < cn.snl; i++) { pm[(size_t)p * SD_TUN_MAXL + i] = cn.smat[i]; pt[(size_t)p * SD_TUN_MAXL + i] = cn.st[i]; }
+    }
+    // paths are built contact by contact: contact c owns poff[c]..poff[c+1]
+    for (int p = 0; p < np; p++) poff[G.tp_c[p] + 1]++;
+    for (int c = 0; c < G.n_cons; c++) poff[c + 1] += poff[c];
+    // touched nodes (ascending) with their column entries and path starts,
+    // both in ascending (= the reference's summation) order
+    std::vector<int> tn(G.col_k);
+    std::sort(tn.begin(), tn.end());
+    tn.erase(std::unique(tn.begin(), tn.end()), tn.end());
+    const int ntn = (int)tn.size();
+    std::unordered_map<int, int> at;
+    at.reserve((size_t)ntn * 2);
+    for (int t = 0; t < ntn; t++) at[tn[t]] = t;
+    std::vector<std::vector<int>> eK(ntn), eG(ntn);
+    for (int q = 0; q < (int)G.col_k.size(); q++) eK[at[G.col_k[q]]].push_back(q);
+    for (int p = 0; p < np; p++) eG[at[G.tp_k[p]]].push_back(p);
+    std::vector<int> offK(ntn + 1, 0), offG(ntn + 1, 0), entK, entG;
+    for (int t = 0; t < ntn; t++) {
+        entK.insert(entK.end(), eK[t].begin(), eK[t].end()); offK[t + 1] = (int)entK.size();
+        entG.insert(entG.end(), eG[t].begin(), eG[t].end()); offG[t + 1] = (int)entG.size();
+    }
+    T.np = np; T.ntn = ntn;
+    T.pk = dput(G.tp_k); T.pc = dput(G.tp_c); T.pnl = dput(pnl); T.pc0 = dput(G.tp_c0); T.pcn = dput(G.tp_cn);
+    T.pm = dput(pm); T.pA = dput(G.tp_A); T.pt = dput(pt);
+    T.ck = dput(G.col_k); T.cl = dput(G.col_l); T.ch = dput(G.col_h);
+    T.cK = dalloc<double>(2 * G.col_k.size()); T.pG = dalloc<double>(2 * (size_t)np);
+    T.tn = dput(tn); T.offK = dput(offK); T.entK = dput(entK); T.offG = dput(offG); T.entG = dput(entG);
+    G.d_tp_off = dput(poff);
+    if (G.models & 8) {
+        HC(hipMemcpyAsync(G.d_qcn, G.qcn.data(), (size_t)N * 8, hipMemcpyHostToDevice, G.st));
+        HC(hipMemcpyAsync(G.d_qcp, G.qcp.data(), (size_t)N * 8, hipMemcpyHostToDevice, G.st));
+        T.qcn = G.d_qcn; T.qcp = G.d_qcp;
+    }
+    HC(hipMemsetAsync(T.cK, 0, 2 * G.col_k.size() * 8, G.st));
+    HC(hipMemsetAsync(T.pG, 0, 2 * (size_t)np * 8, G.st));
+    HC(hipMemsetAsync(G.d_tk, 0, (size_t)N * 8, G.st));
+    HC(hipMemsetAsync(G.d_tg, 0, (size_t)N * 8, G.st));
+    if (g_err) G.tun = TunDev{};
+}
+
 int eng_prepare() {
     if (!G.allocated || G.n_mats < 1) return 0;
     {   // physics outside this core is rejected where it would take effect, never approximated
         const char *why = nullptr;
-        if (G.models & 32) why = "gate tunnelling (models bit 5)";
         for (int i = 0; i < G.n_mats && !why; i++)
             if ((G.models & 16) && G.mats[i].id == 4) why = "Coulomb surface mobility of 4H-SiC (models bit 4)";
         for (int c = 0; c < G.n_cons && !why; c++)
@@ -472,6 +587,7 @@ int eng_prepare() {
         }
     }
     for (int k = 0; k < N; k++) if (G.gate_mask[k] && (G.con_mask[k] || G.robA[k] == 0.)) G.gate_mask[k] = 0;
+    if (!tun_build()) return 0;
     // box contacts: area = interface with non-contact semiconductor
     for (int c = 0; c < G.n_cons; c++) {
         HostCon &cn = G.cons[c];
@@ -590,6 +706,7 @@ int eng_prepare() {
         HC(hipMemcpyAsync(G.d_ear, ear.data(), ear.size() * 8, hipMemcpyHostToDevice, s));
     }
     HC(hipMemcpyAsync(G.d_eoff, eoff.data(), eoff.size() * 4, hipMemcpyHostToDevice, s));
+    tun_upload();
     eng_sync();
     G.dirty = 0;
     return eng_ok();
@@ -1235,6 +1352,8 @@ static int float_fix_host(int hole, int m) {
                 double vol = G.Lx[i] * G.Ly[j] * G.Lz[l];
                 Gv += vol * Rr;
                 dG += vol * (hole ? dRp : dRn) * c;
+                Gv += r[23] * c - r[24] - r[22];     // tunnelling sink/source, impact ionisation
+                dG += r[23] * c;
             }
             if (!(dG > 0.)) break;
             double ds = -Gv / dG;
@@ -1266,6 +1385,7 @@ static int solve_cont(int hole, double *dq, double lo, double hi, double *dmax) 
     const int N = G.N;
     hipStream_t s = G.st;
     DevView d = eng_view();
+    if (G.tun.np) ph_tun_gather(s, G.tun, hole, G.d_tk, G.d_tg);
     ph_cont(s, d, hole, G.A, G.d_rhs, G.d_sc, G.d_dsc);
     G.mg_w = G.d_dsc;
     G.mg_qsign = hole ? -1 : 1;
@@ -1287,6 +1407,8 @@ static int solve_cont(int hole, double *dq, double lo, double hi, double *dmax) 
 static void terminal_currents(double *I, double *Sv) {
     CurEdges e;
     e.k = G.d_ek; e.kb = G.d_ekb; e.h = G.d_eh; e.ar = G.d_ear; e.off = G.d_eoff; e.ncon = G.n_cons;
+    e.poff = G.tun.np ? G.d_tp_off : nullptr;
+    e.tun = G.tun;
     ph_currents(G.st, eng_view(), e, G.rb);
     rd(S_CUR, S_CUR + 3 * std::max(1, G.n_cons));
     for (int c = 0; c < G.n_cons; c++) {
@@ -1382,6 +1504,7 @@ static int gummel(const double *Va, int maxit, double gtol, int *iters, double *
         sd_copy(s, G.d_qp0, G.d_phip, N);
         eng_rebuild_np();
         eng_upd_mob();
+        if (G.tun.np) ph_tun_update(s, d, G.tun);
         int iiok = 1;
         if (iion) {
             ph_ii_update(s, d);
@@ -2001,6 +2124,7 @@ void sdx_test_upd_mob(double *fmu6) {
 }
 void sdx_test_set_fmu(const double *fmu6) { eng_upload(G.d_fmu, fmu6, 6 * (size_t)G.N); }
 void sdx_test_cont(int hole, double *A7, double *rhs, double *sc, double *dsc) {
+    if (G.tun.np) ph_tun_gather(G.st, G.tun, hole, G.d_tk, G.d_tg);
     ph_cont(G.st, eng_view(), hole, G.A, G.d_rhs, G.d_sc, G.d_dsc);
     get_A7(A7);
     eng_download(rhs, G.d_rhs, G.N); eng_download(sc, G.d_sc, G.N); eng_download(dsc, G.d_dsc, G.N);
@@ -2058,6 +2182,32 @@ void sdx_test_ii_update(double *gii) {
     ph_ii_update(G.st, eng_view());
     G.models = m;
     eng_download(gii, G.d_gii, G.N);
+}
+// gate tunnelling paths of the prepared device: returns the number of paths,
+// *ncol the number of column entries; arrays (any may be null): per path
+// node, contact, first column entry, column length, area; per column entry
+// node, cell length, spacing
+int sdx_test_tun_paths(int *ncol, int *pk, int *pc, int *pc0, int *pcn, double *pA, int *ck, double *cl, double *ch) {
+    if (ncol) *ncol = (int)G.col_k.size();
+    auto put = [](auto *dst, const auto &v) { if (dst) std::copy(v.begin(), v.end(), dst); };
+    put(pk, G.tp_k); put(pc, G.tp_c); put(pc0, G.tp_c0); put(pcn, G.tp_cn); put(pA, G.tp_A);
+    put(ck, G.col_k); put(cl, G.col_l); put(ch, G.col_h);
+    return (int)G.tp_k.size();
+}
+// one tunnelling update at the injected state and contact voltages Va: the
+// linearised escape cK (2 per column entry) and injection pG (2 per path),
+// then the gathered sink tk and source tg of carrier `hole` (N each)
+int sdx_test_tun(const double *Va, int hole, double *cK, double *pG, double *tk, double *tg) {
+    eng_upload_cons(Va);
+    ph_tun_update(G.st, eng_view(), G.tun);
+    ph_tun_gather(G.st, G.tun, hole, G.d_tk, G.d_tg);
+    if (G.tun.np) {
+        eng_download(cK, G.tun.cK, 2 * G.col_k.size());
+        eng_download(pG, G.tun.pG, 2 * G.tp_k.size());
+    }
+    eng_download(tk, G.d_tk, G.N);
+    eng_download(tg, G.d_tg, G.N);
+    return G.tun.np;
 }
 void sdx_test_aa_step(const double *x3, double *g3, int reset) {
     const int N = G.N;

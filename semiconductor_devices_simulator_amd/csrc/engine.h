@@ -16,6 +16,8 @@ struct HostCon {
     double V, phi_m, tox, qox;
     double Vprev, I, A, In, Iflr, Iraw;
     int te, ntp;
+    int snl, smat[6];                   // gate stack for tunnelling (api3_set_gate_stack), semiconductor side first
+    double st[6];
 };
 
 template <class T> struct Hier {
@@ -88,6 +90,14 @@ struct Engine {
     double *d_aaf, *d_aag;
     double *d_Jx, *d_Jy, *d_Jz, *d_Rt, *d_mudn, *d_mudp;
     double *d_qd = nullptr;            // distance to the nearest interface (Lombardi)
+    double *d_tk = nullptr, *d_tg = nullptr;     // gate tunnelling sink/source of the carrier being solved
+    double *d_qcn = nullptr, *d_qcp = nullptr;   // MLDA terms (tunnelling supply)
+    TunDev tun{};                      // gate tunnelling paths (np = 0: none)
+    int *d_tp_off = nullptr;           // paths of each contact (n_cons+1)
+    // host copies of the paths (test hooks): node, contact, first column entry,
+    // column length; column nodes, lengths, spacings
+    std::vector<int> tp_k, tp_c, tp_c0, tp_cn, col_k;
+    std::vector<double> tp_A, col_l, col_h;
     std::vector<double> qd;            // its host copy (source of the upload)
     std::vector<double> qcn, qcp;      // MLDA confinement terms of ln Nc, ln Nv (0 without MLDA)
     MatDev *d_mats = nullptr;

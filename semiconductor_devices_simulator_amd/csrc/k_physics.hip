@@ -230,6 +230,7 @@ __global__ void k_cont(DevView d, int hole, Sten7<double> A, double *rhs, double
     double vol = g.Lx[i] * g.Ly[j] * g.Lz[l];
     dg += vol * dR;
     b += vol * (dR * car[k] - R);
+    if (d.tk) { dg += d.tk[k]; b += d.tg[k]; }     // gate tunnelling: sink tk*c - tg
     if (d.gii) b += d.gii[k];
     double w = 1.0 / (dg * sk);
     A.c[k] = 1.0;
@@ -375,6 +376,200 @@ __device__ __forceinline__ void face_flux(const DevView &d, int ka, int kb, doub
     Jn = a1 - a2; Jp = b1 - b2; Sn = a1 + a2; Sp = b1 + b2;
 }
 
+// ------------------------------------------------------- gate tunnelling
+// Tsu-Esaki current with WKB transmission through the layer stack of each
+// path, semiconductor_3d.c:3230-3390.  The flux of a carrier between the
+// semiconductor and the gate metal is
+//     F = 4 pi m_s q^2 kT / h^3  Int T(E) ln(1 + exp((E_F - E)/kT)) dE
+// (1/m^2 s), integrated by Simpson's rule over 64 intervals from the
+// effective band edge up to min(max(E_F - E0, 0) + 25 kT, 3 eV) (holes
+// mirrored).  The supply of carriers pressed against the interface by the
+// field follows from the contact theorem summed over the path's column;
+// carriers pushed away use the local density extrapolated to the interface.
+#define SD_HBAR    1.054571817e-34
+#define SD_M0      9.1093837015e-31
+#define SD_HPLANCK 6.62607015e-34
+#define SD_TUN_NE   64
+#define SD_TUN_EMAX 3.0
+__device__ __forceinline__ double tun_softplus(double x) { return x > 35. ? x + log1p(exp(-x)) : log1p(exp(x)); }
+// 2 Int kappa dx over a layer of length L whose barrier above the carrier
+// energy runs linearly from u0 to u1 (eV); kappa = sqrt(2 m q u) / hbar
+__device__ __forceinline__ double tun_wkb(double m, double L, double u0, double u1) {
+    double a = u0 > 0. ? u0 : 0., b = u1 > 0. ? u1 : 0.;
+    if (a == 0. && b == 0.) return 0.;
+    double c = 2. * sqrt(2. * m * SD_M0 * SD_Q) / SD_HBAR, du = u1 - u0, I;
+    if (fabs(du) < 1e-9 * (a + b)) I = L * sqrt(0.5 * (a + b));
+    else I = L * (2. / 3.) * (b * sqrt(b) - a * sqrt(a)) / du;
+    return c * I;
+}
+struct TunStack {
+    int nl, mi[SD_TUN_MAXL];
+    double t[SD_TUN_MAXL], p0[SD_TUN_MAXL], dp[SD_TUN_MAXL];
+    double D, psis;
+};
+// potentials along the stack between the node at psik and the metal at
+// potential psim (= V + Cref - work function): D = (psim - psik) / sum t/eps,
+// psi linear in every layer (tun_stack :3300-3312)
+__device__ void tun_stack(const MatDev *mats, TunStack &P, int nl, const int *mi, const double *t, double psik,
+                          double psim) {
+    P.nl = nl;
+    double R = 0.;
+    for (int i = 0; i < nl; i++) { P.mi[i] = mi[i]; P.t[i] = t[i]; R += t[i] / (mats[mi[i]].eps_r * SD_EPS0); }
+    double ps = psik;
+    P.D = R > 0. ? (psim - psik) / R : 0.;
+    P.psis = psik;
+    for (int i = 0; i < nl; i++) {
+        P.p0[i] = ps; P.dp[i] = P.D * t[i] / (mats[mi[i]].eps_r * SD_EPS0); ps += P.dp[i];
+        if (i == 0 && !mats[mi[0]].insulator) P.psis = ps;
+    }
+}
+// WKB transmission at energy E (eV) through the insulating layers
+__device__ double tun_T(const MatDev *mats, const TunStack &P, double E, int hole) {
+    double s = 0.;
+    for (int i = 0; i < P.nl; i++) {
+        const MatDev &m = mats[P.mi[i]];
+        if (!m.insulator) continue;
+        double mt = hole ? m.mt_h : m.mt_e;
+        if (!(mt > 0.)) mt = 0.5;
+        double ea = -P.p0[i] - m.chi, eb = -(P.p0[i] + P.dp[i]) - m.chi;     // Ec of the layer at both ends
+        double u0, u1;
+        if (hole) { u0 = E - (ea - m.Eg); u1 = E - (eb - m.Eg); }
+        else { u0 = ea - E; u1 = eb - E; }
+        s += tun_wkb(mt, P.t[i], u0, u1);
+        if (s > 745.) return 0.;
+    }
+    return exp(-s);
+}
+// Int T(E) ln(1 + exp(+-(EF - E)/kT)) dE: the 65 Simpson nodes spread over
+// the lanes of one wavefront (lane 0 also takes the last); the sum is valid
+// in lane 0
+__device__ double tun_int_wave(const MatDev *mats, const TunStack &P, double E0, double EF, double kT, int hole,
+                               int lane) {
+    double sg = hole ? -1. : 1.;
+    double top = sg * (EF - E0);
+    if (top < 0.) top = 0.;
+    double span = top + 25. * kT;
+    if (span > SD_TUN_EMAX) span = SD_TUN_EMAX;
+    double h = span / SD_TUN_NE, s = 0.;
+    for (int q = lane; q <= SD_TUN_NE; q += 64) {
+        double E = E0 + sg * q * h;
+        double f = tun_T(mats, P, E, hole) * tun_softplus(sg * (EF - E) / kT);
+        s += f * (q == 0 || q == SD_TUN_NE ? 1. : (q & 1 ? 4. : 2.));
+    }
+    s = sd_wave_sum(s);
+    return s * h / 3.;
+}
+// the four fluxes (1/m^2 s): electrons s->g, g->s, holes s->g, g->s, for
+// effective band edges Ecp, Evp and Fermi levels EFn, EFp (semiconductor),
+// EFm (metal), in eV; valid in lane 0
+__device__ void tun_flux_wave(const MatDev *mats, const MatDev &ms, const TunStack &P, double Ecp, double Evp,
+                              double EFn, double EFp, double EFm, double *F, int lane) {
+    double kT = ms.VT, h3 = SD_HPLANCK * SD_HPLANCK * SD_HPLANCK;
+    double ce = 4. * M_PI * ms.ms_e * SD_M0 * SD_Q * SD_Q * kT / h3;
+    double chh = 4. * M_PI * ms.ms_h * SD_M0 * SD_Q * SD_Q * kT / h3;
+    F[0] = ce * tun_int_wave(mats, P, Ecp, EFn, kT, 0, lane);
+    F[1] = ce * tun_int_wave(mats, P, Ecp, EFm, kT, 0, lane);
+    F[2] = chh * tun_int_wave(mats, P, Evp, EFp, kT, 1, lane);
+    F[3] = chh * tun_int_wave(mats, P, Evp, EFm, kT, 1, lane);
+}
+// escape rates (1/s) of path p at the current densities (tun_escape :3382-3389)
+__device__ __forceinline__ void tun_escape(const DevView &d, const TunDev &T, int p, double &se, double &sh) {
+    double a = 0., b = 0.;
+    for (int q = T.pc0[p]; q < T.pc0[p] + T.pcn[p]; q++) {
+        int kq = T.ck[q];
+        a += T.cK[2 * q] * d.n[kq]; b += T.cK[2 * q + 1] * d.p[kq];
+    }
+    se = a; sh = b;
+}
+// one wavefront per path (tun_path :3326-3366): effective supply densities
+// from the column, the fluxes, and the linearised exchange: cK (escape per
+// unit density of each column node), pG (injection from the metal)
+__global__ void __launch_bounds__(SD_BLOCK) k_tun_path(DevView d, TunDev T) {
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * (SD_BLOCK / 64) + (threadIdx.x >> 6);
+    if (p >= T.np) return;                        // uniform over the wavefront
+    const int k = T.pk[p], c = T.pc[p];
+    const MatDev &ms = d.mats[d.mat[k]];
+    const double VT = ms.VT, A = T.pA[p];
+    TunStack P;
+    tun_stack(d.mats, P, T.pnl[p], T.pm + (size_t)p * SD_TUN_MAXL, T.pt + (size_t)p * SD_TUN_MAXL, d.psi[k],
+              d.cons[c].VGp);
+    const int q0 = T.pc0[p], nq = T.pcn[p];
+    const double Fs = P.D / (ms.eps_r * SD_EPS0);  // surface field, + pushes electrons to the wall
+    double neff[2];
+    for (int hole = 0; hole < 2; hole++) {
+        const double sg = hole ? -1. : 1., Fsc = sg * Fs;
+        const double *car = hole ? d.p : d.n;
+        double *K = T.cK + 2 * q0 + hole;        // column weights, written by lane 0
+        if (Fsc > 0.) {                           // pressed against the wall: contact theorem
+            double P0 = 0., Fl = Fsc;
+            for (int q = 0; q < nq; q++) {
+                int kq = T.ck[q0 + q];
+                double Fr = (q < nq - 1 && T.ch[q0 + q] > 0.)
+                                ? sg * (d.psi[kq] - d.psi[T.ck[q0 + q + 1]]) / T.ch[q0 + q] : Fl;
+                double Fm = 0.5 * (Fl + Fr);
+                if (Fm < 0.) Fm = 0.;
+                double w = Fm * T.cl[q0 + q] / VT;
+                if (lane == 0) K[2 * q] = q == nq - 1 ? w + 1. : w;     // + kT n(z1) at the column's end
+                P0 += w * car[kq];
+                Fl = Fr;
+            }
+            P0 += car[T.ck[q0 + nq - 1]];
+            neff[hole] = P0;
+        } else {                                  // pushed away: local density at the interface
+            const double *qc = hole ? T.qcp : T.qcn;
+            double qv = qc ? qc[k] : 0.;
+            neff[hole] = car[k] * exp(-qv + sg * (P.psis - d.psi[k]) / VT);
+            if (lane == 0)
+                for (int q = 0; q < nq; q++) K[2 * q] = q == 0 ? neff[hole] / fmax(car[k], SD_NFLOOR) : 0.;
+        }
+        if (!(neff[hole] > 1e-100)) neff[hole] = 1e-100;
+    }
+    const double EFn = -d.phin[k] - d.Cref, EFp = -d.phip[k] - d.Cref, EFm = -d.cons[c].V - d.Cref;
+    const double Ecp = EFn - VT * log(neff[0] / ms.Nc), Evp = EFp + VT * log(neff[1] / ms.Nv);
+    double F[4];
+    tun_flux_wave(d.mats, ms, P, Ecp, Evp, EFn, EFp, EFm, F, lane);
+    if (lane == 0) {
+        double be = A * F[0] / neff[0], bh = A * F[2] / neff[1];
+        for (int q = q0; q < q0 + nq; q++) { T.cK[2 * q] *= be; T.cK[2 * q + 1] *= bh; }
+        T.pG[2 * p] = A * F[1]; T.pG[2 * p + 1] = A * F[3];
+    }
+}
+void ph_tun_update(hipStream_t s, const DevView &d, const TunDev &T) {
+    if (T.np > 0) k_tun_path<<<sd_nblocks(T.np, SD_BLOCK / 64), SD_BLOCK, 0, s>>>(d, T);
+}
+// sink tk and source tg of one carrier at every node a column touches, summed
+// in the reference's order (tun_scatter :3373-3380); other nodes stay 0
+__global__ void k_tun_gather(TunDev T, int hole, double *tk, double *tg) {
+    int t = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (t >= T.ntn) return;
+    double a = 0., b = 0.;
+    for (int e = T.offK[t]; e < T.offK[t + 1]; e++) a += T.cK[2 * T.entK[e] + hole];
+    for (int e = T.offG[t]; e < T.offG[t + 1]; e++) b += T.pG[2 * T.entG[e] + hole];
+    tk[T.tn[t]] = a; tg[T.tn[t]] = b;
+}
+void ph_tun_gather(hipStream_t s, const TunDev &T, int hole, double *tk, double *tg) {
+    if (T.ntn > 0) k_tun_gather<<<sd_nblocks(T.ntn), SD_BLOCK, 0, s>>>(T, hole, tk, tg);
+}
+// gate tunnelling current density (A/m^2, into the device through the gate)
+// at the interface nodes, for display (upd_J :3648-3657); out is 0 elsewhere
+__global__ void k_tun_J(DevView d, TunDev T, double *out) {
+    int t = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (t >= T.ntn) return;
+    double J = 0., A = 0.;
+    for (int e = T.offG[t]; e < T.offG[t + 1]; e++) {
+        int p = T.entG[e];
+        double se, sh;
+        tun_escape(d, T, p, se, sh);
+        J += SD_Q * ((se - T.pG[2 * p]) - (sh - T.pG[2 * p + 1]));
+        A += T.pA[p];
+    }
+    out[T.tn[t]] = A > 0. ? J / A : 0.;
+}
+void ph_tun_J(hipStream_t s, const DevView &d, const TunDev &T, double *out) {
+    if (T.ntn > 0) k_tun_J<<<sd_nblocks(T.ntn), SD_BLOCK, 0, s>>>(d, T, out);
+}
+
 // terminal currents: one workgroup per contact, fixed-order tree sums
 __global__ void __launch_bounds__(SD_BLOCK) k_currents(DevView d, CurEdges e, RedBuf rb) {
     __shared__ double lds[3][SD_BLOCK / 64];
@@ -385,6 +580,16 @@ __global__ void __launch_bounds__(SD_BLOCK) k_currents(DevView d, CurEdges e, Re
         face_flux(d, e.k[q], e.kb[q], e.h[q], e.ar[q], Jn, Jp, Sn, Sp);
         a0 += Jn + Jp; a1 += Sn + Sp; a2 += Jn;
     }
+    // gate tunnelling (:3614-3621): electrons leaving to the gate are current
+    // into the device through it, holes leaving are current out of it
+    if (e.poff)
+        for (int p = e.poff[c] + threadIdx.x; p < e.poff[c + 1]; p += SD_BLOCK) {
+            double se, sh;
+            tun_escape(d, e.tun, p, se, sh);
+            const double G0 = e.tun.pG[2 * p], G1 = e.tun.pG[2 * p + 1];
+            double Ie = SD_Q * (se - G0), Ih = -SD_Q * (sh - G1);
+            a0 += Ie + Ih; a1 += SD_Q * (se + G0 + sh + G1); a2 += Ie;
+        }
     a0 = sd_wave_sum(a0); a1 = sd_wave_sum(a1); a2 = sd_wave_sum(a2);
     int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
     if (ln == 0) { lds[0][w] = a0; lds[1][w] = a1; lds[2][w] = a2; }
@@ -518,6 +723,9 @@ __global__ void k_maj_gather(DevView d, int hole, const double *u, const double 
         q[1] = d.fmu[(size_t)((hole ? 3 : 0) + dir / 2) * g.N + kl];
         q[2] = d.con_mask[kb] ? car[kb] : s[kb] * u[kb];
     }
+    o[22] = d.gii ? d.gii[k] : 0.;
+    o[23] = d.tk ? d.tk[k] : 0.;
+    o[24] = d.tg ? d.tg[k] : 0.;
 }
 void ph_maj_gather(hipStream_t s, const DevView &d, int hole, const double *u, const double *sc, const int *list,
                    int m, double *out) {

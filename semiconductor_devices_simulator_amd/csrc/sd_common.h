@@ -36,6 +36,29 @@ struct MatDev {
     double T;                            // K
     int lb_on;                           // Lombardi surface mobility: bit 0 electrons, bit 1 holes
     double lbB[2], lbC[2], lbL[2], lbD[2], lbE[2];
+    // gate tunnelling: relative permittivity, raw band-edge densities of
+    // states (m^-3), supply masses of a semiconductor and tunnelling masses
+    // of an insulator (units of m0)
+    double eps_r, Nc, Nv, ms_e, ms_h, mt_e, mt_h;
+};
+
+// Gate tunnelling paths (semiconductor node -> gate metal through a layer
+// stack) and their supply columns, semiconductor_3d.c:1147-1247, 3230-3390.
+// Paths are stored grouped by contact (poff), in the reference's order.
+#define SD_TUN_MAXL 6
+struct TunDev {
+    int np, ntn;
+    const int *pk, *pc, *pnl, *pc0, *pcn;     // node, contact, layers, first column entry, column length
+    const int *pm;                            // SD_TUN_MAXL material indices per path
+    const double *pA, *pt;                    // area (m^2); SD_TUN_MAXL layer thicknesses (m) per path
+    const int *ck;                            // column entry: node
+    const double *cl, *ch;                    // column entry: cell length along the path, spacing to the next
+    double *cK;                               // linearised escape (m^3/s): 2 per column entry (e, h)
+    double *pG;                               // injection from the metal (1/s): 2 per path (e, h)
+    const int *tn;                            // nodes touched by a column, ascending
+    const int *offK, *entK;                   // per touched node: its column entries (ascending)
+    const int *offG, *entG;                   // per touched node: the paths starting there (ascending)
+    const double *qcn, *qcp;                  // MLDA terms (nullptr: none)
 };
 
 // Per-contact data read by kernels (semiconductor_3d.c:191-208).
@@ -67,6 +90,8 @@ struct DevView {
     double *fmu;                         // 6 face mobilities, block q at fmu + q*N
     double *gii;                         // impact-ionisation generation (1/s per node)
     const double *qd;                    // distance to the nearest interface (m); nullptr: no Lombardi
+    const double *tk, *tg;               // gate tunnelling of the carrier being solved: sink rate
+                                         // (m^3/s, times the density) and source (1/s); nullptr: none
     const MatDev *mats;
     const ConDev *cons;
     int models, caxis, ref_mat;

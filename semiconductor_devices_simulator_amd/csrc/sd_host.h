@@ -26,6 +26,9 @@ struct HostMat {
     double lbB[2], lbC[2], lbL[2], lbD[2], lbE[2];
     // MLDA confinement masses (units of m0) and valley weights (:146, :457, :619-623)
     double mq_e1, mq_e2, wq_e1, mq_h1, mq_h2, wq_h1;
+    // gate tunnelling (:147-151): supply masses of a semiconductor, tunnelling
+    // masses of an insulator (units of m0; 0 = none)
+    double ms_e, ms_h, mt_e, mt_h;
     double ii_a[2][2], ii_b[2][2], ii_E0[2], ii_m[2], ii_hw, ii_g;
     double lnc, lnv;
 };
@@ -75,6 +78,7 @@ inline bool host_mat_init(HostMat *m, int id, double T) {
         m->lbB[1] = 9.925e6; m->lbC[1] = 2947.; m->lbL[1] = 0.0317; m->lbD[1] = 2.0546e14;
         // MLDA: electrons in the 2 Delta-2 (0.916) and 4 Delta-4 (0.19) valleys, holes heavy / light
         m->mq_e1 = 0.916; m->mq_e2 = 0.19; m->wq_e1 = 1. / 3.; m->mq_h1 = 0.29; m->mq_h2 = 0.20; m->wq_h1 = 0.84;
+        m->ms_e = 0.50; m->ms_h = 0.50; m->mt_e = 0.26; m->mt_h = 0.39;
         break;
     case 1:  // Ge
         m->Eg = 0.742 - 4.77e-4 * T * T / (T + 235.); m->chi = 4.00; m->eps_r = 16.2;
@@ -129,18 +133,19 @@ inline bool host_mat_init(HostMat *m, int id, double T) {
         m->vsat_n = 1.5e5; m->vsat_p = 1e5; m->beta_n = 1.; m->beta_p = 1.;
         rec(1e-9, 1e-9, 1e-43, 1e-43, 1e-16);
         break;
-    case 8: m->insulator = 1; m->Eg = 9.0; m->chi = 0.95; m->eps_r = 3.9; break;     // SiO2
-    case 9: m->insulator = 1; m->Eg = 6.8; m->chi = 1.35; m->eps_r = 9.0; break;     // Al2O3
-    case 10: m->insulator = 1; m->Eg = 5.8; m->chi = 2.55; m->eps_r = 22.0; break;   // HfO2
-    case 11: m->insulator = 1; m->Eg = 5.1; m->chi = 2.05; m->eps_r = 7.5; break;    // Si3N4
-    case 12: m->insulator = 1; m->Eg = 10.; m->chi = 0.; m->eps_r = 1.0; break;      // void
-    case 13: m->insulator = 1; m->Eg = 10.; m->chi = 0.; m->eps_r = 1.0e3; break;    // floating metal inclusion
+    case 8: m->insulator = 1; m->Eg = 9.0; m->chi = 0.95; m->eps_r = 3.9; m->mt_e = 0.42; m->mt_h = 0.33; break;     // SiO2
+    case 9: m->insulator = 1; m->Eg = 6.8; m->chi = 1.35; m->eps_r = 9.0; m->mt_e = 0.35; m->mt_h = 0.35; break;     // Al2O3
+    case 10: m->insulator = 1; m->Eg = 5.8; m->chi = 2.55; m->eps_r = 22.0; m->mt_e = 0.18; m->mt_h = 0.2; break;   // HfO2
+    case 11: m->insulator = 1; m->Eg = 5.1; m->chi = 2.05; m->eps_r = 7.5; m->mt_e = 0.5; m->mt_h = 0.5; break;    // Si3N4
+    case 12: m->insulator = 1; m->Eg = 10.; m->chi = 0.; m->eps_r = 1.0; m->mt_e = 1.0; m->mt_h = 1.0; break;      // void
+    case 13: m->insulator = 1; m->Eg = 10.; m->chi = 0.; m->eps_r = 1.0e3; m->mt_e = 1.0; m->mt_h = 1.0; break;    // floating metal inclusion
     default: return false;
     }
     if (!m->insulator && !(m->mq_e1 > 0.) && id >= 0 && id < 8) {   // single DOS-like masses
         static const double me[8] = {0.916, 0.082, 0.067, 0.08, 0.37, 0.20, 0.09, 0.23};
         static const double mh[8] = {0.29, 0.28, 0.50, 0.60, 1.00, 1.00, 0.55, 1.00};
         m->mq_e1 = m->mq_e2 = me[id]; m->wq_e1 = 1.; m->mq_h1 = m->mq_h2 = mh[id]; m->wq_h1 = 1.;
+        m->ms_e = me[id] < 0.5 ? me[id] : 0.5; m->ms_h = 0.5;
     }
     m->ni = m->insulator ? 0. : sqrt(m->Nc * m->Nv) * exp(-m->Eg / (2. * m->VT));
     m->ii_g = m->ii_hw > 0. ? tanh(m->ii_hw / (2. * SD_KB * 300. / SD_Q)) / tanh(m->ii_hw / (2. * m->VT)) : 1.;
@@ -196,6 +201,7 @@ inline MatDev host_mat_dev(const HostMat &m) {
     MatDev d;
     memset(&d, 0, sizeof d);
     d.VT = m.VT; d.lnc = m.lnc; d.lnv = m.lnv; d.ni = m.ni; d.Eg = m.Eg; d.chi = m.chi;
+    d.eps_r = m.eps_r; d.Nc = m.Nc; d.Nv = m.Nv; d.ms_e = m.ms_e; d.ms_h = m.ms_h; d.mt_e = m.mt_e; d.mt_h = m.mt_h;
     d.tau_n0 = m.tau_n0; d.tau_p0 = m.tau_p0; d.Cn = m.Cn; d.Cp = m.Cp; d.Brad = m.Brad; d.tau_Nref = m.tau_Nref;
     d.vsat_n = m.vsat_n; d.vsat_p = m.vsat_p; d.beta_n = m.beta_n; d.beta_p = m.beta_p;
     d.ani_n = m.ani_n; d.ani_p = m.ani_p;

@@ -10,8 +10,13 @@ struct CurEdges {
     const double *h, *ar;
     const int *off;       // device, n_cons+1
     int ncon;
+    const int *poff;      // gate tunnelling paths of each contact (n_cons+1; nullptr: none)
+    TunDev tun;
 };
 
+void ph_tun_update(hipStream_t s, const DevView &d, const TunDev &T);
+void ph_tun_gather(hipStream_t s, const TunDev &T, int hole, double *tk, double *tg);
+void ph_tun_J(hipStream_t s, const DevView &d, const TunDev &T, double *out);
 void ph_set_tile_mode(int m);          // c_tile_mode of k_physics.hip
 void ph_poisson(hipStream_t s, const DevView &d, const double *phi, int build, double *F, Sten7<double> A,
                 double *sc, RedBuf rb, int slot);
@@ -34,7 +39,7 @@ void ph_update_psi(hipStream_t s, int N, double *phi, const double *dx, double l
 void ph_maj_flags(hipStream_t s, const DevView &d, int hole, const double *u, const double *sc, unsigned char *flag);
 void ph_maj_propagate(hipStream_t s, const DevView &d, unsigned char *flag, int axis, int *changed);
 void ph_maj_count(hipStream_t s, int N, const unsigned char *flag, RedBuf rb, int slot);
-#define SD_MAJ_REC 22   // doubles per node record of ph_maj_gather
+#define SD_MAJ_REC 25   // doubles per node record of ph_maj_gather
 void ph_maj_list(hipStream_t s, int N, const unsigned char *flag, int *cnt, int *list, int cap);
 void ph_maj_gather(hipStream_t s, const DevView &d, int hole, const double *u, const double *sc, const int *list,
                    int m, double *out);

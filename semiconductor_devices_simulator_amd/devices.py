@@ -164,6 +164,9 @@ class DeviceSpec:
     solver: tuple = (2000, 1000, 500, 1e-10, 1e-9, 1.0, 1.0)
     tau: tuple = None              # (tau_n0, tau_p0) of material 0, or None
     sweep: list = field(default_factory=list)   # [(contact id, V start, V end, npts, {cid: V})]
+    # gate stacks for tunnelling (models bit 5): {contact index: [(material id, thickness nm), ...]},
+    # semiconductor side first (main.py:3203-3211)
+    stacks: dict = field(default_factory=dict)
 
     @property
     def shape(self):
@@ -349,4 +352,31 @@ def load_device(lib, spec, solver=None):
         cids.append(cid)
     if spec.tau is not None:
         lib.api3_set_mat_tau(mids[0], spec.tau[0], spec.tau[1])
+    known = dict(zip(spec.materials, mids))
+    for ci, layers in spec.stacks.items():
+        ids = []
+        for m, _ in layers:
+            if m not in known:
+                known[m] = lib.api3_add_material(m, spec.T)
+                if known[m] < 0:
+                    raise ValueError("too many materials")
+            ids.append(known[m])
+        mi = np.asarray(ids, np.int32)
+        th = np.asarray([t * 1e-9 for _, t in layers], np.float64)
+        if not lib.api3_set_gate_stack(cids[ci], len(layers), iptr(mi), dptr(th)):
+            raise ValueError(f"bad gate stack for contact {ci}")
     return cids
+
+
+MAT_SIO2 = 8
+
+
+def with_gate_tunnelling(spec, gates=None):
+    """The device with gate tunnelling on (models bit 5): every Robin gate
+    (or the contacts listed in `gates`) gets a SiO2 stack of its oxide
+    thickness (main.py:3203-3211 with the gate's oxide layer)."""
+    gates = [i for i, c in enumerate(spec.contacts) if c.bc == 2] if gates is None else gates
+    spec.models |= 32
+    spec.stacks = {i: [(MAT_SIO2, spec.contacts[i].tox)] for i in gates}
+    spec.name += "_tun"
+    return spec

@@ -29,6 +29,13 @@ def golden_contacts(g):
     return json.loads(str(g["contacts"]))
 
 
+def golden_stacks(g):
+    """Gate stacks of a fixture: {contact index: [(material id, thickness nm)]}."""
+    if "stacks" not in g:
+        return {}
+    return {int(k): [tuple(x) for x in v] for k, v in json.loads(str(g["stacks"])).items()}
+
+
 def oracle_device(g):
     """The oracle's prepared device for a kernel fixture."""
     import semisim_oracle as O
@@ -64,6 +71,10 @@ def load_fixture_device(lib, g):
                                 c["tox"], c["qox"] / 1.602176634e-19)
     if float(g["tau"][0]) > 0:
         lib.api3_set_mat_tau(0, float(g["tau"][0]), float(g["tau"][1]))
+    for ci, layers in golden_stacks(g).items():
+        ids = np.asarray([lib.api3_add_material(m, 300.0) for m, _ in layers], np.int32)
+        th = np.asarray([t * 1e-9 for _, t in layers], np.float64)
+        assert lib.api3_set_gate_stack(ci, len(layers), iptr(ids), dptr(th)) == 1
     assert lib.sdx_test_prepare() == 1
     psi, phin, phip = (np.ascontiguousarray(g[k], np.float64) for k in ("psi", "phin", "phip"))
     lib.sdx_test_set_state(dptr(psi), dptr(phin), dptr(phip))

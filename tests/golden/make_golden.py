@@ -43,6 +43,7 @@ def load_harness():
         "refh_neutral_psi": (D, [ctypes.c_longlong, D]), "refh_set_fmu": (None, [DP]),
         "refh_apply_bc": (None, [DP]), "refh_ii_update": (None, [DP]), "refh_get_qd": (None, [DP]), "refh_get_qc": (None, [DP, DP]),
         "refh_aa_reset": (None, []), "refh_aa_step": (None, [DP, DP]), "refh_aa_m": (I, []),
+        "refh_tun_paths": (I, [IP, IP, IP, IP, IP, DP, IP, DP, DP]), "refh_tun": (None, [DP, I, DP, DP, DP, DP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -371,6 +372,52 @@ def lombardi_fixture(lib, spec, name, Va, rng, perturb=1e-3):
     print("wrote", name, N, "nodes; nodes within 10 nm of an interface", int((qd < 1e-8).sum()))
 
 
+def tunnel_fixture(lib, spec, name, Va, rng, perturb=1e-3):
+    """Gate tunnelling (models bit 5) at a perturbed solved state: the paths
+    and supply columns prepare builds, one tunnelling update (escape per unit
+    density of every column entry, injection of every path) and the gathered
+    sink and source of both carriers."""
+    cids = devices.load_device(lib, spec)
+    for c, v in enumerate(Va):
+        lib.api3_set_contact_V(cids[c], v)
+    assert lib.api3_solve() == 1, "reference solve failed"
+    N = spec.N
+    psi, phin, phip = np.zeros(N), np.zeros(N), np.zeros(N)
+    lib.api3_get_state(core.dptr(psi), core.dptr(phin), core.dptr(phip))
+    assert lib.refh_prepare() == 1
+    kind = np.zeros(N, np.uint8)
+    lib.refh_get_kind(kind.ctypes.data)
+    cm = np.zeros(N, np.int32)
+    lib.refh_get_conmask(core.iptr(cm))
+    free = (cm == 0) & (kind == 0)
+    psi = psi + perturb * free * rng.standard_normal(N)
+    phin = phin + 0.3 * perturb * free * rng.standard_normal(N)
+    phip = phip + 0.3 * perturb * free * rng.standard_normal(N)
+    lib.refh_set_raw_state(core.dptr(psi), core.dptr(phin), core.dptr(phip))
+    ncol = ctypes.c_int(0)
+    npth = lib.refh_tun_paths(ctypes.byref(ncol), None, None, None, None, None, None, None, None)
+    nc = ncol.value
+    pk, pc, pc0, pcn = (np.zeros(npth, np.int32) for _ in range(4))
+    pA = np.zeros(npth)
+    ck = np.zeros(nc, np.int32)
+    cl, ch = np.zeros(nc), np.zeros(nc)
+    lib.refh_tun_paths(ctypes.byref(ncol), core.iptr(pk), core.iptr(pc), core.iptr(pc0), core.iptr(pcn),
+                       core.dptr(pA), core.iptr(ck), core.dptr(cl), core.dptr(ch))
+    out = dict(xs=spec.xs, ys=spec.ys, zs=spec.zs, Nd=spec.Nd, Na=spec.Na, models=spec.models,
+               tau=np.asarray(spec.tau if spec.tau else (0.0, 0.0)), Va=np.asarray(Va, np.float64),
+               contacts=json.dumps(contacts_of(spec)), stacks=json.dumps({str(k): v for k, v in spec.stacks.items()}),
+               psi=psi, phin=phin, phip=phip, kind=kind, con_mask=cm,
+               pk=pk, pc=pc, pc0=pc0, pcn=pcn, pA=pA, ck=ck, cl=cl, ch=ch)
+    Vd = np.asarray(Va, np.float64)
+    for hole in (0, 1):
+        cK, pG, tk, tg = np.zeros(2 * nc), np.zeros(2 * npth), np.zeros(N), np.zeros(N)
+        lib.refh_tun(core.dptr(Vd), hole, core.dptr(cK), core.dptr(pG), core.dptr(tk), core.dptr(tg))
+        out["cK"], out["pG"], out["tk%d" % hole], out["tg%d" % hole] = cK, pG, tk, tg
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print("wrote", name, "paths", npth, "column entries", nc, "injection e/h", float(out["pG"][0::2].sum()),
+          float(out["pG"][1::2].sum()))
+
+
 def anderson_fixture(lib, spec, name, Va, rng, steps=8):
     """The reference's Anderson mixing on a prepared MOSFET state: a sequence
     of (x, g) pairs from a synthetic contraction toward a perturbed state
@@ -413,6 +460,15 @@ def anderson_fixture(lib, spec, name, Va, rng, steps=8):
 
 def main():
     lib = load_harness()
+    if "--tunnel" in sys.argv:         # gate tunnelling (models bit 5)
+        for nm, shape, Va, seed in (("k_tun_mosfet_12x8x6_inv", (12, 8, 6), [0.0, 0.5, 1.5, 0.0], 91),
+                                    ("k_tun_mosfet_12x8x6_acc", (12, 8, 6), [0.0, 0.2, -1.5, 0.0], 92),
+                                    ("k_tun_mosfet_20x14x9_inv", (20, 14, 9), [0.0, 1.0, 2.0, 0.0], 93)):
+            tunnel_fixture(lib, devices.with_gate_tunnelling(devices.mosfet(*shape, tox=1.5)), nm, Va,
+                           np.random.default_rng(seed))
+        spec = devices.with_gate_tunnelling(devices.mosfet(24, 16, 8, npts=7, tox=1.5))
+        sweep_fixture(lib, spec, "e2e_tun_mosfet_24x16x8")
+        return
     if "--anderson" in sys.argv:       # Anderson mixing of the Gummel map
         anderson_fixture(lib, devices.mosfet(12, 8, 6), "aa_mosfet_12x8x6", [0.0, 0.5, 0.8, 0.0],
                          np.random.default_rng(5151))

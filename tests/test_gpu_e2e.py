@@ -102,11 +102,21 @@ def test_state_injection_round_trip(hip):
 
 def test_out_of_scope_physics_is_rejected(hip):
     spec = devices.pn_diode(8, 8, 32)
+    spec.contacts[1].bc = 1                          # Schottky cathode
     devices.load_device(hip, spec)
-    hip.api3_set_models(32)                          # gate tunnelling
+    hip.api3_set_models(64)                          # thermionic emission at a Schottky face contact
     assert hip.api3_solve() == 0
     hip.api3_set_models(0)
     assert hip.api3_solve() == 1
+    # gate tunnelling through a meshed insulator layer (the x = 0 plane is SiO2)
+    spec = devices.pn_diode(8, 8, 32)
+    spec.materials = [devices.MAT_SI, devices.MAT_SIO2]
+    mat = spec.mat.reshape(32, 8, 8).copy()
+    mat[:, :, 0] = 1
+    spec.mat = mat.ravel()
+    spec.models = 32
+    devices.load_device(hip, spec)
+    assert hip.api3_solve() == 0
 
 
 def test_readback_quantities(hip):
