@@ -1117,23 +1117,28 @@ def tun_wkb(m, L, u0, u1):
 
 
 def tun_stack(layers, psik, psim):
-    """Potentials along an all-insulator stack from the node (psik) to the
-    metal (psim): D = (psim - psik) / sum t/eps, psi linear in every layer
-    (tun_stack :3300-3312).  Returns (D, psis, [(mat, t, p0, dp)])."""
+    """Potentials along a stack from the node (psik) to the metal (psim):
+    D = (psim - psik) / sum t/eps, psi linear in every layer; a first layer
+    that is not an insulator (the semiconductor half cell) puts the interface
+    at its far end (tun_stack :3300-3312).  Returns (D, psis, [(mat, t, p0, dp)])."""
     R = sum(t / (m["eps_r"] * EPS0) for m, t in layers)
     D = (psim - psik) / R if R > 0 else 0.0
-    out, ps = [], psik
-    for m, t in layers:
+    out, ps, psis = [], psik, psik
+    for i, (m, t) in enumerate(layers):
         dp = D * t / (m["eps_r"] * EPS0)
         out.append((m, t, ps, dp))
         ps += dp
-    return D, psik, out
+        if i == 0 and not m.get("insulator", True):
+            psis = ps
+    return D, psis, out
 
 
 def tun_T(stack, E, hole):
     """WKB transmission at energy E (eV) (tun_T :3275-3288)."""
     s = 0.0
     for m, t, p0, dp in stack:
+        if not m.get("insulator", True):
+            continue
         mt = m["mt_h"] if hole else m["mt_e"]
         ea, eb = -p0 - m["chi"], -(p0 + dp) - m["chi"]
         if hole:
@@ -1168,6 +1173,18 @@ def tun_flux(dev, stack, Ecp, Evp, EFn, EFp, EFm):
     ch = 4.0 * math.pi * SI_SUPPLY_MASS[1] * M0 * Q * Q * kT / h3
     return (ce * tun_int(stack, Ecp, EFn, kT, 0), ce * tun_int(stack, Ecp, EFm, kT, 0),
             ch * tun_int(stack, Evp, EFp, kT, 1), ch * tun_int(stack, Evp, EFm, kT, 1))
+
+
+def tun_eval(m, Cref, layers, psi, phin, phip, Vg, pm):
+    """Fluxes through a stack from a semiconductor (material m) whose surface
+    is at psi, with local supply at the band edges of the surface
+    (api3_tun_eval :4855-4867): (e s->g, e g->s, h s->g, h g->s) in 1/m^2 s
+    and the net current density into the semiconductor (A/m^2)."""
+    D, psis, stack = tun_stack(layers, psi, Vg + Cref - (pm if pm > 0 else Cref))
+    Ec = -psis - m["chi"]
+    dev = type("S", (), {"m": m})()
+    F = tun_flux(dev, stack, Ec, Ec - m["Eg"], -phin - Cref, -phip - Cref, -Vg - Cref)
+    return F, Q * ((F[0] - F[1]) - (F[2] - F[3]))
 
 
 def tun_update(dev, P, psi, phin, phip, n, p, Va):

@@ -205,10 +205,13 @@ int api3_get_ntp(int cid) {
     if (cid < 0) return (int)G.tp_k.size();
     return cid < G.n_cons ? G.cons[cid].ntp : 0;
 }
-double api3_tun_eval(int, int, const int *, const double *, double, double, double, double, double, double *out) {
-    unsupported("gate tunnelling");
-    if (out) for (int i = 0; i < 4; i++) out[i] = NAN;
-    return NAN;
+// tunnelling fluxes through a stack from a semiconductor surface (test hook
+// of the reference, :4848-4867); 0 on invalid arguments, as the reference
+double api3_tun_eval(int smat, int nl, const int *mat, const double *t, double psi, double phin, double phip,
+                     double Vg, double pm, double *out) {
+    if (smat < 0 || smat >= G.n_mats || nl < 1 || nl > 6) return 0.;
+    for (int i = 0; i < nl; i++) if (mat[i] < 0 || mat[i] >= G.n_mats) return 0.;
+    return eng_tun_eval(smat, nl, mat, t, psi, phin, phip, Vg, pm, out);
 }
 double api3_mlda_factor(int, double, double, int) { unsupported("quantum confinement (MLDA)"); return NAN; }
 double api3_coulomb_inv(int, double, double, double, double) { return NAN; }

@@ -293,6 +293,30 @@ static void mat_prepare() {
     }
 }
 
+// Tunnelling fluxes through a stack (api3_tun_eval :4855-4867), evaluated on
+// the device by the kernels of the solve; F: e s->g, e g->s, h s->g, h g->s
+// (1/m^2 s).  Returns the net current density into the semiconductor (A/m^2).
+double eng_tun_eval(int smat, int nl, const int *mat, const double *t, double psi, double phin, double phip,
+                    double Vg, double pm, double *F) {
+    mat_prepare();
+    MatDev md[SD_NM_MAX] = {};
+    for (int i = 0; i < G.n_mats; i++) md[i] = host_mat_dev(G.mats[i]);
+    MatDev *d_md = nullptr;
+    double *d_F = nullptr, h[4] = {NAN, NAN, NAN, NAN};
+    hipStream_t s = G.st;
+    if (hipMalloc(&d_md, sizeof md) == hipSuccess && hipMalloc(&d_F, 4 * sizeof(double)) == hipSuccess &&
+        hipMemcpyAsync(d_md, md, sizeof md, hipMemcpyHostToDevice, s) == hipSuccess) {
+        ph_tun_eval(s, d_md, smat, nl, mat, t, psi, phin, phip, Vg, pm, G.Cref, d_F);
+        if (hipMemcpyAsync(h, d_F, sizeof h, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            for (double &v : h) v = NAN;
+    }
+    if (d_md) hipFree(d_md);
+    if (d_F) hipFree(d_F);
+    if (F) for (int i = 0; i < 4; i++) F[i] = h[i];
+    return SD_Q * ((h[0] - h[1]) - (h[2] - h[3]));
+}
+
 // nodes of a face contact (face 0..5) or of a box (face 6), v6 index
 // convention, clipped to the mesh (semiconductor_3d.c:1047-1065)
 template <class F> static void for_con_nodes(const HostCon &cn, F body) {

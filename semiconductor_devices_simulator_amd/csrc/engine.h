@@ -171,6 +171,8 @@ void host_unidx(int k, int &i, int &j, int &l);
 void eng_wall_distance(std::vector<double> &qd);
 // the same, with the MLDA confinement terms per carrier (qcn, qcp)
 void eng_walls(std::vector<double> &qd, std::vector<double> *qcn, std::vector<double> *qcp);
+double eng_tun_eval(int smat, int nl, const int *mat, const double *t, double psi, double phin, double phip,
+                    double Vg, double pm, double *F);
 void eng_sync();
 void eng_download(double *dst, const double *src, size_t n);
 void eng_upload(double *dst, const double *src, size_t n);

@@ -535,6 +535,31 @@ __global__ void __launch_bounds__(SD_BLOCK) k_tun_path(DevView d, TunDev T) {
         T.pG[2 * p] = A * F[1]; T.pG[2 * p + 1] = A * F[3];
     }
 }
+// fluxes through a given stack from a semiconductor surface at psi with
+// quasi-Fermi potentials phin, phip, local supply (api3_tun_eval :4855-4867)
+struct TunEvalArgs {
+    int smat, nl, mi[SD_TUN_MAXL];
+    double t[SD_TUN_MAXL];
+    double psi, phin, phip, Vg, pm, Cref;
+};
+__global__ void __launch_bounds__(64) k_tun_eval(const MatDev *mats, TunEvalArgs a, double *F) {
+    const int lane = threadIdx.x & 63;
+    const MatDev &ms = mats[a.smat];
+    double pm = a.pm > 0. ? a.pm : a.Cref;
+    TunStack P;
+    tun_stack(mats, P, a.nl, a.mi, a.t, a.psi, a.Vg + a.Cref - pm);
+    double Ec = -P.psis - ms.chi, Ev = Ec - ms.Eg, Fl[4];
+    tun_flux_wave(mats, ms, P, Ec, Ev, -a.phin - a.Cref, -a.phip - a.Cref, -a.Vg - a.Cref, Fl, lane);
+    if (lane == 0) for (int i = 0; i < 4; i++) F[i] = Fl[i];
+}
+void ph_tun_eval(hipStream_t s, const MatDev *mats, int smat, int nl, const int *mi, const double *t, double psi,
+                 double phin, double phip, double Vg, double pm, double Cref, double *F) {
+    TunEvalArgs a{};
+    a.smat = smat; a.nl = nl;
+    for (int i = 0; i < nl; i++) { a.mi[i] = mi[i]; a.t[i] = t[i]; }
+    a.psi = psi; a.phin = phin; a.phip = phip; a.Vg = Vg; a.pm = pm; a.Cref = Cref;
+    k_tun_eval<<<1, 64, 0, s>>>(mats, a, F);
+}
 void ph_tun_update(hipStream_t s, const DevView &d, const TunDev &T) {
     if (T.np > 0) k_tun_path<<<sd_nblocks(T.np, SD_BLOCK / 64), SD_BLOCK, 0, s>>>(d, T);
 }

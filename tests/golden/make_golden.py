@@ -418,6 +418,35 @@ def tunnel_fixture(lib, spec, name, Va, rng, perturb=1e-3):
           float(out["pG"][1::2].sum()))
 
 
+TUN_EVAL_CASES = [   # (layers as (material id, nm), psi, phin, phip, Vg, pm)
+    ([(8, 1.5)], 0.3, 0.0, 0.0, 1.5, 4.65),
+    ([(8, 1.5)], -0.2, 0.1, -0.1, -1.5, 4.65),
+    ([(8, 2.5)], 0.5, 0.0, 0.0, 3.0, 4.1),
+    ([(8, 0.8), (10, 2.0)], 0.4, 0.0, 0.0, 1.2, 4.65),
+    ([(0, 0.5), (8, 1.2)], 0.35, 0.02, -0.03, 1.4, 0.0),
+    ([(11, 1.0), (8, 1.0)], -0.6, 0.0, 0.0, -2.0, 5.0),
+]
+
+
+def tun_eval_fixture(lib, name):
+    """The reference's api3_tun_eval (tunnelling fluxes through a given stack)
+    for a set of stacks: oxide, oxide on high-k, a semiconductor half cell
+    then oxide, nitride on oxide; inversion and accumulation biases."""
+    lib.api3_init()
+    ids = {m: lib.api3_add_material(m, 300.0) for m in (0, 8, 10, 11)}
+    F, J = [], []
+    for layers, psi, phin, phip, Vg, pm in TUN_EVAL_CASES:
+        mi = np.asarray([ids[m] for m, _ in layers], np.int32)
+        th = np.asarray([t * 1e-9 for _, t in layers], np.float64)
+        out = np.zeros(4)
+        J.append(lib.api3_tun_eval(ids[0], len(layers), core.iptr(mi), core.dptr(th), psi, phin, phip, Vg, pm,
+                                   core.dptr(out)))
+        F.append(out)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), cases=json.dumps(TUN_EVAL_CASES), F=np.array(F),
+                        J=np.array(J), ids=json.dumps(ids))
+    print("wrote", name, np.array(J))
+
+
 def anderson_fixture(lib, spec, name, Va, rng, steps=8):
     """The reference's Anderson mixing on a prepared MOSFET state: a sequence
     of (x, g) pairs from a synthetic contraction toward a perturbed state
@@ -460,6 +489,9 @@ def anderson_fixture(lib, spec, name, Va, rng, steps=8):
 
 def main():
     lib = load_harness()
+    if "--tuneval" in sys.argv:        # tunnelling fluxes through a given stack
+        tun_eval_fixture(lib, "tun_eval_stacks")
+        return
     if "--tunnel" in sys.argv:         # gate tunnelling (models bit 5)
         for nm, shape, Va, seed in (("k_tun_mosfet_12x8x6_inv", (12, 8, 6), [0.0, 0.5, 1.5, 0.0], 91),
                                     ("k_tun_mosfet_12x8x6_acc", (12, 8, 6), [0.0, 0.2, -1.5, 0.0], 92),

@@ -83,3 +83,24 @@ def test_tunnelling_sweep_matches_reference(hip, precision):
                     ((c, c.to_idx6(spec.xs, spec.ys, spec.zs)) for c in spec.contacts)], models=spec.models)
     dev_area[:] = dev.robA
     assert np.sum(J * dev_area) == pytest.approx(R["I"][-1, 2], rel=1e-9)
+
+
+def test_tun_eval_matches_reference(hip):
+    """api3_tun_eval (fluxes through a given stack, evaluated by the device
+    kernels of the solve) against the reference's for six stacks."""
+    import json
+    print("\n[gpu tunnel] api3_tun_eval, 6 stacks", flush=True)
+    g = load_golden("tun_eval_stacks")
+    hip.api3_init()
+    ids = {m: hip.api3_add_material(m, 300.0) for m in (0, 8, 10, 11)}
+    for c, (layers, psi, phin, phip, Vg, pm) in enumerate(json.loads(str(g["cases"]))):
+        mi = np.asarray([ids[m] for m, _ in layers], np.int32)
+        th = np.asarray([t * 1e-9 for _, t in layers], np.float64)
+        out = np.zeros(4)
+        J = hip.api3_tun_eval(ids[0], len(layers), iptr(mi), dptr(th), psi, phin, phip, Vg, pm, dptr(out))
+        np.testing.assert_allclose(out, g["F"][c], rtol=1e-11, atol=0, err_msg=str(layers))
+        assert J == pytest.approx(float(g["J"][c]), rel=1e-9, abs=1e-12 * max(out) * 1.602176634e-19)
+    # invalid arguments return 0 and leave the output alone, as the reference
+    out = np.full(4, 7.0)
+    assert hip.api3_tun_eval(ids[0], 0, iptr(mi), dptr(th), 0.0, 0.0, 0.0, 1.0, 4.65, dptr(out)) == 0.0
+    assert np.all(out == 7.0)

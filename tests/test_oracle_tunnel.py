@@ -94,3 +94,23 @@ def test_gate_current_sign_and_balance(case):
     gate = 2
     assert abs(I[gate]) > 0 and S[gate] >= abs(I[gate])
     assert np.all(I[[0, 1, 3]] == 0)
+
+
+def eval_case(layers, psi, phin, phip, Vg, pm):
+    si = O.silicon()
+    Cref = si["chi"] + 0.5 * si["Eg"] + 0.5 * si["VT"] * math.log(si["Nc"] / si["Nv"])
+    lay = [((dict(si, insulator=False) if m == 0 else O.INSULATORS[m]), t * 1e-9) for m, t in layers]
+    return O.tun_eval(si, Cref, lay, psi, phin, phip, Vg, pm)
+
+
+def test_tun_eval_matches_reference():
+    """api3_tun_eval of the reference for six stacks (tests/golden/tun_eval_stacks.npz)."""
+    import json
+    g = load_golden("tun_eval_stacks")
+    for c, case in enumerate(json.loads(str(g["cases"]))):
+        F, J = eval_case(*case)
+        np.testing.assert_allclose(F, g["F"][c], rtol=1e-11, atol=0, err_msg=str(case))
+        assert J == pytest.approx(float(g["J"][c]), rel=1e-9, abs=1e-12 * max(F) * O.Q)
+    # a semiconductor half cell moves the interface potential by its share of the field
+    D, psis, _ = O.tun_stack([(dict(O.silicon(), insulator=False), 0.5e-9), (O.INSULATORS[8], 1.2e-9)], 0.35, 1.0)
+    assert psis == pytest.approx(0.35 + D * 0.5e-9 / (11.7 * O.EPS0), rel=1e-14)
