@@ -31,6 +31,8 @@ SIGNATURES = [
     ("api3_set_gate_stack", I, [I, I, IP, DP]),
     ("api3_get_ntp", I, [I]),
     ("api3_tun_eval", D, [I, I, IP, DP, D, D, D, D, D, DP]),
+    ("api3_mlda_factor", D, [I, D, D, I]),
+    ("api3_lombardi", D, [I, D, D, D, D, I]),
     ("api3_set_grid", None, [I, I, I, D, D, D]),
     ("api3_set_grid_coords", I, [I, I, I, DP, DP, DP]),
     ("api3_set_solver", None, [I, I, I, D, D, D, D]),

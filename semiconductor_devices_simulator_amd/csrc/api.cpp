@@ -213,10 +213,19 @@ double api3_tun_eval(int smat, int nl, const int *mat, const double *t, double p
     for (int i = 0; i < nl; i++) if (mat[i] < 0 || mat[i] >= G.n_mats) return 0.;
     return eng_tun_eval(smat, nl, mat, t, psi, phin, phip, Vg, pm, out);
 }
-double api3_mlda_factor(int, double, double, int) { unsupported("quantum confinement (MLDA)"); return NAN; }
+// MLDA density factor of material mi at the cell range [a, b] (m) from a wall
+double api3_mlda_factor(int mi, double a, double b, int hole) {
+    if (mi < 0 || mi >= G.n_mats) return 1.;
+    return eng_mlda_factor(mi, a, b, hole);
+}
 double api3_coulomb_inv(int, double, double, double, double) { return NAN; }
 int api3_set_surface_mobility(int mi, double, double, double) { return (mi >= 0 && mi < G.n_mats) ? 1 : 0; }
-double api3_lombardi(int, double, double, double, double, int) { unsupported("Lombardi surface mobility"); return NAN; }
+// Lombardi edge mobility (m^2/Vs) for bulk mobility mub, normal field F (V/m),
+// total doping Nt (m^-3) and distance factor Dw, evaluated on the device
+double api3_lombardi(int mi, double mub, double F, double Nt, double Dw, int hole) {
+    if (mi < 0 || mi >= G.n_mats) return mub;
+    return eng_lombardi(mi, mub, F, Nt, Dw, hole);
+}
 void api3_set_contact_V(int cid, double V) { if (cid >= 0 && cid < G.n_cons) G.cons[cid].V = V; }
 int api3_get_ncon(void) { return G.n_cons; }
 double api3_get_contact_I(int cid) { return (cid >= 0 && cid < G.n_cons) ? G.cons[cid].I : 0.; }

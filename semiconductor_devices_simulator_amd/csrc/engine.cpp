@@ -317,6 +317,24 @@ double eng_tun_eval(int smat, int nl, const int *mat, const double *t, double ps
     return SD_Q * ((h[0] - h[1]) - (h[2] - h[3]));
 }
 
+// Lombardi edge mobility (m^2/Vs) of material mi by the device function of
+// the mobility kernel (api3_lombardi :4894-4897)
+double eng_lombardi(int mi, double mub, double Fv, double Nt, double Dw, int hole) {
+    double *d_o = nullptr, h = NAN;
+    hipStream_t s = G.st;
+    if (hipMalloc(&d_o, sizeof(double)) == hipSuccess) {
+        ph_lombardi_eval(s, host_mat_dev(G.mats[mi]), mub, Fv, Nt, Dw, hole, d_o);
+        if (hipMemcpyAsync(&h, d_o, sizeof h, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+            h = NAN;
+        hipFree(d_o);
+    }
+    return h;
+}
+// MLDA density factor of material mi at the cell range [a, b] from a wall
+// (api3_mlda_factor :4869-4872): the host function prepare uses
+static double mlda_factor(const HostMat &m, double a, double b, int hole);
+double eng_mlda_factor(int mi, double a, double b, int hole) { return mlda_factor(G.mats[mi], a, b, hole); }
+
 // nodes of a face contact (face 0..5) or of a box (face 6), v6 index
 // convention, clipped to the mesh (semiconductor_3d.c:1047-1065)
 template <class F> static void for_con_nodes(const HostCon &cn, F body) {

@@ -173,6 +173,8 @@ void eng_wall_distance(std::vector<double> &qd);
 void eng_walls(std::vector<double> &qd, std::vector<double> *qcn, std::vector<double> *qcp);
 double eng_tun_eval(int smat, int nl, const int *mat, const double *t, double psi, double phin, double phip,
                     double Vg, double pm, double *F);
+double eng_lombardi(int mi, double mub, double Fv, double Nt, double Dw, int hole);
+double eng_mlda_factor(int mi, double a, double b, int hole);
 void eng_sync();
 void eng_download(double *dst, const double *src, size_t n);
 void eng_upload(double *dst, const double *src, size_t n);

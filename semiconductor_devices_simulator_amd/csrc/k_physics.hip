@@ -136,6 +136,14 @@ __device__ __forceinline__ double lombardi(const MatDev &m, double mub, double F
     const double musr = Ec > 0. ? 1. / (F * F / Dc + F * F * F / Ec) : Dc / (F * F);
     return 1. / (1. / mub + Dw * 1e4 * (1. / muac + 1. / musr));    // cm^2/Vs -> m^2/Vs
 }
+// one Lombardi evaluation for a material (api3_lombardi :4894-4897)
+__global__ void k_lombardi_eval(MatDev m, double mub, double Fv, double Nt, double Dw, int hole, double *out) {
+    if (threadIdx.x == 0) out[0] = lombardi(m, mub, Fv, Nt, Dw, hole);
+}
+void ph_lombardi_eval(hipStream_t s, const MatDev &m, double mub, double Fv, double Nt, double Dw, int hole,
+                      double *out) {
+    k_lombardi_eval<<<1, 64, 0, s>>>(m, mub, Fv, Nt, Dw, hole, out);
+}
 #define SD_LB_LCRIT 1e-8
 __global__ void k_upd_mob(DevView d, const double *qn, const double *qp) {
     const Geo g = d.g;

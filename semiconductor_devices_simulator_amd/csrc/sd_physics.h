@@ -17,6 +17,8 @@ struct CurEdges {
 void ph_tun_update(hipStream_t s, const DevView &d, const TunDev &T);
 void ph_tun_gather(hipStream_t s, const TunDev &T, int hole, double *tk, double *tg);
 void ph_tun_J(hipStream_t s, const DevView &d, const TunDev &T, double *out);
+void ph_lombardi_eval(hipStream_t s, const MatDev &m, double mub, double Fv, double Nt, double Dw, int hole,
+                      double *out);
 void ph_tun_eval(hipStream_t s, const MatDev *mats, int smat, int nl, const int *mi, const double *t, double psi,
                  double phin, double phip, double Vg, double pm, double Cref, double *F);
 void ph_set_tile_mode(int m);          // c_tile_mode of k_physics.hip

@@ -447,6 +447,23 @@ def tun_eval_fixture(lib, name):
     print("wrote", name, np.array(J))
 
 
+def surface_fixture(lib, name):
+    """The reference's api3_mlda_factor and api3_lombardi over grids of
+    inputs: silicon (two-valley masses) at 300 and 400 K and germanium
+    (single masses); Lombardi over field, doping, distance factor, carrier."""
+    lib.api3_init()
+    mats = [(0, 300.0), (1, 300.0), (0, 400.0)]
+    idx = [lib.api3_add_material(m, T) for m, T in mats]
+    ranges = [(0.0, 0.2e-9), (0.0, 0.5e-9), (0.5e-9, 2e-9), (2e-9, 5e-9), (5e-9, 2e-8), (3e-9, 3.0000001e-9)]
+    ML = np.array([[[lib.api3_mlda_factor(i, a, b, h) for h in (0, 1)] for a, b in ranges] for i in idx])
+    lb_in = [(mub, F, Nt, Dw, h) for h, mub in ((0, 0.14), (1, 0.045)) for F in (0.0, 1e5, 1e7, 1e8, 5e8)
+             for Nt in (0.0, 1e22, 1e24) for Dw in (0.0, 0.3, 1.0)]
+    LB = np.array([[lib.api3_lombardi(i, *x) for x in lb_in] for i in (idx[0], idx[2])])
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), mats=np.array(mats), ranges=np.array(ranges), mlda=ML,
+                        lb_in=np.array(lb_in), lombardi=LB)
+    print("wrote", name, ML.shape, LB.shape)
+
+
 def anderson_fixture(lib, spec, name, Va, rng, steps=8):
     """The reference's Anderson mixing on a prepared MOSFET state: a sequence
     of (x, g) pairs from a synthetic contraction toward a perturbed state
@@ -489,6 +506,9 @@ def anderson_fixture(lib, spec, name, Va, rng, steps=8):
 
 def main():
     lib = load_harness()
+    if "--surface" in sys.argv:        # MLDA factor and Lombardi mobility entry points
+        surface_fixture(lib, "surface_models")
+        return
     if "--tuneval" in sys.argv:        # tunnelling fluxes through a given stack
         tun_eval_fixture(lib, "tun_eval_stacks")
         return

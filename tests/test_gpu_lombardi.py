@@ -73,3 +73,23 @@ def test_mlda_sweep_matches_reference(hip, precision):
     qn = np.zeros(spec.N)
     hip.api3_bulk_copy(28, dptr(qn))
     assert np.nanmin(qn) >= 0.0 and 0.0 < np.nanmax(qn) < 0.05     # eV, positive near the gate only
+
+
+def test_surface_entry_points_match_reference(hip):
+    """api3_mlda_factor (the host setup function of prepare) and api3_lombardi
+    (the device function of the mobility kernel) against the reference's for
+    silicon at 300 and 400 K and germanium (tests/golden/surface_models.npz)."""
+    print("\n[gpu lombardi] api3_mlda_factor, api3_lombardi entry points", flush=True)
+    g = load_golden("surface_models")
+    hip.api3_init()
+    idx = [hip.api3_add_material(int(m), float(T)) for m, T in g["mats"]]
+    for r, i in enumerate(idx):
+        for q, (a, b) in enumerate(g["ranges"]):
+            rtol = 1e-13 + 4e-16 * b / (b - a)          # cancellation of the erf difference on thin cells
+            got = [hip.api3_mlda_factor(i, a, b, h) for h in (0, 1)]
+            np.testing.assert_allclose(got, g["mlda"][r][q], rtol=rtol, atol=0)
+    for r, i in enumerate((idx[0], idx[2])):
+        got = [hip.api3_lombardi(i, mub, F, Nt, Dw, int(h)) for mub, F, Nt, Dw, h in g["lb_in"]]
+        np.testing.assert_allclose(got, g["lombardi"][r], rtol=1e-13, atol=0)
+    assert hip.api3_lombardi(99, 0.14, 1e7, 1e23, 1.0, 0) == 0.14        # bad index: the bulk mobility
+    assert hip.api3_mlda_factor(99, 0.0, 1e-9, 0) == 1.0

@@ -57,3 +57,13 @@ def test_lombardi_closed_forms():
     assert O.lombardi(m, mub, 0.0, 0.0, 1.0, 1) == pytest.approx(O.lombardi(m, mub, 1e2, 1e6, 1.0, 1), rel=1e-15)
     # a stronger normal field lowers the surface mobility
     assert O.lombardi(m, mub, 1e8, 1e23, 1.0, 0) < O.lombardi(m, mub, 1e7, 1e23, 1.0, 0)
+
+
+def test_entry_point_values_match_reference():
+    """api3_lombardi of the reference for silicon at 300 and 400 K over field,
+    doping, distance factor and carrier (tests/golden/surface_models.npz)."""
+    g = load_golden("surface_models")
+    for r, T in enumerate((300.0, 400.0)):
+        m = O.silicon(T)
+        got = [O.lombardi(m, mub, F, Nt, Dw, int(h)) for mub, F, Nt, Dw, h in g["lb_in"]]
+        np.testing.assert_allclose(got, g["lombardi"][r], rtol=1e-13, atol=0)

@@ -46,3 +46,16 @@ def test_mlda_closed_forms():
     l2 = O.HBAR / math.sqrt(2 * 0.19 * O.M0 * kT)
     lo, hi = sorted((O.mlda_pop(0.5e-9, 1.5e-9, l1), O.mlda_pop(0.5e-9, 1.5e-9, l2)))
     assert lo <= f2 <= hi
+
+
+def test_entry_point_values_match_reference():
+    """api3_mlda_factor of the reference for silicon at 300 and 400 K over six
+    cell ranges, both carriers (tests/golden/surface_models.npz)."""
+    g = load_golden("surface_models")
+    for r, T in ((0, 300.0), (2, 400.0)):
+        m = O.silicon(T)
+        for q, (a, b) in enumerate(g["ranges"]):
+            # the erf-difference form loses b/(b-a) ulps to cancellation on thin cells
+            rtol = 1e-13 + 4e-16 * b / (b - a)
+            got = [O.mlda_factor(m, a, b, h) for h in (0, 1)]
+            np.testing.assert_allclose(got, g["mlda"][r][q], rtol=rtol, atol=0)
