@@ -42,7 +42,7 @@ int    api3_add_contact_ex(int face, int i0, int i1, int j0, int j1, int k0, int
                            int bc, double V, double pm, double tox, double Nox); /* :4757 */
 void   api3_add_contact(int face, int i0, int i1, int j0, int j1, int bc, double V, double pm); /* :4772 */
 void   api3_clear_contacts(void);                                         /* :4775 */
-int    api3_set_gate_stack(int cid, int nl, const int *mat, const double *t); /* :4779 */
+int    api3_set_gate_stack(int cid, int nl, const int *mat, const double *t); /* :4779 (tunnelling, models bit 5) */
 int    api3_set_mesh_graph(int N, int E, const int *rp, const int *cj, const double *h,
                            const double *ar, const double *vol, const double *pos); /* :4795 (rejected: prism mesh) */
 int    api3_is_graph(void);                                               /* :4806 */
@@ -51,11 +51,11 @@ int    api3_set_con_nodes(int cid, int n, const int *nodes, const double *areas)
 void   api3_set_walls(const double *w);                                   /* :4838 */
 int    api3_get_ntp(int cid);                                             /* :4844 */
 double api3_tun_eval(int smat, int nl, const int *mat, const double *t, double psi,
-                     double phin, double phip, double Vg, double pm, double *out); /* :4855 (rejected) */
-double api3_mlda_factor(int mi, double a, double b, int hole);           /* :4869 (rejected) */
-double api3_coulomb_inv(int mi, double F, double n, double NC, double NA); /* :4878 */
+                     double phin, double phip, double Vg, double pm, double *out); /* :4855 */
+double api3_mlda_factor(int mi, double a, double b, int hole);           /* :4869 */
+double api3_coulomb_inv(int mi, double F, double n, double NC, double NA); /* :4878 (NaN: no SiC Coulomb term) */
 int    api3_set_surface_mobility(int mi, double delta, double eta, double mu1); /* :4885 */
-double api3_lombardi(int mi, double mub, double F, double Nt, double Dw, int hole); /* :4894 (rejected) */
+double api3_lombardi(int mi, double mub, double F, double Nt, double Dw, int hole); /* :4894 */
 /* contacts and results */
 void   api3_set_contact_V(int cid, double V);                             /* :4898 */
 int    api3_get_ncon(void);                                               /* :4899 */
@@ -96,8 +96,8 @@ double api3_mat_ni(int mi);                                               /* :50
 double api3_mat_Eg(int mi);                                               /* :5049 */
 void   api3_bulk_copy(int which, double *out);                            /* :5127 */
 void   api3_carrier_sums(double *out);                                    /* :5137 */
-int    api3_charge_partition(const int *cterm);                           /* :5192 (rejected: C-V) */
-int    api3_terminal_charges(int nt, double *Qt);                         /* :5248 (rejected: C-V) */
+int    api3_charge_partition(const int *cterm);                           /* :5192 */
+int    api3_terminal_charges(int nt, double *Qt);                         /* :5248 */
 int    api3_face_J(int ax, int carrier, double *out);                     /* :5284 */
 int    api3_charge_owner(int *out);                                       /* :5300 */
 double api3_ins_field_max(double *pos);                                   /* :5308 */
