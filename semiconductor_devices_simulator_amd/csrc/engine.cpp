@@ -473,18 +473,40 @@ void eng_wall_distance(std::vector<double> &qd) { eng_walls(qd, nullptr, nullptr
 // Paths through meshed insulator layers to a metal gate (:1214-1237) are not
 // built: such a device is rejected.
 #define TUN_COL 5e-9
+#define TUN_LMAX 30e-9
+#define TUN_MAXL 6
 static int tun_build() {
     G.tp_k.clear(); G.tp_c.clear(); G.tp_c0.clear(); G.tp_cn.clear(); G.col_k.clear();
     G.tp_A.clear(); G.col_l.clear(); G.col_h.clear();
     for (int c = 0; c < G.n_cons; c++) G.cons[c].ntp = 0;
     if (!(G.models & 32)) return 1;
+    // a path through meshed insulator nodes exists where the straight march
+    // from a semiconductor node through insulator (at most TUN_LMAX of it,
+    // TUN_MAXL material runs) ends on the metal node of a gate (:1214-1237)
     for (int k = 0; k < G.N; k++) {
         if (G.kind[k] != K_SEMI || G.con_mask[k]) continue;
         int i, j, l;
         host_unidx(k, i, j, l);
         for (int dir = 0; dir < 6; dir++) {
             int kb; double h, ar;
-            if (host_nbr(i, j, l, k, dir, kb, h, ar) && G.kind[kb] == K_INS && !G.con_mask[kb]) {
+            if (!host_nbr(i, j, l, k, dir, kb, h, ar) || G.kind[kb] != K_INS || G.con_mask[kb]) continue;
+            int nl = 1, last = -1, ok = 0, cur = kb;
+            double tins = 0., hin = h;
+            for (int guard = 0; guard < 100000; guard++) {
+                int ci, cj, cl, kn; double h2, ar2;
+                host_unidx(cur, ci, cj, cl);
+                if (!host_nbr(ci, cj, cl, cur, dir, kn, h2, ar2)) break;                // domain edge
+                int mi = G.mat_idx[cur];
+                bool end = G.kind[kn] == K_METAL && G.con_mask[kn] && G.cons[G.con_mask[kn] - 1].bc == 2;
+                double seg = 0.5 * hin + (end ? h2 : 0.5 * h2);
+                if (!(nl > 1 && last == mi)) { if (nl >= TUN_MAXL) break; last = mi; nl++; }
+                tins += seg;
+                if (tins > TUN_LMAX) break;
+                if (end) { ok = 1; break; }
+                if (G.kind[kn] != K_INS || G.con_mask[kn]) break;
+                cur = kn; hin = h2;
+            }
+            if (ok) {
                 fprintf(stderr, "semisim-hip: gate tunnelling through meshed insulator layers is outside this core: "
                                 "not solved\n");
                 return 0;
