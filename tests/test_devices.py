@@ -91,3 +91,12 @@ def test_vdmos_cell_structure():
     assert all(c.tox == 50.0 and c.pm == 4.1 for c in v.contacts if c.bc == 2)
     full = D.vdmos()
     assert full.N == 8388608 and full.bias_points() == 61
+
+
+def test_gate_tunnelling_spec():
+    spec = devices.with_gate_tunnelling(devices.mosfet(12, 8, 6, tox=1.5))
+    assert spec.models & 32 and spec.models & 7 == 7
+    gates = [i for i, c in enumerate(spec.contacts) if c.bc == 2]
+    assert gates == [2]
+    assert spec.stacks == {2: [(devices.MAT_SIO2, 1.5)]}
+    assert spec.name.endswith("_tun")
