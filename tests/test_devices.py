@@ -94,9 +94,9 @@ def test_vdmos_cell_structure():
 
 
 def test_gate_tunnelling_spec():
-    spec = devices.with_gate_tunnelling(devices.mosfet(12, 8, 6, tox=1.5))
+    spec = D.with_gate_tunnelling(D.mosfet(12, 8, 6, tox=1.5))
     assert spec.models & 32 and spec.models & 7 == 7
     gates = [i for i, c in enumerate(spec.contacts) if c.bc == 2]
     assert gates == [2]
-    assert spec.stacks == {2: [(devices.MAT_SIO2, 1.5)]}
+    assert spec.stacks == {2: [(D.MAT_SIO2, 1.5)]}
     assert spec.name.endswith("_tun")
