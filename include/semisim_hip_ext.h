@@ -26,6 +26,7 @@ void sdx_reset_stats(void);
    launches, out[1] their total ms, out[2] algorithmic bytes per launch,
    out[3] launches seen.  Reading stops the timer. */
 void sdx_ktimer_enable(int kind, int every);
+void sdx_ktimer_pause(int paused);        /* 1: stop sampling, keep what was gathered; 0: resume */
 int  sdx_ktimer_read(double *out);
 
 /* kernel microbenchmark on the solved device: ms per launch of `what`
@@ -82,6 +83,11 @@ void   sdx_ws_reset_stats(void);
 void   sdx_ws_config(int chmask, double thr);   /* fields corrected: psi 1, phi_n 2, phi_p 4; smallest correction (V) */
 void   sdx_ws_accept(double ratio);              /* mode 2 keeps a correction whose starting Poisson residual is
                                                     below ratio x the plain prediction's (default 0.5) */
+/* floating majority blocks (float_fix, semiconductor_3d.c:3159-3228) since
+   the last reset: out[0] continuity solves that listed unanchored majority
+   nodes, out[1] blocks found, out[2] blocks rescaled to their balance */
+void   sdx_float_stats(double *out);
+void   sdx_float_reset_stats(void);
 /* network: G a power of two >= 8, channels multiples of 32 with c3 == c2 */
 int    sdx_nn_init(int G, int c0, int c1, int c2, int c3, unsigned seed);
 void   sdx_nn_free(void);
@@ -97,6 +103,21 @@ int    sdx_nn_dataset_size(void);
 int    sdx_nn_dataset_get(int i, float *in8, float *tgt3);
 double sdx_nn_train(int steps, int n_train, double lr, unsigned seed, float *loss);  /* Adam; mean loss */
 double sdx_nn_eval(int i0, int i1);                     /* mean loss over samples [i0, i1) */
+
+/* ---- CSR export of a 7-point stencil operator ----
+   The reference stores tensor-mesh operators as seven diagonals (Sten,
+   semiconductor_3d.c:210) with no CSR form; this lists every nonzero entry of
+   such an operator in CSR, columns ascending within a row (z-, y-, x-,
+   diagonal, x+, y+, z+), so that its sparsity pattern can be compared bit for
+   bit with the reference's assembled operator.  diag: a host array of 7
+   device pointers c, xm, xp, ym, yp, zm, zp (N = Nx Ny Nz doubles each, node
+   k = i + Nx (j + Ny l); c null: unit diagonal); rowptr (N + 1), col and val
+   (cap entries) are device arrays.  Returns the number of nonzeros (entries
+   past cap are not written), -1 bad arguments, -2 more nonzeros than an int
+   row pointer holds, -3 a HIP call failed.  Allocates a scratch of one int
+   per 1024 rows for the call; synchronises the null stream. */
+long long sdx_csr_from_stencil(int Nx, int Ny, int Nz, const double *const *diag, int *rowptr, int *col,
+                               double *val, long long cap);
 
 /* ---- kernel-level hooks on the prepared device (tests) ----
    A7: 7 blocks of N doubles (c, xm, xp, ym, yp, zm, zp). */
@@ -125,6 +146,9 @@ int    sdx_test_tun(const double *Va, int hole, double *cK, double *pG, double *
 void   sdx_test_aa_step(const double *x3, double *g3, int reset);
                                                    /* one Anderson step: x3, g3 the 3 blocks psi|phin|phip,
                                                       g3 overwritten by the mixed iterate; reset clears the history */
+long long sdx_test_csr(int *rowptr, int *col, double *val, long long cap);
+                                                   /* CSR of the operator last assembled (host arrays: N + 1
+                                                      row pointers, cap entries); returns the nonzeros */
 int    sdx_test_maj_flags(int hole, const double *u, const double *sc, unsigned char *f0, unsigned char *f1);
                                                    /* majority flags before/after the anchoring closure */
 

@@ -566,6 +566,27 @@ def matvec(A, dims, x, b=None):
     return y if b is None else b - y
 
 
+def stencil_csr(A, dims):
+    """CSR (rowptr, col, val) of a 7-point stencil operator given as its seven
+    diagonals (c, xm, xp, ym, yp, zm, zp in the reference's Sten layout,
+    semiconductor_3d.c:210): every stored entry that is nonzero, columns
+    ascending within a row.  The reference keeps no CSR of tensor-mesh
+    operators; its matrix (the operator matvec_L :1975-1993 applies) holds
+    exactly these entries."""
+    Nx, Ny, Nz = dims
+    N = Nx * Ny * Nz
+    k = np.arange(N)
+    i, j, l = k % Nx, (k // Nx) % Ny, k // (Nx * Ny)
+    c, xm, xp, ym, yp, zm, zp = (np.asarray(a, np.float64).reshape(N) for a in A)
+    slots = [(l > 0, k - Nx * Ny, zm), (j > 0, k - Nx, ym), (i > 0, k - 1, xm), (np.ones(N, bool), k, c),
+             (i < Nx - 1, k + 1, xp), (j < Ny - 1, k + Nx, yp), (l < Nz - 1, k + Nx * Ny, zp)]
+    present = np.stack([ok & (v != 0.0) for ok, _, v in slots], axis=1)          # N x 7, column order
+    cols = np.stack([cc for _, cc, _ in slots], axis=1)
+    vals = np.stack([v for _, _, v in slots], axis=1)
+    rowptr = np.concatenate([[0], np.cumsum(present.sum(axis=1))]).astype(np.int64)
+    return rowptr, cols[present].astype(np.int64), vals[present]
+
+
 def mg_hierarchy(dims):
     """Level shapes: 2x2x2 aggregation of every axis longer than 2 until
     N <= 64, mg_alloc semiconductor_3d.c:1894-1921.  Returns [(Nx,Ny,Nz,cx,cy,cz)]."""

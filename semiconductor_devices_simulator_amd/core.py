@@ -98,9 +98,12 @@ EXT_SIGNATURES = [
     ("sdx_tune", None, [I, I]),
     ("sdx_ktimer_enable", None, [I, I]),
     ("sdx_ktimer_read", I, [DP]),
+    ("sdx_ktimer_pause", None, [I]),
     ("sdx_ws_mode", None, [I]),
     ("sdx_ws_stats", None, [DP]),
     ("sdx_ws_reset_stats", None, []),
+    ("sdx_float_stats", None, [DP]),
+    ("sdx_float_reset_stats", None, []),
     ("sdx_ws_config", None, [I, D]),
     ("sdx_ws_accept", None, [D]),
     ("sdx_nn_init", I, [I, I, I, I, I, ctypes.c_uint]),
@@ -138,6 +141,9 @@ EXT_SIGNATURES = [
     ("sdx_test_aa_step", None, [DP, DP, I]),
     ("sdx_test_tun_paths", I, [IP, IP, IP, IP, IP, DP, IP, DP, DP]),
     ("sdx_test_tun", I, [DP, I, DP, DP, DP, DP]),
+    ("sdx_test_csr", LL, [IP, IP, DP, LL]),
+    ("sdx_csr_from_stencil", LL, [I, I, I, ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, LL]),
     ("sdx_test_maj_flags", I, [I, DP, DP, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(ctypes.c_ubyte)]),
 ]
 

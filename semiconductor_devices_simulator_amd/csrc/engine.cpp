@@ -8,6 +8,7 @@
 // work runs on the device (k_physics.hip, k_linalg.hip); the host keeps the
 // scalar decisions, reading a handful of reduction results per iteration.
 #include "engine.h"
+#include "sd_csr.h"
 #include <type_traits>
 #include "sd_nn.h"
 #include <chrono>
@@ -889,6 +890,7 @@ static void mg_setup() {
 // figures; off unless sdx_ktimer_enable is called.
 struct KTimer {
     int on = 0, kind = 0, every = 8;
+    int armed = 0;                     // enabled and not yet read (pause/resume toggles `on`)
     bool sampling = false;             // kind 0: the running V-cycle is timed
     long long count = 0, sampled = 0, vcycles = 0;
     double total_ms = 0.;
@@ -1438,6 +1440,7 @@ static int float_fix_host(int hole, int m) {
         eng_sync();                             // lamv is pageable host memory
     }
     (void)N;
+    G.ff_stat[0]++; G.ff_stat[1] += blk; G.ff_stat[2] += nfix;
     if (G.verbose > 2) fprintf(stderr, "    float_fix(%c): %d unanchored nodes, %d blocks, %d rescaled\n",
                                hole ? 'p' : 'n', m, blk, nfix);
     return nfix;
@@ -2116,6 +2119,10 @@ void sdx_ws_mode(int mode) { G.ws_mode = mode >= 1 && mode <= 3 ? mode : 0; }
 // out[0] predictions tried, [1] kept, [2] rejected (plain prediction used), [3] samples recorded
 void sdx_ws_stats(double *out) { for (int i = 0; i < 4; i++) out[i] = (double)G.ws_stat[i]; }
 void sdx_ws_reset_stats(void) { for (int i = 0; i < 4; i++) G.ws_stat[i] = 0; }
+// floating majority blocks since the last reset: [0] continuity solves that
+// listed unanchored majority nodes, [1] blocks found, [2] blocks rescaled
+void sdx_float_stats(double *out) { for (int i = 0; i < 3; i++) out[i] = (double)G.ff_stat[i]; }
+void sdx_float_reset_stats(void) { for (int i = 0; i < 3; i++) G.ff_stat[i] = 0; }
 // which of psi (1), phi_n (2), phi_p (4) the correction changes, and the
 // smallest correction applied (V)
 void sdx_ws_config(int chmask, double thr) { G.ws_chmask = chmask & 7; G.ws_thr = thr > 0. ? thr : 0.; }
@@ -2124,8 +2131,13 @@ void sdx_ws_config(int chmask, double thr) { G.ws_chmask = chmask & 7; G.ws_thr 
 void sdx_ws_accept(double ratio) { G.ws_accept = ratio > 0. ? ratio : 0.5; }
 void sdx_ktimer_enable(int kind, int every) {
     kt_flush();
-    KT.on = 1; KT.kind = kind; KT.every = every > 0 ? every : 8;
+    KT.on = KT.armed = 1; KT.kind = kind; KT.every = every > 0 ? every : 8;
     KT.count = KT.sampled = 0; KT.total_ms = 0.;
+}
+// pause (1) or resume (0) the sampling without resetting what it gathered
+void sdx_ktimer_pause(int paused) {
+    if (paused) { KT.on = 0; KT.sampling = false; }
+    else KT.on = KT.armed;
 }
 // out: [sampled launches, total ms of them, algorithmic bytes per launch, launches seen]
 int sdx_ktimer_read(double *out) {
@@ -2146,7 +2158,7 @@ int sdx_ktimer_read(double *out) {
         bytes = (double)G.N * 11.0 * 8.0;   // 7 diagonals + x + dot vector(s) + y (FP64)
     }
     out[0] = (double)KT.sampled; out[1] = KT.total_ms; out[2] = bytes; out[3] = (double)KT.count;
-    KT.on = 0;
+    KT.on = KT.armed = 0;
     return 1;
 }
 // majority-node flags of carrier `hole` for the quantities u, sc: before
@@ -2192,6 +2204,33 @@ void sdx_test_cont(int hole, double *A7, double *rhs, double *sc, double *dsc) {
     ph_cont(G.st, eng_view(), hole, G.A, G.d_rhs, G.d_sc, G.d_dsc);
     get_A7(A7);
     eng_download(rhs, G.d_rhs, G.N); eng_download(sc, G.d_sc, G.N); eng_download(dsc, G.d_dsc, G.N);
+}
+// CSR of the device operator last assembled (Poisson Jacobian or scaled
+// continuity rows) into host arrays rowptr (N + 1), col and val (cap
+// entries); returns the number of nonzeros (< 0: error)
+long long sdx_test_csr(int *rowptr, int *col, double *val, long long cap) {
+    const size_t N = (size_t)G.N;
+    int *d_rp = nullptr, *d_col = nullptr, *d_s = nullptr;
+    double *d_val = nullptr;
+    long long nnz = SD_CSR_EDEVICE;
+    const size_t c = cap > 0 ? (size_t)cap : 1;
+    if (hipMalloc(&d_rp, (N + 1) * sizeof(int)) == hipSuccess && hipMalloc(&d_col, c * sizeof(int)) == hipSuccess &&
+        hipMalloc(&d_val, c * sizeof(double)) == hipSuccess &&
+        hipMalloc(&d_s, (size_t)csr_scratch_words((long)N) * sizeof(int)) == hipSuccess) {
+        const double *diag[7] = {G.A.c, G.A.xm, G.A.xp, G.A.ym, G.A.yp, G.A.zm, G.A.zp};
+        nnz = csr_from_stencil(G.st, G.Nx, G.Ny, G.Nz, diag, d_rp, d_col, d_val, cap, d_s);
+        if (nnz >= 0) {
+            HC(hipMemcpyAsync(rowptr, d_rp, (N + 1) * sizeof(int), hipMemcpyDeviceToHost, G.st));
+            const size_t m = (size_t)std::min<long long>(nnz, cap);
+            if (m) {
+                HC(hipMemcpyAsync(col, d_col, m * sizeof(int), hipMemcpyDeviceToHost, G.st));
+                HC(hipMemcpyAsync(val, d_val, m * sizeof(double), hipMemcpyDeviceToHost, G.st));
+            }
+            eng_sync();
+        }
+    }
+    for (void *p : {(void *)d_rp, (void *)d_col, (void *)d_val, (void *)d_s}) if (p) hipFree(p);
+    return nnz;
 }
 void sdx_test_load_op(const double *A7, int qsign, const double *w) {
     double *a[7] = {G.A.c, G.A.xm, G.A.xp, G.A.ym, G.A.yp, G.A.zm, G.A.zp};

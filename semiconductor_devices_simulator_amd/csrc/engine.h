@@ -149,6 +149,9 @@ struct Engine {
 
     // statistics of the kernels (see sdx_kernel_stats)
     long long n_vcycles = 0, n_spmv = 0;
+    // floating majority blocks: continuity solves that listed unanchored
+    // nodes, blocks found, blocks rescaled (see sdx_float_stats)
+    long long ff_stat[3] = {0, 0, 0};
 };
 
 extern Engine G;

@@ -299,6 +299,81 @@ def vdmos(Nx=256, Ny=128, Nz=256, vd_max=60.0, npts=61, vgs_on=10.0, npts_on=0, 
                       models=7 | 256, sweep=sweep, solver=(2000, 1000, 500, 1e-10, tol_c, 1.0, 1.0))
 
 
+def soi_mosfet(Nx=24, Ny=16, Nz=8, tsi=80.0, tbox=100.0, lg=300.0, nbody=1e18, nsd=1e20, tox=1.5,
+               vds=(0.05, 1.0), npts=7, tunnelling=True):
+    """Floating-body SOI n-MOSFET: a tsi nm silicon film (p-body nbody cm^-3,
+    n+ source/drain implants from the top reaching through the film) on a
+    tbox nm buried oxide whose bottom face is a metal back gate at 0 V (an
+    ohmic contact on insulator nodes), a Robin front gate of tox nm SiO2
+    (work function 4.65 eV) and no body contact.  The p-body is a hole-majority
+    block touching no contact, so the continuity solves rescale it to its
+    global balance (the reference's float_fix); with `tunnelling` the gate
+    stack's tunnelling sink/source (models bit 5) enters that balance.
+    Id-Vg 0 -> 1.2 V in npts points at every drain voltage in vds."""
+    Lx, Ly, Lz = 400.0 + lg, tsi + tbox, 200.0
+    xs = graded_grid(Lx, Nx, [200.0, 200.0 + lg])
+    ys = graded_grid(Ly, Ny, [0.0, tsi])
+    zs = np.linspace(0, Lz * 1e-9, Nz)
+    N = Nx * Ny * Nz
+    P = node_positions_nm(xs, ys, zs)
+    L3 = (Lx, Ly, Lz)
+    j0, j1 = span_idx(ys, 0.0, tsi * 1e-9)
+    yl = np.tile(np.repeat(np.arange(Ny), Nx), Nz)
+    film = yl <= j1
+    mat = np.where(film, 0, 1).astype(np.int32)
+    Na = np.where(film, nbody * 1e6, 0.0)
+    nd = Implant("n", nsd, "y0", 0.0, 200.0, 0.0, 1e9, "gauss", tsi, 0.0, 0.8).conc(P, L3)
+    nd += Implant("n", nsd, "y0", 200.0 + lg, 1e9, 0.0, 1e9, "gauss", tsi, 0.0, 0.8).conc(P, L3)
+    Nd = np.where(film, nd * 1e6, 0.0)
+    a, b = 200.0 / Lx, (200.0 + lg) / Lx
+    cons = [Contact(2, 0.0, a, 0.0, 1.0, 0, 0.0, label="Source"),
+            Contact(2, b, 1.0, 0.0, 1.0, 0, vds[0], label="Drain"),
+            Contact(2, a, b, 0.0, 1.0, 2, 0.0, pm=4.65, tox=tox, label="Gate"),
+            Contact(3, 0.0, 1.0, 0.0, 1.0, 0, 0.0, label="BackGate")]
+    sweep = [(2, 0.0, 1.2, npts, {1: vd}) for vd in vds]
+    spec = DeviceSpec("soi_mosfet_%dx%dx%d" % (Nx, Ny, Nz), L3, xs, ys, zs, mat, Nd, Na, cons,
+                      materials=[MAT_SI, MAT_SIO2], models=7, sweep=sweep)
+    if tunnelling:
+        spec = with_gate_tunnelling(spec, gates=[2])
+    return spec
+
+
+def guard_ring_diode(Nx=32, Ny=16, Nz=4, v_end=-10.0, npts=11, gate_v=None, tox=2.0):
+    """Planar p+/n- diode with a floating guard ring, 8 x 4 x 1 um (x lateral,
+    y depth with the top at y = 0, z width).  n- drift 1e15 cm^-3 over an n+
+    substrate (1e19 cm^-3 below y = 3.5 um, cathode on the bottom face); p+
+    main junction (Gaussian, peak 1e19 cm^-3, Lv 0.5 um) under the anode
+    window x < 2 um; a second p+ implant, the guard ring, at 4.5-5.3 um with no
+    contact.  The ring is a hole-majority block that touches no contact, so
+    every hole continuity solve rescales it to its global balance (the
+    reference's float_fix).  With gate_v set, a Robin gate of tox nm SiO2 over
+    the ring (work function 4.65 eV) is held at gate_v with gate tunnelling on
+    (models bit 5), so that the tunnelling sink/source enters the ring's
+    balance.  Reverse sweep of the anode 0 -> v_end in npts points."""
+    Lx, Ly, Lz = 8000.0, 4000.0, 1000.0
+    xs = graded_grid(Lx, Nx, [2000.0, 4500.0, 5300.0])
+    ys = graded_grid(Ly, Ny, [0.0, 600.0, 3500.0])
+    zs = np.linspace(0, Lz * 1e-9, Nz)
+    N = Nx * Ny * Nz
+    mat = np.zeros(N, np.int32)
+    P = node_positions_nm(xs, ys, zs)
+    L3 = (Lx, Ly, Lz)
+    Nd = np.full(N, 1e15) + np.where(P[:, 1] >= 3500.0, 1e19, 0.0)
+    Na = Implant("p", 1e19, "y0", 0.0, 2000.0, 0.0, 1e9, "gauss", 500.0, 0.0, 0.8).conc(P, L3)
+    Na += Implant("p", 1e19, "y0", 4500.0, 5300.0, 0.0, 1e9, "gauss", 500.0, 0.0, 0.8).conc(P, L3)
+    cons = [Contact(2, 0.0, 1500.0 / Lx, 0.0, 1.0, 0, 0.0, label="Anode"),
+            Contact(3, 0.0, 1.0, 0.0, 1.0, 0, 0.0, label="Cathode")]
+    spec = DeviceSpec("guard_ring_diode_%dx%dx%d" % (Nx, Ny, Nz), L3, xs, ys, zs, mat, Nd * 1e6, Na * 1e6, cons,
+                      models=7, sweep=[(0, 0.0, v_end, npts, {})])
+    if gate_v is not None:
+        spec.contacts.append(Contact(2, 4500.0 / Lx, 5300.0 / Lx, 0.0, 1.0, 2, gate_v, pm=4.65, tox=tox,
+                                     label="Gate"))
+        spec.sweep = [(0, 0.0, v_end, npts, {2: gate_v})]
+        spec = with_gate_tunnelling(spec, gates=[2])
+        spec.name = spec.name.replace("guard_ring_diode", "guard_ring_gate_diode")
+    return spec
+
+
 def random_mosfet(seed, Nx=96, Ny=64, Nz=64, npts=25):
     """A MOSFET of the warm-start training family, drawn with
     numpy.random.default_rng(seed): substrate log-uniform 1e15-1e18 cm^-3,

@@ -79,3 +79,23 @@ def load_fixture_device(lib, g):
     psi, phin, phip = (np.ascontiguousarray(g[k], np.float64) for k in ("psi", "phin", "phip"))
     lib.sdx_test_set_state(dptr(psi), dptr(phin), dptr(phip))
     return N
+
+
+def current_parity(I, I_ref, F_ref):
+    """Terminal-current parity against the reference: |I - I_ref| <= 1e-8
+    |I_ref| + floor_ref on every terminal of every bias point, with floor_ref
+    the reference's own resolution floor of that current
+    (api3_get_contact_Ifloor; run_continuation, semiconductor_3d.c:4151-4171).
+    Returns (ok mask, largest |I - I_ref| / (1e-8 |I_ref| + floor_ref))."""
+    I, I_ref, F_ref = (np.asarray(a, np.float64) for a in (I, I_ref, F_ref))
+    bound = 1e-8 * np.abs(I_ref) + F_ref
+    d = np.abs(I - I_ref)
+    ratio = d / np.maximum(bound, 1e-300)
+    return d <= bound, float(ratio.max()) if ratio.size else 0.0
+
+
+def assert_current_parity(I, I_ref, F_ref, what=""):
+    ok, worst = current_parity(I, I_ref, F_ref)
+    print(f"[parity] {what} worst |dI|/(1e-8|I_ref|+floor_ref) = {worst:.3g}", flush=True)
+    assert ok.all(), (what, np.argwhere(~ok), np.abs(np.asarray(I) - I_ref)[~ok], np.asarray(I_ref)[~ok],
+                      np.asarray(F_ref)[~ok])

@@ -504,8 +504,72 @@ def anderson_fixture(lib, spec, name, Va, rng, steps=8):
           "mixing moved g by", [float(np.max(np.abs(a - b))) for a, b in zip(Gout, Gin)])
 
 
+def graded_grid_fixture(name):
+    """Input/output vectors of the reference front end's graded_grid and
+    span_idx (main.py:316-384).  main.py cannot be imported here (it needs
+    Tk), so the two functions alone are taken from its syntax tree and run
+    with numpy; only their inputs and outputs are written."""
+    import ast
+    src = open("/root/reference/main.py").read()
+    tree = ast.parse(src)
+    keep = [n for n in tree.body if isinstance(n, ast.FunctionDef) and n.name in ("graded_grid", "span_idx")]
+    assert len(keep) == 2, "graded_grid/span_idx not found"
+    ns = {"np": np, "numpy": np}
+    exec(compile(ast.Module(body=keep, type_ignores=[]), "main.py", "exec"), ns)
+    cases = [(4000.0, 64, [2000.0], 8.0, None), (800.0, 96, [200.0, 400.0], 8.0, None),
+             (300.0, 64, [0.0, 40.0], 8.0, None), (600.0, 192, [200.0, 400.0], 8.0, None),
+             (20000.0, 256, [4000.0, 7000.0, 13000.0, 16000.0], 8.0, None), (4000.0, 128, [0.0, 1000.0, 3500.0], 8.0, None),
+             (1000.0, 16, [], 8.0, None), (1000.0, 40, [500.0], 4.0, 50.0), (8000.0, 32, [2000.0, 4500.0, 5300.0], 8.0, None),
+             (150.0, 16, [0.0, 80.0], 8.0, None)]
+    out = {"cases": json.dumps(cases)}
+    for i, (L, n, dense, ratio, sig) in enumerate(cases):
+        out[f"g{i}"] = np.asarray(ns["graded_grid"](L, n, dense, ratio, sig), np.float64)
+    rng = np.random.default_rng(2024)
+    spans = []
+    for i in range(len(cases)):
+        c = out[f"g{i}"]
+        for _ in range(12):
+            a, b = np.sort(rng.uniform(-0.1, 1.1, 2) * c[-1])
+            spans.append((i, float(a), float(b), *[int(v) for v in ns["span_idx"](c, a, b)]))
+        for a, b in ((0.0, c[-1]), (c[1], c[1]), (c[2], c[3]), (0.5 * (c[3] + c[4]), 0.5 * (c[3] + c[4]) + 1e-15)):
+            spans.append((i, float(a), float(b), *[int(v) for v in ns["span_idx"](c, a, b)]))
+    out["spans"] = np.array(spans, np.float64)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print("wrote", name, len(cases), "grids", len(spans), "spans")
+
+
 def main():
+    if "--grid" in sys.argv:           # front-end mesh helpers (no core needed)
+        graded_grid_fixture("graded_grid")
+        return
     lib = load_harness()
+    if "--float" in sys.argv:          # floating majority blocks (float_fix)
+        sweep_fixture(lib, devices.guard_ring_diode(), "e2e_ring_diode_32x16x4")
+        sweep_fixture(lib, devices.guard_ring_diode(gate_v=2.0), "e2e_ring_gate_diode_32x16x4")
+        return
+    if "--combined" in sys.argv:       # BGN, tau(N), v-sat, MLDA, Lombardi and gate tunnelling together
+        spec = devices.with_gate_tunnelling(devices.mosfet(12, 8, 6, tox=1.5))
+        spec.models |= 8 | 16
+        tunnel_fixture(lib, spec, "k_tun_mlda_lb_mosfet_12x8x6", [0.0, 0.5, 1.5, 0.0], np.random.default_rng(95))
+        spec = devices.with_gate_tunnelling(devices.mosfet(24, 16, 8, npts=7, tox=1.5))
+        spec.models |= 8 | 16
+        sweep_fixture(lib, spec, "e2e_m63_mosfet_24x16x8")
+        return
+    if "--combined96" in sys.argv:     # the combined model set on the 96x64x64 MOSFET (about 15 minutes)
+        spec = devices.with_gate_tunnelling(devices.mosfet(96, 64, 64, tox=1.5))
+        spec.models |= 8 | 16
+        sweep_fixture(lib, spec, "e2e_m63_mosfet_96x64x64", fields=False, sample=4096)
+        return
+    if "--tight" in sys.argv:          # the 96x64x64 sweep converged far below the default tolerances
+        spec = devices.mosfet(96, 64, 64)
+        spec.solver = (2000, 1000, 500, 1e-12, 1e-11, 1.0, 1.0)
+        sweep_fixture(lib, spec, "e2e_mosfet_96x64x64_tight", fields=False, sample=4096)
+        return
+    if "--tight192" in sys.argv:       # the same for the 192x128x128 sweep (about 1.5 hours on 8 cores)
+        spec = devices.mosfet(192, 128, 128)
+        spec.solver = (2000, 1000, 500, 1e-12, 1e-11, 1.0, 1.0)
+        sweep_fixture(lib, spec, "e2e_mosfet_192x128x128_tight", fields=False, sample=4096)
+        return
     if "--surface" in sys.argv:        # MLDA factor and Lombardi mobility entry points
         surface_fixture(lib, "surface_models")
         return

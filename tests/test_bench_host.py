@@ -15,20 +15,40 @@ import bench  # noqa: E402
 from semiconductor_devices_simulator_amd import devices  # noqa: E402
 
 
-def test_parity_against_the_reference_sweep_fixture():
+def test_parity_against_the_reference_sweep_fixtures():
     g = np.load(os.path.join(ROOT, "tests", "golden", "e2e_mosfet_96x64x64.npz"))
-    W, K = 2, 10
-    I = g["I"][W:W + K].copy()
-    p = bench.parity(I, W, K, [96, 64, 64])
-    assert p["points_within_tolerance"] == K and p["max_rel_drain_current_err"] == 0.0
+    timed = bench.timed_indices(50, 5, 20)
+    I = g["I"][timed].copy()
+    p = bench.parity(I, timed, [96, 64, 64])
+    assert p["reference"]["points_within_tolerance"] == 20 and p["reference"]["worst_err_over_bound"] == 0.0
+    # the default-tolerance reference is itself within the bound of the tight one
+    assert p["reference_tight"]["points_within_tolerance"] == 20
+    assert p["reference_tight"]["worst_err_over_bound"] < 1.0
     # a drain current off by 1e-6 relative at a point well above its floor fails there
-    j = int(np.argmax(np.abs(I[:, 1]) / (g["F"][W:W + K, 1] + 1e-300)))
+    F = g["F"][timed]
+    j = int(np.argmax(np.abs(I[:, 1]) / (F[:, 1] + 1e-300)))
     I[j, 1] *= 1 + 1e-6
-    p = bench.parity(I, W, K, [96, 64, 64])
-    assert p["points_within_tolerance"] == K - 1
-    assert p["max_rel_drain_current_err"] == pytest.approx(1e-6, rel=1e-6)
+    p = bench.parity(I, timed, [96, 64, 64])
+    assert p["reference"]["points_within_tolerance"] == 19
+    assert p["reference"]["max_rel_drain_current_err_resolved"] == pytest.approx(1e-6, rel=1e-6)
+    # one floor, not two: an error of 1.5 floors at a floor-governed current fails
+    I = g["I"][timed].copy()
+    k = int(np.argmin(np.abs(I[:, 1]) / (F[:, 1] + 1e-300)))
+    I[k, 1] += 1.5 * F[k, 1] + 1e-8 * abs(I[k, 1])
+    assert bench.parity(I, timed, [96, 64, 64])["reference"]["points_within_tolerance"] == 19
     # no fixture for the mesh: no claim
-    assert bench.parity(I, W, K, [7, 7, 7]) is None
+    assert bench.parity(I, timed, [7, 7, 7]) is None
+
+
+def test_timed_points_span_both_drain_biases():
+    spec = devices.mosfet(12, 8, 6)
+    pts = bench.sweep_points(spec)
+    timed = bench.timed_indices(50, 5, 20)
+    assert len(timed) == 20 and timed[0] == 5 and timed[-1] == 49
+    assert {pts[i][2][1] for i in timed} == {0.05, 1.0}
+    text = bench.describe_timed(pts, timed, "mosfet")
+    assert "Vd 0.05 V" in text and "Vd 1 V" in text
+    assert bench.timed_indices(50, 2, 48) == list(range(2, 50))
 
 
 def test_mixed_vs_fp64_criterion_and_fields():

@@ -100,3 +100,21 @@ def test_gate_tunnelling_spec():
     assert gates == [2]
     assert spec.stacks == {2: [(D.MAT_SIO2, 1.5)]}
     assert spec.name.endswith("_tun")
+
+
+def test_graded_grid_and_span_idx_match_the_front_end():
+    """devices.graded_grid and span_idx against the reference front end's own
+    functions (main.py:316-384), run on ten grids and 160 intervals by
+    tests/golden/make_golden.py --grid (tests/golden/graded_grid.npz): node
+    coordinates bit for bit, index ranges exactly."""
+    import json
+    from conftest import load_golden
+    from semiconductor_devices_simulator_amd import devices
+    g = load_golden("graded_grid")
+    cases = json.loads(str(g["cases"]))
+    for i, (L, n, dense, ratio, sig) in enumerate(cases):
+        xs = devices.graded_grid(L, n, dense, ratio, sig)
+        np.testing.assert_array_equal(xs, g[f"g{i}"], err_msg=str(cases[i]))
+    for i, a, b, i0, i1 in g["spans"]:
+        c = g[f"g{int(i)}"]
+        assert devices.span_idx(c, a, b) == (int(i0), int(i1)), (i, a, b)

@@ -1,0 +1,84 @@
+"""The MOSFET model set of the reference's NMOS template without the
+metal-contact bits (main.py:962): bandgap narrowing, tau(N), velocity
+saturation, MLDA quantum confinement, Lombardi surface mobility and gate
+tunnelling through a 1.5 nm SiO2 stack (models 63), all on together.
+Kernel level on k_tun_mlda_lb_mosfet_12x8x6 (the tunnelling supply reads the
+MLDA terms for carriers pushed away from the interface); end to end, Id-Vg
+sweeps of a 24x16x8 and the 96x64x64 MOSFET against the reference's sweeps,
+both methods, every terminal within 1e-8 |I_ref| + floor_ref."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import semisim_oracle as O
+from conftest import GOLDEN, assert_current_parity, golden_stacks, load_golden, load_fixture_device, oracle_device
+from semiconductor_devices_simulator_amd import devices, sweep
+from semiconductor_devices_simulator_amd.core import dptr
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tunnelling_with_mlda_and_lombardi_matches_reference(hip):
+    print("\n[gpu combined] k_tun_mlda_lb_mosfet_12x8x6", flush=True)
+    g = load_golden("k_tun_mlda_lb_mosfet_12x8x6")
+    assert int(g["models"]) & (8 | 16 | 32) == 8 | 16 | 32
+    N = load_fixture_device(hip, g)
+    ncol = ctypes.c_int(0)
+    npth = hip.sdx_test_tun_paths(ctypes.byref(ncol), None, None, None, None, None, None, None, None)
+    assert npth == len(g["pk"]) and ncol.value == len(g["ck"])
+    Va = np.ascontiguousarray(g["Va"], np.float64)
+    dev = oracle_device(g)
+    P = O.tun_paths(dev, golden_stacks(g))
+    n, p = dev.carriers(g["psi"], g["phin"], g["phip"])
+    oK, oG = O.tun_update(dev, P, g["psi"], g["phin"], g["phip"], n, p, g["Va"])
+    for hole in (0, 1):
+        cK, pG, tk, tg = np.zeros(2 * ncol.value), np.zeros(2 * npth), np.zeros(N), np.zeros(N)
+        assert hip.sdx_test_tun(dptr(Va), hole, dptr(cK), dptr(pG), dptr(tk), dptr(tg)) == npth
+        np.testing.assert_allclose(cK, g["cK"], rtol=1e-11, atol=0)
+        np.testing.assert_allclose(pG, g["pG"], rtol=1e-11, atol=0)
+        np.testing.assert_allclose(cK, oK, rtol=1e-11, atol=0)
+        np.testing.assert_allclose(pG, oG, rtol=1e-11, atol=0)
+        np.testing.assert_allclose(tk, g["tk%d" % hole], rtol=1e-11, atol=0)
+        np.testing.assert_allclose(tg, g["tg%d" % hole], rtol=1e-11, atol=0)
+
+
+def _m63(shape, npts=25):
+    spec = devices.with_gate_tunnelling(devices.mosfet(*shape, npts=npts, tox=1.5))
+    spec.models |= 8 | 16
+    return spec
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_combined_models_small_mosfet_sweep(hip, precision):
+    print(f"\n[gpu combined] mosfet 24x16x8 models 63 precision {precision}", flush=True)
+    spec = _m63((24, 16, 8), npts=7)
+    g = load_golden("e2e_m63_mosfet_24x16x8")
+    cids = devices.load_device(hip, spec)
+    assert hip.sdx_set_precision(precision) == 1
+    assert hip.api3_solve() == 1
+    R = sweep.run_sweep(hip, spec, cids)
+    assert np.all(R["conv"])
+    assert_current_parity(R["I"], g["I"], g["F"], spec.name)
+    psi = np.zeros(spec.N)
+    hip.api3_get_state(dptr(psi), dptr(np.zeros(spec.N)), dptr(np.zeros(spec.N)))
+    assert np.max(np.abs(psi - g["psi"])) <= 1e-9
+
+
+@pytest.mark.skipif(not os.path.isfile(os.path.join(GOLDEN, "e2e_m63_mosfet_96x64x64.npz")),
+                    reason="96x64x64 combined-model reference sweep not generated")
+@pytest.mark.parametrize("precision", [0, 1])
+def test_combined_models_mosfet96_sweep(hip, precision):
+    print(f"\n[gpu combined] mosfet 96x64x64 models 63 precision {precision}", flush=True)
+    spec = _m63((96, 64, 64))
+    g = load_golden("e2e_m63_mosfet_96x64x64")
+    cids = devices.load_device(hip, spec)
+    assert hip.sdx_set_precision(precision) == 1
+    assert hip.api3_solve() == 1
+    R = sweep.run_sweep(hip, spec, cids)
+    assert np.all(R["conv"])
+    assert_current_parity(R["I"], g["I"], g["F"], spec.name)
+    psi = np.zeros(spec.N)
+    hip.api3_get_state(dptr(psi), dptr(np.zeros(spec.N)), dptr(np.zeros(spec.N)))
+    assert np.max(np.abs(psi[g["idx"]] - g["psi"])) <= 1e-9

@@ -62,6 +62,36 @@ def test_continuity_rows(hip, dev, hole):
     close(dsc, g[f"cont{t}_dsc"], 1e-11)
 
 
+@pytest.mark.parametrize("op", ["poisson", "n", "p"])
+def test_csr_pattern_is_bit_exact(hip, dev, op):
+    """The sparsity pattern of the device operators (Poisson Jacobian, both
+    scaled continuity operators) equals the reference's exactly: the nonzero
+    mask of every diagonal, and the CSR export of the device operator
+    (row pointers and columns) against the CSR of the reference's
+    diagonals; values to the assembly tolerance."""
+    g, N = dev
+    Va = np.ascontiguousarray(g["Va"], np.float64)
+    if op == "poisson":
+        A = np.zeros(7 * N)
+        hip.sdx_test_poisson(1, dptr(Va), dptr(np.zeros(N)), dptr(A), dptr(np.zeros(N)))
+        A_ref = g["poisson_A"]
+    else:
+        hip.sdx_test_set_fmu(dptr(np.ascontiguousarray(g["fmu"].ravel())))
+        A = np.zeros(7 * N)
+        hip.sdx_test_cont(op == "p", dptr(A), dptr(np.zeros(N)), dptr(np.zeros(N)), dptr(np.zeros(N)))
+        A_ref = g[f"cont{op}_A"]
+    A = A.reshape(7, N)
+    np.testing.assert_array_equal(A != 0.0, A_ref != 0.0)
+    rp_ref, col_ref, val_ref = O.stencil_csr(A_ref, tuple(int(len(g[k])) for k in ("xs", "ys", "zs")))
+    cap = 7 * N
+    rp, col, val = np.zeros(N + 1, np.int32), np.zeros(cap, np.int32), np.zeros(cap)
+    nnz = hip.sdx_test_csr(iptr(rp), iptr(col), dptr(val), cap)
+    assert nnz == len(col_ref)
+    np.testing.assert_array_equal(rp, rp_ref)
+    np.testing.assert_array_equal(col[:nnz], col_ref)
+    close(val[:nnz], val_ref, 1e-11, scale=np.max(np.abs(val_ref)))
+
+
 def test_matvec(hip, dev):
     g, N = dev
     hip.sdx_test_load_op(dptr(np.ascontiguousarray(g["poisson_A"].ravel())), 0, None)

@@ -1,30 +1,43 @@
 """The 192x128x128 n-MOSFET (3.1M nodes) on the GPU against the reference
-core's full Id-Vg sweep (tests/golden/e2e_mosfet_192x128x128.npz), mixed
-precision (FP32 multigrid): the first 12 bias points of the Vd = 0.05 V sweep
-(the bench times all 50)."""
+core's full Id-Vg sweep (tests/golden/e2e_mosfet_192x128x128.npz), both
+methods: all 50 bias points (Vg 0-1.2 V at Vd 0.05 V, then at Vd 1.0 V),
+every terminal within 1e-8 |I_ref| + floor_ref, and psi at the fixture's
+4096 sampled nodes of the last point (Vd 1.0 V, Vg 1.2 V) within 1e-9 V."""
 import os
 
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, load_golden
+from conftest import GOLDEN, assert_current_parity, load_golden
 from semiconductor_devices_simulator_amd import devices, sweep
+from semiconductor_devices_simulator_amd.core import dptr
 
 pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.skipif(not os.path.isfile(os.path.join(GOLDEN, "e2e_mosfet_192x128x128.npz")),
                     reason="full-size reference sweep fixture not generated")
-def test_mosfet_192x128x128_first_points_match_reference(hip):
-    print("\n[gpu e2e] mosfet 192x128x128 FP32 multigrid, points 0-11", flush=True)
+@pytest.mark.parametrize("precision", [1, 0])
+def test_mosfet_192x128x128_sweep_matches_reference(hip, precision):
+    print(f"\n[gpu e2e] mosfet 192x128x128 precision {precision}, 50 points", flush=True)
     spec = devices.mosfet(192, 128, 128)
     g = load_golden("e2e_mosfet_192x128x128")
     cids = devices.load_device(hip, spec)
-    assert hip.sdx_set_precision(1) == 1
+    assert hip.sdx_set_precision(precision) == 1
     assert hip.api3_solve() == 1
-    R = sweep.run_sweep(hip, spec, cids, max_points=12)
-    n = len(R["I"])
-    assert n == 12 and np.all(R["conv"])
-    d = np.abs(R["I"] - g["I"][:n])
-    ok = d <= 1e-8 * np.abs(g["I"][:n]) + g["F"][:n] + R["F"]
-    assert ok.all(), (np.argwhere(~ok), d[~ok])
+
+    def progress(i, R):
+        if i % 10 == 9:
+            print(f"[gpu e2e]   point {i + 1} of 50", flush=True)
+
+    R = sweep.run_sweep(hip, spec, cids, on_point=progress)
+    assert len(R["I"]) == 50 and np.all(R["conv"])
+    assert_current_parity(R["I"], g["I"], g["F"], spec.name)
+    for name in ("e2e_mosfet_192x128x128_tight",):
+        if os.path.isfile(os.path.join(GOLDEN, name + ".npz")):
+            t = load_golden(name)
+            assert_current_parity(R["I"], t["I"], t["F"], spec.name + " vs the tight reference sweep")
+    N = spec.N
+    psi, a, b = np.zeros(N), np.zeros(N), np.zeros(N)
+    hip.api3_get_state(dptr(psi), dptr(a), dptr(b))
+    assert np.max(np.abs(psi[g["idx"]] - g["psi"])) <= 1e-9

@@ -113,6 +113,66 @@ def test_matvec(case):
     close(y, g["matvec_y"], 1e-13)
 
 
+def _struct_pattern(dev, op):
+    """Structural pattern of the reference's operators from the node kinds and
+    contact masks alone (N x 7 in the order c, xm, xp, ym, yp, zm, zp):
+    Poisson (poisson_eval :2732-2789) - Dirichlet rows identity, a coupling
+    toward every existing neighbour that is not Dirichlet; continuity
+    (cont_assemble :3067-3106) - identity rows off the free semiconductor
+    nodes, couplings toward free semiconductor neighbours only."""
+    N = dev.N
+    k = np.arange(N)
+    i, j, l = k % dev.Nx, (k // dev.Nx) % dev.Ny, k // (dev.Nx * dev.Ny)
+    nb = [(i > 0, k - 1), (i < dev.Nx - 1, k + 1), (j > 0, k - dev.Nx), (j < dev.Ny - 1, k + dev.Nx),
+          (l > 0, k - dev.Nx * dev.Ny), (l < dev.Nz - 1, k + dev.Nx * dev.Ny)]
+    if op == "poisson":
+        row = dev.con_mask == 0
+        ok = lambda kb: dev.con_mask[kb] == 0          # noqa: E731
+    else:
+        row = (dev.kind == 0) & (dev.con_mask == 0)
+        ok = lambda kb: (dev.kind[kb] == 0) & (dev.con_mask[kb] == 0)   # noqa: E731
+    P = np.zeros((7, N), bool)
+    P[0] = True
+    for q, (has, kb) in enumerate(nb):
+        kbc = np.where(has, kb, 0)
+        P[q + 1] = row & has & ok(kbc)
+    return P
+
+
+@pytest.mark.parametrize("op", ["poisson", "n", "p"])
+def test_operator_sparsity_pattern(case, op):
+    """The nonzero pattern of the reference's assembled operators is exactly
+    the structural pattern of the kinds and contact masks, and the oracle's
+    operators have the same pattern (integer data: compared exactly)."""
+    g, dev = case
+    A_ref = g["poisson_A"] if op == "poisson" else g[f"cont{op}_A"]
+    np.testing.assert_array_equal(A_ref != 0.0, _struct_pattern(dev, op))
+    if op == "poisson":
+        A = O.poisson_eval(dev, g["psi"], g["phin"], g["phip"], g["Va"])[1]
+    else:
+        n, p = dev.carriers(g["psi"], g["phin"], g["phip"])
+        A = O.cont_assemble(dev, op == "p", g["psi"], n, p, g["fmu"])[0]
+    np.testing.assert_array_equal(np.asarray(A) != 0.0, A_ref != 0.0)
+
+
+def test_stencil_csr(case):
+    """CSR of the reference's Poisson operator: the rows applied to a vector
+    give the reference's own matvec (matvec_L :1975-1993), columns ascend
+    within rows, and the entry count equals the nonzero count of the
+    diagonals."""
+    g, dev = case
+    A = g["poisson_A"]
+    rp, col, val = O.stencil_csr(A, (dev.Nx, dev.Ny, dev.Nz))
+    assert rp[0] == 0 and rp[-1] == np.count_nonzero(A) == len(col)
+    x = g["matvec_x"]
+    y = np.zeros(dev.N)
+    np.add.at(y, np.repeat(np.arange(dev.N), np.diff(rp)), val * x[col])
+    close(y, g["matvec_y"], 1e-13)
+    for r in range(dev.N):
+        c = col[rp[r]:rp[r + 1]]
+        assert np.all(np.diff(c) > 0) and r in c
+
+
 def test_galerkin_poisson(case):
     g, dev = case
     levs = O.mg_hierarchy((dev.Nx, dev.Ny, dev.Nz))
