@@ -104,6 +104,13 @@ int    sdx_nn_dataset_get(int i, float *in8, float *tgt3);
 double sdx_nn_train(int steps, int n_train, double lr, unsigned seed, float *loss);  /* Adam; mean loss */
 double sdx_nn_eval(int i0, int i1);                     /* mean loss over samples [i0, i1) */
 
+/* device memory for callers without a HIP runtime of their own: allocate
+   (null on failure), free, copy (dir 1 host->device, 2 device->host, 3
+   device->device; 1 = done) */
+void  *sdx_dev_alloc(long long bytes);
+void   sdx_dev_free(void *p);
+int    sdx_dev_copy(void *dst, const void *src, long long bytes, int dir);
+
 /* ---- CSR export of a 7-point stencil operator ----
    The reference stores tensor-mesh operators as seven diagonals (Sten,
    semiconductor_3d.c:210) with no CSR form; this lists every nonzero entry of

@@ -142,6 +142,9 @@ EXT_SIGNATURES = [
     ("sdx_test_tun_paths", I, [IP, IP, IP, IP, IP, DP, IP, DP, DP]),
     ("sdx_test_tun", I, [DP, I, DP, DP, DP, DP]),
     ("sdx_test_csr", LL, [IP, IP, DP, LL]),
+    ("sdx_dev_alloc", ctypes.c_void_p, [LL]),
+    ("sdx_dev_free", None, [ctypes.c_void_p]),
+    ("sdx_dev_copy", I, [ctypes.c_void_p, ctypes.c_void_p, LL, I]),
     ("sdx_csr_from_stencil", LL, [I, I, I, ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_void_p,
                                   ctypes.c_void_p, LL]),
     ("sdx_test_maj_flags", I, [I, DP, DP, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(ctypes.c_ubyte)]),

@@ -159,6 +159,20 @@ static bool csr_args_ok(int Nx, int Ny, int Nz, const double *const *diag, const
 }
 
 extern "C" {
+// device memory for callers without a HIP runtime of their own (tests, the
+// ctypes front end): allocate, free and copy (dir 1 host->device, 2
+// device->host, 3 device->device); 0 / null on failure
+void *sdx_dev_alloc(long long bytes) {
+    void *p = nullptr;
+    if (bytes <= 0 || hipMalloc(&p, (size_t)bytes) != hipSuccess) return nullptr;
+    return p;
+}
+void sdx_dev_free(void *p) { if (p) (void)hipFree(p); }
+int sdx_dev_copy(void *dst, const void *src, long long bytes, int dir) {
+    if (bytes < 0 || (!dst || !src) || dir < 1 || dir > 3) return 0;
+    const hipMemcpyKind k = dir == 1 ? hipMemcpyHostToDevice : (dir == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    return hipMemcpy(dst, src, (size_t)bytes, k) == hipSuccess;
+}
 // CSR of a 7-point stencil operator, all arrays in device memory (see
 // include/semisim_hip_ext.h).  Allocates its scratch (one int per 1024 rows)
 // for the call; runs on the null stream.

@@ -338,7 +338,7 @@ def soi_mosfet(Nx=24, Ny=16, Nz=8, tsi=80.0, tbox=100.0, lg=300.0, nbody=1e18, n
     return spec
 
 
-def guard_ring_diode(Nx=32, Ny=16, Nz=4, v_end=-10.0, npts=11, gate_v=None, tox=2.0):
+def guard_ring_diode(Nx=32, Ny=16, Nz=4, v_end=-8.0, npts=9, gate_v=None, tox=2.0):
     """Planar p+/n- diode with a floating guard ring, 8 x 4 x 1 um (x lateral,
     y depth with the top at y = 0, z width).  n- drift 1e15 cm^-3 over an n+
     substrate (1e19 cm^-3 below y = 3.5 um, cathode on the bottom face); p+
@@ -349,7 +349,9 @@ def guard_ring_diode(Nx=32, Ny=16, Nz=4, v_end=-10.0, npts=11, gate_v=None, tox=
     reference's float_fix).  With gate_v set, a Robin gate of tox nm SiO2 over
     the ring (work function 4.65 eV) is held at gate_v with gate tunnelling on
     (models bit 5), so that the tunnelling sink/source enters the ring's
-    balance.  Reverse sweep of the anode 0 -> v_end in npts points."""
+    balance.  Reverse sweep of the anode 0 -> v_end in npts points (beyond
+    about -8 V the ungated ring's potential is no longer determined to the
+    solver's tolerances: its balance currents sink below their floor)."""
     Lx, Ly, Lz = 8000.0, 4000.0, 1000.0
     xs = graded_grid(Lx, Nx, [2000.0, 4500.0, 5300.0])
     ys = graded_grid(Ly, Ny, [0.0, 600.0, 3500.0])

@@ -99,3 +99,38 @@ def assert_current_parity(I, I_ref, F_ref, what=""):
     print(f"[parity] {what} worst |dI|/(1e-8|I_ref|+floor_ref) = {worst:.3g}", flush=True)
     assert ok.all(), (what, np.argwhere(~ok), np.abs(np.asarray(I) - I_ref)[~ok], np.asarray(I_ref)[~ok],
                       np.asarray(F_ref)[~ok])
+
+
+def load_tight(name):
+    """The reference's sweep converged far below its default tolerances
+    (tol_p 1e-12, tol_c 1e-11: <name>_tight.npz), or None."""
+    path = os.path.join(GOLDEN, name + "_tight.npz")
+    return np.load(path, allow_pickle=False) if os.path.isfile(path) else None
+
+
+def assert_sweep_parity(R, name):
+    """Terminal currents of a sweep run R (sweep.run_sweep) against the
+    reference core's sweeps of the same device:
+    * against its sweep converged far below the default tolerances
+      (<name>_tight.npz, which stands for the exact discrete solution):
+      |I - I_t| <= 1e-8 |I_t| + floor_t on every terminal of every point the
+      tight sweep converged;
+    * against its sweep at the same default tolerances (<name>.npz):
+      |I - I_ref| <= 1e-8 |I_ref| + floor_ref + floor_run.  At these
+      tolerances each run pins a current only to its own floor (the Gummel
+      loop accepts current changes below 3e-14 S): the reference against
+      itself at 1 and 8 OpenMP threads differs by up to 4.65 floor_ref on the
+      pn diode."""
+    g = load_golden(name)
+    I, F = np.asarray(R["I"]), np.asarray(R["F"])
+    ok, worst = current_parity(I, g["I"], g["F"] + F)
+    print(f"[parity] {name} vs the reference sweep: worst |dI|/(1e-8|I_ref|+floor_ref+floor) = {worst:.3g}",
+          flush=True)
+    assert ok.all(), (name, np.argwhere(~ok), np.abs(I - g["I"])[~ok], g["I"][~ok], g["F"][~ok])
+    t = load_tight(name)
+    if t is not None:
+        c = np.asarray(t["conv"], bool)
+        ok, worst = current_parity(I[c], t["I"][c], t["F"][c])
+        print(f"[parity] {name} vs the tight reference sweep: worst |dI|/(1e-8|I_t|+floor_t) = {worst:.3g}",
+              flush=True)
+        assert ok.all(), (name + " (tight)", np.argwhere(~ok), np.abs(I[c] - t["I"][c])[~ok], t["I"][c][~ok])

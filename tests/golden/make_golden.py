@@ -538,6 +538,23 @@ def graded_grid_fixture(name):
     print("wrote", name, len(cases), "grids", len(spans), "spans")
 
 
+def tight_small_cases():
+    """The small end-to-end sweeps of the parity tests, as (device, fixture name)."""
+    tun = devices.with_gate_tunnelling(devices.mosfet(24, 16, 8, npts=7, tox=1.5))
+    lb = devices.mosfet(24, 16, 8, npts=7)
+    lb.models = 7 | 16
+    ml = devices.mosfet(24, 16, 8, npts=7)
+    ml.models = 7 | 8
+    m63 = devices.with_gate_tunnelling(devices.mosfet(24, 16, 8, npts=7, tox=1.5))
+    m63.models |= 8 | 16
+    return [(devices.pn_diode(), "e2e_pn_diode_16x16x64"), (devices.mosfet(24, 16, 8, npts=7), "e2e_mosfet_24x16x8"),
+            (tun, "e2e_tun_mosfet_24x16x8"), (lb, "e2e_lb_mosfet_24x16x8"), (ml, "e2e_mlda_mosfet_24x16x8"),
+            (m63, "e2e_m63_mosfet_24x16x8"), (devices.avalanche_diode(8, 8, 48), "e2e_ii_diode_8x8x48"),
+            (devices.vdmos(40, 24, 4, vd_max=50.0, npts=11, npts_on=5, tol_c=1e-11), "e2e_vdmos_40x24x4"),
+            (devices.guard_ring_diode(), "e2e_ring_diode_32x16x4"),
+            (devices.guard_ring_diode(gate_v=2.0), "e2e_ring_gate_diode_32x16x4")]
+
+
 def main():
     if "--grid" in sys.argv:           # front-end mesh helpers (no core needed)
         graded_grid_fixture("graded_grid")
@@ -546,6 +563,9 @@ def main():
     if "--float" in sys.argv:          # floating majority blocks (float_fix)
         sweep_fixture(lib, devices.guard_ring_diode(), "e2e_ring_diode_32x16x4")
         sweep_fixture(lib, devices.guard_ring_diode(gate_v=2.0), "e2e_ring_gate_diode_32x16x4")
+        for spec, name in tight_small_cases()[-2:]:
+            spec.solver = spec.solver[:3] + (1e-12, 1e-11, 1.0, 1.0)
+            sweep_fixture(lib, spec, name + "_tight")
         return
     if "--combined" in sys.argv:       # BGN, tau(N), v-sat, MLDA, Lombardi and gate tunnelling together
         spec = devices.with_gate_tunnelling(devices.mosfet(12, 8, 6, tox=1.5))
@@ -564,6 +584,12 @@ def main():
         spec = devices.mosfet(96, 64, 64)
         spec.solver = (2000, 1000, 500, 1e-12, 1e-11, 1.0, 1.0)
         sweep_fixture(lib, spec, "e2e_mosfet_96x64x64_tight", fields=False, sample=4096)
+        return
+    if "--tight-small" in sys.argv:    # every small end-to-end sweep converged far below the default tolerances
+        for spec, name in tight_small_cases():
+            spec.solver = spec.solver[:3] + (1e-12, min(spec.solver[4], 1e-11) * (0.1 if spec.solver[4] <= 1e-11 else 1.0),
+                                             1.0, 1.0)
+            sweep_fixture(lib, spec, name + "_tight")
         return
     if "--tight192" in sys.argv:       # the same for the 192x128x128 sweep (about 1.5 hours on 8 cores)
         spec = devices.mosfet(192, 128, 128)

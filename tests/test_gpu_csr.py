@@ -1,7 +1,8 @@
 """sdx_csr_from_stencil on device arrays against the oracle's CSR of the same
 seven diagonals: random 7-point operators with structural zeros (boundary
 couplings, identity rows, dropped couplings), a unit diagonal given as a null
-pointer, a too-short output (pattern only), and bad arguments."""
+pointer, a too-short output (pattern only), and bad arguments.  Device
+memory comes from the library's own sdx_dev_* entry points."""
 import ctypes
 
 import numpy as np
@@ -27,20 +28,48 @@ def _stencil(shape, seed, unit_diag=False):
     return A
 
 
-def _run(hip, A, shape, cap=None):
-    import torch
-    dev = torch.device("cuda:0")
+class Dev:
+    """Device buffers through the library (freed on exit)."""
+
+    def __init__(self, hip):
+        self.hip, self.bufs = hip, []
+
+    def put(self, a):
+        a = np.ascontiguousarray(a)
+        p = self.hip.sdx_dev_alloc(max(a.nbytes, 8))
+        assert p
+        self.bufs.append(p)
+        if a.nbytes:
+            assert self.hip.sdx_dev_copy(p, a.ctypes.data, a.nbytes, 1) == 1
+        return p
+
+    def get(self, p, n, dtype):
+        out = np.zeros(n, dtype)
+        if out.nbytes:
+            assert self.hip.sdx_dev_copy(out.ctypes.data, p, out.nbytes, 2) == 1
+        return out
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        for p in self.bufs:
+            self.hip.sdx_dev_free(p)
+
+
+def _run(hip, A, shape, cap=None, unit_diag=False):
     N = A.shape[1]
-    D = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in A]
     cap = 7 * N if cap is None else cap
-    rp = torch.zeros(N + 1, dtype=torch.int32, device=dev)
-    col = torch.zeros(max(cap, 1), dtype=torch.int32, device=dev)
-    val = torch.zeros(max(cap, 1), dtype=torch.float64, device=dev)
-    ptrs = (ctypes.c_void_p * 7)(*[d.data_ptr() for d in D])
-    torch.cuda.synchronize()
-    nnz = hip.sdx_csr_from_stencil(*shape, ptrs, ctypes.c_void_p(rp.data_ptr()), ctypes.c_void_p(col.data_ptr()),
-                                   ctypes.c_void_p(val.data_ptr()), cap)
-    return nnz, rp.cpu().numpy(), col.cpu().numpy(), val.cpu().numpy()
+    with Dev(hip) as d:
+        D = [d.put(a) for a in A]
+        if unit_diag:
+            D[0] = None
+        rp = d.put(np.zeros(N + 1, np.int32))
+        col = d.put(np.zeros(max(cap, 1), np.int32))
+        val = d.put(np.zeros(max(cap, 1), np.float64))
+        ptrs = (ctypes.c_void_p * 7)(*D)
+        nnz = hip.sdx_csr_from_stencil(*shape, ptrs, rp, col, val, cap)
+        return nnz, d.get(rp, N + 1, np.int32), d.get(col, max(cap, 1), np.int32), d.get(val, max(cap, 1), np.float64)
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -58,22 +87,13 @@ def test_csr_matches_oracle(hip, shape):
 def test_csr_unit_diagonal_and_short_output(hip):
     shape = (40, 12, 6)
     print(f"\n[gpu csr] unit diagonal, shape {shape}", flush=True)
-    import torch
     A = _stencil(shape, 7, unit_diag=True)
     rp_ref, col_ref, val_ref = O.stencil_csr(A, shape)
-    dev = torch.device("cuda:0")
-    N = A.shape[1]
-    D = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in A]
-    rp = torch.zeros(N + 1, dtype=torch.int32, device=dev)
-    col = torch.zeros(7 * N, dtype=torch.int32, device=dev)
-    val = torch.zeros(7 * N, dtype=torch.float64, device=dev)
-    ptrs = (ctypes.c_void_p * 7)(None, *[d.data_ptr() for d in D[1:]])      # c null: unit diagonal
-    torch.cuda.synchronize()
-    nnz = hip.sdx_csr_from_stencil(*shape, ptrs, ctypes.c_void_p(rp.data_ptr()), ctypes.c_void_p(col.data_ptr()),
-                                   ctypes.c_void_p(val.data_ptr()), 7 * N)
+    nnz, rp, col, val = _run(hip, A, shape, unit_diag=True)      # c null: unit diagonal
     assert nnz == len(col_ref)
-    np.testing.assert_array_equal(rp.cpu().numpy(), rp_ref)
-    np.testing.assert_array_equal(col.cpu().numpy()[:nnz], col_ref)
+    np.testing.assert_array_equal(rp, rp_ref)
+    np.testing.assert_array_equal(col[:nnz], col_ref)
+    np.testing.assert_array_equal(val[:nnz], val_ref)
     # too short an output: the row pointers (the pattern) are still complete
     nnz2, rp2, _, _ = _run(hip, A, shape, cap=10)
     assert nnz2 == nnz
@@ -85,3 +105,4 @@ def test_csr_rejects_bad_arguments(hip):
     ptrs = (ctypes.c_void_p * 7)()
     assert hip.sdx_csr_from_stencil(0, 4, 4, ptrs, None, None, None, 0) == -1
     assert hip.sdx_csr_from_stencil(4, 4, 4, None, None, None, None, 0) == -1
+    assert hip.sdx_csr_from_stencil(4, 4, 4, ptrs, None, None, None, 0) == -1

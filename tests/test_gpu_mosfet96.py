@@ -7,13 +7,13 @@ default tolerances (e2e_mosfet_96x64x64_tight.npz: tol_p 1e-12, tol_c 1e-11),
 which stands for the exact discrete solution: at the default tolerances the
 subthreshold currents are pinned only to about 1e-5 relative, but that lies
 below their resolution floor, so every terminal must still be within
-1e-8 |I| + floor of the tight sweep."""
+1e-8 |I| + floor of the tight sweep (conftest.assert_sweep_parity)."""
 import os
 
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, assert_current_parity, load_golden
+from conftest import GOLDEN, assert_sweep_parity, load_golden, load_tight
 from semiconductor_devices_simulator_amd import devices, sweep
 from semiconductor_devices_simulator_amd.core import dptr
 
@@ -32,9 +32,8 @@ def test_mosfet_96x64x64_sweep_matches_reference(hip, precision):
     assert hip.api3_solve() == 1
     R = sweep.run_sweep(hip, spec, cids)
     assert np.all(R["conv"])
-    assert_current_parity(R["I"], g["I"], g["F"], spec.name)
-    t = load_golden("e2e_mosfet_96x64x64_tight")
-    assert_current_parity(R["I"], t["I"], t["F"], spec.name + " vs the tight reference sweep")
+    assert_sweep_parity(R, "e2e_mosfet_96x64x64")
+    t = load_tight("e2e_mosfet_96x64x64")
     N = spec.N
     psi, a, b = np.zeros(N), np.zeros(N), np.zeros(N)
     hip.api3_get_state(dptr(psi), dptr(a), dptr(b))

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import semisim_oracle as O
-from conftest import assert_current_parity, golden_stacks, load_golden, load_fixture_device, oracle_device
+from conftest import assert_sweep_parity, golden_stacks, load_golden, load_fixture_device, oracle_device
 from semiconductor_devices_simulator_amd import devices, sweep
 from semiconductor_devices_simulator_amd.core import dptr, iptr
 
@@ -66,7 +66,7 @@ def test_tunnelling_sweep_matches_reference(hip, precision):
     assert hip.api3_get_ntp(2) > 0
     R = sweep.run_sweep(hip, spec, cids)
     assert np.all(R["conv"])
-    assert_current_parity(R["I"], g["I"], g["F"], spec.name)
+    assert_sweep_parity(R, "e2e_tun_mosfet_24x16x8")
     assert np.all(np.abs(R["I"][:, 2]) > 1e-13)               # a gate current, resolved
     psi = np.zeros(spec.N)
     hip.api3_get_state(dptr(psi), dptr(np.zeros(spec.N)), dptr(np.zeros(spec.N)))
