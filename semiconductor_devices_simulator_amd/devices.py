@@ -180,6 +180,14 @@ class DeviceSpec:
         return sum(s[3] for s in self.sweep)
 
 
+def tight_solver(solver):
+    """The solver settings of a sweep converged far below the default
+    tolerances (api3_set_solver): tol_p 1e-12 V and tol_c 1e-11 relative
+    current, or a tenth of tol_c where that is already 1e-11 or tighter."""
+    tol_c = solver[4]
+    return tuple(solver[:3]) + (1e-12, 1e-11 if tol_c > 1e-11 else 0.1 * tol_c) + tuple(solver[5:])
+
+
 def node_positions_nm(xs, ys, zs):
     Z, Y, X = np.meshgrid(np.asarray(zs) * 1e9, np.asarray(ys) * 1e9, np.asarray(xs) * 1e9, indexing="ij")
     return np.c_[X.ravel(), Y.ravel(), Z.ravel()]

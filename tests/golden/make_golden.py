@@ -564,7 +564,7 @@ def main():
         sweep_fixture(lib, devices.guard_ring_diode(), "e2e_ring_diode_32x16x4")
         sweep_fixture(lib, devices.guard_ring_diode(gate_v=2.0), "e2e_ring_gate_diode_32x16x4")
         for spec, name in tight_small_cases()[-2:]:
-            spec.solver = spec.solver[:3] + (1e-12, 1e-11, 1.0, 1.0)
+            spec.solver = devices.tight_solver(spec.solver)
             sweep_fixture(lib, spec, name + "_tight")
         return
     if "--combined" in sys.argv:       # BGN, tau(N), v-sat, MLDA, Lombardi and gate tunnelling together
@@ -582,18 +582,17 @@ def main():
         return
     if "--tight" in sys.argv:          # the 96x64x64 sweep converged far below the default tolerances
         spec = devices.mosfet(96, 64, 64)
-        spec.solver = (2000, 1000, 500, 1e-12, 1e-11, 1.0, 1.0)
+        spec.solver = devices.tight_solver(spec.solver)
         sweep_fixture(lib, spec, "e2e_mosfet_96x64x64_tight", fields=False, sample=4096)
         return
     if "--tight-small" in sys.argv:    # every small end-to-end sweep converged far below the default tolerances
         for spec, name in tight_small_cases():
-            spec.solver = spec.solver[:3] + (1e-12, min(spec.solver[4], 1e-11) * (0.1 if spec.solver[4] <= 1e-11 else 1.0),
-                                             1.0, 1.0)
+            spec.solver = devices.tight_solver(spec.solver)
             sweep_fixture(lib, spec, name + "_tight")
         return
     if "--tight192" in sys.argv:       # the same for the 192x128x128 sweep (about 1.5 hours on 8 cores)
         spec = devices.mosfet(192, 128, 128)
-        spec.solver = (2000, 1000, 500, 1e-12, 1e-11, 1.0, 1.0)
+        spec.solver = devices.tight_solver(spec.solver)
         sweep_fixture(lib, spec, "e2e_mosfet_192x128x128_tight", fields=False, sample=4096)
         return
     if "--surface" in sys.argv:        # MLDA factor and Lombardi mobility entry points

@@ -57,7 +57,8 @@ class Dev:
             self.hip.sdx_dev_free(p)
 
 
-def _run(hip, A, shape, cap=None, unit_diag=False):
+def _run(hip, A, shape, cap=None, unit_diag=False, fn=None):
+    fn = fn or hip.sdx_csr_from_stencil
     N = A.shape[1]
     cap = 7 * N if cap is None else cap
     with Dev(hip) as d:
@@ -68,7 +69,7 @@ def _run(hip, A, shape, cap=None, unit_diag=False):
         col = d.put(np.zeros(max(cap, 1), np.int32))
         val = d.put(np.zeros(max(cap, 1), np.float64))
         ptrs = (ctypes.c_void_p * 7)(*D)
-        nnz = hip.sdx_csr_from_stencil(*shape, ptrs, rp, col, val, cap)
+        nnz = fn(*shape, ptrs, rp, col, val, cap)
         return nnz, d.get(rp, N + 1, np.int32), d.get(col, max(cap, 1), np.int32), d.get(val, max(cap, 1), np.float64)
 
 

@@ -376,4 +376,8 @@ def test_scalar_readback_matches_reference(hip, precision):
     assert hip.api3_get_Emax() == pytest.approx(float(g["Emax"]), rel=1e-6)
     pos = np.zeros(3)
     assert hip.api3_ins_field_max(dptr(pos)) == pytest.approx(float(g["ins_field_max"]), rel=1e-6)
-    np.testing.assert_allclose(pos, g["ins_field_pos"], rtol=0, atol=1e-15)
+    # the device is uniform along z (the width), so the oxide field takes its
+    # largest value at every z of one (x, y) to rounding: the position must
+    # agree in x and y, and z is any mesh plane
+    np.testing.assert_allclose(pos[:2], g["ins_field_pos"][:2], rtol=0, atol=1e-15)
+    assert np.min(np.abs(spec.zs - pos[2])) <= 1e-15

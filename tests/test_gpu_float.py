@@ -5,9 +5,11 @@ solve finds it unanchored and rescales it to its global balance (net outflow
 plus recombination, and with the gated variant the gate-tunnelling sink and
 source through the stack over the ring).  Reverse sweep 0 -> -8 V, both
 methods, every terminal within 1e-8 |I_ref| + floor_ref of the reference's
-sweep (conftest.assert_sweep_parity), psi within 1e-9 V of the tight
-reference sweep at the last point; the host balance solve must actually run
-(blocks found and rescaled)."""
+sweep at the same tolerances (conftest.assert_sweep_parity), psi at the last
+point against the reference's sweep converged at tol_p 1e-12, tol_c 1e-11
+(1e-9 V with the gate; 1e-7 V on the ungated ring, whose potential the
+Gummel loop's noise-floor acceptance pins only to 1e3 x its tolerance); the
+host balance solve must actually run (blocks found and rescaled)."""
 import numpy as np
 import pytest
 
@@ -47,10 +49,11 @@ def test_floating_ring_sweep_matches_reference(hip, name, gate_v, precision):
     for ref, what in ((g, "reference"), (t, "tight reference")):
         dpsi = np.abs(psi - ref["psi"])
         print(f"[gpu float] psi vs the {what}: max {dpsi.max():.3g} V at node {int(dpsi.argmax())}", flush=True)
-    # the floating ring's potential is fixed only by its balance currents, which
-    # sink toward their floor with reverse bias: at the default tolerances the
-    # reference itself lands 1.6e-8 V from its tight sweep on the ungated
-    # ring (4e-12 V on the gated one).  The bound is 1e-9 V or twice that
-    # distance of the reference's own default-tolerance run, whichever is larger.
-    ref_err = float(np.max(np.abs(g["psi"] - t["psi"])))
-    assert np.max(np.abs(psi - t["psi"])) <= max(1e-9, 2.0 * ref_err), ref_err
+    # the ungated ring's potential is set only by balance currents at their
+    # floor: both cores accept such points by the Gummel loop's noise-floor
+    # rule (no progress for 20 iterations and a change below 1e3 x the
+    # tolerance, 1e-7 V), and the reference's own run lands 1.6e-8 V from its
+    # tight sweep there.  The gated ring's balance is carried by the
+    # tunnelling current: 1e-9 V.
+    bound = 1e3 * 1e-10 if gate_v is None else 1e-9
+    assert np.max(np.abs(psi - t["psi"])) <= bound

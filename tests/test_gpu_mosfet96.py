@@ -13,7 +13,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, assert_sweep_parity, load_golden, load_tight
+from conftest import GOLDEN, assert_current_parity, assert_sweep_parity, load_golden, load_tight
 from semiconductor_devices_simulator_amd import devices, sweep
 from semiconductor_devices_simulator_amd.core import dptr
 
@@ -33,7 +33,10 @@ def test_mosfet_96x64x64_sweep_matches_reference(hip, precision):
     R = sweep.run_sweep(hip, spec, cids)
     assert np.all(R["conv"])
     assert_sweep_parity(R, "e2e_mosfet_96x64x64")
+    # the metric's accuracy claim: at the default tolerances every current of
+    # this sweep is within 1e-8 |I| + floor of the exact discrete solution
     t = load_tight("e2e_mosfet_96x64x64")
+    assert_current_parity(R["I"], t["I"], t["F"], spec.name + " vs the tight reference sweep")
     N = spec.N
     psi, a, b = np.zeros(N), np.zeros(N), np.zeros(N)
     hip.api3_get_state(dptr(psi), dptr(a), dptr(b))
