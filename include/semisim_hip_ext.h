@@ -125,6 +125,15 @@ int    sdx_dev_copy(void *dst, const void *src, long long bytes, int dir);
    per 1024 rows for the call; synchronises the null stream. */
 long long sdx_csr_from_stencil(int Nx, int Ny, int Nz, const double *const *diag, int *rowptr, int *col,
                                double *val, long long cap);
+/* The same export without allocating anything: no counterpart in the
+   reference; for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+   scan partials live in a fixed device array of the library (256 KB per GPU,
+   held until the process ends), so it takes at most 67,108,864 rows: more
+   return -4 before any device work.  Calls that overlap on one GPU share that
+   array and race.  Same arguments and results otherwise. */
+long long sdx_csr_from_stencil_noalloc(int Nx, int Ny, int Nz, const double *const *diag, int *rowptr, int *col,
+                                       double *val, long long cap);
 
 /* ---- kernel-level hooks on the prepared device (tests) ----
    A7: 7 blocks of N doubles (c, xm, xp, ym, yp, zm, zp). */

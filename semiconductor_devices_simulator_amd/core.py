@@ -147,6 +147,8 @@ EXT_SIGNATURES = [
     ("sdx_dev_copy", I, [ctypes.c_void_p, ctypes.c_void_p, LL, I]),
     ("sdx_csr_from_stencil", LL, [I, I, I, ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_void_p,
                                   ctypes.c_void_p, LL]),
+    ("sdx_csr_from_stencil_noalloc", LL, [I, I, I, ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, LL]),
     ("sdx_test_maj_flags", I, [I, DP, DP, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(ctypes.c_ubyte)]),
 ]
 

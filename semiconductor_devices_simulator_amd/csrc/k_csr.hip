@@ -22,6 +22,7 @@
 #include "sd_csr.h"
 #include <hip/hip_runtime.h>
 #include <cstdio>
+#include <mutex>
 
 #define CSR_ROWS 1024                  // rows per workgroup of the count and fill passes
 
@@ -158,6 +159,40 @@ static bool csr_args_ok(int Nx, int Ny, int Nz, const double *const *diag, const
     return true;
 }
 
+// ------------------------------------------------ allocation-free variant
+// The scan partials of sdx_csr_from_stencil_noalloc live in this file-scope
+// device array: one int per 1024 rows plus the 8-byte total, so rows up to
+// CSR_NOALLOC_ROWS.  Every GPU the library runs on holds its own copy until
+// the process ends.  In host code `scratch` is only a symbol: its device
+// address on the current GPU comes from hipGetSymbolAddress, looked up once
+// per device and kept in g_scratch_ptr.  Nothing in it needs clearing between
+// calls (the count pass writes every partial the scan reads, the scan writes
+// the total), but calls that overlap on one GPU race on it.
+#define CSR_NOALLOC_ROWS (64L << 20)
+#define CSR_NOALLOC_WORDS ((int)(CSR_NOALLOC_ROWS / CSR_ROWS) + 4)
+static __device__ __align__(256) int scratch[CSR_NOALLOC_WORDS];
+#define CSR_MAX_DEVICES 64
+static int *g_scratch_ptr[CSR_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// device address of `scratch` on the current GPU (null on failure)
+static int *scratch_on_current_device() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= CSR_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            fprintf(stderr, "semisim-hip: scratch of the allocation-free CSR export not found on device %d: %s\n", dev,
+                    hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
 extern "C" {
 // device memory for callers without a HIP runtime of their own (tests, the
 // ctypes front end): allocate, free and copy (dir 1 host->device, 2
@@ -189,6 +224,31 @@ long long sdx_csr_from_stencil(int Nx, int Ny, int Nz, const double *const *diag
     if (hipFree(scratch) != hipSuccess && nnz >= 0) nnz = SD_CSR_EDEVICE;
     if (nnz > 0x7fffffffLL) {
         fprintf(stderr, "semisim-hip: sdx_csr_from_stencil: %lld nonzeros exceed an int row pointer\n", nnz);
+        return SD_CSR_ERANGE;
+    }
+    return nnz;
+}
+
+// The same export without allocating: its scratch is the file-scope array
+// above (rows up to CSR_NOALLOC_ROWS), the caller passes the inputs and
+// outputs.  No hipMalloc, hipFree or allocating library call.
+long long sdx_csr_from_stencil_noalloc(int Nx, int Ny, int Nz, const double *const *diag, int *rowptr, int *col,
+                                       double *val, long long cap) {
+    if (!csr_args_ok(Nx, Ny, Nz, diag, rowptr) || cap < 0 || (cap > 0 && (!col || !val))) {
+        fprintf(stderr, "semisim-hip: sdx_csr_from_stencil_noalloc: bad arguments\n");
+        return SD_CSR_EARG;
+    }
+    const long N = (long)Nx * Ny * Nz;
+    if (N > CSR_NOALLOC_ROWS) {
+        fprintf(stderr, "semisim-hip: sdx_csr_from_stencil_noalloc: %ld rows exceed its fixed scratch (at most %ld)\n",
+                N, (long)CSR_NOALLOC_ROWS);
+        return SD_CSR_ESIZE;
+    }
+    int *ptr = scratch_on_current_device();
+    if (!ptr) return SD_CSR_EDEVICE;
+    const long long nnz = csr_from_stencil(nullptr, Nx, Ny, Nz, diag, rowptr, col, val, cap, ptr);
+    if (nnz > 0x7fffffffLL) {
+        fprintf(stderr, "semisim-hip: sdx_csr_from_stencil_noalloc: %lld nonzeros exceed an int row pointer\n", nnz);
         return SD_CSR_ERANGE;
     }
     return nnz;

@@ -5,6 +5,7 @@
 #define SD_CSR_EARG (-1LL)        // bad shape or pointer
 #define SD_CSR_ERANGE (-2LL)      // more nonzeros than an int row pointer holds
 #define SD_CSR_EDEVICE (-3LL)     // a HIP call failed
+#define SD_CSR_ESIZE (-4LL)       // more rows than the allocation-free variant's fixed scratch holds
 
 // ints of scratch csr_from_stencil needs for N rows
 int csr_scratch_words(long N);
