@@ -294,6 +294,8 @@ def main():
     ap.add_argument("--workload", choices=["mosfet", "vdmos"], default="mosfet")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--sweep-points", type=int, default=0,
+                    help="solve only the first N points of the sweep (short profiling runs)")
     ap.add_argument("--cpu-baseline-worker", default="")
     a = ap.parse_args()
     if a.cpu_baseline_worker:
@@ -312,6 +314,8 @@ def main():
     if a.workload != "mosfet":          # the trained network and the CPU sample are MOSFET-only
         a.no_learned = a.no_cpu_baseline = True
     npts = spec.bias_points()
+    if a.sweep_points > 0:
+        npts = min(npts, a.sweep_points)
     W = max(0, min(a.warmup, npts - 1))
     K = max(1, min(a.steps, npts - W))
     pts = sweep_points(spec)
