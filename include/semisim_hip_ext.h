@@ -63,7 +63,10 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
                                               point-GS levels as one launch (1) or two (0),
                                               23: Krylov updates read the V-cycle results from the
                                               level-0 arrays they were left in (1) or from FP64 copies
-                                              written by the products (0) */
+                                              written by the products (0), 24: the level-0 line sweeps
+                                              of the FP32 method read each node's seven BF16
+                                              coefficients as one packed 16-byte record (1) or as
+                                              seven 2-byte loads (0) */
 
 /* ---- learned warm start ----
    A 3-D convolutional U-Net on a G^3 grid resampled from the mesh (index

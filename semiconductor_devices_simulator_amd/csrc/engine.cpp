@@ -76,7 +76,7 @@ static void vgraph_clear() {
 
 template <class T> static void hier_free(Hier<T> &H) {
     H.lev.clear(); H.inv = nullptr; H.alloc = false;
-    H.A0 = Sten7<T>{}; H.rs0 = nullptr; H.b0 = H.x0 = H.x0b = nullptr; H.c16 = nullptr;
+    H.A0 = Sten7<T>{}; H.rs0 = nullptr; H.b0 = H.x0 = H.x0b = nullptr; H.c16 = nullptr; H.r16 = nullptr;
 }
 // Storage order of each level: the line-smoothed levels above the coarse tail
 // are stored parity-split along x (MGLevel::px) when their lines fit the scan
@@ -820,6 +820,7 @@ template <class T> static Hier<T> &hier();
 template <> Hier<double> &hier<double>() { return G.H64; }
 template <> Hier<float> &hier<float>() { return G.H32; }
 
+static int g_rec16 = 1;         // 1: the level-0 sweeps read the packed 16-byte coefficient records
 // BF16 copies of level 0's off-diagonals and line factors for the smoother
 // (FP32 hierarchy, when enabled); else the smoother reads the FP32 arrays
 static void level0_bf16(Hier<float> &H) {
@@ -827,16 +828,26 @@ static void level0_bf16(Hier<float> &H) {
     const bool on = G.mg_bf16 && G.n_axes > 0 && G.line_levels > 0 && H.lev.size() > 1;
     if (on && !H.c16) {
         H.c16 = dalloc<__bf16>((size_t)15 * v0.N);
-        if (g_err) { H.c16 = nullptr; g_err = 0; G.mg_bf16 = 0; return level0_bf16(H); }
+        H.r16 = dalloc<uint4>((size_t)3 * v0.N);
+        if (g_err) { H.c16 = nullptr; H.r16 = nullptr; g_err = 0; G.mg_bf16 = 0; return level0_bf16(H); }
     }
     const bool was = v0.o16[0] != nullptr;
     for (int q = 0; q < 6; q++) v0.o16[q] = on ? H.c16 + (size_t)q * v0.N : nullptr;
     for (int a = 0; a < 3; a++)
         for (int q = 0; q < 3; q++) v0.lu16[a][q] = on ? H.c16 + (size_t)(6 + 3 * a + q) * v0.N : nullptr;
+    for (int a = 0; a < 3; a++) v0.rec16[a] = nullptr;
     if (was != on) vgraph_clear();
     if (!on) return;
     mg_o16(G.st, v0);
-    for (int a = 0; a < G.n_axes; a++) mg_lu16(G.st, v0, G.mg_axes[a]);
+    for (int a = 0; a < G.n_axes; a++) {
+        const int ax = G.mg_axes[a];
+        mg_lu16(G.st, v0, ax);
+        if (g_rec16) {
+            uint4 *r = H.r16 + (size_t)ax * v0.N;
+            mg_rec16(G.st, v0, ax, r);
+            v0.rec16[ax] = r;
+        }
+    }
 }
 
 template <class T> static void mg_setup_T() {
@@ -2099,6 +2110,7 @@ void sdx_tune(int what, int value) {
     if (what == 21) g_rb_fuse = value != 0;
     if (what == 22) g_rb_fuse_post = value != 0;
     if (what == 23) G.stage_read = value != 0;
+    if (what == 24) g_rec16 = value != 0;
     if (what == 11) {
         const int m = value >= 0 && value <= 2 ? value : SD_TILE_MODE_DEFAULT;
         hipDeviceSynchronize();

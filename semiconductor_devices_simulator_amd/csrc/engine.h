@@ -35,6 +35,7 @@ template <class T> struct Hier {
     // BF16 smoother coefficients of level 0 (FP32 hierarchy; allocated on
     // first use): 6 off-diagonals, then 3 factors for each of the 3 axes
     __bf16 *c16 = nullptr;
+    uint4 *r16 = nullptr;          // the same packed per node and axis (3 N records)
 };
 
 struct Engine {

@@ -1228,14 +1228,27 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
         kk[m] = k;
         vb[m] = b[k];
         if constexpr (H16) {
-            vai[m] = (T)v.lu16[a_lu][0][k]; viw[m] = (T)v.lu16[a_lu][1][k]; vcp[m] = (T)v.lu16[a_lu][2][k];
+            if (v.rec16[a_lu]) {
+                // the node's seven BF16 coefficients in one 16-byte load (a
+                // BF16 value is the upper half of the FP32 word)
+                const uint4 r = v.rec16[a_lu][k];
+                vai[m] = __uint_as_float(r.x << 16); viw[m] = __uint_as_float(r.x & 0xffff0000u);
+                vcp[m] = __uint_as_float(r.y << 16);
+                if constexpr (!zero) {
+                    cf[m][0] = __uint_as_float(r.y & 0xffff0000u); cf[m][1] = __uint_as_float(r.z << 16);
+                    cf[m][2] = __uint_as_float(r.z & 0xffff0000u); cf[m][3] = __uint_as_float(r.w << 16);
+                }
+            } else {
+                vai[m] = (T)v.lu16[a_lu][0][k]; viw[m] = (T)v.lu16[a_lu][1][k]; vcp[m] = (T)v.lu16[a_lu][2][k];
+                if constexpr (!zero) {
+                    cf[m][0] = (T)h0p[k]; cf[m][1] = (T)h1p[k]; cf[m][2] = (T)h2p[k]; cf[m][3] = (T)h3p[k];
+                }
+            }
         } else {
             vai[m] = v.lu[a_lu][0][k]; viw[m] = v.lu[a_lu][1][k]; vcp[m] = v.lu[a_lu][2][k];
         }
         if constexpr (!zero) {
-            if constexpr (H16) {
-                cf[m][0] = (T)h0p[k]; cf[m][1] = (T)h1p[k]; cf[m][2] = (T)h2p[k]; cf[m][3] = (T)h3p[k];
-            } else {
+            if constexpr (!H16) {
                 cf[m][0] = o0[k]; cf[m][1] = o1[k]; cf[m][2] = o2[k]; cf[m][3] = o3[k];
             }
 #pragma unroll
@@ -1342,7 +1355,7 @@ template <class T, int P, int MD>
 static void ls1_launch_md(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color,
                           const MGLevel<T> &cc, int xcd) {
     if constexpr (std::is_same_v<T, float> && P <= 2) {
-        if (v.o16[0] && v.lu16[ax][0]) {
+        if (v.rec16[ax] || (v.o16[0] && v.lu16[ax][0])) {
             ls1_launch_h<T, P, MD, true>(s, S, nb, v, b, x, ax, color, cc, xcd);
             return;
         }
@@ -1846,6 +1859,29 @@ __global__ void k_lu16(MGLevel<float> v, int ax) {
     if (k >= v.N) return;
 #pragma unroll
     for (int q = 0; q < 3; q++) v.lu16[ax][q][k] = (__bf16)v.lu[ax][q][k];
+}
+// one 16-byte record per node for the sweeps along axis ax: the line factors
+// ai, iw, cp and the off-line couplings in the order k_line_scan1 subtracts
+// them (x-lines: ym, yp, zm, zp; y-lines: xm, xp, zm, zp; z-lines: xm, xp, ym,
+// yp), each rounded to BF16 exactly as k_o16 / k_lu16 round them
+__device__ __forceinline__ unsigned bf16_bits(float f) {
+    const __bf16 h = (__bf16)f;
+    return (unsigned)__builtin_bit_cast(unsigned short, h);
+}
+__global__ void k_rec16(MGLevel<float> v, int ax, uint4 *rec) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= v.N) return;
+    const float *o0 = ax == 0 ? v.A.ym : v.A.xm, *o1 = ax == 0 ? v.A.yp : v.A.xp;
+    const float *o2 = ax == 2 ? v.A.ym : v.A.zm, *o3 = ax == 2 ? v.A.yp : v.A.zp;
+    uint4 r;
+    r.x = bf16_bits(v.lu[ax][0][k]) | (bf16_bits(v.lu[ax][1][k]) << 16);
+    r.y = bf16_bits(v.lu[ax][2][k]) | (bf16_bits(o0[k]) << 16);
+    r.z = bf16_bits(o1[k]) | (bf16_bits(o2[k]) << 16);
+    r.w = bf16_bits(o3[k]);
+    rec[k] = r;
+}
+void mg_rec16(hipStream_t s, const MGLevel<float> &v, int ax, uint4 *rec) {
+    k_rec16<<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(v, ax, rec);
 }
 void mg_o16(hipStream_t s, const MGLevel<float> &v) { k_o16<<<sd_nblocks(v.N), SD_BLOCK, 0, s>>>(v); }
 void mg_lu16(hipStream_t s, const MGLevel<float> &v, int ax) {

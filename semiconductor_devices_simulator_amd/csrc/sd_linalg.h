@@ -46,6 +46,11 @@ template <class T> struct MGLevel {
     // set (FP32 level 0 with BF16 smoother coefficients)
     __bf16 *o16[6];
     __bf16 *lu16[3][3];
+    // the same BF16 values packed per node into one 16-byte record for the
+    // sweeps along axis a (rec16[a][k] = ai, iw, cp, then the four off-line
+    // couplings in the order the sweep subtracts them, then 0), so that a
+    // lane fetches a node's seven coefficients with one 16-byte load
+    const uint4 *rec16[3];
 };
 
 // storage position of node (i, j, l) of a level
@@ -166,5 +171,6 @@ template <class T> void mg_sten_in(hipStream_t s, Sten7<double> A, const MGLevel
 // BF16 copies of a level's off-diagonals and of the line factors of axis ax
 void mg_o16(hipStream_t s, const MGLevel<float> &v);
 void mg_lu16(hipStream_t s, const MGLevel<float> &v, int ax);
+void mg_rec16(hipStream_t s, const MGLevel<float> &v, int ax, uint4 *rec);
 template <class T> void mg_rs_in(hipStream_t s, const double *w, const MGLevel<T> &v, double *rs);
 template <class T> void sd_spmv_T(hipStream_t s, const Geo &g, Sten7<T> A, const T *x, const T *b, T *y);

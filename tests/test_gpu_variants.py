@@ -84,3 +84,21 @@ def test_fused_post_sweep_matches_split(hip, solved, precision):
     assert np.all(np.isfinite(fused)) and scale > 0
     tol = 1e-6 if precision == 1 else 1e-13
     assert np.max(np.abs(fused - split)) <= tol * scale
+
+
+def test_packed_coefficient_records_are_bitwise(hip, solved):
+    """The level-0 line sweeps of the FP32 method reading each node's seven
+    BF16 coefficients as one packed 16-byte record (tune 24) see the same
+    values as with seven 2-byte loads: the V-cycle is bit-identical."""
+    N = solved
+    hip.sdx_set_precision(1)
+    r = np.random.default_rng(9).standard_normal(N)
+    out = {}
+    for packed in (1, 0):
+        hip.sdx_tune(24, packed)
+        assert hip.sdx_test_mg_setup() >= 4
+        out[packed] = _vcycle(hip, N, r, {})
+    hip.sdx_tune(24, 1)
+    hip.sdx_test_mg_setup()
+    assert np.all(np.isfinite(out[1])) and np.max(np.abs(out[1])) > 0
+    np.testing.assert_array_equal(out[1], out[0])
