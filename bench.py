@@ -336,7 +336,9 @@ def main():
     learned = None
     if os.path.isfile(warmstart.MODEL) and not a.no_learned:
         meta = warmstart.load(lib, warmstart.MODEL)
-        tL, IL, _, _, itsL, stL = run_gpu(lib, spec, 1, timed, ktimer=False, keep=[last], ws=2)
+        lib.sdx_ws_config(1, 0.0)       # the network's psi correction, applied at every bias point
+        tL, IL, _, _, itsL, stL = run_gpu(lib, spec, 1, timed, ktimer=False, keep=[last], ws=3)
+        lib.sdx_ws_config(7, 0.0)
         learned = dict(points_per_s=round(K / tL, 4), seconds=tL, iterations=itsL, warm_start=warmstart.ws_stats(lib),
                        model=dict(file=os.path.relpath(warmstart.MODEL, ROOT), samples=meta.get("samples"),
                                   heldout_loss=meta.get("heldout_loss_after")),

@@ -66,7 +66,8 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
                                               written by the products (0), 24: the level-0 line sweeps
                                               of the FP32 method read each node's seven BF16
                                               coefficients as one packed 16-byte record (1) or as
-                                              seven 2-byte loads (0) */
+                                              seven 2-byte loads (0), 25: V-cycles replayed from
+                                              captured HIP graphs (1) or launched kernel by kernel (0) */
 
 /* ---- learned warm start ----
    A 3-D convolutional U-Net on a G^3 grid resampled from the mesh (index

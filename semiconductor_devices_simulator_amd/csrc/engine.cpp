@@ -65,9 +65,9 @@ static void rd(int a, int b) {
 static double S(int slot) { return G.h_scal[slot]; }
 
 // Captured preconditioner launch sequences (see precond): they bake in the
-// hierarchy's pointers, shapes and smoother choices, so every change of
-// those drops them.
-struct VGraph { int prec, off; const double *r; double *z; hipGraphExec_t ex; };
+// hierarchy's pointers, shapes, smoother choices and the coarse-correction
+// scale of the operator kind, so every change of those drops them.
+struct VGraph { int prec, off, poisson, sin, sout, second; const double *r; double *z; hipGraphExec_t ex; };
 static std::vector<VGraph> g_vgraphs;
 static void vgraph_clear() {
     for (auto &v : g_vgraphs) hipGraphExecDestroy(v.ex);
@@ -1029,11 +1029,10 @@ template <class T> static void smooth_only_T(const T *b, T *x) {
     smooth_post<T>(H.lev[0], b, x, nullptr);
 }
 // The level-0 staging of the current hierarchy (off when level 0 works on the
-// Krylov vectors directly, or when V-cycles replay as graphs, which convert
-// their input and output themselves)
+// Krylov vectors directly)
 static Stage0 stage0() {
     Stage0 st{nullptr, nullptr, 0, 0, 0, nullptr};
-    if (!G.stage_on || G.use_graphs) return st;
+    if (!G.stage_on) return st;
     if (G.precision == 1 && !G.H32.lev.empty()) {
         const MGLevel<float> &v0 = G.H32.lev[0];
         st = Stage0{G.d_b32, G.d_x32, v0.Nx, v0.px, 1, G.d_x32b};
@@ -1069,14 +1068,17 @@ static void precond_launch(const double *r, double *z, bool sin = false, bool so
 }
 // The V-cycle is 20-40 short kernels whose launch sequence is fixed between
 // hierarchy changes.  It is captured into a HIP graph once per (precision,
-// smoother-only mode, r, z) and replayed with one hipGraphLaunch, so the host
-// no longer paces the GPU kernel by kernel.  The graph holds kernels only (no
-// memset or copy nodes).  With the level-0 smoother timer on, every n-th
-// V-cycle runs as plain launches bracketed by HIP events.
+// smoother-only mode, operator kind, staging, r, z) and replayed with one
+// hipGraphLaunch, so the host no longer paces the GPU kernel by kernel: on
+// the coarse levels the kernels take about as long as a launch costs the
+// host.  The graph holds kernels only (no memset or copy nodes).  With the
+// level-0 smoother timer on, every n-th V-cycle runs as plain launches
+// bracketed by HIP events.
 static void precond(const double *r, double *z, bool sin = false, bool sout = false, bool second = false) {
     G.n_vcycles++;
     bool sample = KT.on && KT.kind == 0 && (KT.vcycles++ % KT.every) == 0;
-    if (!G.use_graphs || sample || sin || sout) {
+    const bool whole_timed = KT.on && KT.kind == 2;
+    if (!G.use_graphs || sample || whole_timed) {
         KT.sampling = sample;
         const bool whole = kt_begin(2);          // kind 2: the whole preconditioner application
         precond_launch(r, z, sin, sout, second);
@@ -1084,8 +1086,13 @@ static void precond(const double *r, double *z, bool sin = false, bool sout = fa
         KT.sampling = false;
         return;
     }
+    const int poisson = G.mg_qsign == 0, si = sin, so = sout, se = second;
+    // staged input/output live in the hierarchy's level-0 arrays: r and z unused
+    const double *rk = sin ? nullptr : r;
+    double *zk = sout ? nullptr : z;
     for (auto &v : g_vgraphs)
-        if (v.prec == G.precision && v.off == G.mg_off && v.r == r && v.z == z) {
+        if (v.prec == G.precision && v.off == G.mg_off && v.poisson == poisson && v.sin == si && v.sout == so &&
+            v.second == se && v.r == rk && v.z == zk) {
             HC(hipGraphLaunch(v.ex, G.st));
             return;
         }
@@ -1093,7 +1100,7 @@ static void precond(const double *r, double *z, bool sin = false, bool sout = fa
     hipGraphExec_t ex = nullptr;
     hipError_t e = hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        precond_launch(r, z);
+        precond_launch(r, z, sin, sout, second);
         e = hipStreamEndCapture(G.st, &gr);
     }
     if (e == hipSuccess && gr) e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
@@ -1104,10 +1111,10 @@ static void precond(const double *r, double *z, bool sin = false, bool sout = fa
                 hipGetErrorString(e));
         (void)hipGetLastError();
         G.use_graphs = 0;
-        precond_launch(r, z);
+        precond_launch(r, z, sin, sout, second);
         return;
     }
-    g_vgraphs.push_back(VGraph{G.precision, G.mg_off, r, z, ex});
+    g_vgraphs.push_back(VGraph{G.precision, G.mg_off, poisson, si, so, se, rk, zk, ex});
     HC(hipGraphLaunch(ex, G.st));
 }
 static inline void matvec(const double *x, double *y) {
@@ -2111,6 +2118,7 @@ void sdx_tune(int what, int value) {
     if (what == 22) g_rb_fuse_post = value != 0;
     if (what == 23) G.stage_read = value != 0;
     if (what == 24) g_rec16 = value != 0;
+    if (what == 25) G.use_graphs = value != 0;
     if (what == 11) {
         const int m = value >= 0 && value <= 2 ? value : SD_TILE_MODE_DEFAULT;
         hipDeviceSynchronize();

@@ -346,7 +346,7 @@ def soi_mosfet(Nx=24, Ny=16, Nz=8, tsi=80.0, tbox=100.0, lg=300.0, nbody=1e18, n
     return spec
 
 
-def guard_ring_diode(Nx=32, Ny=16, Nz=4, v_end=-8.0, npts=9, gate_v=None, tox=2.0):
+def guard_ring_diode(Nx=32, Ny=16, Nz=4, v_end=-8.0, npts=9, gate_v=None, tox=2.0, ring_contact=False):
     """Planar p+/n- diode with a floating guard ring, 8 x 4 x 1 um (x lateral,
     y depth with the top at y = 0, z width).  n- drift 1e15 cm^-3 over an n+
     substrate (1e19 cm^-3 below y = 3.5 um, cathode on the bottom face); p+
@@ -375,6 +375,11 @@ def guard_ring_diode(Nx=32, Ny=16, Nz=4, v_end=-8.0, npts=9, gate_v=None, tox=2.
             Contact(3, 0.0, 1.0, 0.0, 1.0, 0, 0.0, label="Cathode")]
     spec = DeviceSpec("guard_ring_diode_%dx%dx%d" % (Nx, Ny, Nz), L3, xs, ys, zs, mat, Nd * 1e6, Na * 1e6, cons,
                       models=7, sweep=[(0, 0.0, v_end, npts, {})])
+    if ring_contact:
+        # an ohmic contact on the ring, for the front end's floating-terminal
+        # search (sweep.floating_sweep finds its zero-current voltage)
+        spec.contacts.append(Contact(2, 4700.0 / Lx, 5100.0 / Lx, 0.0, 1.0, 0, 0.0, label="Ring"))
+        spec.name = spec.name.replace("guard_ring_diode", "guard_ring_contact_diode")
     if gate_v is not None:
         spec.contacts.append(Contact(2, 4500.0 / Lx, 5300.0 / Lx, 0.0, 1.0, 2, gate_v, pm=4.65, tox=tox,
                                      label="Gate"))

@@ -139,6 +139,8 @@ def main(argv=None):
                    help="2: keep a correction only if it lowers the starting residual, 3: always keep it")
     e.add_argument("--config", nargs="*", default=["7,0"],
                    help="FIELDS,THR pairs: fields corrected (psi 1, phi_n 2, phi_p 4), smallest correction in V")
+    e.add_argument("--accept", type=float, nargs="*", default=[0.5],
+                   help="mode 2: keep a correction whose starting Poisson residual is below this x the plain one's")
     a = ap.parse_args(argv)
     lib = core.load_core()
     log = lambda m: print(m, flush=True)
@@ -169,7 +171,8 @@ def main(argv=None):
     variants = [tuple(float(v) for v in c.split(",")) for c in a.config]
     for spec in specs:
         r0 = sweep_with(lib, spec, 0)
-        for mode in a.modes:
+        for mode, acc in [(m, x) for m in a.modes for x in (a.accept if m == 2 else [0.5])]:
+            lib.sdx_ws_accept(acc)
             for chm, thr in variants:
                 lib.sdx_ws_config(int(chm), thr)
                 r1 = sweep_with(lib, spec, mode)
@@ -178,11 +181,13 @@ def main(argv=None):
                 # relative deviation where the current stands well above its floors
                 big = np.abs(r0["I"]) > 100.0 * (r0["F"] + r1["F"])
                 dev = float(np.max(np.where(big, d / np.where(big, np.abs(r0["I"]), 1.0), 0.0)))
-                log("[ws] " + json.dumps(dict(device=spec.name, mode=mode, fields=int(chm), thr=thr, plain=r0["its"],
+                log("[ws] " + json.dumps(dict(device=spec.name, mode=mode, accept=acc, fields=int(chm), thr=thr,
+                                              plain=r0["its"],
                                               learned=r1["its"], ws=r1["ws"],
                                               seconds=[round(r0["seconds"], 2), round(r1["seconds"], 2)],
                                               currents_agree=ok, max_rel_current_dev=dev)))
     lib.sdx_ws_config(7, 0.0)
+    lib.sdx_ws_accept(0.5)
     return 0
 
 

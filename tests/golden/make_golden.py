@@ -567,6 +567,15 @@ def main():
             spec.solver = devices.tight_solver(spec.solver)
             sweep_fixture(lib, spec, name + "_tight")
         return
+    if "--floating" in sys.argv:       # the front end's floating-terminal search (ring contact floating)
+        lib.api3_set_threads(1)
+        spec = devices.guard_ring_diode(ring_contact=True)
+        cids = devices.load_device(lib, spec)
+        assert lib.api3_solve() == 1
+        R = sweep.floating_sweep(lib, spec, cids, 0, 0.0, -8.0, 5, [2])
+        np.savez_compressed(os.path.join(HERE, "fl_ring_diode_32x16x4.npz"), **R)
+        print("wrote fl_ring_diode_32x16x4", R["Vf"], R["nev"], R["conv"])
+        return
     if "--combined" in sys.argv:       # BGN, tau(N), v-sat, MLDA, Lombardi and gate tunnelling together
         spec = devices.with_gate_tunnelling(devices.mosfet(12, 8, 6, tox=1.5))
         spec.models |= 8 | 16

@@ -1,8 +1,12 @@
 """Learned warm start on the device: recording training samples from a sweep
 (network input resampled from the mesh, checked against a numpy resampling),
-and what the trained network (semiconductor_devices_simulator_amd/
-warmstart_unet64.npz) gains on a MOSFET it was not trained on: the same
-currents (1e-8 relative + floors) with the iteration counts reported."""
+and the trained network (semiconductor_devices_simulator_amd/
+warmstart_unet64.npz) seeding every bias point of the 96x64x64 MOSFET sweep
+with its correction kept: the currents still meet 1e-8 |I| + floor against
+the reference's sweep converged at tol_p 1e-12, tol_c 1e-11 (the exact
+discrete solution) and psi 1e-9 V, with the Gummel iterations against the
+plain continuation reported (the network does not reduce them: measured
+neutral with the psi correction, 9% more with all three fields)."""
 import os
 
 import numpy as np
@@ -74,26 +78,28 @@ def test_capture_records_samples(hip):
 
 
 @pytest.mark.skipif(not os.path.isfile(warmstart.MODEL), reason="trained warm-start model not present")
-def test_trained_network_on_unseen_mosfet(hip):
+def test_learned_warm_start_kept_at_reference_accuracy(hip):
+    from conftest import assert_current_parity, load_golden, load_tight
+    from semiconductor_devices_simulator_amd.core import dptr
     meta = warmstart.load(hip, warmstart.MODEL)
     try:
-        spec = devices.random_mosfet(10007, 48, 32, 32)
+        spec = devices.mosfet(96, 64, 64)
         r0 = warmstart.sweep_with(hip, spec, 0)
-        r1 = warmstart.sweep_with(hip, spec, 2)
-        print(f"\n[warm start] trained on {meta.get('samples')} samples; unseen MOSFET 48x32x32: "
-              f"Gummel {r0['its']['gummel']} -> {r1['its']['gummel']}, Newton {r0['its']['newton']} -> "
+        hip.sdx_ws_config(1, 0.0)                     # the psi correction
+        r1 = warmstart.sweep_with(hip, spec, 3)       # applied at every bias point
+        N = spec.N
+        psi = np.zeros(N)
+        hip.api3_get_state(dptr(psi), dptr(np.zeros(N)), dptr(np.zeros(N)))
+        print(f"\n[warm start] trained on {meta.get('samples')} samples; 96x64x64 MOSFET sweep: Gummel "
+              f"{r0['its']['gummel']} -> {r1['its']['gummel']}, Newton {r0['its']['newton']} -> "
               f"{r1['its']['newton']}, corrections kept {r1['ws']['kept']}/{r1['ws']['tried']}", flush=True)
         assert np.all(r0["conv"]) and np.all(r1["conv"])
-        # both runs converge to the solver's tolerances from different starting
-        # states: the currents agree to the convergence band of the Gummel
-        # loop (about 2e-5 relative measured), not iterate for iterate
-        d = np.abs(r1["I"] - r0["I"])
-        big = np.abs(r0["I"]) > 100.0 * (r0["F"] + r1["F"])
-        assert np.all(d[big] <= 1e-4 * np.abs(r0["I"][big]))
-        assert np.all(d[~big] <= 1e-4 * np.abs(r0["I"][~big]) + 100.0 * (r0["F"] + r1["F"])[~big])
-        assert r1["ws"]["tried"] > 0
-        # the fallback keeps most points on the plain path: the iteration
-        # count stays close to the plain method's
-        assert r1["its"]["gummel"] <= 1.10 * r0["its"]["gummel"] + 2
+        assert r1["ws"]["tried"] == 50 and r1["ws"]["kept"] == 50
+        t = load_tight("e2e_mosfet_96x64x64")
+        assert_current_parity(r1["I"], t["I"], t["F"], "learned warm start vs the tight reference sweep")
+        assert_current_parity(r0["I"], t["I"], t["F"], "plain continuation vs the tight reference sweep")
+        assert np.max(np.abs(psi[t["idx"]] - t["psi"])) <= 1e-9
+        assert r1["its"]["gummel"] <= 1.05 * r0["its"]["gummel"]
     finally:
+        hip.sdx_ws_config(7, 0.0)
         hip.sdx_nn_free()
