@@ -105,6 +105,7 @@ double sdx_nn_grad_host(const float *in8, const float *tgt3, float *grad);   /* 
 int    sdx_nn_dataset_alloc(int nmax);
 int    sdx_nn_dataset_size(void);
 int    sdx_nn_dataset_get(int i, float *in8, float *tgt3);
+int    sdx_nn_dataset_put(const float *in8, const float *tgt3);   /* append a host sample; its index or -1 */
 double sdx_nn_train(int steps, int n_train, double lr, unsigned seed, float *loss);  /* Adam; mean loss */
 double sdx_nn_eval(int i0, int i1);                     /* mean loss over samples [i0, i1) */
 

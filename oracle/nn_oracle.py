@@ -186,3 +186,14 @@ def planar_to_grid(a, G, C):
 
 def grid_to_planar(a):
     return np.ascontiguousarray(a.transpose(3, 0, 1, 2)).reshape(a.shape[3], -1)
+
+
+def adam(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step (Kingma & Ba 2015, Algorithm 1) at step t >= 1 on
+    parameters p with gradient g and moments m, v (updated in place);
+    returns the new parameters."""
+    m[:] = b1 * m + (1.0 - b1) * g
+    v[:] = b2 * v + (1.0 - b2) * g * g
+    mh = m / (1.0 - b1 ** t)
+    vh = v / (1.0 - b2 ** t)
+    return p - lr * mh / (np.sqrt(vh) + eps)

@@ -77,6 +77,7 @@ SIGNATURES = [
     ("api3_set_verbose", None, [I]),
     ("api3_get_aa_depth", I, []),
     ("api3_set_node_data", None, [IP, DP, DP, DP]),
+    ("api3_set_con_nodes", I, [I, I, IP, DP]),
     ("api3_set_mat_tau", I, [I, D, D]),
     ("api3_carrier_sums", None, [DP]),
     ("api3_ins_field_max", D, [DP]),
@@ -118,6 +119,7 @@ EXT_SIGNATURES = [
     ("sdx_nn_dataset_alloc", I, [I]),
     ("sdx_nn_dataset_size", I, []),
     ("sdx_nn_dataset_get", I, [I, FP, FP]),
+    ("sdx_nn_dataset_put", I, [FP, FP]),
     ("sdx_nn_train", D, [I, I, D, ctypes.c_uint, FP]),
     ("sdx_nn_eval", D, [I, I]),
     ("sdx_test_prepare", I, []),

@@ -3,10 +3,13 @@
 // Same entry points, signatures, units and return conventions as the
 // reference core's public API (semiconductor_3d.c:4567-5353), so the
 // reference Python front end loads this library through SEMISIM_SO
-// unchanged.  Physics outside this core's scope (deep traps, gate
-// tunnelling, quantum confinement, surface mobility, metal-contact emission,
-// incomplete ionisation, prism meshes, C-V partitions) is rejected with a
-// diagnostic and a failure return, never approximated.
+// unchanged.  Gate tunnelling through Robin gate stacks, MLDA quantum
+// confinement, Lombardi surface mobility, impact ionisation and the C-V
+// charge partition are implemented.  Physics outside this core's scope (deep
+// traps, tunnelling through meshed insulator layers, metal-contact emission,
+// incomplete ionisation, the Coulomb term of 4H-SiC, prism meshes and shaped
+// contact node lists) is rejected with a diagnostic and a failure return,
+// never approximated.
 #include "engine.h"
 #include "../../include/semisim_api.h"
 #include "../../include/semisim_hip_ext.h"
@@ -31,6 +34,7 @@ extern "C" {
 
 void api3_init(void) {
     eng_free();
+    for (int c = 0; c < SD_NC_MAX; c++) { G.con_nodes[c].clear(); G.con_areas[c].clear(); }
     G.n_mats = 0; G.n_cons = 0; G.Nx = G.Ny = G.Nz = G.N = 0;
     G.converged = 0; G.conv_iter = 0; G.conv_res = 0.;
     G.max_iter_p = 2000; G.max_iter_c = 1000; G.max_gummel = 500;
@@ -152,6 +156,8 @@ int api3_add_contact_ex(int face, int i0, int i1, int j0, int j1, int k0, int k1
     if (G.n_cons >= SD_NC_MAX) return -1;
     HostCon &c = G.cons[G.n_cons];
     memset(&c, 0, sizeof c);
+    G.con_nodes[G.n_cons].clear();
+    G.con_areas[G.n_cons].clear();
     c.face = face < 0 ? 0 : (face > 6 ? 6 : face);
     c.i0 = i0; c.i1 = i1; c.j0 = j0; c.j1 = j1; c.k0 = k0; c.k1 = k1;
     c.bc = bc < 0 ? 0 : (bc > 2 ? 2 : bc);
@@ -164,7 +170,11 @@ int api3_add_contact_ex(int face, int i0, int i1, int j0, int j1, int k0, int k1
 void api3_add_contact(int face, int i0, int i1, int j0, int j1, int bc, double V, double pm) {
     (void)api3_add_contact_ex(face, i0, i1, j0, j1, 0, 0, bc, V, pm, 10e-9, 0.);
 }
-void api3_clear_contacts(void) { G.n_cons = 0; mark_changed(); }
+void api3_clear_contacts(void) {
+    for (int c = 0; c < SD_NC_MAX; c++) { G.con_nodes[c].clear(); G.con_areas[c].clear(); }
+    G.n_cons = 0;
+    mark_changed();
+}
 int api3_set_gate_stack(int cid, int nl, const int *mat, const double *t) {
     if (cid < 0 || cid >= G.n_cons || nl < 0 || nl > 6) return 0;
     for (int i = 0; i < nl; i++) if (mat[i] < 0 || mat[i] >= G.n_mats || !(t[i] > 0.)) return 0;
@@ -193,11 +203,22 @@ void api3_set_node_data(const int *mat, const double *Nd, const double *Na, cons
     }
     mark_changed();
 }
-int api3_set_con_nodes(int cid, int n, const int *, const double *) {
+// nodes of contact cid as an explicit list (a shaped contact on the tensor
+// mesh; api3_set_con_nodes :4821-4834): they replace the contact's index box
+// wherever prepare visits its nodes; areas = the boundary face area of each
+// node for a face contact (ignored, 0, for a box contact); n = 0 restores the box
+int api3_set_con_nodes(int cid, int n, const int *nodes, const double *areas) {
     if (cid < 0 || cid >= G.n_cons || n < 0) return 0;
-    if (n == 0) return 1;
-    unsupported("shaped contact node lists");
-    return 0;
+    G.con_nodes[cid].clear();
+    G.con_areas[cid].clear();
+    if (n == 0) { mark_changed(); return 1; }
+    for (int q = 0; q < n; q++) {
+        if (nodes[q] < 0 || nodes[q] >= G.N) { G.con_nodes[cid].clear(); G.con_areas[cid].clear(); return 0; }
+        G.con_nodes[cid].push_back(nodes[q]);
+        G.con_areas[cid].push_back(areas ? areas[q] : 0.);
+    }
+    mark_changed();
+    return 1;
 }
 void api3_set_walls(const double *) {}
 // tunnelling paths of the last prepared device (all, or those of contact cid)

@@ -48,6 +48,26 @@ template <class T> static T *dalloc(size_t n) {
     return (T *)p;
 }
 
+// buffers whose size prepare derives from the device (contact edges,
+// tunnelling paths): released by the next prepare, so re-preparing the same
+// grid (new models, gate stacks or contacts) does not grow device memory
+template <class T> static T *palloc(size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) { g_err = 1; return nullptr; }
+    G.prep_allocs.push_back(p);
+    return (T *)p;
+}
+static void prep_free() {
+    if (G.st) hipStreamSynchronize(G.st);
+    for (void *p : G.prep_allocs) hipFree(p);
+    G.prep_allocs.clear();
+    G.tun = TunDev{};
+    G.d_tp_off = nullptr;
+    G.d_ek = G.d_ekb = G.d_eoff = nullptr;
+    G.d_eh = G.d_ear = nullptr;
+}
+
 void eng_sync() { HC(hipStreamSynchronize(G.st)); }
 void eng_download(double *dst, const double *src, size_t n) {
     HC(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, G.st));
@@ -101,6 +121,7 @@ template <class T> static void hier_layout(Hier<T> &H) {
 void eng_free() {
     if (G.st) hipStreamSynchronize(G.st);
     vgraph_clear();
+    prep_free();
     for (void *p : G.dev_allocs) hipFree(p);
     G.dev_allocs.clear();
     if (G.d_mrec) hipFree(G.d_mrec);
@@ -110,6 +131,8 @@ void eng_free() {
     hier_free(G.H32);
     G.d_b32 = G.d_x32 = G.d_x32b = nullptr;
     G.d_ws[0] = G.d_ws[1] = G.d_ws[2] = nullptr;
+    G.d_e3[0] = G.d_e3[1] = G.d_e3[2] = nullptr;
+    G.e3_ok = 0;
     G.ws_have = 0;
     G.allocated = 0;
     G.have_solution = 0;
@@ -337,8 +360,21 @@ static double mlda_factor(const HostMat &m, double a, double b, int hole);
 double eng_mlda_factor(int mi, double a, double b, int hole) { return mlda_factor(G.mats[mi], a, b, hole); }
 
 // nodes of a face contact (face 0..5) or of a box (face 6), v6 index
-// convention, clipped to the mesh (semiconductor_3d.c:1047-1065)
+// convention, clipped to the mesh (semiconductor_3d.c:1047-1065); a contact
+// given as a node list (a shaped contact, api3_set_con_nodes) visits its
+// listed nodes instead, with their listed areas (FOR_CON_ANY :1422-1426)
 template <class F> static void for_con_nodes(const HostCon &cn, F body) {
+    const int cid = (int)(&cn - G.cons);
+    if (cid >= 0 && cid < SD_NC_MAX && !G.con_nodes[cid].empty()) {
+        const std::vector<int> &nl = G.con_nodes[cid];
+        for (size_t q = 0; q < nl.size(); q++) {
+            const int k = nl[q];
+            int i, j, l;
+            host_unidx(k, i, j, l);
+            body(i, j, l, k, G.con_areas[cid][q]);
+        }
+        return;
+    }
     int ia = 0, ib = G.Nx - 1, ja = 0, jb = G.Ny - 1, la = 0, lb = G.Nz - 1;
     switch (cn.face) {
     case 0: ia = ib = 0; ja = cn.i0; jb = cn.i1; la = cn.j0; lb = cn.j1; break;
@@ -544,7 +580,7 @@ static int tun_build() {
     return 1;
 }
 template <class T> static T *dput(const std::vector<T> &v) {
-    T *p = dalloc<T>(v.size());
+    T *p = palloc<T>(v.size());
     if (p && !v.empty()) HC(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, G.st));
     return p;
 }
@@ -586,7 +622,7 @@ static void tun_upload() {
     T.pk = dput(G.tp_k); T.pc = dput(G.tp_c); T.pnl = dput(pnl); T.pc0 = dput(G.tp_c0); T.pcn = dput(G.tp_cn);
     T.pm = dput(pm); T.pA = dput(G.tp_A); T.pt = dput(pt);
     T.ck = dput(G.col_k); T.cl = dput(G.col_l); T.ch = dput(G.col_h);
-    T.cK = dalloc<double>(2 * G.col_k.size()); T.pG = dalloc<double>(2 * (size_t)np);
+    T.cK = palloc<double>(2 * G.col_k.size()); T.pG = palloc<double>(2 * (size_t)np);
     T.tn = dput(tn); T.offK = dput(offK); T.entK = dput(entK); T.offG = dput(offG); T.entG = dput(entG);
     G.d_tp_off = dput(poff);
     if (G.models & 8) {
@@ -615,6 +651,7 @@ int eng_prepare() {
         if (why) { fprintf(stderr, "semisim-hip: %s is outside this core: not solved\n", why); return 0; }
     }
     if (!G.dirty) return 1;
+    prep_free();                       // the previous prepare's edge and path buffers
     const int N = G.N;
     mat_prepare();
     G.kind.assign(N, 0); G.mat8.assign(N, 0); G.eps.assign(N, 0.); G.bgn.assign(N, 0.);
@@ -760,10 +797,11 @@ int eng_prepare() {
     MatDev md[SD_NM_MAX];
     for (int i = 0; i < G.n_mats; i++) md[i] = host_mat_dev(G.mats[i]);
     HC(hipMemcpyAsync(G.d_mats, md, sizeof(MatDev) * G.n_mats, hipMemcpyHostToDevice, s));
-    // edge arrays are reallocated per prepare (small)
-    G.d_ek = dalloc<int>(std::max(1, G.n_edges)); G.d_ekb = dalloc<int>(std::max(1, G.n_edges));
-    G.d_eh = dalloc<double>(std::max(1, G.n_edges)); G.d_ear = dalloc<double>(std::max(1, G.n_edges));
-    G.d_eoff = dalloc<int>(G.n_cons + 1);
+    // edge arrays are reallocated per prepare (small; the previous prepare's
+    // were released at its start)
+    G.d_ek = palloc<int>(std::max(1, G.n_edges)); G.d_ekb = palloc<int>(std::max(1, G.n_edges));
+    G.d_eh = palloc<double>(std::max(1, G.n_edges)); G.d_ear = palloc<double>(std::max(1, G.n_edges));
+    G.d_eoff = palloc<int>(G.n_cons + 1);
     if (G.n_edges) {
         HC(hipMemcpyAsync(G.d_ek, ek.data(), ek.size() * 4, hipMemcpyHostToDevice, s));
         HC(hipMemcpyAsync(G.d_ekb, ekb.data(), ekb.size() * 4, hipMemcpyHostToDevice, s));
@@ -1839,6 +1877,16 @@ static int cont_path() {
             if (r >= -2. && r <= 3. && fabs(r) > 1e-12 && err <= 1e-9 + 1e-6 * span) rx = r;
         }
     }
+    // quadratic sweep predictor (when enabled): the previous two sweep steps
+    // were equal, so the last three converged states are equally spaced
+    int q2 = 0;
+    if (G.pred2 && rx != 0. && G.e_n >= 3 && G.e3_ok) {
+        q2 = 1;
+        for (int c = 0; c < nc; c++) {
+            const double s1 = Vf[c] - G.Ve2[c], s2 = G.Ve2[c] - G.Ve3[c];
+            if (fabs(s1 - s2) > 1e-9 + 1e-6 * fabs(s1)) q2 = 0;
+        }
+    }
     int first = 1;
     double t = 0., tp = -1., dt = span > 0. ? fmin(1., 0.1 / span) : 1.;
     // bias step of the contact that moves most, in units of 50 mV (warm-start input)
@@ -1858,6 +1906,11 @@ static int cont_path() {
             double r = (tn - t) / (t - tp);
             ph_extrap(G.st, N, G.d_psi, G.d_phin, G.d_phip, G.d_h1[0], G.d_h1[1], G.d_h1[2], G.d_h2[0], G.d_h2[1],
                       G.d_h2[2], r);
+        } else if (rx != 0. && q2) {
+            double *d3[3] = {G.d_psi, G.d_phin, G.d_phip};
+            const double *a3[3] = {G.d_h1[0], G.d_h1[1], G.d_h1[2]}, *b3[3] = {G.d_e2[0], G.d_e2[1], G.d_e2[2]};
+            const double *c3[3] = {G.d_e3[0], G.d_e3[1], G.d_e3[2]};
+            ph_extrap2(G.st, N, d3, a3, b3, c3, tn * rx);
         } else if (rx != 0.) {
             double r = tn * rx;
             ph_extrap(G.st, N, G.d_psi, G.d_phin, G.d_phip, G.d_h1[0], G.d_h1[1], G.d_h1[2], G.d_e2[0], G.d_e2[1],
@@ -1870,6 +1923,13 @@ static int cont_path() {
             }
         }
         if (first) {
+            if (G.pred2) {
+                if (!G.d_e3[0])
+                    for (int a = 0; a < 3; a++) G.d_e3[a] = dalloc<double>(N);
+                // d_e2 holds the previous bias point's state when this one continues a sweep
+                for (int a = 0; a < 3; a++) std::swap(G.d_e3[a], G.d_e2[a]);
+                G.e3_ok = G.e_n >= 1;
+            }
             for (int a = 0; a < 3; a++) sd_copy(G.st, G.d_e2[a], G.d_h1[a], N);
             first = 0;
         }
@@ -1902,9 +1962,9 @@ static int cont_path() {
     if (ok_all && span > 0. && G.ws_mode == 1 && steps == 1 && fails == 0 && G.ws_have) ws_capture();
     G.ws_have = 0;
     if (ok_all && span > 0.) {
-        for (int c = 0; c < nc; c++) { G.Ve2[c] = Vf[c]; G.Ve1[c] = Vt[c]; }
-        G.e_n = G.e_n < 2 ? G.e_n + 1 : 2;
-    } else if (!ok_all) G.e_n = 0;
+        for (int c = 0; c < nc; c++) { G.Ve3[c] = G.Ve2[c]; G.Ve2[c] = Vf[c]; G.Ve1[c] = Vt[c]; }
+        G.e_n = G.e_n < 3 ? G.e_n + 1 : 3;
+    } else if (!ok_all) { G.e_n = 0; G.e3_ok = 0; }
     if (!ok_all && (span > 0. || g_stop)) {
         load_state(G.d_h1);
         for (int c = 0; c < nc; c++) Va[c] = G.cons[c].Vprev;
@@ -1974,6 +2034,8 @@ void eng_read_env() {
     G.verbose = (e = getenv("SEMISIM_VERBOSE")) ? atoi(e) : 0;
     if ((e = getenv("SEMISIM_AAM"))) { int q = atoi(e); AA_DEPTH = q < 0 ? 0 : (q > 12 ? 12 : q); }
     if ((e = getenv("SEMISIM_PRED"))) PRED_ON = atoi(e) != 0;
+    G.pred2 = (e = getenv("SEMISIM_PRED2")) ? atoi(e) != 0 : 0;
+    G.e3_ok = 0;
     if ((e = getenv("SEMISIM_GROW"))) { double g = atof(e); if (g >= 1. && g <= 4.) GROW_SLOW = g; }
     if ((e = getenv("SEMISIM_CATOL"))) CONT_ATOL = atof(e);
     if ((e = getenv("SEMISIM_MG_FP32"))) G.precision = atoi(e) ? 1 : 0;
@@ -2119,6 +2181,7 @@ void sdx_tune(int what, int value) {
     if (what == 23) G.stage_read = value != 0;
     if (what == 24) g_rec16 = value != 0;
     if (what == 25) G.use_graphs = value != 0;
+    if (what == 26) { G.pred2 = value != 0; G.e3_ok = 0; }
     if (what == 11) {
         const int m = value >= 0 && value <= 2 ? value : SD_TILE_MODE_DEFAULT;
         hipDeviceSynchronize();

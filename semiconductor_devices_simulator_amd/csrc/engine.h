@@ -48,6 +48,10 @@ struct Engine {
     int n_mats = 0;
     HostCon cons[SD_NC_MAX];
     int n_cons = 0;
+    // shaped contacts: an explicit node list (and per-node boundary areas)
+    // replacing the contact's index box (api3_set_con_nodes); empty = the box
+    std::vector<int> con_nodes[SD_NC_MAX];
+    std::vector<double> con_areas[SD_NC_MAX];
     int models = 7, caxis = 1;
     int max_iter_p = 2000, max_iter_c = 1000, max_gummel = 500;
     double tol_p = 1e-9, tol_c = 1e-6, omega_p = 1., omega_c = 1.;
@@ -70,8 +74,11 @@ struct Engine {
     int ref_mat = 0, n_rob = 0;
 
     // sweep history (continuation predictor)
-    double Ve1[SD_NC_MAX], Ve2[SD_NC_MAX];
+    double Ve1[SD_NC_MAX], Ve2[SD_NC_MAX], Ve3[SD_NC_MAX];
     int e_n = 0;
+    int pred2 = 0;                     // quadratic sweep predictor through the last three points (off: the reference's secant)
+    int e3_ok = 0;                     // d_e3 holds the state of the bias point before the one in d_e2
+    double *d_e3[3] = {nullptr, nullptr, nullptr};
 
     // ------------------------------------------------------ device state
     hipStream_t st = nullptr;
@@ -109,6 +116,7 @@ struct Engine {
     RedBuf rb{};
     double *h_scal = nullptr;          // pinned mirror of the scalar slots
     std::vector<void *> dev_allocs;
+    std::vector<void *> prep_allocs;   // buffers sized by prepare (contact edges, tunnelling paths): freed by the next prepare
 
     Hier<double> H64;
     Hier<float> H32;

@@ -651,6 +651,26 @@ void ph_extrap(hipStream_t s, int N, double *dst0, double *dst1, double *dst2, c
                const double *a2, const double *b0, const double *b1, const double *b2, double r) {
     k_extrap<<<sd_nblocks(N), SD_BLOCK, 0, s>>>(N, dst0, dst1, dst2, a0, a1, a2, b0, b1, b2, r);
 }
+// quadratic extrapolation through three states a (step 0), b (step -1), c
+// (step -2) of uniform step, to step r: a + r (a - b) + r (r + 1) / 2 (a - 2 b + c)
+struct Ptr3 { double *p[3]; };
+struct CPtr3 { const double *p[3]; };
+__global__ void k_extrap2v(int N, Ptr3 d, CPtr3 a, CPtr3 b, CPtr3 c, double r) {
+    int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= N) return;
+    const double q = 0.5 * r * (r + 1.0);
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        const double x0 = a.p[f][k], x1 = b.p[f][k], x2 = c.p[f][k];
+        d.p[f][k] = x0 + r * (x0 - x1) + q * (x0 - 2.0 * x1 + x2);
+    }
+}
+void ph_extrap2(hipStream_t s, int N, double *const d[3], const double *const a[3], const double *const b[3],
+                const double *const c[3], double r) {
+    Ptr3 D{{d[0], d[1], d[2]}};
+    CPtr3 A{{a[0], a[1], a[2]}}, B{{b[0], b[1], b[2]}}, C{{c[0], c[1], c[2]}};
+    k_extrap2v<<<sd_nblocks(N), SD_BLOCK, 0, s>>>(N, D, A, B, C, r);
+}
 
 // --------------------------------------------- floating majority regions
 __global__ void k_maj_flags(DevView d, int hole, const double *u, const double *s, unsigned char *flag) {

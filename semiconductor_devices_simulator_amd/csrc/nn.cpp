@@ -366,6 +366,23 @@ int sdx_nn_dataset_get(int i, float *in, float *tgt3) {
     }
     return !g_nn_err;
 }
+// append a host sample (planar: NN_IN input and 3 target channels) to the
+// dataset, as the capture path stores it (input channel-last BF16, zero
+// padded; target FP32); returns its index or -1
+int sdx_nn_dataset_put(const float *in, const float *tgt3) {
+    if (!g_net || !in || !tgt3 || g_net->ds_n >= g_net->ds_max) return -1;
+    Net &n = *g_net;
+    const size_t V = vox(n.G);
+    std::vector<__bf16> x(V * NN_CIN, (__bf16)0.0f);
+    std::vector<float> t(V * 3);
+    for (size_t v = 0; v < V; v++) {
+        for (int c = 0; c < NN_IN; c++) x[v * NN_CIN + c] = (__bf16)in[c * V + v];
+        for (int c = 0; c < 3; c++) t[v * 3 + c] = tgt3[c * V + v];
+    }
+    NC(hipMemcpy(n.ds_x + (size_t)n.ds_n * V * NN_CIN, x.data(), x.size() * sizeof(__bf16), hipMemcpyHostToDevice));
+    NC(hipMemcpy(n.ds_t + (size_t)n.ds_n * V * 3, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    return g_nn_err ? -1 : n.ds_n++;
+}
 // Adam steps over samples drawn from the dataset (sample i < n_train only);
 // per-step losses into loss (optional); returns the mean loss
 double sdx_nn_train(int steps, int n_train, double lr, unsigned seed, float *loss) {

@@ -34,6 +34,8 @@ void ph_apply_bc(hipStream_t s, const DevView &d, int have_solution);
 void ph_init_eq(hipStream_t s, const DevView &d);
 void ph_neutral_pred(hipStream_t s, const DevView &d, const double *dqn, const double *dqp);
 void ph_currents(hipStream_t s, const DevView &d, CurEdges e, RedBuf rb);
+void ph_extrap2(hipStream_t s, int N, double *const d[3], const double *const a[3], const double *const b[3],
+                const double *const c[3], double r);
 void ph_extrap(hipStream_t s, int N, double *dst0, double *dst1, double *dst2, const double *a0, const double *a1,
                const double *a2, const double *b0, const double *b1, const double *b2, double r);
 void ph_update_psi(hipStream_t s, int N, double *phi, const double *dx, double lam, double *out);

@@ -140,7 +140,9 @@ class Contact:
     label: str = ""
 
     def to_idx6(self, xs, ys, zs):
-        ax = {0: (ys, zs), 1: (ys, zs), 2: (xs, zs), 3: (xs, zs), 4: (xs, ys), 5: (xs, ys)}[self.face]
+        # a box contact (face 6) here spans the x and y fractions on the z = 0
+        # plane; a shaped box contact replaces it by its node list
+        ax = {0: (ys, zs), 1: (ys, zs), 2: (xs, zs), 3: (xs, zs), 4: (xs, ys), 5: (xs, ys), 6: (xs, ys)}[self.face]
         c1, c2 = np.asarray(ax[0]), np.asarray(ax[1])
         i0, i1 = span_idx(c1, c1[0] + self.i0p * (c1[-1] - c1[0]), c1[0] + self.i1p * (c1[-1] - c1[0]))
         j0, j1 = span_idx(c2, c2[0] + self.j0p * (c2[-1] - c2[0]), c2[0] + self.j1p * (c2[-1] - c2[0]))
@@ -167,6 +169,9 @@ class DeviceSpec:
     # gate stacks for tunnelling (models bit 5): {contact index: [(material id, thickness nm), ...]},
     # semiconductor side first (main.py:3203-3211)
     stacks: dict = field(default_factory=dict)
+    # shaped contacts: {contact index: node indices} sent with api3_set_con_nodes
+    # (main.py:3352-3358: the staircase of a box contact's outline)
+    con_nodes: dict = field(default_factory=dict)
 
     @property
     def shape(self):
@@ -346,6 +351,23 @@ def soi_mosfet(Nx=24, Ny=16, Nz=8, tsi=80.0, tbox=100.0, lg=300.0, nbody=1e18, n
     return spec
 
 
+def shaped_anode_diode(Nx=16, Ny=16, Nz=64, radius=0.35):
+    """The pn diode of pn_diode() with its anode a shaped box contact: the
+    nodes of the two lowest planes (z = 0 and the next) within `radius` um of
+    the face centre, a staircase disc sent as a node list
+    (api3_set_con_nodes, as the front end sends shaped contacts); the rest of
+    the bottom face is uncontacted.  Forward sweep 0 -> 0.8 V."""
+    spec = pn_diode(Nx, Ny, Nz)
+    spec.name = "shaped_anode_diode_%dx%dx%d" % (Nx, Ny, Nz)
+    X, Y = np.meshgrid(spec.xs, spec.ys, indexing="xy")
+    disc = ((X - 0.5e-6) ** 2 + (Y - 0.5e-6) ** 2 <= (radius * 1e-6) ** 2).ravel()
+    plane = np.nonzero(disc)[0]
+    nodes = np.concatenate([plane, plane + Nx * Ny])
+    spec.contacts[0] = Contact(6, 0.0, 1.0, 0.0, 1.0, 0, 0.0, label="Anode")
+    spec.con_nodes = {0: nodes}
+    return spec
+
+
 def guard_ring_diode(Nx=32, Ny=16, Nz=4, v_end=-8.0, npts=9, gate_v=None, tox=2.0, ring_contact=False):
     """Planar p+/n- diode with a floating guard ring, 8 x 4 x 1 um (x lateral,
     y depth with the top at y = 0, z width).  n- drift 1e15 cm^-3 over an n+
@@ -440,6 +462,10 @@ def load_device(lib, spec, solver=None):
         if cid < 0:
             raise ValueError("too many contacts")
         cids.append(cid)
+    for ci, nodes in spec.con_nodes.items():
+        nodes = np.ascontiguousarray(nodes, np.int32)
+        if not lib.api3_set_con_nodes(cids[ci], len(nodes), iptr(nodes), dptr(np.zeros(len(nodes)))):
+            raise ValueError(f"bad node list for contact {ci}")
     if spec.tau is not None:
         lib.api3_set_mat_tau(mids[0], spec.tau[0], spec.tau[1])
     known = dict(zip(spec.materials, mids))

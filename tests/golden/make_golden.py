@@ -567,6 +567,12 @@ def main():
             spec.solver = devices.tight_solver(spec.solver)
             sweep_fixture(lib, spec, name + "_tight")
         return
+    if "--shaped" in sys.argv:         # a shaped box contact (node list)
+        sweep_fixture(lib, devices.shaped_anode_diode(), "e2e_shaped_anode_diode_16x16x64")
+        spec = devices.shaped_anode_diode()
+        spec.solver = devices.tight_solver(spec.solver)
+        sweep_fixture(lib, spec, "e2e_shaped_anode_diode_16x16x64_tight")
+        return
     if "--floating" in sys.argv:       # the front end's floating-terminal search (ring contact floating)
         lib.api3_set_threads(1)
         spec = devices.guard_ring_diode(ring_contact=True)

@@ -68,28 +68,42 @@ def test_gradients_match_oracle(hip, net):
             assert rel < 5e-2 and cos > 0.998, (i, name, rel, cos)
 
 
-def test_training_lowers_loss(hip, net):
-    """Fit a fixed smooth target on a few samples: Adam on the device."""
+def test_device_adam_matches_oracle_and_lowers_loss(hip, net):
+    """sdx_nn_train runs Adam on the device over dataset samples: three steps
+    on one stored sample match the oracle's Adam (Kingma & Ba) fed with the
+    device gradients at each step, and 150 steps on four samples of a fixed
+    smooth target lower the loss."""
     G = 16
     net(G)
     assert hip.sdx_nn_dataset_alloc(4) == 1
-    # samples come from the engine's capture path in use; here the host API
-    # drives the same loss/gradient code on one sample repeatedly
-    x, t = _sample(G, 3)
     zz, yy, xx = np.meshgrid(*(np.linspace(-1, 1, G),) * 3, indexing="ij")
-    t[..., 0] = 0.3 * np.sin(2 * xx) * np.cos(yy)
-    t[..., 1] = 0.2 * zz
-    t[..., 2] = 0.1 * xx * yy
-    xin = NO.grid_to_planar(x).astype(np.float32)
-    tin = NO.grid_to_planar(t).astype(np.float32)
-    g = np.zeros(hip.sdx_nn_nparams(), dtype=np.float32)
-    l0 = hip.sdx_nn_grad_host(fptr(xin), fptr(tin), fptr(g))
-    # plain gradient steps through the parameter API
-    p = np.zeros_like(g)
-    hip.sdx_nn_get_params(fptr(p), len(p))
-    for _ in range(30):
-        hip.sdx_nn_grad_host(fptr(xin), fptr(tin), fptr(g))
-        p -= 0.5 * g / max(np.abs(g).max(), 1e-12) * 1e-2
-        hip.sdx_nn_set_params(fptr(p), len(p))
-    l1 = hip.sdx_nn_grad_host(fptr(xin), fptr(tin), fptr(g))
+    samples = []
+    for q in range(4):
+        x, t = _sample(G, 3 + q)
+        t[..., 0] = 0.3 * np.sin(2 * xx + q) * np.cos(yy)
+        t[..., 1] = 0.2 * zz
+        t[..., 2] = 0.1 * xx * yy
+        xin = NO.grid_to_planar(x).astype(np.float32)
+        tin = NO.grid_to_planar(t).astype(np.float32)
+        assert hip.sdx_nn_dataset_put(fptr(xin), fptr(tin)) == q
+        samples.append((xin, tin))
+    n = hip.sdx_nn_nparams()
+    p = np.zeros(n, dtype=np.float32)
+    hip.sdx_nn_get_params(fptr(p), n)
+    pr = p.astype(np.float64)
+    m, v = np.zeros(n), np.zeros(n)
+    g = np.zeros(n, dtype=np.float32)
+    lr = 1e-3
+    xin, tin = samples[0]
+    for step in range(1, 4):
+        hip.sdx_nn_grad_host(fptr(xin), fptr(tin), fptr(g))          # gradient at the present weights
+        pr = NO.adam(pr, g.astype(np.float64), m, v, step, lr)
+        loss = np.zeros(1, dtype=np.float32)
+        assert hip.sdx_nn_train(1, 1, lr, 7, fptr(loss)) >= 0.0        # one device step on sample 0
+        hip.sdx_nn_get_params(fptr(p), n)
+        np.testing.assert_allclose(p, pr, rtol=0, atol=2e-6)       # FP32 master weights
+    l0 = hip.sdx_nn_eval(0, 4)
+    L = np.zeros(150, dtype=np.float32)
+    assert hip.sdx_nn_train(150, 4, 3e-3, 11, fptr(L)) >= 0.0
+    l1 = hip.sdx_nn_eval(0, 4)
     assert l1 < 0.7 * l0, (l0, l1)
