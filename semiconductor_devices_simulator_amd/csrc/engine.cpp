@@ -1594,122 +1594,187 @@ static void aa_step(double *const x[3], double *const g[3]) {
 }
 
 // ------------------------------------------------------------ Gummel
+// The outer iteration of a bias point (semiconductor_3d.c:3781-3930): refresh
+// the carrier-dependent models, solve both continuity equations for the
+// quasi-Fermi potentials, re-solve Poisson from them, mix with Anderson.  The
+// stopping rules decide the converged state, so they are the reference's:
+// both updates below gtol and every terminal current settled (plus, with
+// impact ionisation, the generation series settled); or, once the residual
+// has stopped halving for 20 iterations, a state within 1e3 gtol whose
+// currents are settled; giving up after 60 iterations without progress.
 static double newton_tol(const double *Va) {
     double sc = 1.;
     for (int c = 0; c < G.n_cons; c++) if (fabs(Va[c]) > sc) sc = fabs(Va[c]);
     return 2e-14 + 2e-15 * sc;
 }
 
-static int gummel(const double *Va, int maxit, double gtol, int *iters, double *res) {
-    const int N = G.N, nc = G.n_cons;
+// The impact-ionisation generation enters each iteration from the previous
+// one's carriers, so its total builds up as a geometric series whose ratio
+// (the ionisation integral) nears 1 close to breakdown.  `observe` takes the
+// total of one iteration and estimates what the series still has to add from
+// the ratio of its last two changes (capped at 0.99).
+struct AvalancheSeries {
+    double total = -1.;         // last total, < 0 before the first
+    double change = 0.;         // |last change|
+    double remainder = 0.;      // estimated rest of the series
+    double rel = 1.;            // remainder / total
+    double rel_best = 1e300;    // smallest rel that counted as progress
+    bool observe(double g, double tol) {
+        if (!(g > 0.)) { remainder = 0.; rel = 0.; return true; }
+        bool settled = false;
+        if (total < 0.) remainder = g;
+        else {
+            const double step = fabs(g - total);
+            double ratio = change > 0. ? step / change : 0.;
+            if (ratio > 0.99) ratio = 0.99;
+            remainder = step + step * ratio / (1. - ratio);
+            settled = step <= tol * g && remainder <= tol * g;
+            change = step;
+        }
+        rel = remainder / g;
+        total = g;
+        return settled;
+    }
+};
+
+// Terminal currents of successive iterations: settled when no contact's
+// current moved by more than rtol of itself or its round-off floor 3e-14 S.
+struct CurrentTrace {
+    int nc = 0, have = 0;
+    double I[SD_NC_MAX], S[SD_NC_MAX], last[SD_NC_MAX];
+    int settled(double rtol) const {
+        if (!have) return 0;
+        for (int c = 0; c < nc; c++)
+            if (fabs(I[c] - last[c]) > fmax(rtol * fabs(I[c]), 3e-14 * S[c])) return 0;
+        return 1;
+    }
+    double flux_max() const {
+        double m = 0.;
+        for (int c = 0; c < nc; c++) if (S[c] > m) m = S[c];
+        return m;
+    }
+    void roll() { memcpy(last, I, sizeof(double) * nc); have = 1; }
+};
+
+// Progress of the outer iteration: the residual halving, or the avalanche
+// remainder halving relative to its total.
+struct OuterProgress {
+    double halved = 1e300;      // residual at the last halving
+    double lowest = 1e300;      // lowest residual (Anderson restart reference)
+    int at = 0;                 // iteration of the last progress
+    void note(int it, double r, AvalancheSeries *ii) {
+        if (r < 0.5 * halved) { halved = r; at = it; }
+        if (ii && ii->rel < 0.5 * ii->rel_best) { ii->rel_best = ii->rel; at = it; }
+    }
+    int idle(int it) const { return it - at; }
+};
+
+// models that follow the carriers: clamp the quasi-Fermi levels, keep the
+// iteration's starting values, carriers, mobilities, tunnelling; returns the
+// total impact-ionisation generation (0 without the model)
+static double outer_refresh(int avalanche) {
+    const int N = G.N;
     hipStream_t s = G.st;
-    DevView d = eng_view();
-    double lo = G.vlo, hi = G.vhi, nt0 = newton_tol(Va);
-    double I[SD_NC_MAX], Sv[SD_NC_MAX], Iold[SD_NC_MAX];
-    int haveI = 0;
-    double itol = G.tol_c > 0. ? G.tol_c : 1e-6;
-    const int iion = (G.models & 256) != 0;
-    const double tol_ii = fmax(itol, 1e-4);
-    double Gs_old = -1., dG_old = 0., ii_rel = 1., ii_best = 1e300, Erem = 0.;
-    double brtol0 = G.bicg_rtol;
-    if (iion && G.bicg_rtol > 1e-8) G.bicg_rtol = 1e-8;
+    eng_clamp_qf();
+    sd_copy(s, G.d_qn0, G.d_phin, N);
+    sd_copy(s, G.d_qp0, G.d_phip, N);
+    eng_rebuild_np();
+    eng_upd_mob();
+    if (G.tun.np) ph_tun_update(s, eng_view(), G.tun);
+    if (!avalanche) return 0.;
+    ph_ii_update(s, eng_view());
+    sd_dot(s, G.d_gii, nullptr, N, G.rb, S_TMP1);              // sum of the generation
+    rd(S_TMP1, S_TMP1 + 1);
+    return S(S_TMP1);
+}
+
+// Poisson from the new quasi-Fermi levels, starting from the neutral-region
+// prediction; the previous potential stays in d_xk.  Returns max |dpsi|.
+static double outer_potential(double ntol) {
+    const int N = G.N;
+    hipStream_t s = G.st;
+    sd_copy(s, G.d_xk, G.d_psi, N);
+    ph_neutral_pred(s, eng_view(), G.d_dqn, G.d_dqp);
+    double t = wtime();
+    poisson_newton(ntol, 100);
+    G.t_pois += wtime() - t;
+    sd_maxabsdiff(s, G.d_psi, G.d_xk, N, G.rb, S_MAX2);
+    rd(S_MAX2, S_MAX2 + 1);
+    return S(S_MAX2);
+}
+
+static int gummel(const double *Va, int maxit, double gtol, int *iters, double *res) {
+    const int N = G.N;
+    const double nt0 = newton_tol(Va), lo = G.vlo, hi = G.vhi;
+    const double itol = G.tol_c > 0. ? G.tol_c : 1e-6;
+    const int avalanche = (G.models & 256) != 0;
+    const double ii_tol = fmax(itol, 1e-4);
+    // with impact ionisation the continuity solves run at 1e-8 for this point
+    const double brtol_saved = G.bicg_rtol;
+    if (avalanche && G.bicg_rtol > 1e-8) G.bicg_rtol = 1e-8;
+    AvalancheSeries ii;
+    CurrentTrace cur;
+    cur.nc = G.n_cons;
+    OuterProgress prog;
     G.ii_err = 0.;
     apply_bc(Va);
     aa_reset();
-    poisson_newton(fmax(nt0, 1e-6), 100);
-    int it, conv = 0, it_mark = 0;
-    double dpsi = 1., dq = 1., best = 1e300, prev = 1., best_mark = 1e300;
-    for (it = 1; it <= maxit; it++) {
-        if (g_stop) break;
+    poisson_newton(fmax(nt0, 1e-6), 100);      // psi consistent with the predicted quasi-Fermi levels
+    int it, conv = 0;
+    double dpsi = 1., dphi = 1., r_prev = 1.;
+    for (it = 1; it <= maxit && !g_stop; it++) {
         double t0 = wtime();
-        eng_clamp_qf();
-        sd_copy(s, G.d_qn0, G.d_phin, N);
-        sd_copy(s, G.d_qp0, G.d_phip, N);
-        eng_rebuild_np();
-        eng_upd_mob();
-        if (G.tun.np) ph_tun_update(s, d, G.tun);
-        int iiok = 1;
-        if (iion) {
-            ph_ii_update(s, d);
-            sd_dot(s, G.d_gii, nullptr, N, G.rb, S_TMP1);        // total generation (sum)
-            rd(S_TMP1, S_TMP1 + 1);
-            double Gs = S(S_TMP1);
-            if (Gs > 0.) {
-                if (Gs_old < 0.) { iiok = 0; Erem = Gs; }
-                else {
-                    double dG = fabs(Gs - Gs_old), rho = dG_old > 0. ? dG / dG_old : 0.;
-                    if (rho > 0.99) rho = 0.99;
-                    Erem = dG + dG * rho / (1. - rho);
-                    iiok = dG <= tol_ii * Gs && Erem <= tol_ii * Gs;
-                    dG_old = dG;
-                }
-                ii_rel = Erem / Gs;
-                Gs_old = Gs;
-            } else { Erem = 0.; ii_rel = 0.; }
-        }
+        const double gen = outer_refresh(avalanche);
+        int ii_ok = avalanche ? ii.observe(gen, ii_tol) : 1;
         double t1 = wtime();
         G.t_mob += t1 - t0;
         double dn = 0., dp = 0.;
-        if (!solve_cont(0, G.d_dqn, lo, hi, &dn)) break;
-        if (!solve_cont(1, G.d_dqp, lo, hi, &dp)) break;
+        if (!solve_cont(0, G.d_dqn, lo, hi, &dn) || !solve_cont(1, G.d_dqp, lo, hi, &dp)) break;
         double t2 = wtime();
         G.t_cont += t2 - t1;
-        dq = fmax(dn, dp);
-        terminal_currents(I, Sv);
+        dphi = fmax(dn, dp);
+        terminal_currents(cur.I, cur.S);
         G.t_cur += wtime() - t2;
-        int icv = haveI;
-        if (haveI)
-            for (int c = 0; c < nc; c++)
-                if (fabs(I[c] - Iold[c]) > fmax(itol * fabs(I[c]), 3e-14 * Sv[c])) icv = 0;
-        if (iion && !iiok) {
-            double Sm = 0.;
-            for (int c = 0; c < nc; c++) if (Sv[c] > Sm) Sm = Sv[c];
-            if (SD_Q * Erem <= 3e-14 * Sm) iiok = 1;
-        }
-        memcpy(Iold, I, sizeof(double) * nc);
-        haveI = 1;
-        sd_copy(s, G.d_xk, G.d_psi, N);
-        ph_neutral_pred(s, d, G.d_dqn, G.d_dqp);
-        double ntol = fmax(nt0, G.ntol_fac * fmin(prev, 1.));
-        double t3 = wtime();
-        poisson_newton(ntol, 100);
-        G.t_pois += wtime() - t3;
-        sd_maxabsdiff(s, G.d_psi, G.d_xk, N, G.rb, S_MAX2);
-        rd(S_MAX2, S_MAX2 + 1);
-        dpsi = S(S_MAX2);
+        const int I_ok = cur.settled(itol);
+        // a remainder below the currents' round-off floor leaves nothing to converge
+        if (!ii_ok && SD_Q * ii.remainder <= 3e-14 * cur.flux_max()) ii_ok = 1;
+        cur.roll();
+        dpsi = outer_potential(fmax(nt0, G.ntol_fac * fmin(r_prev, 1.)));
         if (G.verbose > 1)
-            fprintf(stderr, "  gummel %3d dpsi=%.3e dq=%.3e I0=%.6e aa=%d\n", it, dpsi, dq, nc ? I[0] : 0., G.aa_n);
-        if (dpsi != dpsi || dq != dq) break;
-        double r = fmax(dpsi, dq);
+            fprintf(stderr, "semisim-hip:   outer %d: max|dpsi| %.3e max|dphi| %.3e I[0] %.6e (Anderson %d)\n", it, dpsi,
+                    dphi, G.n_cons ? cur.I[0] : 0., G.aa_n);
+        if (dpsi != dpsi || dphi != dphi) break;
+        const double r = fmax(dpsi, dphi);
         g_prog[1] = it; g_prog[2] = r; g_prog[5] = (double)(G.gummel_its + it);
-        if (dq < gtol && dpsi < gtol && icv && iiok) {      // accept psi_k with its carriers
-            sd_copy(s, G.d_psi, G.d_xk, N);
+        const int settled = I_ok && ii_ok;
+        if (dphi < gtol && dpsi < gtol && settled) {
+            sd_copy(G.st, G.d_psi, G.d_xk, N);       // psi of this iteration's carriers
             conv = 1;
             break;
         }
-        prev = r;
-        if (r < 0.5 * best_mark) { best_mark = r; it_mark = it; }
-        if (iion && ii_rel < 0.5 * ii_best) { ii_best = ii_rel; it_mark = it; }
-        if (it - it_mark > 20 && r <= 1e3 * gtol && icv && iiok) {
-            sd_copy(s, G.d_psi, G.d_xk, N);
+        r_prev = r;
+        prog.note(it, r, avalanche ? &ii : nullptr);
+        if (prog.idle(it) > 20 && r <= 1e3 * gtol && settled) {
+            sd_copy(G.st, G.d_psi, G.d_xk, N);
             conv = 1;
-            if (G.verbose) fprintf(stderr, "semisim-hip: accepted at noise floor (res %.2e)\n", r);
+            if (G.verbose)
+                fprintf(stderr, "semisim-hip: residual %.2e stalled within 1e3 x tolerance, currents settled: accepted\n", r);
             break;
         }
-        if (it - it_mark > 60) break;
-        if (r < best) best = r;
-        else if (r > 30. * best) aa_reset();
+        if (prog.idle(it) > 60) break;
+        if (r < prog.lowest) prog.lowest = r;
+        else if (r > 30. * prog.lowest) aa_reset();  // the mixing diverged: drop its history
         double t4 = wtime();
         double *xx[3] = {G.d_xk, G.d_qn0, G.d_qp0}, *gg[3] = {G.d_psi, G.d_phin, G.d_phip};
         aa_step(xx, gg);
         G.t_aa += wtime() - t4;
     }
     if (it > maxit) it = maxit;
-    if (iion && brtol0 > 1e-8 && G.bicg_rtol == 1e-8) G.bicg_rtol = brtol0;
-    if (iion) G.ii_err = SD_Q * Erem;
+    if (avalanche && brtol_saved > 1e-8 && G.bicg_rtol == 1e-8) G.bicg_rtol = brtol_saved;
+    if (avalanche) G.ii_err = SD_Q * ii.remainder;
     G.gummel_its += it;
     *iters = it;
-    *res = fmax(dpsi, dq);
+    *res = fmax(dpsi, dphi);
     return conv;
 }
 
@@ -1850,167 +1915,236 @@ static void ws_capture() {
     if (nn_capture(G.st, ws_fill_target, nullptr)) G.ws_stat[3]++;
 }
 
-static int cont_path() {
-    const int nc = G.n_cons, N = G.N;
-    double Vf[SD_NC_MAX], Vt[SD_NC_MAX], Va[SD_NC_MAX], dV[SD_NC_MAX];
-    double span = 0.;
+// The sweep secant predictor (semiconductor_3d.c:4045-4064).  When this bias
+// change continues the previous one along the same line (the next point of a
+// sweep, or a secant step of a floating-terminal search, r < 0), the first
+// sub-step starts from x1 + r (x1 - x2), x1 and x2 the states after and
+// before the previous point, instead of from the neutral-region shift.
+// Returns r, or 0 when the change does not continue the last one.
+static double sweep_ratio(const double *from, const double *to, double span) {
+    const int nc = G.n_cons;
+    if (!PRED_ON || G.e_n < 1 || !(span > 0.)) return 0.;
+    double ab = 0., bb = 0.;
     for (int c = 0; c < nc; c++) {
-        Vf[c] = G.cons[c].Vprev; Vt[c] = G.cons[c].V;
-        if (fabs(Vt[c] - Vf[c]) > span) span = fabs(Vt[c] - Vf[c]);
+        if (fabs(from[c] - G.Ve1[c]) > 1e-12) return 0.;     // the last point did not end here
+        const double a = to[c] - from[c], b = from[c] - G.Ve2[c];
+        ab += a * b;
+        bb += b * b;
     }
-    int maxg = G.max_gummel > 0 ? G.max_gummel : 100;
-    double gtol = gummel_tol();
-    eng_upd_mob();
-    save_state(G.d_h1);
-    double rx = 0.;
-    if (PRED_ON && G.e_n >= 1 && span > 0.) {
-        double num = 0., den = 0., err = 0.;
-        int same = 1;
-        for (int c = 0; c < nc; c++) {
-            double a = Vt[c] - Vf[c], b = Vf[c] - G.Ve2[c];
-            num += a * b; den += b * b;
-            if (fabs(Vf[c] - G.Ve1[c]) > 1e-12) same = 0;
-        }
-        if (same && den > 0.) {
-            double r = num / den;
-            for (int c = 0; c < nc; c++) { double a = Vt[c] - Vf[c], b = Vf[c] - G.Ve2[c]; err = fmax(err, fabs(a - r * b)); }
-            if (r >= -2. && r <= 3. && fabs(r) > 1e-12 && err <= 1e-9 + 1e-6 * span) rx = r;
-        }
+    if (!(bb > 0.)) return 0.;
+    const double r = ab / bb;
+    double off = 0.;                                          // distance from the line
+    for (int c = 0; c < nc; c++) off = fmax(off, fabs((to[c] - from[c]) - r * (from[c] - G.Ve2[c])));
+    return (r >= -2. && r <= 3. && fabs(r) > 1e-12 && off <= 1e-9 + 1e-6 * span) ? r : 0.;
+}
+
+// quadratic form of the predictor (optional): the last two sweep steps were
+// equal, so the last three converged states are equally spaced
+static int sweep_is_even(const double *from) {
+    if (!G.pred2 || G.e_n < 3 || !G.e3_ok) return 0;
+    for (int c = 0; c < G.n_cons; c++) {
+        const double s1 = from[c] - G.Ve2[c], s2 = G.Ve2[c] - G.Ve3[c];
+        if (fabs(s1 - s2) > 1e-9 + 1e-6 * fabs(s1)) return 0;
     }
-    // quadratic sweep predictor (when enabled): the previous two sweep steps
-    // were equal, so the last three converged states are equally spaced
-    int q2 = 0;
-    if (G.pred2 && rx != 0. && G.e_n >= 3 && G.e3_ok) {
-        q2 = 1;
-        for (int c = 0; c < nc; c++) {
-            const double s1 = Vf[c] - G.Ve2[c], s2 = G.Ve2[c] - G.Ve3[c];
-            if (fabs(s1 - s2) > 1e-9 + 1e-6 * fabs(s1)) q2 = 0;
-        }
+    return 1;
+}
+
+// sub-step length factor after a converged sub-step of `its` outer
+// iterations; a slow step does not shorten the next one (the iteration count
+// follows the Gummel contraction rate there, not the step)
+static double step_growth(int its) { return its <= 8 ? 2.0 : (its <= 25 ? 1.5 : GROW_SLOW); }
+
+// One bias ramp from the voltages of the last solution (Vprev) to the
+// requested ones (V), all contacts together, in adaptive sub-steps t in (0, 1]
+// (semiconductor_3d.c:4018-4129).
+struct BiasRamp {
+    int nc = 0;
+    double from[SD_NC_MAX], to[SD_NC_MAX];
+    double span = 0.;                       // largest |V - Vprev|
+    double secant = 0.;                     // sweep predictor ratio, 0 when off
+    int quadratic = 0;
+    void bias_at(double t, double *v) const {
+        for (int c = 0; c < nc; c++) v[c] = from[c] + t * (to[c] - from[c]);
     }
-    int first = 1;
-    double t = 0., tp = -1., dt = span > 0. ? fmin(1., 0.1 / span) : 1.;
-    // bias step of the contact that moves most, in units of 50 mV (warm-start input)
-    double dvn = 0.;
-    for (int c = 0; c < nc; c++)
-        if (fabs(Vt[c] - Vf[c]) >= span && span > 0.) dvn = (Vt[c] - Vf[c]) / 0.05;
-    int ok_all = 0, its = 0, fails = 0, cfail = 0, steps = 0;
-    double res = 0.;
-    int max_cfail = (G.models & 256) ? 3 : (1 << 30);
-    g_prog[0] = 0.; g_prog[3] = 0.;
-    for (int guard = 0; guard < 5000; guard++) {
-        if (g_stop) break;
-        double tn = fmin(1., t + dt);
-        for (int c = 0; c < nc; c++) Va[c] = Vf[c] + tn * (Vt[c] - Vf[c]);
-        eng_set_vrange(Va);
-        if (tp >= 0.) {
-            double r = (tn - t) / (t - tp);
-            ph_extrap(G.st, N, G.d_psi, G.d_phin, G.d_phip, G.d_h1[0], G.d_h1[1], G.d_h1[2], G.d_h2[0], G.d_h2[1],
-                      G.d_h2[2], r);
-        } else if (rx != 0. && q2) {
-            double *d3[3] = {G.d_psi, G.d_phin, G.d_phip};
-            const double *a3[3] = {G.d_h1[0], G.d_h1[1], G.d_h1[2]}, *b3[3] = {G.d_e2[0], G.d_e2[1], G.d_e2[2]};
-            const double *c3[3] = {G.d_e3[0], G.d_e3[1], G.d_e3[2]};
-            ph_extrap2(G.st, N, d3, a3, b3, c3, tn * rx);
-        } else if (rx != 0.) {
-            double r = tn * rx;
-            ph_extrap(G.st, N, G.d_psi, G.d_phin, G.d_phip, G.d_h1[0], G.d_h1[1], G.d_h1[2], G.d_e2[0], G.d_e2[1],
-                      G.d_e2[2], r);
+    // starting state of the sub-step t -> t_next: extrapolation of the last
+    // two converged sub-steps, else the sweep predictor, else the last state
+    // with the quasi-neutral contact regions shifted by the bias change
+    void start(double t, double t_prev, double t_next) const {
+        const int N = G.N;
+        hipStream_t s = G.st;
+        if (t_prev >= 0.) {
+            ph_extrap(s, N, G.d_psi, G.d_phin, G.d_phip, G.d_h1[0], G.d_h1[1], G.d_h1[2], G.d_h2[0], G.d_h2[1],
+                      G.d_h2[2], (t_next - t) / (t - t_prev));
+        } else if (secant != 0. && quadratic) {
+            double *dst[3] = {G.d_psi, G.d_phin, G.d_phip};
+            const double *x1[3] = {G.d_h1[0], G.d_h1[1], G.d_h1[2]}, *x2[3] = {G.d_e2[0], G.d_e2[1], G.d_e2[2]};
+            const double *x3[3] = {G.d_e3[0], G.d_e3[1], G.d_e3[2]};
+            ph_extrap2(s, N, dst, x1, x2, x3, t_next * secant);
+        } else if (secant != 0.) {
+            ph_extrap(s, N, G.d_psi, G.d_phin, G.d_phip, G.d_h1[0], G.d_h1[1], G.d_h1[2], G.d_e2[0], G.d_e2[1],
+                      G.d_e2[2], t_next * secant);
         } else {
             load_state(G.d_h1);
             if (span > 0.) {
-                for (int c = 0; c < nc; c++) dV[c] = (tn - t) * (Vt[c] - Vf[c]);
+                double dV[SD_NC_MAX];
+                for (int c = 0; c < nc; c++) dV[c] = (t_next - t) * (to[c] - from[c]);
                 region_shift(dV);
             }
         }
-        if (first) {
-            if (G.pred2) {
-                if (!G.d_e3[0])
-                    for (int a = 0; a < 3; a++) G.d_e3[a] = dalloc<double>(N);
-                // d_e2 holds the previous bias point's state when this one continues a sweep
-                for (int a = 0; a < 3; a++) std::swap(G.d_e3[a], G.d_e2[a]);
-                G.e3_ok = G.e_n >= 1;
-            }
-            for (int a = 0; a < 3; a++) sd_copy(G.st, G.d_e2[a], G.d_h1[a], N);
-            first = 0;
-        }
+    }
+};
+
+// the sweep history (x2 of the predictor) takes this point's start state;
+// with the quadratic predictor the previous x2 moves to x3
+static void sweep_history_push() {
+    if (G.pred2) {
+        if (!G.d_e3[0])
+            for (int a = 0; a < 3; a++) G.d_e3[a] = dalloc<double>(G.N);
+        for (int a = 0; a < 3; a++) std::swap(G.d_e3[a], G.d_e2[a]);
+        G.e3_ok = G.e_n >= 1;
+    }
+    for (int a = 0; a < 3; a++) sd_copy(G.st, G.d_e2[a], G.d_h1[a], G.N);
+}
+
+static int cont_path() {
+    BiasRamp R;
+    R.nc = G.n_cons;
+    for (int c = 0; c < R.nc; c++) {
+        R.from[c] = G.cons[c].Vprev;
+        R.to[c] = G.cons[c].V;
+        R.span = fmax(R.span, fabs(R.to[c] - R.from[c]));
+    }
+    const int max_outer = G.max_gummel > 0 ? G.max_gummel : 100;
+    const double gtol = gummel_tol();
+    eng_upd_mob();
+    save_state(G.d_h1);
+    R.secant = sweep_ratio(R.from, R.to, R.span);
+    R.quadratic = R.secant != 0. && sweep_is_even(R.from);
+    // bias step of the contact that moves most, in units of 50 mV (warm-start input)
+    double step50 = 0.;
+    for (int c = 0; c < R.nc; c++)
+        if (R.span > 0. && fabs(R.to[c] - R.from[c]) >= R.span) step50 = (R.to[c] - R.from[c]) / 0.05;
+    // with impact ionisation a failure near breakdown is physical: three in a
+    // row end the ramp instead of shrinking the step to 2e-5
+    const int fail_limit = (G.models & 256) ? 3 : (1 << 30);
+    double t = 0., t_prev = -1., dt = R.span > 0. ? fmin(1., 0.1 / R.span) : 1.;
+    int reached = 0, its = 0, failures = 0, in_row = 0, steps = 0;
+    double res = 0., v[SD_NC_MAX];
+    g_prog[0] = 0.; g_prog[3] = 0.;
+    for (int attempt = 0; attempt < 5000 && !g_stop; attempt++) {
+        const double t_next = fmin(1., t + dt);
+        R.bias_at(t_next, v);
+        eng_set_vrange(v);
+        R.start(t, t_prev, t_next);
+        if (attempt == 0) sweep_history_push();
         eng_clamp_qf();
-        apply_bc(Va);
+        apply_bc(v);
         eng_rebuild_np();
-        if (guard == 0 && tp < 0. && span > 0. && G.ws_mode) ws_before(Va, dvn);
-        int ok = gummel(Va, maxg, gtol, &its, &res);
+        if (attempt == 0 && t_prev < 0. && R.span > 0. && G.ws_mode) ws_before(v, step50);
+        const int ok = gummel(v, max_outer, gtol, &its, &res);
         G.conv_iter = its;
         G.conv_res = res;
         if (G.verbose)
-            fprintf(stderr, "semisim-hip: step t=%.4f (V0=%.4f) %s its=%d res=%.2e\n", tn, nc ? Va[0] : 0., ok ? "ok" : "FAIL",
-                    its, res);
-        if (ok) {
-            for (int a = 0; a < 3; a++) std::swap(G.d_h1[a], G.d_h2[a]);
-            save_state(G.d_h1);
-            tp = t; t = tn; steps++;
-            g_prog[0] = t; g_prog[3] = steps;
-            for (int c = 0; c < nc; c++) G.cons[c].Vprev = Va[c];
-            if (t >= 1.) { ok_all = 1; break; }
-            dt *= its <= 8 ? 2.0 : (its <= 25 ? 1.5 : GROW_SLOW);
-            cfail = 0;
-        } else {
-            fails++; cfail++; rx = 0.;
-            if (span == 0. || dt * span < 2e-5 || cfail >= max_cfail) break;
+            fprintf(stderr, "semisim-hip: sub-step to t = %.4f (V[0] = %.4f) %s, %d outer iterations, residual %.2e\n",
+                    t_next, R.nc ? v[0] : 0., ok ? "converged" : "failed", its, res);
+        if (!ok) {
+            failures++;
+            in_row++;
+            R.secant = 0.;                              // retry from the neutral-region shift
+            if (R.span == 0. || dt * R.span < 2e-5 || in_row >= fail_limit) break;
             dt *= 0.3;
+            continue;
         }
+        for (int a = 0; a < 3; a++) std::swap(G.d_h1[a], G.d_h2[a]);   // x2 <- x1, x1 <- this state
+        save_state(G.d_h1);
+        t_prev = t;
+        t = t_next;
+        steps++;
+        g_prog[0] = t; g_prog[3] = steps;
+        for (int c = 0; c < R.nc; c++) G.cons[c].Vprev = v[c];
+        if (t >= 1.) { reached = 1; break; }
+        dt *= step_growth(its);
+        in_row = 0;
     }
-    // a bias point reached in one step from the saved prediction: training sample
-    if (ok_all && span > 0. && G.ws_mode == 1 && steps == 1 && fails == 0 && G.ws_have) ws_capture();
+    // a bias point reached in one sub-step from the saved prediction is a training sample
+    if (reached && R.span > 0. && G.ws_mode == 1 && steps == 1 && failures == 0 && G.ws_have) ws_capture();
     G.ws_have = 0;
-    if (ok_all && span > 0.) {
-        for (int c = 0; c < nc; c++) { G.Ve3[c] = G.Ve2[c]; G.Ve2[c] = Vf[c]; G.Ve1[c] = Vt[c]; }
+    if (reached && R.span > 0.) {
+        for (int c = 0; c < R.nc; c++) { G.Ve3[c] = G.Ve2[c]; G.Ve2[c] = R.from[c]; G.Ve1[c] = R.to[c]; }
         G.e_n = G.e_n < 3 ? G.e_n + 1 : 3;
-    } else if (!ok_all) { G.e_n = 0; G.e3_ok = 0; }
-    if (!ok_all && (span > 0. || g_stop)) {
+    } else if (!reached) {
+        G.e_n = 0;
+        G.e3_ok = 0;
+    }
+    if (!reached && (R.span > 0. || g_stop)) {          // keep the last converged sub-step
         load_state(G.d_h1);
-        for (int c = 0; c < nc; c++) Va[c] = G.cons[c].Vprev;
-        eng_set_vrange(Va);
-        apply_bc(Va);
+        for (int c = 0; c < R.nc; c++) v[c] = G.cons[c].Vprev;
+        eng_set_vrange(v);
+        apply_bc(v);
         eng_rebuild_np();
         eng_upd_mob();
-        if (G.verbose) fprintf(stderr, "semisim-hip: continuation stopped at t=%.4f\n", t);
+        if (G.verbose) fprintf(stderr, "semisim-hip: bias ramp ended at t = %.4f\n", t);
     }
-    (void)fails;
-    return ok_all;
+    return reached;
 }
 
-int eng_run_continuation() {
+// Terminal currents and their floors (semiconductor_3d.c:4151-4171).  The
+// flux out of a contact is a small difference of large drift and diffusion
+// terms, so its round-off floor is 3e-14 of the flux magnitudes S.  A carrier
+// contact whose own floor exceeds the other terminals' together reports minus
+// their sum (Kirchhoff) with their floor instead; every floor widens when the
+// currents fail to sum to zero within the floors; a converged point with
+// impact ionisation adds its series remainder.
+static void report_currents(int ok) {
     const int nc = G.n_cons;
-    int gch = 0, cch = 0, ok_all;
-    for (int c = 0; c < nc; c++) {
-        if (fabs(G.cons[c].V - G.cons[c].Vprev) <= 1e-12) continue;
-        if (G.cons[c].bc == 2) gch = 1; else cch = 1;
-    }
-    if (gch && cch) {
-        double Vt[SD_NC_MAX];
-        for (int c = 0; c < nc; c++) { Vt[c] = G.cons[c].V; if (G.cons[c].bc != 2) G.cons[c].V = G.cons[c].Vprev; }
-        g_prog[4] = 2.;
-        ok_all = cont_path();
-        for (int c = 0; c < nc; c++) G.cons[c].V = Vt[c];
-        g_prog[4] = 3.;
-        if (ok_all) ok_all = cont_path();
-    } else {
-        g_prog[4] = gch ? 2. : 3.;
-        ok_all = cont_path();
-    }
-    double I[SD_NC_MAX], Sv[SD_NC_MAX];
+    double I[SD_NC_MAX], Sv[SD_NC_MAX], own[SD_NC_MAX];
     terminal_currents(I, Sv);
-    double ks = 0., F = 0., fl[SD_NC_MAX];
-    int ncar = 0;
-    for (int c = 0; c < nc; c++) { ks += I[c]; fl[c] = 3e-14 * Sv[c]; F += fl[c]; if (G.cons[c].bc != 2) ncar++; }
-    double scl = F > 0. ? fmax(1., 2. * fabs(ks) / F) : 1.;
+    double net = 0., floors = 0.;
+    int carriers = 0;
     for (int c = 0; c < nc; c++) {
-        double Iv = I[c], fv = fl[c], fo = F - fl[c];
-        if (G.cons[c].bc != 2 && ncar > 1 && fo < fv) { Iv = -(ks - I[c]); fv = fo; }
-        G.cons[c].I = Iv;
-        G.cons[c].Iflr = fv * scl + (ok_all ? G.ii_err : 0.);
+        net += I[c];
+        own[c] = 3e-14 * Sv[c];
+        floors += own[c];
+        if (G.cons[c].bc != 2) carriers++;
+    }
+    const double widen = floors > 0. ? fmax(1., 2. * fabs(net) / floors) : 1.;
+    for (int c = 0; c < nc; c++) {
+        double Ic = I[c], fc = own[c];
+        const double others = floors - own[c];
+        if (G.cons[c].bc != 2 && carriers > 1 && others < fc) { Ic = -(net - I[c]); fc = others; }
+        G.cons[c].I = Ic;
+        G.cons[c].Iflr = fc * widen + (ok ? G.ii_err : 0.);
         G.cons[c].Iraw = I[c];
     }
-    return ok_all;
+}
+
+// When gates and carrier terminals both change, the gates ramp first with
+// the terminals held, then the terminals (semiconductor_3d.c:4135-4150).
+int eng_run_continuation() {
+    const int nc = G.n_cons;
+    int gates = 0, terminals = 0;
+    for (int c = 0; c < nc; c++) {
+        if (fabs(G.cons[c].V - G.cons[c].Vprev) <= 1e-12) continue;
+        if (G.cons[c].bc == 2) gates = 1; else terminals = 1;
+    }
+    int ok;
+    if (gates && terminals) {
+        double target[SD_NC_MAX];
+        for (int c = 0; c < nc; c++) {
+            target[c] = G.cons[c].V;
+            if (G.cons[c].bc != 2) G.cons[c].V = G.cons[c].Vprev;
+        }
+        g_prog[4] = 2.;
+        ok = cont_path();
+        for (int c = 0; c < nc; c++) G.cons[c].V = target[c];
+        g_prog[4] = 3.;
+        ok = ok && cont_path();
+    } else {
+        g_prog[4] = gates ? 2. : 3.;
+        ok = cont_path();
+    }
+    report_currents(ok);
+    return ok;
 }
 
 void eng_display() {

@@ -238,7 +238,7 @@ __global__ void k_cont(DevView d, int hole, Sten7<double> A, double *rhs, double
     double vol = g.Lx[i] * g.Ly[j] * g.Lz[l];
     dg += vol * dR;
     b += vol * (dR * car[k] - R);
-    if (d.tk) { dg += d.tk[k]; b += d.tg[k]; }     // gate tunnelling: sink tk*c - tg
+    if (d.tk) { dg += d.tk[k]; b += d.tg[k]; }     // tunnelling loss linear in the carrier, plus its source
     if (d.gii) b += d.gii[k];
     double w = 1.0 / (dg * sk);
     A.c[k] = 1.0;

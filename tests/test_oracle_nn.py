@@ -68,3 +68,26 @@ def test_bf16_rounding():
     assert r[0] == 1.0 and r[1] == 1.0          # tie to even
     assert r[2] == 1.0 + 2 ** -7                 # nearest of the 2^-7 spaced values
     assert r[3] == -3.140625 and r[4] == 0.0
+
+
+def test_adam_closed_forms():
+    """Adam's bias correction makes the moment estimates of a constant
+    gradient exact at every step (m_hat = g, v_hat = g^2), so each step moves a
+    parameter by lr g / (|g| + eps); a dropped correction (a first step of
+    lr (1-b1)/sqrt(1-b2) = 3.16 lr) or swapped betas fail.  A gradient that
+    changes sign mixes the moments as Kingma & Ba's Algorithm 1 states."""
+    rng = np.random.default_rng(4)
+    g = rng.standard_normal(64)
+    p0 = rng.standard_normal(64)
+    m, v, p = np.zeros(64), np.zeros(64), p0.copy()
+    lr, eps = 1e-3, 1e-8
+    for t in range(1, 6):
+        p = NO.adam(p, g, m, v, t, lr)
+        np.testing.assert_allclose(p, p0 - t * lr * g / (np.abs(g) + eps), rtol=0, atol=1e-15 * t)
+    # second step with the gradient reversed: m = 0.09 g - 0.1 g, v = (0.000999 + 0.001) g^2
+    m, v, p = np.zeros(64), np.zeros(64), p0.copy()
+    p1 = NO.adam(p, g, m, v, 1, lr)
+    p2 = NO.adam(p1, -g, m, v, 2, lr)
+    mh = (0.9 * 0.1 - 0.1) * g / (1 - 0.9 ** 2)
+    vh = (0.999 * 0.001 + 0.001) * g * g / (1 - 0.999 ** 2)
+    np.testing.assert_allclose(p2, p1 - lr * mh / (np.sqrt(vh) + eps), rtol=1e-13, atol=1e-16)
