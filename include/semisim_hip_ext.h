@@ -150,6 +150,15 @@ void   sdx_test_upd_mob(double *fmu6);
 void   sdx_test_set_fmu(const double *fmu6);
 void   sdx_test_cont(int hole, double *A7, double *rhs, double *sc, double *dsc);
 void   sdx_test_load_op(const double *A7, int qsign, const double *w);
+/* graph devices (api3_set_mesh_graph): the operator last assembled as its
+   diagonal and one value per directed edge (returns the edge count, -1 on a
+   tensor device); loading such an operator; the edge mobilities, electrons
+   then holes per undirected edge (set = 0 read, 1 replace; returns their
+   count); the number of levels of the graph multigrid */
+int    sdx_test_graph_op(double *dg, double *va);
+int    sdx_test_graph_load_op(const double *dg, const double *va, int qsign, const double *w);
+int    sdx_test_graph_gmu(double *gmu, int set);
+int    sdx_test_graph_levels(void);
 void   sdx_test_matvec(const double *x, double *y);
 int    sdx_test_mg_setup(void);                     /* number of levels */
 void   sdx_test_level_dims(int L, int *dims);      /* Nx Ny Nz cx cy cz */

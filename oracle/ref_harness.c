@@ -10,6 +10,8 @@
  *   terminal_currents semiconductor_3d.c:3598-3623
  *   cont_update    semiconductor_3d.c:3112-3146
  *   ii_update      semiconductor_3d.c:4486-4523
+ *   graph devices (unstr): the operator diagonal and per-edge values (gval),
+ *   edge mobilities (gmu), upd_mob_ref_g :4240-4292
  * Built by oracle/Makefile into oracle/_ref/libsemisim_refh.so.
  */
 #include REF_SRC
@@ -159,4 +161,18 @@ void refh_tun(const double *Va, int hole, double *cK, double *pG, double *tk, do
     tun_scatter(&gdev,hole);
     memcpy(cK,gdev.col_K,(size_t)2*gdev.n_col*8); memcpy(pG,gdev.tp_G,(size_t)2*gdev.n_tp*8);
     memcpy(tk,gdev.tk,N*8); memcpy(tg,gdev.tg,N*8);
+}
+
+/* graph devices: operator diagonal and off-diagonal per directed edge, edge
+   mobilities (electrons, then holes, per undirected edge) */
+int refh_nE(void){ return gdev.nE; }
+int refh_nUE(void){ return gdev.nUE; }
+void refh_get_diag(double *out){ memcpy(out,gdev.A.c,NN(&gdev)*8); }
+void refh_get_gval(double *out){ memcpy(out,gdev.gval,(size_t)gdev.nE*8); }
+void refh_upd_mob_g(double *gmu){
+    upd_mob_ref_g(&gdev,gdev.phi_n,gdev.phi_p);
+    for(int q=0;q<2;q++) for(int u=0;u<gdev.nUE;u++) gmu[(size_t)q*gdev.nUE+u]=(double)gdev.gmu[q][u];
+}
+void refh_set_gmu(const double *gmu){
+    for(int q=0;q<2;q++) for(int u=0;u<gdev.nUE;u++) gdev.gmu[q][u]=(mu_t)gmu[(size_t)q*gdev.nUE+u];
 }

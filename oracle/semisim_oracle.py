@@ -16,7 +16,8 @@ reference function it restates.  Pinning (see tests/test_oracle_*.py):
     mg_hierarchy/galerkin/restrict/prolong, terminal_currents, cont_update,
     ii_update, walls (interface distance, MLDA), face_mobility with Lombardi,
     charge_partition, terminal_charges, readback, Anderson, tun_paths,
-    tun_update, tun_gather, tun_gate_currents:
+    tun_update, tun_gather, tun_gate_currents, and on graph meshes
+    GraphDevice, graph_poisson, graph_mobility, graph_cont, graph_currents:
     compared with the reference harness on small devices (golden fixtures
     tests/golden/*.npz written by tests/golden/make_golden.py) and with
     invariants (Poisson Jacobian symmetric on free rows, row sums of the
@@ -1283,4 +1284,194 @@ def tun_gate_currents(dev, P, cK, pG, n, p):
         I[c] += Ie + Ih
         S[c] += Q * (se + pG[2 * t] + sh + pG[2 * t + 1])
         In[c] += Ie
+    return I, S, In
+
+
+# ------------------------------------------------------- graph (prism) meshes
+class GraphDevice:
+    """Prepared graph device (alloc_dev_graph :981-1040 and prepare
+    :1638-1775 on an unstructured mesh) of one semiconductor (material 0):
+    N nodes, directed edges in CSR (rp, cj) with length h and dual face area
+    ar, control volumes vol, positions pos (N, 3).  Contacts are node lists:
+    dicts with bc, V, pm, tox (m), qox (C/m^2), face, and `nodes`, `areas`
+    (the boundary area of each node for a face contact)."""
+
+    def __init__(self, rp, cj, h, ar, vol, pos, Nd, Na, contacts, models=7, tau=None, T=300.0, Nf=None):
+        self.rp, self.cj = np.asarray(rp, np.int64), np.asarray(cj, np.int64)
+        self.h, self.ar, self.vol = (np.asarray(a, np.float64) for a in (h, ar, vol))
+        self.pos = np.asarray(pos, np.float64).reshape(-1, 3)
+        self.N, self.E = len(self.vol), len(self.cj)
+        self.row = np.repeat(np.arange(self.N), np.diff(self.rp))        # source node of every directed edge
+        key = self.row * self.N + self.cj                                 # reverse edge and undirected id
+        order = np.argsort(key)
+        rkey = self.cj * self.N + self.row
+        self.rev = order[np.searchsorted(key[order], rkey)]
+        self.ue = np.full(self.E, -1, np.int64)
+        lo = self.row < self.cj
+        self.ue[lo] = np.arange(int(lo.sum()))
+        self.ue[~lo] = self.ue[self.rev[~lo]]
+        self.nue = int(lo.sum())
+        self.m = silicon(T)
+        if tau is not None:
+            self.m["tau"] = tuple(t if t > 0 else 1e30 for t in tau)
+        self.models = models
+        self.Nd, self.Na = np.asarray(Nd, np.float64), np.asarray(Na, np.float64)
+        self.Nf = np.zeros(self.N) if Nf is None else np.asarray(Nf, np.float64)
+        self.contacts = contacts
+        m = self.m
+        VT = m["VT"]
+        self.Cref = m["chi"] + 0.5 * m["Eg"] + 0.5 * VT * math.log(m["Nc"] / m["Nv"])
+        self.lnc = math.log(m["Nc"]) + (m["chi"] - self.Cref) / VT
+        self.lnv = math.log(m["Nv"]) + (self.Cref - m["chi"] - m["Eg"]) / VT
+        self.kind = np.zeros(self.N, np.uint8)
+        self.eps = np.full(self.N, m["eps_r"] * EPS0)
+        self.bgn = bgn_half(m, self.Nd + self.Na) if (models & 1) else np.zeros(self.N)
+        self.con_mask = np.zeros(self.N, np.int32)
+        self.gate_mask = np.zeros(self.N, np.int32)
+        self.robA = np.zeros(self.N)
+        for c, cn in enumerate(contacts):                        # FOR_CON_ANY over the node lists :1656-1668
+            for k in cn["nodes"]:
+                if cn["bc"] == 2 and cn["face"] <= 5 and self.kind[k] == K_SEMI:
+                    self.gate_mask[k] = c + 1
+                elif cn["bc"] == 2 or self.kind[k] != K_SEMI:
+                    self.kind[k] = K_METAL
+                    self.con_mask[k] = c + 1
+                    self.gate_mask[k] = 0
+                else:
+                    self.con_mask[k] = c + 1
+                    self.gate_mask[k] = 0
+        for c, cn in enumerate(contacts):                        # Robin list :1676-1686
+            if cn["bc"] != 2 or cn["face"] > 5:
+                continue
+            for k, a in zip(cn["nodes"], cn["areas"]):
+                if self.gate_mask[k] == c + 1 and not self.con_mask[k]:
+                    self.robA[k] = a
+        self.lncK = self.lnc + self.bgn
+        self.lnvK = self.lnv + self.bgn
+        self.niK = m["ni"] * np.exp(self.bgn)
+        semi = self.kind == K_SEMI
+        self.mu0n = np.where(semi, mob_low(m, self.Nd + self.Na, 0), 0.0)
+        self.mu0p = np.where(semi, mob_low(m, self.Nd + self.Na, 1), 0.0)
+        if (models & 2) and m["tau_Nref"] > 0:
+            self.tauf = 1.0 / (1.0 + (self.Nd + self.Na) / m["tau_Nref"])
+        else:
+            self.tauf = np.ones(self.N)
+
+    def carriers(self, psi, phin, phip):
+        VT = self.m["VT"]
+        semi = self.kind == K_SEMI
+        n = np.exp(np.clip(self.lncK + (psi - phin) / VT, LN_MIN, LN_MAX))
+        p = np.exp(np.clip(self.lnvK + (phip - psi) / VT, LN_MIN, LN_MAX))
+        return np.where(semi, n, 0.0), np.where(semi, p, 0.0)
+
+
+def graph_poisson(dev, psi, phin, phip, Va):
+    """Poisson residual F, Jacobian (diagonal dg, one value per directed edge
+    va), row scale and ||F/sc|| on a graph, poisson_core_g :4181-4213 with the
+    Robin gate terms of poisson_eval :2776-2782."""
+    r, j = dev.row, dev.cj
+    ek, eb = dev.eps[r], dev.eps[j]
+    cf = (2.0 * ek * eb / (ek + eb)) * dev.ar / dev.h
+    F = np.zeros(dev.N)
+    dg = np.zeros(dev.N)
+    np.add.at(F, r, cf * (psi[j] - psi[r]))
+    np.add.at(dg, r, cf)
+    free = dev.con_mask == 0
+    va = np.where(free[j] & free[r], -cf, 0.0)
+    VT = dev.m["VT"]
+    semi = dev.kind == K_SEMI
+    n, p = dev.carriers(psi, phin, phip)
+    F += np.where(semi, dev.vol * Q * (p - n + dev.Nd - dev.Na + dev.Nf), dev.vol * Q * dev.Nf)
+    dg += np.where(semi, dev.vol * Q * (n + p) / VT, 0.0)
+    for c, cn in enumerate(dev.contacts):
+        if cn["bc"] != 2 or cn["face"] > 5:
+            continue
+        rob = (dev.gate_mask == c + 1) & free & (dev.robA > 0)
+        Cox = EPS_OX / (cn["tox"] if cn["tox"] > 0 else 10e-9)
+        pm = cn["pm"] if cn["pm"] > 0 else dev.Cref
+        VG = Va[c] + dev.Cref - pm
+        F[rob] += dev.robA[rob] * (Cox * (VG - psi[rob]) + cn.get("qox", 0.0))
+        dg[rob] += Cox * dev.robA[rob]
+    F[~free] = 0.0
+    dg[~free] = 1.0
+    return F, dg, va, dg.copy(), float(np.sqrt(np.sum((F / dg) ** 2)))
+
+
+def graph_mobility(dev, psi, qn, qp):
+    """Edge mobilities (electrons, holes) per undirected edge, upd_mob_ref_g
+    :4240-4292 (isotropic material, no Lombardi, no metal contacts)."""
+    lo = dev.row < dev.cj
+    k, kp, h = dev.row[lo], dev.cj[lo], dev.h[lo]
+    ok = (dev.kind[k] == K_SEMI) & (dev.kind[kp] == K_SEMI)
+    mn = 0.5 * (dev.mu0n[k] + dev.mu0n[kp])
+    mp = 0.5 * (dev.mu0p[k] + dev.mu0p[kp])
+    if dev.models & 4:
+        m = dev.m
+        dps = psi[kp] - psi[k]
+        sn = mob_hf(mn, heat_field(qn[kp] - qn[k], dps, h), m["vsat"][0], m["beta"][0])
+        sp = mob_hf(mp, heat_field(qp[kp] - qp[k], dps, h), m["vsat"][1], m["beta"][1])
+        mn = mn + 1.0 * (sn - mn)
+        mp = mp + 1.0 * (sp - mp)
+    gmu = np.zeros((2, dev.nue))
+    gmu[0, dev.ue[lo]] = np.where(ok, mn, 0.0)
+    gmu[1, dev.ue[lo]] = np.where(ok, mp, 0.0)
+    return gmu
+
+
+def graph_cont(dev, hole, psi, n, p, gmu):
+    """Scaled Scharfetter-Gummel rows on a graph, cont_assemble_g :4313-4346:
+    unit diagonal, edge values va, rhs, sc, dsc."""
+    VT = dev.m["VT"]
+    car = p if hole else n
+    semi = dev.kind == K_SEMI
+    fix = dev.con_mask != 0
+    eq = semi & ~fix
+    r, j = dev.row, dev.cj
+    lnk = dev.lnvK if hole else dev.lncK
+    dl = lnk[j] - lnk[r]
+    D = (psi[j] - psi[r]) / VT + (-dl if hole else dl)
+    g = dev.ar * gmu[1 if hole else 0, dev.ue] * VT / dev.h
+    self_ = g * (bernoulli(D) if hole else bernoulli(-D))
+    nb = g * (bernoulli(-D) if hole else bernoulli(D))
+    act = eq[r] & semi[j]
+    dg = np.zeros(dev.N)
+    b = np.zeros(dev.N)
+    np.add.at(dg, r[act], self_[act])
+    fj = act & fix[j]
+    np.add.at(b, r[fj], (nb * car[j])[fj])
+    va = np.where(act & ~fix[j], -nb * np.maximum(car[j], NFLOOR), 0.0)
+    R, dRn, dRp = recomb(dev.m, dev.niK, dev.tauf, n, p)
+    dR = np.maximum(dRp if hole else dRn, 0.0)
+    dg = dg + dev.vol * dR
+    b = b + dev.vol * (dR * car - R)
+    sk = np.maximum(car, NFLOOR)
+    with np.errstate(invalid="ignore", over="ignore", divide="ignore"):
+        w = 1.0 / (dg * sk)
+        va = np.where(eq[r], va * w[r], 0.0)
+        rhs = np.where(eq, b * w, np.where(semi, 1.0, 0.0))
+    sc = np.where(eq, sk, np.where(semi, car, 0.0))
+    dsc = np.where(eq, dg * sk, 0.0)
+    return np.ones(dev.N), va, rhs, sc, dsc
+
+
+def graph_currents(dev, psi, n, p, gmu):
+    """Terminal currents on a graph (A), their flux scales and electron parts,
+    terminal_flux_g :4412-4424 with face_flux_g :4215-4226."""
+    nc = len(dev.contacts)
+    I, S, In = np.zeros(nc), np.zeros(nc), np.zeros(nc)
+    r, j = dev.row, dev.cj
+    c = dev.con_mask[r]
+    sel = (c > 0) & (dev.kind[r] == K_SEMI) & (dev.kind[j] == K_SEMI) & (dev.con_mask[j] != c)
+    ka, kb, e = r[sel], j[sel], np.nonzero(sel)[0]
+    VT = dev.m["VT"]
+    dps = (psi[kb] - psi[ka]) / VT
+    Dn = dps + (dev.lncK[kb] - dev.lncK[ka])
+    Dp = dps - (dev.lnvK[kb] - dev.lnvK[ka])
+    gn = Q * dev.ar[e] * gmu[0, dev.ue[e]] * VT / dev.h[e]
+    gp = Q * dev.ar[e] * gmu[1, dev.ue[e]] * VT / dev.h[e]
+    a1, a2 = gn * n[kb] * bernoulli(Dn), gn * n[ka] * bernoulli(-Dn)
+    b1, b2 = gp * p[ka] * bernoulli(Dp), gp * p[kb] * bernoulli(-Dp)
+    np.add.at(I, c[sel] - 1, (a1 - a2) + (b1 - b2))
+    np.add.at(S, c[sel] - 1, (a1 + a2) + (b1 + b2))
+    np.add.at(In, c[sel] - 1, a1 - a2)
     return I, S, In

@@ -19,7 +19,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result", "-Wno-unused-value"]
-SOURCES = ["k_linalg.hip", "k_physics.hip", "k_nn.hip", "k_csr.hip", "engine.cpp", "nn.cpp", "api.cpp"]
+SOURCES = ["k_linalg.hip", "k_physics.hip", "k_nn.hip", "k_csr.hip", "k_graph.hip", "engine.cpp", "graph.cpp", "nn.cpp",
+           "api.cpp"]
 
 
 def build(verbose=False, jobs=4):

@@ -78,6 +78,9 @@ SIGNATURES = [
     ("api3_get_aa_depth", I, []),
     ("api3_set_node_data", None, [IP, DP, DP, DP]),
     ("api3_set_con_nodes", I, [I, I, IP, DP]),
+    ("api3_set_mesh_graph", I, [I, I, IP, IP, DP, DP, DP, DP]),
+    ("api3_is_graph", I, []),
+    ("api3_set_walls", None, [DP]),
     ("api3_set_mat_tau", I, [I, D, D]),
     ("api3_carrier_sums", None, [DP]),
     ("api3_ins_field_max", D, [DP]),
@@ -130,6 +133,10 @@ EXT_SIGNATURES = [
     ("sdx_test_set_fmu", None, [DP]),
     ("sdx_test_cont", None, [I, DP, DP, DP, DP]),
     ("sdx_test_load_op", None, [DP, I, DP]),
+    ("sdx_test_graph_op", I, [DP, DP]),
+    ("sdx_test_graph_load_op", I, [DP, DP, I, DP]),
+    ("sdx_test_graph_gmu", I, [DP, I]),
+    ("sdx_test_graph_levels", I, []),
     ("sdx_test_matvec", None, [DP, DP]),
     ("sdx_test_mg_setup", I, []),
     ("sdx_test_level_dims", None, [I, IP]),

@@ -184,12 +184,23 @@ int api3_set_gate_stack(int cid, int nl, const int *mat, const double *t) {
     mark_changed();
     return 1;
 }
-int api3_set_mesh_graph(int, int, const int *, const int *, const double *, const double *, const double *,
-                        const double *) {
-    unsupported("the triangular-prism (graph) mesh");
-    return 0;
+// A device on a general box-method mesh (api3_set_mesh_graph :4795-4804): N
+// nodes, E directed edges as a symmetric CSR graph (rp[N+1], cj[E]); per
+// directed edge its length h and dual face area ar, per node its control
+// volume vol and position pos (3N).  Replaces the tensor grid; 0 on a bad
+// graph (not symmetric, a self edge, h <= 0, ar < 0, vol <= 0) or no memory.
+// The graph's aggregation multigrid leaves smooth continuity errors larger
+// than the tensor multigrid does for the same residual, so its BiCGSTAB
+// tolerance is 1e-6 unless SEMISIM_BRTOL sets one (as the reference's).
+int api3_set_mesh_graph(int N, int E, const int *rp, const int *cj, const double *h, const double *ar,
+                        const double *vol, const double *pos) {
+    if (!rp || !cj || !h || !ar || !vol || !pos) return 0;
+    const int ok = eng_set_graph(N, E, rp, cj, h, ar, vol, pos);
+    if (ok && !getenv("SEMISIM_BRTOL")) G.bicg_rtol = 1e-6;
+    disp_valid = 0;
+    return ok;
 }
-int api3_is_graph(void) { return 0; }
+int api3_is_graph(void) { return G.graph; }
 void api3_set_node_data(const int *mat, const double *Nd, const double *Na, const double *Nf) {
     if (!G.allocated) return;
     for (int k = 0; k < G.N; k++) {
@@ -220,7 +231,14 @@ int api3_set_con_nodes(int cid, int n, const int *nodes, const double *areas) {
     mark_changed();
     return 1;
 }
-void api3_set_walls(const double *) {}
+// interface walls of a graph device (:4836-4842): per node two walls, each
+// the node's distance to it and the range a..b of its cell measured from it
+// (m); a distance >= 0.5 m is no wall
+void api3_set_walls(const double *w) {
+    if (!G.allocated || !G.graph || !w) return;
+    G.gwall.assign(w, w + (size_t)6 * G.N);
+    mark_changed();
+}
 // tunnelling paths of the last prepared device (all, or those of contact cid)
 int api3_get_ntp(int cid) {
     if (cid < 0) return (int)G.tp_k.size();
@@ -333,6 +351,7 @@ static std::vector<double> dl(const double *src) {
 
 double api3_get_Emax(void) {
     if (!G.allocated || !G.have_solution) return 0.;
+    if (G.graph) return gr_emax();
     std::vector<double> phi = dl(G.d_psi);
     double Em = 0.;
     for (int k = 0; k < G.N; k++) {

@@ -58,6 +58,9 @@ template <class T> static T *palloc(size_t n) {
     G.prep_allocs.push_back(p);
     return (T *)p;
 }
+void *eng_dalloc_bytes(size_t bytes) { return dalloc<unsigned char>(bytes); }
+void *eng_palloc_bytes(size_t bytes) { return palloc<unsigned char>(bytes); }
+void eng_set_error() { g_err = 1; }
 static void prep_free() {
     if (G.st) hipStreamSynchronize(G.st);
     for (void *p : G.prep_allocs) hipFree(p);
@@ -66,6 +69,7 @@ static void prep_free() {
     G.d_tp_off = nullptr;
     G.d_ek = G.d_ekb = G.d_eoff = nullptr;
     G.d_eh = G.d_ear = nullptr;
+    G.d_gck = G.d_gce = G.d_gcoff = nullptr;
 }
 
 void eng_sync() { HC(hipStreamSynchronize(G.st)); }
@@ -136,6 +140,13 @@ void eng_free() {
     G.ws_have = 0;
     G.allocated = 0;
     G.have_solution = 0;
+    G.graph = 0;
+    G.nE = G.nUE = 0;
+    G.grp.clear(); G.gcol.clear(); G.grev.clear(); G.gue.clear();
+    G.gh.clear(); G.ga.clear(); G.gvol.clear(); G.gpos.clear(); G.gwall.clear();
+    G.gd = GraphDev{};
+    G.d_gval = G.d_gmu = nullptr;
+    gr_free();
 }
 
 // Multigrid hierarchy shape: 2x2x2 aggregation of every axis longer than 2
@@ -210,9 +221,10 @@ template <class T> static int hier_alloc(Hier<T> &H, bool level0_own) {
     return H.alloc;
 }
 
-int eng_alloc(int Nx, int Ny, int Nz) {
+int eng_alloc(int Nx, int Ny, int Nz, int graph) {
     eng_free();
     g_err = 0;
+    G.graph = graph;
     if (!G.st) {
         // The solvers read a few device scalars back after most reductions,
         // so the host's wait latency is on the critical path: wait by
@@ -264,8 +276,11 @@ int eng_alloc(int Nx, int Ny, int Nz) {
                       G.d_mudn, G.d_mudp})
         HC(hipMemsetAsync(p, 0, N * 8, G.st));
     HC(hipMemsetAsync(G.d_fmu, 0, 6 * N * 8, G.st));
-    if (!hier_alloc(G.H64, false)) { eng_free(); return 0; }
-    G.H64.lev[0].A = G.A;
+    // the tensor multigrid (a graph device has its own, built by eng_set_graph)
+    if (!graph) {
+        if (!hier_alloc(G.H64, false)) { eng_free(); return 0; }
+        G.H64.lev[0].A = G.A;
+    }
     eng_sync();
     if (g_err) { eng_free(); return 0; }
     G.allocated = 1;
@@ -375,6 +390,7 @@ template <class F> static void for_con_nodes(const HostCon &cn, F body) {
         }
         return;
     }
+    if (G.graph) return;                 // a contact of a graph device has no nodes but its list
     int ia = 0, ib = G.Nx - 1, ja = 0, jb = G.Ny - 1, la = 0, lb = G.Nz - 1;
     switch (cn.face) {
     case 0: ia = ib = 0; ja = cn.i0; jb = cn.i1; la = cn.j0; lb = cn.j1; break;
@@ -648,6 +664,11 @@ int eng_prepare() {
                 why = "metal-contact emission on a Schottky face contact (models bits 6-7)";
         for (int i = 0; i < G.n_mats && !why; i++)
             if ((G.models & 512) && G.mats[i].id == 4) why = "incomplete ionisation (models bit 9)";
+        if (G.graph && !why) {
+            if (G.models & 16) why = "Lombardi surface mobility on a graph mesh (models bit 4)";
+            else if (G.models & 32) why = "gate tunnelling on a graph mesh (models bit 5)";
+            else if (G.models & 256) why = "impact ionisation on a graph mesh (models bit 8)";
+        }
         if (why) { fprintf(stderr, "semisim-hip: %s is outside this core: not solved\n", why); return 0; }
     }
     if (!G.dirty) return 1;
@@ -689,9 +710,10 @@ int eng_prepare() {
         }
     }
     for (int k = 0; k < N; k++) if (G.gate_mask[k] && (G.con_mask[k] || G.robA[k] == 0.)) G.gate_mask[k] = 0;
-    if (!tun_build()) return 0;
-    // box contacts: area = interface with non-contact semiconductor
-    for (int c = 0; c < G.n_cons; c++) {
+    if (!G.graph && !tun_build()) return 0;
+    // box contacts: area = interface with non-contact semiconductor (graph:
+    // gr_prepare_edges below)
+    for (int c = 0; c < G.n_cons && !G.graph; c++) {
         HostCon &cn = G.cons[c];
         if (cn.face <= 5) continue;
         double Ab = 0.;
@@ -707,7 +729,10 @@ int eng_prepare() {
     // node band data and low-field mobility (constant for the device); with
     // MLDA (models bit 3) the confinement factors qcn, qcp of the effective
     // densities of states near interfaces (lncK, lnvK, niK :736-738)
-    if (G.models & 8) {
+    if (G.graph) {
+        std::vector<double> qd;
+        gr_walls(qd, G.qcn, G.qcp, (G.models & 8) != 0, mlda_factor);
+    } else if (G.models & 8) {
         std::vector<double> qd;
         eng_walls(qd, &G.qcn, &G.qcp);
     } else {
@@ -730,7 +755,7 @@ int eng_prepare() {
     std::vector<int> ek, ekb, eoff(G.n_cons + 1, 0);
     std::vector<double> eh, ear;
     std::vector<std::vector<int>> per(G.n_cons);
-    for (int k = 0; k < N; k++) {
+    for (int k = 0; k < N && !G.graph; k++) {
         int c = G.con_mask[k];
         if (!c || G.kind[k] != K_SEMI || G.cons[c - 1].te) continue;
         per[c - 1].push_back(k);
@@ -752,7 +777,7 @@ int eng_prepare() {
     // line-smoother axes: the two axes with the largest geometric coupling
     // sum_edges area/h (weaker of the two first); point red-black GS if asked
     // for (SEMISIM_MG_SMOOTH=rb) or if a line is longer than the kernel takes
-    {
+    if (!G.graph) {
         double sx = 0., sy = 0., sz = 0., SLx = 0., SLy = 0., SLz = 0.;
         for (double v : G.Lx) SLx += v;
         for (double v : G.Ly) SLy += v;
@@ -790,7 +815,7 @@ int eng_prepare() {
     const std::vector<double> *hv[] = {&G.Nd, &G.Na, &G.Nf, &G.eps, &G.bgn, &G.robA, &G.lncK, &G.lnvK, &G.niK, &G.mu0n, &G.mu0p};
     double *dv[] = {G.d_Nd, G.d_Na, G.d_Nf, G.d_eps, G.d_bgn, G.d_robA, G.d_lncK, G.d_lnvK, G.d_niK, G.d_mun, G.d_mup};
     for (int q = 0; q < 11; q++) HC(hipMemcpyAsync(dv[q], hv[q]->data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
-    if (G.models & 16) {                  // Lombardi: distance of every node to the nearest interface
+    if ((G.models & 16) && !G.graph) {   // Lombardi: distance of every node to the nearest interface
         eng_wall_distance(G.qd);
         HC(hipMemcpyAsync(G.d_qd, G.qd.data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
     }
@@ -809,7 +834,9 @@ int eng_prepare() {
         HC(hipMemcpyAsync(G.d_ear, ear.data(), ear.size() * 8, hipMemcpyHostToDevice, s));
     }
     HC(hipMemcpyAsync(G.d_eoff, eoff.data(), eoff.size() * 4, hipMemcpyHostToDevice, s));
-    tun_upload();
+    if (G.graph) {
+        if (!gr_prepare_edges()) return 0;
+    } else tun_upload();
     eng_sync();
     G.dirty = 0;
     return eng_ok();
@@ -846,7 +873,10 @@ void eng_set_vrange(const double *Va) {
 }
 void eng_clamp_qf() { ph_clamp_qf(G.st, eng_view(), G.vlo, G.vhi); }
 void eng_rebuild_np() { ph_rebuild_np(G.st, eng_view()); }
-void eng_upd_mob() { ph_upd_mob(G.st, eng_view(), G.d_phin, G.d_phip); }
+void eng_upd_mob() {
+    if (G.graph) gr_upd_mob(G.d_phin, G.d_phip);
+    else ph_upd_mob(G.st, eng_view(), G.d_phin, G.d_phip);
+}
 void eng_apply_bc() { ph_apply_bc(G.st, eng_view(), G.have_solution); }
 static void apply_bc(const double *Va) {
     eng_upload_cons(Va);
@@ -928,7 +958,8 @@ template <class T> static void mg_setup_T() {
 }
 static void mg_setup() {
     double t0 = wtime();
-    if (G.precision == 1) mg_setup_T<float>();
+    if (G.graph) gr_mg_setup();
+    else if (G.precision == 1) mg_setup_T<float>();
     else mg_setup_T<double>();
     G.t_mg += wtime() - t0;
 }
@@ -1070,7 +1101,7 @@ template <class T> static void smooth_only_T(const T *b, T *x) {
 // Krylov vectors directly)
 static Stage0 stage0() {
     Stage0 st{nullptr, nullptr, 0, 0, 0, nullptr};
-    if (!G.stage_on) return st;
+    if (!G.stage_on || G.graph) return st;
     if (G.precision == 1 && !G.H32.lev.empty()) {
         const MGLevel<float> &v0 = G.H32.lev[0];
         st = Stage0{G.d_b32, G.d_x32, v0.Nx, v0.px, 1, G.d_x32b};
@@ -1113,6 +1144,7 @@ static void precond_launch(const double *r, double *z, bool sin = false, bool so
 // level-0 smoother timer on, every n-th V-cycle runs as plain launches
 // bracketed by HIP events.
 static void precond(const double *r, double *z, bool sin = false, bool sout = false, bool second = false) {
+    if (G.graph) { gr_precond(r, z); return; }
     G.n_vcycles++;
     bool sample = KT.on && KT.kind == 0 && (KT.vcycles++ % KT.every) == 0;
     const bool whole_timed = KT.on && KT.kind == 2;
@@ -1157,7 +1189,14 @@ static void precond(const double *r, double *z, bool sin = false, bool sout = fa
 }
 static inline void matvec(const double *x, double *y) {
     G.n_spmv++;
-    sd_spmv(G.st, eng_view().g, G.A, x, nullptr, y);
+    if (G.graph) gr_spmv(x, nullptr, y, false);
+    else sd_spmv(G.st, eng_view().g, G.A, x, nullptr, y);
+}
+// y = A x fused with dots (sd_spmv_dots); the graph operator takes no staging
+static inline void spmv_dots(const Geo &g, const Sten7<double> &A, const double *x, double *y, const double *u0,
+                             const double *u1, int nv, int slot, int post, Stage0 st = Stage0{}) {
+    if (G.graph) gr_spmv_dots(x, y, u0, u1, nv, slot, post, A.c == nullptr);
+    else sd_spmv_dots(G.st, g, A, x, y, u0, u1, nv, G.rb, slot, post, st);
 }
 
 // ------------------------------------------------------------ Krylov
@@ -1168,7 +1207,8 @@ static int pcg(const double *b, double *x, int maxit, double rtol) {
     double *r = G.d_wk[0], *z = G.d_wk[1], *p = G.d_wk[2], *q = G.d_wk[3];
     mg_setup();
     Geo g = eng_view().g;
-    sd_spmv(s, g, G.A, x, b, r);          // r = b - A x
+    if (G.graph) gr_spmv(x, b, r, false);  // r = b - A x
+    else sd_spmv(s, g, G.A, x, b, r);
     precond(r, z);
     sd_dot(s, r, z, N, G.rb, S_RZ);
     rd(S_RZ, S_RZ + 1);
@@ -1182,7 +1222,7 @@ static int pcg(const double *b, double *x, int maxit, double rtol) {
     for (it = 1; it <= maxit; it++) {
         G.n_spmv++;
         { bool kt = kt_begin(1);
-          sd_spmv_dots(s, g, G.A, p, q, p, nullptr, 1, G.rb, S_PQ, SOP_PCG_AL);
+          spmv_dots(g, G.A, p, q, p, nullptr, 1, S_PQ, SOP_PCG_AL);
           kt_end(kt); }
         sd_pcg_update_xr(s, x, r, p, q, N, G.rb, st);      // r, staged for the V-cycle
         precond(r, z, sg, sg);
@@ -1216,7 +1256,8 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
     if (!G.mg_off) mg_setup();
     matvec(x, t);
     sd_bi_init(s, r, rh, p, v, b, t, N, G.rb);
-    sd_floor(s, g, G.A, x, b, G.rb, S_FL);
+    if (G.graph) gr_floor(x, b, S_FL);
+    else sd_floor(s, g, G.A, x, b, G.rb, S_FL);
     rd(S_RR, S_FL + 1);
     double rr = S(S_RR), fl = S(S_FL);
     double r0 = sqrt(rr), tgt = fmax(rtol * r0, atol * sqrt(fl));
@@ -1245,7 +1286,7 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
         precond(p, ph, sg, sg);
         G.n_spmv++;
         { bool kt = kt_begin(1);
-          sd_spmv_dots(s, g, Ak, sr ? nullptr : ph, v, rh, nullptr, 1, G.rb, S_RV, SOP_BI_AL, st);   // (also writes ph)
+          spmv_dots(g, Ak, sr ? nullptr : ph, v, rh, nullptr, 1, S_RV, SOP_BI_AL, st);   // (also writes ph)
           kt_end(kt); }
         sd_bi_r1(s, r, v, N, G.rb, st);
         rd(S_AL, S_SS + 1);
@@ -1271,7 +1312,7 @@ static double bicgstab_core(const double *b, double *x, int maxit, double rtol, 
         precond(r, sh, sg, sg, sr);
         G.n_spmv++;
         { bool kt = kt_begin(1);
-          sd_spmv_dots(s, g, Ak, sr ? nullptr : sh, t, nullptr, r, 2, G.rb, S_TT, SOP_BI_OM, sr ? st2 : st);
+          spmv_dots(g, Ak, sr ? nullptr : sh, t, nullptr, r, 2, S_TT, SOP_BI_OM, sr ? st2 : st);
           kt_end(kt); }
         if (sr) sd_bi_xr2_st(s, x, r, st, t, G.bi_fuse ? rh : nullptr, N, G.rb);
         else sd_bi_xr2(s, x, r, ph, sh, t, G.bi_fuse ? rh : nullptr, N, G.rb);
@@ -1294,7 +1335,7 @@ static double bicgstab(const double *b, double *x, int maxit, double rtol, doubl
     const int N = G.N;
     sd_copy(G.st, G.d_xsave, x, N);
     double r0, rn = bicgstab_core(b, x, maxit, rtol, atol, its, &r0);
-    int nl = (int)(G.precision == 1 ? G.H32.lev.size() : G.H64.lev.size());
+    int nl = G.graph ? gr_nlev() : (int)(G.precision == 1 ? G.H32.lev.size() : G.H64.lev.size());
     if (!(rn <= fmax(r0, 1e-300)) && nl > 1) {
         if (G.verbose) fprintf(stderr, "semisim-hip: MG-BiCGSTAB diverged (%.2e > %.2e), retry smoother only\n", rn, r0);
         sd_copy(G.st, x, G.d_xsave, N);
@@ -1313,7 +1354,8 @@ static double bicgstab(const double *b, double *x, int maxit, double rtol, doubl
 
 // ------------------------------------------------------------ Poisson
 static double poisson_eval(const double *phi, int build, double *F) {
-    ph_poisson(G.st, eng_view(), phi, build, F, G.A, G.d_sc, G.rb, S_NRM);
+    if (G.graph) gr_poisson(phi, build, F, S_NRM);
+    else ph_poisson(G.st, eng_view(), phi, build, F, G.A, G.d_sc, G.rb, S_NRM);
     rd(S_NRM, S_NRM + 1);
     return sqrt(S(S_NRM));
 }
@@ -1383,6 +1425,7 @@ static int maj_closure() {
     return round + 1;
 }
 static int float_fix(int hole) {
+    if (G.graph) return gr_float_fix(hole);
     DevView d = eng_view();
     hipStream_t s = G.st;
     ph_maj_flags(s, d, hole, G.d_sol, G.d_sc, G.d_flag);
@@ -1509,7 +1552,8 @@ static int solve_cont(int hole, double *dq, double lo, double hi, double *dmax) 
     hipStream_t s = G.st;
     DevView d = eng_view();
     if (G.tun.np) ph_tun_gather(s, G.tun, hole, G.d_tk, G.d_tg);
-    ph_cont(s, d, hole, G.A, G.d_rhs, G.d_sc, G.d_dsc);
+    if (G.graph) gr_cont(hole);
+    else ph_cont(s, d, hole, G.A, G.d_rhs, G.d_sc, G.d_dsc);
     G.mg_w = G.d_dsc;
     G.mg_qsign = hole ? -1 : 1;
     sd_fill(s, G.d_sol, 1.0, N);
@@ -1528,6 +1572,16 @@ static int solve_cont(int hole, double *dq, double lo, double hi, double *dmax) 
 }
 
 static void terminal_currents(double *I, double *Sv) {
+    if (G.graph) {
+        gr_currents();
+        rd(S_CUR, S_CUR + 3 * std::max(1, G.n_cons));
+        for (int c = 0; c < G.n_cons; c++) {
+            I[c] = S(S_CUR + 3 * c);
+            Sv[c] = S(S_CUR + 3 * c + 1);
+            G.cons[c].In = S(S_CUR + 3 * c + 2);
+        }
+        return;
+    }
     CurEdges e;
     e.k = G.d_ek; e.kb = G.d_ekb; e.h = G.d_eh; e.ar = G.d_ear; e.off = G.d_eoff; e.ncon = G.n_cons;
     e.poff = G.tun.np ? G.d_tp_off : nullptr;
@@ -1788,6 +1842,7 @@ static void load_state(double **h) {
 
 // quasi-neutral regions of ohmic contacts move rigidly with their voltage (host BFS)
 static void region_shift(const double *dV) {
+    if (G.graph) { gr_region_shift(dV); return; }
     const int N = G.N;
     bool any = false;
     for (int c = 0; c < G.n_cons; c++) if (G.cons[c].bc == 0 && fabs(dV[c]) >= 1e-15) any = true;
@@ -2148,6 +2203,7 @@ int eng_run_continuation() {
 }
 
 void eng_display() {
+    if (G.graph) { gr_display(); eng_sync(); return; }
     DevView d = eng_view();
     ph_upd_J(G.st, d, G.d_Jx, G.d_Jy, G.d_Jz, G.d_Rt);
     ph_mob_display(G.st, d, G.d_mudn, G.d_mudp);
@@ -2186,12 +2242,13 @@ void eng_read_env() {
 int eng_aa_depth() { return G.aa_m < AA_DEPTH ? G.aa_m : AA_DEPTH; }
 void eng_set_predictor(int on) { PRED_ON = on != 0; }
 int eng_set_precision(int p) {
-    if (p == 1 && G.allocated && !ensure_h32()) return 0;
+    // a graph device keeps its FP64 aggregation multigrid in either setting
+    if (p == 1 && G.allocated && !G.graph && !ensure_h32()) return 0;
     G.precision = p ? 1 : 0;
     return 1;
 }
 int eng_prepare_precision() {
-    if (G.precision == 1) return ensure_h32();
+    if (G.precision == 1 && !G.graph) return ensure_h32();
     return 1;
 }
 
@@ -2418,7 +2475,8 @@ void sdx_test_upd_mob(double *fmu6) {
 void sdx_test_set_fmu(const double *fmu6) { eng_upload(G.d_fmu, fmu6, 6 * (size_t)G.N); }
 void sdx_test_cont(int hole, double *A7, double *rhs, double *sc, double *dsc) {
     if (G.tun.np) ph_tun_gather(G.st, G.tun, hole, G.d_tk, G.d_tg);
-    ph_cont(G.st, eng_view(), hole, G.A, G.d_rhs, G.d_sc, G.d_dsc);
+    if (G.graph) gr_cont(hole);
+    else ph_cont(G.st, eng_view(), hole, G.A, G.d_rhs, G.d_sc, G.d_dsc);
     get_A7(A7);
     eng_download(rhs, G.d_rhs, G.N); eng_download(sc, G.d_sc, G.N); eng_download(dsc, G.d_dsc, G.N);
 }
@@ -2449,6 +2507,33 @@ long long sdx_test_csr(int *rowptr, int *col, double *val, long long cap) {
     for (void *p : {(void *)d_rp, (void *)d_col, (void *)d_val, (void *)d_s}) if (p) hipFree(p);
     return nnz;
 }
+// graph device: the operator last assembled (diagonal, one value per
+// directed edge); returns the edge count (-1: not a graph device)
+int sdx_test_graph_op(double *dg, double *va) {
+    if (!G.graph) return -1;
+    eng_download(dg, G.A.c, G.N);
+    eng_download(va, G.d_gval, G.nE);
+    return G.nE;
+}
+// load a graph operator (diagonal, edge values) as sdx_test_load_op does
+int sdx_test_graph_load_op(const double *dg, const double *va, int qsign, const double *w) {
+    if (!G.graph) return -1;
+    eng_upload(G.A.c, dg, G.N);
+    eng_upload(G.d_gval, va, G.nE);
+    if (w) { eng_upload(G.d_dsc, w, G.N); G.mg_w = G.d_dsc; } else G.mg_w = nullptr;
+    G.mg_qsign = qsign;
+    return G.nE;
+}
+// edge mobilities of a graph device (electrons then holes per undirected
+// edge): read (set = 0) or replace (set = 1); returns the edge count
+int sdx_test_graph_gmu(double *gmu, int set) {
+    if (!G.graph) return -1;
+    if (set) eng_upload(G.d_gmu, gmu, (size_t)2 * G.nUE);
+    else eng_download(gmu, G.d_gmu, (size_t)2 * G.nUE);
+    return G.nUE;
+}
+// levels of the graph multigrid (0: not a graph device)
+int sdx_test_graph_levels(void) { return G.graph ? gr_nlev() : 0; }
 void sdx_test_load_op(const double *A7, int qsign, const double *w) {
     double *a[7] = {G.A.c, G.A.xm, G.A.xp, G.A.ym, G.A.yp, G.A.zm, G.A.zp};
     for (int q = 0; q < 7; q++) eng_upload(a[q], A7 + (size_t)q * G.N, G.N);

@@ -8,6 +8,7 @@
 // come back to the host to steer the iteration.
 #pragma once
 #include "sd_physics.h"
+#include "sd_graph.h"
 #include "sd_host.h"
 #include <vector>
 
@@ -156,6 +157,19 @@ struct Engine {
     int aa_m = 12, aa_n = 0, aa_pos = 0, aa_have = 0;
     double aa_gram[12][12];
 
+    // graph device (api3_set_mesh_graph): node arrays as above with Nx = N,
+    // Ny = Nz = 1; the CSR graph of directed edges, its reverse and undirected
+    // edge ids, edge lengths and dual areas, control volumes, positions and the
+    // walls the host supplied (api3_set_walls, 6 per node)
+    int graph = 0;
+    int nE = 0, nUE = 0;
+    std::vector<int> grp, gcol, grev, gue;
+    std::vector<double> gh, ga, gvol, gpos, gwall;
+    GraphDev gd{};
+    double *d_gval = nullptr;          // operator off-diagonals, one per directed edge (diagonal in A.c)
+    double *d_gmu = nullptr;           // edge mobilities: electrons [nUE], then holes [nUE]
+    int *d_gck = nullptr, *d_gce = nullptr, *d_gcoff = nullptr;   // terminal-current edges per contact
+
     // statistics of the kernels (see sdx_kernel_stats)
     long long n_vcycles = 0, n_spmv = 0;
     // floating majority blocks: continuity solves that listed unanchored
@@ -168,7 +182,34 @@ extern volatile int g_stop;
 extern volatile double g_prog[8];
 
 // engine API used by api.cpp
-int eng_alloc(int Nx, int Ny, int Nz);
+int eng_alloc(int Nx, int Ny, int Nz, int graph = 0);
+// device memory released with the device (eng_free) or by the next prepare
+void *eng_dalloc_bytes(size_t bytes);
+void *eng_palloc_bytes(size_t bytes);
+void eng_set_error();
+// graph path (graph.cpp)
+int eng_set_graph(int N, int E, const int *rp, const int *cj, const double *h, const double *ar, const double *vol,
+                  const double *pos);
+void gr_free();
+int gr_nlev();
+int gr_prepare_edges();
+void gr_walls(std::vector<double> &qd, std::vector<double> &qcn, std::vector<double> &qcp, int qc,
+              double (*factor)(const HostMat &, double, double, int));
+void gr_mg_setup();
+void gr_precond(const double *r, double *z);
+void gr_spmv(const double *x, const double *b, double *y, bool unit);
+void gr_spmv_dots(const double *x, double *y, const double *u0, const double *u1, int nv, int slot, int post,
+                  bool unit);
+void gr_floor(const double *x, const double *b, int slot);
+void gr_poisson(const double *phi, int build, double *F, int slot);
+void gr_upd_mob(const double *qn, const double *qp);
+void gr_cont(int hole);
+void gr_currents();
+void gr_display();
+void gr_node_J(int ax, int carrier, double *out);
+int gr_float_fix(int hole);
+void gr_region_shift(const double *dV);
+double gr_emax();
 void eng_free();
 int eng_prepare();
 int eng_equilibrium();
