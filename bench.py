@@ -13,7 +13,10 @@ config.timed_points name exactly which points were timed.  --mesh 96 64 64
 runs the 393,216-node mesh instead.  --workload vdmos runs the 55 V VDMOS
 cell with impact ionisation (256x128x256, 8,388,608 nodes; drain 0 -> 60 V in
 1 V steps at Vgs = 0) as a supplementary line: the headline metric is quoted
-on the MOSFET.
+on the MOSFET.  --workload prism runs the graph (prism-mesh) path: the planar
+n-MOSFET on a triangular-prism mesh (257 x 129 lattice x 16 layers, 531,000
+nodes; Id-Vg 0 -> 1.2 V in 0.1 V steps at Vd = 0.05 V), FP64 aggregation
+multigrid, a supplementary line whose roofline is the graph Krylov SpMV.
 
 Up to three solver configurations run back to back on the same sweep: the
 plain FP64 method (FP64 multigrid), the mixed-precision method (FP32
@@ -39,7 +42,7 @@ the same sweep spanning both drain biases: at each sample position it resumes
 from the GPU's converged state, solves one point untimed and is timed on the
 next.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--mesh NX NY NZ] [--workload mosfet|vdmos]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--mesh NX NY NZ] [--workload mosfet|vdmos|prism]
 """
 import argparse
 import ctypes
@@ -57,9 +60,23 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 SWEEP = {"mosfet": "Id-Vg sweep (0-1.2 V x 0.05 V at Vd 0.05 and 1.0 V)",
-         "vdmos": "drain sweep (0-60 V x 1 V at Vgs 0, impact ionisation)"}
-DEFAULT_MESH = {"mosfet": [192, 128, 128], "vdmos": [256, 128, 256]}
-DRAIN = {"mosfet": 1, "vdmos": 4}          # contact index of the drain
+         "vdmos": "drain sweep (0-60 V x 1 V at Vgs 0, impact ionisation)",
+         "prism": "Id-Vg sweep (0-1.2 V x 0.1 V at Vd 0.05 V) on a triangular-prism mesh"}
+DEFAULT_MESH = {"mosfet": [192, 128, 128], "vdmos": [256, 128, 256], "prism": [257, 129, 16]}
+DRAIN = {"mosfet": 1, "vdmos": 4, "prism": 1, "prism_mosfet": 1}   # contact index of the drain
+PRISM_NPTS = 13
+
+
+def make_spec(workload, mesh):
+    from semiconductor_devices_simulator_amd import devices, prism
+    if workload == "prism":
+        return prism.prism_mosfet(*mesh, npts=PRISM_NPTS)
+    return devices.mosfet(*mesh) if workload == "mosfet" else devices.vdmos(*mesh)
+
+
+def load_spec(lib, spec):
+    from semiconductor_devices_simulator_amd import devices, prism
+    return (prism.load_graph_device if isinstance(spec, prism.GraphSpec) else devices.load_device)(lib, spec)
 
 
 def sweep_points(spec):
@@ -96,14 +113,15 @@ def timed_indices(npts, warmup, steps):
     return [int(i) for i in idx]
 
 
-def run_gpu(lib, spec, precision, timed, ktimer, keep=(), ws=0):
+def run_gpu(lib, spec, precision, timed, ktimer, keep=(), ws=0, kt_kind=0):
     """Solve the sweep up to the last timed point, timing the points listed in
     `timed`; returns (seconds, currents, current floors, kernel timer,
     iteration counts of the timed points, {index: (psi, phi_n, phi_p)} of the
-    converged states listed in keep, read outside the timed calls)."""
-    from semiconductor_devices_simulator_amd import devices
+    converged states listed in keep, read outside the timed calls).  The
+    kernel timer samples kind kt_kind (0 level-0 smoother sweep, 1 Krylov
+    SpMV)."""
     from semiconductor_devices_simulator_amd.core import dptr
-    cids = devices.load_device(lib, spec)
+    cids = load_spec(lib, spec)
     assert lib.sdx_set_precision(precision) == 1
     lib.sdx_ws_mode(0)
     t0 = time.perf_counter()
@@ -117,7 +135,7 @@ def run_gpu(lib, spec, precision, timed, ktimer, keep=(), ws=0):
     its = dict(pcg=0, bicgstab=0, newton=0, gummel=0)
     dt, I, F, states = 0.0, [], [], {}
     if ktimer:
-        lib.sdx_ktimer_enable(0, 8)
+        lib.sdx_ktimer_enable(kt_kind, 8)
     for i in range(0, max(timed) + 1):
         on = i in tset
         if ktimer:
@@ -200,7 +218,7 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(states, mesh):
+def cpu_baseline(states, mesh, workload="mosfet"):
     ref = os.path.join(ROOT, "oracle", "_ref", "libsemisim_ref.so")
     if not os.path.isfile(ref) or not states:
         return None
@@ -210,7 +228,7 @@ def cpu_baseline(states, mesh):
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
     with tempfile.TemporaryDirectory() as td:
         path = os.path.join(td, "states.npz")
-        np.savez(path, idx=np.array(sorted(states)), mesh=np.array(mesh),
+        np.savez(path, idx=np.array(sorted(states)), mesh=np.array(mesh), workload=np.array(workload),
                  **{f"s{i}": np.stack(states[i]) for i in states})
         r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker", path],
                            env=env, capture_output=True, text=True, timeout=900)
@@ -227,13 +245,13 @@ def cpu_baseline(states, mesh):
 
 
 def cpu_baseline_worker(path):
-    from semiconductor_devices_simulator_amd import core, devices
+    from semiconductor_devices_simulator_amd import core
     from semiconductor_devices_simulator_amd.core import dptr
     lib = core.bind(ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libsemisim_ref.so")))
     data = np.load(path)
     mesh = [int(v) for v in data["mesh"]]
-    spec = devices.mosfet(*mesh)
-    cids = devices.load_device(lib, spec)
+    spec = make_spec(str(data["workload"]), mesh)
+    cids = load_spec(lib, spec)
     pts = sweep_points(spec)
     dt, per, timed = 0.0, [], []
     for i in data["idx"]:
@@ -261,7 +279,7 @@ def describe_timed(pts, timed, workload):
     for i in timed:
         tgt, v, hold = pts[i]
         groups.setdefault(tuple(sorted(hold.items())), []).append(v)
-    name = {"mosfet": ("Vd", "Vg", {1: "Vd"}), "vdmos": ("Vgs", "Vd", {2: "Vgs"})}[workload]
+    name = {"mosfet": ("Vd", "Vg", {1: "Vd"}), "vdmos": ("Vgs", "Vd", {2: "Vgs"}), "prism": ("Vd", "Vg", {1: "Vd"})}[workload]
     parts = []
     for hold, vs in groups.items():
         h = dict(hold)
@@ -291,7 +309,7 @@ def main():
     ap.add_argument("--steps", type=int, default=48)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mesh", type=int, nargs=3, default=None)
-    ap.add_argument("--workload", choices=["mosfet", "vdmos"], default="mosfet")
+    ap.add_argument("--workload", choices=["mosfet", "vdmos", "prism"], default="mosfet")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--sweep-points", type=int, default=0,
@@ -308,6 +326,8 @@ def main():
     from semiconductor_devices_simulator_amd import core, devices, warmstart
     lib = core.load_core()
     a.mesh = a.mesh or DEFAULT_MESH[a.workload]
+    if a.workload == "prism":
+        return main_prism(a, lib)
     if a.workload == "vdmos" and a.mesh == DEFAULT_MESH["vdmos"] and "--steps" not in sys.argv:
         a.steps = 43          # the blocking sweep to 44 V: this mesh breaks down at 46 V
     spec = devices.mosfet(*a.mesh) if a.workload == "mosfet" else devices.vdmos(*a.mesh)
@@ -393,6 +413,50 @@ def main():
                    "fp32_mg_vs_fp64": mixed_vs_fp64(I32, I64, F64, st32.get(last), states.get(last), D),
                    "learned_fp32_mg_vs_fp64": (mixed_vs_fp64(learned["I"], I64, F64, learned["last"], states.get(last),
                                                              D) if learned else None)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def main_prism(a, lib):
+    """Supplementary line: the graph (prism-mesh) path, FP64 aggregation
+    multigrid, with the graph Krylov SpMV as the roofline kernel and the
+    reference core's graph path on the host cores beside it."""
+    spec = make_spec("prism", a.mesh)
+    pts = sweep_points(spec)
+    npts = len(pts)
+    W = max(0, min(a.warmup, npts - 1))
+    K = max(1, min(a.steps, npts - W))
+    timed = timed_indices(npts, W, K)
+    K = len(timed)
+    workload = "prism_mosfet_%dx%dx%d %s; %d nodes, %d directed edges; timed %s" % (
+        *a.mesh, SWEEP["prism"], spec.N, spec.E, describe_timed(pts, timed, "prism"))
+    print(f"[bench] {workload}; points {timed}", file=sys.stderr, flush=True)
+    windows = [i for i in (3, 8) if i + 2 < npts and i <= timed[-1]]
+    t, I, F, kt, its, states = run_gpu(lib, spec, 0, timed, ktimer=True, keep=windows, kt_kind=1)
+    print(f"[bench] graph path: {K / t:.3f} points/s", file=sys.stderr, flush=True)
+    roof = None
+    if kt[0] > 0:
+        avg_s = kt[1] / kt[0] * 1e-3
+        ach = kt[2] / avg_s / 1e9
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="k_gspmv_dots (graph CSR Krylov product fused with its dots)",
+                    bytes_per_launch=kt[2], avg_launch_us=round(avg_s * 1e6, 2), launches_timed=int(kt[0]))
+    cpu = None if a.no_cpu_baseline else cpu_baseline({i: states[i] for i in windows if i in states}, a.mesh,
+                                                      "prism")
+    out = {
+        "metric": "bias points/s at 1e-8 rel current err, graph (prism-mesh) path on GPU",
+        "value": round(K / t, 4), "unit": "bias points/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(1e3 * t / K, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": workload, "nodes": spec.N, "edges": spec.E, "mesh": list(a.mesh),
+                   "timed_points": [[round(pts[i][2].get(1, 0.0), 4), round(pts[i][1], 4)] for i in timed],
+                   "timed_point_fields": ["Vd", "Vg"], "method": "FP64 aggregation multigrid (graph)"},
+        "iterations": its,
+        "parity": parity(I, timed, a.mesh, "prism_mosfet"),
         "roofline": roof,
         "cpu_baseline": cpu,
     }

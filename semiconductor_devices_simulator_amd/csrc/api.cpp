@@ -464,7 +464,8 @@ void api3_bulk_copy(int which, double *out) {
     case 19: for (int k = 0; k < N; k++) out[k] = G.bgn[k] * mt(k).VT * 2.; return;
     case 20: case 21: case 22: case 23: case 24: case 25: {
         int ax = (which - 20) % 3, hole = which >= 23;
-        ph_node_Jc(G.st, d, ax, hole, G.d_wk[0]);
+        if (G.graph) gr_node_J(ax, 1 + hole, G.d_wk[0]);
+        else ph_node_Jc(G.st, d, ax, hole, G.d_wk[0]);
         eng_download(out, G.d_wk[0], N);
         return;
     }
@@ -481,8 +482,9 @@ void api3_bulk_copy(int which, double *out) {
         return;
     }
     case 30: {
-        std::vector<double> qd;
-        eng_wall_distance(qd);
+        std::vector<double> qd, qcn, qcp;
+        if (G.graph) gr_walls(qd, qcn, qcp, 0, nullptr);
+        else eng_wall_distance(qd);
         for (int k = 0; k < N; k++) out[k] = G.kind[k] == K_SEMI ? qd[k] : NAN;
         return;
     }
@@ -524,6 +526,22 @@ void api3_carrier_sums(double *out) {
 // the same bits on every run).
 static std::vector<int> g_qown;
 static int g_qct[SD_NC_MAX];
+// neighbours b of node k with the edge's length h and dual face area ar, on
+// either mesh (a graph device: its CSR row, in stored order; qnbrs :5172-5187)
+extern "C++" {
+template <class F> static void for_nbrs(int k, F body) {
+    if (G.graph) {
+        for (int e = G.grp[k]; e < G.grp[k + 1]; e++) body(G.gcol[e], G.gh[e], G.ga[e]);
+        return;
+    }
+    int i, j, l;
+    host_unidx(k, i, j, l);
+    for (int dir = 0; dir < 6; dir++) {
+        int b; double h, ar;
+        if (host_nbr(i, j, l, k, dir, b, h, ar)) body(b, h, ar);
+    }
+}
+}
 int api3_charge_partition(const int *cterm) {
     if (!G.allocated || !G.have_solution || !cterm) return -1;
     const int N = G.N;
@@ -544,32 +562,25 @@ int api3_charge_partition(const int *cterm) {
     }
     for (size_t qh = 0; qh < qu.size(); qh++) {
         const int k = qu[qh], t = g_qown[k];
-        int i, j, l;
-        host_unidx(k, i, j, l);
-        for (int dir = 0; dir < 6; dir++) {
-            int b; double h, ar;
-            if (!host_nbr(i, j, l, k, dir, b, h, ar) || G.con_mask[b]) continue;
+        for_nbrs(k, [&](int b, double, double) {
+            if (G.con_mask[b]) return;
             const bool ok = G.kind[k] == K_METAL ? G.kind[b] == K_METAL
                           : (G.kind[k] == K_SEMI && G.kind[b] == K_SEMI && qtype(b) == qtype(k));
-            if (!ok) continue;
+            if (!ok) return;
             if (g_qown[b] < 0) { g_qown[b] = t; qu.push_back(b); }
             else if (g_qown[b] != t) meet++;
-        }
+        });
     }
     // second pass: semiconductor no ohmic front reached joins its neighbour's terminal
     qu.clear();
     for (int k = 0; k < N; k++) if (g_qown[k] >= 0 && G.kind[k] == K_SEMI) qu.push_back(k);
     for (size_t qh = 0; qh < qu.size(); qh++) {
         const int k = qu[qh], t = g_qown[k];
-        int i, j, l;
-        host_unidx(k, i, j, l);
-        for (int dir = 0; dir < 6; dir++) {
-            int b; double h, ar;
-            if (!host_nbr(i, j, l, k, dir, b, h, ar)) continue;
-            if (G.con_mask[b] || G.kind[b] != K_SEMI || g_qown[b] >= 0) continue;
+        for_nbrs(k, [&](int b, double, double) {
+            if (G.con_mask[b] || G.kind[b] != K_SEMI || g_qown[b] >= 0) return;
             g_qown[b] = t;
             qu.push_back(b);
-        }
+        });
     }
     return meet;
 }
@@ -581,18 +592,14 @@ int api3_terminal_charges(int nt, double *Qt) {
     for (int k = 0; k < N; k++) {
         const int t = g_qown[k];
         if (t < 0 || t >= nt) continue;
-        int i, j, l;
-        host_unidx(k, i, j, l);
-        for (int dir = 0; dir < 6; dir++) {
-            int b; double h, ar;
-            if (!host_nbr(i, j, l, k, dir, b, h, ar)) continue;
+        for_nbrs(k, [&](int b, double h, double ar) {
             const int tb = g_qown[b];
-            if (tb == t) continue;
+            if (tb == t) return;
             const double ek = G.eps[k], eb = G.eps[b];
             const double f = (2. * ek * eb / (ek + eb)) * ar / h * (phi[k] - phi[b]);
             Qt[t] += f;
             if (tb < 0 || tb >= nt) Qt[nt] += f;
-        }
+        });
     }
     for (int k = 0; k < N; k++) {                 // face gates: D through the oxide
         if (!(G.robA[k] > 0.) || !G.gate_mask[k]) continue;
@@ -612,8 +619,9 @@ int api3_charge_owner(int *out) {
     memcpy(out, g_qown.data(), (size_t)G.N * sizeof(int));
     return 1;
 }
+// tensor mesh only: 0 on a graph device (as the reference, :5287)
 int api3_face_J(int ax, int carrier, double *out) {
-    if (!G.allocated || !G.have_solution || !out || ax < 0 || ax > 2) return 0;
+    if (!G.allocated || !G.have_solution || !out || ax < 0 || ax > 2 || G.graph) return 0;
     ph_face_J(G.st, eng_view(), ax, carrier, G.d_wk[0]);
     eng_download(out, G.d_wk[0], G.N);
     return 1;
@@ -624,7 +632,16 @@ double api3_ins_field_max(double *pos) {
     std::vector<double> phi = dl(G.d_psi);
     double best = 0.;
     long kbest = -1;
-    for (int k = 0; k < G.N; k++) {
+    for (int k = 0; k < G.N && G.graph; k++) {      // graph: every edge between two non-semiconductor nodes
+        if (G.kind[k] == K_SEMI) continue;
+        for_nbrs(k, [&](int b, double h, double) {
+            if (b < k || G.kind[b] == K_SEMI) return;
+            if (G.kind[k] != K_INS && G.kind[b] != K_INS) return;
+            const double F = fabs(phi[b] - phi[k]) / h;
+            if (F > best) { best = F; kbest = G.kind[k] == K_INS ? k : b; }
+        });
+    }
+    for (int k = 0; k < G.N && !G.graph; k++) {
         if (G.kind[k] == K_SEMI) continue;
         int l = k / (G.Nx * G.Ny), r = k - l * G.Nx * G.Ny, j = r / G.Nx, i = r - j * G.Nx;
         int st[3] = {1, G.Nx, G.Nx * G.Ny}, idx[3] = {i, j, l}, nn[3] = {G.Nx, G.Ny, G.Nz};
@@ -649,7 +666,9 @@ double api3_ins_field_max(double *pos) {
         double F = fabs(phi[k] - (cn.V + G.Cref - pm)) / tox;
         if (F > best) { best = F; kbest = k; }
     }
-    if (pos && kbest >= 0) {
+    if (pos && kbest >= 0 && G.graph) {
+        for (int q = 0; q < 3; q++) pos[q] = G.gpos[3 * kbest + q];
+    } else if (pos && kbest >= 0) {
         int l = kbest / (G.Nx * G.Ny), r = kbest - l * G.Nx * G.Ny, j = r / G.Nx, i = r - j * G.Nx;
         pos[0] = G.xs[i]; pos[1] = G.ys[j]; pos[2] = G.zs[l];
     }

@@ -2428,6 +2428,10 @@ int sdx_ktimer_read(double *out) {
         bytes = 0.5 * G.N * (h16 ? 3.0 * w + 7.0 * 2.0 : 10.0 * w);
     } else if (KT.kind == 2) {
         bytes = 0.;                              // a whole V-cycle: time only
+    } else if (G.graph) {
+        // CSR product fused with one dot: row pointers, per directed edge its
+        // column and value, per node the diagonal, x, y and the dot operand
+        bytes = 4.0 * (G.N + 1) + 12.0 * G.nE + 32.0 * G.N;
     } else {
         bytes = (double)G.N * 11.0 * 8.0;   // 7 diagonals + x + dot vector(s) + y (FP64)
     }

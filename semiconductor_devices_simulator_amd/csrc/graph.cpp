@@ -25,11 +25,14 @@
 
 static const int GMG_COARSE = 160;      // coarsest level: at most this many unknowns (explicit inverse)
 static const int GMG_DENSE_MAX = 320;   // an aggregation that stalls above this keeps Gauss-Seidel sweeps
+static const int GMG_TAIL_N = 4096;     // levels of at most this many unknowns run in one workgroup (gla_tail)
+static int GMG_JACOBI = 1;              // levels 1.. above the tail smoothed by two l1-Jacobi sweeps (0: Gauss-Seidel)
 static int GMG_PASSES = 3;              // matching passes per level (aggregates of up to 2^passes nodes)
 static double GMG_BETA = 0.5;           // a pair joins at >= beta of the node's strongest coupling
 
 struct GLevHost {
     int N = 0, nnz = 0;
+    int lanes = 8;                      // lanes per row of the smoother and residual kernels
     std::vector<int> rp, cj;            // pattern (level 0: the mesh graph)
     std::vector<double> wg;             // geometric coupling per entry (aggregation only)
     std::vector<int> agg, mrp, mem, emap;
@@ -39,8 +42,11 @@ struct GLevHost {
 static struct {
     std::vector<GLevHost> lev;
     double *inv = nullptr, *work = nullptr;   // coarsest inverse and its elimination workspace
+    double *rmem = nullptr;                   // residual of the finest level in member order (restriction)
     bool dense = false;
     bool pw_one = false;
+    int tail = -1;                            // first level run by the single-workgroup tail (-1: none)
+    GTail T{};
 } GM;
 
 static void gfail(const char *what, hipError_t e) {
@@ -59,7 +65,7 @@ template <class T> static T *galloc(size_t n) { return (T *)eng_dalloc_bytes(std
 // one matching pass: every free node joins its strongest free neighbour whose
 // coupling is at least beta of the node's strongest; returns the group count
 static int pair_pass(int n, const std::vector<int> &rp, const std::vector<int> &cj, const std::vector<double> &w,
-                     std::vector<int> &grp) {
+                     std::vector<int> &grp, double beta) {
     grp.assign(n, -1);
     int ng = 0;
     for (int i = 0; i < n; i++) {
@@ -70,7 +76,7 @@ static int pair_pass(int n, const std::vector<int> &rp, const std::vector<int> &
         double bw = -1.;
         for (int e = rp[i]; e < rp[i + 1]; e++) {
             const int j = cj[e];
-            if (j == i || grp[j] >= 0 || !(w[e] > 0.) || w[e] < GMG_BETA * mx) continue;
+            if (j == i || grp[j] >= 0 || !(w[e] > 0.) || w[e] < beta * mx) continue;
             if (w[e] > bw) { bw = w[e]; best = j; }
         }
         grp[i] = ng;
@@ -105,7 +111,7 @@ static void contract(int n, const std::vector<int> &rp, const std::vector<int> &
 }
 // aggregates of level f: GMG_PASSES matching passes on successively
 // contracted graphs; returns their number
-static int aggregate(const GLevHost &f, std::vector<int> &agg) {
+static int aggregate(const GLevHost &f, std::vector<int> &agg, double beta) {
     const int N = f.N;
     agg.resize(N);
     for (int i = 0; i < N; i++) agg[i] = i;
@@ -113,7 +119,7 @@ static int aggregate(const GLevHost &f, std::vector<int> &agg) {
     std::vector<double> w = f.wg;
     int n = N, na = N;
     for (int pass = 0; pass < GMG_PASSES && n > 1; pass++) {
-        na = pair_pass(n, rp, cj, w, grp);
+        na = pair_pass(n, rp, cj, w, grp, beta);
         for (int i = 0; i < N; i++) agg[i] = grp[agg[i]];
         if (pass == GMG_PASSES - 1 || na == n) break;
         std::vector<int> nrp, ncj;
@@ -181,8 +187,10 @@ int gr_nlev() { return (int)GM.lev.size(); }
 // the device arrays are the engine's (released by eng_free); this drops the host side
 void gr_free() {
     GM.lev.clear();
-    GM.inv = GM.work = nullptr;
+    GM.inv = GM.work = GM.rmem = nullptr;
     GM.dense = GM.pw_one = false;
+    GM.tail = -1;
+    GM.T = GTail{};
 }
 
 static void gmg_build() {
@@ -190,6 +198,7 @@ static void gmg_build() {
     GM.inv = GM.work = nullptr;
     GM.pw_one = false;
     if (const char *e = getenv("SEMISIM_GPASS")) GMG_PASSES = std::max(1, atoi(e));
+    if (const char *e = getenv("SEMISIM_GJACOBI")) GMG_JACOBI = atoi(e) != 0;
     if (const char *e = getenv("SEMISIM_GBETA")) GMG_BETA = atof(e);
     GM.lev.emplace_back();
     GLevHost &f0 = GM.lev[0];
@@ -197,11 +206,16 @@ static void gmg_build() {
     f0.rp = G.grp; f0.cj = G.gcol;
     f0.wg.resize(std::max(1, G.nE));
     for (int e = 0; e < G.nE; e++) f0.wg[e] = G.gh[e] > 0. ? G.ga[e] / G.gh[e] : 0.;
-    while ((int)GM.lev.size() < 24) {
+    while ((int)GM.lev.size() < GTAIL_MAXL) {
         GLevHost &f = GM.lev.back();
         if (f.N <= GMG_COARSE) break;
         std::vector<int> agg;
-        const int na = aggregate(f, agg);
+        // where strongly anisotropic couplings leave most nodes without a
+        // partner above beta of their strongest one, the level is matched
+        // again with any free neighbour, so the hierarchy reaches a coarsest
+        // level small enough for its explicit inverse
+        int na = aggregate(f, agg, GMG_BETA);
+        if (na < 1 || (double)na > 0.7 * (double)f.N) na = aggregate(f, agg, 0.0);
         if (na < 1 || (double)na > 0.7 * (double)f.N) break;
         f.agg = agg;
         GLevHost c;
@@ -209,7 +223,10 @@ static void gmg_build() {
         coarse_pattern(f, c);
         GM.lev.push_back(std::move(c));
     }
-    for (auto &v : GM.lev) colour(v);
+    for (auto &v : GM.lev) {
+        colour(v);
+        v.lanes = gla_lanes(v.N, v.nnz);
+    }
     const int nl = (int)GM.lev.size();
     // device arrays: level 0 works on the engine's operator (diagonal A.c,
     // values d_gval) and on the Krylov vectors
@@ -226,6 +243,8 @@ static void gmg_build() {
             d.ph = galloc<double>(v.N);
         }
         d.perm = gput(v.perm);
+        d.coff = gput(v.coff);
+        d.ncol = (int)v.coff.size() - 1;
         if (L < nl - 1) {
             d.agg = gput(v.agg); d.mrp = gput(v.mrp); d.mem = gput(v.mem); d.emap = gput(v.emap);
             d.pw = galloc<double>(v.N);
@@ -237,10 +256,19 @@ static void gmg_build() {
         GM.inv = galloc<double>((size_t)nz * nz);
         GM.work = galloc<double>((size_t)2 * nz * nz);
     }
+    GM.rmem = galloc<double>(GM.lev[0].N);
+    GM.tail = -1;
+    for (int L = 1; L < nl && GM.tail < 0; L++) if (GM.lev[L].N <= GMG_TAIL_N) GM.tail = L;
+    GM.T = GTail{};
+    for (int L = 0; L < nl; L++) GM.T.lev[L] = GM.lev[L].dev;
+    GM.T.nl = nl;
+    GM.T.dense = GM.dense;
+    GM.T.inv = GM.inv;
     if (G.verbose) {
         fprintf(stderr, "semisim-hip: graph multigrid %d levels:", nl);
         for (auto &v : GM.lev) fprintf(stderr, " %d (%d colours)", v.N, (int)v.coff.size() - 1);
-        fprintf(stderr, "%s\n", GM.dense ? "" : "; coarsest by Gauss-Seidel sweeps");
+        fprintf(stderr, "%s; one-workgroup tail from level %d\n", GM.dense ? "" : "; coarsest by Gauss-Seidel sweeps",
+                GM.tail);
     }
 }
 
@@ -389,13 +417,14 @@ static void sweep(const GLevHost &v, const GOp &A, const double *b, double *x, b
     const int nc = (int)v.coff.size() - 1;
     for (int q = 0; q < nc; q++) {
         const int c = forward ? q : nc - 1 - q;
-        gla_gs(G.st, A, b, x, v.dev.perm, v.coff[c], v.coff[c + 1]);
+        gla_gs(G.st, A, b, x, v.dev.perm, v.coff[c], v.coff[c + 1], v.lanes);
     }
 }
 static void gvcycle(int L, const double *b, double *x) {
     GLevHost &v = GM.lev[L];
     const int nl = (int)GM.lev.size();
     hipStream_t s = G.st;
+    if (L == GM.tail) { gla_tail(s, GM.T, L, b, x); return; }
     const GOp A = L == 0 ? level0_op() : v.dev.A;
     if (L == nl - 1) {
         if (GM.dense) { gla_dense_apply(s, v.N, GM.inv, b, x); return; }
@@ -404,14 +433,26 @@ static void gvcycle(int L, const double *b, double *x) {
         return;
     }
     GLevHost &c = GM.lev[L + 1];
-    sd_fill(s, x, 0.0, v.N);
-    sweep(v, A, b, x, true);
+    // coarse levels: two l1-Jacobi sweeps (one launch each, through the
+    // member-order scratch) instead of one launch per colour
+    const bool jac = GMG_JACOBI && L > 0;
+    if (jac) {
+        gla_jacobi(s, A, b, nullptr, GM.rmem, v.lanes);
+        gla_jacobi(s, A, b, GM.rmem, x, v.lanes);
+    } else {
+        sd_fill(s, x, 0.0, v.N);
+        sweep(v, A, b, x, true);
+    }
     GLev f = v.dev;
     f.A = A;
-    gla_resid_restrict(s, c.N, f, b, x, L == 0 ? G.mg_w : nullptr, c.dev.b);
+    // levels above the tail: residual one lane per row, then per-aggregate sums
+    gla_resid_restrict(s, c.N, f, b, x, L == 0 ? G.mg_w : nullptr, c.dev.b, GM.rmem, v.lanes);
     gvcycle(L + 1, c.dev.b, c.dev.x);
     gla_prolong(s, v.N, f, c.dev.x, x, 1.0);
-    sweep(v, A, b, x, false);
+    if (jac) {
+        gla_jacobi(s, A, b, x, GM.rmem, v.lanes);
+        gla_jacobi(s, A, b, GM.rmem, x, v.lanes);
+    } else sweep(v, A, b, x, false);
 }
 void gr_precond(const double *r, double *z) {
     G.n_vcycles++;
@@ -435,7 +476,9 @@ void gr_spmv_dots(const double *x, double *y, const double *u0, const double *u1
                   bool unit) {
     GOp A = level0_op();
     if (unit) A.dg = nullptr;
-    gla_spmv_dots(G.st, A, x, y, u0, u1, nv, G.rb, slot, post);
+    // one lane per row: the rows of the Krylov product are consecutive, and
+    // measured faster so than by lane groups (28 against 31 us at 531k nodes)
+    gla_spmv_dots(G.st, A, x, y, u0, u1, nv, G.rb, slot, post, 1);
 }
 void gr_floor(const double *x, const double *b, int slot) { gla_floor(G.st, level0_op(), x, b, G.rb, slot); }
 

@@ -307,6 +307,21 @@ void gph_node_J(hipStream_t s, const DevView &d, const GraphDev &g, const double
 }
 
 // ---------------------------------------------------------- linear algebra
+// Rows walked by LPR lanes together: lane j of the row's group takes entries
+// j, j + LPR, ..., so that the group's loads of a row's (contiguous) columns
+// and values are coalesced; a shuffle sum closes the row.  LPR is chosen per
+// level near its mean row length (gla_lanes).
+template <int LPR> __device__ __forceinline__ double grow_part(const GOp &A, const double *x, int k, int j) {
+    double s = 0.;
+    for (int e = A.rp[k] + j; e < A.rp[k + 1]; e += LPR) s += A.va[e] * x[A.cj[e]];
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
+    return s;
+}
+int gla_lanes(int n, int nnz) {
+    const double m = n > 0 ? (double)nnz / n : 1.;
+    return m <= 4.5 ? 4 : (m <= 9. ? 8 : (m <= 18. ? 16 : 32));
+}
 __device__ __forceinline__ double grow(const GOp &A, const double *x, int k) {
     double s = A.dg ? A.dg[k] * x[k] : x[k];
     for (int e = A.rp[k]; e < A.rp[k + 1]; e++) s += A.va[e] * x[A.cj[e]];
@@ -333,10 +348,39 @@ __global__ void __launch_bounds__(SD_BLOCK) k_gspmv_dots(GOp A, const double *x,
     }
     red_partial<2, false>(acc, rb);
 }
+template <int LPR>
+__global__ void __launch_bounds__(SD_BLOCK) k_gspmv_dots_v(GOp A, const double *x, double *y, const double *u0,
+                                                           const double *u1, int nv, RedBuf rb) {
+    double acc[2] = {0.0, 0.0};
+    const int j = threadIdx.x % LPR, rows = SD_BLOCK / LPR;
+    for (int k0 = blockIdx.x * rows; k0 < A.N; k0 += gridDim.x * rows) {
+        const int k = k0 + (int)threadIdx.x / LPR;
+        if (k >= A.N) continue;                         // whole row groups leave together
+        double s = grow_part<LPR>(A, x, k, j);
+        if (j == 0) {
+            s += A.dg ? A.dg[k] * x[k] : x[k];
+            y[k] = s;
+            acc[0] += s * (u0 ? u0[k] : s);
+            if (nv > 1) acc[1] += s * u1[k];
+        }
+    }
+    red_partial<2, false>(acc, rb);
+}
 void gla_spmv_dots(hipStream_t s, GOp A, const double *x, double *y, const double *u0, const double *u1, int nv,
-                   RedBuf rb, int slot, int post) {
-    const int nb = red_grid(A.N);
-    k_gspmv_dots<<<nb, SD_BLOCK, 0, s>>>(A, x, y, u0, u1, nv, rb);
+                   RedBuf rb, int slot, int post, int lanes) {
+    if (lanes <= 1) {
+        const int nb = red_grid(A.N);
+        k_gspmv_dots<<<nb, SD_BLOCK, 0, s>>>(A, x, y, u0, u1, nv, rb);
+        red_final(s, rb, nv, nb, slot, 0, post);
+        return;
+    }
+    const int nb = red_grid((int)(((long)A.N * lanes + SD_BLOCK - 1) / SD_BLOCK * SD_BLOCK));
+    switch (lanes) {
+    case 4: k_gspmv_dots_v<4><<<nb, SD_BLOCK, 0, s>>>(A, x, y, u0, u1, nv, rb); break;
+    case 8: k_gspmv_dots_v<8><<<nb, SD_BLOCK, 0, s>>>(A, x, y, u0, u1, nv, rb); break;
+    case 16: k_gspmv_dots_v<16><<<nb, SD_BLOCK, 0, s>>>(A, x, y, u0, u1, nv, rb); break;
+    default: k_gspmv_dots_v<32><<<nb, SD_BLOCK, 0, s>>>(A, x, y, u0, u1, nv, rb); break;
+    }
     red_final(s, rb, nv, nb, slot, 0, post);
 }
 // rounding floor of a continuity solve (gfloor :2556-2566)
@@ -423,8 +467,23 @@ __global__ void k_ggs(GOp A, const double *b, double *x, const int *perm, int n0
     for (int e = A.rp[k]; e < A.rp[k + 1]; e++) s -= A.va[e] * x[A.cj[e]];
     x[k] = A.dg ? s / A.dg[k] : s;
 }
-void gla_gs(hipStream_t s, GOp A, const double *b, double *x, const int *perm, int n0, int n1) {
-    if (n1 > n0) k_ggs<<<sd_nblocks(n1 - n0), SD_BLOCK, 0, s>>>(A, b, x, perm, n0, n1);
+template <int LPR> __global__ void k_ggs_v(GOp A, const double *b, double *x, const int *perm, int n0, int n1) {
+    const int t = n0 + (blockIdx.x * SD_BLOCK + (int)threadIdx.x) / LPR, j = threadIdx.x % LPR;
+    if (t >= n1) return;                                // whole row groups leave together
+    const int k = perm[t];
+    const double s = grow_part<LPR>(A, x, k, j);
+    if (j == 0) x[k] = A.dg ? (b[k] - s) / A.dg[k] : b[k] - s;
+}
+void gla_gs(hipStream_t s, GOp A, const double *b, double *x, const int *perm, int n0, int n1, int lanes) {
+    if (n1 <= n0) return;
+    const int nb = sd_nblocks((long)(n1 - n0) * (lanes > 1 ? lanes : 1));
+    switch (lanes) {
+    case 4: k_ggs_v<4><<<nb, SD_BLOCK, 0, s>>>(A, b, x, perm, n0, n1); break;
+    case 8: k_ggs_v<8><<<nb, SD_BLOCK, 0, s>>>(A, b, x, perm, n0, n1); break;
+    case 16: k_ggs_v<16><<<nb, SD_BLOCK, 0, s>>>(A, b, x, perm, n0, n1); break;
+    case 32: k_ggs_v<32><<<nb, SD_BLOCK, 0, s>>>(A, b, x, perm, n0, n1); break;
+    default: k_ggs<<<nb, SD_BLOCK, 0, s>>>(A, b, x, perm, n0, n1);
+    }
 }
 // residual b - A x of the members of every aggregate, summed with the
 // restriction weights into the coarse right-hand side (g_restrict :2590-2598
@@ -440,9 +499,71 @@ __global__ void k_gresid_restrict(int nc, GLev f, const double *b, const double 
     }
     bc[I] = s;
 }
+// One l1-Jacobi sweep (groups of LPR lanes per row): xo = xi + (b - A xi) /
+// (|dg| + sum |off-diagonal|); xi == nullptr: from xi = 0.  The l1 diagonal
+// makes the sweep convergent for the symmetric positive definite Poisson
+// operator, and equal pre- and post-smoothing keeps the V-cycle symmetric.
+template <int LPR> __global__ void k_gjacobi(GOp A, const double *b, const double *xi, double *xo) {
+    const int k = (blockIdx.x * SD_BLOCK + (int)threadIdx.x) / LPR, j = threadIdx.x % LPR;
+    if (k >= A.N) return;
+    double s = 0., a = 0.;
+    for (int e = A.rp[k] + j; e < A.rp[k + 1]; e += LPR) {
+        const double v = A.va[e];
+        if (xi) s += v * xi[A.cj[e]];
+        a += fabs(v);
+    }
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) { s += __shfl_xor(s, o, LPR); a += __shfl_xor(a, o, LPR); }
+    if (j == 0) {
+        const double d = A.dg ? A.dg[k] : 1.0, x0 = xi ? xi[k] : 0.;
+        xo[k] = x0 + (b[k] - d * x0 - s) / (fabs(d) + a);
+    }
+}
+void gla_jacobi(hipStream_t s, GOp A, const double *b, const double *xi, double *xo, int lanes) {
+    const int nb = sd_nblocks((long)A.N * lanes);
+    switch (lanes) {
+    case 4: k_gjacobi<4><<<nb, SD_BLOCK, 0, s>>>(A, b, xi, xo); break;
+    case 8: k_gjacobi<8><<<nb, SD_BLOCK, 0, s>>>(A, b, xi, xo); break;
+    case 16: k_gjacobi<16><<<nb, SD_BLOCK, 0, s>>>(A, b, xi, xo); break;
+    default: k_gjacobi<32><<<nb, SD_BLOCK, 0, s>>>(A, b, xi, xo); break;
+    }
+}
+
+// residual of every node (one lane per row), weighted, stored in member order
+// of the aggregates: r[q] for q = position of node mem[q]
+template <int LPR>
+__global__ void k_gresid_mem(int n, GLev f, const double *b, const double *x, const double *w, double *r) {
+    const int q = (blockIdx.x * SD_BLOCK + (int)threadIdx.x) / LPR, j = threadIdx.x % LPR;
+    if (q >= n) return;
+    const int i = f.mem[q];
+    const double s = grow_part<LPR>(f.A, x, i, j);
+    if (j == 0) {
+        const double v = b[i] - (f.A.dg ? f.A.dg[i] * x[i] : x[i]) - s;
+        r[q] = w ? w[i] * v : v;
+    }
+}
+// the coarse right-hand side: each aggregate's members are contiguous in r
+__global__ void k_gsum_mem(int nc, GLev f, const double *r, double *bc) {
+    const int I = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (I >= nc) return;
+    double s = 0.;
+    for (int q = f.mrp[I]; q < f.mrp[I + 1]; q++) s += r[q];
+    bc[I] = s;
+}
 void gla_resid_restrict(hipStream_t s, int nc, const GLev &f, const double *b, const double *x, const double *w,
-                        double *bc) {
-    k_gresid_restrict<<<sd_nblocks(nc), SD_BLOCK, 0, s>>>(nc, f, b, x, w, bc);
+                        double *bc, double *r, int lanes) {
+    if (!r) {
+        k_gresid_restrict<<<sd_nblocks(nc), SD_BLOCK, 0, s>>>(nc, f, b, x, w, bc);
+        return;
+    }
+    const int n = f.A.N, nb = sd_nblocks((long)n * lanes);
+    switch (lanes) {
+    case 4: k_gresid_mem<4><<<nb, SD_BLOCK, 0, s>>>(n, f, b, x, w, r); break;
+    case 8: k_gresid_mem<8><<<nb, SD_BLOCK, 0, s>>>(n, f, b, x, w, r); break;
+    case 16: k_gresid_mem<16><<<nb, SD_BLOCK, 0, s>>>(n, f, b, x, w, r); break;
+    default: k_gresid_mem<32><<<nb, SD_BLOCK, 0, s>>>(n, f, b, x, w, r); break;
+    }
+    k_gsum_mem<<<sd_nblocks(nc), SD_BLOCK, 0, s>>>(nc, f, r, bc);
 }
 __global__ void k_gprolong(int n, GLev f, const double *xc, double *x, double alpha) {
     const int i = blockIdx.x * SD_BLOCK + threadIdx.x;
@@ -457,9 +578,12 @@ void gla_prolong(hipStream_t s, int n, const GLev &f, const double *xc, double *
 // major, in `work`) by one workgroup; barriers order the steps.  A pivot
 // below 1e-300 is replaced by it (the reference's dense LU does the same).
 #define GDENSE_T 1024
-__global__ void __launch_bounds__(GDENSE_T) k_gdense_inverse(GOp A, double *M, double *inv) {
+#define GDENSE_LDS_N 80            // [A | I] in LDS up to 80 unknowns (100 KB of the 160 KB of a CU)
+__global__ void __launch_bounds__(GDENSE_T) k_gdense_inverse(GOp A, double *Mg, double *inv) {
     const int n = A.N, w = 2 * n, t = threadIdx.x;
     __shared__ int piv;
+    __shared__ double Ml[2 * GDENSE_LDS_N * GDENSE_LDS_N];
+    double *M = n <= GDENSE_LDS_N ? Ml : Mg;
     for (int q = t; q < n * w; q += GDENSE_T) M[q] = 0.;
     __syncthreads();
     for (int r = t; r < n; r += GDENSE_T) {
@@ -520,4 +644,93 @@ __global__ void k_gdense_apply(int n, const double *inv, const double *b, double
 }
 void gla_dense_apply(hipStream_t s, int n, const double *inv, const double *b, double *x) {
     k_gdense_apply<<<sd_nblocks(n), SD_BLOCK, 0, s>>>(n, inv, b, x);
+}
+
+// The small levels of a V-cycle in one workgroup: per level, Gauss-Seidel
+// colour by colour from x = 0, the residual restricted to the next level;
+// the coarsest solved; then back up, prolongation and the backward sweep.
+// Barriers separate every dependent step; the levels' data stay in L2.
+#define GTAIL_T 1024
+#define GTAIL_LPR 16
+__global__ void __launch_bounds__(GTAIL_T) k_gtail(GTail T, int L0, const double *b0, double *x0) {
+    const int t = threadIdx.x;
+    auto B = [&](int L) -> const double * { return L == L0 ? b0 : T.lev[L].b; };
+    auto X = [&](int L) -> double * { return L == L0 ? x0 : T.lev[L].x; };
+    // rows by groups of GTAIL_LPR lanes (the coarse rows hold tens of entries)
+    const int j = t % GTAIL_LPR, grp = t / GTAIL_LPR;
+    constexpr int NG = GTAIL_T / GTAIL_LPR;
+    auto sweep = [&](int L, bool fwd) {
+        const GLev &v = T.lev[L];
+        const double *b = B(L);
+        double *x = X(L);
+        for (int q = 0; q < v.ncol; q++) {
+            const int c = fwd ? q : v.ncol - 1 - q;
+            for (int i0 = v.coff[c]; i0 < v.coff[c + 1]; i0 += NG) {
+                const int i = i0 + grp;
+                if (i < v.coff[c + 1]) {
+                    const int k = v.perm[i];
+                    const double sum = grow_part<GTAIL_LPR>(v.A, x, k, j);
+                    if (j == 0) x[k] = v.A.dg ? (b[k] - sum) / v.A.dg[k] : b[k] - sum;
+                }
+            }
+            __syncthreads();
+        }
+    };
+    const int last = T.nl - 1;
+    for (int L = L0; L < last; L++) {
+        const GLev &v = T.lev[L];
+        double *x = X(L);
+        for (int k = t; k < v.A.N; k += GTAIL_T) x[k] = 0.;
+        __syncthreads();
+        sweep(L, true);
+        const int nc = T.lev[L + 1].A.N;
+        const double *b = B(L);
+        // residuals of the members (groups of lanes per row) into the next
+        // level's x as scratch in member order, then the per-aggregate sums
+        double *rq = T.lev[L + 1].x;
+        for (int q0 = 0; q0 < v.A.N; q0 += NG) {
+            const int q = q0 + grp;
+            if (q < v.A.N) {
+                const int i = v.mem[q];
+                const double sum = grow_part<GTAIL_LPR>(v.A, x, i, j);
+                if (j == 0) rq[q] = b[i] - (v.A.dg ? v.A.dg[i] * x[i] : x[i]) - sum;
+            }
+        }
+        __syncthreads();
+        for (int I = t; I < nc; I += GTAIL_T) {
+            double sum = 0.;
+            for (int q = v.mrp[I]; q < v.mrp[I + 1]; q++) sum += rq[q];
+            T.lev[L + 1].b[I] = sum;
+        }
+        __syncthreads();
+    }
+    {
+        const GLev &v = T.lev[last];
+        const double *b = B(last);
+        double *x = X(last);
+        const int n = v.A.N;
+        if (T.dense) {
+            for (int r = t; r < n; r += GTAIL_T) {
+                double sum = 0.;
+                for (int c = 0; c < n; c++) sum += T.inv[(size_t)r * n + c] * b[c];
+                x[r] = sum;
+            }
+            __syncthreads();
+        } else {
+            for (int k = t; k < n; k += GTAIL_T) x[k] = 0.;
+            __syncthreads();
+            for (int it = 0; it < 4; it++) { sweep(last, true); sweep(last, false); }
+        }
+    }
+    for (int L = last - 1; L >= L0; L--) {
+        const GLev &v = T.lev[L];
+        double *x = X(L);
+        const double *xc = X(L + 1);
+        for (int i = t; i < v.A.N; i += GTAIL_T) x[i] += v.pw[i] * xc[v.agg[i]];
+        __syncthreads();
+        sweep(L, false);
+    }
+}
+void gla_tail(hipStream_t s, const GTail &T, int L0, const double *b0, double *x0) {
+    k_gtail<<<1, GTAIL_T, 0, s>>>(T, L0, b0, x0);
 }

@@ -40,7 +40,19 @@ struct GLev {
     double *pw;                    // prolongation weight of each node to its aggregate
     double *ph;                    // Slotboom reference potential / VT (coarse levels)
     const int *perm;
+    const int *coff;               // colour c: perm[coff[c] .. coff[c+1]) (device copy)
+    int ncol;
     double *x, *b;
+};
+
+// The small levels of the hierarchy (levels L0 .. nl-1), run by one
+// workgroup in one launch (gla_tail): the coarsest by its explicit inverse
+// (dense) or by Gauss-Seidel sweeps
+#define GTAIL_MAXL 16
+struct GTail {
+    GLev lev[GTAIL_MAXL];
+    int nl, dense;
+    const double *inv;
 };
 
 // physics (one launch each; the reductions are deterministic)
@@ -67,15 +79,24 @@ void gph_node_J(hipStream_t s, const DevView &d, const GraphDev &g, const double
 
 // linear algebra
 void gla_spmv(hipStream_t s, GOp A, const double *x, const double *b, double *y);
+// lanes: lanes per row (4, 8, 16, 32; 1: one lane per row)
 void gla_spmv_dots(hipStream_t s, GOp A, const double *x, double *y, const double *u0, const double *u1, int nv,
-                   RedBuf rb, int slot, int post);
+                   RedBuf rb, int slot, int post, int lanes = 1);
+// lanes per row suited to a level of n rows and nnz off-diagonal entries
+int gla_lanes(int n, int nnz);
 void gla_floor(hipStream_t s, GOp A, const double *x, const double *b, RedBuf rb, int slot);
 void gla_weights(hipStream_t s, const DevView &d, int L, int nc, const GLev &f, const GLev &c, int qsign, double VT);
 void gla_pw_one(hipStream_t s, int n, double *pw);
 void gla_galerkin(hipStream_t s, int nc, const GLev &f, const GLev &c, const double *w);
-void gla_gs(hipStream_t s, GOp A, const double *b, double *x, const int *perm, int n0, int n1);
+void gla_gs(hipStream_t s, GOp A, const double *b, double *x, const int *perm, int n0, int n1, int lanes = 1);
+// one l1-Jacobi sweep from xi (nullptr: 0) into xo
+void gla_jacobi(hipStream_t s, GOp A, const double *b, const double *xi, double *xo, int lanes);
+// r != nullptr: the residual is formed one lane per row into r (member order,
+// N values) and summed per aggregate by a second launch
 void gla_resid_restrict(hipStream_t s, int nc, const GLev &f, const double *b, const double *x, const double *w,
-                        double *bc);
+                        double *bc, double *r = nullptr, int lanes = 8);
 void gla_prolong(hipStream_t s, int n, const GLev &f, const double *xc, double *x, double alpha);
 void gla_dense_inverse(hipStream_t s, GOp A, double *work, double *inv);
 void gla_dense_apply(hipStream_t s, int n, const double *inv, const double *b, double *x);
+// V-cycle of levels L0 .. nl-1 from level L0's right-hand side b0 into x0
+void gla_tail(hipStream_t s, const GTail &T, int L0, const double *b0, double *x0);

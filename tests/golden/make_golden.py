@@ -360,9 +360,10 @@ def ivsweep_fixture(lib, spec, name, cid, Vs, Ve, npts, hold):
 def readback_fixture(lib, spec, name, Va):
     """The reference's readback at a solved bias point: the state, all 36
     api3_bulk_copy quantities and api3_face_J of both carriers' sums and
-    electrons along every axis."""
+    electrons along every axis (a graph device has none: the call returns 0)."""
     lib.api3_set_threads(8)
-    cids = devices.load_device(lib, spec)
+    graph = isinstance(spec, prism.GraphSpec)
+    cids = (prism.load_graph_device if graph else devices.load_device)(lib, spec)
     for c, v in enumerate(Va):
         lib.api3_set_contact_V(cids[c], v)
     assert lib.api3_solve() == 1
@@ -375,7 +376,7 @@ def readback_fixture(lib, spec, name, Va):
     fJ = np.zeros((3, 2, N))
     for ax in range(3):
         for car in range(2):
-            assert lib.api3_face_J(ax, car, core.dptr(fJ[ax, car])) == 1
+            assert lib.api3_face_J(ax, car, core.dptr(fJ[ax, car])) == (0 if graph else 1)
     nc = len(cids)
     con = np.array([[lib.api3_get_contact_I(cids[c]), lib.api3_get_contact_In(cids[c]), lib.api3_get_contact_Iraw(cids[c]),
                      lib.api3_get_contact_Ifloor(cids[c]), lib.api3_get_contact_A(cids[c])] for c in range(nc)])
@@ -396,7 +397,8 @@ def cv_fixture(lib, spec, name, Va, cterm, tgt, dV=5e-3):
     capacitance matrix by central differences of contact tgt (+-dV, one
     resolve each), as the front end's cap_matrix forms it (main.py:4034-4087)."""
     lib.api3_set_threads(8)
-    cids = devices.load_device(lib, spec)
+    graph = isinstance(spec, prism.GraphSpec)
+    cids = (prism.load_graph_device if graph else devices.load_device)(lib, spec)
     for c, v in enumerate(Va):
         lib.api3_set_contact_V(cids[c], v)
     assert lib.api3_solve() == 1
@@ -416,10 +418,12 @@ def cv_fixture(lib, spec, name, Va, cterm, tgt, dV=5e-3):
         Q = np.zeros(nt + 1)
         assert lib.api3_terminal_charges(nt, core.dptr(Q)) == 1
         Qs.append(Q)
+    extra = {} if graph else dict(xs=spec.xs, ys=spec.ys, zs=spec.zs, Nd=spec.Nd, Na=spec.Na, models=spec.models,
+                                  tau=np.asarray(spec.tau if spec.tau else (0.0, 0.0)),
+                                  contacts=json.dumps(contacts_of(spec)))
     np.savez_compressed(os.path.join(HERE, name + ".npz"), Va=np.asarray(Va, np.float64), cterm=ct, meet=meet,
                         owner=own, Q0=Q0, Qp=Qs[0], Qm=Qs[1], tgt=tgt, dV=dV, psi=st[0], phin=st[1], phip=st[2],
-                        xs=spec.xs, ys=spec.ys, zs=spec.zs, Nd=spec.Nd, Na=spec.Na, models=spec.models,
-                        tau=np.asarray(spec.tau if spec.tau else (0.0, 0.0)), contacts=json.dumps(contacts_of(spec)))
+                        **extra)
     print("wrote", name, "meet", meet, "Q0", Q0, "K col", (Qs[0] - Qs[1]) / (2 * dV))
 
 
@@ -644,6 +648,15 @@ def main():
         graded_grid_fixture("graded_grid")
         return
     lib = load_harness()
+    if "--graphbench" in sys.argv:     # the graph path's bench mesh (531,472 nodes; several minutes)
+        spec = prism.prism_mosfet(257, 129, 16, npts=13)
+        sweep_fixture(lib, spec, "e2e_prism_mosfet_257x129x16", fields=False, sample=4096)
+        return
+    if "--graphrb" in sys.argv:        # readback and terminal charges of a graph device
+        spec = prism.prism_mosfet(13, 9, 4)
+        readback_fixture(lib, spec, "rb_prism_mosfet_13x9x4", [0.0, 0.5, 1.0, 0.0])
+        cv_fixture(lib, spec, "cv_prism_mosfet_13x9x4", [0.0, 0.5, 1.0, 0.0], [0, 1, 2, 3], 2)
+        return
     if "--graph" in sys.argv:          # the graph (prism-mesh) path
         spec = prism.prism_mosfet(13, 9, 4)
         graph_kernel_fixture(lib, spec, "k_graph_prism_mosfet_13x9x4", [0.0, 0.5, 1.0, 0.0], np.random.default_rng(71))

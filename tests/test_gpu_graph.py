@@ -178,3 +178,92 @@ def test_bad_graphs_and_models_are_refused(hip):
     prism.load_graph_device(hip, spec)
     assert hip.api3_is_graph() == 1
     assert hip.api3_solve() == 0
+
+
+def test_graph_readback_matches_reference(hip):
+    """All 36 api3_bulk_copy quantities of the 13x9x4 prism MOSFET solved at
+    Vd = 0.5 V, Vg = 1.0 V against the reference's (rb_prism_mosfet_13x9x4.npz):
+    node current densities by least squares over the edge fluxes
+    (g_node_J), node mobility as the mean of its edges'; scalar readbacks;
+    api3_face_J refuses a graph device as the reference does."""
+    g = load_golden("rb_prism_mosfet_13x9x4")
+    spec = prism.prism_mosfet(13, 9, 4)
+    cids = prism.load_graph_device(hip, spec)
+    for c, v in enumerate(g["Va"]):
+        hip.api3_set_contact_V(cids[c], float(v))
+    assert hip.api3_solve() == 1
+    N = spec.N
+    bulk = np.zeros((36, N))
+    for w in range(36):
+        hip.api3_bulk_copy(w, dptr(bulk[w]))
+    ref = g["bulk"]
+
+    def close(w, rtol, atol_frac=0.0, atol=0.0, scale=None):
+        a, b = bulk[w], ref[w]
+        np.testing.assert_array_equal(np.isnan(a), np.isnan(b), err_msg=f"quantity {w}: NaN pattern")
+        m = ~np.isnan(b)
+        sc = np.max(np.abs(b[m])) if scale is None else scale
+        err = np.abs(a[m] - b[m])
+        assert np.all(err <= rtol * np.abs(b[m]) + atol_frac * sc + atol), (w, float(err.max()))
+
+    close(0, 0, atol=1e-9)
+    for w in (1, 2, 10, 11):
+        close(w, 1e-6)
+    for w in (3, 4):
+        close(w, 0, atol=1e-9)
+    for w in (5, 6, 15, 16):
+        close(w, 0, atol=1e-6)
+    sc = max(np.max(np.abs(ref[w])) for w in (7, 8, 9))
+    for w in (7, 8, 9, 20, 21, 22, 23, 24, 25):
+        close(w, 1e-4, atol_frac=1e-6, scale=sc)
+    close(12, 1e-4, atol_frac=1e-6)
+    for w in (13, 14, 17, 18, 27, 28, 29, 31, 32, 33, 34, 35):
+        close(w, 0)
+    for w in (19, 26, 30):
+        close(w, 1e-12)
+    assert hip.api3_face_J(0, 0, dptr(np.zeros(N))) == 0
+    ref_c = g["contacts_IInIrawFA"]
+    for c in range(len(cids)):
+        assert hip.api3_get_contact_A(cids[c]) == pytest.approx(ref_c[c, 4], rel=1e-12)
+        I, F = hip.api3_get_contact_I(cids[c]), ref_c[c, 3]
+        assert abs(I - ref_c[c, 0]) <= 1e-8 * abs(ref_c[c, 0]) + F + hip.api3_get_contact_Ifloor(cids[c])
+    sums = np.zeros(2)
+    hip.api3_carrier_sums(dptr(sums))
+    np.testing.assert_allclose(sums, g["carrier_sums"], rtol=1e-6)
+    assert hip.api3_get_Emax() == pytest.approx(float(g["Emax"]), rel=1e-6)
+    pos = np.zeros(3)
+    assert hip.api3_ins_field_max(dptr(pos)) == pytest.approx(float(g["ins_field_max"]), rel=1e-6)
+    np.testing.assert_allclose(pos[:2], g["ins_field_pos"][:2], rtol=0, atol=1e-15)
+
+
+def test_graph_terminal_charges_match_reference(hip):
+    """api3_charge_partition / api3_charge_owner / api3_terminal_charges on
+    the prism MOSFET at Vd 0.5 / Vg 1.0 V (each contact its own terminal),
+    and the gate column of the capacitance matrix by central differences."""
+    g = load_golden("cv_prism_mosfet_13x9x4")
+    spec = prism.prism_mosfet(13, 9, 4)
+    cids = prism.load_graph_device(hip, spec)
+    Va = [float(v) for v in g["Va"]]
+    for c, v in enumerate(Va):
+        hip.api3_set_contact_V(cids[c], v)
+    assert hip.api3_solve() == 1
+    ct = np.ascontiguousarray(g["cterm"], np.int32)
+    nt = int(ct.max()) + 1
+    meet = hip.api3_charge_partition(iptr(ct))
+    own = np.zeros(spec.N, np.int32)
+    assert hip.api3_charge_owner(iptr(own)) == 1
+    np.testing.assert_array_equal(own, g["owner"])
+    assert meet == int(g["meet"])
+    Q0 = np.zeros(nt + 1)
+    assert hip.api3_terminal_charges(nt, dptr(Q0)) == 1
+    assert np.all(np.abs(Q0 - g["Q0"]) <= 1e-6 * np.max(np.abs(g["Q0"]))), (Q0, g["Q0"])
+    tgt, dV = int(g["tgt"]), float(g["dV"])
+    Qs = []
+    for sg in (1.0, -1.0):
+        hip.api3_set_contact_V(cids[tgt], Va[tgt] + sg * dV)
+        assert hip.api3_resolve() == 1
+        Q = np.zeros(nt + 1)
+        assert hip.api3_terminal_charges(nt, dptr(Q)) == 1
+        Qs.append(Q)
+    K, Kr = (Qs[0] - Qs[1]) / (2 * dV), (g["Qp"] - g["Qm"]) / (2 * dV)
+    assert np.all(np.abs(K - Kr) <= 1e-4 * np.max(np.abs(Kr))), (K, Kr)
