@@ -1294,9 +1294,12 @@ class GraphDevice:
     N nodes, directed edges in CSR (rp, cj) with length h and dual face area
     ar, control volumes vol, positions pos (N, 3).  Contacts are node lists:
     dicts with bc, V, pm, tox (m), qox (C/m^2), face, and `nodes`, `areas`
-    (the boundary area of each node for a face contact)."""
+    (the boundary area of each node for a face contact).  walls (N, 6): per
+    node two interface walls (distance of the node, range a..b of its cell;
+    1 = none), api3_set_walls :4838-4841, read by MLDA and Lombardi."""
 
-    def __init__(self, rp, cj, h, ar, vol, pos, Nd, Na, contacts, models=7, tau=None, T=300.0, Nf=None):
+    def __init__(self, rp, cj, h, ar, vol, pos, Nd, Na, contacts, models=7, tau=None, T=300.0, Nf=None,
+                 walls=None):
         self.rp, self.cj = np.asarray(rp, np.int64), np.asarray(cj, np.int64)
         self.h, self.ar, self.vol = (np.asarray(a, np.float64) for a in (h, ar, vol))
         self.pos = np.asarray(pos, np.float64).reshape(-1, 3)
@@ -1346,9 +1349,13 @@ class GraphDevice:
             for k, a in zip(cn["nodes"], cn["areas"]):
                 if self.gate_mask[k] == c + 1 and not self.con_mask[k]:
                     self.robA[k] = a
-        self.lncK = self.lnc + self.bgn
-        self.lnvK = self.lnv + self.bgn
-        self.niK = m["ni"] * np.exp(self.bgn)
+        self.gwall = np.ones((self.N, 6)) if walls is None else np.asarray(walls, np.float64).reshape(-1, 6)
+        self.qd, self.qcn, self.qcp = graph_walls(self, (models & 8) != 0)
+        self.lncK = self.lnc + self.bgn + self.qcn
+        self.lnvK = self.lnv + self.bgn + self.qcp
+        self.niK = m["ni"] * np.exp(self.bgn + 0.5 * (self.qcn + self.qcp))
+        if models & 16:
+            self.lsq = graph_lsq(self)
         semi = self.kind == K_SEMI
         self.mu0n = np.where(semi, mob_low(m, self.Nd + self.Na, 0), 0.0)
         self.mu0p = np.where(semi, mob_low(m, self.Nd + self.Na, 1), 0.0)
@@ -1363,6 +1370,54 @@ class GraphDevice:
         n = np.exp(np.clip(self.lncK + (psi - phin) / VT, LN_MIN, LN_MAX))
         p = np.exp(np.clip(self.lnvK + (phip - psi) / VT, LN_MIN, LN_MAX))
         return np.where(semi, n, 0.0), np.where(semi, p, 0.0)
+
+
+def graph_walls(dev, mlda):
+    """Interface distance qd and, with mlda, the MLDA terms qcn, qcp of every
+    semiconductor node from its two host-supplied walls, walls_graph
+    semiconductor_3d.c:1252-1270 (walls within 40 nm count)."""
+    qd, qcn, qcp = np.ones(dev.N), np.zeros(dev.N), np.zeros(dev.N)
+    for k in np.nonzero(dev.kind == K_SEMI)[0]:
+        for w in range(2):
+            g = dev.gwall[k, 3 * w:3 * w + 3]
+            if not g[0] < 0.5:
+                continue
+            qd[k] = min(qd[k], g[0])
+            if mlda and g[1] < 40e-9:
+                qcn[k] += math.log(max(mlda_factor(dev.m, g[1], g[2], 0), 1e-30))
+                qcp[k] += math.log(max(mlda_factor(dev.m, g[1], g[2], 1), 1e-30))
+    return qd, np.maximum(qcn, -60.0), np.maximum(qcp, -60.0)
+
+
+def graph_lsq(dev):
+    """Inverse least-squares gradient matrix (N, 3, 3) of every semiconductor
+    node over its semiconductor neighbours, weights 1/|dr|^2, 1e-6 of the
+    trace added to the diagonal, g_lsq_setup semiconductor_3d.c:2682-2701."""
+    r, j = dev.row, dev.cj
+    ok = (dev.kind[r] == K_SEMI) & (dev.kind[j] == K_SEMI)
+    dr = dev.pos[j] - dev.pos[r]
+    w = np.where(ok, 1.0 / (np.sum(dr * dr, axis=1) + 1e-40), 0.0)
+    M = np.zeros((dev.N, 3, 3))
+    np.add.at(M, r, w[:, None, None] * dr[:, :, None] * dr[:, None, :])
+    eps = 1e-6 * np.trace(M, axis1=1, axis2=2) + 1e-30
+    M += eps[:, None, None] * np.eye(3)
+    out = np.zeros_like(M)
+    det = np.linalg.det(M)
+    good = np.abs(det) > 0
+    out[good] = np.linalg.inv(M[good])
+    return out
+
+
+def graph_gradient(dev, fld):
+    """Least-squares node gradient (N, 3) of a node field over the
+    semiconductor neighbours, g_grad semiconductor_3d.c:2702-2714."""
+    r, j = dev.row, dev.cj
+    ok = (dev.kind[r] == K_SEMI) & (dev.kind[j] == K_SEMI)
+    dr = dev.pos[j] - dev.pos[r]
+    w = np.where(ok, (fld[j] - fld[r]) / (np.sum(dr * dr, axis=1) + 1e-40), 0.0)
+    rhs = np.zeros((dev.N, 3))
+    np.add.at(rhs, r, w[:, None] * dr)
+    return np.einsum("kab,kb->ka", dev.lsq, rhs)
 
 
 def graph_poisson(dev, psi, phin, phip, Va):
@@ -1399,14 +1454,30 @@ def graph_poisson(dev, psi, phin, phip, Va):
 
 def graph_mobility(dev, psi, qn, qp):
     """Edge mobilities (electrons, holes) per undirected edge, upd_mob_ref_g
-    :4240-4292 (isotropic material, no Lombardi, no metal contacts)."""
+    :4241-4292 (isotropic material, no metal contacts): Lombardi with models
+    bit 16 from the node gradients of psi (nodes within 0.5 m of a wall) and
+    the field normal to the edge, then velocity saturation with bit 4."""
     lo = dev.row < dev.cj
     k, kp, h = dev.row[lo], dev.cj[lo], dev.h[lo]
     ok = (dev.kind[k] == K_SEMI) & (dev.kind[kp] == K_SEMI)
     mn = 0.5 * (dev.mu0n[k] + dev.mu0n[kp])
     mp = 0.5 * (dev.mu0p[k] + dev.mu0p[kp])
+    m = dev.m
+    if dev.models & 16:
+        gr = graph_gradient(dev, psi)
+        gr[~((dev.kind == K_SEMI) & (dev.qd < 0.5))] = 0.0
+        Dw = np.exp(-0.5 * (dev.qd[k] + dev.qd[kp]) / 1e-8)
+        t = (dev.pos[kp] - dev.pos[k]) / h[:, None]
+        ga, gb = gr[k], gr[kp]
+        pa, pb = np.sum(ga * t, axis=1), np.sum(gb * t, axis=1)
+        Fa = np.sum(ga * ga, axis=1) - pa * pa
+        Fb = np.sum(gb * gb, axis=1) - pb * pb
+        F = np.sqrt(np.maximum(0.5 * (Fa + Fb), 0.0))
+        Nt = 0.5 * (dev.Nd[k] + dev.Na[k] + dev.Nd[kp] + dev.Na[kp])
+        near = Dw > 1e-8
+        mn = np.where(near, lombardi(m, mn, F, Nt, Dw, 0), mn)
+        mp = np.where(near, lombardi(m, mp, F, Nt, Dw, 1), mp)
     if dev.models & 4:
-        m = dev.m
         dps = psi[kp] - psi[k]
         sn = mob_hf(mn, heat_field(qn[kp] - qn[k], dps, h), m["vsat"][0], m["beta"][0])
         sp = mob_hf(mp, heat_field(qp[kp] - qp[k], dps, h), m["vsat"][1], m["beta"][1])

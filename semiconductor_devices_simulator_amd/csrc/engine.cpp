@@ -665,8 +665,7 @@ int eng_prepare() {
         for (int i = 0; i < G.n_mats && !why; i++)
             if ((G.models & 512) && G.mats[i].id == 4) why = "incomplete ionisation (models bit 9)";
         if (G.graph && !why) {
-            if (G.models & 16) why = "Lombardi surface mobility on a graph mesh (models bit 4)";
-            else if (G.models & 32) why = "gate tunnelling on a graph mesh (models bit 5)";
+            if (G.models & 32) why = "gate tunnelling on a graph mesh (models bit 5)";
             else if (G.models & 256) why = "impact ionisation on a graph mesh (models bit 8)";
         }
         if (why) { fprintf(stderr, "semisim-hip: %s is outside this core: not solved\n", why); return 0; }
@@ -730,8 +729,7 @@ int eng_prepare() {
     // MLDA (models bit 3) the confinement factors qcn, qcp of the effective
     // densities of states near interfaces (lncK, lnvK, niK :736-738)
     if (G.graph) {
-        std::vector<double> qd;
-        gr_walls(qd, G.qcn, G.qcp, (G.models & 8) != 0, mlda_factor);
+        gr_walls(G.qd, G.qcn, G.qcp, (G.models & 8) != 0, mlda_factor);
     } else if (G.models & 8) {
         std::vector<double> qd;
         eng_walls(qd, &G.qcn, &G.qcp);
@@ -815,8 +813,8 @@ int eng_prepare() {
     const std::vector<double> *hv[] = {&G.Nd, &G.Na, &G.Nf, &G.eps, &G.bgn, &G.robA, &G.lncK, &G.lnvK, &G.niK, &G.mu0n, &G.mu0p};
     double *dv[] = {G.d_Nd, G.d_Na, G.d_Nf, G.d_eps, G.d_bgn, G.d_robA, G.d_lncK, G.d_lnvK, G.d_niK, G.d_mun, G.d_mup};
     for (int q = 0; q < 11; q++) HC(hipMemcpyAsync(dv[q], hv[q]->data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
-    if ((G.models & 16) && !G.graph) {   // Lombardi: distance of every node to the nearest interface
-        eng_wall_distance(G.qd);
+    if (G.models & 16) {                  // Lombardi: distance of every node to the nearest interface
+        if (!G.graph) eng_wall_distance(G.qd);
         HC(hipMemcpyAsync(G.d_qd, G.qd.data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
     }
     MatDev md[SD_NM_MAX];

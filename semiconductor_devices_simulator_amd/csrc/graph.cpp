@@ -309,6 +309,8 @@ int eng_set_graph(int N, int E, const int *rp, const int *cj, const double *h, c
     g.N = N; g.E = E; g.nue = nue;
     g.rp = gput(G.grp); g.cj = gput(G.gcol); g.ue = gput(G.gue);
     g.h = gput(G.gh); g.ar = gput(G.ga); g.vol = gput(G.gvol); g.pos = gput(G.gpos);
+    g.lsq = galloc<double>((size_t)6 * N);
+    g.grad = galloc<double>((size_t)3 * N);
     G.d_gval = galloc<double>(E);
     G.d_gmu = galloc<double>((size_t)2 * std::max(1, nue));
     gfail("graph mobility clear", hipMemsetAsync(G.d_gmu, 0, (size_t)2 * std::max(1, nue) * 8, G.st));
@@ -320,10 +322,40 @@ int eng_set_graph(int N, int E, const int *rp, const int *cj, const double *h, c
 }
 
 // prepare's graph part: box-contact areas from the dual faces to the
-// semiconductor outside the contact (prepare :1686-1697), and the
-// terminal-current edges grouped by contact in node order
+// semiconductor outside the contact (prepare :1686-1697), the
+// terminal-current edges grouped by contact in node order, and the
+// least-squares gradient matrices of the semiconductor nodes over their
+// semiconductor neighbours (g_lsq_setup :2651-2676, weights 1/|dr|^2, the
+// inverse with 1e-6 of the trace added to the diagonal)
 int gr_prepare_edges() {
     const int N = G.N;
+    {
+        std::vector<double> lsq((size_t)6 * N, 0.);
+        for (int k = 0; k < N; k++) {
+            double m[6] = {0, 0, 0, 0, 0, 0};
+            double *o = &lsq[(size_t)6 * k];
+            if (G.kind[k] == K_SEMI)
+                for (int e = G.grp[k]; e < G.grp[k + 1]; e++) {
+                    const int b = G.gcol[e];
+                    if (G.kind[b] != K_SEMI) continue;
+                    const double dx = G.gpos[3 * b] - G.gpos[3 * k], dy = G.gpos[3 * b + 1] - G.gpos[3 * k + 1],
+                                 dz = G.gpos[3 * b + 2] - G.gpos[3 * k + 2];
+                    const double w = 1. / (dx * dx + dy * dy + dz * dz + 1e-40);
+                    m[0] += w * dx * dx; m[1] += w * dx * dy; m[2] += w * dx * dz;
+                    m[3] += w * dy * dy; m[4] += w * dy * dz; m[5] += w * dz * dz;
+                }
+            const double tr = m[0] + m[3] + m[5], eps = 1e-6 * tr + 1e-30;
+            m[0] += eps; m[3] += eps; m[5] += eps;
+            const double a = m[0], b = m[1], c = m[2], dd = m[3], ee = m[4], f = m[5];
+            const double A = dd * f - ee * ee, Bq = -(b * f - c * ee), Cc = b * ee - c * dd;
+            const double D = a * f - c * c, Ei = -(a * ee - b * c), F = a * dd - b * b;
+            const double det = a * A + b * Bq + c * Cc;
+            if (!(fabs(det) > 0.)) continue;
+            o[0] = A / det; o[1] = Bq / det; o[2] = Cc / det; o[3] = D / det; o[4] = Ei / det; o[5] = F / det;
+        }
+        gfail("gradient matrices", hipMemcpyAsync((void *)G.gd.lsq, lsq.data(), lsq.size() * 8, hipMemcpyHostToDevice, G.st));
+        eng_sync();
+    }
     for (int c = 0; c < G.n_cons; c++) {
         HostCon &cn = G.cons[c];
         if (cn.face <= 5) continue;

@@ -84,6 +84,31 @@ __device__ __forceinline__ double ani_edge(const DevView &d, const GraphDev &g, 
     }
     return L2 > 0. ? 1. + (a - 1.) * dc * dc / L2 : 1.;
 }
+// Least-squares gradient of psi over the semiconductor neighbours of a
+// semiconductor node near an interface (qd < 0.5 m; 0 elsewhere): g = M^-1
+// sum_e (psi_b - psi_k) dr_e / |dr_e|^2 (g_grad :2678-2690, upd_mob_ref_g
+// :4250-4256)
+__global__ void k_ggrad(DevView d, GraphDev g) {
+    const int k = blockIdx.x * SD_BLOCK + threadIdx.x;
+    if (k >= g.N) return;
+    double r[3] = {0., 0., 0.};
+    if (d.kind[k] == K_SEMI && d.qd[k] < 0.5) {
+        for (int e = g.rp[k]; e < g.rp[k + 1]; e++) {
+            const int b = g.cj[e];
+            if (d.kind[b] != K_SEMI) continue;
+            const double dx = g.pos[3 * b] - g.pos[3 * k], dy = g.pos[3 * b + 1] - g.pos[3 * k + 1],
+                         dz = g.pos[3 * b + 2] - g.pos[3 * k + 2];
+            const double w = (d.psi[b] - d.psi[k]) / (dx * dx + dy * dy + dz * dz + 1e-40);
+            r[0] += w * dx; r[1] += w * dy; r[2] += w * dz;
+        }
+        const double *m = g.lsq + 6 * (size_t)k;
+        g.grad[3 * k] = m[0] * r[0] + m[1] * r[1] + m[2] * r[2];
+        g.grad[3 * k + 1] = m[1] * r[0] + m[3] * r[1] + m[4] * r[2];
+        g.grad[3 * k + 2] = m[2] * r[0] + m[4] * r[1] + m[5] * r[2];
+    } else {
+        g.grad[3 * k] = g.grad[3 * k + 1] = g.grad[3 * k + 2] = 0.;
+    }
+}
 // every undirected edge once, by its lower node
 __global__ void k_gupd_mob(DevView d, GraphDev g, const double *qn, const double *qp, double *gmu) {
     const int k = blockIdx.x * SD_BLOCK + threadIdx.x;
@@ -98,6 +123,23 @@ __global__ void k_gupd_mob(DevView d, GraphDev g, const double *qn, const double
         double mn = 0.5 * (d.mu_n[k] + d.mu_n[kp]), mp = 0.5 * (d.mu_p[k] + d.mu_p[kp]);
         mn *= ani_edge(d, g, k, kp, m.ani_n);
         mp *= ani_edge(d, g, k, kp, m.ani_p);
+        if (d.qd && m.lb_on) {          // Lombardi: the field normal to the edge from the node gradients
+            const double Dw = exp(-0.5 * (d.qd[k] + d.qd[kp]) / SD_LB_LCRIT);
+            if (Dw > 1e-8) {
+                const double h = g.h[e];
+                double t[3];
+                for (int q = 0; q < 3; q++) t[q] = (g.pos[3 * kp + q] - g.pos[3 * k + q]) / h;
+                const double *ga = g.grad + 3 * (size_t)k, *gb = g.grad + 3 * (size_t)kp;
+                const double pa = ga[0] * t[0] + ga[1] * t[1] + ga[2] * t[2];
+                const double pb = gb[0] * t[0] + gb[1] * t[1] + gb[2] * t[2];
+                const double Fa = ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2] - pa * pa;
+                const double Fb = gb[0] * gb[0] + gb[1] * gb[1] + gb[2] * gb[2] - pb * pb;
+                const double F = sqrt(fmax(0.5 * (Fa + Fb), 0.));
+                const double Nt = 0.5 * (d.Nd[k] + d.Na[k] + d.Nd[kp] + d.Na[kp]);
+                if (m.lb_on & 1) mn = sd_lombardi(m, mn, F, Nt, Dw, 0);
+                if (m.lb_on & 2) mp = sd_lombardi(m, mp, F, Nt, Dw, 1);
+            }
+        }
         if (sat) {                      // velocity saturation; no metal-contact column on this path (weight 1)
             const double h = g.h[e], dps = d.psi[kp] - d.psi[k];
             const double Fn = sd_heat_field(qn[kp] - qn[k], dps, h), Fp = sd_heat_field(qp[kp] - qp[k], dps, h);
@@ -111,6 +153,7 @@ __global__ void k_gupd_mob(DevView d, GraphDev g, const double *qn, const double
 }
 void gph_upd_mob(hipStream_t s, const DevView &d, const GraphDev &g, const double *qn, const double *qp,
                  double *gmu) {
+    if (d.qd) k_ggrad<<<sd_nblocks(g.N), SD_BLOCK, 0, s>>>(d, g);
     k_gupd_mob<<<sd_nblocks(g.N), SD_BLOCK, 0, s>>>(d, g, qn, qp, gmu);
 }
 

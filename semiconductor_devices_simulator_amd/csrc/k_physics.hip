@@ -124,27 +124,14 @@ __device__ __forceinline__ double fnorm2(const DevView &d, int i, int j, int l, 
     const double g1 = grad_semi(d, i, j, l, k, (ax + 1) % 3), g2 = grad_semi(d, i, j, l, k, (ax + 2) % 3);
     return g1 * g1 + g2 * g2;
 }
-// Lombardi surface mobility combined with the bulk value mub (m^2/Vs) by
-// Matthiessen's rule, weighted by the interface proximity Dw; F normal field
-// (V/m), Nt total doping (m^-3); lombardi, semiconductor_3d.c:2924-2934
-// (without the Coulomb term, which only 4H-SiC carries)
-__device__ __forceinline__ double lombardi(const MatDev &m, double mub, double Fv, double Nt, double Dw, int hole) {
-    double F = Fv * 1e-2; if (F < 1.) F = 1.;                       // V/cm
-    double N = Nt * 1e-6; if (N < 1.) N = 1.;                       // cm^-3
-    const double Bc = m.lbB[hole], Cc = m.lbC[hole], Lc = m.lbL[hole], Dc = m.lbD[hole], Ec = m.lbE[hole];
-    const double muac = Bc / F + Cc * pow(N, Lc) / (cbrt(F) * (m.T / 300.));
-    const double musr = Ec > 0. ? 1. / (F * F / Dc + F * F * F / Ec) : Dc / (F * F);
-    return 1. / (1. / mub + Dw * 1e4 * (1. / muac + 1. / musr));    // cm^2/Vs -> m^2/Vs
-}
 // one Lombardi evaluation for a material (api3_lombardi :4894-4897)
 __global__ void k_lombardi_eval(MatDev m, double mub, double Fv, double Nt, double Dw, int hole, double *out) {
-    if (threadIdx.x == 0) out[0] = lombardi(m, mub, Fv, Nt, Dw, hole);
+    if (threadIdx.x == 0) out[0] = sd_lombardi(m, mub, Fv, Nt, Dw, hole);
 }
 void ph_lombardi_eval(hipStream_t s, const MatDev &m, double mub, double Fv, double Nt, double Dw, int hole,
                       double *out) {
     k_lombardi_eval<<<1, 64, 0, s>>>(m, mub, Fv, Nt, Dw, hole, out);
 }
-#define SD_LB_LCRIT 1e-8
 __global__ void k_upd_mob(DevView d, const double *qn, const double *qp) {
     const Geo g = d.g;
     int k = sd_tile() * SD_BLOCK + threadIdx.x;
@@ -169,8 +156,8 @@ __global__ void k_upd_mob(DevView d, const double *qn, const double *qp) {
                 const int ip = i + (ax == 0), jp = j + (ax == 1), lp = l + (ax == 2);
                 const double F = sqrt(0.5 * (fnorm2(d, i, j, l, k, ax) + fnorm2(d, ip, jp, lp, kp, ax)));
                 const double Nt = 0.5 * (d.Nd[k] + d.Na[k] + d.Nd[kp] + d.Na[kp]);
-                if (m.lb_on & 1) mn = lombardi(m, mn, F, Nt, Dw, 0);
-                if (m.lb_on & 2) mp = lombardi(m, mp, F, Nt, Dw, 1);
+                if (m.lb_on & 1) mn = sd_lombardi(m, mn, F, Nt, Dw, 0);
+                if (m.lb_on & 2) mp = sd_lombardi(m, mp, F, Nt, Dw, 1);
             }
         }
         if (sat) {

@@ -167,6 +167,20 @@ __device__ __forceinline__ double sd_heat_field(double dq, double dps, double h)
     return 2. * a * b / ((a + b) * h);
 }
 
+// Lombardi surface mobility combined with the bulk value mub (m^2/Vs) by
+// Matthiessen's rule, weighted by the interface proximity Dw; F normal field
+// (V/m), Nt total doping (m^-3); lombardi, semiconductor_3d.c:2924-2934
+// (without the Coulomb term, which only 4H-SiC carries)
+__device__ __forceinline__ double sd_lombardi(const MatDev &m, double mub, double Fv, double Nt, double Dw, int hole) {
+    double F = Fv * 1e-2; if (F < 1.) F = 1.;                       // V/cm
+    double N = Nt * 1e-6; if (N < 1.) N = 1.;                       // cm^-3
+    const double Bc = m.lbB[hole], Cc = m.lbC[hole], Lc = m.lbL[hole], Dc = m.lbD[hole], Ec = m.lbE[hole];
+    const double muac = Bc / F + Cc * pow(N, Lc) / (cbrt(F) * (m.T / 300.));
+    const double musr = Ec > 0. ? 1. / (F * F / Dc + F * F * F / Ec) : Dc / (F * F);
+    return 1. / (1. / mub + Dw * 1e4 * (1. / muac + 1. / musr));    // cm^2/Vs -> m^2/Vs
+}
+#define SD_LB_LCRIT 1e-8      // interface-proximity length of the Lombardi terms (m)
+
 // carrier densities at node k (qcn = qcp = 0: no quantum confinement)
 __device__ __forceinline__ double sd_nK(const DevView &d, int k, double psi, double qn, double VT) {
     return exp(sd_clampe(d.lncK[k] + (psi - qn) / VT));

@@ -19,6 +19,8 @@ struct GraphDev {
     const double *h, *ar;          // edge length (m), dual face area (m^2)
     const double *vol;             // control volume (m^3)
     const double *pos;             // node positions, 3 per node (m)
+    const double *lsq;             // least-squares gradient matrix M^-1 per node (xx xy xz yy yz zz)
+    double *grad;                  // gradient of psi per node (3), Lombardi surface mobility
 };
 
 // Operator on a graph level: diagonal dg (nullptr: unit diagonal, not read)

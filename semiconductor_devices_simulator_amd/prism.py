@@ -52,6 +52,7 @@ class GraphSpec:
     tau: tuple = None
     sweep: list = field(default_factory=list)
     Nf: np.ndarray = None
+    walls: np.ndarray = None        # (N, 6): two walls per node, each (distance, cell range a, b) (m); None: none
 
     @property
     def N(self):
@@ -63,23 +64,6 @@ class GraphSpec:
 
     def bias_points(self):
         return sum(s[3] for s in self.sweep)
-
-
-def _csr(N, edges):
-    """Symmetric CSR of the undirected edges {(a, b): (h, ar)} with ascending
-    columns in every row."""
-    rows = [[] for _ in range(N)]
-    for (a, b), (h, ar) in edges.items():
-        rows[a].append((b, h, ar))
-        rows[b].append((a, h, ar))
-    rp = np.zeros(N + 1, np.int32)
-    cj, hh, aa = [], [], []
-    for k in range(N):
-        r = sorted(rows[k])
-        rp[k + 1] = rp[k] + len(r)
-        for b, h, ar in r:
-            cj.append(b); hh.append(h); aa.append(ar)
-    return rp, np.asarray(cj, np.int32), np.asarray(hh), np.asarray(aa)
 
 
 def _fv_len(c):
@@ -277,7 +261,8 @@ def prism_mesh(Lx, Ly, zs, nx, ny):
     return pos, rp, cj, h, ar, vol, P, row, side, Lz
 
 
-def prism_mosfet(nx=31, ny=17, nz=5, npts=7, vd=0.05, lg=200.0, tox=2.0, nsub=1e17, nsd=1e20, sigma=40.0):
+def prism_mosfet(nx=31, ny=17, nz=5, npts=7, vd=0.05, lg=200.0, tox=2.0, nsub=1e17, nsd=1e20, sigma=40.0,
+                 models=7):
     """Planar n-MOSFET of devices.mosfet (x channel, y depth with the top at
     y = 0, z width; 600 x 300 x 200 nm at lg = 200 nm) on the triangular-prism
     mesh of prism_mesh: nx points per even lattice row along x, ny rows, nz
@@ -317,8 +302,19 @@ def prism_mosfet(nx=31, ny=17, nz=5, npts=7, vd=0.05, lg=200.0, tox=2.0, nsub=1e
             (Contact(2, 0, 0, 0, 0, 2, 0.0, pm=4.65, tox=tox, label="Gate"),) + face(top, a, b),
             (Contact(3, 0, 0, 0, 0, 0, 0.0, label="Body"),) + face(bot, 0.0, Lx * 1e-9)]
     sweep = [(2, 0.0, 1.2, npts, {1: vd})]
-    return GraphSpec("prism_mosfet_%dx%dx%d" % (nx, ny, nz), pos, rp, cj, h, ar, vol, mat, Nd, Na, cons,
-                     models=7, sweep=sweep)
+    # with MLDA or Lombardi (models bits 3, 4) one wall per node: the top
+    # surface (the gate oxide interface), at the node's depth y, its cell
+    # spanning half a row spacing either side; otherwise none
+    walls = None
+    if models & (8 | 16):
+        hy = Ly * 1e-9 / (ny - 1)
+        y = pos[:, 1]
+        walls = np.ones((N, 6))
+        walls[:, 0] = y
+        walls[:, 1] = np.maximum(y - 0.5 * hy, 0.0)
+        walls[:, 2] = y + 0.5 * hy
+    name = "prism_mosfet_%dx%dx%d" % (nx, ny, nz) + ("" if models == 7 else "_m%d" % models)
+    return GraphSpec(name, pos, rp, cj, h, ar, vol, mat, Nd, Na, cons, models=models, sweep=sweep, walls=walls)
 
 
 def load_graph_device(lib, spec, solver=None):
@@ -352,7 +348,7 @@ def load_graph_device(lib, spec, solver=None):
         if not lib.api3_set_con_nodes(cid, len(n), iptr(n), dptr(a)):
             raise ValueError(f"bad node list for contact {c.label}")
         cids.append(cid)
-    walls = np.ones(6 * spec.N)
+    walls = np.ones(6 * spec.N) if spec.walls is None else np.ascontiguousarray(spec.walls, np.float64).ravel()
     lib.api3_set_walls(dptr(walls))
     if spec.tau is not None:
         lib.api3_set_mat_tau(mids[0], spec.tau[0], spec.tau[1])

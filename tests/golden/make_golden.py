@@ -256,6 +256,8 @@ def graph_kernel_fixture(lib, spec, name, Va, rng, perturb=1e-3):
                con_nodes=np.concatenate([n for _, n, _ in spec.contacts]),
                con_off=np.cumsum([0] + [len(n) for _, n, _ in spec.contacts]),
                con_areas=np.concatenate([a for _, _, a in spec.contacts]))
+    if spec.walls is not None:
+        out["walls"] = spec.walls
     for w, key in ((3, "n"), (4, "p"), (6, "bgn")):
         a = np.zeros(N)
         lib.refh_get_node(w, core.dptr(a))
@@ -667,6 +669,15 @@ def main():
         ts = devices.mosfet(24, 16, 8, npts=7)
         ts.sweep = ts.sweep[:1]
         sweep_fixture(lib, prism.tensor_as_graph(ts), "e2e_graph_mosfet_24x16x8")
+        return
+    if "--graphsurf" in sys.argv:      # MLDA and Lombardi on a graph device (walls from the host)
+        spec = prism.prism_mosfet(13, 9, 4, models=7 | 8 | 16)
+        graph_kernel_fixture(lib, spec, "k_graph_prism_mosfet_13x9x4_m31", [0.0, 0.5, 1.0, 0.0],
+                             np.random.default_rng(73))
+        spec = prism.prism_mosfet(models=7 | 8 | 16)
+        sweep_fixture(lib, spec, "e2e_prism_mosfet_31x17x5_m31")
+        spec.solver = devices.tight_solver(spec.solver)
+        sweep_fixture(lib, spec, "e2e_prism_mosfet_31x17x5_m31_tight")
         return
     if "--float" in sys.argv:          # floating majority blocks (float_fix)
         sweep_fixture(lib, devices.guard_ring_diode(), "e2e_ring_diode_32x16x4")

@@ -9,9 +9,12 @@ unstructured transport :4175-4463).
   (poisson_core_g, upd_mob_ref_g, cont_assemble_g, terminal_flux_g).
 * The aggregation multigrid inside PCG and BiCGSTAB solves those operators
   (residual checked with the reference's own operator applied on the CPU).
+* The same with MLDA and Lombardi surface mobility (models 8 | 16, walls at
+  the gate oxide interface): carriers, Poisson and edge mobilities.
 * Sweeps: the prism MOSFET's Id-Vg sweep against the reference's (both
   criteria of conftest: the sum of the floors against its default sweep, one
-  floor against its sweep converged at tol_p 1e-12, tol_c 1e-11), and psi.
+  floor against its sweep converged at tol_p 1e-12, tol_c 1e-11), and psi;
+  also with MLDA and Lombardi.
 * The identity the reference's validation checks with a prism mesh holding
   every tensor node (validate.py:652-660): the tensor MOSFET re-expressed as
   a graph gives the tensor path's currents, here GPU graph path against GPU
@@ -36,9 +39,11 @@ def _close(a, b, rtol, what):
     assert err <= rtol, (what, err)
 
 
-def test_graph_kernels_match_reference(hip):
-    g = load_golden("k_graph_prism_mosfet_13x9x4")
-    spec = prism.prism_mosfet(13, 9, 4)
+@pytest.mark.parametrize("models", [7, 7 | 8 | 16])
+def test_graph_kernels_match_reference(hip, models):
+    """models 7 | 8 | 16 adds MLDA and Lombardi (walls: the gate interface)."""
+    spec = prism.prism_mosfet(13, 9, 4, models=models)
+    g = load_golden("k_graph_" + spec.name)
     assert np.array_equal(spec.rp, g["rp"]) and np.array_equal(spec.cj, g["cj"])
     cids = prism.load_graph_device(hip, spec)
     assert hip.api3_is_graph() == 1
@@ -126,9 +131,9 @@ def test_graph_multigrid_krylov_solves(hip):
     assert np.isfinite(rn) and np.linalg.norm(r) <= 1e-8 * np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_prism_mosfet_sweep_matches_reference(hip, precision):
-    spec = prism.prism_mosfet()
+@pytest.mark.parametrize("precision,models", [(0, 7), (1, 7), (0, 7 | 8 | 16)])
+def test_prism_mosfet_sweep_matches_reference(hip, precision, models):
+    spec = prism.prism_mosfet(models=models)
     cids = prism.load_graph_device(hip, spec)
     hip.sdx_set_precision(precision)
     try:
@@ -137,8 +142,8 @@ def test_prism_mosfet_sweep_matches_reference(hip, precision):
     finally:
         hip.sdx_set_precision(0)
     assert np.all(R["conv"])
-    assert_sweep_parity(R, "e2e_prism_mosfet_31x17x5")
-    t = load_tight("e2e_prism_mosfet_31x17x5")
+    assert_sweep_parity(R, "e2e_" + spec.name)
+    t = load_tight("e2e_" + spec.name)
     assert_current_parity(R["I"], t["I"], t["F"], "prism MOSFET vs the tight reference sweep")
     psi = np.zeros(spec.N)
     hip.api3_get_state(dptr(psi), dptr(np.zeros(spec.N)), dptr(np.zeros(spec.N)))
@@ -174,7 +179,7 @@ def test_bad_graphs_and_models_are_refused(hip):
     assert hip.api3_set_mesh_graph(spec.N, spec.E, iptr(args[0]), iptr(args[1]), dptr(args[2]), dptr(args[3]),
                                    dptr(args[4]), dptr(args[5])) == 0
     assert hip.api3_is_graph() == 0
-    spec.models |= 16                               # Lombardi on a graph mesh is not built
+    spec.models |= 32                               # gate tunnelling on a graph mesh is not built
     prism.load_graph_device(hip, spec)
     assert hip.api3_is_graph() == 1
     assert hip.api3_solve() == 0

@@ -19,12 +19,14 @@ def _device(g):
         cn["nodes"] = g["con_nodes"][off[c]:off[c + 1]]
         cn["areas"] = g["con_areas"][off[c]:off[c + 1]]
     return O.GraphDevice(g["rp"], g["cj"], g["h"], g["ar"], g["vol"], g["pos"], g["Nd"], g["Na"], cons,
-                         models=int(g["models"]))
+                         models=int(g["models"]), walls=g["walls"] if "walls" in g.files else None)
 
 
-@pytest.fixture(scope="module")
-def case():
-    g = load_golden("k_graph_prism_mosfet_13x9x4")
+# the default models (BGN, SRH doping, velocity saturation), and with MLDA and
+# Lombardi added (walls: the gate oxide interface)
+@pytest.fixture(scope="module", params=["k_graph_prism_mosfet_13x9x4", "k_graph_prism_mosfet_13x9x4_m31"])
+def case(request):
+    g = load_golden(request.param)
     return g, _device(g)
 
 
@@ -102,6 +104,27 @@ def test_graph_mobility(case):
     g, dev = case
     gmu = O.graph_mobility(dev, g["psi"], g["phin"], g["phip"])
     close(gmu.ravel(), g["gmu"], 1e-12)
+    if dev.models & 16:                     # Lombardi lowers the mobility of edges near the gate only
+        plain = O.GraphDevice(dev.rp, dev.cj, dev.h, dev.ar, dev.vol, dev.pos, dev.Nd, dev.Na, dev.contacts,
+                              models=dev.models & ~16, walls=dev.gwall)
+        g0 = O.graph_mobility(plain, g["psi"], g["phin"], g["phip"])
+        lo = dev.row < dev.cj
+        y = 0.5 * (dev.pos[dev.row[lo], 1] + dev.pos[dev.cj[lo], 1])[np.argsort(dev.ue[lo])]
+        assert np.all(gmu <= g0 * (1 + 1e-12))
+        assert np.any(gmu[0][y < 20e-9] < 0.9 * g0[0][y < 20e-9])
+        np.testing.assert_allclose(gmu[:, y > 200e-9], g0[:, y > 200e-9], rtol=1e-12)
+
+
+def test_graph_gradient_is_exact_for_linear_fields():
+    """The least-squares node gradient (g_lsq_setup, g_grad) returns the
+    gradient of a linear field at every node up to the 1e-6 trace
+    regularisation of its matrix."""
+    spec = prism.prism_mosfet(13, 9, 4, models=7 | 16)
+    dev = O.GraphDevice(spec.rp, spec.cj, spec.h, spec.ar, spec.vol, spec.pos, spec.Nd, spec.Na, [],
+                        models=spec.models, walls=spec.walls)
+    a = np.array([3.0e6, -1.5e6, 0.7e6])
+    gr = O.graph_gradient(dev, spec.pos @ a + 0.2)
+    np.testing.assert_allclose(gr, np.broadcast_to(a, gr.shape), rtol=1e-5)
 
 
 def test_graph_continuity_and_currents(case):
