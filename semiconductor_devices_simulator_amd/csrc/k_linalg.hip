@@ -122,17 +122,36 @@ void sd_axpy(hipStream_t s, double *y, const double *x, double a, int n) {
 }
 
 // -------------------------------------------------------------- stencil
+// s + a xv (nb_add) or s - a xv (nb_sub) where the neighbour exists, s
+// otherwise.  The callers load a and xv whether or not the neighbour exists
+// (its index clamped to the node itself), so that no load sits inside a
+// branch: a load whose value is used in a conditional block makes the
+// compiler wait for it inside the block, and the loads of a row then
+// complete one after another instead of together.  The select leaves the
+// arithmetic (one fused multiply-add where the neighbour exists) unchanged.
+// Index of such a neighbour load: the neighbour's, or the node's own where
+// there is none.  The empty asm hides the value from the compiler, which
+// would otherwise fold the clamped load into the node's own load of the same
+// array and move the neighbour's back into a branch.
+__device__ __forceinline__ int nb_idx(bool has, int nb, int self) {
+    int v = has ? nb : self;
+    asm("" : "+v"(v));
+    return v;
+}
+template <class V> __device__ __forceinline__ V nb_add(bool has, V s, V a, V xv) { return has ? s + a * xv : s; }
+template <class V> __device__ __forceinline__ V nb_sub(bool has, V s, V a, V xv) { return has ? s - a * xv : s; }
 template <class T, class V>
 __device__ __forceinline__ V sten_apply(const Sten7<T> &A, const V *x, int k, int i, int j, int l,
                                         int Nx, int Ny, int Nz, bool unitdiag) {
     int NxNy = Nx * Ny;
+    const bool h0 = i > 0, h1 = i < Nx - 1, h2 = j > 0, h3 = j < Ny - 1, h4 = l > 0, h5 = l < Nz - 1;
     V s = unitdiag ? (V)x[k] : (V)A.c[k] * x[k];
-    if (i > 0) s += (V)A.xm[k] * x[k - 1];
-    if (i < Nx - 1) s += (V)A.xp[k] * x[k + 1];
-    if (j > 0) s += (V)A.ym[k] * x[k - Nx];
-    if (j < Ny - 1) s += (V)A.yp[k] * x[k + Nx];
-    if (l > 0) s += (V)A.zm[k] * x[k - NxNy];
-    if (l < Nz - 1) s += (V)A.zp[k] * x[k + NxNy];
+    s = nb_add<V>(h0, s, (V)A.xm[k], x[nb_idx(h0, k - 1, k)]);
+    s = nb_add<V>(h1, s, (V)A.xp[k], x[nb_idx(h1, k + 1, k)]);
+    s = nb_add<V>(h2, s, (V)A.ym[k], x[nb_idx(h2, k - Nx, k)]);
+    s = nb_add<V>(h3, s, (V)A.yp[k], x[nb_idx(h3, k + Nx, k)]);
+    s = nb_add<V>(h4, s, (V)A.zm[k], x[nb_idx(h4, k - NxNy, k)]);
+    s = nb_add<V>(h5, s, (V)A.zp[k], x[nb_idx(h5, k + NxNy, k)]);
     return s;
 }
 
@@ -160,8 +179,9 @@ template void sd_spmv_T<float>(hipStream_t, const Geo &, Sten7<float>, const flo
 // level, type TS), copied to xw in FP64 natural order on the way
 // NT: the operator is read with non-temporal loads (streamed once per
 // product, so it need not displace the multigrid data from the caches)
-// A.c == nullptr: unit diagonal (not read)
-template <class TS, bool NT>
+// UD (A.c == nullptr): unit diagonal (not read); a template argument, so that
+// no branch separates the node's own load from its use
+template <class TS, bool NT, bool UD>
 __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, Sten7<double> A, const double *x,
                                                         double *y, const double *u0, const double *u1, int nv,
                                                         RedBuf rb, Stage0 in, double *xw) {
@@ -181,23 +201,25 @@ __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, 
             auto xp = [&](int ii) { return in.px ? (ii & 1) * h + (ii >> 1) : ii; };
             const int q = r0 + xp(i);
             xk = st_get<TS>(in, q);
-            double s = A.c ? ld(A.c, k) * xk : xk;
-            if (i > 0) s += ld(A.xm, k) * st_get<TS>(in, r0 + xp(i - 1));
-            if (i < Nx - 1) s += ld(A.xp, k) * st_get<TS>(in, r0 + xp(i + 1));
-            if (j > 0) s += ld(A.ym, k) * st_get<TS>(in, q - Nx);
-            if (j < Ny - 1) s += ld(A.yp, k) * st_get<TS>(in, q + Nx);
-            if (l > 0) s += ld(A.zm, k) * st_get<TS>(in, q - NxNy);
-            if (l < Nz - 1) s += ld(A.zp, k) * st_get<TS>(in, q + NxNy);
+            const bool h0 = i > 0, h1 = i < Nx - 1, h2 = j > 0, h3 = j < Ny - 1, h4 = l > 0, h5 = l < Nz - 1;
+            double s = UD ? xk : ld(A.c, k) * xk;
+            s = nb_add<double>(h0, s, ld(A.xm, k), st_get<TS>(in, nb_idx(h0, r0 + xp(i - 1), q)));
+            s = nb_add<double>(h1, s, ld(A.xp, k), st_get<TS>(in, nb_idx(h1, r0 + xp(i + 1), q)));
+            s = nb_add<double>(h2, s, ld(A.ym, k), st_get<TS>(in, nb_idx(h2, q - Nx, q)));
+            s = nb_add<double>(h3, s, ld(A.yp, k), st_get<TS>(in, nb_idx(h3, q + Nx, q)));
+            s = nb_add<double>(h4, s, ld(A.zm, k), st_get<TS>(in, nb_idx(h4, q - NxNy, q)));
+            s = nb_add<double>(h5, s, ld(A.zp, k), st_get<TS>(in, nb_idx(h5, q + NxNy, q)));
             return s;
         } else {
             xk = 0.0;
-            double s = A.c ? ld(A.c, k) * x[k] : x[k];
-            if (i > 0) s += ld(A.xm, k) * x[k - 1];
-            if (i < Nx - 1) s += ld(A.xp, k) * x[k + 1];
-            if (j > 0) s += ld(A.ym, k) * x[k - Nx];
-            if (j < Ny - 1) s += ld(A.yp, k) * x[k + Nx];
-            if (l > 0) s += ld(A.zm, k) * x[k - NxNy];
-            if (l < Nz - 1) s += ld(A.zp, k) * x[k + NxNy];
+            const bool h0 = i > 0, h1 = i < Nx - 1, h2 = j > 0, h3 = j < Ny - 1, h4 = l > 0, h5 = l < Nz - 1;
+            double s = UD ? x[k] : ld(A.c, k) * x[k];
+            s = nb_add<double>(h0, s, ld(A.xm, k), x[nb_idx(h0, k - 1, k)]);
+            s = nb_add<double>(h1, s, ld(A.xp, k), x[nb_idx(h1, k + 1, k)]);
+            s = nb_add<double>(h2, s, ld(A.ym, k), x[nb_idx(h2, k - Nx, k)]);
+            s = nb_add<double>(h3, s, ld(A.yp, k), x[nb_idx(h3, k + Nx, k)]);
+            s = nb_add<double>(h4, s, ld(A.zm, k), x[nb_idx(h4, k - NxNy, k)]);
+            s = nb_add<double>(h5, s, ld(A.zp, k), x[nb_idx(h5, k + NxNy, k)]);
             return s;
         }
     };
@@ -209,12 +231,16 @@ __global__ void __launch_bounds__(SD_BLOCK) k_spmv_dots(int Nx, int Ny, int Nz, 
         NodeWalk nb = nw;
         nb.next();
         double xa, xb = 0.0, sb = 0.0;
-        const double sa = row(k, nw.i, nw.j, nw.l, xa);
         const bool hb = kb < k1;
+        // the dot vectors' values are loaded with the rows' operands
+        const int kc = hb ? kb : k;
+        const double u0a = u0 ? u0[k] : 0.0, u0b = u0 ? u0[kc] : 0.0;
+        const double va = nv > 1 ? u1[k] : 0.0, v1b = nv > 1 ? u1[kc] : 0.0;
+        const double sa = row(k, nw.i, nw.j, nw.l, xa);
         if (hb) sb = row(kb, nb.i, nb.j, nb.l, xb);
-        const double ua = u0 ? u0[k] : sa, va = nv > 1 ? u1[k] : 0.0;
+        const double ua = u0 ? u0a : sa;
         double ub = 0.0, vb = 0.0;
-        if (hb) { ub = u0 ? u0[kb] : sb; vb = nv > 1 ? u1[kb] : 0.0; }
+        if (hb) { ub = u0 ? u0b : sb; vb = v1b; }
         y[k] = sa;
         if constexpr (!std::is_void_v<TS>) { if (xw) xw[k] = xa; }
         acc[0] += sa * ua;
@@ -234,17 +260,18 @@ void sd_spmv_dots(hipStream_t s, const Geo &g, Sten7<double> A, const double *x,
                   const double *u1, int nv, RedBuf rb, int slot, int post, Stage0 in) {
     int nb = red_grid(g.N);
     double *xw = const_cast<double *>(x);
-    auto go = [&](auto nt) {
-        constexpr bool NT = decltype(nt)::value;
+    auto go = [&](auto nt, auto ud) {
+        constexpr bool NT = decltype(nt)::value, UD = decltype(ud)::value;
         if (in.x && in.f32)
-            k_spmv_dots<float, NT><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, xw);
+            k_spmv_dots<float, NT, UD><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, xw);
         else if (in.x)
-            k_spmv_dots<double, NT><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, xw);
+            k_spmv_dots<double, NT, UD><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, xw);
         else
-            k_spmv_dots<void, NT><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, nullptr);
+            k_spmv_dots<void, NT, UD><<<nb, SD_BLOCK, 0, s>>>(g.Nx, g.Ny, g.Nz, A, x, y, u0, u1, nv, rb, in, nullptr);
     };
-    if (g_spmv_nt) go(std::true_type{});
-    else go(std::false_type{});
+    const bool ud = A.c == nullptr;
+    if (g_spmv_nt) { if (ud) go(std::true_type{}, std::true_type{}); else go(std::true_type{}, std::false_type{}); }
+    else { if (ud) go(std::false_type{}, std::true_type{}); else go(std::false_type{}, std::false_type{}); }
     red_final(s, rb, nv, nb, slot, 0, post);
 }
 
@@ -608,27 +635,28 @@ __device__ __forceinline__ void rb_node(const MGLevel<T> &v, const T *b, T *x, i
     int i = 2 * i2 + ((j + l + color) & 1);
     if (i >= Nx) return;
     int k = i + Nx * (j + Ny * l);
+    const bool h0 = i > 0, h1 = i < Nx - 1, h2 = j > 0, h3 = j < Ny - 1, h4 = l > 0, h5 = l < Nz - 1;
+    T d = v.A.c ? v.A.c[k] : (T)1;          // loaded with the neighbours
     T s = b[k];
     if (pro) {
         auto xc = [&](int ii, int jj, int ll, int kk) -> T {
             int I = v.cx ? ii >> 1 : ii, J = v.cy ? jj >> 1 : jj, L = v.cz ? ll >> 1 : ll;
             return x[kk] + v.pw[kk] * c.x[I + c.Nx * (J + c.Ny * L)];
         };
-        if (i > 0) s -= v.A.xm[k] * xc(i - 1, j, l, k - 1);
-        if (i < Nx - 1) s -= v.A.xp[k] * xc(i + 1, j, l, k + 1);
-        if (j > 0) s -= v.A.ym[k] * xc(i, j - 1, l, k - Nx);
-        if (j < Ny - 1) s -= v.A.yp[k] * xc(i, j + 1, l, k + Nx);
-        if (l > 0) s -= v.A.zm[k] * xc(i, j, l - 1, k - NxNy);
-        if (l < Nz - 1) s -= v.A.zp[k] * xc(i, j, l + 1, k + NxNy);
+        s = nb_sub<T>(h0, s, v.A.xm[k], xc(i - h0, j, l, k - h0));
+        s = nb_sub<T>(h1, s, v.A.xp[k], xc(i + h1, j, l, k + h1));
+        s = nb_sub<T>(h2, s, v.A.ym[k], xc(i, j - h2, l, k - h2 * Nx));
+        s = nb_sub<T>(h3, s, v.A.yp[k], xc(i, j + h3, l, k + h3 * Nx));
+        s = nb_sub<T>(h4, s, v.A.zm[k], xc(i, j, l - h4, k - h4 * NxNy));
+        s = nb_sub<T>(h5, s, v.A.zp[k], xc(i, j, l + h5, k + h5 * NxNy));
     } else {
-        if (i > 0) s -= v.A.xm[k] * x[k - 1];
-        if (i < Nx - 1) s -= v.A.xp[k] * x[k + 1];
-        if (j > 0) s -= v.A.ym[k] * x[k - Nx];
-        if (j < Ny - 1) s -= v.A.yp[k] * x[k + Nx];
-        if (l > 0) s -= v.A.zm[k] * x[k - NxNy];
-        if (l < Nz - 1) s -= v.A.zp[k] * x[k + NxNy];
+        s = nb_sub<T>(h0, s, v.A.xm[k], x[nb_idx(h0, k - 1, k)]);
+        s = nb_sub<T>(h1, s, v.A.xp[k], x[nb_idx(h1, k + 1, k)]);
+        s = nb_sub<T>(h2, s, v.A.ym[k], x[nb_idx(h2, k - Nx, k)]);
+        s = nb_sub<T>(h3, s, v.A.yp[k], x[nb_idx(h3, k + Nx, k)]);
+        s = nb_sub<T>(h4, s, v.A.zm[k], x[nb_idx(h4, k - NxNy, k)]);
+        s = nb_sub<T>(h5, s, v.A.zp[k], x[nb_idx(h5, k + NxNy, k)]);
     }
-    T d = v.A.c ? v.A.c[k] : (T)1;
     if (!(d != (T)0)) d = (T)1;
     x[k] = s / d;
 }
@@ -646,13 +674,16 @@ __device__ __forceinline__ void rb_zero2_node(const MGLevel<T> &v, const T *b, T
         return d != (T)0 ? d : (T)1;
     };
     if (((i + j + l) & 1) == color) { x[k] = b[k] / dg(k); return; }
+    const bool h0 = i > 0, h1 = i < Nx - 1, h2 = j > 0, h3 = j < Ny - 1, h4 = l > 0, h5 = l < Nz - 1;
+    const int n0 = nb_idx(h0, k - 1, k), n1 = nb_idx(h1, k + 1, k), n2 = nb_idx(h2, k - Nx, k), n3 = nb_idx(h3, k + Nx, k),
+              n4 = nb_idx(h4, k - NxNy, k), n5 = nb_idx(h5, k + NxNy, k);
     T s = b[k];
-    if (i > 0) s -= v.A.xm[k] * (b[k - 1] / dg(k - 1));
-    if (i < Nx - 1) s -= v.A.xp[k] * (b[k + 1] / dg(k + 1));
-    if (j > 0) s -= v.A.ym[k] * (b[k - Nx] / dg(k - Nx));
-    if (j < Ny - 1) s -= v.A.yp[k] * (b[k + Nx] / dg(k + Nx));
-    if (l > 0) s -= v.A.zm[k] * (b[k - NxNy] / dg(k - NxNy));
-    if (l < Nz - 1) s -= v.A.zp[k] * (b[k + NxNy] / dg(k + NxNy));
+    s = nb_sub<T>(h0, s, v.A.xm[k], b[n0] / dg(n0));
+    s = nb_sub<T>(h1, s, v.A.xp[k], b[n1] / dg(n1));
+    s = nb_sub<T>(h2, s, v.A.ym[k], b[n2] / dg(n2));
+    s = nb_sub<T>(h3, s, v.A.yp[k], b[n3] / dg(n3));
+    s = nb_sub<T>(h4, s, v.A.zm[k], b[n4] / dg(n4));
+    s = nb_sub<T>(h5, s, v.A.zp[k], b[n5] / dg(n5));
     x[k] = s / dg(k);
 }
 // zero: 1 the colour's half sweep from x = 0, 2 the whole sweep from x = 0
@@ -1156,11 +1187,15 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan(MGLevel<T> v, const T 
 // is then a compile-time power of two, the indices of every element are
 // computed once and kept for the three phases, and the x positions of a
 // line and of its neighbours across x follow from the line's parity without
-// division.
-template <class T, int P, int S, int MD, bool H16, int AX>
+// division.  HM: coefficients in T (0), in separate BF16 arrays (1), or in
+// one packed 16-byte BF16 record per node (2).  The choice is a template
+// argument: a run-time branch between the layouts joins after each node's
+// loads, so the compiler waits for every load in flight there and a lane's
+// gathers no longer overlap.
+template <class T, int P, int S, int MD, int HM, int AX>
 __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T *__restrict__ b, T *__restrict__ x,
                                                            int color, MGLevel<T> c, int xcd) {
-    constexpr bool zero = MD == 0, pro = MD == 2;
+    constexpr bool zero = MD == 0, pro = MD == 2, H16 = HM != 0;
     constexpr int SL = 64 * P, SP = SL + 1, NLW = LS_WAVES * S, M = P * S;
     __shared__ T s_r[NLW * SP], s_a[NLW * SP], s_c[NLW * SP];
     const int Nx = v.Nx, Ny = v.Ny, Nz = v.Nz, NxNy = Nx * Ny;
@@ -1227,22 +1262,20 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
         msk[m] = (int)ok | ((int)hm1 << 1) | ((int)hp1 << 2) | ((int)hm2 << 3) | ((int)hp2 << 4);
         kk[m] = k;
         vb[m] = b[k];
-        if constexpr (H16) {
-            if (v.rec16[a_lu]) {
-                // the node's seven BF16 coefficients in one 16-byte load (a
-                // BF16 value is the upper half of the FP32 word)
-                const uint4 r = v.rec16[a_lu][k];
-                vai[m] = __uint_as_float(r.x << 16); viw[m] = __uint_as_float(r.x & 0xffff0000u);
-                vcp[m] = __uint_as_float(r.y << 16);
-                if constexpr (!zero) {
-                    cf[m][0] = __uint_as_float(r.y & 0xffff0000u); cf[m][1] = __uint_as_float(r.z << 16);
-                    cf[m][2] = __uint_as_float(r.z & 0xffff0000u); cf[m][3] = __uint_as_float(r.w << 16);
-                }
-            } else {
-                vai[m] = (T)v.lu16[a_lu][0][k]; viw[m] = (T)v.lu16[a_lu][1][k]; vcp[m] = (T)v.lu16[a_lu][2][k];
-                if constexpr (!zero) {
-                    cf[m][0] = (T)h0p[k]; cf[m][1] = (T)h1p[k]; cf[m][2] = (T)h2p[k]; cf[m][3] = (T)h3p[k];
-                }
+        if constexpr (HM == 2) {
+            // the node's seven BF16 coefficients in one 16-byte load (a
+            // BF16 value is the upper half of the FP32 word)
+            const uint4 r = v.rec16[a_lu][k];
+            vai[m] = __uint_as_float(r.x << 16); viw[m] = __uint_as_float(r.x & 0xffff0000u);
+            vcp[m] = __uint_as_float(r.y << 16);
+            if constexpr (!zero) {
+                cf[m][0] = __uint_as_float(r.y & 0xffff0000u); cf[m][1] = __uint_as_float(r.z << 16);
+                cf[m][2] = __uint_as_float(r.z & 0xffff0000u); cf[m][3] = __uint_as_float(r.w << 16);
+            }
+        } else if constexpr (HM == 1) {
+            vai[m] = (T)v.lu16[a_lu][0][k]; viw[m] = (T)v.lu16[a_lu][1][k]; vcp[m] = (T)v.lu16[a_lu][2][k];
+            if constexpr (!zero) {
+                cf[m][0] = (T)h0p[k]; cf[m][1] = (T)h1p[k]; cf[m][2] = (T)h2p[k]; cf[m][3] = (T)h3p[k];
             }
         } else {
             vai[m] = v.lu[a_lu][0][k]; viw[m] = v.lu[a_lu][1][k]; vcp[m] = v.lu[a_lu][2][k];
@@ -1332,8 +1365,8 @@ static void ls_launch_md(hipStream_t s, int S, int nb, const MGLevel<T> &v, cons
 }
 int g_line_scan1 = 1;         // 1: axis-specialised kernel for one line per slot
 // launch of k_line_scan1 (one line per slot): slots per wavefront S, sweep
-// kind MD, BF16 coefficients H, axis from the runtime value
-template <class T, int P, int MD, bool H, int AX>
+// kind MD, coefficient layout H, axis from the runtime value
+template <class T, int P, int MD, int H, int AX>
 static void ls1_launch_ax(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int color,
                           const MGLevel<T> &cc, int xcd) {
     constexpr int SM = ls_smax<T>(P);
@@ -1344,7 +1377,7 @@ static void ls1_launch_ax(hipStream_t s, int S, int nb, const MGLevel<T> &v, con
     else
         k_line_scan1<T, P, 1, MD, H, AX><<<nb, LS_THREADS, 0, s>>>(v, b, x, color, cc, xcd);
 }
-template <class T, int P, int MD, bool H>
+template <class T, int P, int MD, int H>
 static void ls1_launch_h(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color,
                          const MGLevel<T> &cc, int xcd) {
     if (ax == 0) ls1_launch_ax<T, P, MD, H, 0>(s, S, nb, v, b, x, color, cc, xcd);
@@ -1355,12 +1388,16 @@ template <class T, int P, int MD>
 static void ls1_launch_md(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color,
                           const MGLevel<T> &cc, int xcd) {
     if constexpr (std::is_same_v<T, float> && P <= 2) {
-        if (v.rec16[ax] || (v.o16[0] && v.lu16[ax][0])) {
-            ls1_launch_h<T, P, MD, true>(s, S, nb, v, b, x, ax, color, cc, xcd);
+        if (v.rec16[ax]) {
+            ls1_launch_h<T, P, MD, 2>(s, S, nb, v, b, x, ax, color, cc, xcd);
+            return;
+        }
+        if (v.o16[0] && v.lu16[ax][0]) {
+            ls1_launch_h<T, P, MD, 1>(s, S, nb, v, b, x, ax, color, cc, xcd);
             return;
         }
     }
-    ls1_launch_h<T, P, MD, false>(s, S, nb, v, b, x, ax, color, cc, xcd);
+    ls1_launch_h<T, P, MD, 0>(s, S, nb, v, b, x, ax, color, cc, xcd);
 }
 template <class T, int P>
 static void ls_launch(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero,
@@ -1442,7 +1479,8 @@ int mg_line(hipStream_t s, const MGLevel<T> &v, const T *b, T *x, int ax, int co
 // (0 outside the aggregate or on a skipped row)
 // NT: the operator and row scales are read with non-temporal loads (level 0
 // with BF16 smoother coefficients, where nothing else reads them in the V-cycle)
-template <class T, bool NT = false>
+// UD: unit diagonal (f.A.c not read), known at compile time
+template <class T, bool NT = false, bool UD = false>
 __device__ __forceinline__ double agg_resid(const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x, int skip,
                                             int I, int J, int LL, int di, int dj, int dl) {
     const int i = f.cx ? 2 * I + di : I, j = f.cy ? 2 * J + dj : J, l = f.cz ? 2 * LL + dl : LL;
@@ -1459,17 +1497,17 @@ __device__ __forceinline__ double agg_resid(const MGLevel<T> &f, const MGLevel<T
     };
     const int k = mg_idx(f, i, j, l), kx = k - mg_xpos(f, i);
     const double w = f.rs ? ld(f.rs, k) : 1.0;
-    if (w == 0.0) return 0.0;
     const int NxNy = f.Nx * f.Ny;
-    T s = f.A.c ? ld(f.A.c, k) * x[k] : x[k];
-    if (i > 0) s += ld(f.A.xm, k) * x[kx + mg_xpos(f, i - 1)];
-    if (i < f.Nx - 1) s += ld(f.A.xp, k) * x[kx + mg_xpos(f, i + 1)];
-    if (j > 0) s += ld(f.A.ym, k) * x[k - f.Nx];
-    if (j < f.Ny - 1) s += ld(f.A.yp, k) * x[k + f.Nx];
-    if (l > 0) s += ld(f.A.zm, k) * x[k - NxNy];
-    if (l < f.Nz - 1) s += ld(f.A.zp, k) * x[k + NxNy];
+    const bool h0 = i > 0, h1 = i < f.Nx - 1, h2 = j > 0, h3 = j < f.Ny - 1, h4 = l > 0, h5 = l < f.Nz - 1;
+    T s = UD ? x[k] : (f.A.c ? ld(f.A.c, k) * x[k] : x[k]);
+    s = nb_add<T>(h0, s, ld(f.A.xm, k), x[nb_idx(h0, kx + mg_xpos(f, i - 1), k)]);
+    s = nb_add<T>(h1, s, ld(f.A.xp, k), x[nb_idx(h1, kx + mg_xpos(f, i + 1), k)]);
+    s = nb_add<T>(h2, s, ld(f.A.ym, k), x[nb_idx(h2, k - f.Nx, k)]);
+    s = nb_add<T>(h3, s, ld(f.A.yp, k), x[nb_idx(h3, k + f.Nx, k)]);
+    s = nb_add<T>(h4, s, ld(f.A.zm, k), x[nb_idx(h4, k - NxNy, k)]);
+    s = nb_add<T>(h5, s, ld(f.A.zp, k), x[nb_idx(h5, k + NxNy, k)]);
     T r = b[k] - s;
-    return w * (double)r;
+    return w == 0.0 ? 0.0 : w * (double)r;
 }
 template <class T>
 __device__ __forceinline__ double agg_resid(const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x, int skip,
@@ -1532,7 +1570,7 @@ __device__ __forceinline__ void act_offset(int skip, int w, int &di, int &dj, in
     default: di = hi; dj = hi; dl = lo; break;        // z lines: (i + j) even
     }
 }
-template <class T, bool NT>
+template <class T, bool NT, bool UD>
 __global__ void __launch_bounds__(256) k_resid_restrict_act(MGLevel<T> f, MGLevel<T> c, const T *b, const T *x,
                                                             int skip, double alpha) {
     __shared__ double part[4][64];
@@ -1548,7 +1586,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict_act(MGLevel<T> f, MGLeve
         I -= c.Nx;
         if (++J == c.Ny) { J = 0; LL++; }
     }
-    part[w][lane] = K < c.N ? agg_resid<T, NT>(f, c, b, x, skip, I, J, LL, di, dj, dl) : 0.0;
+    part[w][lane] = K < c.N ? agg_resid<T, NT, UD>(f, c, b, x, skip, I, J, LL, di, dj, dl) : 0.0;
     __syncthreads();
     if (threadIdx.x < 64 && K < c.N) {
         const double sum = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
@@ -1560,10 +1598,12 @@ int g_restrict_act = 1;      // 1: active-lane restriction where it applies
 template <class T> void mg_resid_restrict(hipStream_t s, const MGLevel<T> &f, const MGLevel<T> &c, const T *b, const T *x,
                                           int skip, double alpha) {
     if (g_restrict_act && skip >= 1 && skip <= 4 && f.cx && f.cy && f.cz) {
-        if (f.o16[0] && g_spmv_nt)
-            k_resid_restrict_act<T, true><<<(c.N + 63) / 64, 256, 0, s>>>(f, c, b, x, skip, alpha);
-        else
-            k_resid_restrict_act<T, false><<<(c.N + 63) / 64, 256, 0, s>>>(f, c, b, x, skip, alpha);
+        const int nb = (c.N + 63) / 64;
+        const bool nt = f.o16[0] && g_spmv_nt, ud = f.A.c == nullptr;
+        if (nt && ud) k_resid_restrict_act<T, true, true><<<nb, 256, 0, s>>>(f, c, b, x, skip, alpha);
+        else if (nt) k_resid_restrict_act<T, true, false><<<nb, 256, 0, s>>>(f, c, b, x, skip, alpha);
+        else if (ud) k_resid_restrict_act<T, false, true><<<nb, 256, 0, s>>>(f, c, b, x, skip, alpha);
+        else k_resid_restrict_act<T, false, false><<<nb, 256, 0, s>>>(f, c, b, x, skip, alpha);
     }
     else
         k_resid_restrict<T><<<(c.N + 31) / 32, 256, 0, s>>>(f, c, b, x, skip, alpha);
@@ -1658,12 +1698,14 @@ __global__ void __launch_bounds__(TAIL_BLOCK) k_tail_lds(const MGLevel<T> *lv, i
             if (((i + j + l) & 1) == color) {
                 T s = sb[off[L] + t];
                 if (!zero) {
-                    if (i > 0) s -= a[L][0] * x[t - 1];
-                    if (i < Nx - 1) s -= a[L][1] * x[t + 1];
-                    if (j > 0) s -= a[L][2] * x[t - Nx];
-                    if (j < v.Ny - 1) s -= a[L][3] * x[t + Nx];
-                    if (l > 0) s -= a[L][4] * x[t - NxNy];
-                    if (l < v.Nz - 1) s -= a[L][5] * x[t + NxNy];
+                    const bool h0 = i > 0, h1 = i < Nx - 1, h2 = j > 0, h3 = j < v.Ny - 1, h4 = l > 0,
+                               h5 = l < v.Nz - 1;
+                    s = nb_sub<T>(h0, s, a[L][0], x[nb_idx(h0, t - 1, t)]);
+                    s = nb_sub<T>(h1, s, a[L][1], x[nb_idx(h1, t + 1, t)]);
+                    s = nb_sub<T>(h2, s, a[L][2], x[nb_idx(h2, t - Nx, t)]);
+                    s = nb_sub<T>(h3, s, a[L][3], x[nb_idx(h3, t + Nx, t)]);
+                    s = nb_sub<T>(h4, s, a[L][4], x[nb_idx(h4, t - NxNy, t)]);
+                    s = nb_sub<T>(h5, s, a[L][5], x[nb_idx(h5, t + NxNy, t)]);
                 }
                 x[t] = s;
             } else if (zero) {
@@ -1685,13 +1727,14 @@ __global__ void __launch_bounds__(TAIL_BLOCK) k_tail_lds(const MGLevel<T> *lv, i
             double w = 0.0;
             if (!((i + j + l) & 1)) {
                 const T *x = sx + off[L];
+                const bool h0 = i > 0, h1 = i < Nx - 1, h2 = j > 0, h3 = j < v.Ny - 1, h4 = l > 0, h5 = l < v.Nz - 1;
                 T s = sb[off[L] + t] - x[t];
-                if (i > 0) s -= a[L][0] * x[t - 1];
-                if (i < Nx - 1) s -= a[L][1] * x[t + 1];
-                if (j > 0) s -= a[L][2] * x[t - Nx];
-                if (j < v.Ny - 1) s -= a[L][3] * x[t + Nx];
-                if (l > 0) s -= a[L][4] * x[t - NxNy];
-                if (l < v.Nz - 1) s -= a[L][5] * x[t + NxNy];
+                s = nb_sub<T>(h0, s, a[L][0], x[nb_idx(h0, t - 1, t)]);
+                s = nb_sub<T>(h1, s, a[L][1], x[nb_idx(h1, t + 1, t)]);
+                s = nb_sub<T>(h2, s, a[L][2], x[nb_idx(h2, t - Nx, t)]);
+                s = nb_sub<T>(h3, s, a[L][3], x[nb_idx(h3, t + Nx, t)]);
+                s = nb_sub<T>(h4, s, a[L][4], x[nb_idx(h4, t - NxNy, t)]);
+                s = nb_sub<T>(h5, s, a[L][5], x[nb_idx(h5, t + NxNy, t)]);
                 w = rs[L] * (double)s;
             }
             sr[t] = w;
