@@ -16,7 +16,9 @@ extern "C" {
 int  sdx_set_precision(int mode);
 int  sdx_get_precision(void);
 int  sdx_device_name(char *buf, int n);
-/* out[0] V-cycles, out[1] level-0 SpMVs, out[2] nodes, out[3] MG levels */
+/* out[0] V-cycles, out[1] level-0 SpMVs, out[2] nodes, out[3] MG levels,
+   out[4] FP32 inner solves, out[5] their FP64 corrections, out[6] inner
+   continuity solves redone with FP64 Krylov; returns the count filled (7) */
 int  sdx_kernel_stats(double *out, int n);
 void sdx_reset_stats(void);
 
@@ -67,7 +69,11 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
                                               of the FP32 method read each node's seven BF16
                                               coefficients as one packed 16-byte record (1) or as
                                               seven 2-byte loads (0), 25: V-cycles replayed from
-                                              captured HIP graphs (1) or launched kernel by kernel (0) */
+                                              captured HIP graphs (1) or launched kernel by kernel (0),
+                                              26: quadratic sweep predictor (1) or the secant (0),
+                                              27: FP32 method solves with FP32 Krylov inside FP64
+                                              defect correction (1, on devices without a large
+                                              floating block; 2 on any device) or with FP64 Krylov (0) */
 
 /* ---- learned warm start ----
    A 3-D convolutional U-Net on a G^3 grid resampled from the mesh (index

@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result", "-Wno-unused-value"]
-SOURCES = ["k_linalg.hip", "k_physics.hip", "k_nn.hip", "k_csr.hip", "k_graph.hip", "engine.cpp", "graph.cpp", "nn.cpp",
+SOURCES = ["k_linalg.hip", "k_krylov32.hip", "k_physics.hip", "k_nn.hip", "k_csr.hip", "k_graph.hip", "engine.cpp", "graph.cpp", "nn.cpp",
            "api.cpp"]
 
 

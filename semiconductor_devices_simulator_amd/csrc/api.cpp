@@ -705,10 +705,12 @@ int sdx_device_name(char *buf, int n) {
     return 1;
 }
 int sdx_kernel_stats(double *out, int n) {
-    double v[4] = {(double)G.n_vcycles, (double)G.n_spmv, (double)G.N, (double)(G.precision == 1 ? G.H32.lev.size() : G.H64.lev.size())};
-    for (int i = 0; i < n && i < 4; i++) out[i] = v[i];
-    return 4;
+    double v[7] = {(double)G.n_vcycles, (double)G.n_spmv, (double)G.N,
+                   (double)(G.precision == 1 ? G.H32.lev.size() : G.H64.lev.size()),
+                   (double)G.n_inner, (double)G.n_outer, (double)G.n_inner_fb};
+    for (int i = 0; i < n && i < 7; i++) out[i] = v[i];
+    return 7;
 }
-void sdx_reset_stats(void) { G.n_vcycles = G.n_spmv = 0; }
+void sdx_reset_stats(void) { G.n_vcycles = G.n_spmv = G.n_inner = G.n_outer = G.n_inner_fb = 0; }
 
 }  // extern "C"

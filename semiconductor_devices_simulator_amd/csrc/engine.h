@@ -123,6 +123,10 @@ struct Engine {
     Hier<float> H32;
     float *d_b32 = nullptr, *d_x32 = nullptr;   // level-0 FP32 vectors
     float *d_x32b = nullptr;                    // second level-0 FP32 solution (BiCGSTAB sh)
+    // FP32 method: each linear solve is FP64 defect correction around an FP32
+    // Krylov solve on level-0 storage (8 FP32 vectors: r, rh, p, v, ph, sh, t, d)
+    int inner32 = 1;
+    float *d_k32 = nullptr;
     const double *mg_w = nullptr;
     int mg_qsign = 0;
     int mg_off = 0;
@@ -172,6 +176,14 @@ struct Engine {
 
     // statistics of the kernels (see sdx_kernel_stats)
     long long n_vcycles = 0, n_spmv = 0;
+    // FP32 inner solves: solves, outer FP64 corrections, fallbacks to the
+    // FP64-Krylov form (an inner solve that did not bring the FP64 residual down)
+    long long n_inner = 0, n_outer = 0, n_inner_fb = 0;
+    // since the last prepare: whether a continuity solve of each carrier (0
+    // electrons, 1 holes) has run, and whether one found a floating majority
+    // block of more than 1% of the nodes (a near-singular operator: the FP32
+    // method then keeps FP64 Krylov)
+    int ff_seen[2] = {0, 0}, ff_off = 0;
     // floating majority blocks: continuity solves that listed unanchored
     // nodes, blocks found, blocks rescaled (see sdx_float_stats)
     long long ff_stat[3] = {0, 0, 0};

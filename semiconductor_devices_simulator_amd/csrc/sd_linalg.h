@@ -174,3 +174,30 @@ void mg_lu16(hipStream_t s, const MGLevel<float> &v, int ax);
 void mg_rec16(hipStream_t s, const MGLevel<float> &v, int ax, uint4 *rec);
 template <class T> void mg_rs_in(hipStream_t s, const double *w, const MGLevel<T> &v, double *rs);
 template <class T> void sd_spmv_T(hipStream_t s, const Geo &g, Sten7<T> A, const T *x, const T *b, T *y);
+
+// ------------------------------------------------ FP32 inner Krylov (k_krylov32.hip)
+// Level-0 geometry and storage order of the FP32 multigrid: the inner solves'
+// vectors are stored as the V-cycle reads and writes them.
+struct L0Geo {
+    int Nx, Ny, Nz, px;
+};
+// y = A x with the FP32 level-0 operator (unit: diagonal not read); slot <-
+// y.(u0 or y), slot+1 <- y.u1 (nv 2); FP64 accumulation; then scalar op post
+void k32_spmv_dots_launch(hipStream_t s, const L0Geo &g, Sten7<float> A, bool unit, const float *x, float *y,
+                          const float *u0, const float *u1, int nv, RedBuf rb, int slot, int post);
+// r = b - A x in FP64 (FP64 operator, natural order), stored FP32 at level-0
+// positions; r.r -> slot.  rh != nullptr: also rh = r, p = v = d = 0
+void k32_resid_launch(hipStream_t s, const L0Geo &g, Sten7<double> A, bool unit, const double *b, const double *x,
+                      float *r, float *rh, float *p, float *v, float *d, RedBuf rb, int slot);
+void k32_add_out_launch(hipStream_t s, const L0Geo &g, double *x, const float *d);      // x += d (FP64)
+void k32_dot_launch(hipStream_t s, const float *a, const float *b, int n, RedBuf rb, int slot, int post = -1);
+void k32_fill_launch(hipStream_t s, float *a, float v, int n);
+void k32_copy_launch(hipStream_t s, float *d, const float *a, int n);
+void k32_pcg_xr_launch(hipStream_t s, float *d, float *r, const float *p, const float *q, int n, RedBuf rb);
+void k32_pcg_p_launch(hipStream_t s, float *p, const float *z, int n, RedBuf rb);
+void k32_bi_p_launch(hipStream_t s, float *p, const float *r, const float *v, int n, RedBuf rb);
+void k32_bi_s_launch(hipStream_t s, float *r, const float *v, int n, RedBuf rb, int slot);
+void k32_bi_x1_launch(hipStream_t s, float *d, const float *ph, int n, RedBuf rb);
+void k32_bi_xr2_launch(hipStream_t s, float *d, float *r, const float *ph, const float *sh, const float *t,
+                       const float *rh, int n, RedBuf rb);
+void k32_bi_restart_launch(hipStream_t s, float *rh, float *p, float *v, const float *r, int n);

@@ -32,12 +32,18 @@ def test_floating_ring_sweep_matches_reference(hip, name, gate_v, precision):
     assert hip.sdx_set_precision(precision) == 1
     assert hip.api3_solve() == 1
     hip.sdx_float_reset_stats()
+    hip.sdx_reset_stats()
     R = sweep.run_sweep(hip, spec, cids)
     st = np.zeros(3)
     hip.sdx_float_stats(dptr(st))
     print(f"[gpu float] solves with unanchored nodes {st[0]:.0f}, blocks {st[1]:.0f}, rescaled {st[2]:.0f}",
           flush=True)
     assert st[1] > 0 and st[2] > 0, st          # the balance solve ran on a real block
+    # the ring holds 30% of the nodes: the FP32 method keeps FP64 Krylov
+    # around its FP32 V-cycle on this device (no FP32 inner solve)
+    ks = np.zeros(7)
+    hip.sdx_kernel_stats(dptr(ks), 7)
+    assert ks[4] == 0, ks
     assert np.all(R["conv"])
     assert_sweep_parity(R, name)
     if gate_v is not None:

@@ -261,6 +261,73 @@ def test_bicgstab_solves_continuity(hip, dev, hole):
     assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(r0) + 1e-300
 
 
+def _inner_stats(hip):
+    st = np.zeros(7)
+    hip.sdx_kernel_stats(dptr(st), 7)
+    return st
+
+
+@pytest.mark.parametrize("hole", [0, 1])
+def test_fp32_inner_bicgstab_reaches_fp64_residual(hip, dev, hole):
+    """The FP32 method's continuity solve (FP32 inner BiCGSTAB on level-0
+    storage inside FP64 defect correction, tune 27 = 2 forces it on): the
+    oracle's FP64 residual of the returned x reaches the same 1e-10 |r0| as
+    the FP64 Krylov solve of test_bicgstab_solves_continuity.  Where an FP32
+    correction does not halve the FP64 residual (an operator too
+    ill-conditioned for FP32: some of these fixture states) the solve is
+    redone with FP64 Krylov, and the residual bound holds either way."""
+    g, N = dev
+    t = "p" if hole else "n"
+    hip.sdx_set_precision(1)
+    hip.sdx_tune(27, 2)
+    try:
+        hip.sdx_test_load_op(dptr(np.ascontiguousarray(g[f"cont{t}_A"].ravel())), -1 if hole else 1,
+                             dptr(np.ascontiguousarray(g[f"cont{t}_dsc"])))
+        b = np.ascontiguousarray(g[f"cont{t}_rhs"])
+        x = np.ones(N)
+        its = np.zeros(1, np.int32)
+        hip.sdx_reset_stats()
+        rn = hip.sdx_test_bicgstab(dptr(b), dptr(x), 1000, 1e-10, 0.0, iptr(its))
+        st = _inner_stats(hip)
+    finally:
+        hip.sdx_tune(27, 1)
+        hip.sdx_set_precision(0)
+    assert np.isfinite(rn)
+    r = O.matvec(g[f"cont{t}_A"], _dims(g), x, b)
+    r0 = O.matvec(g[f"cont{t}_A"], _dims(g), np.ones(N), b)
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(r0) + 1e-300
+    print(f"[gpu kernels] inner solves {st[4]:.0f}, corrections {st[5]:.0f}, FP64 redone {st[6]:.0f}", flush=True)
+    assert st[4] == 1 and st[5] >= 1, st
+    assert st[6] == 1 or st[5] >= 2, st          # 1e-10 takes more than one 1e-5 correction
+
+
+def test_fp32_inner_pcg_newton_step(hip, dev):
+    """The FP32 method's Newton-step PCG (FP32 vectors on level-0 storage,
+    FP64 dots, the same M^-1-norm stop, tune 27 = 2): at rtol 1e-4 it takes
+    the FP64 PCG's iteration count within one and leaves an FP64 residual
+    (oracle) within 2x of the FP64 PCG's."""
+    g, N = dev
+    A7 = np.ascontiguousarray(g["poisson_A"].ravel())
+    b = np.ascontiguousarray(g["poisson_F"])
+    out = {}
+    hip.sdx_set_precision(1)
+    try:
+        for inner in (0, 2):
+            hip.sdx_tune(27, inner)
+            hip.sdx_test_load_op(dptr(A7), 0, None)
+            x = np.zeros(N)
+            hip.sdx_reset_stats()
+            its = hip.sdx_test_pcg(dptr(b), dptr(x), 500, 1e-4)
+            out[inner] = (its, np.linalg.norm(O.matvec(g["poisson_A"], _dims(g), x, b)), _inner_stats(hip))
+    finally:
+        hip.sdx_tune(27, 1)
+        hip.sdx_set_precision(0)
+    (i64, r64, s64), (i32, r32, s32) = out[0], out[2]
+    assert s64[4] == 0 and s32[4] == 1, (s64, s32)
+    assert abs(i32 - i64) <= 1, (i32, i64)
+    assert r32 <= 2.0 * r64 and r32 < 1e-2 * np.linalg.norm(b), (r32, r64)
+
+
 def test_terminal_currents(hip, dev):
     g, N = dev
     hip.sdx_test_set_fmu(dptr(np.ascontiguousarray(g["fmu"].ravel())))

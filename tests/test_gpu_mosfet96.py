@@ -1,7 +1,8 @@
 """The 96x64x64 n-MOSFET Id-Vg sweep (50 bias points, Vd 0.05 and 1.0 V) on
 the GPU against the reference core's full sweep (tests/golden/
 e2e_mosfet_96x64x64.npz: currents and floors of every point and psi at 4096
-sampled nodes of the last point), plain FP64 and FP32 multigrid.  The same
+sampled nodes of the last point), plain FP64 and the FP32 method (FP32
+multigrid and FP32 inner Krylov inside FP64 defect correction).  The same
 run is also held against the reference's sweep converged far below its
 default tolerances (e2e_mosfet_96x64x64_tight.npz: tol_p 1e-12, tol_c 1e-11),
 which stands for the exact discrete solution: at the default tolerances the
@@ -30,8 +31,19 @@ def test_mosfet_96x64x64_sweep_matches_reference(hip, precision):
     cids = devices.load_device(hip, spec)
     assert hip.sdx_set_precision(precision) == 1
     assert hip.api3_solve() == 1
+    hip.sdx_reset_stats()
     R = sweep.run_sweep(hip, spec, cids)
     assert np.all(R["conv"])
+    # the FP32 method solves with FP32 inner Krylov inside FP64 defect
+    # correction (the pockets float_fix finds here are far below 1% of the
+    # nodes), and no continuity solve falls back to FP64 Krylov
+    st = np.zeros(7)
+    hip.sdx_kernel_stats(dptr(st), 7)
+    print(f"[gpu e2e] inner solves {st[4]:.0f}, corrections {st[5]:.0f}, fallbacks {st[6]:.0f}", flush=True)
+    if precision == 1:
+        assert st[4] > 0 and st[6] == 0, st
+    else:
+        assert st[4] == 0, st
     assert_sweep_parity(R, "e2e_mosfet_96x64x64")
     # the metric's accuracy claim: at the default tolerances every current of
     # this sweep is within 1e-8 |I| + floor of the exact discrete solution
