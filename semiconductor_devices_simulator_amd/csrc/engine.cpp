@@ -1589,7 +1589,7 @@ static int poisson_newton(double ntol, int maxit) {
         sd_fill(s, dx, 0.0, N);
         G.mg_w = nullptr;
         G.mg_qsign = 0;
-        pcg(G.d_rhs, dx, G.max_iter_p > 0 ? G.max_iter_p : 2000, G.pcg_rtol);
+        pcg(G.d_rhs, dx, G.max_iter_p > 0 ? G.max_iter_p : 2000, inner32_on() ? G.pcg_rtol32 : G.pcg_rtol);
         sd_maxabs(s, dx, N, G.rb, S_MAX0);
         rd(S_MAX0, S_MAX0 + 1);
         double dm = S(S_MAX0);
@@ -1781,7 +1781,8 @@ static int solve_cont(int hole, double *dq, double lo, double hi, double *dmax) 
     double atol = CONT_ATOL * G.cont_atol_f;
     int its;
     G.unit_diag = g_unit_diag;            // k_cont writes every diagonal entry as 1
-    double rn = bicgstab(G.d_rhs, G.d_sol, G.max_iter_c > 0 ? G.max_iter_c : 1000, G.bicg_rtol, atol, &its);
+    const double rtol = inner32_on() ? G.bicg_rtol32 : G.bicg_rtol;
+    double rn = bicgstab(G.d_rhs, G.d_sol, G.max_iter_c > 0 ? G.max_iter_c : 1000, rtol, atol, &its);
     G.unit_diag = 0;
     if (G.verbose > 2) fprintf(stderr, "    bicgstab(%c) its=%d |r|=%.3e\n", hole ? 'p' : 'n', its, rn);
     int ok = std::isfinite(rn);
@@ -1985,8 +1986,9 @@ static int gummel(const double *Va, int maxit, double gtol, int *iters, double *
     const int avalanche = (G.models & 256) != 0;
     const double ii_tol = fmax(itol, 1e-4);
     // with impact ionisation the continuity solves run at 1e-8 for this point
-    const double brtol_saved = G.bicg_rtol;
+    const double brtol_saved = G.bicg_rtol, brtol32_saved = G.bicg_rtol32;
     if (avalanche && G.bicg_rtol > 1e-8) G.bicg_rtol = 1e-8;
+    if (avalanche && G.bicg_rtol32 > 1e-8) G.bicg_rtol32 = 1e-8;
     AvalancheSeries ii;
     CurrentTrace cur;
     cur.nc = G.n_cons;
@@ -2046,6 +2048,7 @@ static int gummel(const double *Va, int maxit, double gtol, int *iters, double *
     }
     if (it > maxit) it = maxit;
     if (avalanche && brtol_saved > 1e-8 && G.bicg_rtol == 1e-8) G.bicg_rtol = brtol_saved;
+    if (avalanche && brtol32_saved > 1e-8 && G.bicg_rtol32 == 1e-8) G.bicg_rtol32 = brtol32_saved;
     if (avalanche) G.ii_err = SD_Q * ii.remainder;
     G.gummel_its += it;
     *iters = it;
@@ -2442,6 +2445,8 @@ void eng_read_env() {
     G.pcg_rtol = (e = getenv("SEMISIM_PCGRTOL")) ? atof(e) : 1e-3;
     G.ntol_fac = (e = getenv("SEMISIM_NTOLF")) ? atof(e) : 0.1;
     G.bicg_rtol = (e = getenv("SEMISIM_BRTOL")) ? atof(e) : 1e-4;
+    G.pcg_rtol32 = (e = getenv("SEMISIM_PCGRTOL32")) ? atof(e) : 1e-2;
+    G.bicg_rtol32 = (e = getenv("SEMISIM_BRTOL32")) ? atof(e) : 1e-3;
     G.verbose = (e = getenv("SEMISIM_VERBOSE")) ? atoi(e) : 0;
     if ((e = getenv("SEMISIM_AAM"))) { int q = atoi(e); AA_DEPTH = q < 0 ? 0 : (q > 12 ? 12 : q); }
     if ((e = getenv("SEMISIM_PRED"))) PRED_ON = atoi(e) != 0;

@@ -57,6 +57,11 @@ struct Engine {
     int max_iter_p = 2000, max_iter_c = 1000, max_gummel = 500;
     double tol_p = 1e-9, tol_c = 1e-6, omega_p = 1., omega_c = 1.;
     double pcg_rtol = 1e-3, ntol_fac = 0.1, bicg_rtol = 1e-4, cont_atol_f = 1.;
+    // relative tolerances of the FP32 method's inner solves (Newton-step PCG,
+    // continuity BiCGSTAB): the Gummel loop's own stopping rule sets the
+    // accuracy of a converged point, and an inexact solve only perturbs an
+    // iteration by a fraction of its own update
+    double pcg_rtol32 = 1e-2, bicg_rtol32 = 1e-3;
     int converged = 0, conv_iter = 0;
     double conv_res = 0.;
     int allocated = 0, have_solution = 0, dirty = 1, verbose = 0, nthreads = 1;
