@@ -73,7 +73,9 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
                                               26: quadratic sweep predictor (1) or the secant (0),
                                               27: FP32 method solves with FP32 Krylov inside FP64
                                               defect correction (1, on devices without a large
-                                              floating block; 2 on any device) or with FP64 Krylov (0) */
+                                              floating block; 2 on any device) or with FP64 Krylov (0),
+                                              28: level-1 line sweeps of the FP32 method read packed
+                                              BF16 records (1) or the FP32 arrays (0) */
 
 /* ---- learned warm start ----
    A 3-D convolutional U-Net on a G^3 grid resampled from the mesh (index

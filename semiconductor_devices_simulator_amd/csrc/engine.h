@@ -37,6 +37,7 @@ template <class T> struct Hier {
     // first use): 6 off-diagonals, then 3 factors for each of the 3 axes
     __bf16 *c16 = nullptr;
     uint4 *r16 = nullptr;          // the same packed per node and axis (3 N records)
+    uint4 *r16b = nullptr;         // packed BF16 records of level 1 (3 N1), when it is line-smoothed
 };
 
 struct Engine {
@@ -57,11 +58,15 @@ struct Engine {
     int max_iter_p = 2000, max_iter_c = 1000, max_gummel = 500;
     double tol_p = 1e-9, tol_c = 1e-6, omega_p = 1., omega_c = 1.;
     double pcg_rtol = 1e-3, ntol_fac = 0.1, bicg_rtol = 1e-4, cont_atol_f = 1.;
-    // relative tolerances of the FP32 method's inner solves (Newton-step PCG,
-    // continuity BiCGSTAB): the Gummel loop's own stopping rule sets the
-    // accuracy of a converged point, and an inexact solve only perturbs an
-    // iteration by a fraction of its own update
-    double pcg_rtol32 = 1e-2, bicg_rtol32 = 1e-3;
+    // relative tolerances of the FP32 method's inner solves: the Newton-step
+    // PCG (the Newton iteration corrects an inexact step), and the continuity
+    // BiCGSTAB while the Gummel update exceeds loose_r32 (V); closer to
+    // convergence the continuity solves return to bicg_rtol, which sets how
+    // closely a converged point's currents approach the exact discrete
+    // solution (measured on the pn diode at 0.5 V: 1e-3 to the end left the
+    // current 4 floors from it, 1e-4 one floor)
+    double pcg_rtol32 = 1e-2, bicg_rtol32 = 1e-3, loose_r32 = 1e-6;
+    double gummel_r = 0.;              // the running Gummel loop's last update max(|dpsi|, |dphi|) (V)
     int converged = 0, conv_iter = 0;
     double conv_res = 0.;
     int allocated = 0, have_solution = 0, dirty = 1, verbose = 0, nthreads = 1;
@@ -153,6 +158,7 @@ struct Engine {
     double mg_alpha = 1.1;
     double mg_alpha_p = 1.3;
     int mg_bf16 = 1;                          // FP32 method: BF16 level-0 smoother coefficients
+    int mg_bf16_l1 = 1;                       // and level 1's (packed records; SEMISIM_MG_BF16_L1)
     int mg_perm = 1;                          // line-smoothed levels stored parity-split along x (SEMISIM_MG_PERM)
     // learned warm start: mode, saved prediction of the running bias point
     int ws_mode = 0, ws_have = 0;

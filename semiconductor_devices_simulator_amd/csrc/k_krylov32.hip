@@ -34,6 +34,9 @@ __device__ __forceinline__ int k32_xidx(const L0Geo &g, int p) {
 // y = A x on the level-0 storage with the FP32 operator (unit diagonal not
 // read when UD); slot <- y.(u0, or y when u0 is null), slot+1 <- y.u1 (NV 2).
 // Loop over storage positions; NodeWalk's i is the position within the row.
+// Two nodes per iteration, both rows' loads issued before either is summed;
+// the operator is streamed with non-temporal loads (read once per product,
+// so it need not displace the multigrid's data from the caches).
 template <bool UD, int NV>
 __global__ void __launch_bounds__(SD_BLOCK) k32_spmv_dots(L0Geo g, Sten7<float> A, const float *__restrict__ x,
                                                           float *__restrict__ y, const float *__restrict__ u0,
@@ -42,26 +45,46 @@ __global__ void __launch_bounds__(SD_BLOCK) k32_spmv_dots(L0Geo g, Sten7<float> 
     double acc[NV] = {};
     int k0, st, k1;
     sd_node_loop(N, k0, st, k1);
-    NodeWalk nw(k0, st, Nx, Ny);
-    for (int q = k0; q < k1; q += st, nw.next()) {
-        const int i = k32_xidx(g, nw.i), r0 = q - nw.i, j = nw.j, l = nw.l;
+    auto ld = [](const float *p, int q) -> float { return __builtin_nontemporal_load(p + q); };
+    auto row = [&](int q, const NodeWalk &w) -> float {
+        const int i = k32_xidx(g, w.i), r0 = q - w.i, j = w.j, l = w.l;
         const bool h0 = i > 0, h1 = i < Nx - 1, h2 = j > 0, h3 = j < Ny - 1, h4 = l > 0, h5 = l < Nz - 1;
         const float xq = x[q];
-        const float a0 = A.xm[q], a1 = A.xp[q], a2 = A.ym[q], a3 = A.yp[q], a4 = A.zm[q], a5 = A.zp[q];
+        const float a0 = ld(A.xm, q), a1 = ld(A.xp, q), a2 = ld(A.ym, q), a3 = ld(A.yp, q);
+        const float a4 = ld(A.zm, q), a5 = ld(A.zp, q);
         const float x0 = x[k32_nb(h0, r0 + k32_xpos(g, i - 1), q)], x1 = x[k32_nb(h1, r0 + k32_xpos(g, i + 1), q)];
         const float x2 = x[k32_nb(h2, q - Nx, q)], x3 = x[k32_nb(h3, q + Nx, q)];
         const float x4 = x[k32_nb(h4, q - NxNy, q)], x5 = x[k32_nb(h5, q + NxNy, q)];
-        float s = UD ? xq : A.c[q] * xq;
-        if (h0) s += a0 * x0;
-        if (h1) s += a1 * x1;
-        if (h2) s += a2 * x2;
-        if (h3) s += a3 * x3;
-        if (h4) s += a4 * x4;
-        if (h5) s += a5 * x5;
-        y[q] = s;
-        const double sd = s;
-        acc[0] += sd * (u0 ? (double)u0[q] : sd);
-        if constexpr (NV == 2) acc[1] += sd * (double)u1[q];
+        float s = UD ? xq : ld(A.c, q) * xq;
+        s = h0 ? s + a0 * x0 : s;
+        s = h1 ? s + a1 * x1 : s;
+        s = h2 ? s + a2 * x2 : s;
+        s = h3 ? s + a3 * x3 : s;
+        s = h4 ? s + a4 * x4 : s;
+        s = h5 ? s + a5 * x5 : s;
+        return s;
+    };
+    NodeWalk nw(k0, st, Nx, Ny);
+    for (int q = k0; q < k1; q += 2 * st) {
+        const int qb = q + st;
+        NodeWalk nb = nw;
+        nb.next();
+        const bool hb = qb < k1;
+        const int qc = hb ? qb : q;
+        const float ua = u0 ? u0[q] : 0.f, ub = u0 ? u0[qc] : 0.f;
+        const float va = NV == 2 ? u1[q] : 0.f, vb = NV == 2 ? u1[qc] : 0.f;
+        const float sa = row(q, nw);
+        const float sb = hb ? row(qb, nb) : 0.f;
+        y[q] = sa;
+        acc[0] += (double)sa * (u0 ? (double)ua : (double)sa);
+        if constexpr (NV == 2) acc[1] += (double)sa * (double)va;
+        if (hb) {
+            y[qb] = sb;
+            acc[0] += (double)sb * (u0 ? (double)ub : (double)sb);
+            if constexpr (NV == 2) acc[1] += (double)sb * (double)vb;
+        }
+        nb.next();
+        nw = nb;
     }
     red_partial<NV, false>(acc, rb);
 }

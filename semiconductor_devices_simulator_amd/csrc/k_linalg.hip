@@ -785,10 +785,23 @@ __device__ __forceinline__ LineGeom line_geom(int Nx, int Ny, int Nz, int ax) {
     return g;
 }
 // One lane per line (both colours), sequential over the line.  The
-// coefficients of LF_CHUNK consecutive elements are loaded before any of them
-// is used, so a line waits for memory once per chunk rather than once per
-// element; 64-lane workgroups spread the few thousand lines over all CUs.
+// coefficients of the next LF_CHUNK elements are loaded while the current
+// chunk is factored, so a line waits for memory about once (the first chunk)
+// rather than once per chunk; 64-lane workgroups spread the few thousand
+// lines over all CUs.  The FP64 hierarchy factors in FP64 with IEEE division;
+// the FP32 hierarchy in FP32 with the hardware reciprocal refined by one
+// Newton step (the factors are stored in FP32, and BF16 on level 0, either
+// way): the dependent chain along a line is then a few single-precision
+// operations per element instead of an FP64 division.
 #define LF_CHUNK 16
+template <class T> __device__ __forceinline__ T lf_recip(T w) {
+    if constexpr (std::is_same_v<T, float>) {
+        float r = __builtin_amdgcn_rcpf(w);
+        return r * (2.0f - w * r);
+    } else {
+        return 1.0 / w;
+    }
+}
 template <class T>
 __global__ void __launch_bounds__(64) k_line_factor(MGLevel<T> v, int ax) {
     LineGeom g = line_geom(v.Nx, v.Ny, v.Nz, ax);
@@ -802,32 +815,43 @@ __global__ void __launch_bounds__(64) k_line_factor(MGLevel<T> v, int ax) {
     auto node = [&](int q) {
         return ax == 0 ? mg_idx(v, q, td, p3) : (ax == 1 ? mg_idx(v, td, q, p3) : mg_idx(v, td, p3, q));
     };
-    double cprev = 0.0;
-    for (int q0 = 0; q0 < g.n; q0 += LF_CHUNK) {
-        T dv[LF_CHUNK], av[LF_CHUNK], cv[LF_CHUNK];
-        int kk[LF_CHUNK];
+    T dv[LF_CHUNK], av[LF_CHUNK], cv[LF_CHUNK];
+    int kk[LF_CHUNK];
+    auto load = [&](int q0, T (&d)[LF_CHUNK], T (&a)[LF_CHUNK], T (&c)[LF_CHUNK], int (&kq)[LF_CHUNK]) {
 #pragma unroll
         for (int u = 0; u < LF_CHUNK; u++) {
             const int q = q0 + u < g.n ? q0 + u : g.n - 1;
             const int k = node(q);
-            kk[u] = k;
-            dv[u] = v.A.c ? v.A.c[k] : (T)1;
-            av[u] = am[k];
-            cv[u] = ap[k];
+            kq[u] = k;
+            d[u] = v.A.c ? v.A.c[k] : (T)1;
+            a[u] = am[k];
+            c[u] = ap[k];
         }
+    };
+    load(0, dv, av, cv, kk);
+    T cprev = 0;
+    for (int q0 = 0; q0 < g.n; q0 += LF_CHUNK) {
+        T dn[LF_CHUNK], an[LF_CHUNK], cn[LF_CHUNK];
+        int kn[LF_CHUNK];
+        const bool more = q0 + LF_CHUNK < g.n;
+        if (more) load(q0 + LF_CHUNK, dn, an, cn, kn);
 #pragma unroll
         for (int u = 0; u < LF_CHUNK; u++) {
             const int q = q0 + u;
             if (q >= g.n) break;
-            double dg = (double)dv[u];
-            if (!(dg != 0.0)) dg = 1.0;
-            double a = q > 0 ? (double)av[u] : 0.0, c = q < g.n - 1 ? (double)cv[u] : 0.0;
-            double w = dg - a * cprev;
-            if (!(w != 0.0)) w = dg;
-            double iwq = 1.0 / w;
+            T dg = dv[u];
+            if (!(dg != (T)0)) dg = 1;
+            const T a = q > 0 ? av[u] : (T)0, c = q < g.n - 1 ? cv[u] : (T)0;
+            T w = dg - a * cprev;
+            if (!(w != (T)0)) w = dg;
+            const T iwq = lf_recip<T>(w);
             cprev = c * iwq;
             const int k = kk[u];
-            ai[k] = (T)(a * iwq); iw[k] = (T)iwq; cp[k] = (T)cprev;
+            ai[k] = a * iwq; iw[k] = iwq; cp[k] = cprev;
+        }
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < LF_CHUNK; u++) { dv[u] = dn[u]; av[u] = an[u]; cv[u] = cn[u]; kk[u] = kn[u]; }
         }
     }
 }
