@@ -19,9 +19,13 @@ nodes; Id-Vg 0 -> 1.2 V in 0.1 V steps at Vd = 0.05 V), FP64 aggregation
 multigrid, a supplementary line whose roofline is the graph Krylov SpMV.
 
 Up to three solver configurations run back to back on the same sweep: the
-plain FP64 method (FP64 multigrid), the mixed-precision method (FP32
-multigrid, smoothers and preconditioner SpMV inside FP64 Krylov/Newton), and
-the mixed-precision method seeded by the learned warm start when the trained
+plain FP64 method (FP64 multigrid inside FP64 PCG/BiCGSTAB at the reference's
+tolerances), the mixed-precision method (FP32 multigrid; each linear solve is
+FP64 defect correction around an FP32 Krylov solve whose products run on the
+FP32 level-0 operator, with FP64 residuals and FP64-accumulated dots; its own
+inner tolerances: Newton-step PCG 1e-2, continuity BiCGSTAB 1e-3 while the
+Gummel update exceeds 1e-6 V and the reference's 1e-4 after), and the
+mixed-precision method seeded by the learned warm start when the trained
 network (semiconductor_devices_simulator_amd/warmstart_unet64.npz) is present.
 `value` is the learned method's rate when its warm start kept corrections and
 beat the mixed-precision method alone, else the mixed-precision rate; every
@@ -394,7 +398,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (Krylov, Newton, assembly) + f32 multigrid",
+        "dtype": "f64 (residuals, dots, Newton, assembly, Bernoulli) + f32 (multigrid, inner Krylov vectors)",
         "data": "synthetic",
         "config": {"workload": workload, "nodes": spec.N, "mesh": list(spec.shape),
                    "timed_points": [[round(pts[i][2].get(D if a.workload == "mosfet" else 2, 0.0), 4),

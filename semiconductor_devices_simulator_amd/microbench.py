@@ -18,6 +18,13 @@ rocprofv3 FETCH_SIZE / WRITE_SIZE counters), precond (the preconditioner as
 the Krylov solvers call it: precision conversions and the V-cycle, replayed
 from its captured HIP graph), roundtrip (a dot product whose value the host
 reads back, host wall clock per round trip).
+
+With --sweep it instead times the workload's whole Id-Vg sweep in one
+method (no kernel launches of its own), the run whose kernel trace
+(rocprofv3 --kernel-trace) tracestat.py summarises: the kernels' busy time
+against this wall time says how much of the sweep the GPU idles.
+
+    python -m semiconductor_devices_simulator_amd.microbench --mesh 192 128 128 --precision 1 --sweep
 """
 import argparse
 import os
@@ -50,6 +57,7 @@ def main(argv=None):
     ap.add_argument("--precision", type=int, default=1)
     ap.add_argument("--kernel", default="line_y", choices=KERNELS + ["all"])
     ap.add_argument("--iters", type=int, default=200)
+    ap.add_argument("--sweep", action="store_true", help="time the whole Id-Vg sweep in one method")
     ap.add_argument("--tune", action="append", default=[],
                     help="WHAT=VALUE passed to sdx_tune before the solve (4=0: the k_line_gs line kernel)")
     a = ap.parse_args(argv)
@@ -62,6 +70,8 @@ def main(argv=None):
     lib.sdx_set_precision(a.precision)
     if not lib.api3_solve():
         raise RuntimeError("equilibrium solve failed")
+    if a.sweep:
+        return time_sweep(lib, spec, cids, a)
     for c, v in ((1, 1.0), (2, 0.6)):
         lib.api3_set_contact_V(cids[c], v)
     if not lib.api3_resolve():
@@ -73,6 +83,23 @@ def main(argv=None):
         out = dict(kernel=name, mesh=a.mesh, precision=a.precision, us_per_launch=round(ms * 1e3, 3),
                    bytes_per_launch=by, GBps=None if by is None else round(by / ms / 1e6, 1))
         print(json.dumps(out), flush=True)
+    return 0
+
+
+def time_sweep(lib, spec, cids, a):
+    """The whole sweep of the workload device, from its equilibrium."""
+    import time
+    import numpy as np
+    from . import sweep
+    g0 = [lib.api3_get_stat(q) for q in range(4)]
+    t0 = time.perf_counter()
+    R = sweep.run_sweep(lib, spec, cids)
+    dt = time.perf_counter() - t0
+    its = [int(lib.api3_get_stat(q) - g) for q, g in enumerate(g0)]
+    out = dict(sweep=spec.name, mesh=a.mesh, precision=a.precision, points=len(R["V"]),
+               converged=int(np.sum(R["conv"])), seconds=round(dt, 3), points_per_s=round(len(R["V"]) / dt, 4),
+               iterations=dict(zip(("pcg", "bicgstab", "newton", "gummel"), its)))
+    print(json.dumps(out), flush=True)
     return 0
 
 

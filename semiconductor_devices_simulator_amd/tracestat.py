@@ -1,0 +1,72 @@
+"""Summary of a rocprofv3 kernel trace (--kernel-trace --output-format csv).
+
+Prints the window the kernels span, the time at least one kernel ran (busy),
+the gaps between kernels, and the kernels by total time.  --from NAME starts
+the window at the first launch of a kernel whose name starts with NAME (the
+sweep's first continuity assembly, k_cont, skips the set-up and the
+equilibrium solve).
+
+    python -m semiconductor_devices_simulator_amd.tracestat kernel_trace.csv --from k_cont --top 40
+"""
+import argparse
+import csv
+import sys
+from collections import defaultdict
+
+
+def load(path):
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+    return rows
+
+
+def summarise(rows, start=None):
+    if start:
+        t = min((s for s, _, n in rows if n.startswith(start)), default=None)
+        if t is None:
+            raise SystemExit(f"no kernel named {start}*")
+        rows = [r for r in rows if r[0] >= t]
+    busy, gaps = 0, []
+    cur_s = cur_e = None
+    for s, e, _ in rows:
+        if cur_e is None:
+            cur_s, cur_e = s, e
+        elif s > cur_e:
+            busy += cur_e - cur_s
+            gaps.append(s - cur_e)
+            cur_s, cur_e = s, e
+        else:
+            cur_e = max(cur_e, e)
+    busy += cur_e - cur_s
+    per = defaultdict(lambda: [0, 0])
+    for s, e, n in rows:
+        k = n.split("(")[0]
+        per[k][0] += 1
+        per[k][1] += e - s
+    return dict(window=cur_e - rows[0][0], busy=busy, launches=len(rows), gaps=sorted(gaps), per=per)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("trace")
+    ap.add_argument("--from", dest="start", default=None)
+    ap.add_argument("--top", type=int, default=30)
+    a = ap.parse_args(argv)
+    S = summarise(load(a.trace), a.start)
+    g = S["gaps"]
+    med = g[len(g) // 2] / 1e3 if g else 0.0
+    print(f"window {S['window'] / 1e9:.3f} s, kernels busy {S['busy'] / 1e9:.3f} s "
+          f"({100 * S['busy'] / S['window']:.1f}%), {S['launches']} launches, {len(g)} gaps "
+          f"(median {med:.2f} us, total {sum(g) / 1e9:.3f} s)")
+    print("| kernel | launches | avg us | % of busy |")
+    print("|---|---:|---:|---:|")
+    for k, (c, t) in sorted(S["per"].items(), key=lambda x: -x[1][1])[:a.top]:
+        print(f"| `{k[:80]}` | {c} | {t / c / 1e3:.2f} | {100 * t / S['busy']:.1f} |")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -84,3 +84,21 @@ def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU only" in r.stderr
+
+
+def test_trace_summary_busy_time_and_window(tmp_path):
+    """tracestat: the busy time is the union of the kernels' intervals (an
+    overlap counts once), gaps are the idle stretches between them, and the
+    window can start at a named kernel."""
+    from semiconductor_devices_simulator_amd import tracestat
+    p = tmp_path / "kernel_trace.csv"
+    rows = [("k_setup", 0, 100), ("k_cont", 1000, 1400), ("k_a", 1300, 1500), ("k_b", 2500, 2600)]
+    with open(p, "w") as f:
+        f.write('"Kernel_Name","Start_Timestamp","End_Timestamp"\n')
+        for n, s, e in rows:
+            f.write(f'"{n}(int)",{s},{e}\n')
+    S = tracestat.summarise(tracestat.load(str(p)))
+    assert S["window"] == 2600 and S["busy"] == 100 + 500 + 100 and S["gaps"] == [900, 1000]
+    S = tracestat.summarise(tracestat.load(str(p)), "k_cont")
+    assert S["window"] == 1600 and S["busy"] == 600 and S["launches"] == 3
+    assert S["per"]["k_a"] == [1, 200]
