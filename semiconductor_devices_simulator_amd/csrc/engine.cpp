@@ -91,12 +91,6 @@ static double S(int slot) { return G.h_scal[slot]; }
 // Captured preconditioner launch sequences (see precond): they bake in the
 // hierarchy's pointers, shapes, smoother choices and the coarse-correction
 // scale of the operator kind, so every change of those drops them.
-struct VGraph { int prec, off, poisson, sin, sout, second; const double *r; double *z; hipGraphExec_t ex; };
-static std::vector<VGraph> g_vgraphs;
-static void vgraph_clear() {
-    for (auto &v : g_vgraphs) hipGraphExecDestroy(v.ex);
-    g_vgraphs.clear();
-}
 
 template <class T> static void hier_free(Hier<T> &H) {
     H.lev.clear(); H.inv = nullptr; H.alloc = false;
@@ -110,7 +104,6 @@ template <class T> static void hier_free(Hier<T> &H) {
 // solve) keep the natural order.
 template <class T> static void hier_layout(Hier<T> &H) {
     const int nl = (int)H.lev.size();
-    bool changed = false;
     for (int L = 0; L < nl; L++) {
         MGLevel<T> &v = H.lev[L];
         int px = G.mg_perm && G.n_axes > 0 && L < G.line_levels && L < nl - 1 && (H.tail < 0 || L < H.tail);
@@ -118,14 +111,12 @@ template <class T> static void hier_layout(Hier<T> &H) {
             int len = G.mg_axes[a] == 0 ? v.Nx : (G.mg_axes[a] == 1 ? v.Ny : v.Nz);
             if (len > 256) px = 0;
         }
-        if (v.px != px) { v.px = px; changed = true; }
+        v.px = px;
     }
-    if (changed) vgraph_clear();
 }
 
 void eng_free() {
     if (G.st) hipStreamSynchronize(G.st);
-    vgraph_clear();
     prep_free();
     for (void *p : G.dev_allocs) hipFree(p);
     G.dev_allocs.clear();
@@ -156,7 +147,6 @@ void eng_free() {
 // the coarse end (levels of at most G.tail_n nodes, not level 0) runs as one
 // single-workgroup kernel; its level descriptors are copied to the device
 template <class T> static void hier_tail(Hier<T> &H) {
-    vgraph_clear();
     H.tail = -1;
     int nl = (int)H.lev.size();
     for (int L = 1; L < nl; L++) if (H.lev[L].N <= G.tail_n) { H.tail = L; break; }
@@ -796,9 +786,6 @@ int eng_prepare() {
         else if (mode && !strcmp(mode, "lines1")) want = 1;
         int maxlen = std::max(G.Nx, std::max(G.Ny, G.Nz));
         if (maxlen > 1024) want = 0;
-        int same = G.n_axes == want;
-        for (int a = 0; a < want; a++) same = same && G.mg_axes[a] == ord[3 - want + a];
-        if (!same) vgraph_clear();
         G.n_axes = want;
         for (int a = 0; a < want; a++) G.mg_axes[a] = ord[3 - want + a];
     }
@@ -911,12 +898,10 @@ static void level0_bf16(Hier<float> &H) {
         H.r16 = dalloc<uint4>((size_t)3 * v0.N);
         if (g_err) { H.c16 = nullptr; H.r16 = nullptr; g_err = 0; G.mg_bf16 = 0; return level0_bf16(H); }
     }
-    const bool was = v0.o16[0] != nullptr;
     for (int q = 0; q < 6; q++) v0.o16[q] = on ? H.c16 + (size_t)q * v0.N : nullptr;
     for (int a = 0; a < 3; a++)
         for (int q = 0; q < 3; q++) v0.lu16[a][q] = on ? H.c16 + (size_t)(6 + 3 * a + q) * v0.N : nullptr;
     for (int a = 0; a < 3; a++) v0.rec16[a] = nullptr;
-    if (was != on) vgraph_clear();
     if (!on) return;
     // the separate BF16 arrays are read only by the sweeps that do not take
     // the packed records: the generic line kernel (several lines per slot,
@@ -1173,57 +1158,18 @@ static void precond_launch(const double *r, double *z, bool sin = false, bool so
         else vcycle_T<double>(0, r, z);
     }
 }
-// The V-cycle is 20-40 short kernels whose launch sequence is fixed between
-// hierarchy changes.  It is captured into a HIP graph once per (precision,
-// smoother-only mode, operator kind, staging, r, z) and replayed with one
-// hipGraphLaunch, so the host no longer paces the GPU kernel by kernel: on
-// the coarse levels the kernels take about as long as a launch costs the
-// host.  The graph holds kernels only (no memset or copy nodes).  With the
-// level-0 smoother timer on, every n-th V-cycle runs as plain launches
-// bracketed by HIP events.
+// The preconditioner as the Krylov solvers call it.  With the level-0
+// smoother timer on, every n-th V-cycle has its level-0 sweeps bracketed by
+// HIP events; kind 2 times whole applications.
 static void precond(const double *r, double *z, bool sin = false, bool sout = false, bool second = false) {
     if (G.graph) { gr_precond(r, z); return; }
     G.n_vcycles++;
-    bool sample = KT.on && KT.kind == 0 && (KT.vcycles++ % KT.every) == 0;
-    const bool whole_timed = KT.on && KT.kind == 2;
-    if (!G.use_graphs || sample || whole_timed) {
-        KT.sampling = sample;
-        const bool whole = kt_begin(2);          // kind 2: the whole preconditioner application
-        precond_launch(r, z, sin, sout, second);
-        kt_end(whole);
-        KT.sampling = false;
-        return;
-    }
-    const int poisson = G.mg_qsign == 0, si = sin, so = sout, se = second;
-    // staged input/output live in the hierarchy's level-0 arrays: r and z unused
-    const double *rk = sin ? nullptr : r;
-    double *zk = sout ? nullptr : z;
-    for (auto &v : g_vgraphs)
-        if (v.prec == mg_prec() && v.off == G.mg_off && v.poisson == poisson && v.sin == si && v.sout == so &&
-            v.second == se && v.r == rk && v.z == zk) {
-            HC(hipGraphLaunch(v.ex, G.st));
-            return;
-        }
-    hipGraph_t gr = nullptr;
-    hipGraphExec_t ex = nullptr;
-    hipError_t e = hipStreamBeginCapture(G.st, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-        precond_launch(r, z, sin, sout, second);
-        e = hipStreamEndCapture(G.st, &gr);
-    }
-    if (e == hipSuccess && gr) e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
-    if (gr) hipGraphDestroy(gr);
-    if (e != hipSuccess || !ex) {
-        // not a numerical failure: report it and keep plain launches
-        fprintf(stderr, "semisim-hip: V-cycle graph capture failed (%s); using plain launches\n",
-                hipGetErrorString(e));
-        (void)hipGetLastError();
-        G.use_graphs = 0;
-        precond_launch(r, z, sin, sout, second);
-        return;
-    }
-    g_vgraphs.push_back(VGraph{mg_prec(), G.mg_off, poisson, si, so, se, rk, zk, ex});
-    HC(hipGraphLaunch(ex, G.st));
+    const bool sample = KT.on && KT.kind == 0 && (KT.vcycles++ % KT.every) == 0;
+    KT.sampling = sample;
+    const bool whole = kt_begin(2);          // kind 2: the whole preconditioner application
+    precond_launch(r, z, sin, sout, second);
+    kt_end(whole);
+    KT.sampling = false;
 }
 static inline void matvec(const double *x, double *y) {
     G.n_spmv++;
@@ -2501,7 +2447,6 @@ void eng_read_env() {
     if ((e = getenv("SEMISIM_MG_COARSEAXES"))) { int q = atoi(e); G.coarse_axes = q > 0 ? q : 1; }
     if ((e = getenv("SEMISIM_MG_ALPHA_P"))) { double q = atof(e); G.mg_alpha_p = q > 0. ? q : 1.; }
     if ((e = getenv("SEMISIM_MG_ALPHA"))) { double q = atof(e); G.mg_alpha = q > 0. ? q : 1.; }
-    G.use_graphs = (e = getenv("SEMISIM_GRAPHS")) ? atoi(e) != 0 : 0;
     G.inner32 = (e = getenv("SEMISIM_INNER32")) ? atoi(e) != 0 : 1;
     G.mg_perm = (e = getenv("SEMISIM_MG_PERM")) ? atoi(e) != 0 : 1;
     G.mg_bf16 = (e = getenv("SEMISIM_MG_BF16")) ? atoi(e) != 0 : G.mg_bf16;
@@ -2619,7 +2564,6 @@ double sdx_microbench(int what, int iters) {
     return (double)ms / iters;
 }
 void sdx_tune(int what, int value) {
-    vgraph_clear();
     if (what == 0) g_line_nl_max = value;
     if (what == 4) g_line_scan = value;
     if (what == 5) g_line_slots = value >= 4 ? 4 : (value >= 2 ? 2 : 1);
@@ -2639,7 +2583,6 @@ void sdx_tune(int what, int value) {
     if (what == 22) g_rb_fuse_post = value != 0;
     if (what == 23) G.stage_read = value != 0;
     if (what == 24) g_rec16 = value != 0;
-    if (what == 25) G.use_graphs = value != 0;
     if (what == 26) { G.pred2 = value != 0; G.e3_ok = 0; }
     if (what == 28) G.mg_bf16_l1 = value != 0;
     if (what == 27) G.inner32 = value >= 0 && value <= 2 ? value : 1;

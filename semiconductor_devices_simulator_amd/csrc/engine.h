@@ -145,7 +145,6 @@ struct Engine {
     int tail_sweeps = 1;                      // red-black sweeps per side in the tail kernel
     int coarse_axes = 3;                      // axes line-smoothed on the coarse line levels (strongest)
     int tail_n = 1024;                        // levels of at most this many nodes run in the tail kernel
-    int use_graphs = 0;                       // replay the captured V-cycle (SEMISIM_GRAPHS=1)
     int bi_fuse = 1;                          // BiCGSTAB: rh.r computed with the x/r update
     int unit_diag = 0;                        // the operator's diagonal is 1 (scaled continuity rows):
                                               // the Krylov products skip reading it

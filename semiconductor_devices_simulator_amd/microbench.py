@@ -15,8 +15,7 @@ Gauss-Seidel sweep), rbgs (one colour of a point red-black sweep), spmv
 assembly), cont (continuity assembly), copy_big (8M doubles, 64 MB each way:
 larger than the L2 of all XCDs together, the calibration reference of the
 rocprofv3 FETCH_SIZE / WRITE_SIZE counters), precond (the preconditioner as
-the Krylov solvers call it: precision conversions and the V-cycle, replayed
-from its captured HIP graph), roundtrip (a dot product whose value the host
+the FP64 Krylov solvers call it: precision conversions and the V-cycle), roundtrip (a dot product whose value the host
 reads back, host wall clock per round trip).
 
 With --sweep it instead times the workload's whole Id-Vg sweep in one

@@ -103,31 +103,3 @@ def test_packed_coefficient_records_are_bitwise(hip, solved):
     assert np.all(np.isfinite(out[1])) and np.max(np.abs(out[1])) > 0
     np.testing.assert_array_equal(out[1], out[0])
 
-
-@pytest.mark.parametrize("precision", [1, 0])
-def test_graph_replays_match_plain_launches(hip, solved, precision):
-    """V-cycles replayed from a captured HIP graph (tune 25) run the same
-    kernels in the same order as plain launches: every replay, with uploads,
-    downloads and host synchronisation in between (sdx_test_precond), gives
-    the plain launches' result bit for bit."""
-    N = solved
-    hip.sdx_set_precision(precision)
-    assert hip.sdx_test_mg_setup() >= 4
-    rs = [np.random.default_rng(11 + q).standard_normal(N) for q in range(4)]
-    def pre(r):                   # no tune calls in between: they drop the captured graphs
-        z = np.zeros(N)
-        hip.sdx_test_precond(dptr(r), dptr(z))
-        return z
-
-    _vcycle(hip, N, rs[0], {})    # the default switches
-    hip.sdx_tune(25, 0)
-    plain = [pre(r) for r in rs]
-    hip.sdx_tune(25, 1)
-    try:
-        for rep in range(2):
-            for r, want in zip(rs, plain):
-                got = pre(r)
-                assert np.all(np.isfinite(got)) and np.max(np.abs(got)) > 0
-                np.testing.assert_array_equal(got, want, err_msg=f"replay {rep}")
-    finally:
-        hip.sdx_tune(25, 0)
