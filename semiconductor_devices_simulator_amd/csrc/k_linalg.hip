@@ -1829,7 +1829,10 @@ template <class T> void mg_prolong(hipStream_t s, const MGLevel<T> &f, const MGL
 // Coarsest level (<= 64 unknowns): dense LU with partial pivoting in LDS
 // (semiconductor_3d.c:2050-2081), then the explicit inverse, one column per
 // lane.  A zero pivot is replaced by 1e-300 as in the reference.
-template <class T>
+// NM: the coarsest level's size rounded up to 16, 32 or 64.  Every loop over
+// the rows runs to NM (rows past n do nothing), so the solution column y of
+// each lane is indexed by constants after unrolling and stays in registers.
+template <class T, int NM>
 __global__ void __launch_bounds__(64) k_coarse_factor(MGLevel<T> v, double *inv) {
     __shared__ double M[64][65];
     __shared__ int piv[64];
@@ -1869,16 +1872,49 @@ __global__ void __launch_bounds__(64) k_coarse_factor(MGLevel<T> v, double *inv)
     }
     // column t of the inverse: solve LU y = P e_t
     if (t < n) {
-        double y[64];
-        for (int r = 0; r < n; r++) y[r] = (r == t) ? 1.0 : 0.0;
-        for (int c = 0; c < n; c++) { int p = piv[c]; if (p != c) { double tt = y[c]; y[c] = y[p]; y[p] = tt; } }
-        for (int r = 0; r < n; r++) { double s = y[r]; for (int q = 0; q < r; q++) s -= M[r][q] * y[q]; y[r] = s; }
-        for (int r = n - 1; r >= 0; r--) { double s = y[r]; for (int q = r + 1; q < n; q++) s -= M[r][q] * y[q]; y[r] = s / M[r][r]; }
-        for (int r = 0; r < n; r++) inv[r * n + t] = y[r];
+        double y[NM];
+#pragma unroll
+        for (int r = 0; r < NM; r++) y[r] = (r == t) ? 1.0 : 0.0;
+#pragma unroll
+        for (int c = 0; c < NM; c++) {
+            if (c < n) {
+                const int p = piv[c];
+                double yc = y[c], yp = 0.0;
+#pragma unroll
+                for (int r = c + 1; r < NM; r++) if (r == p) yp = y[r];
+                if (p != c) {
+#pragma unroll
+                    for (int r = c + 1; r < NM; r++) if (r == p) y[r] = yc;
+                    y[c] = yp;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NM; r++) {
+            if (r < n) {
+                double s = y[r];
+#pragma unroll
+                for (int q = 0; q < r; q++) s -= M[r][q] * y[q];
+                y[r] = s;
+            }
+        }
+#pragma unroll
+        for (int r = NM - 1; r >= 0; r--) {
+            if (r < n) {
+                double s = y[r];
+#pragma unroll
+                for (int q = r + 1; q < NM; q++) if (q < n) s -= M[r][q] * y[q];
+                y[r] = s / M[r][r];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NM; r++) if (r < n) inv[r * n + t] = y[r];
     }
 }
 template <class T> void mg_coarse_factor(hipStream_t s, const MGLevel<T> &v, double *inv) {
-    k_coarse_factor<T><<<1, 64, 0, s>>>(v, inv);
+    if (v.N <= 16) k_coarse_factor<T, 16><<<1, 64, 0, s>>>(v, inv);
+    else if (v.N <= 32) k_coarse_factor<T, 32><<<1, 64, 0, s>>>(v, inv);
+    else k_coarse_factor<T, 64><<<1, 64, 0, s>>>(v, inv);
 }
 template <class T>
 __global__ void __launch_bounds__(64) k_coarse_solve(int n, const double *inv, const T *b, T *x) {
