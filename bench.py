@@ -332,8 +332,12 @@ def main():
     a.mesh = a.mesh or DEFAULT_MESH[a.workload]
     if a.workload == "prism":
         return main_prism(a, lib)
-    if a.workload == "vdmos" and a.mesh == DEFAULT_MESH["vdmos"] and "--steps" not in sys.argv:
-        a.steps = 43          # the blocking sweep to 44 V: this mesh breaks down at 46 V
+    if a.workload == "vdmos" and a.mesh == DEFAULT_MESH["vdmos"]:
+        # the blocking sweep to 44 V: this mesh breaks down at 46 V
+        if a.sweep_points <= 0 or a.sweep_points > 45:
+            a.sweep_points = 45
+        if "--steps" not in sys.argv:
+            a.steps = 43
     spec = devices.mosfet(*a.mesh) if a.workload == "mosfet" else devices.vdmos(*a.mesh)
     if a.workload != "mosfet":          # the trained network and the CPU sample are MOSFET-only
         a.no_learned = a.no_cpu_baseline = True

@@ -997,6 +997,7 @@ struct KTimer {
     bool sampling = false;             // kind 0: the running V-cycle is timed
     long long count = 0, sampled = 0, vcycles = 0;
     double total_ms = 0.;
+    double bytes = 0.;                 // algorithmic bytes of the sampled launches (kind 0)
     std::vector<hipEvent_t> pool, pend;
     std::vector<int> pend_n;           // launches inside each bracketed pair of pend
 };
@@ -1029,9 +1030,10 @@ static bool kt_begin(int kind) {
     KT.pend.push_back(e);
     return true;
 }
-// n: kernel launches between the two events
-static void kt_end(bool started, int n = 1) {
+// n: kernel launches between the two events; by: their algorithmic bytes
+static void kt_end(bool started, int n = 1, double by = 0.) {
     if (!started) return;
+    KT.bytes += by;
     hipEvent_t e = kt_event();
     HC(hipEventRecord(e, G.st));
     KT.pend.push_back(e);
@@ -1048,6 +1050,16 @@ static void kt_end(bool started, int n = 1) {
 static inline int line_axis0(int L) {
     return L > 0 && G.coarse_axes < G.n_axes ? G.n_axes - G.coarse_axes : 0;
 }
+// Algorithmic bytes of one level-0 line sweep along axis ax (one colour, N/2
+// nodes): b, the other colour's x and the x written in T, and 7 coefficients
+// (3 line factors, 4 off-line couplings) in BF16 where the sweep reads the
+// BF16 copies (FP32 levels, lines of at most 128), else in T
+template <class T> static double sweep_bytes(const MGLevel<T> &v, int ax) {
+    const double w = sizeof(T);
+    const int n = ax == 0 ? v.Nx : (ax == 1 ? v.Ny : v.Nz);
+    const bool h16 = sizeof(T) == 4 && (n + 63) / 64 <= 2 && (v.rec16[ax] || v.o16[0]);
+    return 0.5 * v.N * (3.0 * w + 7.0 * (h16 ? 2.0 : w));
+}
 template <class T> static int smooth_pre(const MGLevel<T> &v, const T *b, T *x, int L = 0) {
     hipStream_t s = G.st;
     if (G.n_axes == 0 || L >= G.line_levels) {
@@ -1062,12 +1074,14 @@ template <class T> static int smooth_pre(const MGLevel<T> &v, const T *b, T *x, 
     // in an untimed V-cycle), the time divided over its launches
     int first = 1;
     bool t = v.N == G.N && kt_begin(0);
+    double by = 0.;
     for (int a = line_axis0(L); a < G.n_axes; a++)
         for (int color = 0; color < 2; color++) {
             mg_line(s, v, b, x, G.mg_axes[a], color, first);
+            by += sweep_bytes(v, G.mg_axes[a]);
             first = 0;
         }
-    kt_end(t, 2 * (G.n_axes - line_axis0(L)));
+    kt_end(t, 2 * (G.n_axes - line_axis0(L)), by);
     // rows of the last sweep's colour-1 lines have no residual
     return 2 + G.mg_axes[G.n_axes - 1];
 }
@@ -1089,12 +1103,14 @@ template <class T> static T *smooth_post(const MGLevel<T> &v, const T *b, T *x, 
         return x;
     }
     bool t = v.N == G.N && kt_begin(0);
+    double by = 0.;
     for (int a = G.n_axes - 1; a >= line_axis0(L); a--)
         for (int color = 1; color >= 0; color--) {
             mg_line(s, v, b, x, G.mg_axes[a], color, 0, c);
+            by += sweep_bytes(v, G.mg_axes[a]);
             c = nullptr;
         }
-    kt_end(t, 2 * (G.n_axes - line_axis0(L)));
+    kt_end(t, 2 * (G.n_axes - line_axis0(L)), by);
     return x;
 }
 // V-cycle from level L; returns where level L's result lies (x, or the
@@ -2619,7 +2635,7 @@ void sdx_ws_accept(double ratio) { G.ws_accept = ratio > 0. ? ratio : 0.5; }
 void sdx_ktimer_enable(int kind, int every) {
     kt_flush();
     KT.on = KT.armed = 1; KT.kind = kind; KT.every = every > 0 ? every : 8;
-    KT.count = KT.sampled = 0; KT.total_ms = 0.;
+    KT.count = KT.sampled = 0; KT.total_ms = 0.; KT.bytes = 0.;
 }
 // pause (1) or resume (0) the sampling without resetting what it gathered
 void sdx_ktimer_pause(int paused) {
@@ -2630,15 +2646,12 @@ void sdx_ktimer_pause(int paused) {
 int sdx_ktimer_read(double *out) {
     eng_sync();
     kt_flush();
-    double w = mg_prec() == 1 ? 4.0 : 8.0;
     double bytes;
     if (KT.kind == 0) {
-        // one colour of zebra line GS: for the lines' N/2 nodes 3 in-line
-        // coefficients, 4 off-line couplings, b, the x written, plus the other
-        // colour's N/2 x values read (point GS: 6 couplings + diag + b + x + x_nb);
-        // with BF16 coefficients (FP32 level 0) the 7 coefficients take 2 B each
-        const bool h16 = mg_prec() == 1 && !G.H32.lev.empty() && G.H32.lev[0].o16[0];
-        bytes = 0.5 * G.N * (h16 ? 3.0 * w + 7.0 * 2.0 : 10.0 * w);
+        // mean over the sampled level-0 sweeps of their algorithmic bytes,
+        // each counted when it ran (sweep_bytes: its precision, axis and
+        // coefficient storage)
+        bytes = KT.sampled > 0 ? KT.bytes / (double)KT.sampled : 0.;
     } else if (KT.kind == 2) {
         bytes = 0.;                              // a whole V-cycle: time only
     } else if (G.graph) {
