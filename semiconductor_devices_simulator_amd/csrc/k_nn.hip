@@ -27,84 +27,156 @@ __device__ __forceinline__ int nn_off(int G, int t, int x, int y, int z, int &u)
 
 // ---------------------------------------------------------------- forward
 // out[v][n] = bias[n] + sum_t sum_k W[t][n][k] in[v + off(t)][k], zero padded.
-// One wavefront computes 32 voxels x all N (N/32 tiles); four per workgroup.
+// One wavefront computes 64 voxels (two 32-voxel row tiles) x all N (NT =
+// N/32 column tiles, 1 or 2); four per workgroup.  Per tap and 16-channel
+// step a lane loads its two activation operands once and each weight operand
+// once, and issues 2 NT MFMAs: every operand load feeds two or more MFMAs.
 // relu: out = max(out, 0).  outf (optional): the same values in FP32.
+template <int NT>
 __global__ void __launch_bounds__(256) k_conv_fwd(int G, const __bf16 *__restrict__ in, int K,
                                                   const __bf16 *__restrict__ W, const float *__restrict__ bias, int N,
                                                   __bf16 *__restrict__ out, float *__restrict__ outf, int relu) {
     const int V = G * G * G;
     const int lane = threadIdx.x & 63, kh = lane >> 5, row = lane & 31;
-    const int v0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
+    const int v0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
     if (v0 >= V) return;                       // no barriers below
-    const int v = v0 + row;
-    const int x = v % G, y = (v / G) % G, z = v / (G * G);
-    const bool vin = v < V;                    // rows past the grid read zeros
+    int x[2], y[2], z[2];
+    bool vin[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int v = v0 + 32 * h + row;
+        vin[h] = v < V;                        // rows past the grid read zeros
+        x[h] = v % G; y[h] = (v / G) % G; z[h] = v / (G * G);
+    }
     const v8bf zero8 = {};
-    for (int nt = 0; nt < N / 32; nt++) {
-        v16f acc = {};
-        for (int t = 0; t < 27; t++) {
-            int u;
-            const int ok = vin && nn_off(G, t, x, y, z, u);
-            const __bf16 *ip = in + (size_t)u * K + kh * 8;
-            const __bf16 *wp = W + ((size_t)(t * N + nt * 32 + row)) * K + kh * 8;
-            for (int kc = 0; kc < K; kc += 16) {
-                const v8bf a = ok ? *(const v8bf *)(ip + kc) : zero8;
-                const v8bf b = *(const v8bf *)(wp + kc);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    v16f acc[2][NT];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) acc[h][nt] = v16f{};
+    for (int t = 0; t < 27; t++) {
+        int u[2], ok[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) ok[h] = vin[h] && nn_off(G, t, x[h], y[h], z[h], u[h]);
+        const __bf16 *ip0 = in + (size_t)u[0] * K + kh * 8, *ip1 = in + (size_t)u[1] * K + kh * 8;
+        const __bf16 *wp = W + ((size_t)(t * N + row)) * K + kh * 8;
+        for (int kc = 0; kc < K; kc += 16) {
+            const v8bf a0 = ok[0] ? *(const v8bf *)(ip0 + kc) : zero8;
+            const v8bf a1 = ok[1] ? *(const v8bf *)(ip1 + kc) : zero8;
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++) {
+                const v8bf b = *(const v8bf *)(wp + (size_t)nt * 32 * K + kc);
+                acc[0][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc[0][nt], 0, 0, 0);
+                acc[1][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc[1][nt], 0, 0, 0);
             }
         }
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
         const int n = nt * 32 + row;
         const float bn = bias ? bias[n] : 0.f;
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int i = 8 * (r >> 2) + 4 * kh + (r & 3);
-            if (v0 + i >= V) continue;         // grids of fewer than 32 voxels
-            float val = acc[r] + bn;
-            if (relu) val = fmaxf(val, 0.f);
-            out[(size_t)(v0 + i) * N + n] = (__bf16)val;
-            if (outf) outf[(size_t)(v0 + i) * N + n] = val;
-        }
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int i = 32 * h + 8 * (r >> 2) + 4 * kh + (r & 3);
+                if (v0 + i >= V) continue;         // grids of fewer than 64 voxels
+                float val = acc[h][nt][r] + bn;
+                if (relu) val = fmaxf(val, 0.f);
+                out[(size_t)(v0 + i) * N + n] = (__bf16)val;
+                if (outf) outf[(size_t)(v0 + i) * N + n] = val;
+            }
     }
 }
 void nn_conv_fwd(hipStream_t s, int G, const __bf16 *in, int K, const __bf16 *W, const float *bias, int N,
                  __bf16 *out, float *outf, int relu) {
-    const int V = G * G * G;
-    k_conv_fwd<<<((V + 31) / 32 + 3) / 4, 256, 0, s>>>(G, in, K, W, bias, N, out, outf, relu);
+    const int V = G * G * G, nb = ((V + 63) / 64 + 3) / 4;
+    if (N == 64) k_conv_fwd<2><<<nb, 256, 0, s>>>(G, in, K, W, bias, N, out, outf, relu);
+    else if (N == 32) k_conv_fwd<1><<<nb, 256, 0, s>>>(G, in, K, W, bias, N, out, outf, relu);
+    else fprintf(stderr, "semisim-hip nn: convolution with %d outputs (32 or 64 supported)\n", N);
 }
 
 // ---------------------------------------------------------------- weight gradient
-// part[c][t][n][k] = sum over the voxels of chunk c of dy[v][n] in[v + off(t)][k]:
-// rows n (32 per tile) x columns k (32 per tile) x 16 voxels per MFMA.  The
-// sum over chunks (fixed order) is k_wgrad_reduce.  One wavefront per
-// (t, n tile, k tile, chunk).  K may be 16 (the input layer): columns past K
-// read zeros and are not stored.
-__global__ void __launch_bounds__(64) k_conv_wgrad(int G, const __bf16 *__restrict__ dy, int N,
-                                                   const __bf16 *__restrict__ in, int K, int chunk,
-                                                   float *__restrict__ part) {
+// part[c][t][n][k] = sum over the voxels of chunk c of dy[v][n] in[v + off(t)][k].
+// As a GEMM the sum runs over voxels, but activations are channel-last, so a
+// lane's MFMA operand (one channel, eight voxels) is strided in memory.  A
+// workgroup (four wavefronts, one chunk and one tap t) therefore stages
+// 128-voxel blocks of dy and of the shifted input with coalesced 16-byte
+// loads and writes them to LDS transposed (channel-major, padded rows), where
+// each lane reads its eight voxels as one 16-byte load.  Output tiles of 32 x
+// 32 (N/32 x K/32 of them, at most four) are spread over the wavefronts; where
+// fewer tiles than wavefronts, the wavefronts of a tile split the block's
+// 16-voxel groups and sum their accumulators through LDS in a fixed order.
+// K = 16 (the input layer): the rows past K are zero and are not stored.
+// pb (optional): the centre tap's workgroups also sum dy over their chunk per
+// channel from the staged rows, the bias gradient's partials pb[c][n].
+#define WG_VB 128
+#define WG_PITCH (WG_VB + 8)
+__global__ void __launch_bounds__(256) k_conv_wgrad(int G, const __bf16 *__restrict__ dy, int N,
+                                                    const __bf16 *__restrict__ in, int K, int chunk,
+                                                    float *__restrict__ part, float *__restrict__ pb) {
+    extern __shared__ __align__(16) unsigned char wg_sm[];
+    __bf16 *s_dy = (__bf16 *)wg_sm;
+    const int Kp = (K + 31) / 32 * 32;
+    __bf16 *s_in = s_dy + (size_t)N * WG_PITCH;
     const int V = G * G * G;
-    const int lane = threadIdx.x, kh = lane >> 5, row = lane & 31;
-    const int nKt = (K + 31) / 32, nNt = N / 32;
-    int b = blockIdx.x;
-    const int kt = b % nKt; b /= nKt;
-    const int nt = b % nNt; b /= nNt;
-    const int t = b % 27; b /= 27;
-    const int c = b;                                  // chunk index
+    const int t = blockIdx.x % 27, c = blockIdx.x / 27;
     const int vb = c * chunk, ve = min(V, vb + chunk);
-    const int n = nt * 32 + row, k = kt * 32 + row;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, kh = lane >> 5, row = lane & 31;
+    const int nNt = N / 32, nKt = Kp / 32, ntiles = nNt * nKt;       // 1, 2 or 4
+    const int per = 4 / ntiles;                                       // wavefronts per tile
+    const int tile = wave % ntiles, phase = wave / ntiles;
+    const int nt = tile / nKt, kt = tile % nKt;
+    const v8bf zero8 = {};
     v16f acc = {};
-    for (int v0 = vb; v0 < ve; v0 += 16) {
-        v8bf a, bb;
+    const bool bias = pb && t == 13 && (int)threadIdx.x < N;
+    float bsum = 0.f;
+    const int dvec = N / 8, kvec = Kp / 8;
+    for (int v0 = vb; v0 < ve; v0 += WG_VB) {
+        __syncthreads();                               // the previous block's operands are read
+        for (int q = threadIdx.x; q < WG_VB * dvec; q += 256) {
+            const int vl = q / dvec, cg = q - vl * dvec, v = v0 + vl;
+            const v8bf val = v < ve ? *(const v8bf *)(dy + (size_t)v * N + cg * 8) : zero8;
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int v = v0 + kh * 8 + e;
-            const int x = v % G, y = (v / G) % G, z = v / (G * G);
-            int u;
-            const int ok = v < ve && nn_off(G, t, x, y, z, u);
-            a[e] = v < ve ? dy[(size_t)v * N + n] : (__bf16)0.f;
-            bb[e] = (ok && k < K) ? in[(size_t)u * K + k] : (__bf16)0.f;
+            for (int i = 0; i < 8; i++) s_dy[(cg * 8 + i) * WG_PITCH + vl] = val[i];
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc, 0, 0, 0);
+        for (int q = threadIdx.x; q < WG_VB * kvec; q += 256) {
+            const int vl = q / kvec, cg = q - vl * kvec, v = v0 + vl;
+            v8bf val = zero8;
+            if (v < ve && cg * 8 < K) {
+                const int x = v % G, y = (v / G) % G, z = v / (G * G);
+                int u;
+                if (nn_off(G, t, x, y, z, u)) val = *(const v8bf *)(in + (size_t)u * K + cg * 8);
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++) s_in[(cg * 8 + i) * WG_PITCH + vl] = val[i];
+        }
+        __syncthreads();
+        if (bias)
+            for (int vl = 0; vl < WG_VB; vl++) bsum += (float)s_dy[threadIdx.x * WG_PITCH + vl];
+        for (int g = phase; g < WG_VB / 16; g += per) {
+            const v8bf a = *(const v8bf *)(s_dy + (nt * 32 + row) * WG_PITCH + g * 16 + kh * 8);
+            const v8bf b = *(const v8bf *)(s_in + (kt * 32 + row) * WG_PITCH + g * 16 + kh * 8);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
     }
+    if (bias) pb[(size_t)c * N + threadIdx.x] = bsum;
+    // wavefronts that share a tile: sum in wavefront order through LDS
+    if (per > 1) {
+        __syncthreads();
+        float *s_acc = (float *)wg_sm;                 // 4 x 16 x 64 floats
+#pragma unroll
+        for (int r = 0; r < 16; r++) s_acc[(wave * 16 + r) * 64 + lane] = acc[r];
+        __syncthreads();
+        if (phase != 0) return;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            float sum = 0.f;
+            for (int p = 0; p < per; p++) sum += s_acc[((tile + p * ntiles) * 16 + r) * 64 + lane];
+            acc[r] = sum;
+        }
+    }
+    const int k = kt * 32 + row;
     const size_t base = ((size_t)c * 27 + t) * N;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
@@ -113,37 +185,39 @@ __global__ void __launch_bounds__(64) k_conv_wgrad(int G, const __bf16 *__restri
     }
 }
 void nn_conv_wgrad(hipStream_t s, int G, const __bf16 *dy, int N, const __bf16 *in, int K, int chunk, int nchunk,
-                   float *part) {
-    const int nb = nchunk * 27 * (N / 32) * ((K + 31) / 32);
-    k_conv_wgrad<<<nb, 64, 0, s>>>(G, dy, N, in, K, chunk, part);
+                   float *part, float *pb) {
+    const int Kp = (K + 31) / 32 * 32;
+    size_t lds = (size_t)(N + Kp) * WG_PITCH * sizeof(__bf16);
+    if (lds < 4 * 16 * 64 * sizeof(float)) lds = 4 * 16 * 64 * sizeof(float);
+    k_conv_wgrad<<<nchunk * 27, 256, lds, s>>>(G, dy, N, in, K, chunk, part, pb);
 }
-// g[i] = sum_c part[c][i] (chunks in order)
-__global__ void k_wgrad_reduce(const float *part, int nchunk, int n, float *g) {
+// g = sum_c part[c] (chunks in order) over the padded [27][N][K] layout,
+// stored as the real [27][Nr][Kr] block of the gradient vector
+__global__ void k_wgrad_reduce(const float *part, int nchunk, int N, int K, int Nr, int Kr, float *g) {
     const int i = blockIdx.x * 256 + threadIdx.x;
+    const int n = 27 * N * K;
     if (i >= n) return;
+    const int k = i % K, nn = (i / K) % N, t = i / (K * N);
+    if (nn >= Nr || k >= Kr) return;
     float s = 0.f;
     for (int c = 0; c < nchunk; c++) s += part[(size_t)c * n + i];
-    g[i] = s;
+    g[((size_t)t * Nr + nn) * Kr + k] = s;
 }
-void nn_wgrad_reduce(hipStream_t s, const float *part, int nchunk, int n, float *g) {
-    k_wgrad_reduce<<<(n + 255) / 256, 256, 0, s>>>(part, nchunk, n, g);
+void nn_wgrad_reduce(hipStream_t s, const float *part, int nchunk, int N, int K, int Nr, int Kr, float *g) {
+    const int n = 27 * N * K;
+    k_wgrad_reduce<<<(n + 255) / 256, 256, 0, s>>>(part, nchunk, N, K, Nr, Kr, g);
 }
-// bias gradient: gb[n] = sum_v dy[v][n]; one workgroup per channel, fixed order
-__global__ void __launch_bounds__(256) k_bgrad(int V, const __bf16 *dy, int N, float *gb) {
-    __shared__ float red[256];
-    const int n = blockIdx.x;
+// bias gradient gb[n] = sum over the chunks (in order) of the partials pb[c][n]
+// the weight-gradient kernel's centre tap wrote, n < Nr
+__global__ void k_bgrad_sum(const float *pb, int nchunk, int N, int Nr, float *gb) {
+    const int n = threadIdx.x;
+    if (n >= Nr) return;
     float s = 0.f;
-    for (int v = threadIdx.x; v < V; v += 256) s += (float)dy[(size_t)v * N + n];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) gb[n] = red[0];
+    for (int c = 0; c < nchunk; c++) s += pb[(size_t)c * N + n];
+    gb[n] = s;
 }
-void nn_bgrad(hipStream_t s, int V, const __bf16 *dy, int N, float *gb) {
-    k_bgrad<<<N, 256, 0, s>>>(V, dy, N, gb);
+void nn_bgrad(hipStream_t s, const float *pb, int nchunk, int N, int Nr, float *gb) {
+    k_bgrad_sum<<<1, 64, 0, s>>>(pb, nchunk, N, Nr, gb);
 }
 
 // ---------------------------------------------------------------- weights

@@ -123,19 +123,14 @@ void conv_bwd(Net &n, int i, const __bf16 *dy, const __bf16 *in, __bf16 *dx) {
     if (chunk < 512) chunk = 512;
     chunk = (chunk + 15) / 16 * 16;
     const int nch = (int)((V + chunk - 1) / chunk);
-    nn_conv_wgrad(s, l.G, dy, l.N, in, l.K, chunk, nch, n.part);
-    // the partial sums are laid out [chunk][t][N][K]; reduce into a padded
-    // gradient, then keep the real part
+    // the weight partials are laid out [chunk][t][N][K], the bias partials
+    // [chunk][N] after them; their sums over the chunks go straight into the
+    // real (Nr x Kr) block and the Nr biases of the gradient vector
     const int nw = 27 * l.N * l.K;
-    float *gpad = n.part + (size_t)nch * nw;
-    nn_wgrad_reduce(s, n.part, nch, nw, gpad);
-    // copy the real (Nr x Kr) block into the gradient vector
-    for (int t = 0; t < 27; t++)
-        NC(hipMemcpy2DAsync(n.Gr + l.wofs + (size_t)t * l.Nr * l.Kr, l.Kr * sizeof(float),
-                            gpad + (size_t)t * l.N * l.K, l.K * sizeof(float), l.Kr * sizeof(float), l.Nr,
-                            hipMemcpyDeviceToDevice, s));
-    nn_bgrad(s, (int)V, dy, l.N, gpad);
-    NC(hipMemcpyAsync(n.Gr + l.bofs, gpad, l.Nr * sizeof(float), hipMemcpyDeviceToDevice, s));
+    float *pb = n.part + (size_t)nch * nw;
+    nn_conv_wgrad(s, l.G, dy, l.N, in, l.K, chunk, nch, n.part, pb);
+    nn_wgrad_reduce(s, n.part, nch, l.N, l.K, l.Nr, l.Kr, n.Gr + l.wofs);
+    nn_bgrad(s, pb, nch, l.N, l.Nr, n.Gr + l.bofs);
     if (dx) nn_conv_fwd(s, l.G, dy, l.N, l.Wb, nullptr, l.K, dx, nullptr, 0);
 }
 
@@ -202,9 +197,11 @@ int nn_capture(hipStream_t s, void (*fill)(hipStream_t, float *, void *), void *
 extern "C" {
 
 // Create the network (grid G, a power of two >= 8; channels c0..c3 multiples
-// of 32, c3 == c2) with He-initialised weights from seed.  1 = done.
+// of 32 up to 64, c3 == c2) with He-initialised weights from seed.  1 = done.
 int sdx_nn_init(int G, int c0, int c1, int c2, int c3, unsigned seed) {
-    if (G < 8 || (G & (G - 1)) || c0 % 32 || c1 % 32 || c2 % 32 || c3 != c2) {
+    // channels up to 64: the weight-gradient kernel spreads at most four
+    // 32 x 32 output tiles over its wavefronts
+    if (G < 8 || (G & (G - 1)) || c0 % 32 || c1 % 32 || c2 % 32 || c3 != c2 || c0 > 64 || c1 > 64 || c2 > 64) {
         fprintf(stderr, "semisim-hip nn: unsupported shape G=%d channels %d %d %d %d\n", G, c0, c1, c2, c3);
         return 0;
     }

@@ -6,9 +6,12 @@
 void nn_conv_fwd(hipStream_t s, int G, const __bf16 *in, int K, const __bf16 *W, const float *bias, int N,
                  __bf16 *out, float *outf, int relu);
 void nn_conv_wgrad(hipStream_t s, int G, const __bf16 *dy, int N, const __bf16 *in, int K, int chunk, int nchunk,
-                   float *part);
-void nn_wgrad_reduce(hipStream_t s, const float *part, int nchunk, int n, float *g);
-void nn_bgrad(hipStream_t s, int V, const __bf16 *dy, int N, float *gb);
+                   float *part, float *pb);
+// the chunks' partials summed in order into the real [27][Nr][Kr] block g
+void nn_wgrad_reduce(hipStream_t s, const float *part, int nchunk, int N, int K, int Nr, int Kr, float *g);
+// bias gradient of the Nr real outputs into gb from the nchunk x N partials pb
+// that nn_conv_wgrad wrote
+void nn_bgrad(hipStream_t s, const float *pb, int nchunk, int N, int Nr, float *gb);
 void nn_pack_w(hipStream_t s, const float *W32, int Nr, int Kr, int N, int K, int Kb, __bf16 *Wf, __bf16 *Wb);
 void nn_adam(hipStream_t s, int n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2,
              float eps, int step);

@@ -7,6 +7,9 @@ sweep's first continuity assembly, k_cont, skips the set-up and the
 equilibrium solve).
 
     python -m semiconductor_devices_simulator_amd.tracestat kernel_trace.csv --from k_cont --top 40
+
+--by-grid keeps launches of one kernel with different grid sizes apart (the
+layers of the warm-start network); --match keeps only matching kernels.
 """
 import argparse
 import csv
@@ -14,11 +17,14 @@ import sys
 from collections import defaultdict
 
 
-def load(path):
+def load(path, by_grid=False):
     rows = []
     with open(path) as f:
         for r in csv.DictReader(f):
-            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+            name = r["Kernel_Name"]
+            if by_grid:
+                name = name.split("(")[0] + " grid " + r.get("Grid_Size", r.get("Grid_Size_X", "?"))
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), name))
     rows.sort()
     return rows
 
@@ -54,8 +60,13 @@ def main(argv=None):
     ap.add_argument("trace")
     ap.add_argument("--from", dest="start", default=None)
     ap.add_argument("--top", type=int, default=30)
+    ap.add_argument("--by-grid", action="store_true", help="one line per kernel and grid size")
+    ap.add_argument("--match", default=None, help="only kernels whose name contains this")
     a = ap.parse_args(argv)
-    S = summarise(load(a.trace), a.start)
+    rows = load(a.trace, a.by_grid)
+    if a.match:
+        rows = [r for r in rows if a.match in r[2]]
+    S = summarise(rows, a.start)
     g = S["gaps"]
     med = g[len(g) // 2] / 1e3 if g else 0.0
     print(f"window {S['window'] / 1e9:.3f} s, kernels busy {S['busy'] / 1e9:.3f} s "
