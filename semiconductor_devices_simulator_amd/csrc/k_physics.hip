@@ -41,16 +41,25 @@ __global__ void __launch_bounds__(SD_BLOCK) k_poisson(DevView d, const double *p
             continue;
         }
         const int i = nw.i, j = nw.j, l = nw.l;
+        // every neighbour's operands loaded before the first is used
+        Nbr6 nb;
+        sd_nbr6(g, i, j, l, k, nb);
+        double ebv[6], phv[6];
+        int cmv[6];
+#pragma unroll
+        for (int dir = 0; dir < 6; dir++) {
+            const int q = nb.kb[dir];
+            ebv[dir] = d.eps[q]; phv[dir] = phi[q]; cmv[dir] = d.con_mask[q];
+        }
         double ek = d.eps[k], pk = phi[k], f = 0.0, dg = 0.0, off[6];
 #pragma unroll
         for (int dir = 0; dir < 6; dir++) {
-            int kb; double h, ar;
             off[dir] = 0.0;
-            if (!sd_nbr(g, i, j, l, k, dir, kb, h, ar)) continue;
-            double eb = d.eps[kb], cf = (2. * ek * eb / (ek + eb)) * ar / h;
-            f += cf * (phi[kb] - pk);
+            if (!nb.has[dir]) continue;
+            double eb = ebv[dir], cf = (2. * ek * eb / (ek + eb)) * nb.ar[dir] / nb.h[dir];
+            f += cf * (phv[dir] - pk);
             dg += cf;
-            if (!d.con_mask[kb]) off[dir] = -cf;
+            if (!cmv[dir]) off[dir] = -cf;
         }
         double vol = g.Lx[i] * g.Ly[j] * g.Lz[l];
         if (d.kind[k] == K_SEMI) {
@@ -140,12 +149,26 @@ __global__ void k_upd_mob(DevView d, const double *qn, const double *qp) {
     sd_unidx(g, k, i, j, l);
     const int sat = (d.models & 4) != 0;
     const size_t N = g.N;
+    // the three upper neighbours' operands loaded before the first is used
+    Nbr6 ng;
+    sd_nbr6(g, i, j, l, k, ng);
+    int kdv[3];
+    double mnv[3], mpv[3], psv[3], qnv[3], qpv[3];
+#pragma unroll
     for (int ax = 0; ax < 3; ax++) {
-        int kp; double h, ar;
-        bool ok = d.kind[k] == K_SEMI && sd_nbr(g, i, j, l, k, 2 * ax + 1, kp, h, ar) && d.kind[kp] == K_SEMI;
+        const int q = ng.kb[2 * ax + 1];
+        kdv[ax] = d.kind[q]; mnv[ax] = d.mu_n[q]; mpv[ax] = d.mu_p[q];
+        if (sat) { psv[ax] = d.psi[q]; qnv[ax] = qn[q]; qpv[ax] = qp[q]; }
+        else psv[ax] = qnv[ax] = qpv[ax] = 0.0;
+    }
+    const bool semi = d.kind[k] == K_SEMI;
+    for (int ax = 0; ax < 3; ax++) {
+        const int kp = ng.kb[2 * ax + 1];
+        const double h = ng.h[2 * ax + 1];
+        bool ok = semi && ng.has[2 * ax + 1] && kdv[ax] == K_SEMI;
         if (!ok) { d.fmu[ax * N + k] = 0.0; d.fmu[(3 + ax) * N + k] = 0.0; continue; }
         const MatDev &m = d.mats[d.mat[k]];
-        double mn = 0.5 * (d.mu_n[k] + d.mu_n[kp]), mp = 0.5 * (d.mu_p[k] + d.mu_p[kp]);
+        double mn = 0.5 * (d.mu_n[k] + mnv[ax]), mp = 0.5 * (d.mu_p[k] + mpv[ax]);
         if (ax == d.caxis) {
             if (m.ani_n > 0.) mn *= m.ani_n;
             if (m.ani_p > 0.) mp *= m.ani_p;
@@ -162,8 +185,8 @@ __global__ void k_upd_mob(DevView d, const double *qn, const double *qp) {
         }
         if (sat) {
             double wv = 1.0;
-            double dps = d.psi[kp] - d.psi[k];
-            double Fn = sd_heat_field(qn[kp] - qn[k], dps, h), Fp = sd_heat_field(qp[kp] - qp[k], dps, h);
+            double dps = psv[ax] - d.psi[k];
+            double Fn = sd_heat_field(qnv[ax] - qn[k], dps, h), Fp = sd_heat_field(qpv[ax] - qp[k], dps, h);
             double sn = sd_mob_hf(mn, Fn, m.vsat_n, m.beta_n), sp = sd_mob_hf(mp, Fp, m.vsat_p, m.beta_p);
             mn += wv * (sn - mn);
             mp += wv * (sp - mp);
@@ -200,23 +223,33 @@ __global__ void k_cont(DevView d, int hole, Sten7<double> A, double *rhs, double
     const double lk = hole ? d.lnvK[k] : d.lncK[k];
     const size_t N = g.N;
     const double *fm = d.fmu + (size_t)(hole ? 3 : 0) * N;
+    const double *lnd = hole ? d.lnvK : d.lncK;
+    // every neighbour's operands loaded before the first is used
+    Nbr6 ng;
+    sd_nbr6(g, i, j, l, k, ng);
+    double lv[6], psv[6], fmv[6], carv[6];
+    int kdv[6], cmv[6];
+#pragma unroll
+    for (int dir = 0; dir < 6; dir++) {
+        const int q = ng.kb[dir], kl = (dir & 1) ? k : q;
+        kdv[dir] = d.kind[q]; lv[dir] = lnd[q]; psv[dir] = d.psi[q];
+        fmv[dir] = fm[(size_t)(dir >> 1) * N + kl]; cmv[dir] = d.con_mask[q]; carv[dir] = car[q];
+    }
     double dg = 0.0, b = 0.0, off[6];
 #pragma unroll
     for (int dir = 0; dir < 6; dir++) {
-        int kb; double h, ar;
         off[dir] = 0.0;
-        if (!sd_nbr(g, i, j, l, k, dir, kb, h, ar) || d.kind[kb] != K_SEMI) continue;
-        double dl = (hole ? d.lnvK[kb] : d.lncK[kb]) - lk;
-        double D = (d.psi[kb] - pk) / VT + (hole ? -dl : dl);
-        int kl = (dir & 1) ? k : kb;
-        double gg = ar * fm[(size_t)(dir >> 1) * N + kl] * VT / h;
+        if (!ng.has[dir] || kdv[dir] != K_SEMI) continue;
+        double dl = lv[dir] - lk;
+        double D = (psv[dir] - pk) / VT + (hole ? -dl : dl);
+        double gg = ng.ar[dir] * fmv[dir] * VT / ng.h[dir];
         double bp, bm;
         sd_bern2(D, bp, bm);
         double self = hole ? gg * bp : gg * bm;
         double nb = hole ? gg * bm : gg * bp;
         dg += self;
-        if (d.con_mask[kb]) b += nb * car[kb];
-        else off[dir] = -nb * fmax(car[kb], SD_NFLOOR);
+        if (cmv[dir]) b += nb * carv[dir];
+        else off[dir] = -nb * fmax(carv[dir], SD_NFLOOR);
     }
     double R, dRn, dRp;
     sd_recomb(ma, d.niK[k], sd_tauf(d, k, ma), d.n[k], d.p[k], R, dRn, dRp);

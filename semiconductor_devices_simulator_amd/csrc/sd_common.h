@@ -189,6 +189,42 @@ __device__ __forceinline__ double sd_pK(const DevView &d, int k, double psi, dou
     return exp(sd_clampe(d.lnvK[k] + (qp - psi) / VT));
 }
 
+// Index of a neighbour load: the neighbour's, or the node's own where there
+// is none, so that every neighbour load can be issued before the first use.
+// The empty asm hides the value from the compiler, which would otherwise fold
+// the clamped load into the node's own load of the same array and move the
+// neighbour's back into a branch.
+__device__ __forceinline__ int sd_nb_idx(bool has, int nb, int self) {
+    int v = has ? nb : self;
+    asm("" : "+v"(v));
+    return v;
+}
+// The six neighbours of node k = (i, j, l) at once (directions as sd_nbr):
+// whether each exists, its index (k itself where there is none, so loads
+// through it stay in bounds), and the spacing h and dual face area ar, with
+// sd_nbr's values (the same expressions) where the neighbour exists.
+struct Nbr6 {
+    bool has[6];
+    int kb[6];
+    double h[6], ar[6];
+};
+__device__ __forceinline__ void sd_nbr6(const Geo &g, int i, int j, int l, int k, Nbr6 &n) {
+    const int NxNy = g.Nx * g.Ny;
+    n.has[0] = i > 0; n.has[1] = i < g.Nx - 1; n.has[2] = j > 0;
+    n.has[3] = j < g.Ny - 1; n.has[4] = l > 0; n.has[5] = l < g.Nz - 1;
+    n.kb[0] = sd_nb_idx(n.has[0], k - 1, k); n.kb[1] = sd_nb_idx(n.has[1], k + 1, k);
+    n.kb[2] = sd_nb_idx(n.has[2], k - g.Nx, k); n.kb[3] = sd_nb_idx(n.has[3], k + g.Nx, k);
+    n.kb[4] = sd_nb_idx(n.has[4], k - NxNy, k); n.kb[5] = sd_nb_idx(n.has[5], k + NxNy, k);
+    const int im = n.has[0] ? i - 1 : i, ip = n.has[1] ? i + 1 : i, jm = n.has[2] ? j - 1 : j;
+    const int jp = n.has[3] ? j + 1 : j, lm = n.has[4] ? l - 1 : l, lp = n.has[5] ? l + 1 : l;
+    n.h[0] = g.xs[i] - g.xs[im]; n.h[1] = g.xs[ip] - g.xs[i];
+    n.h[2] = g.ys[j] - g.ys[jm]; n.h[3] = g.ys[jp] - g.ys[j];
+    n.h[4] = g.zs[l] - g.zs[lm]; n.h[5] = g.zs[lp] - g.zs[l];
+    n.ar[0] = n.ar[1] = g.Ly[j] * g.Lz[l];
+    n.ar[2] = n.ar[3] = g.Lx[i] * g.Lz[l];
+    n.ar[4] = n.ar[5] = g.Lx[i] * g.Ly[j];
+}
+
 // neighbour geometry along direction dir (0 x-,1 x+,2 y-,3 y+,4 z-,5 z+),
 // semiconductor_3d.c:703-713.  Returns false outside the domain.
 __device__ __forceinline__ bool sd_nbr(const Geo &g, int i, int j, int l, int k, int dir,
