@@ -98,7 +98,7 @@ void   sdx_ws_accept(double ratio);              /* mode 2 keeps a correction wh
    nodes, out[1] blocks found, out[2] blocks rescaled to their balance */
 void   sdx_float_stats(double *out);
 void   sdx_float_reset_stats(void);
-/* network: G a power of two >= 8, channels multiples of 32 with c3 == c2 */
+/* network: G a power of two >= 8, channels multiples of 32 up to 64 with c3 == c2 */
 int    sdx_nn_init(int G, int c0, int c1, int c2, int c3, unsigned seed);
 void   sdx_nn_free(void);
 int    sdx_nn_nparams(void);

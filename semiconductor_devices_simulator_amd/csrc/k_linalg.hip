@@ -1038,6 +1038,35 @@ template <class T> __device__ __forceinline__ T wave_next(T B, int lane) {
     else { T y = __shfl_down(B, 1, 64); return lane == 63 ? (T)0 : y; }
 }
 
+// Element idx of an n-element array through a raw buffer load or store: a
+// 32-bit byte offset from a descriptor held in scalar registers, instead of a
+// 64-bit address per lane, and a read past n bytes returns 0 (a write past it
+// is dropped) instead of leaving the array.  Callers keep n * sizeof(U) below
+// 2^32 (the launcher checks the level size).  Descriptor word 3 is the GFX9
+// raw-buffer form (32-bit data format, no swizzle, stride 0).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <class U> __device__ __forceinline__ U ld_buf(const U *base, unsigned n, unsigned idx) {
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(base, n * (unsigned)sizeof(U));
+    const int off = (int)(idx * (unsigned)sizeof(U));
+    if constexpr (sizeof(U) == 2) return __builtin_bit_cast(U, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0));
+    else if constexpr (sizeof(U) == 4) return __builtin_bit_cast(U, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    else if constexpr (sizeof(U) == 8) return __builtin_bit_cast(U, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    else {
+        static_assert(sizeof(U) == 16, "2, 4, 8 or 16-byte elements");
+        return __builtin_bit_cast(U, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    }
+}
+template <class U> __device__ __forceinline__ void st_buf(U *base, unsigned n, unsigned idx, U val) {
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(base, n * (unsigned)sizeof(U));
+    const int off = (int)(idx * (unsigned)sizeof(U));
+    if constexpr (sizeof(U) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, val), r, off, 0, 0);
+    else {
+        static_assert(sizeof(U) == 8, "4 or 8-byte elements");
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, val), r, off, 0, 0);
+    }
+}
 // The wavefront scans of one sweep: wavefront w solves its S slots of 64 P
 // elements in LDS (right-hand side r iw in s_r, ai in s_a, cp in s_c); the
 // solution replaces s_r.
@@ -1220,17 +1249,20 @@ template <class T, int P, int S, int MD, int HM, int AX>
 __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T *__restrict__ b, T *__restrict__ x,
                                                            int color, MGLevel<T> c, int xcd) {
     constexpr bool zero = MD == 0, pro = MD == 2, H16 = HM != 0;
-    constexpr int SL = 64 * P, SP = SL + 1, NLW = LS_WAVES * S, M = P * S;
+    constexpr int SL = 64 * P, SP = SL + 1, NLW = LS_WAVES * S, M = P * S, NT = LS_THREADS;
     __shared__ T s_r[NLW * SP], s_a[NLW * SP], s_c[NLW * SP];
     const int Nx = v.Nx, Ny = v.Ny, Nz = v.Nz, NxNy = Nx * Ny;
+    const unsigned NU = (unsigned)v.N;
     const int n = AX == 0 ? Nx : (AX == 1 ? Ny : Nz);        // line length
     const int Ntd = AX == 0 ? Ny : Nx;                       // lines of both colours per plane
     const int ntile = (((Ntd + 1) >> 1) + NLW - 1) / NLW;
     const int bid = xcd ? (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3) : blockIdx.x;
     const int p3 = bid / ntile, h0 = (bid - p3 * ntile) * NLW, sh = (p3 + color) & 1;
     const int E = NLW * n, pad = SL - n, hx = (Nx + 1) >> 1;
-    for (int t = threadIdx.x; t < NLW * pad; t += LS_THREADS) {
-        const int q = (t / pad) * SP + n + t % pad;
+    // LDS position of element q of line L of the tile
+    auto lds = [&](int L, int q) { return L * SP + q; };
+    for (int t = threadIdx.x; t < NLW * pad; t += NT) {
+        const int q = lds(t / pad, n + t % pad);
         s_r[q] = 0; s_a[q] = 0; s_c[q] = 0;
     }
     constexpr int a_lu = AX;
@@ -1250,11 +1282,12 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
             L = e % NLW;
         }
     };
-    int kk[M], msk[M];
+    unsigned kk[M];
+    int msk[M];
     T vb[M], vai[M], viw[M], vcp[M], cf[M][4], xn[M][4], pwn[pro ? M : 1][4], xcn[pro ? M : 1][4];
 #pragma unroll
     for (int m = 0; m < M; m++) {
-        const int e = threadIdx.x + LS_THREADS * m;
+        const int e = threadIdx.x + NT * m;
         int L, q;
         split(e, L, q);
         const int a = h0 + L, td = 2 * a + sh;
@@ -1282,14 +1315,16 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
         }
         hm1 = hm1 && ok; hp1 = hp1 && ok; hm2 = hm2 && ok; hp2 = hp2 && ok;
         if (!ok) k = 0;
-        const int nk[4] = {hm1 ? km1 : k, hp1 ? kp1 : k, hm2 ? km2 : k, hp2 ? kp2 : k};
+        const unsigned nk[4] = {(unsigned)(hm1 ? km1 : k), (unsigned)(hp1 ? kp1 : k), (unsigned)(hm2 ? km2 : k),
+                                (unsigned)(hp2 ? kp2 : k)};
         msk[m] = (int)ok | ((int)hm1 << 1) | ((int)hp1 << 2) | ((int)hm2 << 3) | ((int)hp2 << 4);
-        kk[m] = k;
-        vb[m] = b[k];
+        kk[m] = (unsigned)k;
+        const unsigned ku = (unsigned)k;
+        vb[m] = ld_buf(b, NU, ku);
         if constexpr (HM == 2) {
             // the node's seven BF16 coefficients in one 16-byte load (a
             // BF16 value is the upper half of the FP32 word)
-            const uint4 r = v.rec16[a_lu][k];
+            const uint4 r = ld_buf(v.rec16[a_lu], NU, ku);
             vai[m] = __uint_as_float(r.x << 16); viw[m] = __uint_as_float(r.x & 0xffff0000u);
             vcp[m] = __uint_as_float(r.y << 16);
             if constexpr (!zero) {
@@ -1297,19 +1332,21 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
                 cf[m][2] = __uint_as_float(r.z & 0xffff0000u); cf[m][3] = __uint_as_float(r.w << 16);
             }
         } else if constexpr (HM == 1) {
-            vai[m] = (T)v.lu16[a_lu][0][k]; viw[m] = (T)v.lu16[a_lu][1][k]; vcp[m] = (T)v.lu16[a_lu][2][k];
+            vai[m] = (T)ld_buf(v.lu16[a_lu][0], NU, ku); viw[m] = (T)ld_buf(v.lu16[a_lu][1], NU, ku);
+            vcp[m] = (T)ld_buf(v.lu16[a_lu][2], NU, ku);
             if constexpr (!zero) {
-                cf[m][0] = (T)h0p[k]; cf[m][1] = (T)h1p[k]; cf[m][2] = (T)h2p[k]; cf[m][3] = (T)h3p[k];
+                cf[m][0] = (T)ld_buf(h0p, NU, ku); cf[m][1] = (T)ld_buf(h1p, NU, ku);
+                cf[m][2] = (T)ld_buf(h2p, NU, ku); cf[m][3] = (T)ld_buf(h3p, NU, ku);
             }
         } else {
-            vai[m] = v.lu[a_lu][0][k]; viw[m] = v.lu[a_lu][1][k]; vcp[m] = v.lu[a_lu][2][k];
+            vai[m] = ld_buf(v.lu[a_lu][0], NU, ku); viw[m] = ld_buf(v.lu[a_lu][1], NU, ku); vcp[m] = ld_buf(v.lu[a_lu][2], NU, ku);
         }
         if constexpr (!zero) {
             if constexpr (!H16) {
-                cf[m][0] = o0[k]; cf[m][1] = o1[k]; cf[m][2] = o2[k]; cf[m][3] = o3[k];
+                cf[m][0] = ld_buf(o0, NU, ku); cf[m][1] = ld_buf(o1, NU, ku); cf[m][2] = ld_buf(o2, NU, ku); cf[m][3] = ld_buf(o3, NU, ku);
             }
 #pragma unroll
-            for (int t = 0; t < 4; t++) xn[m][t] = x[nk[t]];
+            for (int t = 0; t < 4; t++) xn[m][t] = ld_buf(x, NU, nk[t]);
         }
         if constexpr (pro) {
             // the neighbours' coarse parents (first post-smoothing sweep)
@@ -1320,15 +1357,15 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
                 if (t < 2) { if (AX == 0) jj += on * d; else ii += on * d; }
                 else { if (AX == 2) jj += on * d; else ll += on * d; }
                 const int I = v.cx ? ii >> 1 : ii, J = v.cy ? jj >> 1 : jj, LL = v.cz ? ll >> 1 : ll;
-                pwn[m][t] = v.pw[nk[t]];
-                xcn[m][t] = c.x[ok ? mg_idx(c, I, J, LL) : 0];
+                pwn[m][t] = ld_buf(v.pw, NU, nk[t]);
+                xcn[m][t] = ld_buf(c.x, (unsigned)c.N, (unsigned)(ok ? mg_idx(c, I, J, LL) : 0));
             }
         }
     }
     // right-hand sides into LDS
 #pragma unroll
     for (int m = 0; m < M; m++) {
-        const int e = threadIdx.x + LS_THREADS * m;
+        const int e = threadIdx.x + NT * m;
         if (e >= E) break;
         int L, q;
         split(e, L, q);
@@ -1338,14 +1375,17 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
             if constexpr (!zero) {
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
-                    if (!((msk[m] >> (t + 1)) & 1)) continue;
+                    // the packed records hold 0 for a neighbour outside the
+                    // mesh (and its x is the node's own): nothing to skip
+                    if constexpr (HM != 2)
+                        if (!((msk[m] >> (t + 1)) & 1)) continue;
                     if constexpr (pro) sr -= cf[m][t] * (xn[m][t] + pwn[m][t] * xcn[m][t]);
                     else sr -= cf[m][t] * xn[m][t];
                 }
             }
             r = sr; ai = vai[m]; iw = viw[m]; cp = vcp[m];
         }
-        const int q0 = L * SP + q;
+        const int q0 = lds(L, q);
         s_r[q0] = r * iw; s_a[q0] = ai; s_c[q0] = cp;
     }
     __syncthreads();
@@ -1355,8 +1395,8 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
     for (int m = 0; m < M; m++) {
         if (!(msk[m] & 1)) continue;
         int L, q;
-        split(threadIdx.x + LS_THREADS * m, L, q);
-        x[kk[m]] = s_r[L * SP + q];
+        split(threadIdx.x + NT * m, L, q);
+        st_buf(x, NU, kk[m], s_r[lds(L, q)]);
     }
 }
 // slots per wavefront allowed by the 64 KB of LDS a workgroup may hold
@@ -1426,7 +1466,8 @@ static void ls1_launch_md(hipStream_t s, int S, int nb, const MGLevel<T> &v, con
 template <class T, int P>
 static void ls_launch(hipStream_t s, int S, int nb, const MGLevel<T> &v, const T *b, T *x, int ax, int color, int zero,
                       const MGLevel<T> &cc, int pro, int LW, int xcd) {
-    if (LW == 1 && g_line_scan1) {
+    // k_line_scan1 addresses its arrays with 32-bit byte offsets (16-byte records)
+    if (LW == 1 && g_line_scan1 && (size_t)v.N * 16 < ((size_t)1 << 32)) {
         if (zero) ls1_launch_md<T, P, 0>(s, S, nb, v, b, x, ax, color, cc, xcd);
         else if (pro) ls1_launch_md<T, P, 2>(s, S, nb, v, b, x, ax, color, cc, xcd);
         else ls1_launch_md<T, P, 1>(s, S, nb, v, b, x, ax, color, cc, xcd);
@@ -1976,11 +2017,18 @@ __global__ void k_rec16(MGLevel<float> v, int ax, uint4 *rec) {
     if (k >= v.N) return;
     const float *o0 = ax == 0 ? v.A.ym : v.A.xm, *o1 = ax == 0 ? v.A.yp : v.A.xp;
     const float *o2 = ax == 2 ? v.A.ym : v.A.zm, *o3 = ax == 2 ? v.A.yp : v.A.zp;
+    // node (i, j, l) at storage position k; a coupling to a neighbour outside
+    // the mesh is stored as 0, so that the sweep subtracts it unconditionally
+    const int row = k / v.Nx, p = k - row * v.Nx, j = row % v.Ny, l = row / v.Ny, hx = (v.Nx + 1) >> 1;
+    const int i = v.px ? (p < hx ? 2 * p : 2 * (p - hx) + 1) : p;
+    const bool hx0 = i > 0, hx1 = i < v.Nx - 1, hy0 = j > 0, hy1 = j < v.Ny - 1, hz0 = l > 0, hz1 = l < v.Nz - 1;
+    const bool h0 = ax == 0 ? hy0 : hx0, h1 = ax == 0 ? hy1 : hx1;
+    const bool h2 = ax == 2 ? hy0 : hz0, h3 = ax == 2 ? hy1 : hz1;
     uint4 r;
     r.x = bf16_bits(v.lu[ax][0][k]) | (bf16_bits(v.lu[ax][1][k]) << 16);
-    r.y = bf16_bits(v.lu[ax][2][k]) | (bf16_bits(o0[k]) << 16);
-    r.z = bf16_bits(o1[k]) | (bf16_bits(o2[k]) << 16);
-    r.w = bf16_bits(o3[k]);
+    r.y = bf16_bits(v.lu[ax][2][k]) | (bf16_bits(h0 ? o0[k] : 0.0f) << 16);
+    r.z = bf16_bits(h1 ? o1[k] : 0.0f) | (bf16_bits(h2 ? o2[k] : 0.0f) << 16);
+    r.w = bf16_bits(h3 ? o3[k] : 0.0f);
     rec[k] = r;
 }
 void mg_rec16(hipStream_t s, const MGLevel<float> &v, int ax, uint4 *rec) {
