@@ -319,6 +319,9 @@ def main():
     ap.add_argument("--sweep-points", type=int, default=0,
                     help="solve only the first N points of the sweep (short profiling runs)")
     ap.add_argument("--cpu-baseline-worker", default="")
+    ap.add_argument("--batched", action="store_true",
+                    help="also time the whole sweep with its two drain-bias curves in one worker process after the "
+                         "other and in two at once (sweep-level batching, both methods)")
     a = ap.parse_args()
     if a.cpu_baseline_worker:
         cpu_baseline_worker(a.cpu_baseline_worker)
@@ -424,8 +427,34 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if a.batched and a.workload == "mosfet":
+        out["batched"] = batched(a.mesh, spec)
     print(json.dumps(out))
     return 0
+
+
+def batched(mesh, spec):
+    """Sweep-level batching: the whole Id-Vg sweep (both drain-bias curves,
+    every point) in one worker process (the curves one after the other) and
+    in two worker processes at once, each curve from the equilibrium
+    solution, in both methods; currents of every point against the
+    reference's sweeps."""
+    from semiconductor_devices_simulator_amd import batch
+    n = spec.bias_points()
+    res = {}
+    for prec, tag in ((1, "fp32_mg"), (0, "fp64")):
+        res[tag] = {}
+        for label, groups in (("one_worker", [[0, 1]]), ("two_workers", [[0], [1]])):
+            o = batch.run_concurrent(("mosfet", tuple(mesh), {}), groups, precision=prec)
+            R = batch.concat(o["results"])
+            res[tag][label] = dict(points=o["points"], seconds=round(o["seconds"], 3),
+                                   points_per_s=round(o["points_per_s"], 4),
+                                   gummel=int(sum(int(r["iterations"][3]) for r in o["results"])),
+                                   converged=int(np.sum(R["conv"])),
+                                   parity=parity(R["I"], list(range(n)), mesh) if o["points"] == n else None)
+            print(f"[bench] batched {tag} {label}: {o['points_per_s']:.3f} points/s", file=sys.stderr, flush=True)
+        res[tag]["gain"] = round(res[tag]["two_workers"]["points_per_s"] / res[tag]["one_worker"]["points_per_s"], 3)
+    return dict(workers=2, groups=["Vd 0.05 V curve", "Vd 1.0 V curve"], **res)
 
 
 def main_prism(a, lib):
@@ -468,8 +497,34 @@ def main_prism(a, lib):
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if a.batched and a.workload == "mosfet":
+        out["batched"] = batched(a.mesh, spec)
     print(json.dumps(out))
     return 0
+
+
+def batched(mesh, spec):
+    """Sweep-level batching: the whole Id-Vg sweep (both drain-bias curves,
+    every point) in one worker process (the curves one after the other) and
+    in two worker processes at once, each curve from the equilibrium
+    solution, in both methods; currents of every point against the
+    reference's sweeps."""
+    from semiconductor_devices_simulator_amd import batch
+    n = spec.bias_points()
+    res = {}
+    for prec, tag in ((1, "fp32_mg"), (0, "fp64")):
+        res[tag] = {}
+        for label, groups in (("one_worker", [[0, 1]]), ("two_workers", [[0], [1]])):
+            o = batch.run_concurrent(("mosfet", tuple(mesh), {}), groups, precision=prec)
+            R = batch.concat(o["results"])
+            res[tag][label] = dict(points=o["points"], seconds=round(o["seconds"], 3),
+                                   points_per_s=round(o["points_per_s"], 4),
+                                   gummel=int(sum(int(r["iterations"][3]) for r in o["results"])),
+                                   converged=int(np.sum(R["conv"])),
+                                   parity=parity(R["I"], list(range(n)), mesh) if o["points"] == n else None)
+            print(f"[bench] batched {tag} {label}: {o['points_per_s']:.3f} points/s", file=sys.stderr, flush=True)
+        res[tag]["gain"] = round(res[tag]["two_workers"]["points_per_s"] / res[tag]["one_worker"]["points_per_s"], 3)
+    return dict(workers=2, groups=["Vd 0.05 V curve", "Vd 1.0 V curve"], **res)
 
 
 if __name__ == "__main__":

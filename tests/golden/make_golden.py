@@ -791,9 +791,10 @@ def main():
                       "e2e_vdmos_40x24x4")
         return
     if "--vdmos96" in sys.argv:        # the VDMOS cell on a 96x48x16 mesh (about 20 minutes on 8 cores)
-        # Id-Vds to 15 V: at 20 V the reference's Gummel loop stops without
-        # converging on this mesh
-        sweep_fixture(lib, devices.vdmos(96, 48, 16, vd_max=44.0, npts=12, npts_on=4, vd_on_max=15.0, tol_c=1e-11),
+        # Id-Vds to 10 V: at 20 V the reference's Gummel loop stops without
+        # converging on this mesh, and at 15 V it needs 8738 iterations (a
+        # point at the edge of convergence, not a parity case)
+        sweep_fixture(lib, devices.vdmos(96, 48, 16, vd_max=44.0, npts=12, npts_on=3, vd_on_max=10.0, tol_c=1e-11),
                       "e2e_vdmos_96x48x16", fields=False, sample=4096)
         return
     if "--mosfet96" in sys.argv:       # the full 96x64x64 Id-Vg sweep (about 5 minutes on 8 cores)
