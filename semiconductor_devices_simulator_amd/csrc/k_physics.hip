@@ -716,12 +716,27 @@ __global__ void k_maj_flags(DevView d, int hole, const double *u, const double *
 void ph_maj_flags(hipStream_t s, const DevView &d, int hole, const double *u, const double *sc, unsigned char *flag) {
     k_maj_flags<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d, hole, u, sc, flag);
 }
-// anchoring travels along whole lines of one axis per launch (forward, then
-// back), one lane per line.  The flags of MP_CHUNK consecutive elements are
-// loaded before any is examined, so a lane waits for memory once per chunk.
-#define MP_CHUNK 16
-__global__ void __launch_bounds__(64) k_maj_prop(Geo g, unsigned char *flag, int axis, int *changed) {
-    int t = blockIdx.x * 64 + threadIdx.x;
+// anchoring travels along whole lines of one axis per launch, one wavefront
+// per line: 64 consecutive elements per step, whose flags become two ballot
+// masks (majority M: flag != 0, anchored A: flag == 2).  Within a step the
+// anchored set spreads through runs of M by doubling shifts (after shift s,
+// bit p holds whether an anchored element lies within the 2s elements of its
+// run ending at p); the run's state carries into the next step.  A forward
+// pass, then a backward pass over the flags it left: every majority run of
+// the line that holds an anchored element ends anchored, as a sequential walk
+// each way leaves it.
+__device__ __forceinline__ unsigned long long maj_spread(unsigned long long A, unsigned long long M, bool fwd) {
+    unsigned long long P = M;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        if (fwd) { A |= (A << s) & P; P &= P << s; }
+        else { A |= (A >> s) & P; P &= P >> s; }
+    }
+    return A;
+}
+#define MP_LINES 4
+__global__ void __launch_bounds__(64 * MP_LINES) k_maj_prop(Geo g, unsigned char *flag, int axis, int *changed) {
+    const int lane = threadIdx.x & 63, t = blockIdx.x * MP_LINES + (threadIdx.x >> 6);
     int n, stride, k0;
     if (axis == 0) {
         if (t >= g.Ny * g.Nz) return;
@@ -734,34 +749,30 @@ __global__ void __launch_bounds__(64) k_maj_prop(Geo g, unsigned char *flag, int
         if (t >= g.Nx * g.Ny) return;
         n = g.Nz; stride = g.Nx * g.Ny; k0 = t;
     }
-    int ch = 0;
+    // (every lane of a wavefront has the same t, so whole wavefronts return)
+    const int nch = (n + 63) >> 6;
+    bool ch = false;
     for (int dir = 0; dir < 2; dir++) {
-        unsigned char prev = 0;
-        for (int c0 = 0; c0 < n; c0 += MP_CHUNK) {
-            unsigned char f[MP_CHUNK];
-#pragma unroll
-            for (int u = 0; u < MP_CHUNK; u++) {
-                const int q = c0 + u < n ? c0 + u : n - 1;
-                f[u] = flag[k0 + (dir ? n - 1 - q : q) * stride];
-            }
-#pragma unroll
-            for (int u = 0; u < MP_CHUNK; u++) {
-                if (c0 + u >= n) break;
-                if (f[u] == 1 && prev == 2) {
-                    f[u] = 2;
-                    flag[k0 + (dir ? n - 1 - (c0 + u) : c0 + u) * stride] = 2;
-                    ch = 1;
-                }
-                prev = f[u];
-            }
+        const bool fwd = dir == 0;
+        bool carry = false;
+        for (int c = 0; c < nch; c++) {
+            const int cc = fwd ? c : nch - 1 - c, q = (cc << 6) + lane;
+            const bool ok = q < n;
+            const size_t k = (size_t)k0 + (size_t)q * stride;
+            const unsigned char f = ok ? flag[k] : 0;
+            const unsigned long long M = __ballot(f != 0), A0 = __ballot(f == 2);
+            const unsigned long long in = fwd ? 1ull : (1ull << 63);
+            const unsigned long long A = maj_spread(A0 | (carry ? (M & in) : 0ull), M, fwd);
+            if (((A & ~A0) >> lane) & 1ull) { flag[k] = 2; ch = true; }
+            carry = ((fwd ? A >> 63 : A) & 1ull) != 0;
         }
     }
-    if (ch) *changed = 1;
+    if (__ballot(ch) && lane == 0) *changed = 1;
 }
 void ph_maj_propagate(hipStream_t s, const DevView &d, unsigned char *flag, int axis, int *changed) {
     const Geo &g = d.g;
     int n = axis == 0 ? g.Ny * g.Nz : (axis == 1 ? g.Nx * g.Nz : g.Nx * g.Ny);
-    k_maj_prop<<<sd_nblocks(n, 64), 64, 0, s>>>(g, flag, axis, changed);
+    k_maj_prop<<<(n + MP_LINES - 1) / MP_LINES, 64 * MP_LINES, 0, s>>>(g, flag, axis, changed);
 }
 // indices of the majority nodes whose block reached no contact (flag 1);
 // the order of the list is arbitrary, the host sorts it
