@@ -97,7 +97,10 @@ def test_vdmos_96x48x16_sweeps_match_reference(hip, precision):
     unconverged on this mesh), both methods
     against the reference core's sweep (tol_c 1e-11, as the 40x24x4 case),
     every terminal within 1e-8 |I_ref| + floor_ref + floor, and psi at the
-    fixture's 4096 sampled nodes of the last point within 1e-8 V."""
+    fixture's 4096 sampled nodes of the last point within 5e-8 V: the
+    reference's own sweep at 4 threads instead of the fixture's 8 ends 1.6e-8 V
+    from it there (the avalanche point's Gummel loop contracts slowly: 1389
+    and 1514 iterations)."""
     from semiconductor_devices_simulator_amd import devices, sweep
     print(f"\n[gpu ii] vdmos 96x48x16 precision {precision}", flush=True)
     spec = devices.vdmos(96, 48, 16, vd_max=44.0, npts=12, npts_on=3, vd_on_max=10.0, tol_c=1e-11)
@@ -118,4 +121,4 @@ def test_vdmos_96x48x16_sweeps_match_reference(hip, precision):
     hip.api3_get_state(dptr(psi), dptr(np.zeros(spec.N)), dptr(np.zeros(spec.N)))
     dpsi = np.max(np.abs(psi[g["idx"]] - g["psi"]))
     print(f"[gpu ii] vdmos last point psi max diff {dpsi:.3g} V", flush=True)
-    assert dpsi <= 1e-8
+    assert dpsi <= 5e-8
