@@ -1345,8 +1345,14 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
             if constexpr (!H16) {
                 cf[m][0] = ld_buf(o0, NU, ku); cf[m][1] = ld_buf(o1, NU, ku); cf[m][2] = ld_buf(o2, NU, ku); cf[m][3] = ld_buf(o3, NU, ku);
             }
+            // a neighbour outside the mesh reads as 0 (its index is the
+            // node's own, whose x the sweep has not necessarily written yet)
+            const bool hv[4] = {hm1, hp1, hm2, hp2};
 #pragma unroll
-            for (int t = 0; t < 4; t++) xn[m][t] = ld_buf(x, NU, nk[t]);
+            for (int t = 0; t < 4; t++) {
+                const T xv = ld_buf(x, NU, nk[t]);
+                xn[m][t] = hv[t] ? xv : (T)0;
+            }
         }
         if constexpr (pro) {
             // the neighbours' coarse parents (first post-smoothing sweep)
@@ -1357,8 +1363,10 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
                 if (t < 2) { if (AX == 0) jj += on * d; else ii += on * d; }
                 else { if (AX == 2) jj += on * d; else ll += on * d; }
                 const int I = v.cx ? ii >> 1 : ii, J = v.cy ? jj >> 1 : jj, LL = v.cz ? ll >> 1 : ll;
-                pwn[m][t] = ld_buf(v.pw, NU, nk[t]);
-                xcn[m][t] = ld_buf(c.x, (unsigned)c.N, (unsigned)(ok ? mg_idx(c, I, J, LL) : 0));
+                const T pv = ld_buf(v.pw, NU, nk[t]);
+                const T cv = ld_buf(c.x, (unsigned)c.N, (unsigned)(ok ? mg_idx(c, I, J, LL) : 0));
+                pwn[m][t] = on ? pv : (T)0;
+                xcn[m][t] = on ? cv : (T)0;
             }
         }
     }
@@ -1376,7 +1384,8 @@ __global__ void __launch_bounds__(LS_THREADS) k_line_scan1(MGLevel<T> v, const T
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
                     // the packed records hold 0 for a neighbour outside the
-                    // mesh (and its x is the node's own): nothing to skip
+                    // mesh, and its x, weight and coarse value were read as 0:
+                    // nothing to skip
                     if constexpr (HM != 2)
                         if (!((msk[m] >> (t + 1)) & 1)) continue;
                     if constexpr (pro) sr -= cf[m][t] * (xn[m][t] + pwn[m][t] * xcn[m][t]);
