@@ -170,6 +170,7 @@ int    sdx_test_mg_setup(void);                     /* number of levels */
 void   sdx_test_level_dims(int L, int *dims);      /* Nx Ny Nz cx cy cz */
 void   sdx_test_level_op(int L, double *A7);        /* coarse operator, unscaled (rs * Ahat) */
 void   sdx_test_precond(const double *r, double *z);
+void   sdx_test_fill_mg(double value);              /* every multigrid solution array := value */
 int    sdx_test_pcg(const double *b, double *x, int maxit, double rtol);
 double sdx_test_bicgstab(const double *b, double *x, int maxit, double rtol, double atol, int *its);
 void   sdx_test_currents(const double *Va, double *I, double *S, double *In);

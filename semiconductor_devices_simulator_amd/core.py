@@ -142,6 +142,7 @@ EXT_SIGNATURES = [
     ("sdx_test_level_dims", None, [I, IP]),
     ("sdx_test_level_op", None, [I, DP]),
     ("sdx_test_precond", None, [DP, DP]),
+    ("sdx_test_fill_mg", None, [D]),
     ("sdx_test_pcg", I, [DP, DP, I, D]),
     ("sdx_test_bicgstab", D, [DP, DP, I, D, D, IP]),
     ("sdx_test_currents", None, [DP, DP, DP, DP]),

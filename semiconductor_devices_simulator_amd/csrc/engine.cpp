@@ -2787,6 +2787,28 @@ void sdx_test_level_dims(int L, int *dims) {
 void sdx_test_level_op(int L, double *A7) {
     if (G.precision == 1) level_op_T(G.H32, L, A7); else level_op_T(G.H64, L, A7);
 }
+// every multigrid solution array (each level's x and xs, level-0 FP32
+// vectors, the FP32 Krylov vectors) filled with value: the V-cycle must not
+// read what a previous use left there
+extern "C++" template <class T> static void fill_levels(Hier<T> &H, double value) {
+    for (auto &v : H.lev) {
+        if constexpr (std::is_same_v<T, float>) {
+            if (v.x) k32_fill_launch(G.st, v.x, (float)value, v.N);
+            if (v.xs) k32_fill_launch(G.st, v.xs, (float)value, v.N);
+        } else {
+            if (v.x) sd_fill(G.st, v.x, value, v.N);
+            if (v.xs) sd_fill(G.st, v.xs, value, v.N);
+        }
+    }
+}
+void sdx_test_fill_mg(double value) {
+    fill_levels(G.H32, value);
+    fill_levels(G.H64, value);
+    if (G.d_x32) k32_fill_launch(G.st, G.d_x32, (float)value, G.N);
+    if (G.d_x32b) k32_fill_launch(G.st, G.d_x32b, (float)value, G.N);
+    if (G.d_k32) k32_fill_launch(G.st, G.d_k32, (float)value, 8 * G.N);
+    eng_sync();
+}
 void sdx_test_precond(const double *r, double *z) {
     eng_upload(G.d_wk[2], r, G.N);
     precond(G.d_wk[2], G.d_wk[3]);

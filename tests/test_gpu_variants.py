@@ -103,3 +103,23 @@ def test_packed_coefficient_records_are_bitwise(hip, solved):
     assert np.all(np.isfinite(out[1])) and np.max(np.abs(out[1])) > 0
     np.testing.assert_array_equal(out[1], out[0])
 
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_vcycle_ignores_stale_storage(hip, solved, precision):
+    """The V-cycle starts every level from zero and reads no solution value it
+    has not written in the same cycle: with every multigrid solution array
+    (each level's x, the level-0 FP32 vectors, the FP32 Krylov vectors)
+    filled with NaN beforehand its output is bit-identical to the output
+    with those arrays zeroed (a line sweep once read a boundary node's own,
+    not yet written x, times a zero coupling)."""
+    N = solved
+    hip.sdx_set_precision(precision)
+    assert hip.sdx_test_mg_setup() >= 4
+    r = np.random.default_rng(11).standard_normal(N)
+    out = {}
+    for fill in (0.0, float("nan")):
+        hip.sdx_test_fill_mg(fill)
+        out[str(fill)] = _vcycle(hip, N, r, {})
+    assert np.all(np.isfinite(out["nan"])) and np.max(np.abs(out["0.0"])) > 0
+    np.testing.assert_array_equal(out["nan"], out["0.0"])
