@@ -10,6 +10,12 @@ equilibrium solve).
 
 --by-grid keeps launches of one kernel with different grid sizes apart (the
 layers of the warm-start network); --match keeps only matching kernels.
+
+--pmc reads a counter collection instead (rocprofv3 --pmc, csv output) and
+prints, per kernel and grid size, the dispatches and each counter's mean per
+dispatch:
+
+    python -m semiconductor_devices_simulator_amd.tracestat counter_collection.csv --pmc --match k_line_scan
 """
 import argparse
 import csv
@@ -55,6 +61,33 @@ def summarise(rows, start=None):
     return dict(window=cur_e - rows[0][0], busy=busy, launches=len(rows), gaps=sorted(gaps), per=per)
 
 
+def load_pmc(path, match=None):
+    """Mean counter value per (kernel, grid size, counter) and the number of
+    dispatches it covers."""
+    acc = defaultdict(lambda: [0, 0.0])
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            name = r["Kernel_Name"].split("(")[0]
+            if match and match not in name:
+                continue
+            grid = r.get("Grid_Size", r.get("Grid_Size_X", "?"))
+            a = acc[(name, grid, r["Counter_Name"])]
+            a[0] += 1
+            a[1] += float(r["Counter_Value"])
+    return {k: (n, v / n) for k, (n, v) in acc.items()}
+
+
+def print_pmc(P):
+    counters = sorted({c for _, _, c in P})
+    print("| kernel | grid | dispatches | " + " | ".join(counters) + " |")
+    print("|---|---:|---:|" + "---:|" * len(counters))
+    for name, grid in sorted({(n, g) for n, g, _ in P}):
+        cells = [P.get((name, grid, c)) for c in counters]
+        n = max(x[0] for x in cells if x)
+        print(f"| `{name[:70]}` | {grid} | {n} | " +
+              " | ".join(f"{x[1]:.0f}" if x else "" for x in cells) + " |")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("trace")
@@ -62,7 +95,11 @@ def main(argv=None):
     ap.add_argument("--top", type=int, default=30)
     ap.add_argument("--by-grid", action="store_true", help="one line per kernel and grid size")
     ap.add_argument("--match", default=None, help="only kernels whose name contains this")
+    ap.add_argument("--pmc", action="store_true", help="the file is a counter collection: mean per dispatch")
     a = ap.parse_args(argv)
+    if a.pmc:
+        print_pmc(load_pmc(a.trace, a.match))
+        return 0
     rows = load(a.trace, a.by_grid)
     if a.match:
         rows = [r for r in rows if a.match in r[2]]

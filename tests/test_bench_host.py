@@ -102,6 +102,15 @@ def test_trace_summary_busy_time_and_window(tmp_path):
     S = tracestat.summarise(tracestat.load(str(p)), "k_cont")
     assert S["window"] == 1600 and S["busy"] == 600 and S["launches"] == 3
     assert S["per"]["k_a"] == [1, 200]
+    # a counter collection: mean per dispatch, per kernel and grid size
+    c = tmp_path / "counter_collection.csv"
+    with open(c, "w") as f:
+        f.write('"Dispatch_Id","Kernel_Name","Grid_Size","Counter_Name","Counter_Value"\n')
+        for d, n, g, v in ((1, "k_a", 64, 10.0), (2, "k_a", 64, 30.0), (3, "k_a", 32, 5.0), (4, "k_b", 64, 7.0)):
+            f.write(f'{d},"{n}(int)",{g},"FETCH_SIZE",{v}\n')
+    P = tracestat.load_pmc(str(c))
+    assert P[("k_a", "64", "FETCH_SIZE")] == (2, 20.0) and P[("k_a", "32", "FETCH_SIZE")] == (1, 5.0)
+    assert list(tracestat.load_pmc(str(c), "k_b")) == [("k_b", "64", "FETCH_SIZE")]
 
 
 def test_batch_sub_segments_and_worker_errors():
