@@ -3,13 +3,13 @@
 // Same entry points, signatures, units and return conventions as the
 // reference core's public API (semiconductor_3d.c:4567-5353), so the
 // reference Python front end loads this library through SEMISIM_SO
-// unchanged.  Gate tunnelling through Robin gate stacks, MLDA quantum
-// confinement, Lombardi surface mobility, impact ionisation and the C-V
-// charge partition are implemented.  Physics outside this core's scope (deep
-// traps, tunnelling through meshed insulator layers, metal-contact emission,
-// incomplete ionisation, the Coulomb term of 4H-SiC, prism meshes and shaped
-// contact node lists) is rejected with a diagnostic and a failure return,
-// never approximated.
+// unchanged.  Gate tunnelling (through Robin gate stacks and through meshed
+// insulator layers), MLDA quantum confinement, Lombardi surface mobility,
+// impact ionisation, the C-V charge partition, prism meshes (the graph path)
+// and shaped contact node lists are implemented.  Physics outside this core's
+// scope (deep traps, metal-contact thermionic emission, incomplete ionisation,
+// the Coulomb term of 4H-SiC) is rejected with a diagnostic and a failure
+// return, never approximated.
 #include "engine.h"
 #include "../../include/semisim_api.h"
 #include "../../include/semisim_hip_ext.h"

@@ -67,6 +67,7 @@ static void prep_free() {
     G.prep_allocs.clear();
     G.tun = TunDev{};
     G.d_tp_off = nullptr;
+    G.d_tp_idx = nullptr;
     G.d_ek = G.d_ekb = G.d_eoff = nullptr;
     G.d_eh = G.d_ear = nullptr;
     G.d_gck = G.d_gce = G.d_gcoff = nullptr;
@@ -512,23 +513,48 @@ void eng_walls(std::vector<double> &qd, std::vector<double> *qcn, std::vector<do
 void eng_wall_distance(std::vector<double> &qd) { eng_walls(qd, nullptr, nullptr); }
 
 // ------------------------------------------------------ gate tunnelling
-// Paths of the Robin (oxide-capacitor) gates with a layer stack: one per gate
-// node, in the order of the gate nodes (tun_build :1238-1244, tun_push
-// :1172-1210), each with its supply column: the semiconductor nodes inward
-// from the interface over 5 nm, stopping at contacts and non-semiconductors.
-// Paths through meshed insulator layers to a metal gate (:1214-1237) are not
-// built: such a device is rejected.
+// Tunnelling paths (tun_build :1213-1247, tun_push :1172-1210), in the
+// reference's order: first the paths through meshed insulator layers (every
+// semiconductor node and direction, ascending, whose straight march through
+// insulator nodes, at most TUN_LMAX of it and TUN_MAXL material runs, ends on
+// the metal node of a gate: a semiconductor half cell, then the insulator
+// runs midpoint to midpoint, the last to the metal), then one path per node of
+// a Robin gate with a layer stack.  Each has its supply column: the
+// semiconductor nodes inward from the interface over 5 nm, stopping at
+// contacts and non-semiconductors.
 #define TUN_COL 5e-9
 #define TUN_LMAX 30e-9
 #define TUN_MAXL 6
 static int tun_build() {
     G.tp_k.clear(); G.tp_c.clear(); G.tp_c0.clear(); G.tp_cn.clear(); G.col_k.clear();
     G.tp_A.clear(); G.col_l.clear(); G.col_h.clear();
+    G.tp_nl.clear(); G.tp_m.clear(); G.tp_t.clear();
     for (int c = 0; c < G.n_cons; c++) G.cons[c].ntp = 0;
     if (!(G.models & 32)) return 1;
-    // a path through meshed insulator nodes exists where the straight march
-    // from a semiconductor node through insulator (at most TUN_LMAX of it,
-    // TUN_MAXL material runs) ends on the metal node of a gate (:1214-1237)
+    // one path from node k (toward the metal along dir) with its layers, and
+    // its supply column inward (dir ^ 1)
+    auto push = [&](int k, double A, int c, int dir, int nl, const int *m, const double *t) {
+        G.tp_k.push_back(k); G.tp_c.push_back(c); G.tp_A.push_back(A);
+        G.tp_nl.push_back(nl);
+        for (int i = 0; i < TUN_MAXL; i++) { G.tp_m.push_back(i < nl ? m[i] : 0); G.tp_t.push_back(i < nl ? t[i] : 0.); }
+        G.tp_c0.push_back((int)G.col_k.size());
+        const int in = dir ^ 1, ax = dir / 2;
+        int cnt = 0, cur = k;
+        double depth = 0.;
+        for (;;) {
+            int ii, jj, ll;
+            host_unidx(cur, ii, jj, ll);
+            double L = ax == 0 ? G.Lx[ii] : (ax == 1 ? G.Ly[jj] : G.Lz[ll]);
+            int kn; double h, ar;
+            bool more = host_nbr(ii, jj, ll, cur, in, kn, h, ar) && G.kind[kn] == K_SEMI && !G.con_mask[kn];
+            G.col_k.push_back(cur); G.col_l.push_back(L); G.col_h.push_back(more ? h : 0.);
+            cnt++; depth += L;
+            if (!more || depth >= TUN_COL) break;
+            cur = kn;
+        }
+        G.tp_cn.push_back(cnt);
+        G.cons[c].ntp++;
+    };
     for (int k = 0; k < G.N; k++) {
         if (G.kind[k] != K_SEMI || G.con_mask[k]) continue;
         int i, j, l;
@@ -536,52 +562,34 @@ static int tun_build() {
         for (int dir = 0; dir < 6; dir++) {
             int kb; double h, ar;
             if (!host_nbr(i, j, l, k, dir, kb, h, ar) || G.kind[kb] != K_INS || G.con_mask[kb]) continue;
-            int nl = 1, last = -1, ok = 0, cur = kb;
+            int m[TUN_MAXL]; double t[TUN_MAXL];
+            int nl = 1, ok = 0, cur = kb;
+            m[0] = G.mat_idx[k]; t[0] = 0.5 * h;              // semiconductor half cell
             double tins = 0., hin = h;
             for (int guard = 0; guard < 100000; guard++) {
                 int ci, cj, cl, kn; double h2, ar2;
                 host_unidx(cur, ci, cj, cl);
                 if (!host_nbr(ci, cj, cl, cur, dir, kn, h2, ar2)) break;                // domain edge
-                int mi = G.mat_idx[cur];
-                bool end = G.kind[kn] == K_METAL && G.con_mask[kn] && G.cons[G.con_mask[kn] - 1].bc == 2;
-                double seg = 0.5 * hin + (end ? h2 : 0.5 * h2);
-                if (!(nl > 1 && last == mi)) { if (nl >= TUN_MAXL) break; last = mi; nl++; }
+                const int mi = G.mat_idx[cur];
+                const bool end = G.kind[kn] == K_METAL && G.con_mask[kn] && G.cons[G.con_mask[kn] - 1].bc == 2;
+                const double seg = 0.5 * hin + (end ? h2 : 0.5 * h2);   // midpoint to midpoint (or to the metal)
+                if (nl > 1 && m[nl - 1] == mi) t[nl - 1] += seg;
+                else { if (nl >= TUN_MAXL) break; m[nl] = mi; t[nl] = seg; nl++; }
                 tins += seg;
                 if (tins > TUN_LMAX) break;
-                if (end) { ok = 1; break; }
+                if (end) { ok = G.con_mask[kn]; break; }
                 if (G.kind[kn] != K_INS || G.con_mask[kn]) break;
                 cur = kn; hin = h2;
             }
-            if (ok) {
-                fprintf(stderr, "semisim-hip: gate tunnelling through meshed insulator layers is outside this core: "
-                                "not solved\n");
-                return 0;
-            }
+            if (ok) push(k, ar, ok - 1, dir, nl, m, t);
         }
     }
     for (int c = 0; c < G.n_cons; c++) {
         HostCon &cn = G.cons[c];
         if (cn.bc != 2 || cn.face > 5 || cn.snl < 1) continue;
-        const int dir = cn.face, in = dir ^ 1, ax = dir / 2;
         for_con_nodes(cn, [&](int, int, int, int k, double area) {
             if (G.gate_mask[k] != c + 1 || G.con_mask[k]) return;
-            G.tp_k.push_back(k); G.tp_c.push_back(c); G.tp_A.push_back(area);
-            G.tp_c0.push_back((int)G.col_k.size());
-            int cnt = 0, cur = k;
-            double depth = 0.;
-            for (;;) {
-                int ii, jj, ll;
-                host_unidx(cur, ii, jj, ll);
-                double L = ax == 0 ? G.Lx[ii] : (ax == 1 ? G.Ly[jj] : G.Lz[ll]);
-                int kn; double h, ar;
-                bool more = host_nbr(ii, jj, ll, cur, in, kn, h, ar) && G.kind[kn] == K_SEMI && !G.con_mask[kn];
-                G.col_k.push_back(cur); G.col_l.push_back(L); G.col_h.push_back(more ? h : 0.);
-                cnt++; depth += L;
-                if (!more || depth >= TUN_COL) break;
-                cur = kn;
-            }
-            G.tp_cn.push_back(cnt);
-            cn.ntp++;
+            push(k, area, c, cn.face, cn.snl, cn.smat, cn.st);
         });
     }
     if (G.verbose && !G.tp_k.empty())
@@ -600,16 +608,16 @@ static void tun_upload() {
     T = TunDev{};
     const int np = (int)G.tp_k.size(), N = G.N;
     if (np == 0) return;
-    std::vector<int> pnl(np), pm((size_t)np * SD_TUN_MAXL, 0), poff(G.n_cons + 1, 0);
-    std::vector<double> pt((size_t)np * SD_TUN_MAXL, 0.);
-    for (int p = 0; p < np; p++) {
-        const HostCon &cn = G.cons[G.tp_c[p]];
-        pnl[p] = cn.snl;
-        for (int i = 0; i < cn.snl; i++) { pm[(size_t)p * SD_TUN_MAXL + i] = cn.smat[i]; pt[(size_t)p * SD_TUN_MAXL + i] = cn.st[i]; }
-    }
-    // paths are built contact by contact: contact c owns poff[c]..poff[c+1]
+    static_assert(SD_TUN_MAXL == TUN_MAXL, "one layer limit");
+    // the paths of contact c are pidx[poff[c]..poff[c+1]) (meshed paths come
+    // in node order, so a contact's paths need not be adjacent)
+    std::vector<int> poff(G.n_cons + 1, 0), pidx(np);
     for (int p = 0; p < np; p++) poff[G.tp_c[p] + 1]++;
     for (int c = 0; c < G.n_cons; c++) poff[c + 1] += poff[c];
+    {
+        std::vector<int> fill(poff.begin(), poff.end() - 1);
+        for (int p = 0; p < np; p++) pidx[fill[G.tp_c[p]]++] = p;
+    }
     // touched nodes (ascending) with their column entries and path starts,
     // both in ascending (= the reference's summation) order
     std::vector<int> tn(G.col_k);
@@ -628,12 +636,13 @@ static void tun_upload() {
         entG.insert(entG.end(), eG[t].begin(), eG[t].end()); offG[t + 1] = (int)entG.size();
     }
     T.np = np; T.ntn = ntn;
-    T.pk = dput(G.tp_k); T.pc = dput(G.tp_c); T.pnl = dput(pnl); T.pc0 = dput(G.tp_c0); T.pcn = dput(G.tp_cn);
-    T.pm = dput(pm); T.pA = dput(G.tp_A); T.pt = dput(pt);
+    T.pk = dput(G.tp_k); T.pc = dput(G.tp_c); T.pnl = dput(G.tp_nl); T.pc0 = dput(G.tp_c0); T.pcn = dput(G.tp_cn);
+    T.pm = dput(G.tp_m); T.pA = dput(G.tp_A); T.pt = dput(G.tp_t);
     T.ck = dput(G.col_k); T.cl = dput(G.col_l); T.ch = dput(G.col_h);
     T.cK = palloc<double>(2 * G.col_k.size()); T.pG = palloc<double>(2 * (size_t)np);
     T.tn = dput(tn); T.offK = dput(offK); T.entK = dput(entK); T.offG = dput(offG); T.entG = dput(entG);
     G.d_tp_off = dput(poff);
+    G.d_tp_idx = dput(pidx);
     if (G.models & 8) {
         HC(hipMemcpyAsync(G.d_qcn, G.qcn.data(), (size_t)N * 8, hipMemcpyHostToDevice, G.st));
         HC(hipMemcpyAsync(G.d_qcp, G.qcp.data(), (size_t)N * 8, hipMemcpyHostToDevice, G.st));
@@ -1805,6 +1814,7 @@ static void terminal_currents(double *I, double *Sv) {
     CurEdges e;
     e.k = G.d_ek; e.kb = G.d_ekb; e.h = G.d_eh; e.ar = G.d_ear; e.off = G.d_eoff; e.ncon = G.n_cons;
     e.poff = G.tun.np ? G.d_tp_off : nullptr;
+    e.pidx = G.tun.np ? G.d_tp_idx : nullptr;
     e.tun = G.tun;
     ph_currents(G.st, eng_view(), e, G.rb);
     rd(S_CUR, S_CUR + 3 * std::max(1, G.n_cons));

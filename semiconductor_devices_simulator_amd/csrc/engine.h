@@ -112,11 +112,16 @@ struct Engine {
     double *d_tk = nullptr, *d_tg = nullptr;     // gate tunnelling sink/source of the carrier being solved
     double *d_qcn = nullptr, *d_qcp = nullptr;   // MLDA terms (tunnelling supply)
     TunDev tun{};                      // gate tunnelling paths (np = 0: none)
-    int *d_tp_off = nullptr;           // paths of each contact (n_cons+1)
+    int *d_tp_off = nullptr;           // paths of each contact (n_cons+1) in d_tp_idx
+    int *d_tp_idx = nullptr;           // path indices grouped by contact (each group ascending)
     // host copies of the paths (test hooks): node, contact, first column entry,
     // column length; column nodes, lengths, spacings
     std::vector<int> tp_k, tp_c, tp_c0, tp_cn, col_k;
     std::vector<double> tp_A, col_l, col_h;
+    // each path's layers (semiconductor half cell first on a meshed path):
+    // count, SD_TUN_MAXL materials and thicknesses (m)
+    std::vector<int> tp_nl, tp_m;
+    std::vector<double> tp_t;
     std::vector<double> qd;            // its host copy (source of the upload)
     std::vector<double> qcn, qcp;      // MLDA confinement terms of ln Nc, ln Nv (0 without MLDA)
     MatDev *d_mats = nullptr;

@@ -636,7 +636,8 @@ __global__ void __launch_bounds__(SD_BLOCK) k_currents(DevView d, CurEdges e, Re
     // gate tunnelling (:3614-3621): electrons leaving to the gate are current
     // into the device through it, holes leaving are current out of it
     if (e.poff)
-        for (int p = e.poff[c] + threadIdx.x; p < e.poff[c + 1]; p += SD_BLOCK) {
+        for (int q = e.poff[c] + threadIdx.x; q < e.poff[c + 1]; q += SD_BLOCK) {
+            const int p = e.pidx[q];
             double se, sh;
             tun_escape(d, e.tun, p, se, sh);
             const double G0 = e.tun.pG[2 * p], G1 = e.tun.pG[2 * p + 1];

@@ -10,7 +10,8 @@ struct CurEdges {
     const double *h, *ar;
     const int *off;       // device, n_cons+1
     int ncon;
-    const int *poff;      // gate tunnelling paths of each contact (n_cons+1; nullptr: none)
+    const int *poff;      // gate tunnelling paths of each contact in pidx (n_cons+1; nullptr: none)
+    const int *pidx;      // path indices grouped by contact
     TunDev tun;
 };
 

@@ -220,6 +220,30 @@ def pn_diode(Nx=16, Ny=16, Nz=64):
                       cons, models=0, tau=(1e-7, 1e-7), sweep=[(0, 0.0, 0.8, 17, {})])
 
 
+def meshed_gate_diode(Nx=8, Ny=8, Nz=32, V1=2.0, npts=6):
+    """The pn diode of pn_diode() with its planes x = 0 and x = 1 nm made
+    SiO2 and a metal gate (work function 4.65 eV) over x = 0, y and z in
+    [0.25, 0.75] of the box: the first silicon plane (x = 2.5 nm) sees the gate
+    through one meshed oxide plane, so gate tunnelling (models bit 5) builds
+    paths through meshed insulator nodes (tun_build :1214-1237), 176 of them
+    at 8x8x32.  Models 7 | 32, anode and cathode at 0 V, gate 0 -> V1 in
+    npts points."""
+    spec = pn_diode(Nx, Ny, Nz)
+    spec.name = "meshed_gate_diode_%dx%dx%d" % (Nx, Ny, Nz)
+    spec.xs = np.concatenate([[0.0, 1.0e-9, 2.5e-9], np.linspace(0.2e-6, 1e-6, Nx - 3)])
+    spec.materials = [MAT_SI, MAT_SIO2]
+    mat = spec.mat.reshape(Nz, Ny, Nx).copy()
+    mat[:, :, :2] = 1
+    spec.mat = mat.ravel()
+    ins = spec.mat == 1
+    spec.Nd = np.where(ins, 0.0, spec.Nd)
+    spec.Na = np.where(ins, 0.0, spec.Na)
+    spec.contacts.append(Contact(0, 0.0, 1.0, 0.25, 0.75, 2, 0.0, pm=4.65, label="Gate"))
+    spec.models = 7 | 32
+    spec.sweep = [(2, 0.0, V1, npts, {0: 0.0, 1: 0.0})]
+    return spec
+
+
 def avalanche_diode(Nx=8, Ny=8, Nz=48, V1=-19.0, npts=20):
     """The pn diode of pn_diode() in reverse bias with impact ionisation on
     (models bit 8, van Overstraeten-de Man coefficients): anode 0 -> V1 in

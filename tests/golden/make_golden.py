@@ -790,6 +790,15 @@ def main():
         sweep_fixture(lib, devices.vdmos(40, 24, 4, vd_max=50.0, npts=11, npts_on=5, tol_c=1e-11),
                       "e2e_vdmos_40x24x4")
         return
+    if "--tunmeshed" in sys.argv:      # gate tunnelling through a meshed oxide plane (seconds)
+        spec = devices.meshed_gate_diode()
+        sweep_fixture(lib, spec, "e2e_meshed_gate_diode_8x8x32")
+        path = os.path.join(HERE, "e2e_meshed_gate_diode_8x8x32.npz")
+        g = dict(np.load(path))
+        g["ntp"] = np.array([lib.api3_get_ntp(-1)] + [lib.api3_get_ntp(c) for c in range(len(spec.contacts))])
+        np.savez_compressed(path, **g)
+        print("tunnelling paths", g["ntp"].tolist())
+        return
     if "--vdmos96" in sys.argv:        # the VDMOS cell on a 96x48x16 mesh (about 20 minutes on 8 cores)
         # Id-Vds to 10 V: at 20 V the reference's Gummel loop stops without
         # converging on this mesh, and at 15 V it needs 8738 iterations (a

@@ -129,23 +129,6 @@ def test_out_of_scope_physics_is_rejected(hip):
     assert hip.api3_solve() == 0
     hip.api3_set_models(0)
     assert hip.api3_solve() == 1
-    # gate tunnelling through a meshed insulator layer: the planes x = 0 and
-    # 1 nm are SiO2 and a gate covers x = 0, so the march from the silicon
-    # (2.5 nm) reaches the gate metal through one oxide plane (the reference
-    # core builds 176 such paths); without the gate there is no path and the
-    # device prepares
-    for with_gate in (True, False):
-        spec = devices.pn_diode(8, 8, 32)
-        spec.xs = np.array([0.0, 1.0e-9, 2.5e-9, 0.2e-6, 0.4e-6, 0.6e-6, 0.8e-6, 1e-6])
-        spec.materials = [devices.MAT_SI, devices.MAT_SIO2]
-        mat = spec.mat.reshape(32, 8, 8).copy()
-        mat[:, :, :2] = 1
-        spec.mat = mat.ravel()
-        if with_gate:
-            spec.contacts.append(devices.Contact(0, 0.0, 1.0, 0.25, 0.75, 2, 0.0, pm=4.65, label="Gate"))
-        spec.models = 32
-        devices.load_device(hip, spec)
-        assert hip.sdx_test_prepare() == (0 if with_gate else 1)
 
 
 def test_readback_quantities(hip):

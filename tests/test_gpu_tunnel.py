@@ -102,3 +102,36 @@ def test_tun_eval_matches_reference(hip):
     out = np.full(4, 7.0)
     assert hip.api3_tun_eval(ids[0], 0, iptr(mi), dptr(th), 0.0, 0.0, 0.0, 1.0, 4.65, dptr(out)) == 0.0
     assert np.all(out == 7.0)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_meshed_oxide_tunnelling_sweep_matches_reference(hip, precision):
+    """Gate tunnelling through meshed insulator nodes (tun_build :1214-1237):
+    the diode of devices.meshed_gate_diode, whose planes x = 0 and 1 nm are
+    SiO2 under a metal gate, builds the same 176 paths as the reference (a
+    silicon half cell, then the oxide to the metal) and none without the gate;
+    its gate sweep 0 -> 2 V (models 7|32) matches the reference's sweep at every
+    terminal, the gate's tunnelling current included (up to 1.8e-8 A), within
+    1e-8 |I_ref| + floor_ref + floor, and psi of the last point within 1e-9 V."""
+    print(f"\n[gpu tunnel] meshed gate diode precision {precision}", flush=True)
+    spec = devices.meshed_gate_diode()
+    g = load_golden("e2e_meshed_gate_diode_8x8x32")
+    cids = devices.load_device(hip, spec)
+    assert hip.sdx_set_precision(precision) == 1
+    assert hip.api3_solve() == 1
+    ntp = [hip.api3_get_ntp(-1)] + [hip.api3_get_ntp(c) for c in range(len(cids))]
+    assert ntp == g["ntp"].tolist(), (ntp, g["ntp"])
+    R = sweep.run_sweep(hip, spec, cids)
+    assert np.all(R["conv"])
+    print("[gpu tunnel] gate current", R["I"][:, 2].tolist(), flush=True)
+    assert_sweep_parity(R, "e2e_meshed_gate_diode_8x8x32")
+    N = spec.N
+    psi = np.zeros(N)
+    hip.api3_get_state(dptr(psi), dptr(np.zeros(N)), dptr(np.zeros(N)))
+    assert np.max(np.abs(psi - g["psi"])) <= 1e-9
+    # without the gate the march ends on no metal: no path
+    nogate = devices.meshed_gate_diode()
+    nogate.contacts = nogate.contacts[:2]
+    nogate.sweep = [(0, 0.0, 0.0, 1, {})]
+    devices.load_device(hip, nogate)
+    assert hip.sdx_test_prepare() == 1 and hip.api3_get_ntp(-1) == 0
