@@ -44,10 +44,10 @@ void   api3_add_contact(int face, int i0, int i1, int j0, int j1, int bc, double
 void   api3_clear_contacts(void);                                         /* :4775 */
 int    api3_set_gate_stack(int cid, int nl, const int *mat, const double *t); /* :4779 (tunnelling, models bit 5) */
 int    api3_set_mesh_graph(int N, int E, const int *rp, const int *cj, const double *h,
-                           const double *ar, const double *vol, const double *pos); /* :4795 (rejected: prism mesh) */
+                           const double *ar, const double *vol, const double *pos); /* :4795 (prism mesh: graph path) */
 int    api3_is_graph(void);                                               /* :4806 */
 void   api3_set_node_data(const int *mat, const double *Nd, const double *Na, const double *Nf); /* :4808 */
-int    api3_set_con_nodes(int cid, int n, const int *nodes, const double *areas); /* :4821 (n>0 rejected) */
+int    api3_set_con_nodes(int cid, int n, const int *nodes, const double *areas); /* :4821 (shaped contacts) */
 void   api3_set_walls(const double *w);                                   /* :4838 */
 int    api3_get_ntp(int cid);                                             /* :4844 */
 double api3_tun_eval(int smat, int nl, const int *mat, const double *t, double psi,
