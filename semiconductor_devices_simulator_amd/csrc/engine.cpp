@@ -896,6 +896,7 @@ template <> Hier<double> &hier<double>() { return G.H64; }
 template <> Hier<float> &hier<float>() { return G.H32; }
 
 static int g_rec16 = 1;         // 1: the level-0 sweeps read the packed 16-byte coefficient records
+extern int g_cont_lds;          // 1: the LDS-staged continuity assembly (k_physics.hip)
 static inline int line_axis0(int L);
 // BF16 copies of level 0's off-diagonals and line factors for the smoother
 // (FP32 hierarchy, when enabled); else the smoother reads the FP32 arrays
@@ -2609,6 +2610,7 @@ void sdx_tune(int what, int value) {
     if (what == 22) g_rb_fuse_post = value != 0;
     if (what == 23) G.stage_read = value != 0;
     if (what == 24) g_rec16 = value != 0;
+    if (what == 25) g_cont_lds = value != 0;
     if (what == 26) { G.pred2 = value != 0; G.e3_ok = 0; }
     if (what == 28) G.mg_bf16_l1 = value != 0;
     if (what == 27) G.inner32 = value >= 0 && value <= 2 ? value : 1;

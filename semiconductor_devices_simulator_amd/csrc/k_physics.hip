@@ -268,8 +268,108 @@ __global__ void k_cont(DevView d, int hole, Sten7<double> A, double *rhs, double
     sc[k] = sk;
     dsc[k] = dg * sk;
 }
+// The same assembly over 32x4x2-node tiles whose neighbour operands (kind,
+// contact mask, psi, ln Nc or ln Nv, carrier density) are staged in LDS with a
+// one-node halo: a workgroup reads each value of its 34x6x4 halo tile once
+// from memory instead of every node reading its six neighbours through the
+// caches.  Same operations in the same order as k_cont: identical rows.
+#define CT_X 32
+#define CT_Y 4
+#define CT_Z 2
+#define CT_H ((CT_X + 2) * (CT_Y + 2) * (CT_Z + 2))
+__global__ void __launch_bounds__(CT_X * CT_Y * CT_Z) k_cont_lds(DevView d, int hole, Sten7<double> A, double *rhs,
+                                                                 double *sc, double *dsc) {
+    __shared__ double s_ps[CT_H], s_ln[CT_H], s_car[CT_H];
+    __shared__ int s_cm[CT_H];
+    __shared__ unsigned char s_kd[CT_H];
+    const Geo g = d.g;
+    const int nbx = (g.Nx + CT_X - 1) / CT_X, nby = (g.Ny + CT_Y - 1) / CT_Y;
+    const int b = blockIdx.x, bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int i0 = bx * CT_X - 1, j0 = by * CT_Y - 1, l0 = bz * CT_Z - 1;
+    const double *car = hole ? d.p : d.n;
+    const double *lnd = hole ? d.lnvK : d.lncK;
+    for (int t = threadIdx.x; t < CT_H; t += CT_X * CT_Y * CT_Z) {
+        const int hx = t % (CT_X + 2), r = t / (CT_X + 2), hy = r % (CT_Y + 2), hz = r / (CT_Y + 2);
+        const int i = i0 + hx, j = j0 + hy, l = l0 + hz;
+        if (i >= 0 && i < g.Nx && j >= 0 && j < g.Ny && l >= 0 && l < g.Nz) {
+            const int q = i + g.Nx * (j + g.Ny * l);
+            s_kd[t] = d.kind[q]; s_cm[t] = d.con_mask[q];
+            s_ps[t] = d.psi[q]; s_ln[t] = lnd[q]; s_car[t] = car[q];
+        } else {
+            s_kd[t] = 255; s_cm[t] = 0; s_ps[t] = s_ln[t] = s_car[t] = 0.0;
+        }
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % CT_X, ty = (threadIdx.x / CT_X) % CT_Y, tz = threadIdx.x / (CT_X * CT_Y);
+    const int i = i0 + 1 + tx, j = j0 + 1 + ty, l = l0 + 1 + tz;
+    if (i >= g.Nx || j >= g.Ny || l >= g.Nz) return;
+    const int k = i + g.Nx * (j + g.Ny * l);
+    const int c0 = (tx + 1) + (CT_X + 2) * ((ty + 1) + (CT_Y + 2) * (tz + 1));
+    if (s_kd[c0] != K_SEMI || s_cm[c0]) {
+        A.c[k] = 1.0; A.xm[k] = A.xp[k] = A.ym[k] = A.yp[k] = A.zm[k] = A.zp[k] = 0.0;
+        bool semi = s_kd[c0] == K_SEMI;
+        rhs[k] = semi ? 1.0 : 0.0;
+        sc[k] = semi ? s_car[c0] : 0.0;
+        dsc[k] = 0.0;
+        return;
+    }
+    const MatDev &ma = d.mats[d.mat[k]];
+    const double VT = ma.VT, sk = fmax(s_car[c0], SD_NFLOOR), pk = s_ps[c0];
+    const double lk = s_ln[c0];
+    const size_t N = g.N;
+    const double *fm = d.fmu + (size_t)(hole ? 3 : 0) * N;
+    Nbr6 ng;
+    sd_nbr6(g, i, j, l, k, ng);
+    const int hs[6] = {-1, 1, -(CT_X + 2), CT_X + 2, -(CT_X + 2) * (CT_Y + 2), (CT_X + 2) * (CT_Y + 2)};
+    double fmv[6];
+#pragma unroll
+    for (int dir = 0; dir < 6; dir++) {
+        const int kl = (dir & 1) ? k : ng.kb[dir];
+        fmv[dir] = fm[(size_t)(dir >> 1) * N + kl];
+    }
+    double dg = 0.0, bb = 0.0, off[6];
+#pragma unroll
+    for (int dir = 0; dir < 6; dir++) {
+        off[dir] = 0.0;
+        const int h = c0 + hs[dir];
+        if (!ng.has[dir] || s_kd[h] != K_SEMI) continue;
+        double dl = s_ln[h] - lk;
+        double D = (s_ps[h] - pk) / VT + (hole ? -dl : dl);
+        double gg = ng.ar[dir] * fmv[dir] * VT / ng.h[dir];
+        double bp, bm;
+        sd_bern2(D, bp, bm);
+        double self = hole ? gg * bp : gg * bm;
+        double nb = hole ? gg * bm : gg * bp;
+        dg += self;
+        if (s_cm[h]) bb += nb * s_car[h];
+        else off[dir] = -nb * fmax(s_car[h], SD_NFLOOR);
+    }
+    double R, dRn, dRp;
+    sd_recomb(ma, d.niK[k], sd_tauf(d, k, ma), d.n[k], d.p[k], R, dRn, dRp);
+    double dR = hole ? dRp : dRn;
+    if (dR < 0.) dR = 0.;
+    double vol = g.Lx[i] * g.Ly[j] * g.Lz[l];
+    dg += vol * dR;
+    bb += vol * (dR * s_car[c0] - R);
+    if (d.tk) { dg += d.tk[k]; bb += d.tg[k]; }
+    if (d.gii) bb += d.gii[k];
+    double w = 1.0 / (dg * sk);
+    A.c[k] = 1.0;
+    A.xm[k] = off[0] * w; A.xp[k] = off[1] * w; A.ym[k] = off[2] * w;
+    A.yp[k] = off[3] * w; A.zm[k] = off[4] * w; A.zp[k] = off[5] * w;
+    rhs[k] = bb * w;
+    sc[k] = sk;
+    dsc[k] = dg * sk;
+}
+int g_cont_lds = 1;          // 1: the LDS-staged continuity assembly
 void ph_cont(hipStream_t s, const DevView &d, int hole, Sten7<double> A, double *rhs, double *sc, double *dsc) {
-    k_cont<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d, hole, A, rhs, sc, dsc);
+    if (g_cont_lds) {
+        const Geo &g = d.g;
+        const int nb = ((g.Nx + CT_X - 1) / CT_X) * ((g.Ny + CT_Y - 1) / CT_Y) * ((g.Nz + CT_Z - 1) / CT_Z);
+        k_cont_lds<<<nb, CT_X * CT_Y * CT_Z, 0, s>>>(d, hole, A, rhs, sc, dsc);
+    } else {
+        k_cont<<<sd_nblocks(d.g.N), SD_BLOCK, 0, s>>>(d, hole, A, rhs, sc, dsc);
+    }
 }
 
 // c = s*u -> quasi-Fermi potential clamped to [lo, hi]; weighted max change
