@@ -68,7 +68,9 @@ void   sdx_tune(int what, int value);      /* 0: lines per workgroup of the LDS 
                                               written by the products (0), 24: the level-0 line sweeps
                                               of the FP32 method read each node's seven BF16
                                               coefficients as one packed 16-byte record (1) or as
-                                              seven 2-byte loads (0), 26: quadratic sweep predictor (1) or the secant (0),
+                                              seven 2-byte loads (0), 25: continuity assembly over
+                                              LDS-staged tiles (1) or per node (0),
+                                              26: quadratic sweep predictor (1) or the secant (0),
                                               27: FP32 method solves with FP32 Krylov inside FP64
                                               defect correction (1, on devices without a large
                                               floating block; 2 on any device) or with FP64 Krylov (0),
