@@ -723,12 +723,14 @@ void ph_tun_J(hipStream_t s, const DevView &d, const TunDev &T, double *out) {
     if (T.ntn > 0) k_tun_J<<<sd_nblocks(T.ntn), SD_BLOCK, 0, s>>>(d, T, out);
 }
 
-// terminal currents: one workgroup per contact, fixed-order tree sums
-__global__ void __launch_bounds__(SD_BLOCK) k_currents(DevView d, CurEdges e, RedBuf rb) {
+// terminal currents: each contact's edges (and tunnelling paths) split over
+// `split` workgroups, each writing its three partial sums; a one-workgroup
+// pass then adds every contact's partials in order (fixed-order sums)
+__global__ void __launch_bounds__(SD_BLOCK) k_currents(DevView d, CurEdges e, RedBuf rb, int split) {
     __shared__ double lds[3][SD_BLOCK / 64];
-    int c = blockIdx.x;
+    const int c = blockIdx.x / split, part = blockIdx.x % split, stride = split * SD_BLOCK;
     double a0 = 0., a1 = 0., a2 = 0.;
-    for (int q = e.off[c] + threadIdx.x; q < e.off[c + 1]; q += SD_BLOCK) {
+    for (int q = e.off[c] + part * SD_BLOCK + threadIdx.x; q < e.off[c + 1]; q += stride) {
         double Jn, Jp, Sn, Sp;
         face_flux(d, e.k[q], e.kb[q], e.h[q], e.ar[q], Jn, Jp, Sn, Sp);
         a0 += Jn + Jp; a1 += Sn + Sp; a2 += Jn;
@@ -736,7 +738,7 @@ __global__ void __launch_bounds__(SD_BLOCK) k_currents(DevView d, CurEdges e, Re
     // gate tunnelling (:3614-3621): electrons leaving to the gate are current
     // into the device through it, holes leaving are current out of it
     if (e.poff)
-        for (int q = e.poff[c] + threadIdx.x; q < e.poff[c + 1]; q += SD_BLOCK) {
+        for (int q = e.poff[c] + part * SD_BLOCK + threadIdx.x; q < e.poff[c + 1]; q += stride) {
             const int p = e.pidx[q];
             double se, sh;
             tun_escape(d, e.tun, p, se, sh);
@@ -748,16 +750,28 @@ __global__ void __launch_bounds__(SD_BLOCK) k_currents(DevView d, CurEdges e, Re
     int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
     if (ln == 0) { lds[0][w] = a0; lds[1][w] = a1; lds[2][w] = a2; }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int v = 0; v < 3; v++) {
-            double s = lds[v][0];
-            for (int q = 1; q < SD_BLOCK / 64; q++) s += lds[v][q];
-            rb.scal[S_CUR + 3 * c + v] = s;
-        }
+    if (threadIdx.x < 3) {
+        const int v = threadIdx.x;
+        double s = lds[v][0];
+        for (int q = 1; q < SD_BLOCK / 64; q++) s += lds[v][q];
+        rb.part[v * SD_RED_MAXB + blockIdx.x] = s;
     }
 }
+__global__ void k_currents_sum(RedBuf rb, int ncon, int split) {
+    const int t = threadIdx.x;
+    if (t >= 3 * ncon) return;
+    const int c = t / 3, v = t % 3;
+    double s = 0.;
+    for (int q = 0; q < split; q++) s += rb.part[v * SD_RED_MAXB + c * split + q];
+    rb.scal[S_CUR + 3 * c + v] = s;
+}
 void ph_currents(hipStream_t s, const DevView &d, CurEdges e, RedBuf rb) {
-    if (e.ncon > 0) k_currents<<<e.ncon, SD_BLOCK, 0, s>>>(d, e, rb);
+    if (e.ncon <= 0) return;
+    int split = SD_RED_MAXB / e.ncon;
+    if (split > 32) split = 32;
+    if (split < 1) split = 1;
+    k_currents<<<e.ncon * split, SD_BLOCK, 0, s>>>(d, e, rb, split);
+    k_currents_sum<<<1, (3 * e.ncon + 63) / 64 * 64, 0, s>>>(rb, e.ncon, split);
 }
 
 __global__ void k_extrap(int N, double *d0, double *d1, double *d2, const double *a0, const double *a1,
